@@ -1,0 +1,143 @@
+// Host-side internals shared by the C ABI implementation files: status and
+// error text, the per-context HBM pool with its budget, kernel timing.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gazelle_amd.h"
+
+namespace gz {
+
+// ---- errors ---------------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define GZ_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return ::gz::fail(GZ_HIP_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define GZ_TRY(call)          \
+  do {                        \
+    int rc_ = (call);         \
+    if (rc_ != GZ_OK) return rc_; \
+  } while (0)
+
+// ---- type helpers ---------------------------------------------------------------
+inline int type_width(int t) {
+  switch (t) {
+    case GZ_TYPE_UINT8: case GZ_TYPE_INT8: return 1;
+    case GZ_TYPE_UINT16: case GZ_TYPE_INT16: return 2;
+    case GZ_TYPE_UINT32: case GZ_TYPE_INT32: case GZ_TYPE_FLOAT: case GZ_TYPE_DATE32: return 4;
+    case GZ_TYPE_UINT64: case GZ_TYPE_INT64: case GZ_TYPE_DOUBLE: case GZ_TYPE_DATE64:
+    case GZ_TYPE_TIMESTAMP: return 8;
+    case GZ_TYPE_DECIMAL128: return 16;
+    default: return 0;
+  }
+}
+inline bool type_is_binary(int t) { return t == GZ_TYPE_STRING || t == GZ_TYPE_BINARY; }
+inline bool type_supported(int t) {
+  return t == GZ_TYPE_NA || t == GZ_TYPE_BOOL || type_is_binary(t) || type_width(t) > 0;
+}
+// Buffers a column contributes in the JNI convention.
+inline int type_num_bufs(int t) { return type_is_binary(t) ? 3 : 2; }
+
+// ---- device memory ----------------------------------------------------------------
+// A caching pool: blocks are rounded (power of two up to 64 MiB, 2 MiB multiples
+// above) and kept on free lists; the budget counts live blocks only, and cached
+// blocks are returned to HIP before an allocation would exceed it.
+class DevicePool {
+ public:
+  DevicePool(int device, int64_t budget) : device_(device), budget_(budget) {}
+  ~DevicePool();
+  int alloc(size_t bytes, void** out);
+  void free(void* p);
+  int64_t in_use() const { return in_use_; }
+  int64_t peak() const { return peak_; }
+  int64_t budget() const { return budget_; }
+  int trim();
+
+ private:
+  static size_t round(size_t bytes);
+  int device_;
+  int64_t budget_;
+  int64_t in_use_ = 0, peak_ = 0, cached_ = 0;
+  std::multimap<size_t, void*> free_;
+  std::unordered_map<void*, size_t> live_;
+  std::mutex mu_;
+};
+
+// A device buffer owned by a handle.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+// ---- kernel timing ------------------------------------------------------------------
+struct KernelStat {
+  double ms = 0;
+  int64_t launches = 0;
+};
+
+}  // namespace gz
+
+struct gz_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  gz::DevicePool* pool = nullptr;
+  int profiling = 0;
+  int num_cus = 0;
+  std::mutex stat_mu;
+  std::map<std::string, gz::KernelStat> stats;
+  // events recorded but not yet resolved: (name, start, stop)
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_cache;
+  // pinned staging ring for host->device descriptor uploads
+  struct Staging {
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+  };
+  Staging staging[8];
+  int staging_next = 0;
+  std::mutex staging_mu;
+};
+
+namespace gz {
+
+int ctx_alloc(gz_context* ctx, size_t bytes, DBuf* out);
+void ctx_free(gz_context* ctx, DBuf* b);
+int ctx_ensure(gz_context* ctx, DBuf* b, size_t bytes);  // grow-only scratch
+
+// Brackets a launch with HIP events when profiling is on.
+class KernelTimer {
+ public:
+  KernelTimer(gz_context* ctx, const char* name);
+  ~KernelTimer();
+
+ private:
+  gz_context* ctx_;
+  const char* name_;
+  hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+void ctx_resolve_timings(gz_context* ctx, bool wait);
+
+// Stage a host array into device scratch (synchronous with respect to the host
+// buffer: it may be reused as soon as this returns).
+int upload(gz_context* ctx, DBuf* dst, const void* src, size_t bytes);
+
+}  // namespace gz

@@ -1,0 +1,140 @@
+"""Seeded synthetic inputs of the benchmark workloads.
+
+Seeds are numpy PCG64 with seed = 42 + column index (as the measurement plan
+states); batches are 4096 rows.  ``*_host`` functions return HostBatch lists
+(numpy) for parity tests; ``*_device`` functions build the full-size inputs
+directly in HBM with torch and return DeviceBatch views into them.
+"""
+from __future__ import annotations
+
+from typing import List, Tuple
+
+import numpy as np
+
+from . import types as T
+from .batch import DeviceBatch, HostBatch, HostColumn, bitmap_from_bools
+
+BATCH_ROWS = 4096
+
+SPLIT_TYPES = [T.INT64, T.INT32, T.DOUBLE, T.STRING]
+AGG_TYPES = [T.INT64, T.DOUBLE, T.INT32]
+
+
+def _rng(i: int, seed: int = 42):
+    return np.random.Generator(np.random.PCG64(seed + i))
+
+
+# ------------------------------------------------------------------ split workload
+
+def split_host(num_rows: int, batch_rows: int = BATCH_ROWS, seed: int = 42,
+               null_frac: float = 0.02) -> List[HostBatch]:
+    """int64 key U[0,2^40), int32 uniform, float64 N(0,1) with 2% nulls,
+    utf8 of length U[8,32] over [a-z]."""
+    key = _rng(0, seed).integers(0, 1 << 40, size=num_rows, dtype=np.int64)
+    i32 = _rng(1, seed).integers(-(1 << 31), (1 << 31) - 1, size=num_rows, dtype=np.int64).astype(np.int32)
+    r2 = _rng(2, seed)
+    f64 = r2.standard_normal(num_rows)
+    f64_valid = r2.random(num_rows) >= null_frac
+    r3 = _rng(3, seed)
+    lens = r3.integers(8, 33, size=num_rows).astype(np.int32)
+    chars = r3.integers(97, 123, size=int(lens.sum()), dtype=np.uint8)
+    starts = np.zeros(num_rows + 1, np.int64)
+    starts[1:] = np.cumsum(lens)
+    out = []
+    for b0 in range(0, num_rows, batch_rows):
+        b1 = min(num_rows, b0 + batch_rows)
+        n = b1 - b0
+        off = (starts[b0:b1 + 1] - starts[b0]).astype(np.int32)
+        data = chars[starts[b0]:starts[b1]].copy()
+        valid = f64_valid[b0:b1]
+        out.append(HostBatch(n, [
+            HostColumn(T.INT64, key[b0:b1].copy()),
+            HostColumn(T.INT32, i32[b0:b1].copy()),
+            HostColumn(T.DOUBLE, f64[b0:b1].copy(), bitmap_from_bools(valid) if not valid.all() else None),
+            HostColumn(T.STRING, data, None, off),
+        ]))
+    return out
+
+
+class DeviceTable:
+    """Whole-workload device tensors plus DeviceBatch views of every batch."""
+
+    def __init__(self, tensors, batches: List[DeviceBatch], num_rows: int, nbytes: int):
+        self.tensors = tensors
+        self.batches = batches
+        self.num_rows = num_rows
+        self.nbytes = nbytes
+
+
+def _bitmap_device(valid_bool):
+    """torch bool [n] (n multiple of 8) -> uint8 LSB-first bitmap on the device."""
+    import torch
+
+    n = valid_bool.numel()
+    v = valid_bool.view(-1, 8).to(torch.uint8)
+    w = torch.tensor([1, 2, 4, 8, 16, 32, 64, 128], dtype=torch.uint8, device=v.device)
+    return (v.to(torch.int32) * w.to(torch.int32)).sum(dim=1).to(torch.uint8)
+
+
+def split_device(num_rows: int, batch_rows: int = BATCH_ROWS, seed: int = 42,
+                 device: str = "cuda") -> DeviceTable:
+    """The shuffle-split workload generated in HBM (torch Philox, seeded)."""
+    import torch
+
+    assert num_rows % batch_rows == 0 and batch_rows % 64 == 0
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    key = torch.randint(0, 1 << 40, (num_rows,), dtype=torch.int64, device=device, generator=g)
+    i32 = torch.randint(-(1 << 31), (1 << 31) - 1, (num_rows,), dtype=torch.int64, device=device,
+                        generator=g).to(torch.int32)
+    f64 = torch.randn(num_rows, dtype=torch.float64, device=device, generator=g)
+    valid = torch.rand(num_rows, device=device, generator=g) >= 0.02
+    vbits = _bitmap_device(valid)
+    lens = torch.randint(8, 33, (num_rows,), dtype=torch.int32, device=device, generator=g)
+    offs = torch.zeros(num_rows + 1, dtype=torch.int64, device=device)
+    offs[1:] = torch.cumsum(lens, 0)
+    total = int(offs[-1].item())
+    assert total < (1 << 31), "utf8 data must fit int32 offsets"
+    offs32 = offs.to(torch.int32)
+    chars = torch.randint(97, 123, (total,), dtype=torch.uint8, device=device, generator=g)
+    del offs, lens
+    kb, ib, fb, vb, ob, cb = (t.view(torch.uint8) for t in (key, i32, f64, vbits, offs32, chars))
+    batches = []
+    for b0 in range(0, num_rows, batch_rows):
+        n = batch_rows
+        # offsets are absolute into the shared data buffer (the base of every batch)
+        bufs = [None, kb[b0 * 8:(b0 + n) * 8],
+                None, ib[b0 * 4:(b0 + n) * 4],
+                vb[b0 // 8:(b0 + n) // 8], fb[b0 * 8:(b0 + n) * 8],
+                None, ob[b0 * 4:(b0 + n + 1) * 4], cb]
+        batches.append(DeviceBatch(n, SPLIT_TYPES, bufs))
+    nbytes = num_rows * (8 + 4 + 8 + 4) + num_rows // 8 + total
+    return DeviceTable((key, i32, f64, vbits, offs32, chars), batches, num_rows, nbytes)
+
+
+# ------------------------------------------------------------------ aggregate workload
+
+def agg_host(num_rows: int, num_keys: int = 10000, batch_rows: int = BATCH_ROWS, seed: int = 42,
+             null_frac: float = 0.01) -> List[HostBatch]:
+    """int64 key over num_keys distinct values; float64 price ~ N(100, 25) with
+    1% nulls; int32 qty ~ U[1, 50]."""
+    keyset = _rng(10, seed).integers(-(1 << 62), 1 << 62, size=num_keys, dtype=np.int64)
+    key = keyset[_rng(0, seed).integers(0, num_keys, size=num_rows)]
+    r1 = _rng(1, seed)
+    price = r1.normal(100.0, 25.0, size=num_rows)
+    pvalid = r1.random(num_rows) >= null_frac
+    qty = _rng(2, seed).integers(1, 51, size=num_rows).astype(np.int32)
+    out = []
+    for b0 in range(0, num_rows, batch_rows):
+        b1 = min(num_rows, b0 + batch_rows)
+        v = pvalid[b0:b1]
+        out.append(HostBatch(b1 - b0, [
+            HostColumn(T.INT64, key[b0:b1].copy()),
+            HostColumn(T.DOUBLE, price[b0:b1].copy(), bitmap_from_bools(v) if not v.all() else None),
+            HostColumn(T.INT32, qty[b0:b1].copy()),
+        ]))
+    return out
+
+
+def device_from_host(batches: List[HostBatch], device: str = "cuda") -> List[DeviceBatch]:
+    return [DeviceBatch.from_host(b, device) for b in batches]

@@ -1,0 +1,285 @@
+/*
+ * gazelle_amd.h -- C ABI of the MI355X (gfx950) columnar operator hot path:
+ * the shuffle hash Splitter, the hash aggregate and the hash join.
+ *
+ * Every entry point is plain C: handles are opaque pointers, column data cross
+ * the boundary as device addresses plus sizes, and no exception leaves the
+ * library.  Each function returns a status code whose values are those of
+ * arrow::StatusCode; gz_last_error() returns the message of the last failure on
+ * the calling thread.
+ *
+ * Record batches use the JNI buffer convention of the reference
+ * (native-sql-engine/cpp/src/jni/jni_common.h:309-340, MakeArrayData): for each
+ * column in schema order, a validity buffer (address 0 or size 0 = no nulls),
+ * then an int32 offsets buffer for utf8/binary columns, then the values buffer.
+ * A null-typed column contributes two (empty) buffers.  Buffers carry no
+ * offset: row 0 is bit 0 / element 0.
+ *
+ * Thread safety: a context may be shared by threads; a Splitter, aggregate,
+ * hash relation or probe handle is used by one thread at a time (one Spark task
+ * per handle, as in the reference).
+ */
+#ifndef GAZELLE_AMD_H_
+#define GAZELLE_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (values of arrow::StatusCode) ---------------------------- */
+#define GZ_OK 0
+#define GZ_OUT_OF_MEMORY 1
+#define GZ_KEY_ERROR 2
+#define GZ_TYPE_ERROR 3
+#define GZ_INVALID 4
+#define GZ_IO_ERROR 5
+#define GZ_CAPACITY_ERROR 6
+#define GZ_INDEX_ERROR 7
+#define GZ_UNKNOWN_ERROR 9
+#define GZ_NOT_IMPLEMENTED 10
+#define GZ_HIP_ERROR 100 /* a HIP runtime call failed; message has the HIP error */
+
+/* ---- column types (values of arrow::Type::type) ---------------------------- */
+#define GZ_TYPE_NA 0
+#define GZ_TYPE_BOOL 1
+#define GZ_TYPE_UINT8 2
+#define GZ_TYPE_INT8 3
+#define GZ_TYPE_UINT16 4
+#define GZ_TYPE_INT16 5
+#define GZ_TYPE_UINT32 6
+#define GZ_TYPE_INT32 7
+#define GZ_TYPE_UINT64 8
+#define GZ_TYPE_INT64 9
+#define GZ_TYPE_FLOAT 11
+#define GZ_TYPE_DOUBLE 12
+#define GZ_TYPE_STRING 13
+#define GZ_TYPE_BINARY 14
+#define GZ_TYPE_DATE32 16
+#define GZ_TYPE_DATE64 17
+#define GZ_TYPE_TIMESTAMP 18
+#define GZ_TYPE_DECIMAL128 23
+
+/* ---- context ----------------------------------------------------------------
+ * A context owns one device, one HIP stream and an HBM budget.  All device
+ * memory the library allocates (partition buffers, hash tables, aggregate
+ * states, join outputs) is charged to the budget; an allocation beyond it fails
+ * with GZ_OUT_OF_MEMORY (the analogue of the Splitter's capped memory pool,
+ * reference shuffle_split_test.cc:47-95).
+ */
+typedef struct gz_context gz_context;
+
+/* Replaces: the per-task arrow::MemoryPool handed to the kernels
+ * (jni_wrapper.cc:1081-1085).  hbm_budget_bytes <= 0 means "no cap". */
+int gz_context_create(int device, int64_t hbm_budget_bytes, gz_context** out);
+int gz_context_destroy(gz_context* ctx);
+/* Launch on a caller-owned HIP stream (hipStream_t) instead of the context's. */
+int gz_context_set_stream(gz_context* ctx, void* hip_stream);
+void* gz_context_stream(gz_context* ctx);
+int gz_context_synchronize(gz_context* ctx);
+int64_t gz_context_bytes_in_use(gz_context* ctx);
+int64_t gz_context_peak_bytes(gz_context* ctx);
+/* Kernel timing with HIP events on the launch stream (off by default). */
+int gz_context_set_profiling(gz_context* ctx, int enabled);
+/* Accumulated time (ms) and launch count of a named kernel family: "split_count",
+ * "split_scatter", "agg_update", "join_build", "join_probe", ... */
+int gz_context_kernel_time(gz_context* ctx, const char* name, double* total_ms,
+                           int64_t* launches);
+int gz_context_reset_kernel_times(gz_context* ctx);
+const char* gz_last_error(void);
+
+/* One record batch in the JNI buffer convention (device addresses). */
+typedef struct {
+  int64_t num_rows;
+  const int64_t* buf_addrs;
+  const int64_t* buf_sizes;
+  int32_t num_bufs;
+} gz_batch;
+
+/* ---- shuffle Splitter -------------------------------------------------------
+ * Replaces sparkcolumnarplugin::shuffle::Splitter (cpp/src/shuffle/splitter.h:38-295)
+ * as driven by ShuffleSplitterJniWrapper (jni_wrapper.cc:1031-1234).
+ */
+typedef struct gz_splitter gz_splitter;
+
+typedef struct {
+  int32_t buffer_size;        /* SplitOptions.buffer_size (type.h:41), default 4096 */
+  int32_t prefer_spill;       /* SplitOptions.prefer_spill (type.h:45), default 1 */
+  int64_t offheap_per_task;   /* SplitOptions.offheap_per_task (type.h:40) */
+  int32_t tile_rows;          /* rows per workgroup tile: 0 = autotuned */
+  int32_t hash_seed;          /* 42, the seed Spark uses (splitter.cc:1405) */
+} gz_split_options;
+
+void gz_split_options_defaults(gz_split_options* opts);
+
+/* Replaces Splitter::Make(short_name, schema, num_partitions, expr_vector, options)
+ * (splitter.h:55-58, splitter.cc:190-214).  short_name: "hash", "rr", "range",
+ * "single".  The Gandiva key expressions become column indices (field(k)
+ * references, as ColumnarShuffleExchangeExec.scala:307-321 builds them).  For
+ * "range" column 0 is the int32 partition id and is removed from the output
+ * (splitter.cc:1497-1500). */
+int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
+                  const int32_t* col_types, int32_t num_partitions,
+                  const int32_t* key_cols, int32_t num_keys,
+                  const gz_split_options* opts, gz_splitter** out);
+
+/* Replaces Splitter::Split(const RecordBatch&) (splitter.cc:387-393) /
+ * ShuffleSplitterJniWrapper.split (jni_wrapper.cc:1156-1195): one batch, one
+ * launch sequence (the plain per-batch path).  The buffers are borrowed for the
+ * duration of the call only. */
+int gz_split_batch(gz_splitter* h, int64_t num_rows, const int64_t* buf_addrs,
+                   const int64_t* buf_sizes, int32_t num_bufs);
+
+/* Batched form: many (uneven) batches split in one launch sequence; the result
+ * is the same as calling gz_split_batch on each in order. */
+int gz_split_batches(gz_splitter* h, int32_t num_batches, const gz_batch* batches);
+
+/* Replaces Splitter::Stop (splitter.cc:395-437): completes all device work.
+ * The partitioned output stays resident and is read with the calls below. */
+int gz_split_stop(gz_splitter* h);
+
+/* Replaces Splitter::PartitionLengths in rows (splitter.h:114); out[P]. */
+int gz_split_partition_rows(gz_splitter* h, int64_t* out);
+int gz_split_num_partitions(gz_splitter* h);
+/* Output columns (the "range" pid column excluded) and their types. */
+int gz_split_num_output_columns(gz_splitter* h);
+
+/* The output is a list of segments (one per gz_split_batch/gz_split_batches
+ * call).  Inside a segment, the rows of partition p are contiguous, in input
+ * order, at row positions [row_start[p], row_start[p+1]).  Per-partition
+ * concatenation of the segments in order is the reference's partition stream.
+ *   values:   fixed-width values at global row positions (bool: bitmap)
+ *   validity: bitmap at global row positions, trailing bits 1, or NULL when
+ *             the column has no validity in the input
+ *   offsets:  utf8/binary only; partition p's int32 offsets, starting at 0,
+ *             occupy entries [row_start[p] + p, row_start[p+1] + p]
+ *   data:     utf8/binary bytes; partition p's run starts at byte_start[p]
+ * All pointers are device addresses owned by the splitter. */
+int gz_split_num_segments(gz_splitter* h);
+int gz_split_segment_rows(gz_splitter* h, int32_t seg, int64_t* row_start /* P+1 */);
+int gz_split_segment_column(gz_splitter* h, int32_t seg, int32_t col, void** values,
+                            void** validity, void** offsets, void** data,
+                            int64_t* byte_start /* P+1, may be NULL */,
+                            int64_t* null_count);
+/* Per-batch, per-partition row counts of a segment (counts[b * P + p]); the host
+ * side uses them to reproduce the reference's output batch boundaries. */
+int gz_split_segment_batch_counts(gz_splitter* h, int32_t seg, int32_t* num_batches,
+                                  int32_t* counts /* may be NULL */);
+/* Replaces Splitter::TotalComputePidTime (splitter.h:112), in nanoseconds;
+ * measured only when profiling is enabled on the context. */
+int64_t gz_split_compute_pid_time(gz_splitter* h);
+int gz_split_release(gz_splitter* h);
+
+/* ---- hash aggregate ----------------------------------------------------------
+ * Replaces HashAggregateKernel + HashAggregateResultIterator<number>
+ * (ext/hash_aggregate_kernel.cc:51-928, kernels_ext.h:285-313).  The Gandiva
+ * action nodes become an action list with the reference's action names.
+ */
+typedef struct gz_agg gz_agg;
+
+#define GZ_ACTION_GROUPBY 0      /* action_groupby (UniqueAction, actions_impl.cc:132-290) */
+#define GZ_ACTION_COUNT 1        /* action_count (CountAction, :293-454) */
+#define GZ_ACTION_COUNT_LITERAL 2 /* action_countLiteral_<n> (:634-755) */
+#define GZ_ACTION_SUM 3          /* action_sum (SumAction, :1982-2143) */
+#define GZ_ACTION_AVG 4          /* action_avg (AvgAction, :2656-2829) */
+#define GZ_ACTION_MIN 5          /* action_min[_true] (MinAction, :756-1041) */
+#define GZ_ACTION_MAX 6          /* action_max[_true] (MaxAction, :1372-1654) */
+#define GZ_ACTION_SUM_COUNT 7    /* action_sum_count (SumCountAction, :3012-3200) */
+#define GZ_ACTION_AVG_BY_COUNT 8 /* action_avgByCount (AvgByCountAction, :3746-3925) */
+
+typedef struct {
+  int32_t kind;       /* GZ_ACTION_* */
+  int32_t input0;     /* input column index, -1 if none */
+  int32_t input1;     /* second input (avgByCount count column), -1 if none */
+  int32_t nan_check;  /* action_min_true / action_max_true */
+} gz_agg_action;
+
+typedef struct {
+  int32_t placement;      /* 0 = autotuned, 1 = LDS group table, 2 = HBM table */
+  int32_t tile_rows;      /* 0 = autotuned */
+  double load_factor;     /* HBM table max load, 0 = 0.5 */
+  int64_t expected_groups;/* sizing hint, 0 = unknown */
+} gz_agg_options;
+
+void gz_agg_options_defaults(gz_agg_options* opts);
+
+/* Replaces HashAggregateKernel::Make + MakeResultIterator (hash_aggregate_kernel.cc:1238-1260,
+ * :147-224).  A plan has at most one GZ_ACTION_GROUPBY (single numeric key) or
+ * none (global aggregate). */
+int gz_agg_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types,
+                int32_t num_actions, const gz_agg_action* actions,
+                const gz_agg_options* opts, gz_agg** out);
+/* Replaces ResultIterator::ProcessAndCacheOne (hash_aggregate_kernel.cc:805-893). */
+int gz_agg_process(gz_agg* h, int64_t num_rows, const int64_t* buf_addrs,
+                   const int64_t* buf_sizes, int32_t num_bufs);
+int gz_agg_process_batches(gz_agg* h, int32_t num_batches, const gz_batch* batches);
+/* Number of groups so far (max_group_id + 1), completes device work. */
+int gz_agg_num_groups(gz_agg* h, int64_t* out);
+/* Number of result columns and their types (sum_count yields two). */
+int gz_agg_num_result_columns(gz_agg* h);
+int gz_agg_result_type(gz_agg* h, int32_t col);
+/* Replaces HasNext/Next (hash_aggregate_kernel.cc:895-913): finalizes the
+ * group order (first-seen, as the reference's SparseHashMap assigns ids) and
+ * exposes device result columns of num_groups rows: values (element width of
+ * the result type) and validity as one byte per group (1 = valid). */
+int gz_agg_finish(gz_agg* h);
+int gz_agg_result_column(gz_agg* h, int32_t col, void** values, void** valid_bytes);
+int gz_agg_release(gz_agg* h);
+
+/* ---- hash join -----------------------------------------------------------------
+ * Build replaces HashRelationKernel (ext/hash_relation_kernel.cc:50-344) and
+ * HashRelation (codegen/common/hash_relation.h:123-645); probe replaces
+ * ConditionedProbeKernel / ConditionedProbeResultIterator
+ * (ext/conditioned_probe_kernel.cc:54-1772).
+ */
+typedef struct gz_relation gz_relation;
+typedef struct gz_probe gz_probe;
+
+#define GZ_JOIN_INNER 0
+#define GZ_JOIN_ANTI 2
+#define GZ_JOIN_SEMI 3
+
+/* builder_type 1 = all rows (unsafeHashMap append), 3 = skip duplicate keys
+ * (InsertSkipDup, for semi joins) -- hash_relation_kernel.cc:79-99. */
+int gz_relation_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types,
+                     int32_t key_col, int32_t builder_type, gz_relation** out);
+/* Replaces HashRelationKernel::Evaluate (hash_relation_kernel.cc:170-198): the
+ * batch is copied into relation-owned HBM. */
+int gz_relation_append(gz_relation* r, int64_t num_rows, const int64_t* buf_addrs,
+                       const int64_t* buf_sizes, int32_t num_bufs);
+int gz_relation_append_batches(gz_relation* r, int32_t num_batches, const gz_batch* batches);
+/* Replaces FinishInternal (hash_relation_kernel.cc:200-281). */
+int gz_relation_finish(gz_relation* r);
+int gz_relation_num_keys(gz_relation* r, int64_t* out);
+int gz_relation_has_null_key(gz_relation* r, int32_t* out);
+int gz_relation_release(gz_relation* r);
+
+/* Replaces ConditionedProbeKernel::Make + SetDependencies
+ * (conditioned_probe_kernel.cc:1745-1772, :380-465).  result_cols lists the
+ * output columns: side 0 = build ("left" in the reference), side 1 = probe
+ * ("right"), with the column index on that side. */
+int gz_probe_make(gz_relation* r, int32_t num_probe_cols, const int32_t* probe_types,
+                  int32_t probe_key_col, int32_t join_type, int32_t null_aware_anti,
+                  int32_t num_result_cols, const int32_t* result_sides,
+                  const int32_t* result_indices, gz_probe** out);
+/* Replaces ConditionedProbeResultIterator::Process (:468-519): probes one batch
+ * and materializes one output batch in probe-row order (matches of one probe row
+ * in build insertion order).  Output columns are device buffers owned by the
+ * probe handle, valid until the next call: values + validity bytes (1 = valid);
+ * for utf8 columns offsets + data. */
+int gz_probe_process(gz_probe* p, int64_t num_rows, const int64_t* buf_addrs,
+                     const int64_t* buf_sizes, int32_t num_bufs, int64_t* out_rows);
+/* Batched form: probes many batches in one launch sequence; the output is the
+ * concatenation of the per-batch outputs, out_batch_rows[b] rows each. */
+int gz_probe_process_batches(gz_probe* p, int32_t num_batches, const gz_batch* batches,
+                             int64_t* out_rows, int64_t* out_batch_rows);
+int gz_probe_result_column(gz_probe* p, int32_t col, void** values, void** valid_bytes,
+                           void** offsets, void** data);
+int gz_probe_release(gz_probe* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GAZELLE_AMD_H_ */

@@ -1,0 +1,307 @@
+"""Python wrapper of the CPU oracle (libgz_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: this package is the checker for the HIP path
+(``tests/``, ``__graft_entry__.smoke``) and the CPU baseline leg of ``bench.py``.
+The product (``gazelle_plugin_amd``) never imports it.
+
+Each function restates a reference algorithm; see the C sources for the file:line
+each one follows:
+  split.c  -- Splitter hash partitioning + DoSplit   (cpp/src/shuffle/splitter.cc)
+  agg.c    -- HashAggregateResultIterator + actions  (ext/hash_aggregate_kernel.cc, ext/actions_impl.cc)
+  join.c   -- HashRelation / unsafeHashMap + probes  (codegen/common/hash_relation.h,
+              third_party/row_wise_memory/hashMap.h, ext/conditioned_probe_kernel.cc)
+
+Pins (what checks the oracle itself) are listed in oracle/README.md.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libgz_oracle.so")
+_lib = None
+
+_T_NA, _T_BOOL, _T_STRING, _T_BINARY = 0, 1, 13, 14
+_WIDTH = {2: 1, 3: 1, 4: 2, 5: 2, 6: 4, 7: 4, 11: 4, 16: 4, 8: 8, 9: 8, 12: 8, 17: 8, 18: 8, 23: 16}
+
+
+class _Batch(ctypes.Structure):
+    _fields_ = [("num_rows", ctypes.c_int64), ("bufs", ctypes.POINTER(ctypes.c_void_p))]
+
+
+def build() -> str:
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.gzo_hash_int.restype = ctypes.c_int32
+        L.gzo_hash_int.argtypes = [ctypes.c_int32, ctypes.c_int32]
+        L.gzo_hash_long.restype = ctypes.c_int32
+        L.gzo_hash_long.argtypes = [ctypes.c_int64, ctypes.c_int32]
+        L.gzo_hash_bytes.restype = ctypes.c_int32
+        L.gzo_hash_bytes.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32]
+        L.gzo_hash32_numeric.restype = ctypes.c_int32
+        L.gzo_hash32_numeric.argtypes = [ctypes.c_double]
+        L.gzo_split_pids.restype = ctypes.c_int
+        L.gzo_splitter_make.restype = ctypes.c_void_p
+        L.gzo_splitter_rows.restype = ctypes.c_int64
+        L.gzo_splitter_rows.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.gzo_splitter_split.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Batch)]
+        L.gzo_splitter_free.argtypes = [ctypes.c_void_p]
+        L.gzo_agg_make.restype = ctypes.c_void_p
+        L.gzo_agg_process.argtypes = [ctypes.c_void_p, ctypes.POINTER(_Batch)]
+        L.gzo_agg_num_groups.restype = ctypes.c_int64
+        L.gzo_agg_num_groups.argtypes = [ctypes.c_void_p]
+        L.gzo_agg_num_results.argtypes = [ctypes.c_void_p]
+        L.gzo_agg_result_type.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.gzo_agg_result.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.gzo_agg_free.argtypes = [ctypes.c_void_p]
+        L.gzo_rel_make.restype = ctypes.c_void_p
+        L.gzo_rel_make.argtypes = [ctypes.c_int32, ctypes.c_int32]
+        L.gzo_rel_append.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+        L.gzo_rel_finish.argtypes = [ctypes.c_void_p]
+        L.gzo_rel_num_keys.restype = ctypes.c_int64
+        L.gzo_rel_num_keys.argtypes = [ctypes.c_void_p]
+        L.gzo_rel_has_null.argtypes = [ctypes.c_void_p]
+        L.gzo_probe.restype = ctypes.c_int64
+        L.gzo_probe.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        L.gzo_rel_free.argtypes = [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class _CBatches:
+    """Keeps the ctypes views of a list of HostBatch-like objects alive."""
+
+    def __init__(self, batches):
+        self.keep = []
+        self.arr = (_Batch * max(1, len(batches)))()
+        for i, b in enumerate(batches):
+            slots = []
+            for c in b.columns:
+                for a in (c.validity, c.offsets, c.values):
+                    if a is not None:
+                        a = np.ascontiguousarray(a)
+                        self.keep.append(a)
+                        slots.append(a.ctypes.data)
+                    else:
+                        slots.append(None)
+            bufs = (ctypes.c_void_p * max(1, len(slots)))(*slots)
+            self.keep.append(bufs)
+            self.arr[i].num_rows = b.num_rows
+            self.arr[i].bufs = ctypes.cast(bufs, ctypes.POINTER(ctypes.c_void_p))
+
+
+# ------------------------------------------------------------------ hashing
+
+def hash_int(v: int, seed: int = 42) -> int:
+    return lib().gzo_hash_int(int(np.int32(v)), seed)
+
+
+def hash_long(v: int, seed: int = 42) -> int:
+    return lib().gzo_hash_long(int(np.int64(v)), seed)
+
+
+def hash_bytes(b: bytes, seed: int = 42) -> int:
+    return lib().gzo_hash_bytes(b, len(b), seed)
+
+
+def pmod(h: int, p: int) -> int:
+    r = int(np.fmod(np.int64(h), p))
+    return r + p if r < 0 else r
+
+
+# ------------------------------------------------------------------ split
+
+MODES = {"hash": 0, "rr": 1, "range": 2, "single": 3}
+
+
+def split_pids(batches, types: Sequence[int], P: int, keys: Sequence[int] = (), mode: str = "hash",
+               seed: int = 42) -> np.ndarray:
+    L = lib()
+    out = []
+    rr = ctypes.c_int32(0)
+    t = (ctypes.c_int32 * len(types))(*types)
+    k = (ctypes.c_int32 * max(1, len(keys)))(*keys)
+    for b in batches:
+        cb = _CBatches([b])
+        pids = np.zeros(max(1, b.num_rows), dtype=np.int32)
+        rc = L.gzo_split_pids(ctypes.c_int(MODES[mode]), ctypes.byref(cb.arr[0]), len(types), t, k,
+                              len(keys), ctypes.c_int(P if mode != "single" else 1),
+                              ctypes.c_int32(seed), ctypes.byref(rr), _ptr(pids))
+        if rc != 0:
+            raise ValueError(f"oracle split_pids failed with status {rc}")
+        out.append(pids[: b.num_rows])
+    return np.concatenate(out) if out else np.zeros(0, np.int32)
+
+
+def split(batches, types: Sequence[int], P: int, keys: Sequence[int] = (), mode: str = "hash",
+          seed: int = 42, pids: Optional[np.ndarray] = None) -> Dict:
+    """Concatenated partition layout of the reference split of ``batches``.
+
+    For mode "range" column 0 carries the pids and is dropped from the output,
+    as FallbackRangeSplitter does.
+    """
+    L = lib()
+    if pids is None:
+        pids = split_pids(batches, types, P, keys, mode, seed)
+    if mode == "single":
+        P = 1
+    if mode == "range":
+        batches = [type(b)(b.num_rows, b.columns[1:]) for b in batches]
+        types = list(types[1:])
+    ncols = len(types)
+    nbin = sum(1 for t in types if t in (_T_STRING, _T_BINARY))
+    N = int(sum(b.num_rows for b in batches))
+    cb = _CBatches(batches)
+    tarr = (ctypes.c_int32 * max(1, ncols))(*types)
+    row_start = np.zeros(P + 1, dtype=np.int64)
+    byte_start = np.zeros(max(1, nbin) * (P + 1), dtype=np.int64)
+    pids = np.ascontiguousarray(pids, dtype=np.int32)
+    L.gzo_split_layout(cb.arr, len(batches), ncols, tarr, P, _ptr(pids), _ptr(row_start), _ptr(byte_start))
+    values, validity, offsets, data = [], [], [], []
+    j = 0
+    bs_list = []
+    for c, t in enumerate(types):
+        has_valid = any(b.columns[c].validity is not None for b in batches)
+        nwords = (N + 63) // 64
+        validity.append(np.full(nwords * 8, 0xFF, np.uint8) if has_valid and t != _T_NA else None)
+        if t in (_T_STRING, _T_BINARY):
+            bs = byte_start[j * (P + 1):(j + 1) * (P + 1)].copy()
+            bs_list.append(bs)
+            offsets.append(np.zeros(N + P, np.int32))
+            data.append(np.zeros(max(1, int(bs[-1])), np.uint8))
+            values.append(None)
+            j += 1
+        else:
+            bs_list.append(None)
+            offsets.append(None)
+            data.append(None)
+            if t == _T_BOOL:
+                values.append(np.full(nwords * 8, 0xFF, np.uint8))
+            elif t == _T_NA:
+                values.append(None)
+            else:
+                values.append(np.zeros(max(1, N) * _WIDTH[t], np.uint8))
+    vp = (ctypes.c_void_p * max(1, ncols))(*[v.ctypes.data if v is not None else None for v in values])
+    vb = (ctypes.c_void_p * max(1, ncols))(*[v.ctypes.data if v is not None else None for v in validity])
+    op = (ctypes.c_void_p * max(1, ncols))(*[v.ctypes.data if v is not None else None for v in offsets])
+    dp = (ctypes.c_void_p * max(1, ncols))(*[v.ctypes.data if v is not None else None for v in data])
+    L.gzo_split_scatter(cb.arr, len(batches), ncols, tarr, P, _ptr(pids), _ptr(row_start),
+                        _ptr(byte_start), vp, vb, op, dp)
+    cols = []
+    for c, t in enumerate(types):
+        col = {"type": t, "validity": validity[c]}
+        if t in (_T_STRING, _T_BINARY):
+            col["offsets"] = offsets[c]
+            col["data"] = data[c][: int(bs_list[c][-1])]
+            col["byte_start"] = bs_list[c]
+        elif values[c] is not None:
+            col["values"] = values[c] if t == _T_BOOL else values[c][: N * _WIDTH[t]]
+        cols.append(col)
+    return {"P": P, "num_rows": N, "row_start": row_start, "pids": pids, "columns": cols}
+
+
+class CpuSplitter:
+    """The reference per-batch split loop (for timing the CPU baseline)."""
+
+    def __init__(self, types: Sequence[int], P: int, keys: Sequence[int], seed: int = 42):
+        L = lib()
+        L.gzo_splitter_make.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_int, ctypes.c_int32]
+        self._t = (ctypes.c_int32 * len(types))(*types)
+        self._k = (ctypes.c_int32 * max(1, len(keys)))(*keys)
+        self.P = P
+        self.h = L.gzo_splitter_make(len(types), self._t, P, self._k, len(keys), seed)
+
+    def split(self, cbatches: "_CBatches", i: int):
+        lib().gzo_splitter_split(self.h, ctypes.byref(cbatches.arr[i]))
+
+    def rows(self) -> np.ndarray:
+        return np.array([lib().gzo_splitter_rows(self.h, p) for p in range(self.P)], np.int64)
+
+    def close(self):
+        if self.h:
+            lib().gzo_splitter_free(self.h)
+            self.h = None
+
+
+# ------------------------------------------------------------------ aggregate
+
+def aggregate(batches, types: Sequence[int], actions: Sequence[Sequence[int]]) -> List:
+    """actions: (kind, in0, in1, nan_check) tuples.  Returns [(type, values, valid)]
+    per result column, groups in first-seen order."""
+    L = lib()
+    L.gzo_agg_make.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    na = len(actions)
+    kinds = (ctypes.c_int32 * na)(*[a[0] for a in actions])
+    in0 = (ctypes.c_int32 * na)(*[a[1] for a in actions])
+    in1 = (ctypes.c_int32 * na)(*[a[2] for a in actions])
+    nan = (ctypes.c_int32 * na)(*[a[3] for a in actions])
+    tarr = (ctypes.c_int32 * len(types))(*types)
+    h = L.gzo_agg_make(len(types), tarr, na, kinds, in0, in1, nan)
+    try:
+        for b in batches:
+            cb = _CBatches([b])
+            L.gzo_agg_process(h, ctypes.byref(cb.arr[0]))
+        g = L.gzo_agg_num_groups(h)
+        out = []
+        for c in range(L.gzo_agg_num_results(h)):
+            t = L.gzo_agg_result_type(h, c)
+            vals = np.zeros(max(1, g) * _WIDTH.get(t, 8), np.uint8)
+            valid = np.zeros(max(1, g), np.uint8)
+            L.gzo_agg_result(h, c, _ptr(vals), _ptr(valid))
+            dt = {2: np.uint8, 3: np.int8, 4: np.uint16, 5: np.int16, 6: np.uint32, 7: np.int32,
+                  8: np.uint64, 9: np.int64, 11: np.float32, 12: np.float64, 16: np.int32,
+                  17: np.int64, 18: np.int64}[t]
+            out.append((t, vals[: g * _WIDTH.get(t, 8)].view(dt).copy(), valid[:g].astype(bool)))
+        return out
+    finally:
+        L.gzo_agg_free(h)
+
+
+# ------------------------------------------------------------------ join
+
+def join(build_keys: Sequence, probe_keys, key_type: int, join_type: int, null_aware: bool = False,
+         builder_type: int = 1):
+    """build_keys: list of (values ndarray, validity bitmap or None) per build batch.
+    probe_keys: list of (values, validity, n) per probe batch.
+    Returns per probe batch (probe_rows, build_rows) for inner, probe_rows otherwise.
+    Build rows are global indices (batch order, then row order)."""
+    L = lib()
+    r = L.gzo_rel_make(key_type, builder_type)
+    try:
+        keep = []
+        for vals, valid, n in build_keys:
+            vals = np.ascontiguousarray(vals)
+            keep.append(vals)
+            L.gzo_rel_append(r, _ptr(vals), _ptr(valid), n)
+        L.gzo_rel_finish(r)
+        outs = []
+        for vals, valid, n in probe_keys:
+            vals = np.ascontiguousarray(vals)
+            cnt = L.gzo_probe(r, join_type, int(null_aware), _ptr(vals), _ptr(valid), n, None, None)
+            pr = np.zeros(max(1, cnt), np.int64)
+            br = np.full(max(1, cnt), -1, np.int64)
+            L.gzo_probe(r, join_type, int(null_aware), _ptr(vals), _ptr(valid), n, _ptr(pr), _ptr(br))
+            outs.append((pr[:cnt], br[:cnt]) if join_type == 0 else pr[:cnt])
+        return outs
+    finally:
+        L.gzo_rel_free(r)

@@ -1,0 +1,382 @@
+/*
+ * agg.c -- CPU restatement of the reference hash aggregate
+ * (ext/hash_aggregate_kernel.cc:778-928 with the accumulators of
+ * ext/actions_impl.cc).  TEST INFRASTRUCTURE ONLY (checker + CPU baseline).
+ *
+ * Group ids are dense and assigned in first-seen order, as SparseHashMap<T>
+ * assigns them (third_party/sparsehash/sparse_hash_map.h:39-47); the null key
+ * gets its own id when first seen (:49-58).  The map below is an
+ * open-addressing table; only the id order is observable, and it does not
+ * depend on the map implementation.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gz_oracle.h"
+
+enum { T_NA = 0, T_BOOL = 1, T_UINT8 = 2, T_INT8 = 3, T_UINT16 = 4, T_INT16 = 5,
+       T_UINT32 = 6, T_INT32 = 7, T_UINT64 = 8, T_INT64 = 9, T_FLOAT = 11, T_DOUBLE = 12,
+       T_DATE32 = 16, T_DATE64 = 17, T_TIMESTAMP = 18 };
+enum { A_GROUPBY = 0, A_COUNT = 1, A_COUNT_LITERAL = 2, A_SUM = 3, A_AVG = 4, A_MIN = 5,
+       A_MAX = 6, A_SUM_COUNT = 7, A_AVG_BY_COUNT = 8 };
+
+static int is_float(int32_t t) { return t == T_FLOAT || t == T_DOUBLE; }
+static int is_unsigned(int32_t t) { return t == T_UINT8 || t == T_UINT16 || t == T_UINT32 || t == T_UINT64; }
+static int width_of(int32_t t) {
+  switch (t) {
+    case T_UINT8: case T_INT8: return 1;
+    case T_UINT16: case T_INT16: return 2;
+    case T_UINT32: case T_INT32: case T_FLOAT: case T_DATE32: return 4;
+    default: return 8;
+  }
+}
+
+static inline int bit_get(const uint8_t* bm, int64_t i) { return (bm[i >> 3] >> (i & 7)) & 1; }
+
+/* integer value widened to int64 (unsigned types zero-extended) */
+static int64_t get_i64(const void* v, int32_t t, int64_t r) {
+  switch (t) {
+    case T_INT8: return ((const int8_t*)v)[r];
+    case T_UINT8: return ((const uint8_t*)v)[r];
+    case T_INT16: return ((const int16_t*)v)[r];
+    case T_UINT16: return ((const uint16_t*)v)[r];
+    case T_INT32: case T_DATE32: return ((const int32_t*)v)[r];
+    case T_UINT32: return ((const uint32_t*)v)[r];
+    default: return ((const int64_t*)v)[r];
+  }
+}
+static double get_f64(const void* v, int32_t t, int64_t r) {
+  if (t == T_FLOAT) return ((const float*)v)[r];
+  if (t == T_DOUBLE) return ((const double*)v)[r];
+  if (t == T_UINT64) return (double)((const uint64_t*)v)[r];
+  return (double)get_i64(v, t, r);
+}
+
+typedef struct {
+  int32_t kind, in0, in1, nan_check;
+  int32_t in_type;
+  /* per-group state, grown with the group count (GrowByFactor) */
+  double* dsum;    /* sum / avg / sum_count sum (floating) */
+  int64_t* isum;   /* integer sum, count, countLiteral */
+  int64_t* cnt;    /* avg / sum_count / avgByCount counts */
+  int64_t* imm;    /* integer min/max */
+  double* fmm;     /* floating min/max */
+  uint8_t* valid;  /* cache_validity_ */
+  uint8_t* nullf;  /* UniqueAction null_flag_ */
+  int64_t* key;    /* UniqueAction cache_ (bit pattern) */
+} action_t;
+
+struct gzo_agg {
+  int ncols;
+  int32_t* types;
+  int nactions;
+  action_t* acts;
+  int key_col;          /* -1 = no group by */
+  /* first-seen map */
+  int64_t* mkeys;
+  int32_t* mids;
+  uint8_t* mused;
+  int64_t mcap, msize;
+  int32_t null_id;      /* -1 until the null key is seen */
+  int64_t ngroups, gcap;
+  int global_seen;
+};
+
+static uint64_t mix64(uint64_t x) {
+  x ^= x >> 30; x *= 0xbf58476d1ce4e5b9ULL;
+  x ^= x >> 27; x *= 0x94d049bb133111ebULL;
+  x ^= x >> 31;
+  return x;
+}
+
+static void map_grow(gzo_agg* a) {
+  int64_t nc = a->mcap ? a->mcap * 2 : 1024;
+  int64_t* nk = (int64_t*)malloc(sizeof(int64_t) * nc);
+  int32_t* ni = (int32_t*)malloc(sizeof(int32_t) * nc);
+  uint8_t* nu = (uint8_t*)calloc((size_t)nc, 1);
+  for (int64_t i = 0; i < a->mcap; i++) {
+    if (!a->mused[i]) continue;
+    uint64_t h = mix64((uint64_t)a->mkeys[i]) & (uint64_t)(nc - 1);
+    while (nu[h]) h = (h + 1) & (uint64_t)(nc - 1);
+    nu[h] = 1; nk[h] = a->mkeys[i]; ni[h] = a->mids[i];
+  }
+  free(a->mkeys); free(a->mids); free(a->mused);
+  a->mkeys = nk; a->mids = ni; a->mused = nu; a->mcap = nc;
+}
+
+static void grow_groups(gzo_agg* a, int64_t need) {
+  if (need <= a->gcap) return;
+  int64_t nc = a->gcap ? a->gcap : 128; /* GrowByFactor: at least 128 */
+  while (nc < need) nc *= 2;
+  for (int i = 0; i < a->nactions; i++) {
+    action_t* x = &a->acts[i];
+#define GROW(f, T) do { x->f = (T*)realloc(x->f, sizeof(T) * nc); memset(x->f + a->gcap, 0, sizeof(T) * (nc - a->gcap)); } while (0)
+    GROW(dsum, double); GROW(isum, int64_t); GROW(cnt, int64_t); GROW(imm, int64_t);
+    GROW(fmm, double); GROW(valid, uint8_t); GROW(nullf, uint8_t); GROW(key, int64_t);
+#undef GROW
+  }
+  a->gcap = nc;
+}
+
+/* SparseHashMap::GetOrInsert: the id of a key, a new id (= size) if unseen */
+static int32_t get_or_insert(gzo_agg* a, int64_t k) {
+  if ((a->msize + 1) * 2 > a->mcap) map_grow(a);
+  uint64_t h = mix64((uint64_t)k) & (uint64_t)(a->mcap - 1);
+  while (a->mused[h]) {
+    if (a->mkeys[h] == k) return a->mids[h];
+    h = (h + 1) & (uint64_t)(a->mcap - 1);
+  }
+  a->mused[h] = 1; a->mkeys[h] = k; a->mids[h] = (int32_t)a->ngroups;
+  a->msize++;
+  return (int32_t)a->ngroups++;
+}
+
+gzo_agg* gzo_agg_make(int ncols, const int32_t* types, int nactions, const int32_t* kinds,
+                      const int32_t* in0, const int32_t* in1, const int32_t* nan_check) {
+  gzo_agg* a = (gzo_agg*)calloc(1, sizeof(gzo_agg));
+  a->ncols = ncols;
+  a->types = (int32_t*)malloc(sizeof(int32_t) * ncols);
+  memcpy(a->types, types, sizeof(int32_t) * ncols);
+  a->nactions = nactions;
+  a->acts = (action_t*)calloc(nactions, sizeof(action_t));
+  a->key_col = -1;
+  a->null_id = -1;
+  for (int i = 0; i < nactions; i++) {
+    a->acts[i].kind = kinds[i];
+    a->acts[i].in0 = in0[i];
+    a->acts[i].in1 = in1[i];
+    a->acts[i].nan_check = nan_check[i];
+    a->acts[i].in_type = in0[i] >= 0 ? types[in0[i]] : T_INT64;
+    if (kinds[i] == A_GROUPBY) a->key_col = in0[i];
+  }
+  return a;
+}
+
+/* One row of one action: the on_valid lambdas of actions_impl.cc. */
+static void update(action_t* x, const gzo_batch* b, int64_t r, int32_t g) {
+  const uint8_t* vb = x->in0 >= 0 ? (const uint8_t*)b->bufs[3 * x->in0] : NULL;
+  const void* v = x->in0 >= 0 ? b->bufs[3 * x->in0 + 2] : NULL;
+  int is_null = vb && !bit_get(vb, r);
+  int32_t t = x->in_type;
+  switch (x->kind) {
+    case A_GROUPBY: /* UniqueAction (:159-177): first value or null flag */
+      if (!x->valid[g]) {
+        x->valid[g] = 1;
+        if (is_null) x->nullf[g] = 1;
+        else if (t == T_DOUBLE) memcpy(&x->key[g], (const double*)v + r, 8);
+        else if (t == T_FLOAT) { float f = ((const float*)v)[r]; memcpy(&x->key[g], &f, 4); }
+        else x->key[g] = get_i64(v, t, r);
+      }
+      break;
+    case A_COUNT: { /* CountAction (:329-351): rows whose inputs are all non-null */
+      int nn = !is_null;
+      if (x->in1 >= 0) {
+        const uint8_t* vb1 = (const uint8_t*)b->bufs[3 * x->in1];
+        if (vb1 && !bit_get(vb1, r)) nn = 0;
+      }
+      x->isum[g] += nn;
+      break;
+    }
+    case A_COUNT_LITERAL: x->isum[g] += 1; break; /* (:664-667) */
+    case A_SUM: /* SumAction (:2021-2042) */
+      if (!is_null) {
+        x->valid[g] = 1;
+        if (is_float(t)) x->dsum[g] += get_f64(v, t, r);
+        else x->isum[g] += get_i64(v, t, r);
+      }
+      break;
+    case A_AVG: /* AvgAction (:2690-2710) */
+    case A_SUM_COUNT: /* SumCountAction (:3050-3075): sum in double */
+      if (!is_null) {
+        x->valid[g] = 1;
+        x->dsum[g] += get_f64(v, t, r);
+        x->cnt[g] += 1;
+      }
+      break;
+    case A_AVG_BY_COUNT: { /* AvgByCountAction (:3785-3795) */
+      if (!is_null) {
+        const void* cv = b->bufs[3 * x->in1 + 2];
+        x->dsum[g] += get_f64(v, t, r);
+        x->cnt[g] += ((const int64_t*)cv)[r];
+      }
+      break;
+    }
+    case A_MIN:
+    case A_MAX:
+      if (is_null) break;
+      if (is_float(t)) {
+        double d = get_f64(v, t, r);
+        if (x->kind == A_MIN) {
+          /* MinAction (:905-936): LessNaNSafe when NaN_check */
+          if (!x->valid[g]) { x->fmm[g] = d; x->valid[g] = 1; }
+          int less;
+          if (x->nan_check) {
+            int ln = isnan(d), rn = isnan(x->fmm[g]);
+            less = (ln && rn) ? 0 : ln ? 0 : rn ? 1 : d < x->fmm[g];
+          } else less = d < x->fmm[g];
+          if (less) x->fmm[g] = d;
+        } else {
+          /* MaxAction (:1504-1539): a NaN is sticky when NaN_check */
+          if (x->nan_check && isnan(d)) { x->fmm[g] = d; x->valid[g] = 1; }
+          else {
+            if (!x->valid[g]) { x->fmm[g] = d; x->valid[g] = 1; }
+            if (d > x->fmm[g]) x->fmm[g] = d;
+          }
+        }
+      } else if (t == T_UINT64) {
+        uint64_t u = ((const uint64_t*)v)[r];
+        if (!x->valid[g]) { x->imm[g] = (int64_t)u; x->valid[g] = 1; }
+        if (x->kind == A_MIN ? u < (uint64_t)x->imm[g] : u > (uint64_t)x->imm[g]) x->imm[g] = (int64_t)u;
+      } else {
+        int64_t i = get_i64(v, t, r);
+        if (!x->valid[g]) { x->imm[g] = i; x->valid[g] = 1; }
+        if (x->kind == A_MIN ? i < x->imm[g] : i > x->imm[g]) x->imm[g] = i;
+      }
+      break;
+  }
+}
+
+int gzo_agg_process(gzo_agg* a, const gzo_batch* b) {
+  int64_t n = b->num_rows;
+  if (n == 0) return 0; /* ProcessAndCacheOne returns early on empty input (:808) */
+  int32_t* idx = (int32_t*)malloc(sizeof(int32_t) * n);
+  if (a->key_col < 0) {
+    for (int64_t r = 0; r < n; r++) idx[r] = 0;
+    a->ngroups = 1;
+    a->global_seen = 1;
+  } else {
+    int32_t kt = a->types[a->key_col];
+    const uint8_t* kv = (const uint8_t*)b->bufs[3 * a->key_col];
+    const void* k = b->bufs[3 * a->key_col + 2];
+    for (int64_t r = 0; r < n; r++) {
+      if (kv && !bit_get(kv, r)) {
+        if (a->null_id < 0) a->null_id = (int32_t)a->ngroups++;
+        idx[r] = a->null_id;
+      } else {
+        int64_t key;
+        if (kt == T_DOUBLE) {
+          double d = ((const double*)k)[r];
+          if (isnan(d)) d = NAN;
+          memcpy(&key, &d, 8);
+        } else if (kt == T_FLOAT) {
+          float f = ((const float*)k)[r];
+          int32_t fb;
+          if (isnan(f)) f = NAN;
+          memcpy(&fb, &f, 4);
+          key = fb;
+        } else key = get_i64(k, kt, r);
+        idx[r] = get_or_insert(a, key);
+      }
+    }
+  }
+  grow_groups(a, a->ngroups);
+  /* the reference runs each action's lambda over all rows, action by action
+   * (hash_aggregate_kernel.cc:886-890) */
+  for (int i = 0; i < a->nactions; i++)
+    for (int64_t r = 0; r < n; r++) update(&a->acts[i], b, r, idx[r]);
+  free(idx);
+  return 0;
+}
+
+int64_t gzo_agg_num_groups(gzo_agg* a) { return a->ngroups; }
+
+int gzo_agg_num_results(gzo_agg* a) {
+  int n = 0;
+  for (int i = 0; i < a->nactions; i++) n += a->acts[i].kind == A_SUM_COUNT ? 2 : 1;
+  return n;
+}
+
+static void locate(gzo_agg* a, int col, int* act, int* sub) {
+  int n = 0;
+  for (int i = 0; i < a->nactions; i++) {
+    int w = a->acts[i].kind == A_SUM_COUNT ? 2 : 1;
+    if (col < n + w) { *act = i; *sub = col - n; return; }
+    n += w;
+  }
+  *act = -1; *sub = 0;
+}
+
+int gzo_agg_result_type(gzo_agg* a, int col) {
+  int i, s;
+  locate(a, col, &i, &s);
+  if (i < 0) return -1;
+  action_t* x = &a->acts[i];
+  switch (x->kind) {
+    case A_GROUPBY: case A_MIN: case A_MAX: return x->in_type;
+    case A_COUNT: case A_COUNT_LITERAL: return T_INT64;
+    case A_SUM: return is_float(x->in_type) ? T_DOUBLE : T_INT64; /* FindAccumulatorType (:34-51) */
+    case A_AVG: case A_AVG_BY_COUNT: return T_DOUBLE;
+    case A_SUM_COUNT: return s == 0 ? T_DOUBLE : T_INT64;
+  }
+  return -1;
+}
+
+void gzo_agg_result(gzo_agg* a, int col, void* values, uint8_t* valid) {
+  int i, s;
+  locate(a, col, &i, &s);
+  action_t* x = &a->acts[i];
+  int32_t rt = gzo_agg_result_type(a, col);
+  int w = width_of(rt);
+  for (int64_t g = 0; g < a->ngroups; g++) {
+    uint8_t ok = 1;
+    int64_t iv = 0;
+    double dv = 0;
+    int isf = 0;
+    switch (x->kind) {
+      case A_GROUPBY:
+        ok = x->valid[g] && !x->nullf[g];
+        iv = x->key[g];
+        break;
+      case A_COUNT: case A_COUNT_LITERAL: iv = x->isum[g]; break;
+      case A_SUM:
+        ok = x->valid[g];
+        if (is_float(x->in_type)) { dv = x->dsum[g]; isf = 1; } else iv = x->isum[g];
+        break;
+      case A_AVG:
+        /* grouped: null when no non-null input; the global aggregate is always
+         * valid (AvgAction::Evaluate, :2744) */
+        ok = a->key_col < 0 ? 1 : x->valid[g];
+        dv = x->dsum[g] / (double)x->cnt[g];
+        isf = 1;
+        break;
+      case A_SUM_COUNT:
+        ok = x->valid[g];
+        if (s == 0) { dv = x->dsum[g]; isf = 1; } else iv = x->cnt[g];
+        break;
+      case A_AVG_BY_COUNT:
+        ok = x->cnt[g] != 0;
+        dv = ok ? x->dsum[g] / (double)x->cnt[g] : 0;
+        isf = 1;
+        break;
+      case A_MIN: case A_MAX:
+        ok = x->valid[g];
+        if (is_float(x->in_type)) { dv = x->fmm[g]; isf = 1; } else iv = x->imm[g];
+        break;
+    }
+    valid[g] = ok;
+    uint8_t* dst = (uint8_t*)values + g * w;
+    if (rt == T_DOUBLE) {
+      double d = isf ? dv : (double)iv;
+      memcpy(dst, &d, 8);
+    } else if (rt == T_FLOAT) {
+      if (x->kind == A_GROUPBY) memcpy(dst, &iv, 4);
+      else { float f = (float)dv; memcpy(dst, &f, 4); }
+    } else if (x->kind == A_GROUPBY && is_float(rt)) {
+      memcpy(dst, &iv, (size_t)w);
+    } else {
+      memcpy(dst, &iv, (size_t)w); /* little-endian truncation to the result width */
+    }
+    (void)is_unsigned;
+  }
+}
+
+void gzo_agg_free(gzo_agg* a) {
+  for (int i = 0; i < a->nactions; i++) {
+    action_t* x = &a->acts[i];
+    free(x->dsum); free(x->isum); free(x->cnt); free(x->imm); free(x->fmm);
+    free(x->valid); free(x->nullf); free(x->key);
+  }
+  free(a->acts); free(a->types); free(a->mkeys); free(a->mids); free(a->mused);
+  free(a);
+}

@@ -1,0 +1,92 @@
+"""Writes the golden fixtures transcribed from the reference's own C++ tests.
+
+Each fixture is data: the input literals of a reference test and the expected
+result that test asserts, as the test states it.  Nothing here runs or imports
+the reference; the literals were read from its test sources, cited per fixture.
+
+Run:  python tests/golden/make_fixtures.py   (rewrites tests/golden/*.json)
+"""
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# --------------------------------------------------------------------------- split
+# native-sql-engine/cpp/src/tests/shuffle_split_test.cc:97-208, SplitterTest fixture
+SPLIT_SCHEMA = [
+    ("f_na", "null"), ("f_int8_a", "int8"), ("f_int8_b", "int8"), ("f_int32", "int32"),
+    ("f_uint64", "uint64"), ("f_double", "double"), ("f_bool", "bool"), ("f_string", "utf8"),
+    ("f_nullable_string", "utf8"), ("f_decimal128", "decimal128(10,2)"),
+]
+INPUT_DATA_1 = [  # shuffle_split_test.cc:186-196
+    [None] * 10,
+    [1, 2, 3, None, 4, None, 5, 6, None, 7],
+    [1, -1, None, None, -2, 2, None, None, 3, -3],
+    [1, 2, 3, 4, None, 5, 6, 7, 8, None],
+    [None] * 10,
+    [-0.1234567, None, 0.1234567, None, -0.142857, None, 0.142857, 0.285714, 0.428617, None],
+    [None, True, False, None, True, True, False, True, None, None],
+    ["alice0", "bob1", "alice2", "bob3", "Alice4", "Bob5", "AlicE6", "boB7", "ALICE8", "BOB9"],
+    ["alice", "bob", None, None, "Alice", "Bob", None, "alicE", None, "boB"],
+    ["-1.01", "2.01", "-3.01", None, "0.11", "3.14", "2.27", None, "-3.14", None],
+]
+INPUT_DATA_2 = [  # shuffle_split_test.cc:198-208
+    [None, None],
+    [None, None],
+    [1, -1],
+    [100, None],
+    [1, 1],
+    [0.142857, -0.142857],
+    [True, False],
+    ["bob", "alice" * 40],
+    [None, None],
+    [None, None],
+]
+
+SPLIT_FIXTURES = {
+    "schema": SPLIT_SCHEMA,
+    "batches": {"input_batch_1": INPUT_DATA_1, "input_batch_2": INPUT_DATA_2},
+    "tests": [
+        {   # TestSingleSplitter, shuffle_split_test.cc:210-253
+            "name": "TestSingleSplitter", "short_name": "rr", "num_partitions": 1,
+            "buffer_size": 10,
+            "split": ["input_batch_1", "input_batch_2", "input_batch_1"],
+            # one partition; its record batches equal the three inputs
+            "expected": [[["input_batch_1", list(range(10))], ["input_batch_2", [0, 1]],
+                          ["input_batch_1", list(range(10))]]],
+        },
+        {   # TestRoundRobinSplitter, :255-319
+            "name": "TestRoundRobinSplitter", "short_name": "rr", "num_partitions": 2,
+            "buffer_size": 4,
+            "split": ["input_batch_1", "input_batch_2", "input_batch_1"],
+            "expected": [
+                [["input_batch_1", [0, 2, 4, 6, 8]], ["input_batch_2", [0]],
+                 ["input_batch_1", [0, 2, 4, 6, 8]]],
+                [["input_batch_1", [1, 3, 5, 7, 9]], ["input_batch_2", [1]],
+                 ["input_batch_1", [1, 3, 5, 7, 9]]],
+            ],
+        },
+        {   # TestFallbackRangeSplitter, :390-472 (pid column prepended)
+            "name": "TestFallbackRangeSplitter", "short_name": "range", "num_partitions": 2,
+            "buffer_size": 4,
+            "split": ["input_batch_1", "input_batch_2", "input_batch_1"],
+            "pids": {"input_batch_1": [0, 1, 0, 1, 0, 1, 0, 1, 0, 1], "input_batch_2": [0, 1]},
+            "expected": [
+                [["input_batch_1", [0, 2, 4, 6, 8]], ["input_batch_2", [0]],
+                 ["input_batch_1", [0, 2, 4, 6, 8]]],
+                [["input_batch_1", [1, 3, 5, 7, 9]], ["input_batch_2", [1]],
+                 ["input_batch_1", [1, 3, 5, 7, 9]]],
+            ],
+        },
+    ],
+}
+
+
+def main():
+    with open(os.path.join(HERE, "split_fixtures.json"), "w") as f:
+        json.dump(SPLIT_FIXTURES, f, indent=1)
+    print("wrote split_fixtures.json")
+
+
+if __name__ == "__main__":
+    main()

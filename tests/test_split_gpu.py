@@ -1,0 +1,137 @@
+"""Parity of the HIP Splitter with the oracle (bit-exact), through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle
+from gazelle_plugin_amd import types as T
+from gazelle_plugin_amd import datagen
+from gazelle_plugin_amd.batch import DeviceBatch, HostBatch, HostColumn
+from gazelle_plugin_amd.splitter import SplitOptions, Splitter
+from gazelle_plugin_amd._lib import GazelleError, Invalid
+from golden import fixtures as F
+from splitcheck import assert_layout_equal, random_batches
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu(ctx, batches, types, P, keys, mode="hash", batched=True, tile_rows=0):
+    sp = Splitter.Make(ctx, mode, types, P, keys, SplitOptions(tile_rows=tile_rows))
+    dev = [DeviceBatch.from_host(b) for b in batches]
+    if batched:
+        sp.SplitBatches(dev)
+    else:
+        for d in dev:
+            sp.Split(d)
+    sp.Stop()
+    out = sp.read_partitions()
+    sp.close()
+    return out
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_split_workload_shape_bit_exact(ctx, batched):
+    batches = datagen.split_host(3 * 4096 + 1000)
+    types = datagen.SPLIT_TYPES
+    exp = oracle.split(batches, types, 200, [0])
+    got = run_gpu(ctx, batches, types, 200, [0], batched=batched)
+    assert_layout_equal(got, exp, types)
+
+
+@pytest.mark.parametrize("tile_rows", [1024, 2048, 4096])
+def test_split_all_types_ragged(ctx, tile_rows):
+    types = [T.INT64, T.INT32, T.DOUBLE, T.STRING, T.BOOL, T.INT8, T.UINT16, T.DECIMAL128, T.NA, T.BINARY]
+    batches = random_batches(types, [5000, 0, 1, 4096, 4097, 333], seed=11, null_frac=0.15)
+    exp = oracle.split(batches, types, 37, [0, 3])
+    got = run_gpu(ctx, batches, types, 37, [0, 3], tile_rows=tile_rows)
+    assert_layout_equal(got, exp, types)
+
+
+@pytest.mark.parametrize("P", [1, 2, 255, 256, 257, 2048])
+def test_split_partition_counts(ctx, P):
+    types = [T.INT64, T.STRING, T.DOUBLE]
+    batches = random_batches(types, [4096, 700], seed=P, null_frac=0.05)
+    exp = oracle.split(batches, types, P, [0])
+    got = run_gpu(ctx, batches, types, P, [0])
+    assert_layout_equal(got, exp, types)
+
+
+@pytest.mark.parametrize("key_type", [T.INT32, T.INT16, T.INT8, T.BOOL, T.DATE32, T.DOUBLE, T.FLOAT, T.STRING])
+def test_split_key_types_match_oracle(ctx, key_type):
+    types = [key_type, T.INT64]
+    batches = random_batches(types, [3000, 123], seed=key_type, null_frac=0.1)
+    exp = oracle.split(batches, types, 200, [0])
+    got = run_gpu(ctx, batches, types, 200, [0])
+    assert_layout_equal(got, exp, types)
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_reference_fixtures_on_gpu(ctx, case):
+    fx = F.load("split_fixtures.json")
+    test = fx["tests"][case]
+    schema = fx["schema"]
+    named = {k: F.batch(schema, v) for k, v in fx["batches"].items()}
+    inputs = [named[k] for k in test["split"]]
+    types = [F.type_id(t) for _, t in schema]
+    mode = test["short_name"]
+    if mode == "range":
+        inputs = [HostBatch(b.num_rows, [HostColumn(T.INT32, np.array(test["pids"][k], np.int32))] + b.columns)
+                  for b, k in zip(inputs, test["split"])]
+        types = [T.INT32] + types
+    P = test["num_partitions"]
+    exp = oracle.split(inputs, types, P, [], mode=mode)
+    out_types = types[1:] if mode == "range" else types
+    for batched in (True, False):
+        got = run_gpu(ctx, inputs, types, P, [], mode=mode, batched=batched)
+        assert_layout_equal(got, exp, out_types)
+
+
+def test_range_invalid_pid_raises(ctx):
+    b = HostBatch(3, [HostColumn(T.INT32, np.array([0, 5, 1], np.int32)), HostColumn(T.INT64, np.arange(3))])
+    sp = Splitter.Make(ctx, "range", [T.INT32, T.INT64], 2)
+    with pytest.raises(Invalid):
+        sp.Split(DeviceBatch.from_host(b))
+    sp.close()
+
+
+def test_unknown_partitioning_not_implemented(ctx):
+    with pytest.raises(GazelleError) as ei:
+        Splitter.Make(ctx, "roundrobin2", [T.INT64], 2)
+    assert ei.value.code == 10
+
+
+def test_hbm_budget_out_of_memory():
+    from gazelle_plugin_amd.context import Context
+    from gazelle_plugin_amd._lib import OutOfMemory
+
+    small = Context(0, hbm_budget=1 << 16)
+    batches = datagen.split_host(20000)
+    sp = Splitter.Make(small, "hash", datagen.SPLIT_TYPES, 200, [0])
+    dev = [DeviceBatch.from_host(b) for b in batches]
+    with pytest.raises(OutOfMemory):
+        sp.SplitBatches(dev)
+    sp.close()
+    small.close()
+
+
+def test_device_generated_full_path_properties(ctx):
+    """Size-independent properties on a device-generated input (no oracle run):
+    partitions concatenate to a permutation of the input, counts match."""
+    import torch
+
+    tab = datagen.split_device(64 * 4096)
+    sp = Splitter.Make(ctx, "hash", datagen.SPLIT_TYPES, 200, [0])
+    sp.SplitBatches(tab.batches)
+    sp.Stop()
+    seg = sp.segment(0)
+    key = tab.tensors[0].cpu().numpy()
+    gk = seg["columns"][0]["values"].view(np.int64)
+    assert np.array_equal(np.sort(gk), np.sort(key))
+    # partition of every output key equals its run
+    pids = np.array([oracle.pmod(oracle.hash_long(int(k)), 200) for k in gk[:2000]])
+    rs = seg["row_start"]
+    run = np.searchsorted(rs, np.arange(2000), side="right") - 1
+    assert np.array_equal(pids, run)
+    nulls = int((np.unpackbits(tab.tensors[3].cpu().numpy(), bitorder="little") == 0).sum())
+    assert seg["columns"][2]["null_count"] == nulls
+    sp.close()
+    torch.cuda.synchronize()
