@@ -1,0 +1,1276 @@
+// Hash aggregate for MI355X (gfx950).
+//
+// Replaces HashAggregateKernel + HashAggregateResultIterator<number>
+// (ext/hash_aggregate_kernel.cc:51-928) with its accumulators (ext/actions_impl.cc).
+//
+// Groups live in an open-addressing table in HBM: int64 keys inserted with a
+// 64-bit CAS, linear probing, and per group the first global row that carried
+// the key (atomicMin), which reproduces the reference's first-seen group order
+// (SparseHashMap ids, sparse_hash_map.h:39-47) at finish.  Aggregate states are
+// stored as 8-byte "lanes" (structure of arrays), one per distinct (operation,
+// input column): sum / non-null count / min / max ..., updated with global
+// atomics (FP64 adds with the hardware f64 atomic; integer and ordered-float
+// min/max with 64-bit atomics).  Sums stay FP64 / int64.
+//
+// Low-cardinality instances use an LDS first-level table per workgroup: rows
+// aggregate with LDS atomics, and each workgroup merges its table into the HBM
+// table once per instance.  An instance whose LDS table fills spills rows to
+// the HBM path, so the result never depends on the placement.
+//
+// An aggregate *set* holds many independent instances (one GROUP BY each); one
+// launch sequence processes the tiles of all of them.  The single-aggregate C
+// ABI (gz_agg_*) is a set of one instance.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <string>
+#include <vector>
+
+#include "gz_device.h"
+#include "gz_internal.h"
+
+namespace gz {
+
+constexpr int kAggMaxCols = 32;
+constexpr int kMaxLanes = 24;
+constexpr int kMaxActions = 32;
+constexpr int64_t kEmpty = INT64_MIN;
+constexpr int kHllBits = 11;
+constexpr int kHllRegs = 1 << kHllBits;
+
+enum LaneOp : int {
+  L_CNT = 0, L_NN, L_NN2, L_SUMI, L_SUMF, L_SUMCNT, L_MINI, L_MAXI, L_MINU, L_MAXU,
+  L_MINF, L_MAXF, L_HASNAN, L_FIRSTNN, L_FIRSTNAN
+};
+
+struct Lane {
+  int op, col, col2;
+};
+
+struct AggInst {
+  int64_t* base;          // keys [S], first_row [S], lanes [nl][S] with S = cap + 2
+  uint64_t cap;           // power of two; slot cap = null key, cap + 1 = key equal to kEmpty
+  unsigned int* counter;  // keys inserted
+  int* overflow;
+};
+
+struct AggTile {
+  int32_t batch;
+  int32_t row_begin;
+  int32_t nrows;
+  int32_t inst;
+  int64_t row_base;  // instance-global index of the tile's first row
+};
+
+struct AggParams {
+  const AggTile* tiles;
+  int64_t T;
+  const void* const* bufs;
+  int bstride;
+  int key_col, key_type;
+  int nl;
+  Lane lanes[kMaxLanes];
+  int col_type[kAggMaxCols];
+  const AggInst* inst;
+  int tiles_per_block;
+  int lcap;
+};
+
+__device__ __forceinline__ uint64_t fmix64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+__device__ __forceinline__ const void* abuf(const AggParams& p, int batch, int col, int k) {
+  return p.bufs[(int64_t)batch * p.bstride + col * 3 + k];
+}
+__device__ __forceinline__ bool is_valid(const AggParams& p, int batch, int col, int64_t r) {
+  const uint8_t* vb = (const uint8_t*)abuf(p, batch, col, 0);
+  return !vb || bit_get(vb, r);
+}
+__device__ __forceinline__ int64_t read_i64(const void* v, int t, int64_t r) {
+  switch (t) {
+    case GZ_TYPE_BOOL: return bit_get((const uint8_t*)v, r);
+    case GZ_TYPE_INT8: return ((const int8_t*)v)[r];
+    case GZ_TYPE_UINT8: return ((const uint8_t*)v)[r];
+    case GZ_TYPE_INT16: return ((const int16_t*)v)[r];
+    case GZ_TYPE_UINT16: return ((const uint16_t*)v)[r];
+    case GZ_TYPE_INT32: case GZ_TYPE_DATE32: return ((const int32_t*)v)[r];
+    case GZ_TYPE_UINT32: return ((const uint32_t*)v)[r];
+    default: return ((const int64_t*)v)[r];
+  }
+}
+__device__ __forceinline__ double read_f64(const void* v, int t, int64_t r) {
+  if (t == GZ_TYPE_DOUBLE) return ((const double*)v)[r];
+  if (t == GZ_TYPE_FLOAT) return (double)((const float*)v)[r];
+  if (t == GZ_TYPE_UINT64) return (double)((const uint64_t*)v)[r];
+  return (double)read_i64(v, t, r);
+}
+// Order-preserving map of a non-NaN double to uint64.
+__device__ __forceinline__ uint64_t ordered(double d) {
+  uint64_t b = (uint64_t)__double_as_longlong(d);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ int64_t lane_init(int op) {
+  switch (op) {
+    case L_MINI: return INT64_MAX;
+    case L_MAXI: return INT64_MIN;
+    case L_MINU: case L_MINF: return -1;  // all ones
+    case L_FIRSTNN: case L_FIRSTNAN: return INT64_MAX;
+    default: return 0;
+  }
+}
+
+// Group key of row r (widened to int64); false for a null key.
+__device__ __forceinline__ bool row_key(const AggParams& p, int batch, int64_t r, int64_t* key) {
+  if (p.key_col < 0) {
+    *key = 0;
+    return true;
+  }
+  if (!is_valid(p, batch, p.key_col, r)) return false;
+  *key = read_i64(abuf(p, batch, p.key_col, 2), p.key_type, r);
+  return true;
+}
+
+// Slot of a key in an HBM table (insert if new); -1 when the table is full.
+__device__ __forceinline__ int64_t table_slot(const AggInst& I, bool valid, int64_t key) {
+  if (!valid) return (int64_t)I.cap;
+  if (key == kEmpty) return (int64_t)I.cap + 1;
+  int64_t* keys = I.base;
+  uint64_t mask = I.cap - 1;
+  uint64_t s = fmix64((uint64_t)key) & mask;
+  for (uint64_t i = 0; i < I.cap; i++) {
+    int64_t k = keys[s];
+    if (k == key) return (int64_t)s;
+    if (k == kEmpty) {
+      unsigned long long prev = atomicCAS((unsigned long long*)&keys[s], (unsigned long long)kEmpty,
+                                          (unsigned long long)key);
+      if ((int64_t)prev == kEmpty) {
+        atomicAdd(I.counter, 1u);
+        return (int64_t)s;
+      }
+      if ((int64_t)prev == key) return (int64_t)s;
+    }
+    s = (s + 1) & mask;
+  }
+  atomicOr(I.overflow, 1);
+  return -1;
+}
+
+__device__ __forceinline__ void min_i64(int64_t* p, int64_t v) {
+  if (v < *p) atomicMin((long long*)p, (long long)v);
+}
+__device__ __forceinline__ void max_i64(int64_t* p, int64_t v) {
+  if (v > *p) atomicMax((long long*)p, (long long)v);
+}
+__device__ __forceinline__ void min_u64(int64_t* p, uint64_t v) {
+  if (v < (uint64_t)*p) atomicMin((unsigned long long*)p, (unsigned long long)v);
+}
+__device__ __forceinline__ void max_u64(int64_t* p, uint64_t v) {
+  if (v > (uint64_t)*p) atomicMax((unsigned long long*)p, (unsigned long long)v);
+}
+
+// Apply one row to the lanes of `slot` in a table laid out as base + lane * S.
+// Works on HBM and on LDS (the atomics are generic-address).
+__device__ __forceinline__ void apply_row(const AggParams& p, int64_t* base, int64_t S, int64_t slot,
+                                          int batch, int64_t r, int64_t grow) {
+  int64_t* fr = base + S + slot;
+  if (grow < *fr) atomicMin((long long*)fr, (long long)grow);
+  for (int l = 0; l < p.nl; l++) {
+    const Lane ln = p.lanes[l];
+    int64_t* ptr = base + (2 + l) * S + slot;
+    switch (ln.op) {
+      case L_CNT: atomicAdd((unsigned long long*)ptr, 1ull); break;
+      case L_NN:
+        if (is_valid(p, batch, ln.col, r)) atomicAdd((unsigned long long*)ptr, 1ull);
+        break;
+      case L_NN2:
+        if (is_valid(p, batch, ln.col, r) && is_valid(p, batch, ln.col2, r))
+          atomicAdd((unsigned long long*)ptr, 1ull);
+        break;
+      case L_SUMI:
+        if (is_valid(p, batch, ln.col, r))
+          atomicAdd((unsigned long long*)ptr,
+                    (unsigned long long)read_i64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r));
+        break;
+      case L_SUMF:
+        if (is_valid(p, batch, ln.col, r))
+          unsafeAtomicAdd((double*)ptr, read_f64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r));
+        break;
+      case L_SUMCNT:
+        if (is_valid(p, batch, ln.col, r))
+          atomicAdd((unsigned long long*)ptr,
+                    (unsigned long long)((const int64_t*)abuf(p, batch, ln.col2, 2))[r]);
+        break;
+      case L_MINI:
+      case L_MAXI: {
+        if (!is_valid(p, batch, ln.col, r)) break;
+        int64_t v = read_i64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r);
+        if (ln.op == L_MINI) min_i64(ptr, v); else max_i64(ptr, v);
+        break;
+      }
+      case L_MINU:
+      case L_MAXU: {
+        if (!is_valid(p, batch, ln.col, r)) break;
+        uint64_t v = ((const uint64_t*)abuf(p, batch, ln.col, 2))[r];
+        if (ln.op == L_MINU) min_u64(ptr, v); else max_u64(ptr, v);
+        break;
+      }
+      case L_MINF:
+      case L_MAXF: {
+        if (!is_valid(p, batch, ln.col, r)) break;
+        double d = read_f64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r);
+        if (d != d) break;
+        if (ln.op == L_MINF) min_u64(ptr, ordered(d)); else max_u64(ptr, ordered(d));
+        break;
+      }
+      case L_HASNAN: {
+        if (!is_valid(p, batch, ln.col, r)) break;
+        double d = read_f64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r);
+        if (d != d && *ptr == 0) atomicOr((unsigned long long*)ptr, 1ull);
+        break;
+      }
+      case L_FIRSTNN:
+        if (is_valid(p, batch, ln.col, r) && grow < *ptr) atomicMin((long long*)ptr, (long long)grow);
+        break;
+      case L_FIRSTNAN: {
+        if (!is_valid(p, batch, ln.col, r)) break;
+        double d = read_f64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r);
+        if (d != d && grow < *ptr) atomicMin((long long*)ptr, (long long)grow);
+        break;
+      }
+    }
+  }
+}
+
+// Merge one LDS slot's lanes into an HBM slot.
+__device__ __forceinline__ void merge_slot(const AggParams& p, int64_t* g, int64_t GS, int64_t gslot,
+                                           const int64_t* l, int64_t LS, int64_t lslot) {
+  int64_t fr = l[LS + lslot];
+  int64_t* gfr = g + GS + gslot;
+  if (fr < *gfr) atomicMin((long long*)gfr, (long long)fr);
+  for (int k = 0; k < p.nl; k++) {
+    int64_t v = l[(2 + k) * LS + lslot];
+    int64_t* ptr = g + (2 + k) * GS + gslot;
+    switch (p.lanes[k].op) {
+      case L_CNT: case L_NN: case L_NN2: case L_SUMI: case L_SUMCNT:
+        if (v) atomicAdd((unsigned long long*)ptr, (unsigned long long)v);
+        break;
+      case L_SUMF: {
+        double d = __longlong_as_double(v);
+        if (d != 0.0) unsafeAtomicAdd((double*)ptr, d);
+        break;
+      }
+      case L_MINI: if (v != INT64_MAX) min_i64(ptr, v); break;
+      case L_MAXI: if (v != INT64_MIN) max_i64(ptr, v); break;
+      case L_MINU: case L_MINF: if (v != -1) min_u64(ptr, (uint64_t)v); break;
+      case L_MAXU: case L_MAXF: if (v != 0) max_u64(ptr, (uint64_t)v); break;
+      case L_HASNAN: if (v && *ptr == 0) atomicOr((unsigned long long*)ptr, 1ull); break;
+      case L_FIRSTNN: case L_FIRSTNAN:
+        if (v < *ptr) atomicMin((long long*)ptr, (long long)v);
+        break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ void fill_u32_kernel_agg(uint32_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// Fill of every instance's table: keys empty, first rows max, lanes initial.
+__global__ void agg_init_kernel(const AggInst* inst, int ninst, int nl, const int* lane_ops) {
+  int i = blockIdx.y;
+  if (i >= ninst) return;
+  const AggInst I = inst[i];
+  int64_t S = (int64_t)I.cap + 2;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < S;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    I.base[s] = kEmpty;
+    I.base[S + s] = INT64_MAX;
+    for (int l = 0; l < nl; l++) I.base[(2 + l) * S + s] = lane_init(lane_ops[l]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *I.counter = 0;
+    *I.overflow = 0;
+  }
+}
+
+// HBM path: one workgroup per tile.
+__global__ __launch_bounds__(256) void agg_update_kernel(AggParams p) {
+  int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
+  if (t < 0) return;
+  const AggTile td = p.tiles[t];
+  const AggInst I = p.inst[td.inst];
+  const int64_t S = (int64_t)I.cap + 2;
+  for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
+    int64_t r = (int64_t)td.row_begin + lr;
+    int64_t key;
+    bool valid = row_key(p, td.batch, r, &key);
+    int64_t slot = table_slot(I, valid, key);
+    if (slot < 0) continue;
+    apply_row(p, I.base, S, slot, td.batch, r, td.row_base + lr);
+  }
+}
+
+// LDS path: persistent workgroups over static chunks of consecutive tiles; an
+// LDS table of lcap slots (+2 special) per workgroup, merged into HBM whenever
+// the instance changes and at the end.
+__device__ void lds_merge(const AggParams& p, const AggInst& I, int64_t* l, int64_t LS) {
+  const int64_t GS = (int64_t)I.cap + 2;
+  for (int64_t s = threadIdx.x; s < LS; s += blockDim.x) {
+    bool special = s >= p.lcap;
+    int64_t key = l[s];
+    if (!special && key == kEmpty) continue;
+    if (special && l[LS + s] == INT64_MAX) continue;
+    int64_t g;
+    if (special) g = (int64_t)I.cap + (s - p.lcap);
+    else g = table_slot(I, true, key);
+    if (g < 0) continue;
+    merge_slot(p, I.base, GS, g, l, LS, s);
+  }
+}
+
+__device__ void lds_reset(const AggParams& p, int64_t* l, int64_t LS) {
+  for (int64_t s = threadIdx.x; s < LS; s += blockDim.x) {
+    l[s] = kEmpty;
+    l[LS + s] = INT64_MAX;
+    for (int k = 0; k < p.nl; k++) l[(2 + k) * LS + s] = lane_init(p.lanes[k].op);
+  }
+}
+
+__global__ __launch_bounds__(256) void agg_update_lds_kernel(AggParams p) {
+  extern __shared__ int64_t s_tab[];
+  const int64_t LS = p.lcap + 2;
+  const uint64_t lmask = (uint64_t)p.lcap - 1;
+  int64_t t0 = (int64_t)blockIdx.x * p.tiles_per_block;
+  int64_t t1 = min(p.T, t0 + p.tiles_per_block);
+  if (t0 >= t1) return;
+  int cur = -1;
+  for (int64_t t = t0; t < t1; t++) {
+    const AggTile td = p.tiles[t];
+    if (td.inst != cur) {
+      if (cur >= 0) {
+        __syncthreads();
+        lds_merge(p, p.inst[cur], s_tab, LS);
+      }
+      __syncthreads();
+      lds_reset(p, s_tab, LS);
+      __syncthreads();
+      cur = td.inst;
+    }
+    const AggInst I = p.inst[td.inst];
+    for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
+      int64_t r = (int64_t)td.row_begin + lr;
+      int64_t grow = td.row_base + lr;
+      int64_t key;
+      bool valid = row_key(p, td.batch, r, &key);
+      int64_t ls = -1;
+      if (!valid) ls = p.lcap;
+      else if (key == kEmpty) ls = p.lcap + 1;
+      else {
+        uint64_t s = fmix64((uint64_t)key) & lmask;
+        for (int i = 0; i < 32; i++) {  // short probe: a full LDS table spills to HBM
+          int64_t k = s_tab[s];
+          if (k == key) { ls = (int64_t)s; break; }
+          if (k == kEmpty) {
+            unsigned long long prev = atomicCAS((unsigned long long*)&s_tab[s],
+                                                (unsigned long long)kEmpty, (unsigned long long)key);
+            if ((int64_t)prev == kEmpty || (int64_t)prev == key) { ls = (int64_t)s; break; }
+          }
+          s = (s + 1) & lmask;
+        }
+      }
+      if (ls >= 0) {
+        apply_row(p, s_tab, LS, ls, td.batch, r, grow);
+      } else {
+        int64_t g = table_slot(I, valid, key);
+        if (g >= 0) apply_row(p, I.base, (int64_t)I.cap + 2, g, td.batch, r, grow);
+      }
+    }
+  }
+  __syncthreads();
+  lds_merge(p, p.inst[cur], s_tab, LS);
+}
+
+// HyperLogLog registers per instance (2^11 registers), persistent static chunks
+// with LDS registers merged by atomicMax.
+__global__ __launch_bounds__(256) void agg_hll_kernel(AggParams p, unsigned int* regs) {
+  __shared__ unsigned int s_reg[kHllRegs];
+  int64_t t0 = (int64_t)blockIdx.x * p.tiles_per_block;
+  int64_t t1 = min(p.T, t0 + p.tiles_per_block);
+  if (t0 >= t1) return;
+  int cur = -1;
+  for (int64_t t = t0; t < t1; t++) {
+    const AggTile td = p.tiles[t];
+    if (td.inst != cur) {
+      __syncthreads();
+      if (cur >= 0)
+        for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x)
+          if (s_reg[i]) atomicMax(&regs[(int64_t)cur * kHllRegs + i], s_reg[i]);
+      __syncthreads();
+      for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) s_reg[i] = 0;
+      __syncthreads();
+      cur = td.inst;
+    }
+    for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
+      int64_t key;
+      if (!row_key(p, td.batch, (int64_t)td.row_begin + lr, &key)) continue;
+      uint64_t h = fmix64((uint64_t)key ^ 0x9e3779b97f4a7c15ull);
+      unsigned idx = (unsigned)(h >> (64 - kHllBits));
+      uint64_t w = (h << kHllBits) | (1ull << (kHllBits - 1));
+      unsigned rank = (unsigned)__clzll((long long)w) + 1;
+      if (rank > s_reg[idx]) atomicMax(&s_reg[idx], rank);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x)
+    if (s_reg[i]) atomicMax(&regs[(int64_t)cur * kHllRegs + i], s_reg[i]);
+}
+
+// One block per instance: the HyperLogLog estimate (with the linear-counting
+// correction for small cardinalities).
+__global__ __launch_bounds__(256) void agg_hll_estimate_kernel(const unsigned int* regs, double* est) {
+  __shared__ double s_sum[256];
+  __shared__ int s_zero[256];
+  const unsigned int* r = regs + (int64_t)blockIdx.x * kHllRegs;
+  double sum = 0;
+  int zeros = 0;
+  for (int i = threadIdx.x; i < kHllRegs; i += 256) {
+    sum += ldexp(1.0, -(int)r[i]);
+    zeros += r[i] == 0;
+  }
+  s_sum[threadIdx.x] = sum;
+  s_zero[threadIdx.x] = zeros;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
+      s_zero[threadIdx.x] += s_zero[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double m = kHllRegs;
+    double e = (0.7213 / (1.0 + 1.079 / m)) * m * m / s_sum[0];
+    if (e <= 2.5 * m && s_zero[0] > 0) e = m * log(m / (double)s_zero[0]);
+    est[blockIdx.x] = e;
+  }
+}
+
+// Rehash of an instance into a larger table (keys are unique: plain stores).
+__global__ void agg_rehash_kernel(AggInst from, AggInst to, int nl) {
+  const int64_t FS = (int64_t)from.cap + 2, TS = (int64_t)to.cap + 2;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < FS;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    int64_t key = from.base[s];
+    int64_t d;
+    if (s >= (int64_t)from.cap) {
+      if (from.base[FS + s] == INT64_MAX) continue;
+      d = (int64_t)to.cap + (s - (int64_t)from.cap);
+    } else {
+      if (key == kEmpty) continue;
+      d = table_slot(to, true, key);
+      if (d < 0) continue;
+    }
+    to.base[TS + d] = from.base[FS + s];
+    for (int l = 0; l < nl; l++) to.base[(2 + l) * TS + d] = from.base[(2 + l) * FS + s];
+  }
+}
+
+// Occupied slots of all instances -> (instance << 44 | first_row, slot) pairs.
+__global__ void agg_collect_kernel(const AggInst* inst, int ninst, unsigned long long* keys,
+                                   unsigned long long* vals, unsigned long long* cursor,
+                                   unsigned long long* counts) {
+  int i = blockIdx.y;
+  if (i >= ninst) return;
+  const AggInst I = inst[i];
+  int64_t S = (int64_t)I.cap + 2;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < S;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    int64_t fr = I.base[S + s];
+    if (fr == INT64_MAX) continue;  // a slot is occupied iff a row reached it
+    unsigned long long pos = atomicAdd(cursor, 1ull);
+    keys[pos] = ((unsigned long long)i << 44) | (unsigned long long)fr;
+    vals[pos] = (unsigned long long)s;
+    atomicAdd(&counts[i], 1ull);
+  }
+}
+
+struct ResultSpec {
+  int kind;      // action kind
+  int sub;       // 0 / 1 (sum_count yields two columns)
+  int rtype;     // result type
+  int lane_a, lane_b, lane_c, lane_d;  // lanes used
+  int nan_check;
+  int global;    // no group-by key
+};
+
+constexpr int kMaxResults = 40;
+
+struct FinishParams {
+  const AggInst* inst;
+  const unsigned long long* skeys;  // sorted keys
+  const unsigned long long* svals;  // slots
+  int64_t n;
+  int key_type;
+  int nres;
+  ResultSpec res[kMaxResults];
+  void* out_values[kMaxResults];
+  uint8_t* out_valid[kMaxResults];
+};
+
+__device__ __forceinline__ double decode_ordered(uint64_t u) {
+  uint64_t b = (u & 0x8000000000000000ull) ? (u & 0x7fffffffffffffffull) : ~u;
+  return __longlong_as_double((long long)b);
+}
+
+__device__ __forceinline__ void store_typed(void* out, int64_t i, int t, int64_t iv, double dv, bool isf) {
+  switch (t) {
+    case GZ_TYPE_DOUBLE: ((double*)out)[i] = isf ? dv : (double)iv; break;
+    case GZ_TYPE_FLOAT: ((float*)out)[i] = (float)(isf ? dv : (double)iv); break;
+    case GZ_TYPE_INT8: case GZ_TYPE_UINT8: case GZ_TYPE_BOOL: ((int8_t*)out)[i] = (int8_t)iv; break;
+    case GZ_TYPE_INT16: case GZ_TYPE_UINT16: ((int16_t*)out)[i] = (int16_t)iv; break;
+    case GZ_TYPE_INT32: case GZ_TYPE_UINT32: case GZ_TYPE_DATE32: ((int32_t*)out)[i] = (int32_t)iv; break;
+    default: ((int64_t*)out)[i] = iv; break;
+  }
+}
+
+__global__ void agg_finish_kernel(FinishParams f) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int inst = (int)(f.skeys[i] >> 44);
+    int64_t slot = (int64_t)f.svals[i];
+    const AggInst I = f.inst[inst];
+    const int64_t S = (int64_t)I.cap + 2;
+    const int64_t* b = I.base;
+    auto lane = [&](int l) -> int64_t { return b[(2 + l) * S + slot]; };
+    for (int c = 0; c < f.nres; c++) {
+      const ResultSpec& rs = f.res[c];
+      bool ok = true;
+      int64_t iv = 0;
+      double dv = 0;
+      bool isf = false;
+      switch (rs.kind) {
+        case GZ_ACTION_GROUPBY:
+          ok = slot != (int64_t)I.cap;  // the null-key group
+          iv = b[slot];
+          break;
+        case GZ_ACTION_COUNT:
+        case GZ_ACTION_COUNT_LITERAL: iv = lane(rs.lane_a); break;
+        case GZ_ACTION_SUM:
+          ok = lane(rs.lane_b) != 0;
+          if (rs.rtype == GZ_TYPE_DOUBLE) { dv = __longlong_as_double(lane(rs.lane_a)); isf = true; }
+          else iv = lane(rs.lane_a);
+          break;
+        case GZ_ACTION_AVG: {
+          int64_t n = lane(rs.lane_b);
+          ok = rs.global ? true : n != 0;
+          dv = __longlong_as_double(lane(rs.lane_a)) / (double)n;
+          isf = true;
+          break;
+        }
+        case GZ_ACTION_SUM_COUNT:
+          ok = lane(rs.lane_b) != 0;
+          if (rs.sub == 0) { dv = __longlong_as_double(lane(rs.lane_a)); isf = true; }
+          else iv = lane(rs.lane_b);
+          break;
+        case GZ_ACTION_AVG_BY_COUNT: {
+          int64_t n = lane(rs.lane_b);
+          ok = n != 0;
+          dv = ok ? __longlong_as_double(lane(rs.lane_a)) / (double)n : 0.0;
+          isf = true;
+          break;
+        }
+        case GZ_ACTION_MIN:
+        case GZ_ACTION_MAX: {
+          ok = lane(rs.lane_b) != 0;
+          if (rs.rtype == GZ_TYPE_DOUBLE || rs.rtype == GZ_TYPE_FLOAT) {
+            isf = true;
+            uint64_t u = (uint64_t)lane(rs.lane_a);
+            bool has_value = rs.kind == GZ_ACTION_MIN ? u != ~0ull : u != 0ull;
+            bool nan;
+            if (rs.nan_check) {
+              bool has_nan = lane(rs.lane_c) != 0;
+              nan = rs.kind == GZ_ACTION_MIN ? (!has_value && has_nan) : has_nan;
+            } else {
+              // plain comparisons from the first value: NaN iff the first
+              // non-null value was NaN
+              nan = lane(rs.lane_d) != INT64_MAX && lane(rs.lane_d) == lane(rs.lane_c);
+            }
+            dv = nan ? __longlong_as_double(0x7ff8000000000000ll) : (has_value ? decode_ordered(u) : 0.0);
+          } else {
+            iv = lane(rs.lane_a);
+          }
+          break;
+        }
+      }
+      f.out_valid[c][i] = ok ? 1 : 0;
+      store_typed(f.out_values[c], i, rs.rtype, iv, dv, isf);
+    }
+  }
+}
+
+}  // namespace gz
+
+// =============================================================== host side
+using namespace gz;
+
+struct AggPlan {
+  int ncols = 0;
+  std::vector<int> types;
+  int key_col = -1, key_type = GZ_TYPE_INT64;
+  std::vector<gz_agg_action> actions;
+  std::vector<Lane> lanes;
+  std::vector<ResultSpec> results;
+};
+
+struct gz_agg {
+  gz_context* ctx = nullptr;
+  AggPlan plan;
+  gz_agg_options opts;
+  int ninst = 1;
+  std::vector<AggInst> inst;       // host copies of the device descriptors
+  std::vector<DBuf> tables, counters;
+  std::vector<int64_t> rows_seen;  // per instance
+  std::vector<int64_t> g_bound;    // upper bound of groups per instance
+  std::vector<int> placement;      // 1 = LDS first level, 2 = HBM only
+  DBuf d_inst, d_lane_ops, d_tiles, d_bufs, d_regs, d_est;
+  // finish
+  bool finished = false;
+  std::vector<int64_t> ngroups, group_off;
+  DBuf skeys, svals, tkeys, tvals, cursor, counts, sort_tmp;
+  std::vector<DBuf> res_values, res_valid;
+  int64_t total_groups = 0;
+  bool inst_dirty = true;
+};
+
+static int add_lane(AggPlan& P, int op, int col, int col2) {
+  for (size_t i = 0; i < P.lanes.size(); i++)
+    if (P.lanes[i].op == op && P.lanes[i].col == col && P.lanes[i].col2 == col2) return (int)i;
+  if ((int)P.lanes.size() >= kMaxLanes) return -1;
+  P.lanes.push_back({op, col, col2});
+  return (int)P.lanes.size() - 1;
+}
+
+static bool is_float_t(int t) { return t == GZ_TYPE_FLOAT || t == GZ_TYPE_DOUBLE; }
+static bool is_int_t(int t) {
+  return t == GZ_TYPE_BOOL || t == GZ_TYPE_INT8 || t == GZ_TYPE_UINT8 || t == GZ_TYPE_INT16 ||
+         t == GZ_TYPE_UINT16 || t == GZ_TYPE_INT32 || t == GZ_TYPE_UINT32 || t == GZ_TYPE_INT64 ||
+         t == GZ_TYPE_UINT64 || t == GZ_TYPE_DATE32 || t == GZ_TYPE_DATE64 || t == GZ_TYPE_TIMESTAMP;
+}
+
+static int build_plan(AggPlan& P, int ncols, const int32_t* types, int nact, const gz_agg_action* acts) {
+  P.ncols = ncols;
+  P.types.assign(types, types + ncols);
+  if (nact <= 0 || nact > kMaxActions) return fail(GZ_INVALID, "bad action count");
+  if (ncols > kAggMaxCols) return fail(GZ_NOT_IMPLEMENTED, "more than 32 input columns");
+  P.actions.assign(acts, acts + nact);
+  bool global = true;
+  for (auto& a : P.actions)
+    if (a.kind == GZ_ACTION_GROUPBY) global = false;
+  for (auto& a : P.actions) {
+    auto col_ok = [&](int c) { return c >= 0 && c < ncols; };
+    ResultSpec r{};
+    r.kind = a.kind;
+    r.nan_check = a.nan_check;
+    r.global = global;
+    r.lane_a = r.lane_b = r.lane_c = r.lane_d = 0;
+    int t = col_ok(a.input0) ? types[a.input0] : -1;
+    switch (a.kind) {
+      case GZ_ACTION_GROUPBY:
+        if (P.key_col >= 0) return fail(GZ_NOT_IMPLEMENTED, "multiple group-by keys");
+        if (!col_ok(a.input0)) return fail(GZ_INVALID, "groupby input out of range");
+        if (!is_int_t(t))
+          return fail(GZ_NOT_IMPLEMENTED, "group-by key type " + std::to_string(t) +
+                                              " (the HIP aggregate groups integer keys)");
+        P.key_col = a.input0;
+        P.key_type = t;
+        r.rtype = t;
+        P.results.push_back(r);
+        break;
+      case GZ_ACTION_COUNT:
+        if (!col_ok(a.input0)) return fail(GZ_INVALID, "count input out of range");
+        r.lane_a = col_ok(a.input1) ? add_lane(P, L_NN2, a.input0, a.input1) : add_lane(P, L_NN, a.input0, -1);
+        r.rtype = GZ_TYPE_INT64;
+        P.results.push_back(r);
+        break;
+      case GZ_ACTION_COUNT_LITERAL:
+        r.lane_a = add_lane(P, L_CNT, -1, -1);
+        r.rtype = GZ_TYPE_INT64;
+        P.results.push_back(r);
+        break;
+      case GZ_ACTION_SUM:
+        if (!col_ok(a.input0) || !(is_int_t(t) || is_float_t(t))) return fail(GZ_INVALID, "sum input");
+        r.lane_a = is_float_t(t) ? add_lane(P, L_SUMF, a.input0, -1) : add_lane(P, L_SUMI, a.input0, -1);
+        r.lane_b = add_lane(P, L_NN, a.input0, -1);
+        r.rtype = is_float_t(t) ? GZ_TYPE_DOUBLE : GZ_TYPE_INT64;
+        P.results.push_back(r);
+        break;
+      case GZ_ACTION_AVG:
+      case GZ_ACTION_SUM_COUNT:
+        if (!col_ok(a.input0) || !(is_int_t(t) || is_float_t(t))) return fail(GZ_INVALID, "avg input");
+        r.lane_a = add_lane(P, L_SUMF, a.input0, -1);
+        r.lane_b = add_lane(P, L_NN, a.input0, -1);
+        r.rtype = GZ_TYPE_DOUBLE;
+        P.results.push_back(r);
+        if (a.kind == GZ_ACTION_SUM_COUNT) {
+          r.sub = 1;
+          r.rtype = GZ_TYPE_INT64;
+          P.results.push_back(r);
+        }
+        break;
+      case GZ_ACTION_AVG_BY_COUNT:
+        if (!col_ok(a.input0) || !col_ok(a.input1) || types[a.input1] != GZ_TYPE_INT64)
+          return fail(GZ_INVALID, "avgByCount needs (sum, int64 count) inputs");
+        r.lane_a = add_lane(P, L_SUMF, a.input0, -1);
+        r.lane_b = add_lane(P, L_SUMCNT, a.input0, a.input1);
+        r.rtype = GZ_TYPE_DOUBLE;
+        P.results.push_back(r);
+        break;
+      case GZ_ACTION_MIN:
+      case GZ_ACTION_MAX: {
+        bool mn = a.kind == GZ_ACTION_MIN;
+        if (!col_ok(a.input0)) return fail(GZ_INVALID, "min/max input out of range");
+        r.rtype = t;
+        r.lane_b = add_lane(P, L_NN, a.input0, -1);
+        if (is_float_t(t)) {
+          r.lane_a = add_lane(P, mn ? L_MINF : L_MAXF, a.input0, -1);
+          if (a.nan_check) r.lane_c = add_lane(P, L_HASNAN, a.input0, -1);
+          else {
+            r.lane_c = add_lane(P, L_FIRSTNAN, a.input0, -1);
+            r.lane_d = add_lane(P, L_FIRSTNN, a.input0, -1);
+          }
+        } else if (t == GZ_TYPE_UINT64) {
+          r.lane_a = add_lane(P, mn ? L_MINU : L_MAXU, a.input0, -1);
+        } else if (is_int_t(t)) {
+          r.lane_a = add_lane(P, mn ? L_MINI : L_MAXI, a.input0, -1);
+        } else {
+          return fail(GZ_NOT_IMPLEMENTED, "min/max input type");
+        }
+        P.results.push_back(r);
+        break;
+      }
+      default:
+        return fail(GZ_NOT_IMPLEMENTED, "action kind " + std::to_string(a.kind) + " is not implemented");
+    }
+    if (r.lane_a < 0 || r.lane_b < 0 || r.lane_c < 0 || r.lane_d < 0)
+      return fail(GZ_NOT_IMPLEMENTED, "too many aggregate states");
+  }
+  if ((int)P.results.size() > kMaxResults) return fail(GZ_NOT_IMPLEMENTED, "too many result columns");
+  return GZ_OK;
+}
+
+static uint64_t pow2_at_least(uint64_t x) {
+  uint64_t c = 16;
+  while (c < x) c <<= 1;
+  return c;
+}
+
+static AggParams base_params(gz_agg* h) {
+  AggParams p;
+  memset(&p, 0, sizeof(p));
+  p.key_col = h->plan.key_col;
+  p.key_type = h->plan.key_type;
+  p.nl = (int)h->plan.lanes.size();
+  for (int l = 0; l < p.nl; l++) p.lanes[l] = h->plan.lanes[l];
+  for (int c = 0; c < h->plan.ncols; c++) p.col_type[c] = h->plan.types[c];
+  p.lcap = 1024;
+  return p;
+}
+
+static int lds_bytes(const gz_agg* h, int lcap) {
+  return (int)((lcap + 2) * (2 + h->plan.lanes.size()) * 8);
+}
+
+// (Re)allocate an instance's table with capacity cap (copying existing groups).
+static int inst_alloc(gz_agg* h, int i, uint64_t cap) {
+  gz_context* ctx = h->ctx;
+  const int nl = (int)h->plan.lanes.size();
+  DBuf nb;
+  GZ_TRY(ctx_alloc(ctx, sizeof(int64_t) * (cap + 2) * (2 + nl), &nb));
+  DBuf nc;
+  GZ_TRY(ctx_alloc(ctx, 16, &nc));
+  AggInst ni;
+  ni.base = (int64_t*)nb.p;
+  ni.cap = cap;
+  ni.counter = (unsigned int*)nc.p;
+  ni.overflow = (int*)((char*)nc.p + 8);
+  // initialize this one instance
+  DBuf one;
+  GZ_TRY(upload(ctx, &one, &ni, sizeof(AggInst)));
+  std::vector<int> ops(std::max(nl, 1));
+  for (int l = 0; l < nl; l++) ops[l] = h->plan.lanes[l].op;
+  GZ_TRY(upload(ctx, &h->d_lane_ops, ops.data(), sizeof(int) * ops.size()));
+  unsigned gx = (unsigned)std::min<uint64_t>((cap + 2 + 255) / 256, 4096);
+  hipLaunchKernelGGL(agg_init_kernel, dim3(gx, 1), dim3(256), 0, ctx->stream, (const AggInst*)one.p, 1, nl,
+                     (const int*)h->d_lane_ops.p);
+  if (h->tables[i].p) {
+    AggInst old = h->inst[i];
+    unsigned g2 = (unsigned)std::min<uint64_t>((old.cap + 2 + 255) / 256, 4096);
+    hipLaunchKernelGGL(agg_rehash_kernel, dim3(g2), dim3(256), 0, ctx->stream, old, ni, nl);
+    // the inserted-key counter of the new table counts the moved keys; the old
+    // buffers go back to the pool in stream order
+    ctx_free(ctx, &h->tables[i]);
+    ctx_free(ctx, &h->counters[i]);
+  }
+  GZ_HIP(hipGetLastError());
+  ctx_free(ctx, &one);
+  h->tables[i] = nb;
+  h->counters[i] = nc;
+  h->inst[i] = ni;
+  h->inst_dirty = true;
+  return GZ_OK;
+}
+
+static int read_counts(gz_agg* h, std::vector<int64_t>& groups, int* overflow) {
+  *overflow = 0;
+  groups.assign(h->ninst, 0);
+  std::vector<unsigned int> buf(h->ninst * 4);
+  for (int i = 0; i < h->ninst; i++)
+    GZ_HIP(hipMemcpyAsync(&buf[i * 4], h->counters[i].p, 8, hipMemcpyDeviceToHost, h->ctx->stream));
+  GZ_HIP(hipStreamSynchronize(h->ctx->stream));
+  for (int i = 0; i < h->ninst; i++) {
+    groups[i] = buf[i * 4] + 2;  // + the two special slots
+    if (buf[i * 4 + 2]) *overflow = 1;
+  }
+  return GZ_OK;
+}
+
+static int agg_set_make(gz_context* ctx, int32_t ncols, const int32_t* types, int32_t nact,
+                        const gz_agg_action* acts, const gz_agg_options* opts, int32_t ninst,
+                        gz_agg** out) {
+  if (!ctx || !out || (ncols > 0 && !types) || !acts) return fail(GZ_INVALID, "gz_agg_make: null argument");
+  if (ninst <= 0 || ninst > 65535) return fail(GZ_INVALID, "instance count must be in [1, 65535]");
+  auto* h = new gz_agg();
+  h->ctx = ctx;
+  int rc = build_plan(h->plan, ncols, types, nact, acts);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  gz_agg_options_defaults(&h->opts);
+  if (opts) h->opts = *opts;
+  if (h->opts.load_factor <= 0 || h->opts.load_factor > 0.9) h->opts.load_factor = 0.5;
+  h->ninst = ninst;
+  h->inst.resize(ninst);
+  h->tables.resize(ninst);
+  h->counters.resize(ninst);
+  h->rows_seen.assign(ninst, 0);
+  h->g_bound.assign(ninst, 2);
+  h->placement.assign(ninst, h->opts.placement == 1 ? 1 : 2);
+  *out = h;
+  return GZ_OK;
+}
+
+// Tile table + buffer table of a call.
+static int agg_stage(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of,
+                     std::vector<AggTile>& tiles, std::vector<int64_t>& rows_per_inst, int tile_rows) {
+  const int nc = h->plan.ncols;
+  std::vector<const void*> bufs((size_t)nb * nc * 3, nullptr);
+  rows_per_inst.assign(h->ninst, 0);
+  std::vector<int64_t> base = h->rows_seen;
+  for (int bi = 0; bi < nb; bi++) {
+    const gz_batch& bt = batches[bi];
+    int ii = inst_of ? inst_of[bi] : 0;
+    if (ii < 0 || ii >= h->ninst) return fail(GZ_INDEX_ERROR, "instance out of range");
+    if (bt.num_rows < 0 || bt.num_rows > INT32_MAX) return fail(GZ_INVALID, "bad num_rows");
+    int need = 0;
+    for (int c = 0; c < nc; c++) need += type_num_bufs(h->plan.types[c]);
+    if (bt.num_bufs < need) return fail(GZ_INVALID, "insufficient number of in_buf_addrs");
+    int k = 0;
+    for (int c = 0; c < nc; c++) {
+      int t = h->plan.types[c];
+      const void** slot = &bufs[((size_t)bi * nc + c) * 3];
+      slot[0] = (bt.buf_addrs[k] != 0 && bt.buf_sizes[k] > 0) ? (const void*)bt.buf_addrs[k] : nullptr;
+      k++;
+      if (type_is_binary(t)) slot[1] = (const void*)bt.buf_addrs[k++];
+      slot[2] = (const void*)bt.buf_addrs[k];
+      if (t != GZ_TYPE_NA && bt.num_rows > 0 && !slot[2]) return fail(GZ_INVALID, "missing values buffer");
+      int64_t need_b = t == GZ_TYPE_BOOL ? (bt.num_rows + 7) / 8 : bt.num_rows * type_width(t);
+      if (t != GZ_TYPE_NA && !type_is_binary(t) && bt.buf_sizes[k] < need_b)
+        return fail(GZ_INVALID, "values buffer too small");
+      k++;
+    }
+    for (int64_t r0 = 0; r0 < bt.num_rows; r0 += tile_rows) {
+      AggTile td;
+      td.batch = bi;
+      td.row_begin = (int32_t)r0;
+      td.nrows = (int32_t)std::min<int64_t>(tile_rows, bt.num_rows - r0);
+      td.inst = ii;
+      td.row_base = base[ii] + r0;
+      tiles.push_back(td);
+    }
+    base[ii] += bt.num_rows;
+    rows_per_inst[ii] += bt.num_rows;
+  }
+  GZ_TRY(upload(h->ctx, &h->d_bufs, bufs.data(), sizeof(void*) * std::max<size_t>(bufs.size(), 1)));
+  return GZ_OK;
+}
+
+static int agg_process(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of) {
+  gz_context* ctx = h->ctx;
+  if (h->finished) return fail(GZ_INVALID, "aggregate already finished");
+  GZ_HIP(hipSetDevice(ctx->device));
+  const int tile_rows = h->opts.tile_rows > 0 ? h->opts.tile_rows : 4096;
+  std::vector<AggTile> tiles;
+  std::vector<int64_t> rows;
+  GZ_TRY(agg_stage(h, nb, batches, inst_of, tiles, rows, tile_rows));
+  if (tiles.empty()) return GZ_OK;
+  GZ_TRY(upload(ctx, &h->d_tiles, tiles.data(), sizeof(AggTile) * tiles.size()));
+  AggParams p = base_params(h);
+  p.tiles = (const AggTile*)h->d_tiles.p;
+  p.T = (int64_t)tiles.size();
+  p.bufs = (const void* const*)h->d_bufs.p;
+  p.bstride = h->plan.ncols * 3;
+  const double lf = h->opts.load_factor;
+  // ---- size tables: first use of an instance in a multi-row call estimates its
+  // cardinality with HyperLogLog; later calls grow from an upper bound.
+  bool need_est = false;
+  for (int i = 0; i < h->ninst; i++)
+    if (rows[i] > 0 && !h->tables[i].p && rows[i] > 4096 && h->opts.expected_groups <= 0) need_est = true;
+  std::vector<double> est(h->ninst, 0.0);
+  if (need_est) {
+    GZ_TRY(ctx_ensure(ctx, &h->d_regs, sizeof(unsigned) * kHllRegs * h->ninst));
+    GZ_TRY(ctx_ensure(ctx, &h->d_est, sizeof(double) * h->ninst));
+    int64_t n32 = (int64_t)kHllRegs * h->ninst;
+    hipLaunchKernelGGL(fill_u32_kernel_agg, dim3((unsigned)std::min<int64_t>((n32 + 255) / 256, 4096)), dim3(256), 0,
+                       ctx->stream, (uint32_t*)h->d_regs.p, n32);
+    int64_t blocks = std::min<int64_t>(p.T, (int64_t)ctx->num_cus * 4);
+    p.tiles_per_block = (int)((p.T + blocks - 1) / blocks);
+    blocks = (p.T + p.tiles_per_block - 1) / p.tiles_per_block;
+    {
+      KernelTimer kt(ctx, "agg_hll");
+      hipLaunchKernelGGL(agg_hll_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, p,
+                         (unsigned int*)h->d_regs.p);
+      hipLaunchKernelGGL(agg_hll_estimate_kernel, dim3((unsigned)h->ninst), dim3(256), 0, ctx->stream,
+                         (const unsigned int*)h->d_regs.p, (double*)h->d_est.p);
+    }
+    GZ_HIP(hipMemcpyAsync(est.data(), h->d_est.p, sizeof(double) * h->ninst, hipMemcpyDeviceToHost, ctx->stream));
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  bool sync_needed = false;
+  for (int i = 0; i < h->ninst; i++)
+    if (rows[i] > 0 && h->tables[i].p && (double)(h->g_bound[i] + rows[i]) > lf * h->inst[i].cap)
+      sync_needed = true;
+  std::vector<int64_t> actual;
+  if (sync_needed) {
+    int ovf = 0;
+    GZ_TRY(read_counts(h, actual, &ovf));
+    if (ovf) return fail(GZ_CAPACITY_ERROR, "aggregate hash table overflowed");
+    for (int i = 0; i < h->ninst; i++)
+      if (h->tables[i].p) h->g_bound[i] = actual[i];
+  }
+  bool est_used = false;
+  for (int i = 0; i < h->ninst; i++) {
+    if (rows[i] == 0) continue;
+    if (!h->tables[i].p) {
+      double g;
+      bool estimated = false;
+      if (h->opts.expected_groups > 0) {
+        g = std::min<double>((double)h->opts.expected_groups, (double)rows[i]);
+        estimated = true;
+      } else if (need_est && rows[i] > 4096) {
+        g = std::min<double>(est[i] * 1.3 + 64, (double)rows[i]);
+        estimated = true;
+      } else {
+        g = (double)rows[i];
+      }
+      est_used = est_used || estimated;
+      uint64_t cap = pow2_at_least((uint64_t)(g / lf) + 16);
+      if (h->plan.key_col < 0) cap = 16;
+      GZ_TRY(inst_alloc(h, i, cap));
+      h->g_bound[i] = 2 + (int64_t)std::min<double>(g, (double)rows[i]);
+      // placement: LDS first-level table when the groups fit in LDS
+      if (h->opts.placement == 0)
+        h->placement[i] = (g <= 256.0 || h->plan.key_col < 0) ? 1 : 2;
+      if (!estimated) h->g_bound[i] = 2 + rows[i];
+    } else if ((double)(h->g_bound[i] + rows[i]) > lf * h->inst[i].cap) {
+      uint64_t cap = pow2_at_least((uint64_t)((double)(h->g_bound[i] + rows[i]) * 2 / lf));
+      GZ_TRY(inst_alloc(h, i, cap));
+      h->g_bound[i] += rows[i];
+    } else {
+      h->g_bound[i] += rows[i];
+    }
+  }
+  if (h->inst_dirty) {
+    GZ_TRY(upload(ctx, &h->d_inst, h->inst.data(), sizeof(AggInst) * h->inst.size()));
+    h->inst_dirty = false;
+  }
+  p.inst = (const AggInst*)h->d_inst.p;
+  // ---- update: split tiles by placement
+  std::vector<AggTile> lds_tiles, hbm_tiles;
+  for (auto& t : tiles) (h->placement[t.inst] == 1 ? lds_tiles : hbm_tiles).push_back(t);
+  if (!hbm_tiles.empty()) {
+    if (!lds_tiles.empty())
+      GZ_TRY(upload(ctx, &h->d_tiles, hbm_tiles.data(), sizeof(AggTile) * hbm_tiles.size()));
+    p.tiles = (const AggTile*)h->d_tiles.p;
+    p.T = (int64_t)hbm_tiles.size();
+    int64_t grid = (p.T + 7) / 8 * 8;
+    KernelTimer kt(ctx, "agg_update");
+    hipLaunchKernelGGL(agg_update_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
+  }
+  if (!lds_tiles.empty()) {
+    DBuf* dt = hbm_tiles.empty() ? &h->d_tiles : nullptr;
+    DBuf second;
+    if (!dt) {
+      GZ_TRY(ctx_alloc(ctx, sizeof(AggTile) * lds_tiles.size(), &second));
+      dt = &second;
+    }
+    GZ_TRY(upload(ctx, dt, lds_tiles.data(), sizeof(AggTile) * lds_tiles.size()));
+    p.tiles = (const AggTile*)dt->p;
+    p.T = (int64_t)lds_tiles.size();
+    int64_t blocks = std::min<int64_t>(p.T, (int64_t)ctx->num_cus * 2);
+    p.tiles_per_block = (int)((p.T + blocks - 1) / blocks);
+    blocks = (p.T + p.tiles_per_block - 1) / p.tiles_per_block;
+    p.lcap = 1024;
+    int lds = lds_bytes(h, p.lcap);
+    while (lds > 96 * 1024 && p.lcap > 64) {
+      p.lcap >>= 1;
+      lds = lds_bytes(h, p.lcap);
+    }
+    KernelTimer kt(ctx, "agg_update_lds");
+    hipLaunchKernelGGL(agg_update_lds_kernel, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, p);
+    if (second.p) ctx_free(ctx, &second);
+  }
+  GZ_HIP(hipGetLastError());
+  for (int i = 0; i < h->ninst; i++) h->rows_seen[i] += rows[i];
+  if (est_used) {
+    // an estimate may be wrong: verify this call did not overflow
+    std::vector<int64_t> g;
+    int ovf = 0;
+    GZ_TRY(read_counts(h, g, &ovf));
+    if (ovf) return fail(GZ_CAPACITY_ERROR, "aggregate hash table overflowed (cardinality estimate)");
+    for (int i = 0; i < h->ninst; i++)
+      if (h->tables[i].p) h->g_bound[i] = g[i];
+  }
+  return GZ_OK;
+}
+
+static int agg_finish(gz_agg* h) {
+  if (h->finished) return GZ_OK;
+  gz_context* ctx = h->ctx;
+  GZ_HIP(hipSetDevice(ctx->device));
+  // instances never fed get an empty table so the collect kernel can run
+  for (int i = 0; i < h->ninst; i++)
+    if (!h->tables[i].p) GZ_TRY(inst_alloc(h, i, 16));
+  if (h->inst_dirty) {
+    GZ_TRY(upload(ctx, &h->d_inst, h->inst.data(), sizeof(AggInst) * h->inst.size()));
+    h->inst_dirty = false;
+  }
+  int64_t maxg = 0;
+  uint64_t maxcap = 0;
+  for (int i = 0; i < h->ninst; i++) {
+    maxg += (int64_t)h->inst[i].cap + 2;
+    maxcap = std::max(maxcap, h->inst[i].cap + 2);
+  }
+  GZ_TRY(ctx_ensure(ctx, &h->skeys, sizeof(uint64_t) * maxg));
+  GZ_TRY(ctx_ensure(ctx, &h->svals, sizeof(uint64_t) * maxg));
+  GZ_TRY(ctx_ensure(ctx, &h->tkeys, sizeof(uint64_t) * maxg));
+  GZ_TRY(ctx_ensure(ctx, &h->tvals, sizeof(uint64_t) * maxg));
+  GZ_TRY(ctx_ensure(ctx, &h->cursor, sizeof(uint64_t)));
+  GZ_TRY(ctx_ensure(ctx, &h->counts, sizeof(uint64_t) * h->ninst));
+  hipLaunchKernelGGL(fill_u32_kernel_agg, dim3(1), dim3(256), 0, ctx->stream, (uint32_t*)h->cursor.p, (int64_t)2);
+  int64_t nc32 = 2 * (int64_t)h->ninst;
+  hipLaunchKernelGGL(fill_u32_kernel_agg, dim3((unsigned)std::min<int64_t>((nc32 + 255) / 256, 4096)), dim3(256), 0,
+                     ctx->stream, (uint32_t*)h->counts.p, nc32);
+  {
+    KernelTimer kt(ctx, "agg_collect");
+    unsigned gx = (unsigned)std::min<uint64_t>((maxcap + 255) / 256, 4096);
+    hipLaunchKernelGGL(agg_collect_kernel, dim3(gx, (unsigned)h->ninst), dim3(256), 0, ctx->stream,
+                       (const AggInst*)h->d_inst.p, h->ninst, (unsigned long long*)h->tkeys.p,
+                       (unsigned long long*)h->tvals.p, (unsigned long long*)h->cursor.p,
+                       (unsigned long long*)h->counts.p);
+  }
+  std::vector<unsigned long long> cnt(h->ninst + 1);
+  GZ_HIP(hipMemcpyAsync(cnt.data(), h->counts.p, sizeof(uint64_t) * h->ninst, hipMemcpyDeviceToHost, ctx->stream));
+  GZ_HIP(hipMemcpyAsync(&cnt[h->ninst], h->cursor.p, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  GZ_HIP(hipStreamSynchronize(ctx->stream));
+  const int64_t n = (int64_t)cnt[h->ninst];
+  h->ngroups.assign(h->ninst, 0);
+  h->group_off.assign(h->ninst + 1, 0);
+  for (int i = 0; i < h->ninst; i++) {
+    h->ngroups[i] = (int64_t)cnt[i];
+    // the global aggregate has one output row once any row was processed
+    if (h->plan.key_col < 0) h->ngroups[i] = h->rows_seen[i] > 0 ? 1 : 0;
+    h->group_off[i + 1] = h->group_off[i] + (int64_t)cnt[i];
+  }
+  h->total_groups = n;
+  // sort by (instance, first row): first-seen order inside each instance
+  if (n > 0) {
+    int end_bit = 44;
+    while ((1ll << (end_bit - 44)) < h->ninst) end_bit++;
+    size_t tmp = 0;
+    GZ_HIP(rocprim::radix_sort_pairs(nullptr, tmp, (unsigned long long*)h->tkeys.p,
+                                     (unsigned long long*)h->skeys.p, (unsigned long long*)h->tvals.p,
+                                     (unsigned long long*)h->svals.p, (size_t)n, 0, end_bit, ctx->stream));
+    GZ_TRY(ctx_ensure(ctx, &h->sort_tmp, tmp));
+    KernelTimer kt(ctx, "agg_sort");
+    GZ_HIP(rocprim::radix_sort_pairs(h->sort_tmp.p, tmp, (unsigned long long*)h->tkeys.p,
+                                     (unsigned long long*)h->skeys.p, (unsigned long long*)h->tvals.p,
+                                     (unsigned long long*)h->svals.p, (size_t)n, 0, end_bit, ctx->stream));
+  }
+  const int nres = (int)h->plan.results.size();
+  h->res_values.resize(nres);
+  h->res_valid.resize(nres);
+  FinishParams f;
+  memset(&f, 0, sizeof(f));
+  f.inst = (const AggInst*)h->d_inst.p;
+  f.skeys = (const unsigned long long*)h->skeys.p;
+  f.svals = (const unsigned long long*)h->svals.p;
+  f.n = n;
+  f.key_type = h->plan.key_type;
+  f.nres = nres;
+  for (int c = 0; c < nres; c++) {
+    f.res[c] = h->plan.results[c];
+    int w = std::max(1, type_width(f.res[c].rtype));
+    GZ_TRY(ctx_alloc(ctx, (size_t)std::max<int64_t>(n, 1) * w, &h->res_values[c]));
+    GZ_TRY(ctx_alloc(ctx, (size_t)std::max<int64_t>(n, 1), &h->res_valid[c]));
+    f.out_values[c] = h->res_values[c].p;
+    f.out_valid[c] = (uint8_t*)h->res_valid[c].p;
+  }
+  if (n > 0) {
+    KernelTimer kt(ctx, "agg_finish");
+    unsigned gx = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(agg_finish_kernel, dim3(gx), dim3(256), 0, ctx->stream, f);
+  }
+  GZ_HIP(hipGetLastError());
+  GZ_HIP(hipStreamSynchronize(ctx->stream));
+  h->finished = true;
+  return GZ_OK;
+}
+
+extern "C" {
+
+void gz_agg_options_defaults(gz_agg_options* o) {
+  if (!o) return;
+  o->placement = 0;
+  o->tile_rows = 0;
+  o->load_factor = 0.5;
+  o->expected_groups = 0;
+}
+
+int gz_agg_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types, int32_t num_actions,
+                const gz_agg_action* actions, const gz_agg_options* opts, gz_agg** out) {
+  return agg_set_make(ctx, num_cols, col_types, num_actions, actions, opts, 1, out);
+}
+
+int gz_agg_process(gz_agg* h, int64_t num_rows, const int64_t* buf_addrs, const int64_t* buf_sizes,
+                   int32_t num_bufs) {
+  if (!h) return fail(GZ_INVALID, "null aggregate");
+  if (num_rows == 0) return GZ_OK;  // ProcessAndCacheOne returns early on empty input
+  if (!buf_addrs || !buf_sizes) return fail(GZ_INVALID, "buf_addrs can't be null");
+  gz_batch b{num_rows, buf_addrs, buf_sizes, num_bufs};
+  return agg_process(h, 1, &b, nullptr);
+}
+
+int gz_agg_process_batches(gz_agg* h, int32_t num_batches, const gz_batch* batches) {
+  if (!h) return fail(GZ_INVALID, "null aggregate");
+  if (num_batches < 0 || (num_batches > 0 && !batches)) return fail(GZ_INVALID, "bad batch list");
+  return agg_process(h, num_batches, batches, nullptr);
+}
+
+int gz_agg_num_groups(gz_agg* h, int64_t* out) {
+  if (!h || !out) return fail(GZ_INVALID, "null argument");
+  if (h->finished) {
+    *out = h->ngroups[0];
+    return GZ_OK;
+  }
+  if (h->plan.key_col < 0) {
+    *out = h->rows_seen[0] > 0 ? 1 : 0;
+    return GZ_OK;
+  }
+  if (!h->tables[0].p) {
+    *out = 0;
+    return GZ_OK;
+  }
+  GZ_TRY(agg_finish(h));
+  *out = h->ngroups[0];
+  return GZ_OK;
+}
+
+int gz_agg_num_result_columns(gz_agg* h) { return h ? (int)h->plan.results.size() : 0; }
+
+int gz_agg_result_type(gz_agg* h, int32_t col) {
+  if (!h || col < 0 || col >= (int)h->plan.results.size()) return -1;
+  return h->plan.results[col].rtype;
+}
+
+int gz_agg_finish(gz_agg* h) {
+  if (!h) return fail(GZ_INVALID, "null aggregate");
+  return agg_finish(h);
+}
+
+int gz_agg_result_column(gz_agg* h, int32_t col, void** values, void** valid) {
+  if (!h) return fail(GZ_INVALID, "null aggregate");
+  if (!h->finished) return fail(GZ_INVALID, "call gz_agg_finish first");
+  if (col < 0 || col >= (int)h->plan.results.size()) return fail(GZ_INDEX_ERROR, "result column out of range");
+  int w = std::max(1, type_width(h->plan.results[col].rtype));
+  if (values) *values = (char*)h->res_values[col].p + h->group_off[0] * w;
+  if (valid) *valid = (char*)h->res_valid[col].p + h->group_off[0];
+  return GZ_OK;
+}
+
+int gz_agg_release(gz_agg* h) {
+  if (!h) return GZ_OK;
+  gz_context* ctx = h->ctx;
+  for (auto& b : h->tables) ctx_free(ctx, &b);
+  for (auto& b : h->counters) ctx_free(ctx, &b);
+  for (auto& b : h->res_values) ctx_free(ctx, &b);
+  for (auto& b : h->res_valid) ctx_free(ctx, &b);
+  for (DBuf* b : {&h->d_inst, &h->d_lane_ops, &h->d_tiles, &h->d_bufs, &h->d_regs, &h->d_est, &h->skeys,
+                  &h->svals, &h->tkeys, &h->tvals, &h->cursor, &h->counts, &h->sort_tmp})
+    ctx_free(ctx, b);
+  delete h;
+  return GZ_OK;
+}
+
+// ---- aggregate sets (many independent GROUP BYs in one launch sequence)
+int gz_agg_set_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types, int32_t num_actions,
+                    const gz_agg_action* actions, const gz_agg_options* opts, int32_t num_instances,
+                    gz_agg** out) {
+  return agg_set_make(ctx, num_cols, col_types, num_actions, actions, opts, num_instances, out);
+}
+
+int gz_agg_set_process(gz_agg* h, int32_t num_batches, const gz_batch* batches, const int32_t* instance_of) {
+  if (!h) return fail(GZ_INVALID, "null aggregate set");
+  if (num_batches < 0 || (num_batches > 0 && (!batches || !instance_of))) return fail(GZ_INVALID, "bad batch list");
+  return agg_process(h, num_batches, batches, instance_of);
+}
+
+int gz_agg_set_num_groups(gz_agg* h, int32_t inst, int64_t* out) {
+  if (!h || !out) return fail(GZ_INVALID, "null argument");
+  if (!h->finished) return fail(GZ_INVALID, "call gz_agg_finish first");
+  if (inst < 0 || inst >= h->ninst) return fail(GZ_INDEX_ERROR, "instance out of range");
+  *out = h->ngroups[inst];
+  return GZ_OK;
+}
+
+int gz_agg_set_result_column(gz_agg* h, int32_t inst, int32_t col, void** values, void** valid) {
+  if (!h) return fail(GZ_INVALID, "null aggregate set");
+  if (!h->finished) return fail(GZ_INVALID, "call gz_agg_finish first");
+  if (inst < 0 || inst >= h->ninst) return fail(GZ_INDEX_ERROR, "instance out of range");
+  if (col < 0 || col >= (int)h->plan.results.size()) return fail(GZ_INDEX_ERROR, "result column out of range");
+  int w = std::max(1, type_width(h->plan.results[col].rtype));
+  if (values) *values = (char*)h->res_values[col].p + h->group_off[inst] * w;
+  if (valid) *valid = (char*)h->res_valid[col].p + h->group_off[inst];
+  return GZ_OK;
+}
+
+int gz_agg_set_placement(gz_agg* h, int32_t inst, int32_t* out) {
+  if (!h || !out || inst < 0 || inst >= h->ninst) return fail(GZ_INVALID, "bad argument");
+  *out = h->placement[inst];
+  return GZ_OK;
+}
+
+}  // extern "C"

@@ -227,6 +227,21 @@ int gz_agg_finish(gz_agg* h);
 int gz_agg_result_column(gz_agg* h, int32_t col, void** values, void** valid_bytes);
 int gz_agg_release(gz_agg* h);
 
+/* Aggregate sets: many independent GROUP BYs (instances) sharing one plan,
+ * processed in one launch sequence -- the batched form.  A set is a gz_agg
+ * handle; gz_agg_finish and gz_agg_release apply to it.  No counterpart in the
+ * reference, which runs one HashAggregateResultIterator per task. */
+int gz_agg_set_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types,
+                    int32_t num_actions, const gz_agg_action* actions,
+                    const gz_agg_options* opts, int32_t num_instances, gz_agg** out);
+int gz_agg_set_process(gz_agg* h, int32_t num_batches, const gz_batch* batches,
+                       const int32_t* instance_of_batch);
+int gz_agg_set_num_groups(gz_agg* h, int32_t inst, int64_t* out);
+int gz_agg_set_result_column(gz_agg* h, int32_t inst, int32_t col, void** values,
+                             void** valid_bytes);
+/* Placement the autotuner chose for an instance: 1 = LDS first level, 2 = HBM. */
+int gz_agg_set_placement(gz_agg* h, int32_t inst, int32_t* out);
+
 /* ---- hash join -----------------------------------------------------------------
  * Build replaces HashRelationKernel (ext/hash_relation_kernel.cc:50-344) and
  * HashRelation (codegen/common/hash_relation.h:123-645); probe replaces
