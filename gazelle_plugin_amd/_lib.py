@@ -128,10 +128,23 @@ _OPTIONAL_SIGS = {
 }
 
 
+def _bind_process_hip_runtime():
+    """torch ships its own libamdhip64.so (SONAME libamdhip64.so.7).  Importing
+    torch first makes the dynamic loader bind this library's libamdhip64.so.7
+    dependency to that already-mapped runtime, so one HIP runtime serves the
+    process; loading ours first would map a second runtime that torch cannot
+    share the device with."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Loads the HIP library; raises if it was not built."""
     global _lib
     if _lib is None:
+        _bind_process_hip_runtime()
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                               "(there is no CPU fallback)")

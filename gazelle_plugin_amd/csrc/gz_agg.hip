@@ -893,6 +893,7 @@ static int agg_stage(gz_agg* h, int32_t nb, const gz_batch* batches, const int32
       int t = h->plan.types[c];
       const void** slot = &bufs[((size_t)bi * nc + c) * 3];
       slot[0] = (bt.buf_addrs[k] != 0 && bt.buf_sizes[k] > 0) ? (const void*)bt.buf_addrs[k] : nullptr;
+      if (slot[0] && bt.buf_sizes[k] < (bt.num_rows + 7) / 8) return fail(GZ_INVALID, "validity buffer too small");
       k++;
       if (type_is_binary(t)) slot[1] = (const void*)bt.buf_addrs[k++];
       slot[2] = (const void*)bt.buf_addrs[k];

@@ -1,0 +1,1063 @@
+// Hash join build and probe for MI355X (gfx950).
+//
+// Build replaces HashRelationKernel / HashRelation (ext/hash_relation_kernel.cc,
+// codegen/common/hash_relation.h) on an unsafeHashMap (row_wise_memory/hashMap.h);
+// probe replaces ConditionedProbeResultIterator's inner / semi / anti functions
+// (ext/conditioned_probe_kernel.cc:546-1173).
+//
+// The relation copies every build batch into a column store it owns in HBM (rows
+// in batch order, then row order = the reference's insertion order).  Finish
+// builds an open-addressing table of unique keys (linear probing, 64-bit CAS on
+// the key, capacity 2^(ceil(log2 rows) + 1) so the load stays <= 0.5 as in
+// hash_relation_kernel.cc:205-224).  Each slot keeps its key's row count and
+// first row; when some key repeats, a stable sort of (slot, row) lays the rows of
+// every key out contiguously in insertion order, which is the order the
+// reference's bytesMap chain returns them in (hashMap.h:826-876).
+//
+// A probe call runs count -> tile scan -> write: per probe row the number of
+// output rows (inner: the key's row count; semi: 1 on a match; anti: 1 on no
+// match, with the null-aware rules of :1020-1035), an exclusive scan in row
+// order, and a write pass that gathers probe and build columns into the output
+// in probe-row order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <string>
+#include <vector>
+
+#include "gz_device.h"
+#include "gz_internal.h"
+
+namespace gz {
+
+constexpr int kJoinMaxCols = 32;
+constexpr int64_t kJEmpty = INT64_MIN;
+
+__device__ __forceinline__ uint64_t jmix64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+// Join key of a typed value, widened to 64 bits: integers sign/zero extended,
+// booleans 0/1, floats as the bits of the double value.  *nan marks a NaN key,
+// which never equals anything (the reference compares typed keys with ==).
+__device__ __forceinline__ int64_t join_key(const void* v, int t, int64_t r, bool* nan) {
+  *nan = false;
+  switch (t) {
+    case GZ_TYPE_BOOL: return bit_get((const uint8_t*)v, r);
+    case GZ_TYPE_INT8: return ((const int8_t*)v)[r];
+    case GZ_TYPE_UINT8: return ((const uint8_t*)v)[r];
+    case GZ_TYPE_INT16: return ((const int16_t*)v)[r];
+    case GZ_TYPE_UINT16: return ((const uint16_t*)v)[r];
+    case GZ_TYPE_INT32: case GZ_TYPE_DATE32: return ((const int32_t*)v)[r];
+    case GZ_TYPE_UINT32: return ((const uint32_t*)v)[r];
+    case GZ_TYPE_FLOAT: {
+      double d = (double)((const float*)v)[r];
+      *nan = d != d;
+      return __double_as_longlong(d);
+    }
+    case GZ_TYPE_DOUBLE: {
+      double d = ((const double*)v)[r];
+      *nan = d != d;
+      return __double_as_longlong(d);
+    }
+    default: return ((const int64_t*)v)[r];
+  }
+}
+
+struct JTable {
+  int64_t* keys;    // [cap]
+  int32_t* start;   // [cap] first row (unique keys) or rowlist start
+  int32_t* count;   // [cap]
+  int32_t* rowlist; // rows grouped by key in insertion order, or null (unique keys)
+  uint64_t cap;
+  int64_t special_start, special_count;  // the key equal to kJEmpty
+};
+
+__device__ __forceinline__ int64_t jt_find(const JTable& J, int64_t key) {
+  uint64_t mask = J.cap - 1;
+  uint64_t s = jmix64((uint64_t)key) & mask;
+  for (uint64_t i = 0; i < J.cap; i++) {
+    int64_t k = J.keys[s];
+    if (k == key) return (int64_t)s;
+    if (k == kJEmpty) return -1;
+    s = (s + 1) & mask;
+  }
+  return -1;
+}
+
+// ---------------------------------------------------------------- build store
+// One appended batch column: bits -> bytes for validity and booleans, values
+// copied, utf8 offsets rebased to the store's int64 offsets.
+struct AppendCol {
+  const uint8_t* valid;  // input bitmap or null
+  const void* values;
+  const int32_t* offsets;
+  int type, width;
+  uint8_t* dvalid;       // store validity bytes (row-indexed)
+  uint8_t* dvalues;      // store values (fixed: width per row; bool: 1 byte)
+  int64_t* doffsets;     // store offsets (binary), entry row+1 written
+  uint8_t* ddata;
+  int64_t data_base;     // store byte position of this batch's first byte
+};
+
+struct AppendParams {
+  int ncols;
+  AppendCol col[kJoinMaxCols];
+  int64_t n;
+  int64_t row0;  // store row of the batch's first row
+};
+
+__global__ void join_append_kernel(AppendParams p) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t d = p.row0 + i;
+    for (int c = 0; c < p.ncols; c++) {
+      const AppendCol& a = p.col[c];
+      if (a.type == GZ_TYPE_NA) continue;
+      a.dvalid[d] = a.valid ? (uint8_t)bit_get(a.valid, i) : 1;
+      if (a.type == GZ_TYPE_BOOL) {
+        a.dvalues[d] = (uint8_t)bit_get((const uint8_t*)a.values, i);
+      } else if (a.type == GZ_TYPE_STRING || a.type == GZ_TYPE_BINARY) {
+        a.doffsets[d + 1] = a.data_base + (a.offsets[i + 1] - a.offsets[0]);
+        if (i == 0) a.doffsets[d] = a.data_base;
+      } else {
+        const uint8_t* s = (const uint8_t*)a.values + i * a.width;
+        uint8_t* t = a.dvalues + d * a.width;
+        for (int k = 0; k < a.width; k++) t[k] = s[k];
+      }
+    }
+  }
+}
+
+__global__ void join_copy_bytes_kernel(uint8_t* dst, const uint8_t* src, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------- build table
+__global__ void join_table_init_kernel(int64_t* keys, int32_t* start, int32_t* count, uint64_t cap) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < (int64_t)cap;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    keys[s] = kJEmpty;
+    start[s] = INT32_MAX;
+    count[s] = 0;
+  }
+}
+
+// flags: [0] has null key, [1] has duplicate key, [2] inserted keys,
+//        [3] rows with key kJEmpty, [4] first such row
+__global__ void join_build_kernel(const uint8_t* kvalid, const uint8_t* kvals, int ktype, int kwidth,
+                                  int64_t n, int64_t* keys, int32_t* start, int32_t* count,
+                                  uint64_t cap, int32_t* row_slot, int skip_dup,
+                                  unsigned long long* flags) {
+  const uint64_t mask = cap - 1;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    row_slot[r] = -1;
+    if (!kvalid[r]) {
+      flags[0] = 1;
+      continue;
+    }
+    bool nan;
+    int64_t key;
+    if (ktype == GZ_TYPE_BOOL) {
+      key = kvals[r];
+      nan = false;
+    } else {
+      key = join_key(kvals, ktype, r, &nan);
+    }
+    if (nan) continue;  // a NaN build key can never be matched
+    if (key == kJEmpty) {
+      unsigned long long c = atomicAdd(&flags[3], 1ull);
+      atomicMin((long long*)&flags[4], (long long)r);
+      if (c >= 1) flags[1] = 1;
+      row_slot[r] = -2;
+      continue;
+    }
+    uint64_t s = jmix64((uint64_t)key) & mask;
+    for (uint64_t i = 0; i < cap; i++) {
+      int64_t k = keys[s];
+      if (k == kJEmpty) {
+        unsigned long long prev = atomicCAS((unsigned long long*)&keys[s], (unsigned long long)kJEmpty,
+                                            (unsigned long long)key);
+        if ((int64_t)prev == kJEmpty) {
+          atomicAdd(&flags[2], 1ull);
+          k = key;
+        } else {
+          k = (int64_t)prev;
+        }
+      }
+      if (k == key) {
+        int c = atomicAdd(&count[s], 1);
+        if (c >= 1) flags[1] = 1;
+        atomicMin(&start[s], (int32_t)r);
+        row_slot[r] = (int32_t)s;
+        break;
+      }
+      s = (s + 1) & mask;
+    }
+  }
+  (void)kwidth;
+  (void)skip_dup;
+}
+
+// With skip-duplicate building (semi joins) every key keeps one row.
+__global__ void join_clamp_counts_kernel(int32_t* count, uint64_t cap) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < (int64_t)cap;
+       s += (int64_t)gridDim.x * blockDim.x)
+    if (count[s] > 1) count[s] = 1;
+}
+
+// (slot, row) sort keys of the rows of repeated keys.
+__global__ void join_pair_keys_kernel(const int32_t* row_slot, int64_t n, unsigned long long* out) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    int32_t s = row_slot[r];
+    unsigned long long hi = s >= 0 ? (unsigned long long)s : (s == -2 ? 0x7fffffffull : 0xffffffffull);
+    out[r] = (hi << 32) | (unsigned long long)r;
+  }
+}
+
+__global__ void join_rowlist_kernel(const unsigned long long* sorted, int64_t n, int32_t* rowlist) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    rowlist[i] = (int32_t)(sorted[i] & 0xffffffffull);
+}
+
+// ---------------------------------------------------------------- probe
+struct PTile {
+  int32_t batch;
+  int32_t row_begin;
+  int32_t nrows;
+  int32_t pad;
+};
+
+struct OutCol {
+  int side;        // 0 build, 1 probe
+  int col;         // column on that side
+  int type, width;
+  // build store
+  const uint8_t* bvalid;
+  const uint8_t* bvalues;
+  // output
+  uint8_t* ovalid;
+  uint8_t* ovalues;
+  int binary;
+};
+
+struct ProbeParams {
+  const PTile* tiles;
+  int64_t T;
+  const void* const* bufs;  // probe batches: [batch * bstride + col * 3 + k]
+  int bstride;
+  int key_col, key_type;
+  int join_type, null_aware;
+  int build_has_null;
+  int64_t build_keys;       // keys in the table (null keys excluded)
+  JTable J;
+  int32_t* match;           // per probe row (tile-major): slot, -1 none, -2 the kJEmpty key
+  uint32_t* tile_count;
+  const uint64_t* tile_off;
+  int ncols;
+  OutCol out[kJoinMaxCols];
+  int64_t* pair_probe;      // (tile, row) of each output row when binary columns need a gather
+  int32_t* pair_build;
+  int write_pairs;
+  int tile_rows;
+};
+
+__device__ __forceinline__ const void* pbuf(const ProbeParams& p, int batch, int col, int k) {
+  return p.bufs[(int64_t)batch * p.bstride + col * 3 + k];
+}
+
+__device__ __forceinline__ int out_count(const ProbeParams& p, int32_t m, bool key_null) {
+  if (p.join_type == GZ_JOIN_INNER) {
+    if (m >= 0) return p.J.count[m];
+    if (m == -2) return (int)p.J.special_count;
+    return 0;
+  }
+  bool found = m != -1;
+  if (p.join_type == GZ_JOIN_SEMI) return found ? 1 : 0;
+  // anti
+  if (!p.null_aware) return found ? 0 : 1;
+  if (p.build_keys == 0 && !p.build_has_null) return 1;
+  if (p.build_has_null) return 0;
+  return (!key_null && !found) ? 1 : 0;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void join_probe_count_kernel(ProbeParams p) {
+  __shared__ uint64_t s_scan[BLOCK / kWave + 1];
+  int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
+  if (t < 0) return;
+  const PTile td = p.tiles[t];
+  const uint8_t* kv = (const uint8_t*)pbuf(p, td.batch, p.key_col, 0);
+  const void* kd = pbuf(p, td.batch, p.key_col, 2);
+  uint64_t sum = 0;
+  for (int lr = threadIdx.x; lr < td.nrows; lr += BLOCK) {
+    int64_t r = (int64_t)td.row_begin + lr;
+    bool knull = kv && !bit_get(kv, r);
+    int32_t m = -1;
+    if (!knull) {
+      bool nan;
+      int64_t key = join_key(kd, p.key_type, r, &nan);
+      if (!nan) {
+        if (key == kJEmpty) m = p.J.special_count > 0 ? -2 : -1;
+        else m = (int32_t)jt_find(p.J, key);
+      }
+    }
+    p.match[t * p.tile_rows + lr] = knull ? -3 : m;
+    sum += (uint64_t)out_count(p, m, knull);
+  }
+  uint64_t total;
+  block_exclusive_scan<uint64_t>(sum, s_scan, &total);
+  if (threadIdx.x == 0) p.tile_count[t] = (uint32_t)total;
+}
+
+template <int TILE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void join_probe_write_kernel(ProbeParams p) {
+  constexpr int RPT = TILE / BLOCK;  // consecutive rows per thread
+  __shared__ uint64_t s_scan[BLOCK / kWave + 1];
+  int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
+  if (t < 0) return;
+  const PTile td = p.tiles[t];
+  int lr0 = threadIdx.x * RPT;
+  int cnt[RPT];
+  int32_t mm[RPT];
+  uint64_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    int lr = lr0 + k;
+    int32_t m = lr < td.nrows ? p.match[t * p.tile_rows + lr] : -1;
+    mm[k] = m;
+    cnt[k] = lr < td.nrows ? out_count(p, m == -3 ? -1 : m, m == -3) : 0;
+    sum += (uint64_t)cnt[k];
+  }
+  uint64_t ex = block_exclusive_scan<uint64_t>(sum, s_scan, nullptr);
+  uint64_t o = p.tile_off[t] + ex;
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    int lr = lr0 + k;
+    int64_t r = (int64_t)td.row_begin + lr;
+    int32_t m = mm[k];
+    for (int j = 0; j < cnt[k]; j++, o++) {
+      int32_t b = -1;  // build row (inner)
+      if (p.join_type == GZ_JOIN_INNER) {
+        int64_t st = m >= 0 ? p.J.start[m] : p.J.special_start;
+        b = p.J.rowlist ? p.J.rowlist[st + j] : (int32_t)(st + j);
+      }
+      if (p.write_pairs) {
+        p.pair_probe[o] = ((int64_t)t << 32) | (int64_t)lr;
+        p.pair_build[o] = b;
+      }
+      for (int c = 0; c < p.ncols; c++) {
+        const OutCol& oc = p.out[c];
+        if (oc.binary || oc.type == GZ_TYPE_NA) {
+          if (oc.type == GZ_TYPE_NA) oc.ovalid[o] = 0;
+          continue;
+        }
+        const int w = oc.width;
+        if (oc.side == 0) {
+          if (b < 0) {  // semi / anti: build columns are null (AppendNull)
+            oc.ovalid[o] = 0;
+            for (int x = 0; x < w; x++) oc.ovalues[o * w + x] = 0;
+            continue;
+          }
+          oc.ovalid[o] = oc.bvalid[b];
+          const uint8_t* s = oc.bvalues + (int64_t)b * w;
+          uint8_t* d = oc.ovalues + o * w;
+          if (w == 8) *(uint64_t*)d = *(const uint64_t*)s;
+          else if (w == 4) *(uint32_t*)d = *(const uint32_t*)s;
+          else for (int x = 0; x < w; x++) d[x] = s[x];
+        } else {
+          const uint8_t* vb = (const uint8_t*)pbuf(p, td.batch, oc.col, 0);
+          const uint8_t* vv = (const uint8_t*)pbuf(p, td.batch, oc.col, 2);
+          oc.ovalid[o] = vb ? (uint8_t)bit_get(vb, r) : 1;
+          if (oc.type == GZ_TYPE_BOOL) {
+            oc.ovalues[o] = (uint8_t)bit_get(vv, r);
+          } else if (w == 8) {
+            *(uint64_t*)(oc.ovalues + o * 8) = ((const uint64_t*)vv)[r];
+          } else if (w == 4) {
+            *(uint32_t*)(oc.ovalues + o * 4) = ((const uint32_t*)vv)[r];
+          } else {
+            for (int x = 0; x < w; x++) oc.ovalues[o * w + x] = vv[r * w + x];
+          }
+        }
+      }
+    }
+  }
+}
+
+// utf8/binary output columns: lengths, then (after a scan) bytes.
+struct BinGather {
+  const PTile* tiles;
+  const void* const* bufs;
+  int bstride;
+  int side, col;
+  const int64_t* pair_probe;
+  const int32_t* pair_build;
+  int64_t n;
+  const uint8_t* bvalid;
+  const int64_t* boff;
+  const uint8_t* bdata;
+  uint8_t* ovalid;
+  int64_t* olen;     // pass 1: lengths; after the scan: start offsets
+  int32_t* ooff;     // output int32 offsets (n + 1)
+  uint8_t* odata;
+};
+
+__device__ __forceinline__ void bin_src(const BinGather& g, int64_t i, const uint8_t** src, int64_t* len,
+                                        uint8_t* valid) {
+  if (g.side == 0) {
+    int32_t b = g.pair_build[i];
+    if (b < 0) { *src = nullptr; *len = 0; *valid = 0; return; }
+    *valid = g.bvalid[b];
+    *src = g.bdata + g.boff[b];
+    *len = g.boff[b + 1] - g.boff[b];
+  } else {
+    int64_t pp = g.pair_probe[i];
+    int t = (int)(pp >> 32), lr = (int)(pp & 0xffffffff);
+    PTile td = g.tiles[t];
+    int64_t r = (int64_t)td.row_begin + lr;
+    const uint8_t* vb = (const uint8_t*)g.bufs[(int64_t)td.batch * g.bstride + g.col * 3 + 0];
+    const int32_t* off = (const int32_t*)g.bufs[(int64_t)td.batch * g.bstride + g.col * 3 + 1];
+    const uint8_t* data = (const uint8_t*)g.bufs[(int64_t)td.batch * g.bstride + g.col * 3 + 2];
+    *valid = vb ? (uint8_t)bit_get(vb, r) : 1;
+    *src = data + off[r];
+    *len = off[r + 1] - off[r];
+  }
+}
+
+__global__ void join_bin_len_kernel(BinGather g) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint8_t* s;
+    int64_t len;
+    uint8_t v;
+    bin_src(g, i, &s, &len, &v);
+    g.olen[i] = len;
+    g.ovalid[i] = v;
+  }
+}
+
+__global__ void join_bin_copy_kernel(BinGather g) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint8_t* s;
+    int64_t len;
+    uint8_t v;
+    bin_src(g, i, &s, &len, &v);
+    int64_t st = g.olen[i];
+    g.ooff[i] = (int32_t)st;
+    if (i == g.n - 1) g.ooff[g.n] = (int32_t)(st + len);
+    for (int64_t x = 0; x < len; x++) g.odata[st + x] = s[x];
+  }
+}
+
+__global__ void join_fill_i64_kernel(int64_t* p, int64_t n, int64_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace gz
+
+// =============================================================== host side
+using namespace gz;
+
+struct ColStore {
+  int type = 0, width = 0;
+  DBuf valid, values, offsets, data;
+  int64_t data_bytes = 0, data_cap = 0;
+};
+
+struct gz_relation {
+  gz_context* ctx = nullptr;
+  int ncols = 0;
+  std::vector<int> types;
+  int key_col = 0, builder_type = 1;
+  std::vector<ColStore> cols;
+  int64_t n = 0, cap_rows = 0;
+  // table
+  DBuf keys, start, count, rowlist, row_slot, flags, sort_a, sort_b, sort_tmp;
+  uint64_t cap = 0;
+  bool finished = false, has_null = false, has_dups = false;
+  int64_t nkeys = 0, special_start = 0, special_count = 0;
+};
+
+static int rel_grow(gz_relation* r, int64_t need) {
+  if (need <= r->cap_rows) return GZ_OK;
+  gz_context* ctx = r->ctx;
+  int64_t nc = std::max<int64_t>(need, std::max<int64_t>(r->cap_rows * 2, 4096));
+  for (auto& c : r->cols) {
+    if (c.type == GZ_TYPE_NA) continue;
+    DBuf nv, nval, noff;
+    GZ_TRY(ctx_alloc(ctx, (size_t)nc, &nv));
+    if (r->n) GZ_HIP(hipMemcpyAsync(nv.p, c.valid.p, (size_t)r->n, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx_free(ctx, &c.valid);
+    c.valid = nv;
+    if (type_is_binary(c.type)) {
+      GZ_TRY(ctx_alloc(ctx, sizeof(int64_t) * (nc + 1), &noff));
+      if (r->n)
+        GZ_HIP(hipMemcpyAsync(noff.p, c.offsets.p, sizeof(int64_t) * (r->n + 1), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+      else
+        GZ_HIP(hipMemsetAsync(noff.p, 0, sizeof(int64_t), ctx->stream));
+      ctx_free(ctx, &c.offsets);
+      c.offsets = noff;
+    } else {
+      int w = c.type == GZ_TYPE_BOOL ? 1 : c.width;
+      GZ_TRY(ctx_alloc(ctx, (size_t)nc * w, &nval));
+      if (r->n) GZ_HIP(hipMemcpyAsync(nval.p, c.values.p, (size_t)r->n * w, hipMemcpyDeviceToDevice, ctx->stream));
+      ctx_free(ctx, &c.values);
+      c.values = nval;
+    }
+  }
+  r->cap_rows = nc;
+  return GZ_OK;
+}
+
+static int rel_append_one(gz_relation* r, const gz_batch& bt) {
+  gz_context* ctx = r->ctx;
+  if (r->finished) return fail(GZ_INVALID, "relation already finished");
+  if (bt.num_rows < 0) return fail(GZ_INVALID, "bad num_rows");
+  if (bt.num_rows == 0) return GZ_OK;
+  // ArrayItemIndex limits (array_item_index.h:24-29) do not apply: rows are int32
+  if (r->n + bt.num_rows > INT32_MAX) return fail(GZ_CAPACITY_ERROR, "build side exceeds 2^31 rows");
+  int need = 0;
+  for (int c = 0; c < r->ncols; c++) need += type_num_bufs(r->types[c]);
+  if (bt.num_bufs < need) return fail(GZ_INVALID, "insufficient number of in_buf_addrs");
+  GZ_TRY(rel_grow(r, r->n + bt.num_rows));
+  AppendParams ap;
+  memset(&ap, 0, sizeof(ap));
+  ap.ncols = r->ncols;
+  ap.n = bt.num_rows;
+  ap.row0 = r->n;
+  int k = 0;
+  for (int c = 0; c < r->ncols; c++) {
+    ColStore& cs = r->cols[c];
+    AppendCol& a = ap.col[c];
+    a.type = cs.type;
+    a.width = cs.width;
+    a.valid = (bt.buf_addrs[k] && bt.buf_sizes[k] > 0) ? (const uint8_t*)bt.buf_addrs[k] : nullptr;
+    if (a.valid && bt.buf_sizes[k] < (bt.num_rows + 7) / 8) return fail(GZ_INVALID, "validity buffer too small");
+    k++;
+    if (type_is_binary(cs.type)) {
+      if (bt.buf_sizes[k] < (bt.num_rows + 1) * 4) return fail(GZ_INVALID, "offsets buffer too small");
+      a.offsets = (const int32_t*)bt.buf_addrs[k++];
+      // bytes used by the batch: read both ends of its offsets
+      int32_t ends[2];
+      GZ_HIP(hipMemcpyAsync(&ends[0], a.offsets, 4, hipMemcpyDeviceToHost, ctx->stream));
+      GZ_HIP(hipMemcpyAsync(&ends[1], a.offsets + bt.num_rows, 4, hipMemcpyDeviceToHost, ctx->stream));
+      GZ_HIP(hipStreamSynchronize(ctx->stream));
+      int64_t nb = (int64_t)ends[1] - ends[0];
+      a.values = (const void*)bt.buf_addrs[k];
+      if (cs.data_bytes + nb > cs.data_cap) {
+        int64_t ncap = std::max<int64_t>(cs.data_bytes + nb, cs.data_cap * 2 + 4096);
+        DBuf nd;
+        GZ_TRY(ctx_alloc(ctx, (size_t)ncap, &nd));
+        if (cs.data_bytes)
+          GZ_HIP(hipMemcpyAsync(nd.p, cs.data.p, (size_t)cs.data_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        ctx_free(ctx, &cs.data);
+        cs.data = nd;
+        cs.data_cap = ncap;
+      }
+      if (nb > 0)
+        GZ_HIP(hipMemcpyAsync((uint8_t*)cs.data.p + cs.data_bytes, (const uint8_t*)a.values + ends[0], (size_t)nb,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+      a.data_base = cs.data_bytes;
+      cs.data_bytes += nb;
+      a.doffsets = (int64_t*)cs.offsets.p;
+    } else {
+      int64_t need_b = cs.type == GZ_TYPE_BOOL ? (bt.num_rows + 7) / 8 : bt.num_rows * cs.width;
+      if (cs.type != GZ_TYPE_NA && bt.buf_sizes[k] < need_b) return fail(GZ_INVALID, "values buffer too small");
+      a.values = (const void*)bt.buf_addrs[k];
+      a.dvalues = (uint8_t*)cs.values.p;
+    }
+    k++;
+    a.dvalid = (uint8_t*)cs.valid.p;
+    if (cs.type != GZ_TYPE_NA && !a.values) return fail(GZ_INVALID, "missing values buffer");
+  }
+  unsigned blocks = (unsigned)std::min<int64_t>((bt.num_rows + 255) / 256, 2048);
+  {
+    KernelTimer kt(ctx, "join_append");
+    hipLaunchKernelGGL(join_append_kernel, dim3(blocks), dim3(256), 0, ctx->stream, ap);
+  }
+  GZ_HIP(hipGetLastError());
+  r->n += bt.num_rows;
+  return GZ_OK;
+}
+
+extern "C" {
+
+int gz_relation_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types, int32_t key_col,
+                     int32_t builder_type, gz_relation** out) {
+  if (!ctx || !out || (num_cols > 0 && !col_types)) return fail(GZ_INVALID, "gz_relation_make: null argument");
+  if (num_cols > kJoinMaxCols) return fail(GZ_NOT_IMPLEMENTED, "more than 32 build columns");
+  if (key_col < 0 || key_col >= num_cols) return fail(GZ_INVALID, "key column out of range");
+  if (builder_type != 1 && builder_type != 3)
+    return fail(GZ_NOT_IMPLEMENTED, "builder_type " + std::to_string(builder_type) + " is not supported");
+  for (int c = 0; c < num_cols; c++)
+    if (!type_supported(col_types[c]))
+      return fail(GZ_NOT_IMPLEMENTED, "column type " + std::to_string(col_types[c]));
+  int kt = col_types[key_col];
+  if (type_is_binary(kt) || kt == GZ_TYPE_DECIMAL128 || kt == GZ_TYPE_NA)
+    return fail(GZ_NOT_IMPLEMENTED, "HashRelation Evaluate doesn't support single key type " + std::to_string(kt));
+  auto* r = new gz_relation();
+  r->ctx = ctx;
+  r->ncols = num_cols;
+  r->types.assign(col_types, col_types + num_cols);
+  r->key_col = key_col;
+  r->builder_type = builder_type;
+  r->cols.resize(num_cols);
+  for (int c = 0; c < num_cols; c++) {
+    r->cols[c].type = col_types[c];
+    r->cols[c].width = type_width(col_types[c]);
+  }
+  *out = r;
+  return GZ_OK;
+}
+
+int gz_relation_append(gz_relation* r, int64_t num_rows, const int64_t* buf_addrs,
+                       const int64_t* buf_sizes, int32_t num_bufs) {
+  if (!r) return fail(GZ_INVALID, "null relation");
+  if (!buf_addrs || !buf_sizes) return fail(GZ_INVALID, "buf_addrs can't be null");
+  gz_batch b{num_rows, buf_addrs, buf_sizes, num_bufs};
+  return rel_append_one(r, b);
+}
+
+int gz_relation_append_batches(gz_relation* r, int32_t num_batches, const gz_batch* batches) {
+  if (!r) return fail(GZ_INVALID, "null relation");
+  // reserve once so the column store grows a single time
+  int64_t total = r->n;
+  for (int i = 0; i < num_batches; i++) total += batches[i].num_rows;
+  GZ_TRY(rel_grow(r, total));
+  for (int i = 0; i < num_batches; i++) GZ_TRY(rel_append_one(r, batches[i]));
+  return GZ_OK;
+}
+
+int gz_relation_finish(gz_relation* r) {
+  if (!r) return fail(GZ_INVALID, "null relation");
+  if (r->finished) return GZ_OK;
+  gz_context* ctx = r->ctx;
+  hipStream_t st = ctx->stream;
+  GZ_HIP(hipSetDevice(ctx->device));
+  const int64_t n = r->n;
+  uint64_t cap = 128;
+  if (n > 32) {
+    cap = 1;
+    while (cap < (uint64_t)n) cap <<= 1;
+    cap <<= 1;  // 2^(ceil(log2 rows) + 1)
+  }
+  r->cap = cap;
+  GZ_TRY(ctx_alloc(ctx, sizeof(int64_t) * cap, &r->keys));
+  GZ_TRY(ctx_alloc(ctx, sizeof(int32_t) * cap, &r->start));
+  GZ_TRY(ctx_alloc(ctx, sizeof(int32_t) * cap, &r->count));
+  GZ_TRY(ctx_alloc(ctx, sizeof(int32_t) * std::max<int64_t>(n, 1), &r->row_slot));
+  GZ_TRY(ctx_alloc(ctx, sizeof(unsigned long long) * 8, &r->flags));
+  unsigned gc = (unsigned)std::min<uint64_t>((cap + 255) / 256, 8192);
+  hipLaunchKernelGGL(join_table_init_kernel, dim3(gc), dim3(256), 0, st, (int64_t*)r->keys.p,
+                     (int32_t*)r->start.p, (int32_t*)r->count.p, cap);
+  unsigned long long hflags[8] = {0, 0, 0, 0, (unsigned long long)INT64_MAX, 0, 0, 0};
+  GZ_HIP(hipMemcpyAsync(r->flags.p, hflags, sizeof(hflags), hipMemcpyHostToDevice, st));
+  const ColStore& kc = r->cols[r->key_col];
+  if (n > 0) {
+    KernelTimer kt(ctx, "join_build");
+    unsigned gb = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(join_build_kernel, dim3(gb), dim3(256), 0, st, (const uint8_t*)kc.valid.p,
+                       (const uint8_t*)kc.values.p, kc.type, kc.width, n, (int64_t*)r->keys.p,
+                       (int32_t*)r->start.p, (int32_t*)r->count.p, cap, (int32_t*)r->row_slot.p,
+                       r->builder_type == 3 ? 1 : 0, (unsigned long long*)r->flags.p);
+  }
+  GZ_HIP(hipGetLastError());
+  GZ_HIP(hipMemcpyAsync(hflags, r->flags.p, sizeof(hflags), hipMemcpyDeviceToHost, st));
+  GZ_HIP(hipStreamSynchronize(st));
+  r->has_null = hflags[0] != 0;
+  r->nkeys = (int64_t)hflags[2] + (hflags[3] ? 1 : 0);
+  r->special_count = (int64_t)hflags[3];
+  r->special_start = (int64_t)hflags[4];
+  bool dups = hflags[1] != 0;
+  if (r->builder_type == 3) {
+    // InsertSkipDup keeps the first row of every key
+    hipLaunchKernelGGL(join_clamp_counts_kernel, dim3(gc), dim3(256), 0, st, (int32_t*)r->count.p, cap);
+    if (r->special_count > 1) r->special_count = 1;
+    dups = false;
+  }
+  r->has_dups = dups;
+  if (dups) {
+    // stable (slot, row) order of all keyed rows, then each slot's start in it
+    GZ_TRY(ctx_alloc(ctx, sizeof(unsigned long long) * n, &r->sort_a));
+    GZ_TRY(ctx_alloc(ctx, sizeof(unsigned long long) * n, &r->sort_b));
+    GZ_TRY(ctx_alloc(ctx, sizeof(int32_t) * n, &r->rowlist));
+    unsigned gb = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(join_pair_keys_kernel, dim3(gb), dim3(256), 0, st, (const int32_t*)r->row_slot.p, n,
+                       (unsigned long long*)r->sort_a.p);
+    size_t tmp = 0;
+    GZ_HIP(rocprim::radix_sort_keys(nullptr, tmp, (unsigned long long*)r->sort_a.p,
+                                    (unsigned long long*)r->sort_b.p, (size_t)n, 0, 64, st));
+    GZ_TRY(ctx_ensure(ctx, &r->sort_tmp, tmp));
+    GZ_HIP(rocprim::radix_sort_keys(r->sort_tmp.p, tmp, (unsigned long long*)r->sort_a.p,
+                                    (unsigned long long*)r->sort_b.p, (size_t)n, 0, 64, st));
+    hipLaunchKernelGGL(join_rowlist_kernel, dim3(gb), dim3(256), 0, st, (const unsigned long long*)r->sort_b.p, n,
+                       (int32_t*)r->rowlist.p);
+    // start[slot] = exclusive scan of counts in slot order (the sort order)
+    size_t tmp2 = 0;
+    GZ_HIP(rocprim::exclusive_scan(nullptr, tmp2, (const int32_t*)r->count.p, (int32_t*)r->start.p, 0,
+                                   (size_t)cap, rocprim::plus<int32_t>(), st));
+    GZ_TRY(ctx_ensure(ctx, &r->sort_tmp, tmp2));
+    GZ_HIP(rocprim::exclusive_scan(r->sort_tmp.p, tmp2, (const int32_t*)r->count.p, (int32_t*)r->start.p, 0,
+                                   (size_t)cap, rocprim::plus<int32_t>(), st));
+    // the kJEmpty key sorts after every slot: its rows start after all slot rows
+    int64_t slot_rows = 0;
+    if (r->special_count) {
+      int32_t last[2];
+      GZ_HIP(hipMemcpyAsync(&last[0], (int32_t*)r->start.p + cap - 1, 4, hipMemcpyDeviceToHost, st));
+      GZ_HIP(hipMemcpyAsync(&last[1], (int32_t*)r->count.p + cap - 1, 4, hipMemcpyDeviceToHost, st));
+      GZ_HIP(hipStreamSynchronize(st));
+      slot_rows = (int64_t)last[0] + last[1];
+      r->special_start = slot_rows;
+    }
+    ctx_free(ctx, &r->sort_a);
+    ctx_free(ctx, &r->sort_b);
+  }
+  GZ_HIP(hipGetLastError());
+  r->finished = true;
+  return GZ_OK;
+}
+
+int gz_relation_num_keys(gz_relation* r, int64_t* out) {
+  if (!r || !out) return fail(GZ_INVALID, "null argument");
+  if (!r->finished) return fail(GZ_INVALID, "relation not finished");
+  *out = r->nkeys;
+  return GZ_OK;
+}
+
+int gz_relation_has_null_key(gz_relation* r, int32_t* out) {
+  if (!r || !out) return fail(GZ_INVALID, "null argument");
+  if (!r->finished) return fail(GZ_INVALID, "relation not finished");
+  *out = r->has_null ? 1 : 0;
+  return GZ_OK;
+}
+
+int gz_relation_release(gz_relation* r) {
+  if (!r) return GZ_OK;
+  gz_context* ctx = r->ctx;
+  for (auto& c : r->cols)
+    for (DBuf* b : {&c.valid, &c.values, &c.offsets, &c.data}) ctx_free(ctx, b);
+  for (DBuf* b : {&r->keys, &r->start, &r->count, &r->rowlist, &r->row_slot, &r->flags, &r->sort_a,
+                  &r->sort_b, &r->sort_tmp})
+    ctx_free(ctx, b);
+  delete r;
+  return GZ_OK;
+}
+
+}  // extern "C"
+
+struct gz_probe {
+  gz_relation* rel = nullptr;
+  int nprobe = 0;
+  std::vector<int> ptypes;
+  int key_col = 0, join_type = 0, null_aware = 0;
+  std::vector<int> sides, idx;
+  int tile_rows = 4096;
+  DBuf d_tiles, d_bufs, match, tile_count, tile_off, scan_tmp, pair_probe, pair_build;
+  struct Out {
+    int type = 0, width = 0;
+    DBuf values, valid, offsets, data, lens;
+  };
+  std::vector<Out> outs;
+  int64_t out_rows = 0;
+};
+
+static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* out_rows,
+                     int64_t* out_batch_rows) {
+  gz_relation* r = h->rel;
+  gz_context* ctx = r->ctx;
+  hipStream_t st = ctx->stream;
+  GZ_HIP(hipSetDevice(ctx->device));
+  const int TILE = h->tile_rows;
+  const int nc = h->nprobe;
+  std::vector<PTile> tiles;
+  std::vector<const void*> bufs((size_t)std::max(nb, 1) * nc * 3, nullptr);
+  for (int bi = 0; bi < nb; bi++) {
+    const gz_batch& bt = batches[bi];
+    if (bt.num_rows < 0 || bt.num_rows > INT32_MAX) return fail(GZ_INVALID, "bad num_rows");
+    int need = 0;
+    for (int c = 0; c < nc; c++) need += type_num_bufs(h->ptypes[c]);
+    if (bt.num_bufs < need) return fail(GZ_INVALID, "insufficient number of in_buf_addrs");
+    int k = 0;
+    for (int c = 0; c < nc; c++) {
+      const void** slot = &bufs[((size_t)bi * nc + c) * 3];
+      slot[0] = (bt.buf_addrs[k] && bt.buf_sizes[k] > 0) ? (const void*)bt.buf_addrs[k] : nullptr;
+      if (slot[0] && bt.buf_sizes[k] < (bt.num_rows + 7) / 8) return fail(GZ_INVALID, "validity buffer too small");
+      k++;
+      if (type_is_binary(h->ptypes[c])) {
+        if (bt.buf_sizes[k] < (bt.num_rows + 1) * 4) return fail(GZ_INVALID, "offsets buffer too small");
+        slot[1] = (const void*)bt.buf_addrs[k++];
+      }
+      int t = h->ptypes[c];
+      int64_t need_b = t == GZ_TYPE_BOOL ? (bt.num_rows + 7) / 8 : bt.num_rows * type_width(t);
+      if (t != GZ_TYPE_NA && !type_is_binary(t) && bt.buf_sizes[k] < need_b)
+        return fail(GZ_INVALID, "values buffer too small");
+      slot[2] = (const void*)bt.buf_addrs[k++];
+      if (h->ptypes[c] != GZ_TYPE_NA && bt.num_rows > 0 && !slot[2])
+        return fail(GZ_INVALID, "missing values buffer");
+    }
+    for (int64_t r0 = 0; r0 < bt.num_rows; r0 += TILE) {
+      PTile t;
+      t.batch = bi;
+      t.row_begin = (int32_t)r0;
+      t.nrows = (int32_t)std::min<int64_t>(TILE, bt.num_rows - r0);
+      t.pad = 0;
+      tiles.push_back(t);
+    }
+  }
+  const int64_t T = (int64_t)tiles.size();
+  ProbeParams p;
+  memset(&p, 0, sizeof(p));
+  p.T = T;
+  p.bstride = nc * 3;
+  p.key_col = h->key_col;
+  p.key_type = h->ptypes[h->key_col];
+  p.join_type = h->join_type;
+  p.null_aware = h->null_aware;
+  p.build_has_null = r->has_null;
+  p.build_keys = r->nkeys;
+  p.tile_rows = TILE;
+  p.J.keys = (int64_t*)r->keys.p;
+  p.J.start = (int32_t*)r->start.p;
+  p.J.count = (int32_t*)r->count.p;
+  p.J.rowlist = r->has_dups ? (int32_t*)r->rowlist.p : nullptr;
+  p.J.cap = r->cap;
+  p.J.special_start = r->special_start;
+  p.J.special_count = r->special_count;
+  std::vector<uint32_t> tc(std::max<int64_t>(T, 1), 0);
+  uint64_t total = 0;
+  if (T > 0) {
+    GZ_TRY(upload(ctx, &h->d_tiles, tiles.data(), sizeof(PTile) * T));
+    GZ_TRY(upload(ctx, &h->d_bufs, bufs.data(), sizeof(void*) * bufs.size()));
+    GZ_TRY(ctx_ensure(ctx, &h->match, sizeof(int32_t) * T * TILE));
+    GZ_TRY(ctx_ensure(ctx, &h->tile_count, sizeof(uint32_t) * T));
+    GZ_TRY(ctx_ensure(ctx, &h->tile_off, sizeof(uint64_t) * (T + 1)));
+    p.tiles = (const PTile*)h->d_tiles.p;
+    p.bufs = (const void* const*)h->d_bufs.p;
+    p.match = (int32_t*)h->match.p;
+    p.tile_count = (uint32_t*)h->tile_count.p;
+    p.tile_off = (const uint64_t*)h->tile_off.p;
+    int64_t grid = (T + 7) / 8 * 8;
+    {
+      KernelTimer kt(ctx, "join_probe_count");
+      hipLaunchKernelGGL(join_probe_count_kernel<256>, dim3((unsigned)grid), dim3(256), 0, st, p);
+    }
+    // tile offsets: exclusive scan (uint32 counts -> uint64 offsets)
+    size_t tmp = 0;
+    GZ_HIP(rocprim::exclusive_scan(nullptr, tmp, (const uint32_t*)h->tile_count.p, (uint64_t*)h->tile_off.p,
+                                   (uint64_t)0, (size_t)T + 0, rocprim::plus<uint64_t>(), st));
+    GZ_TRY(ctx_ensure(ctx, &h->scan_tmp, tmp));
+    GZ_HIP(rocprim::exclusive_scan(h->scan_tmp.p, tmp, (const uint32_t*)h->tile_count.p, (uint64_t*)h->tile_off.p,
+                                   (uint64_t)0, (size_t)T, rocprim::plus<uint64_t>(), st));
+    GZ_HIP(hipMemcpyAsync(tc.data(), h->tile_count.p, sizeof(uint32_t) * T, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    for (int64_t t = 0; t < T; t++) total += tc[t];
+  }
+  // outputs
+  bool any_binary = false;
+  const int nres = (int)h->sides.size();
+  p.ncols = nres;
+  for (int c = 0; c < nres; c++) {
+    auto& o = h->outs[c];
+    int t = o.type;
+    OutCol& oc = p.out[c];
+    oc.side = h->sides[c];
+    oc.col = h->idx[c];
+    oc.type = t;
+    oc.width = t == GZ_TYPE_BOOL ? 1 : type_width(t);
+    oc.binary = type_is_binary(t);
+    any_binary |= oc.binary;
+    GZ_TRY(ctx_ensure(ctx, &o.valid, std::max<uint64_t>(total, 1)));
+    if (oc.binary) {
+      GZ_TRY(ctx_ensure(ctx, &o.offsets, sizeof(int32_t) * (total + 1)));
+      GZ_TRY(ctx_ensure(ctx, &o.lens, sizeof(int64_t) * std::max<uint64_t>(total, 1)));
+    } else if (t != GZ_TYPE_NA) {
+      GZ_TRY(ctx_ensure(ctx, &o.values, (size_t)std::max<uint64_t>(total, 1) * oc.width));
+    }
+    oc.ovalid = (uint8_t*)o.valid.p;
+    oc.ovalues = (uint8_t*)o.values.p;
+    if (oc.side == 0) {
+      const ColStore& cs = r->cols[oc.col];
+      oc.bvalid = (const uint8_t*)cs.valid.p;
+      oc.bvalues = (const uint8_t*)cs.values.p;
+    }
+  }
+  if (any_binary) {
+    GZ_TRY(ctx_ensure(ctx, &h->pair_probe, sizeof(int64_t) * std::max<uint64_t>(total, 1)));
+    GZ_TRY(ctx_ensure(ctx, &h->pair_build, sizeof(int32_t) * std::max<uint64_t>(total, 1)));
+    p.pair_probe = (int64_t*)h->pair_probe.p;
+    p.pair_build = (int32_t*)h->pair_build.p;
+    p.write_pairs = 1;
+  }
+  if (T > 0 && total > 0) {
+    int64_t grid = (T + 7) / 8 * 8;
+    KernelTimer kt(ctx, "join_probe_write");
+    if (TILE == 4096)
+      hipLaunchKernelGGL((join_probe_write_kernel<4096, 256>), dim3((unsigned)grid), dim3(256), 0, st, p);
+    else
+      hipLaunchKernelGGL((join_probe_write_kernel<1024, 256>), dim3((unsigned)grid), dim3(256), 0, st, p);
+  }
+  if (any_binary && total > 0) {
+    for (int c = 0; c < nres; c++) {
+      if (!p.out[c].binary) continue;
+      auto& o = h->outs[c];
+      BinGather g;
+      memset(&g, 0, sizeof(g));
+      g.tiles = p.tiles;
+      g.bufs = p.bufs;
+      g.bstride = p.bstride;
+      g.side = p.out[c].side;
+      g.col = p.out[c].col;
+      g.pair_probe = p.pair_probe;
+      g.pair_build = p.pair_build;
+      g.n = (int64_t)total;
+      if (g.side == 0) {
+        const ColStore& cs = r->cols[g.col];
+        g.bvalid = (const uint8_t*)cs.valid.p;
+        g.boff = (const int64_t*)cs.offsets.p;
+        g.bdata = (const uint8_t*)cs.data.p;
+      }
+      g.ovalid = (uint8_t*)o.valid.p;
+      g.olen = (int64_t*)o.lens.p;
+      g.ooff = (int32_t*)o.offsets.p;
+      unsigned gb = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
+      hipLaunchKernelGGL(join_bin_len_kernel, dim3(gb), dim3(256), 0, st, g);
+      size_t tmp = 0;
+      GZ_HIP(rocprim::exclusive_scan(nullptr, tmp, (const int64_t*)o.lens.p, (int64_t*)o.lens.p, (int64_t)0,
+                                     (size_t)total, rocprim::plus<int64_t>(), st));
+      GZ_TRY(ctx_ensure(ctx, &h->scan_tmp, tmp));
+      // in-place scan is not allowed: scan into the pair buffer's space is not
+      // reusable either, so scan via a second buffer
+      DBuf scanned;
+      GZ_TRY(ctx_alloc(ctx, sizeof(int64_t) * total, &scanned));
+      GZ_HIP(rocprim::exclusive_scan(h->scan_tmp.p, tmp, (const int64_t*)o.lens.p, (int64_t*)scanned.p, (int64_t)0,
+                                     (size_t)total, rocprim::plus<int64_t>(), st));
+      // total bytes = last start + last length
+      int64_t last[2];
+      GZ_HIP(hipMemcpyAsync(&last[0], (int64_t*)scanned.p + total - 1, 8, hipMemcpyDeviceToHost, st));
+      GZ_HIP(hipMemcpyAsync(&last[1], (int64_t*)o.lens.p + total - 1, 8, hipMemcpyDeviceToHost, st));
+      GZ_HIP(hipStreamSynchronize(st));
+      int64_t bytes = last[0] + last[1];
+      if (bytes >= INT32_MAX) {
+        ctx_free(ctx, &scanned);
+        return fail(GZ_CAPACITY_ERROR, "utf8 output exceeds int32 offsets");
+      }
+      GZ_TRY(ctx_ensure(ctx, &o.data, (size_t)std::max<int64_t>(bytes, 1)));
+      g.odata = (uint8_t*)o.data.p;
+      g.olen = (int64_t*)scanned.p;
+      hipLaunchKernelGGL(join_bin_copy_kernel, dim3(gb), dim3(256), 0, st, g);
+      GZ_HIP(hipStreamSynchronize(st));
+      ctx_free(ctx, &scanned);
+    }
+  } else if (any_binary) {
+    for (int c = 0; c < nres; c++)
+      if (p.out[c].binary) GZ_HIP(hipMemsetAsync(h->outs[c].offsets.p, 0, 4, st));
+  }
+  GZ_HIP(hipGetLastError());
+  h->out_rows = (int64_t)total;
+  if (out_rows) *out_rows = (int64_t)total;
+  if (out_batch_rows) {
+    for (int bi = 0; bi < nb; bi++) out_batch_rows[bi] = 0;
+    for (int64_t t = 0; t < T; t++) out_batch_rows[tiles[t].batch] += tc[t];
+  }
+  return GZ_OK;
+}
+
+extern "C" {
+
+int gz_probe_make(gz_relation* r, int32_t num_probe_cols, const int32_t* probe_types, int32_t probe_key_col,
+                  int32_t join_type, int32_t null_aware_anti, int32_t num_result_cols,
+                  const int32_t* result_sides, const int32_t* result_indices, gz_probe** out) {
+  if (!r || !out || (num_probe_cols > 0 && !probe_types)) return fail(GZ_INVALID, "gz_probe_make: null argument");
+  if (!r->finished) return fail(GZ_INVALID, "the hash relation must be finished before probing");
+  if (join_type != GZ_JOIN_INNER && join_type != GZ_JOIN_SEMI && join_type != GZ_JOIN_ANTI)
+    return fail(GZ_NOT_IMPLEMENTED, "ConditionedProbeArraysTypedImpl only support join type: InnerJoin, "
+                                    "SemiJoin, AntiJoin");
+  if (num_probe_cols > kJoinMaxCols || num_result_cols > kJoinMaxCols)
+    return fail(GZ_NOT_IMPLEMENTED, "more than 32 columns");
+  if (probe_key_col < 0 || probe_key_col >= num_probe_cols) return fail(GZ_INVALID, "probe key out of range");
+  int pk = probe_types[probe_key_col], bk = r->types[r->key_col];
+  if (pk != bk) return fail(GZ_TYPE_ERROR, "probe and build key types differ");
+  auto* h = new gz_probe();
+  h->rel = r;
+  h->nprobe = num_probe_cols;
+  h->ptypes.assign(probe_types, probe_types + num_probe_cols);
+  h->key_col = probe_key_col;
+  h->join_type = join_type;
+  h->null_aware = null_aware_anti;
+  for (int c = 0; c < num_result_cols; c++) {
+    int s = result_sides[c], i = result_indices[c];
+    if ((s == 0 && (i < 0 || i >= r->ncols)) || (s == 1 && (i < 0 || i >= num_probe_cols)) || (s != 0 && s != 1)) {
+      delete h;
+      return fail(GZ_INVALID, "result column out of range");
+    }
+    h->sides.push_back(s);
+    h->idx.push_back(i);
+    gz_probe::Out o;
+    o.type = s == 0 ? r->types[i] : probe_types[i];
+    o.width = type_width(o.type);
+    if (o.type == GZ_TYPE_DECIMAL128 && s == 0) {
+      // 16-byte build values are gathered as bytes
+    }
+    h->outs.push_back(o);
+  }
+  *out = h;
+  return GZ_OK;
+}
+
+int gz_probe_process(gz_probe* h, int64_t num_rows, const int64_t* buf_addrs, const int64_t* buf_sizes,
+                     int32_t num_bufs, int64_t* out_rows) {
+  if (!h) return fail(GZ_INVALID, "null probe");
+  if (!buf_addrs || !buf_sizes) return fail(GZ_INVALID, "buf_addrs can't be null");
+  gz_batch b{num_rows, buf_addrs, buf_sizes, num_bufs};
+  return probe_run(h, 1, &b, out_rows, nullptr);
+}
+
+int gz_probe_process_batches(gz_probe* h, int32_t num_batches, const gz_batch* batches, int64_t* out_rows,
+                             int64_t* out_batch_rows) {
+  if (!h) return fail(GZ_INVALID, "null probe");
+  if (num_batches < 0 || (num_batches > 0 && !batches)) return fail(GZ_INVALID, "bad batch list");
+  return probe_run(h, num_batches, batches, out_rows, out_batch_rows);
+}
+
+int gz_probe_result_column(gz_probe* h, int32_t col, void** values, void** valid_bytes, void** offsets,
+                           void** data) {
+  if (!h) return fail(GZ_INVALID, "null probe");
+  if (col < 0 || col >= (int)h->outs.size()) return fail(GZ_INDEX_ERROR, "result column out of range");
+  auto& o = h->outs[col];
+  if (values) *values = o.values.p;
+  if (valid_bytes) *valid_bytes = o.valid.p;
+  if (offsets) *offsets = o.offsets.p;
+  if (data) *data = o.data.p;
+  return GZ_OK;
+}
+
+int gz_probe_release(gz_probe* h) {
+  if (!h) return GZ_OK;
+  gz_context* ctx = h->rel->ctx;
+  for (auto& o : h->outs)
+    for (DBuf* b : {&o.values, &o.valid, &o.offsets, &o.data, &o.lens}) ctx_free(ctx, b);
+  for (DBuf* b : {&h->d_tiles, &h->d_bufs, &h->match, &h->tile_count, &h->tile_off, &h->scan_tmp,
+                  &h->pair_probe, &h->pair_build})
+    ctx_free(ctx, b);
+  delete h;
+  return GZ_OK;
+}
+
+}  // extern "C"

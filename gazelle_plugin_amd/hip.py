@@ -12,6 +12,9 @@ H2D, D2H, D2D = 1, 2, 3
 def hip():
     global _hip
     if _hip is None:
+        from ._lib import _bind_process_hip_runtime
+
+        _bind_process_hip_runtime()
         for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
             try:
                 _hip = ctypes.CDLL(name)

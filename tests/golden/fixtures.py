@@ -49,7 +49,7 @@ def column(name: str, values) -> HostColumn:
             raw[2 * i + 1] = -1 if u < 0 else 0
         return HostColumn(t, raw.view(np.uint8), vb)
     dt = T.numpy_dtype(t)
-    arr = np.array([0 if v is None else v for v in values]).astype(dt)
+    arr = np.array([0 if v is None else (float("nan") if v == "NaN" else v) for v in values]).astype(dt)
     return HostColumn(t, arr, vb)
 
 

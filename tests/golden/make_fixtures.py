@@ -82,10 +82,77 @@ SPLIT_FIXTURES = {
 }
 
 
+# --------------------------------------------------------------------------- aggregate
+# native-sql-engine/cpp/src/tests/arrow_compute_test_aggregate.cc.  Doubles that
+# the tests print rounded (6 significant digits) carry "tolerance".
+NAN = "NaN"
+AGG_FIXTURES = {"tests": [
+    {   # MinMaxNaNTest, :222-283 (global min/max with NaN check)
+        "name": "MinMaxNaNTest",
+        "schema": [["f0", "double"]],
+        "actions": [["action_min_true", [0]], ["action_max_true", [0]]],
+        "batches": [
+            [[1, 2, 3, 4, 5, NAN, 6, 7, 8, 9, 10, 16, 19, 42, 78, 12, 5, NAN, 11, 19]],
+            [[16, 7, 8, 9, 10, 10, 9, 6, 7, 7, 6, 6, 6, 9, 9, 8, 18, 19, 12, 13]],
+        ],
+        "expected": [[1], [NAN]],
+    },
+    {   # GroupByMinMaxNaNTest, :284-364
+        "name": "GroupByMinMaxNaNTest",
+        "schema": [["f0", "int64"], ["f1", "double"]],
+        "actions": [["action_groupby", [0]], ["action_min_true", [1]], ["action_max_true", [1]]],
+        "batches": [
+            [[1, 2, 3, 4, 5, None, 4, 1, 2, 2, 1, 1, 1, 4, 4, 3, 5, 5, 5, 5],
+             [1, 2, 3, 4, 5, NAN, 6, 7, 8, 9, 10, 16, 19, 42, 78, 12, 5, NAN, 11, 19]],
+            [[6, 7, 8, 9, 10, 10, 9, 6, 7, 7, 6, 6, 6, 9, 9, 8, 10, 10, 10, 10],
+             [16, 7, 8, 9, 10, NAN, 9, 6, 7, 7, 6, 6, 6, 9, 9, 8, 18, 19, 12, 13]],
+        ],
+        "expected": [[1, 2, 3, 4, 5, None, 6, 7, 8, 9, 10],
+                     [1, 2, 3, 4, 5, NAN, 6, 7, 8, 9, 10],
+                     [19, 9, 12, 78, NAN, NAN, 16, 7, 8, 9, NAN]],
+    },
+    {   # AggregateAllNullTest, :365-421
+        "name": "AggregateAllNullTest",
+        "schema": [["f0", "int32"]],
+        "actions": [["action_sum", [0]]],
+        "batches": [[[None] * 12], [[None] * 12]],
+        "expected": [[None]],
+    },
+    {   # GroupByAggregateTest, :422-547: the in-scope actions; the test's result
+        # expression multiplies sum by count ("sum_count_multiply")
+        "name": "GroupByAggregateTest",
+        "schema": [["f0", "int64"], ["f1", "uint32"], ["f2", "double"], ["f3", "double"], ["f4", "double"]],
+        "actions": [["action_groupby", [0]], ["action_sum_count", [1]], ["action_min", [1]],
+                    ["action_max", [1]], ["action_avgByCount", [2, 0]]],
+        "batches": [
+            [[1, 2, 3, 4, 5, None, 4, 1, 2, 2, 1, 1, 1, 4, 4, 3, 5, 5, 5, 5],
+             [1, 2, 3, 4, 5, 5, 4, 1, 2, 2, 1, 1, 1, 4, 4, 3, 5, 5, 5, 5],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1]],
+            [[6, 7, 8, 9, 10, 10, 9, 6, 7, 7, 6, 6, 6, 9, 9, 8, 10, 10, 10, 10],
+             [6, 7, 8, 9, 10, 10, 9, 6, 7, 7, 6, 6, 6, 9, 9, 8, 10, 10, 10, 10],
+             [7, 8, 4, 5, 6, 1, 34, 54, 65, 66, 78, 12, 32, 24, 32, 45, 12, 24, 35, 46],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1]],
+        ],
+        "expected_named": {
+            "unique": [1, 2, 3, 4, 5, None, 6, 7, 8, 9, 10],
+            "sum_count_multiply": [25, 18, 12, 64, 125, 5, 150, 63, 32, 144, 360],
+            "min": [1, 2, 3, 4, 5, 5, 6, 7, 8, 9, 10],
+            "avg": [16.4, 6.5, 5, 5.875, 5.48, None, 6.1, 6.61905, 3.0625, 2.63889, 2.06667],
+        },
+        "tolerance": 1e-5,
+    },
+]}
+
+
 def main():
     with open(os.path.join(HERE, "split_fixtures.json"), "w") as f:
         json.dump(SPLIT_FIXTURES, f, indent=1)
-    print("wrote split_fixtures.json")
+    with open(os.path.join(HERE, "agg_fixtures.json"), "w") as f:
+        json.dump(AGG_FIXTURES, f, indent=1)
+    print("wrote split_fixtures.json, agg_fixtures.json")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,125 @@
+"""Parity of the HIP hash aggregate with the oracle through the C ABI: integer
+results bit-exact, FP64 sums/averages within 1e-12 relative, groups in the
+reference's first-seen order."""
+import numpy as np
+import pytest
+
+import oracle
+from gazelle_plugin_amd import datagen
+from gazelle_plugin_amd import types as T
+from gazelle_plugin_amd.aggregate import AggOptions, HashAggregateKernel, parse_action
+from gazelle_plugin_amd.batch import DeviceBatch, HostBatch, HostColumn
+from golden import fixtures as F
+from aggcheck import CONFIG0_ACTIONS, assert_results_equal
+from splitcheck import random_batches
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def run_gpu(ctx, batches, types, actions, batched=True, **opts):
+    k = HashAggregateKernel.Make(ctx, types, actions, AggOptions(**opts))
+    it = k.MakeResultIterator()
+    dev = [DeviceBatch.from_host(b) for b in batches]
+    if batched:
+        it.ProcessBatches(dev)
+    else:
+        for d in dev:
+            it.ProcessAndCacheOne(d)
+    res = it.results()
+    it.close()
+    return res
+
+
+def expected(batches, types, actions):
+    return oracle.aggregate(batches, types, [parse_action(n, i) for n, i in actions])
+
+
+@pytest.mark.parametrize("batched", [True, False])
+@pytest.mark.parametrize("placement", [0, 1, 2])
+def test_config0_query_small(ctx, batched, placement):
+    batches = datagen.agg_host(50000, num_keys=10000)
+    got = run_gpu(ctx, batches, datagen.AGG_TYPES, CONFIG0_ACTIONS, batched, placement=placement)
+    assert_results_equal(got, expected(batches, datagen.AGG_TYPES, CONFIG0_ACTIONS), RTOL)
+
+
+@pytest.mark.parametrize("num_keys", [1, 7, 200, 3000, 60000])
+def test_cardinalities(ctx, num_keys):
+    batches = datagen.agg_host(70000, num_keys=num_keys, batch_rows=9000)
+    got = run_gpu(ctx, batches, datagen.AGG_TYPES, CONFIG0_ACTIONS)
+    assert_results_equal(got, expected(batches, datagen.AGG_TYPES, CONFIG0_ACTIONS), RTOL)
+
+
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
+def test_reference_fixtures_on_gpu(ctx, case):
+    fx = F.load("agg_fixtures.json")["tests"][case]
+    schema = fx["schema"]
+    batches = [F.batch(schema, cols) for cols in fx["batches"]]
+    types = [F.type_id(t) for _, t in schema]
+    for batched in (True, False):
+        got = run_gpu(ctx, batches, types, fx["actions"], batched)
+        assert_results_equal(got, expected(batches, types, fx["actions"]), RTOL)
+
+
+def test_nulls_key_types_and_all_actions(ctx):
+    types = [T.INT32, T.DOUBLE, T.INT64, T.UINT64, T.FLOAT, T.INT16]
+    batches = random_batches(types, [3000, 1, 0, 2500], seed=5, null_frac=0.2)
+    for b in batches:  # a small key domain with nulls
+        if b.num_rows:
+            b.columns[0].values = (b.columns[0].values % 37).astype(np.int32)
+            b.columns[2].values = (b.columns[2].values % 100000)
+            b.columns[1].values[::7] = np.nan
+    actions = [("action_groupby", [0]), ("action_sum", [1]), ("action_avg", [1]), ("action_count", [1]),
+               ("action_countLiteral_1", []), ("action_min_true", [1]), ("action_max_true", [1]),
+               ("action_min", [1]), ("action_max", [1]), ("action_sum", [2]), ("action_min", [3]),
+               ("action_max", [3]), ("action_sum_count", [4]), ("action_min", [5]), ("action_max", [2]),
+               ("action_avgByCount", [1, 2])]
+    exp = expected(batches, types, actions)
+    for placement in (1, 2):
+        got = run_gpu(ctx, batches, types, actions, placement=placement)
+        assert_results_equal(got, exp, RTOL)
+
+
+def test_global_aggregate(ctx):
+    batches = datagen.agg_host(20000, num_keys=10)
+    actions = [("action_sum", [1]), ("action_avg", [1]), ("action_countLiteral_1", []),
+               ("action_min", [2]), ("action_max", [2])]
+    got = run_gpu(ctx, batches, datagen.AGG_TYPES, actions)
+    assert_results_equal(got, expected(batches, datagen.AGG_TYPES, actions), RTOL)
+
+
+def test_next_batches_of_ten_thousand(ctx):
+    batches = datagen.agg_host(60000, num_keys=25000)
+    k = HashAggregateKernel.Make(ctx, datagen.AGG_TYPES, CONFIG0_ACTIONS)
+    it = k.MakeResultIterator()
+    it.ProcessBatches([DeviceBatch.from_host(b) for b in batches])
+    sizes = []
+    while it.HasNext():
+        sizes.append(len(it.Next()[0][1]))
+    it.close()
+    n = len(expected(batches, datagen.AGG_TYPES, CONFIG0_ACTIONS)[0][1])
+    assert sum(sizes) == n and all(s == 10000 for s in sizes[:-1])
+
+
+def test_aggregate_set_instances_independent(ctx):
+    """Many independent GROUP BYs in one launch equal each run alone."""
+    from gazelle_plugin_amd.aggregate import HashAggregateResultIterator
+
+    rng = np.random.default_rng(3)
+    inst_batches = []
+    for i in range(12):
+        rows = int(rng.integers(1000, 30000))
+        keys = int(rng.integers(5, 20000))
+        inst_batches.append(datagen.agg_host(rows, num_keys=keys, batch_rows=4096, seed=100 + i))
+    k = HashAggregateKernel.Make(ctx, datagen.AGG_TYPES, CONFIG0_ACTIONS)
+    it = HashAggregateResultIterator(k, num_instances=len(inst_batches))
+    dev, inst_of = [], []
+    for i, bl in enumerate(inst_batches):
+        for b in bl:
+            dev.append(DeviceBatch.from_host(b))
+            inst_of.append(i)
+    it.ProcessInstances(dev, inst_of)
+    for i, bl in enumerate(inst_batches):
+        assert_results_equal(it.results(i), expected(bl, datagen.AGG_TYPES, CONFIG0_ACTIONS), RTOL)
+    it.close()

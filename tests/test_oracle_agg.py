@@ -1,0 +1,106 @@
+"""Pins of the oracle's hash aggregate: the reference's own aggregate tests
+(arrow_compute_test_aggregate.cc), brute force, and pyarrow's group_by."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from gazelle_plugin_amd import datagen
+from gazelle_plugin_amd import types as T
+from gazelle_plugin_amd.aggregate import parse_action
+from golden import fixtures as F
+from aggcheck import CONFIG0_ACTIONS, python_groupby
+
+
+def _same(a, b, tol):
+    if b is None:
+        return a is None
+    if isinstance(b, str) and b == "NaN":
+        return a is not None and math.isnan(a)
+    if a is None or (isinstance(a, float) and math.isnan(a)):
+        return False
+    return abs(a - b) <= tol * max(1.0, abs(b))
+
+
+def _column(res, i):
+    t, v, ok = res[i]
+    return [v[j].item() if ok[j] else None for j in range(len(v))]
+
+
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
+def test_reference_aggregate_fixtures(case):
+    fx = F.load("agg_fixtures.json")["tests"][case]
+    schema = fx["schema"]
+    batches = [F.batch(schema, cols) for cols in fx["batches"]]
+    types = [F.type_id(t) for _, t in schema]
+    actions = [parse_action(n, i) for n, i in fx["actions"]]
+    res = oracle.aggregate(batches, types, actions)
+    tol = fx.get("tolerance", 0.0)
+    if "expected" in fx:
+        for i, exp in enumerate(fx["expected"]):
+            got = _column(res, i)
+            assert len(got) == len(exp)
+            assert all(_same(g, e, tol) for g, e in zip(got, exp)), (fx["name"], i, got, exp)
+    else:
+        e = fx["expected_named"]
+        unique, s, cnt, mn, mx, avg = (_column(res, i) for i in range(6))
+        assert unique == e["unique"]
+        prod = [None if a is None else a * b for a, b in zip(s, cnt)]
+        assert all(_same(g, x, 0) for g, x in zip(prod, e["sum_count_multiply"]))
+        assert mn == e["min"]
+        assert all(_same(g, x, tol) for g, x in zip(avg, e["avg"]))
+
+
+def test_config0_query_brute_force():
+    batches = datagen.agg_host(3000, num_keys=50, batch_rows=700)
+    actions = [parse_action(n, i) for n, i in CONFIG0_ACTIONS]
+    res = oracle.aggregate(batches, datagen.AGG_TYPES, actions)
+    groups = python_groupby(batches, 0, [1, 2])
+    keys = list(groups.keys())
+    assert _column(res, 0) == keys  # first-seen order
+    for j, k in enumerate(keys):
+        price, qty = groups[k]
+        pv = [p for p in price if p is not None]
+        assert math.isclose(res[1][1][j], sum(pv), rel_tol=1e-12)
+        assert math.isclose(res[2][1][j], sum(pv) / len(pv), rel_tol=1e-12)
+        assert res[3][1][j] == len(price)
+        assert res[4][1][j] == min(qty) and res[5][1][j] == max(qty)
+        assert res[6][1][j] == sum(qty)
+
+
+def test_config0_query_matches_pyarrow():
+    pa = pytest.importorskip("pyarrow")
+    import pyarrow.compute as pc
+    from gazelle_plugin_amd.batch import to_arrow
+
+    batches = datagen.agg_host(40000, num_keys=1000)
+    actions = [parse_action(n, i) for n, i in CONFIG0_ACTIONS]
+    res = oracle.aggregate(batches, datagen.AGG_TYPES, actions)
+    tab = pa.Table.from_batches([to_arrow(b) for b in batches]).rename_columns(["k", "p", "q"])
+    agg = tab.group_by("k", use_threads=False).aggregate(
+        [("p", "sum"), ("p", "mean"), ("k", "count", pc.CountOptions("all")), ("q", "min"),
+         ("q", "max"), ("q", "sum")]).sort_by("k")
+    order = np.argsort(res[0][1], kind="stable")
+    np.testing.assert_array_equal(res[0][1][order], agg["k"].to_numpy())
+    np.testing.assert_allclose(res[1][1][order], agg["p_sum"].to_numpy(), rtol=1e-12)
+    np.testing.assert_allclose(res[2][1][order], agg["p_mean"].to_numpy(), rtol=1e-12)
+    np.testing.assert_array_equal(res[3][1][order], agg["k_count"].to_numpy())
+    np.testing.assert_array_equal(res[4][1][order], agg["q_min"].to_numpy())
+    np.testing.assert_array_equal(res[5][1][order], agg["q_max"].to_numpy())
+    np.testing.assert_array_equal(res[6][1][order], agg["q_sum"].to_numpy())
+
+
+def test_min_max_without_nan_check_follow_first_value():
+    """action_min / action_max without the NaN check compare with < / >, so a
+    leading NaN sticks and later NaNs are ignored (MinAction :1000-1016)."""
+    from gazelle_plugin_amd.batch import HostBatch, HostColumn
+
+    k = np.array([1, 1, 1, 2, 2, 2], np.int64)
+    v = np.array([np.nan, 3.0, 1.0, 5.0, np.nan, 2.0])
+    b = HostBatch(6, [HostColumn(T.INT64, k), HostColumn(T.DOUBLE, v)])
+    res = oracle.aggregate([b], [T.INT64, T.DOUBLE],
+                           [parse_action("action_groupby", [0]), parse_action("action_min", [1]),
+                            parse_action("action_max", [1])])
+    assert math.isnan(res[1][1][0]) and math.isnan(res[2][1][0])
+    assert res[1][1][1] == 2.0 and res[2][1][1] == 5.0
