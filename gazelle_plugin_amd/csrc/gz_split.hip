@@ -689,6 +689,8 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   std::vector<size_t> data_bytes(h->base.nbin, 0);
   std::vector<bool> has_validity(nc, false);
   std::vector<int32_t> tile_batch;
+  std::vector<const int32_t*> empty_data;
+  std::vector<int64_t> empty_rows;
   int64_t rr = h->rr_state;
   for (int bi = 0; bi < nb; bi++) {
     const gz_batch& bt = batches[bi];
@@ -717,6 +719,12 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
       int64_t vsz = bt.buf_sizes[k];
       k++;
       if (t == GZ_TYPE_NA) continue;
+      if (type_is_binary(t) && bt.num_rows > 0 && (!slot[2] || vsz == 0)) {
+        // an empty data buffer is valid when every value is empty: checked below
+        empty_data.push_back((const int32_t*)slot[1]);
+        empty_rows.push_back(bt.num_rows);
+        slot[2] = slot[1];  // any valid address; no byte is read from it
+      }
       if (bt.num_rows > 0 && !slot[2]) return fail(GZ_INVALID, "missing values buffer");
       if (type_is_binary(t)) data_bytes[h->base.bin[c]] += (size_t)vsz;
       else if (t == GZ_TYPE_BOOL) {
@@ -736,6 +744,17 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     N += bt.num_rows;
   }
   if (N > INT32_MAX) return fail(GZ_CAPACITY_ERROR, "more than 2^31 rows in one split call");
+  if (!empty_data.empty()) {
+    std::vector<int32_t> ends(2 * empty_data.size());
+    for (size_t i = 0; i < empty_data.size(); i++) {
+      GZ_HIP(hipMemcpyAsync(&ends[2 * i], empty_data[i], 4, hipMemcpyDeviceToHost, ctx->stream));
+      GZ_HIP(hipMemcpyAsync(&ends[2 * i + 1], empty_data[i] + empty_rows[i], 4, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    }
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < empty_data.size(); i++)
+      if (ends[2 * i] != ends[2 * i + 1]) return fail(GZ_INVALID, "missing utf8 data buffer");
+  }
   h->rr_state = (int32_t)rr;
   auto* seg = new SplitSegment();
   seg->N = N;
@@ -780,7 +799,6 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
       if ((rc = ctx_alloc(ctx, nwords * 4, &seg->values[oc]))) return bail(rc);
     } else if (type_is_binary(t)) {
       if ((rc = ctx_alloc(ctx, sizeof(int32_t) * (N + P), &seg->offsets[oc]))) return bail(rc);
-      if ((rc = ctx_alloc(ctx, std::max<size_t>(data_bytes[prm.bin[c]], 4), &seg->data[oc]))) return bail(rc);
     } else {
       if ((rc = ctx_alloc(ctx, (size_t)std::max<int64_t>(N, 1) * type_width(t), &seg->values[oc]))) return bail(rc);
     }
@@ -843,6 +861,23 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     hipLaunchKernelGGL(split_layout_kernel, dim3(1), dim3(256), 0, st, (const uint64_t*)seg->tot.p,
                        P, nbin, (int64_t*)seg->row_start.p, (int64_t*)seg->byte_start.p,
                        (int32_t**)seg->offtab.p, (int*)seg->err.p);
+  }
+  // utf8 data outputs: the sum of the input data buffers bounds them; when that
+  // bound is loose (buffers shared by many batches), size them exactly from the
+  // scanned byte totals instead.
+  for (int c = 0; c < nc; c++) {
+    int oc = prm.out_col[c];
+    if (oc < 0 || !type_is_binary(h->types_in[c])) continue;
+    size_t bytes = data_bytes[prm.bin[c]];
+    if (bytes > ((size_t)1 << 30)) {
+      int64_t total = 0;
+      GZ_HIP(hipMemcpyAsync(&total, (int64_t*)seg->byte_start.p + (int64_t)prm.bin[c] * (P + 1) + P,
+                            sizeof(int64_t), hipMemcpyDeviceToHost, st));
+      GZ_HIP(hipStreamSynchronize(st));
+      bytes = (size_t)total;
+    }
+    if ((rc = ctx_alloc(ctx, std::max<size_t>(bytes, 4), &seg->data[oc]))) return bail(rc);
+    prm.out_data[oc] = (uint8_t*)seg->data[oc].p;
   }
   if (T > 0) {
     KernelTimer kt(ctx, "split_scatter");

@@ -95,21 +95,27 @@ def split_device(num_rows: int, batch_rows: int = BATCH_ROWS, seed: int = 42,
     offs[1:] = torch.cumsum(lens, 0)
     total = int(offs[-1].item())
     assert total < (1 << 31), "utf8 data must fit int32 offsets"
-    offs32 = offs.to(torch.int32)
+    nb = num_rows // batch_rows
+    # per-batch offsets starting at 0, as separate Arrow batches carry them
+    idx = (torch.arange(nb, device=device)[:, None] * batch_rows +
+           torch.arange(batch_rows + 1, device=device)[None, :])
+    starts = offs[:-1:batch_rows]
+    boffs = (offs[idx] - starts[:, None]).to(torch.int32).contiguous()
+    bstart = starts.cpu().tolist()
+    bend = offs[batch_rows::batch_rows].cpu().tolist()
     chars = torch.randint(97, 123, (total,), dtype=torch.uint8, device=device, generator=g)
-    del offs, lens
-    kb, ib, fb, vb, ob, cb = (t.view(torch.uint8) for t in (key, i32, f64, vbits, offs32, chars))
+    del offs, lens, idx
+    kb, ib, fb, vb, cb = (t.view(torch.uint8) for t in (key, i32, f64, vbits, chars))
     batches = []
-    for b0 in range(0, num_rows, batch_rows):
+    for bi, b0 in enumerate(range(0, num_rows, batch_rows)):
         n = batch_rows
-        # offsets are absolute into the shared data buffer (the base of every batch)
         bufs = [None, kb[b0 * 8:(b0 + n) * 8],
                 None, ib[b0 * 4:(b0 + n) * 4],
                 vb[b0 // 8:(b0 + n) // 8], fb[b0 * 8:(b0 + n) * 8],
-                None, ob[b0 * 4:(b0 + n + 1) * 4], cb]
+                None, boffs[bi].view(torch.uint8), cb[bstart[bi]:bend[bi]]]
         batches.append(DeviceBatch(n, SPLIT_TYPES, bufs))
-    nbytes = num_rows * (8 + 4 + 8 + 4) + num_rows // 8 + total
-    return DeviceTable((key, i32, f64, vbits, offs32, chars), batches, num_rows, nbytes)
+    nbytes = num_rows * (8 + 4 + 8) + nb * (batch_rows + 1) * 4 + num_rows // 8 + total
+    return DeviceTable((key, i32, f64, vbits, boffs, chars), batches, num_rows, nbytes)
 
 
 # ------------------------------------------------------------------ aggregate workload
