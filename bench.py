@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also run a device copy of this many MiB (PMC calibration)")
     return ap.parse_args()
 
 
@@ -162,6 +164,14 @@ def main():
     roof["algorithmic_bytes"] = algo_bytes
     roof["kernel_ms"] = scat_avg
 
+    if args.calibrate_copy:
+        # a plain copy of known size: the PMC FETCH_SIZE / WRITE_SIZE it reads
+        # scale the split kernel's counters to bytes
+        src = torch.empty(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     cpu = None if args.no_cpu else cpu_baseline_split(args.cpu_seconds)
     out = {
         "metric": "rows/s + HBM GB/s (% peak), batched vs plain GPU path; p50/p99 fragment latency",

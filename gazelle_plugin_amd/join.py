@@ -1,0 +1,148 @@
+"""Python mirror of the reference hash join over the C ABI.
+
+``HashRelationKernel`` mirrors the build (ext/hash_relation_kernel.cc:50-344):
+``Evaluate`` caches a build batch, ``Finish`` builds the table (FinishInternal).
+``ConditionedProbeKernel`` mirrors the probe (ext/conditioned_probe_kernel.cc):
+``Make(relation, ..., join_type, is_null_aware_anti_join, result)``, then
+``Process(batch)`` returns the output record batch of that probe batch, rows in
+probe order with each probe row's matches in build insertion order.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib as L
+from . import types as T
+from .batch import DeviceBatch
+from .context import Context
+from .hip import to_host
+from .splitter import make_batch_array
+
+JOIN_TYPES = {"inner": 0, "anti": 2, "semi": 3}
+
+_NP = {T.UINT8: np.uint8, T.INT8: np.int8, T.UINT16: np.uint16, T.INT16: np.int16,
+       T.UINT32: np.uint32, T.INT32: np.int32, T.UINT64: np.uint64, T.INT64: np.int64,
+       T.FLOAT: np.float32, T.DOUBLE: np.float64, T.DATE32: np.int32, T.DATE64: np.int64,
+       T.TIMESTAMP: np.int64, T.BOOL: np.uint8}
+
+
+class HashRelation:
+    """The build side: a resident table in HBM (an opaque handle)."""
+
+    def __init__(self, ctx: Context, types: Sequence[int], key_col: int, builder_type: int = 1):
+        self.ctx = ctx
+        self.types = list(types)
+        self.key_col = key_col
+        self._h = ctypes.c_void_p()
+        L.check(L.lib().gz_relation_make(ctx.handle, len(types), L.i32_array(types), key_col, builder_type,
+                                         ctypes.byref(self._h)))
+        self.finished = False
+
+    def Evaluate(self, batch: DeviceBatch):
+        L.check(L.lib().gz_relation_append(self._h, batch.num_rows, batch.addrs, batch.sizes, batch.num_bufs))
+
+    def EvaluateBatches(self, batches: Sequence[DeviceBatch]):
+        L.check(L.lib().gz_relation_append_batches(self._h, len(batches), make_batch_array(batches)))
+
+    def Finish(self):
+        L.check(L.lib().gz_relation_finish(self._h))
+        self.finished = True
+        return self
+
+    def num_keys(self) -> int:
+        n = ctypes.c_int64()
+        L.check(L.lib().gz_relation_num_keys(self._h, ctypes.byref(n)))
+        return n.value
+
+    def has_null_key(self) -> bool:
+        n = ctypes.c_int32()
+        L.check(L.lib().gz_relation_has_null_key(self._h, ctypes.byref(n)))
+        return bool(n.value)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            L.lib().gz_relation_release(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HashRelationKernel:
+    @staticmethod
+    def Make(ctx: Context, types: Sequence[int], key_col: int, builder_type: int = 1) -> HashRelation:
+        return HashRelation(ctx, types, key_col, builder_type)
+
+
+class ConditionedProbeKernel:
+    def __init__(self, relation: HashRelation, probe_types: Sequence[int], probe_key: int, join_type: str,
+                 null_aware: bool, result: Sequence[Tuple[int, int]]):
+        self.rel = relation
+        self.probe_types = list(probe_types)
+        self.result = [tuple(r) for r in result]
+        self.result_types = [relation.types[i] if s == 0 else self.probe_types[i] for s, i in self.result]
+        self._h = ctypes.c_void_p()
+        sides = [s for s, _ in self.result]
+        idx = [i for _, i in self.result]
+        L.check(L.lib().gz_probe_make(relation.handle, len(probe_types), L.i32_array(probe_types), probe_key,
+                                      JOIN_TYPES[join_type], int(null_aware), len(self.result),
+                                      L.i32_array(sides), L.i32_array(idx), ctypes.byref(self._h)))
+
+    @classmethod
+    def Make(cls, relation, probe_types, probe_key, join_type="inner", null_aware=False, result=()):
+        return cls(relation, probe_types, probe_key, join_type, null_aware, result)
+
+    def Process(self, batch: DeviceBatch) -> int:
+        n = ctypes.c_int64()
+        L.check(L.lib().gz_probe_process(self._h, batch.num_rows, batch.addrs, batch.sizes, batch.num_bufs,
+                                         ctypes.byref(n)))
+        return n.value
+
+    def ProcessBatches(self, batches: Sequence[DeviceBatch], arr=None) -> Tuple[int, np.ndarray]:
+        if arr is None:
+            arr = make_batch_array(batches)
+        n = ctypes.c_int64()
+        per = np.zeros(max(1, len(batches)), np.int64)
+        L.check(L.lib().gz_probe_process_batches(self._h, len(batches), arr, ctypes.byref(n),
+                                                 per.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return n.value, per[: len(batches)]
+
+    def result_columns(self, n: int) -> List:
+        """Host copy of the last output: per column (type, values, valid[, offsets, data])."""
+        out = []
+        for c, t in enumerate(self.result_types):
+            vals, valid, offs, data = (ctypes.c_void_p() for _ in range(4))
+            L.check(L.lib().gz_probe_result_column(self._h, c, ctypes.byref(vals), ctypes.byref(valid),
+                                                   ctypes.byref(offs), ctypes.byref(data)))
+            ok = to_host(valid.value, n, np.uint8).astype(bool) if n else np.zeros(0, bool)
+            if T.is_binary(t):
+                o = to_host(offs.value, 4 * (n + 1), np.int32)
+                d = to_host(data.value, int(o[-1])) if n and o[-1] > 0 else np.zeros(0, np.uint8)
+                out.append((t, [bytes(d[o[i]:o[i + 1]]) for i in range(n)], ok))
+            elif t == T.NA:
+                out.append((t, [None] * n, ok))
+            else:
+                dt = np.dtype(_NP[t]) if t != T.DECIMAL128 else np.dtype("V16")
+                out.append((t, to_host(vals.value, n * dt.itemsize, dt) if n else np.zeros(0, dt), ok))
+        return out
+
+    def close(self):
+        if self._h:
+            L.lib().gz_probe_release(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

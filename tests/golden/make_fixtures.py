@@ -147,12 +147,70 @@ AGG_FIXTURES = {"tests": [
 ]}
 
 
+# --------------------------------------------------------------------------- join
+# native-sql-engine/cpp/src/tests/arrow_compute_test_join_wocg.cc.  Results list
+# the output columns per probe batch; the result schema holds probe ("right")
+# columns, as each test's "result" node does.
+_BUILD = [[[10, 3, 1, 2, 3, 1], [10, 3, 1, 2, 13, 11], [10, 3, 1, 2, 13, 11]],
+          [[6, 12, 5, 8, 6, 10], [6, 12, 5, 8, 16, 110], [6, 12, 5, 8, 16, 110]]]
+_PROBE_STR = [[[1, 3, 4, 5, 6], ["BJ", "TY", "NY", "SH", "HZ"]],
+              [[7, 8, 9, 10, 11, 12], ["SH", "NY", "BJ", "IT", "BR", "TL"]]]
+JOIN_FIXTURES = {"tests": [
+    {   # JoinWOCGTestAntiJoinType2, :1592-1712
+        "name": "AntiJoinType2", "join_type": "anti", "null_aware": False,
+        "build_schema": [["table0_f0", "uint32"], ["table0_f1", "uint32"], ["table0_f2", "uint32"]],
+        "probe_schema": [["table1_f0", "uint32"], ["table1_f1", "uint32"]],
+        "build_key": 0, "probe_key": 0, "result": [[1, 0], [1, 1]],
+        "build": _BUILD,
+        "probe": [[[1, 2, 3, 4, 5, 6], [1, 2, 3, 4, 5, 6]], [[7, 8, 9, 10, 11, 12], [7, 8, 9, 10, 11, 12]]],
+        "expected": [[[4], [4]], [[7, 9, 11], [7, 9, 11]]],
+    },
+    {   # JoinWOCGTestAntiJoinSingleKey (null-aware anti), :1714-1818
+        "name": "AntiJoinSingleKey", "join_type": "anti", "null_aware": True,
+        "build_schema": [["table0_f0", "int32"], ["table0_f1", "int32"]],
+        "probe_schema": [["table1_f0", "int32"], ["table1_f1", "int32"]],
+        "build_key": 0, "probe_key": 0, "result": [[1, 0], [1, 1]],
+        "build": [[[2, 2, 3, 4, None, None, 6], [3, 3, 2, 1, None, 5, None]]],
+        "probe": [[[1, 1, 2, 2, 3, None, None, 6], [2, 2, 1, 1, 3, None, 5, None]]],
+        "expected": [[[], []]],
+    },
+    {   # JoinWOCGTestSemiJoinType3 (boolean key), :2056-2154
+        "name": "SemiJoinType3", "join_type": "semi", "null_aware": False,
+        "build_schema": [["table0_f0", "bool"]],
+        "probe_schema": [["table1_f0", "int32"], ["table1_f1", "bool"]],
+        "build_key": 0, "probe_key": 1, "result": [[1, 0], [1, 1]],
+        "build": [[[True]]],
+        "probe": [[[2], [True]]],
+        "expected": [[[2], [True]]],
+    },
+    {   # JoinWOCGTestSemiJoinType2WithUInt64, :2283-2403
+        "name": "SemiJoinType2WithUInt64", "join_type": "semi", "null_aware": False,
+        "build_schema": [["table0_f0", "uint64"], ["table0_f1", "uint32"], ["table0_f2", "uint32"]],
+        "probe_schema": [["table1_f0", "uint64"], ["table1_f1", "utf8"]],
+        "build_key": 0, "probe_key": 0, "result": [[1, 0], [1, 1]],
+        "build": _BUILD, "probe": _PROBE_STR,
+        "expected": [[[1, 3, 5, 6], ["BJ", "TY", "SH", "HZ"]], [[8, 10, 12], ["NY", "IT", "TL"]]],
+    },
+    {   # JoinWOCGTestInnerJoinType2WithUInt16 (repeated build keys), :2405-2532
+        "name": "InnerJoinType2WithUInt16", "join_type": "inner", "null_aware": False,
+        "build_schema": [["table0_f0", "uint16"], ["table0_f1", "uint32"], ["table0_f2", "uint32"]],
+        "probe_schema": [["table1_f0", "uint16"], ["table1_f1", "utf8"]],
+        "build_key": 0, "probe_key": 0, "result": [[1, 0], [1, 1]],
+        "build": _BUILD, "probe": _PROBE_STR,
+        "expected": [[[1, 1, 3, 3, 5, 6, 6], ["BJ", "BJ", "TY", "TY", "SH", "HZ", "HZ"]],
+                     [[8, 10, 10, 12], ["NY", "IT", "IT", "TL"]]],
+    },
+]}
+
+
 def main():
     with open(os.path.join(HERE, "split_fixtures.json"), "w") as f:
         json.dump(SPLIT_FIXTURES, f, indent=1)
     with open(os.path.join(HERE, "agg_fixtures.json"), "w") as f:
         json.dump(AGG_FIXTURES, f, indent=1)
-    print("wrote split_fixtures.json, agg_fixtures.json")
+    with open(os.path.join(HERE, "join_fixtures.json"), "w") as f:
+        json.dump(JOIN_FIXTURES, f, indent=1)
+    print("wrote split_fixtures.json, agg_fixtures.json, join_fixtures.json")
 
 
 if __name__ == "__main__":

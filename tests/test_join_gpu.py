@@ -1,0 +1,199 @@
+"""Parity of the HIP hash join with the oracle through the C ABI: output rows
+bit-exact and in the reference's order (probe order, matches in build insertion
+order)."""
+import numpy as np
+import pytest
+
+import oracle
+from gazelle_plugin_amd import types as T
+from gazelle_plugin_amd.batch import DeviceBatch, HostBatch, HostColumn, bitmap_from_bools, bools_from_bitmap
+from gazelle_plugin_amd.join import ConditionedProbeKernel, HashRelationKernel
+from golden import fixtures as F
+from test_oracle_join import JT, key_of, take
+
+pytestmark = pytest.mark.gpu
+
+
+def host_values(batch, c, rows):
+    """Raw values + validity of rows (bytes for utf8)."""
+    col = batch.columns[c]
+    valid = bools_from_bitmap(col.validity, batch.num_rows)
+    vals = []
+    for r in rows:
+        if col.type_id in (T.STRING, T.BINARY):
+            vals.append(bytes(col.values[col.offsets[r]:col.offsets[r + 1]]))
+        elif col.type_id == T.BOOL:
+            vals.append(int(bools_from_bitmap(col.values, batch.num_rows)[r]))
+        else:
+            vals.append(col.values[r].item())
+    return vals, [bool(valid[r]) for r in rows]
+
+
+def check_output(got, exp_cols):
+    for (t, vals, ok), (ev, eok) in zip(got, exp_cols):
+        assert list(ok) == eok
+        gv = [v.item() if hasattr(v, "item") else v for v in vals]
+        assert [g for g, o in zip(gv, eok) if o] == [e for e, o in zip(ev, eok) if o]
+
+
+def run_case(ctx, build, probe, btypes, ptypes, bkey, pkey, join_type, null_aware, result, batched,
+             builder_type=1):
+    rel = HashRelationKernel.Make(ctx, btypes, bkey, builder_type)
+    if batched:
+        rel.EvaluateBatches([DeviceBatch.from_host(b) for b in build])
+    else:
+        for b in build:
+            rel.Evaluate(DeviceBatch.from_host(b))
+    rel.Finish()
+    pk = ConditionedProbeKernel.Make(rel, ptypes, pkey, join_type, null_aware, result)
+    outs = []
+    if batched:
+        dev = [DeviceBatch.from_host(b) for b in probe]
+        n, per = pk.ProcessBatches(dev)
+        cols = pk.result_columns(n)
+        off = 0
+        for m in per:
+            outs.append([(t, v[off:off + m], ok[off:off + m]) for t, v, ok in cols])
+            off += m
+    else:
+        for b in probe:
+            n = pk.Process(DeviceBatch.from_host(b))
+            outs.append(pk.result_columns(n))
+    pk.close()
+    rel.close()
+    return outs
+
+
+def expected_cols(build, probe, bkey, pkey, ktype, join_type, null_aware, result, builder_type=1):
+    bflat = []
+    for bi, b in enumerate(build):
+        bflat += [(bi, r) for r in range(b.num_rows)]
+    outs = oracle.join([key_of(b, bkey) for b in build], [key_of(b, pkey) for b in probe], ktype,
+                       JT[join_type], null_aware, builder_type)
+    res = []
+    for pb, out in zip(probe, outs):
+        if join_type == "inner":
+            prow, brow = out
+        else:
+            prow, brow = out, None
+        cols = []
+        for s, i in result:
+            if s == 1:
+                cols.append(host_values(pb, i, list(prow)))
+            else:
+                if brow is None:
+                    cols.append(([None] * len(prow), [False] * len(prow)))
+                else:
+                    vals, oks = [], []
+                    for g in brow:
+                        bi, r = bflat[int(g)]
+                        v, o = host_values(build[bi], i, [r])
+                        vals += v
+                        oks += o
+                    cols.append((vals, oks))
+        res.append(cols)
+    return res
+
+
+@pytest.mark.parametrize("case", range(5))
+@pytest.mark.parametrize("batched", [True, False])
+def test_reference_join_fixtures_on_gpu(ctx, case, batched):
+    fx = F.load("join_fixtures.json")["tests"][case]
+    build = [F.batch(fx["build_schema"], b) for b in fx["build"]]
+    probe = [F.batch(fx["probe_schema"], b) for b in fx["probe"]]
+    btypes = [F.type_id(t) for _, t in fx["build_schema"]]
+    ptypes = [F.type_id(t) for _, t in fx["probe_schema"]]
+    outs = run_case(ctx, build, probe, btypes, ptypes, fx["build_key"], fx["probe_key"], fx["join_type"],
+                    fx["null_aware"], fx["result"], batched)
+    for got, exp in zip(outs, fx["expected"]):
+        vals = []
+        for t, v, ok in got:
+            vals.append([x.decode() if isinstance(x, bytes) else (bool(x) if t == T.BOOL else x.item())
+                         for x in v])
+        assert vals == exp, (fx["name"], vals, exp)
+
+
+def _workload(rng, nbuild, nprobe, dup_keys, key_type=T.INT64, nulls=0.02):
+    npt = {T.INT64: np.int64, T.INT32: np.int32, T.UINT16: np.uint16, T.DOUBLE: np.float64}[key_type]
+    if dup_keys:
+        bk = rng.integers(0, nbuild // 3 + 1, size=nbuild)
+    else:
+        bk = rng.permutation(nbuild * 4)[:nbuild]
+    pk = np.where(rng.random(nprobe) < 0.9, bk[rng.integers(0, nbuild, size=nprobe)],
+                  rng.integers(nbuild * 4, nbuild * 8, size=nprobe))
+    bk, pk = bk.astype(npt), pk.astype(npt)
+    if key_type == T.DOUBLE:
+        pk[::97] = np.nan
+    build, probe = [], []
+    for b0 in range(0, nbuild, 3000):
+        b1 = min(nbuild, b0 + 3000)
+        n = b1 - b0
+        v = rng.random(n) >= nulls
+        build.append(HostBatch(n, [HostColumn(key_type, bk[b0:b1].copy(), bitmap_from_bools(v)),
+                                   HostColumn(T.INT32, rng.integers(-1000, 1000, size=n).astype(np.int32)),
+                                   HostColumn(T.DOUBLE, rng.standard_normal(n))]))
+    for b0 in range(0, nprobe, 4096):
+        b1 = min(nprobe, b0 + 4096)
+        n = b1 - b0
+        v = rng.random(n) >= nulls
+        lens = rng.integers(0, 9, size=n).astype(np.int32)
+        off = np.zeros(n + 1, np.int32)
+        off[1:] = np.cumsum(lens)
+        probe.append(HostBatch(n, [HostColumn(key_type, pk[b0:b1].copy(), bitmap_from_bools(v)),
+                                   HostColumn(T.STRING, rng.integers(97, 123, size=int(off[-1]), dtype=np.uint8),
+                                              None, off)]))
+    return build, probe
+
+
+@pytest.mark.parametrize("join_type", ["inner", "semi", "anti"])
+@pytest.mark.parametrize("dup_keys", [False, True])
+@pytest.mark.parametrize("batched", [True, False])
+def test_random_joins_match_oracle(ctx, join_type, dup_keys, batched):
+    rng = np.random.default_rng(hash((join_type, dup_keys)) % 1000)
+    build, probe = _workload(rng, 10000, 20000, dup_keys)
+    btypes, ptypes = [T.INT64, T.INT32, T.DOUBLE], [T.INT64, T.STRING]
+    result = [[1, 0], [0, 1], [0, 2], [1, 1]] if join_type == "inner" else [[1, 0], [1, 1]]
+    got = run_case(ctx, build, probe, btypes, ptypes, 0, 0, join_type, False, result, batched)
+    exp = expected_cols(build, probe, 0, 0, T.INT64, join_type, False, result)
+    for g, e in zip(got, exp):
+        check_output(g, e)
+
+
+@pytest.mark.parametrize("key_type", [T.INT32, T.UINT16, T.DOUBLE])
+def test_key_types(ctx, key_type):
+    rng = np.random.default_rng(key_type)
+    build, probe = _workload(rng, 3000, 9000, True, key_type)
+    btypes, ptypes = [key_type, T.INT32, T.DOUBLE], [key_type, T.STRING]
+    for jt in ("inner", "semi", "anti"):
+        result = [[1, 0], [0, 1]] if jt == "inner" else [[1, 0]]
+        got = run_case(ctx, build, probe, btypes, ptypes, 0, 0, jt, False, result, True)
+        exp = expected_cols(build, probe, 0, 0, key_type, jt, False, result)
+        for g, e in zip(got, exp):
+            check_output(g, e)
+
+
+def test_null_aware_anti_and_semi_builder(ctx):
+    rng = np.random.default_rng(9)
+    build, probe = _workload(rng, 2000, 5000, True)
+    btypes, ptypes = [T.INT64, T.INT32, T.DOUBLE], [T.INT64, T.STRING]
+    # build side has null keys -> null-aware anti emits nothing
+    got = run_case(ctx, build, probe, btypes, ptypes, 0, 0, "anti", True, [[1, 0]], True)
+    assert all(len(g[0][1]) == 0 for g in got)
+    # skip-duplicate builder (type 3) gives the same semi join
+    got = run_case(ctx, build, probe, btypes, ptypes, 0, 0, "semi", False, [[1, 0], [1, 1]], True, 3)
+    exp = expected_cols(build, probe, 0, 0, T.INT64, "semi", False, [[1, 0], [1, 1]], 3)
+    for g, e in zip(got, exp):
+        check_output(g, e)
+
+
+def test_empty_build_side(ctx):
+    build = [HostBatch(0, [HostColumn(T.INT64, np.zeros(0, np.int64)), HostColumn(T.INT32, np.zeros(0, np.int32)),
+                           HostColumn(T.DOUBLE, np.zeros(0))])]
+    rng = np.random.default_rng(1)
+    _, probe = _workload(rng, 100, 1000, False)
+    btypes, ptypes = [T.INT64, T.INT32, T.DOUBLE], [T.INT64, T.STRING]
+    for jt, na, expect_all in (("inner", False, False), ("semi", False, False), ("anti", False, True),
+                               ("anti", True, True)):
+        got = run_case(ctx, build, probe, btypes, ptypes, 0, 0, jt, na, [[1, 0]], True)
+        n = sum(len(g[0][1]) for g in got)
+        assert n == (sum(b.num_rows for b in probe) if expect_all else 0)

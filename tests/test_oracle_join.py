@@ -1,0 +1,110 @@
+"""Pins of the oracle's hash join: the reference's own join tests
+(arrow_compute_test_join_wocg.cc), a brute-force nested loop, and pyarrow's join."""
+import numpy as np
+import pytest
+
+import oracle
+from gazelle_plugin_amd import types as T
+from gazelle_plugin_amd.batch import bools_from_bitmap
+from golden import fixtures as F
+
+JT = {"inner": 0, "anti": 2, "semi": 3}
+
+
+def key_of(batch, c):
+    col = batch.columns[c]
+    return (col.values, col.validity, batch.num_rows)
+
+
+def take(batch, c, rows):
+    col = batch.columns[c]
+    valid = bools_from_bitmap(col.validity, batch.num_rows)
+    out = []
+    for r in rows:
+        if not valid[r]:
+            out.append(None)
+        elif col.type_id == T.STRING:
+            out.append(bytes(col.values[col.offsets[r]:col.offsets[r + 1]]).decode())
+        elif col.type_id == T.BOOL:
+            out.append(bool(bools_from_bitmap(col.values, batch.num_rows)[r]))
+        else:
+            out.append(col.values[r].item())
+    return out
+
+
+@pytest.mark.parametrize("case", range(5))
+def test_reference_join_fixtures(case):
+    fx = F.load("join_fixtures.json")["tests"][case]
+    build = [F.batch(fx["build_schema"], b) for b in fx["build"]]
+    probe = [F.batch(fx["probe_schema"], b) for b in fx["probe"]]
+    kt = F.type_id(fx["build_schema"][fx["build_key"]][1])
+    outs = oracle.join([key_of(b, fx["build_key"]) for b in build],
+                       [key_of(b, fx["probe_key"]) for b in probe], kt, JT[fx["join_type"]],
+                       fx["null_aware"])
+    for pb, out, exp in zip(probe, outs, fx["expected"]):
+        rows = out[0] if fx["join_type"] == "inner" else out
+        got = [take(pb, i, list(rows)) for s, i in fx["result"]]
+        assert got == exp, (fx["name"], got, exp)
+
+
+def brute(build_keys, probe_keys, join_type, null_aware=False):
+    """Nested loop over (key, valid) lists: the semantics the oracle restates."""
+    out = []
+    has_null = any(not v for _, v in build_keys)
+    nkeys = len({k for k, v in build_keys if v})
+    for i, (pk, pv) in enumerate(probe_keys):
+        matches = [b for b, (bk, bv) in enumerate(build_keys) if pv and bv and bk == pk]
+        if join_type == "inner":
+            out += [(i, b) for b in matches]
+        elif join_type == "semi":
+            if matches:
+                out.append(i)
+        else:
+            if not null_aware:
+                emit = not matches
+            elif nkeys == 0 and not has_null:
+                emit = True
+            elif has_null:
+                emit = False
+            else:
+                emit = pv and not matches
+            if emit:
+                out.append(i)
+    return out
+
+
+@pytest.mark.parametrize("join_type", ["inner", "semi", "anti"])
+@pytest.mark.parametrize("null_aware", [False, True])
+def test_random_join_brute_force(join_type, null_aware):
+    from gazelle_plugin_amd.batch import bitmap_from_bools
+
+    rng = np.random.default_rng(4)
+    bk = rng.integers(0, 40, size=300).astype(np.int64)
+    bv = rng.random(300) > 0.05
+    pk = rng.integers(0, 60, size=500).astype(np.int64)
+    pv = rng.random(500) > 0.05
+    # build in two batches
+    b1, b2 = slice(0, 170), slice(170, 300)
+    outs = oracle.join([(bk[b1], bitmap_from_bools(bv[b1]), 170), (bk[b2], bitmap_from_bools(bv[b2]), 130)],
+                       [(pk, bitmap_from_bools(pv), 500)], T.INT64, JT[join_type], null_aware)
+    exp = brute(list(zip(bk.tolist(), bv.tolist())), list(zip(pk.tolist(), pv.tolist())), join_type, null_aware)
+    if join_type == "inner":
+        got = list(zip(outs[0][0].tolist(), outs[0][1].tolist()))
+    else:
+        got = outs[0].tolist()
+    assert got == exp
+
+
+def test_inner_join_matches_pyarrow():
+    pa = pytest.importorskip("pyarrow")
+    rng = np.random.default_rng(5)
+    bk = rng.permutation(20000)[:5000].astype(np.int64)
+    pk = rng.integers(0, 20000, size=40000).astype(np.int64)
+    outs = oracle.join([(bk, None, len(bk))], [(pk, None, len(pk))], T.INT64, 0)
+    pr, br = outs[0]
+    got = sorted(zip(pr.tolist(), br.tolist()))
+    l = pa.table({"k": pk, "pi": np.arange(len(pk))})
+    r = pa.table({"k": bk, "bi": np.arange(len(bk))})
+    j = l.join(r, "k", join_type="inner")
+    exp = sorted(zip(j["pi"].to_pylist(), j["bi"].to_pylist()))
+    assert got == exp
