@@ -33,6 +33,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-rows", type=int, default=0, help="split tile rows (0 = default)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run a device copy of this many MiB (PMC calibration)")
     return ap.parse_args()
@@ -111,8 +112,12 @@ def main():
 
     arr = make_batch_array(tab.batches)
 
+    from gazelle_plugin_amd.splitter import SplitOptions
+
+    opts = SplitOptions(tile_rows=args.tile_rows)
+
     def step():
-        sp = Splitter.Make(ctx, "hash", datagen.SPLIT_TYPES, P, [0])
+        sp = Splitter.Make(ctx, "hash", datagen.SPLIT_TYPES, P, [0], opts)
         sp.SplitBatches(tab.batches, arr)
         sp.close()  # buffers return to the pool in stream order (no host sync)
 

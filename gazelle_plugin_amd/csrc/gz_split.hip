@@ -60,6 +60,7 @@ struct SplitParams {
   int out_col[kMaxCols];     // output column of an input column, -1 if dropped
   int bin[kMaxCols];         // binary column ordinal, -1 otherwise
   int nbin;
+  int bin_col0;              // input column of the first binary column
   // per-segment buffers
   uint32_t* counts;          // [P][T]
   uint32_t* prefix;          // [P][T] exclusive within partition
@@ -75,24 +76,29 @@ struct SplitParams {
   int* err;
 };
 
-__device__ __forceinline__ const void* colbuf(const SplitParams& p, int batch, int col, int k) {
-  return p.bufs[(int64_t)batch * p.bstride + col * 3 + k];
+// Buffer pointers of the tile's batch, staged in LDS once per workgroup.
+struct TilePtrs {
+  const void* v[kMaxCols * 3];
+};
+
+__device__ __forceinline__ void load_tile_ptrs(const SplitParams& p, int batch, const void** s_bp) {
+  for (int i = threadIdx.x; i < p.ncols * 3; i += blockDim.x)
+    s_bp[i] = p.bufs[(int64_t)batch * p.bstride + i];
 }
 
 // Spark hash of key column c at row r, chained through h (Murmur3Hash semantics:
 // a null value leaves the running hash unchanged).
-__device__ __forceinline__ uint32_t hash_key(const SplitParams& p, int batch, int c, int t,
-                                             int64_t r, uint32_t h) {
-  const uint8_t* vb = (const uint8_t*)colbuf(p, batch, c, 0);
+__device__ __forceinline__ uint32_t hash_key(const void* const* bp, int c, int t, int64_t r, uint32_t h) {
+  const uint8_t* vb = (const uint8_t*)bp[c * 3 + 0];
   if (vb && !bit_get(vb, r)) return h;
-  const void* v = colbuf(p, batch, c, 2);
+  const void* v = bp[c * 3 + 2];
   switch (t) {
+    case GZ_TYPE_INT64: return spark_hash_long((uint64_t)((const int64_t*)v)[r], h);
+    case GZ_TYPE_INT32:
+    case GZ_TYPE_DATE32: return spark_hash_int((uint32_t)((const int32_t*)v)[r], h);
     case GZ_TYPE_BOOL: return spark_hash_int(bit_get((const uint8_t*)v, r), h);
     case GZ_TYPE_INT8: return spark_hash_int((uint32_t)(int32_t)((const int8_t*)v)[r], h);
     case GZ_TYPE_INT16: return spark_hash_int((uint32_t)(int32_t)((const int16_t*)v)[r], h);
-    case GZ_TYPE_INT32:
-    case GZ_TYPE_DATE32: return spark_hash_int((uint32_t)((const int32_t*)v)[r], h);
-    case GZ_TYPE_INT64: return spark_hash_long((uint64_t)((const int64_t*)v)[r], h);
     case GZ_TYPE_FLOAT: {
       float f = ((const float*)v)[r];
       uint32_t bits;
@@ -109,7 +115,7 @@ __device__ __forceinline__ uint32_t hash_key(const SplitParams& p, int batch, in
     }
     case GZ_TYPE_STRING:
     case GZ_TYPE_BINARY: {
-      const int32_t* off = (const int32_t*)colbuf(p, batch, c, 1);
+      const int32_t* off = (const int32_t*)bp[c * 3 + 1];
       const uint8_t* d = (const uint8_t*)v + off[r];
       int len = off[r + 1] - off[r];
       int aligned = len & ~3;
@@ -125,16 +131,24 @@ __device__ __forceinline__ uint32_t hash_key(const SplitParams& p, int batch, in
   }
 }
 
-__device__ __forceinline__ int compute_pid(const SplitParams& p, const TileDesc& td, int lr) {
+// Partition id of row lr of a tile.  KEYK = 1: one int64 key without
+// validity (the hot case), else the generic path.
+template <int KEYK>
+__device__ __forceinline__ int compute_pid(const SplitParams& p, const void* const* bp, const TileDesc& td,
+                                           int lr) {
   int64_t r = (int64_t)td.row_begin + lr;
+  if (KEYK == 1) {
+    uint64_t k = (uint64_t)((const int64_t*)bp[p.key_col[0] * 3 + 2])[r];
+    return pmod_hash(spark_hash_long(k, p.seed), p.fm);
+  }
   if (p.mode == 0) {
     uint32_t h = p.seed;
-    for (int k = 0; k < p.nkeys; k++) h = hash_key(p, td.batch, p.key_col[k], p.key_type[k], r, h);
+    for (int k = 0; k < p.nkeys; k++) h = hash_key(bp, p.key_col[k], p.key_type[k], r, h);
     return pmod_hash(h, p.fm);
   }
   if (p.mode == 1) return (int)fastmod_u32((uint32_t)(td.rr + lr), p.fm);
   if (p.mode == 2) {
-    int v = ((const int32_t*)colbuf(p, td.batch, 0, 2))[r];
+    int v = ((const int32_t*)bp[2])[r];
     if (v < 0 || v >= p.P) {
       atomicOr(p.err, 1);
       return 0;
@@ -145,25 +159,54 @@ __device__ __forceinline__ int compute_pid(const SplitParams& p, const TileDesc&
 }
 
 // ---------------------------------------------------------------- pass 1
-template <int BLOCK>
+template <int BLOCK, int KEYK>
 __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
   extern __shared__ uint32_t s_cnt[];  // [P] rows, then [nbin][P] bytes
+  __shared__ const void* s_bp[kMaxCols * 3];
   int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
   if (t < 0) return;
   const TileDesc td = p.tiles[t];
   const int P = p.P;
+  load_tile_ptrs(p, td.batch, s_bp);
   for (int i = threadIdx.x; i < P * (1 + p.nbin); i += BLOCK) s_cnt[i] = 0;
   __syncthreads();
-  for (int lr = threadIdx.x; lr < td.nrows; lr += BLOCK) {
-    int pid = compute_pid(p, td, lr);
-    atomicAdd(&s_cnt[pid], 1u);
-    if (p.nbin) {
-      int64_t r = (int64_t)td.row_begin + lr;
-      for (int c = 0; c < p.ncols; c++) {
-        int j = p.bin[c];
-        if (j < 0) continue;
-        const int32_t* off = (const int32_t*)colbuf(p, td.batch, c, 1);
-        atomicAdd(&s_cnt[P * (1 + j) + pid], (uint32_t)(off[r + 1] - off[r]));
+  if (KEYK == 1 && p.nbin == 1) {
+    // hot path: int64 key, one utf8 column; four independent rows per step
+    const int64_t* key = (const int64_t*)s_bp[p.key_col[0] * 3 + 2] + td.row_begin;
+    int bc = p.bin_col0;
+    const int32_t* off = (const int32_t*)s_bp[bc * 3 + 1] + td.row_begin;
+    for (int lr0 = threadIdx.x; lr0 < td.nrows; lr0 += 4 * BLOCK) {
+      uint64_t k[4];
+      int32_t o0[4], o1[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        int lr = lr0 + u * BLOCK;
+        bool a = lr < td.nrows;
+        k[u] = a ? (uint64_t)key[lr] : 0;
+        o0[u] = a ? off[lr] : 0;
+        o1[u] = a ? off[lr + 1] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (lr0 + u * BLOCK < td.nrows) {
+          int pid = pmod_hash(spark_hash_long(k[u], p.seed), p.fm);
+          atomicAdd(&s_cnt[pid], 1u);
+          atomicAdd(&s_cnt[P + pid], (uint32_t)(o1[u] - o0[u]));
+        }
+      }
+    }
+  } else {
+    for (int lr = threadIdx.x; lr < td.nrows; lr += BLOCK) {
+      int pid = compute_pid<KEYK>(p, s_bp, td, lr);
+      atomicAdd(&s_cnt[pid], 1u);
+      if (p.nbin) {
+        int64_t r = (int64_t)td.row_begin + lr;
+        for (int c = 0; c < p.ncols; c++) {
+          int j = p.bin[c];
+          if (j < 0) continue;
+          const int32_t* off = (const int32_t*)s_bp[c * 3 + 1];
+          atomicAdd(&s_cnt[P * (1 + j) + pid], (uint32_t)(off[r + 1] - off[r]));
+        }
       }
     }
   }
@@ -171,7 +214,7 @@ __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
   // (tiles start on multiples of 8 rows inside a batch).
   for (int c = 0; c < p.ncols; c++) {
     if (p.out_col[c] < 0 || p.kind[c] == K_NA) continue;
-    const uint8_t* vb = (const uint8_t*)colbuf(p, td.batch, c, 0);
+    const uint8_t* vb = (const uint8_t*)s_bp[c * 3 + 0];
     if (!vb) continue;
     uint32_t nulls = 0;
     int nbytes = (td.nrows + 7) >> 3;
@@ -278,12 +321,40 @@ __device__ __forceinline__ void copy_bytes(uint8_t* dst, const uint8_t* src, int
   for (; i < len; i++) dst[i] = src[i];
 }
 
-template <int TILE, int BLOCK>
+// LDS layout of the scatter kernel (bytes from the start of dynamic LDS).
+struct ScatterLds {
+  size_t scan, pbase, bbase, brel, toff, wcnt, bp, pid, pos, spid, stage, total;
+};
+__host__ __device__ inline ScatterLds scatter_layout(int TILE, int BLOCK, int P) {
+  ScatterLds L;
+  size_t o = 0;
+  auto take = [&](size_t bytes, size_t align) {
+    o = (o + align - 1) / align * align;
+    size_t r = o;
+    o += bytes;
+    return r;
+  };
+  int NW = BLOCK / 64;
+  L.scan = take(8 * (NW + 1), 8);
+  L.pbase = take(8 * (size_t)P, 8);
+  L.bbase = take(8 * (size_t)P, 8);
+  L.brel = take(4 * (size_t)P, 4);
+  L.toff = take(4 * (size_t)(P + 1), 4);
+  L.wcnt = take(2 * (size_t)NW * P, 4);
+  L.bp = take(8 * (size_t)kMaxCols * 3, 8);
+  L.pid = take(2 * (size_t)TILE, 4);
+  L.pos = take(2 * (size_t)TILE, 4);
+  L.spid = take(2 * (size_t)TILE, 4);
+  L.stage = take((size_t)TILE * 8 + 4, 16);
+  L.total = o;
+  return L;
+}
+
+template <int TILE, int BLOCK, int KEYK>
 __global__ __launch_bounds__(BLOCK) void split_scatter_kernel(SplitParams p) {
   constexpr int NW = BLOCK / kWave;
-  constexpr int RW = TILE / NW;  // rows per wave
-  constexpr int NJ = RW / kWave; // rows per lane
-  static_assert(NJ * kWave * NW == TILE, "tile must be a multiple of the block");
+  constexpr int RW = TILE / NW;  // rows per wave in the ranking pass
+  static_assert(RW % kWave == 0, "tile must be a multiple of the block");
   extern __shared__ __align__(16) uint8_t smem[];
   int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
   if (t < 0) return;
@@ -291,118 +362,109 @@ __global__ __launch_bounds__(BLOCK) void split_scatter_kernel(SplitParams p) {
   const int n = td.nrows;
   const int P = p.P;
   const int tid = threadIdx.x, w = tid / kWave, lane = __lane_id();
+  const ScatterLds Lo = scatter_layout(TILE, BLOCK, P);
+  uint64_t* s_scan = (uint64_t*)(smem + Lo.scan);
+  int64_t* s_pbase = (int64_t*)(smem + Lo.pbase);
+  int64_t* s_bbase = (int64_t*)(smem + Lo.bbase);
+  uint32_t* s_brel = (uint32_t*)(smem + Lo.brel);
+  uint32_t* s_toff = (uint32_t*)(smem + Lo.toff);
+  uint16_t* s_wcnt = (uint16_t*)(smem + Lo.wcnt);
+  const void** s_bp = (const void**)(smem + Lo.bp);
+  uint16_t* s_pid = (uint16_t*)(smem + Lo.pid);
+  uint16_t* s_pos = (uint16_t*)(smem + Lo.pos);
+  uint16_t* s_spid = (uint16_t*)(smem + Lo.spid);
+  uint8_t* s_stage = smem + Lo.stage;
 
-  // ---- LDS layout
-  uint8_t* sp = smem;
-  uint64_t* s_scan = (uint64_t*)sp;            sp += 8 * (NW + 1);
-  int64_t* s_pbase = (int64_t*)sp;             sp += 8 * P;      // dest row of the tile's run
-  int64_t* s_bbase = (int64_t*)sp;             sp += 8 * P;      // dest byte of the run (one column at a time)
-  uint32_t* s_toff = (uint32_t*)sp;            sp += 4 * (P + 1);
-  uint16_t* s_wcnt = (uint16_t*)sp;            sp += 2 * NW * P;
-  sp = (uint8_t*)(((uintptr_t)sp + 15) & ~(uintptr_t)15);
-  uint16_t* s_spid = (uint16_t*)sp;            sp += 2 * TILE;
-  sp = (uint8_t*)(((uintptr_t)sp + 15) & ~(uintptr_t)15);
-  uint8_t* s_stage = sp;                       // TILE * 8 bytes (also utf8 src/boff)
-
+  load_tile_ptrs(p, td.batch, s_bp);
   for (int i = tid; i < NW * P; i += BLOCK) s_wcnt[i] = 0;
   __syncthreads();
 
-  // ---- 1: partition id and stable rank of every row.  Wave w owns rows
-  // [w*RW, (w+1)*RW); iteration j covers 64 consecutive rows.
-  int pid_r[NJ];
-  int pos_r[NJ];
-#pragma unroll
-  for (int j = 0; j < NJ; j++) {
+  // ---- 1: partition id and stable rank of every row.  Wave w ranks rows
+  // [w*RW, (w+1)*RW) in chunks of 64 consecutive rows.
+  for (int j = 0; j < RW / kWave; j++) {
     int lr = w * RW + j * kWave + lane;
     bool act = lr < n;
-    int pid = act ? compute_pid(p, td, lr) : 0;
+    int pid = act ? compute_pid<KEYK>(p, s_bp, td, lr) : 0;
     uint64_t m = match_bits((uint32_t)pid, act, p.pbits);
     uint64_t lt = m & lanemask_lt();
     int before = act ? s_wcnt[w * P + pid] : 0;
-    pos_r[j] = before + __popcll(lt);
-    pid_r[j] = act ? pid : -1;
-    if (act && lt == 0) s_wcnt[w * P + pid] = (uint16_t)(before + __popcll(m));
+    if (act) {
+      s_pid[lr] = (uint16_t)pid;
+      s_pos[lr] = (uint16_t)(before + __popcll(lt));
+      if (lt == 0) s_wcnt[w * P + pid] = (uint16_t)(before + __popcll(m));
+    }
   }
   __syncthreads();
 
-  // ---- 2: per partition: wave offsets (in place), tile count, then tile offsets
+  // ---- 2: per partition: wave offsets (in place), tile count, tile offsets,
+  // destination bases
   {
-    constexpr int PPT = (kMaxPartitions + BLOCK - 1) / BLOCK;  // partitions per thread
-    uint32_t local[PPT];
+    constexpr int PPT = (kMaxPartitions + BLOCK - 1) / BLOCK;
     uint32_t sum = 0;
-#pragma unroll
     for (int k = 0; k < PPT; k++) {
       int pp = tid * PPT + k;
+      if (pp >= P) break;
       uint32_t c = 0;
-      if (pp < P) {
-        for (int ww = 0; ww < NW; ww++) {
-          uint32_t x = s_wcnt[ww * P + pp];
-          s_wcnt[ww * P + pp] = (uint16_t)c;
-          c += x;
-        }
+      for (int ww = 0; ww < NW; ww++) {
+        uint32_t x = s_wcnt[ww * P + pp];
+        s_wcnt[ww * P + pp] = (uint16_t)c;
+        c += x;
       }
-      local[k] = c;
+      s_toff[pp] = c;  // count for now
       sum += c;
     }
     uint64_t total;
     uint64_t ex = block_exclusive_scan<uint64_t>(sum, s_scan, &total);
     uint32_t run = (uint32_t)ex;
-#pragma unroll
     for (int k = 0; k < PPT; k++) {
       int pp = tid * PPT + k;
-      if (pp < P) {
-        s_toff[pp] = run;
-        s_pbase[pp] = p.row_start[pp] + p.prefix[(int64_t)pp * p.T + t];
-      }
-      run += local[k];
+      if (pp >= P) break;
+      uint32_t c = s_toff[pp];
+      s_toff[pp] = run;
+      s_pbase[pp] = p.row_start[pp] + p.prefix[(int64_t)pp * p.T + t];
+      run += c;
     }
     if (tid == 0) s_toff[P] = (uint32_t)total;
   }
   __syncthreads();
 
   // ---- 3: sorted position of every row; pid of every sorted position
-#pragma unroll
-  for (int j = 0; j < NJ; j++) {
-    int pid = pid_r[j];
-    if (pid >= 0) {
-      int spos = (int)s_toff[pid] + s_wcnt[w * P + pid] + pos_r[j];
-      pos_r[j] = spos;
-      s_spid[spos] = (uint16_t)pid;
-    }
+  for (int lr = tid; lr < n; lr += BLOCK) {
+    int pid = s_pid[lr];
+    int spos = (int)s_toff[pid] + s_wcnt[(lr / RW) * P + pid] + s_pos[lr];
+    s_pos[lr] = (uint16_t)spos;
+    s_spid[spos] = (uint16_t)pid;
   }
   __syncthreads();
 
   // ---- 4: columns
+  const int64_t rb = td.row_begin;
   for (int c = 0; c < p.ncols; c++) {
     int oc = p.out_col[c];
     if (oc < 0) continue;
     int kind = p.kind[c];
     if (kind == K_NA) continue;
-    const uint8_t* vb = (const uint8_t*)colbuf(p, td.batch, c, 0);
-    const uint8_t* src = (const uint8_t*)colbuf(p, td.batch, c, 2);
-    const int64_t rb = td.row_begin;
+    const uint8_t* vb = (const uint8_t*)s_bp[c * 3 + 0];
+    const uint8_t* src = (const uint8_t*)s_bp[c * 3 + 2];
 
     // validity: clear the bit of every null row at its destination
     if (vb && p.out_valid[oc]) {
-#pragma unroll
-      for (int j = 0; j < NJ; j++) {
-        int pid = pid_r[j];
-        int lr = w * RW + j * kWave + lane;
-        if (pid >= 0 && !bit_get(vb, rb + lr)) {
-          int64_t d = s_pbase[pid] + (pos_r[j] - (int)s_toff[pid]);
-          atomicAnd(&p.out_valid[oc][d >> 5], ~(1u << (d & 31)));
+      uint32_t* ov = p.out_valid[oc];
+      for (int lr = tid; lr < n; lr += BLOCK) {
+        if (!bit_get(vb, rb + lr)) {
+          int pid = s_pid[lr];
+          int64_t d = s_pbase[pid] + ((int)s_pos[lr] - (int)s_toff[pid]);
+          atomicAnd(&ov[d >> 5], ~(1u << (d & 31)));
         }
       }
     }
 
     if (kind == K_BOOL) {
       uint32_t* ov = (uint32_t*)p.out_values[oc];
-#pragma unroll
-      for (int j = 0; j < NJ; j++) {
-        int pid = pid_r[j];
-        int lr = w * RW + j * kWave + lane;
-        if (pid >= 0 && !bit_get(src, rb + lr)) {
-          int64_t d = s_pbase[pid] + (pos_r[j] - (int)s_toff[pid]);
+      for (int lr = tid; lr < n; lr += BLOCK) {
+        if (!bit_get(src, rb + lr)) {
+          int pid = s_pid[lr];
+          int64_t d = s_pbase[pid] + ((int)s_pos[lr] - (int)s_toff[pid]);
           atomicAnd(&ov[d >> 5], ~(1u << (d & 31)));
         }
       }
@@ -410,94 +472,80 @@ __global__ __launch_bounds__(BLOCK) void split_scatter_kernel(SplitParams p) {
       const int W = p.width[c];
       const int parts = W > 8 ? W / 8 : 1;  // 16-byte values move as two 8-byte halves
       const int EW = W > 8 ? 8 : W;
+      uint8_t* dst = (uint8_t*)p.out_values[oc];
       for (int part = 0; part < parts; part++) {
-#pragma unroll
-        for (int j = 0; j < NJ; j++) {
-          int pid = pid_r[j];
-          int lr = w * RW + j * kWave + lane;
-          if (pid >= 0) {
-            int64_t r = rb + lr;
-            int sp2 = pos_r[j];
-            switch (EW) {
-              case 1: s_stage[sp2] = src[r]; break;
-              case 2: ((uint16_t*)s_stage)[sp2] = ((const uint16_t*)src)[r]; break;
-              case 4: ((uint32_t*)s_stage)[sp2] = ((const uint32_t*)src)[r]; break;
-              default: ((uint64_t*)s_stage)[sp2] = ((const uint64_t*)src)[r * parts + part]; break;
-            }
-          }
+        if (EW == 8) {
+          const uint64_t* s8 = (const uint64_t*)src;
+          for (int lr = tid; lr < n; lr += BLOCK)
+            ((uint64_t*)s_stage)[s_pos[lr]] = s8[(rb + lr) * parts + part];
+        } else if (EW == 4) {
+          const uint32_t* s4 = (const uint32_t*)src;
+          for (int lr = tid; lr < n; lr += BLOCK) ((uint32_t*)s_stage)[s_pos[lr]] = s4[rb + lr];
+        } else if (EW == 2) {
+          const uint16_t* s2 = (const uint16_t*)src;
+          for (int lr = tid; lr < n; lr += BLOCK) ((uint16_t*)s_stage)[s_pos[lr]] = s2[rb + lr];
+        } else {
+          for (int lr = tid; lr < n; lr += BLOCK) s_stage[s_pos[lr]] = src[rb + lr];
         }
         __syncthreads();
-        uint8_t* dst = (uint8_t*)p.out_values[oc];
         for (int i = tid; i < n; i += BLOCK) {
           int pid = s_spid[i];
           int64_t d = s_pbase[pid] + (i - (int)s_toff[pid]);
-          switch (EW) {
-            case 1: dst[d] = s_stage[i]; break;
-            case 2: ((uint16_t*)dst)[d] = ((uint16_t*)s_stage)[i]; break;
-            case 4: ((uint32_t*)dst)[d] = ((uint32_t*)s_stage)[i]; break;
-            default: ((uint64_t*)dst)[d * parts + part] = ((uint64_t*)s_stage)[i]; break;
-          }
+          if (EW == 8) ((uint64_t*)dst)[d * parts + part] = ((uint64_t*)s_stage)[i];
+          else if (EW == 4) ((uint32_t*)dst)[d] = ((uint32_t*)s_stage)[i];
+          else if (EW == 2) ((uint16_t*)dst)[d] = ((uint16_t*)s_stage)[i];
+          else dst[d] = s_stage[i];
         }
         __syncthreads();
       }
     } else if (kind == K_BINARY) {
       const int j_bin = p.bin[c];
-      const int32_t* soff = (const int32_t*)colbuf(p, td.batch, c, 1);
-      uint32_t* s_src = (uint32_t*)s_stage;          // [TILE] source byte offset
-      uint32_t* s_boff = s_src + TILE;               // [TILE+1] sorted byte prefix
-#pragma unroll
-      for (int j = 0; j < NJ; j++) {
-        int pid = pid_r[j];
-        int lr = w * RW + j * kWave + lane;
-        if (pid >= 0) {
-          int64_t r = rb + lr;
-          int o0 = soff[r], o1 = soff[r + 1];
-          s_src[pos_r[j]] = (uint32_t)o0;
-          s_boff[pos_r[j]] = (uint32_t)(o1 - o0);
-        }
+      const int32_t* soff = (const int32_t*)s_bp[c * 3 + 1];
+      uint32_t* s_src = (uint32_t*)s_stage;  // [TILE] source byte offset
+      uint32_t* s_boff = s_src + TILE;       // [TILE+1] sorted byte prefix
+      for (int lr = tid; lr < n; lr += BLOCK) {
+        int o0 = soff[rb + lr], o1 = soff[rb + lr + 1];
+        int sp = s_pos[lr];
+        s_src[sp] = (uint32_t)o0;
+        s_boff[sp] = (uint32_t)(o1 - o0);
       }
-      // run bases of this column
-      for (int pp = tid; pp < P; pp += BLOCK)
-        s_bbase[pp] = p.byte_start[(int64_t)j_bin * (P + 1) + pp] +
-                      p.bprefix[((int64_t)j_bin * P + pp) * p.T + t];
+      for (int pp = tid; pp < P; pp += BLOCK) {
+        uint32_t rel = p.bprefix[((int64_t)j_bin * P + pp) * p.T + t];
+        s_brel[pp] = rel;
+        s_bbase[pp] = p.byte_start[(int64_t)j_bin * (P + 1) + pp] + rel;
+      }
       __syncthreads();
-      // exclusive scan of lengths in sorted order (each thread: TILE/BLOCK entries)
       {
         constexpr int EPT = TILE / BLOCK;
-        uint32_t v[EPT];
         uint32_t s = 0;
         int i0 = tid * EPT;
-#pragma unroll
-        for (int k = 0; k < EPT; k++) {
-          v[k] = (i0 + k < n) ? s_boff[i0 + k] : 0u;
-          s += v[k];
-        }
+        for (int k = 0; k < EPT; k++)
+          if (i0 + k < n) s += s_boff[i0 + k];
         uint64_t total;
         uint64_t ex = block_exclusive_scan<uint64_t>(s, s_scan, &total);
         uint32_t run = (uint32_t)ex;
-#pragma unroll
         for (int k = 0; k < EPT; k++) {
-          if (i0 + k < n) s_boff[i0 + k] = run;
-          run += v[k];
+          if (i0 + k < n) {
+            uint32_t v = s_boff[i0 + k];
+            s_boff[i0 + k] = run;
+            run += v;
+          }
         }
         if (tid == 0) s_boff[n] = (uint32_t)total;
       }
       __syncthreads();
-      const uint8_t* sdata = src;
       uint8_t* odata = p.out_data[oc];
       int32_t* ooff = p.out_off[oc];
       for (int i = tid; i < n; i += BLOCK) {
         int pid = s_spid[i];
-        int k = i - (int)s_toff[pid];
-        uint32_t run0 = s_boff[s_toff[pid]];
+        uint32_t toff = s_toff[pid];
+        uint32_t run0 = s_boff[toff];
         uint32_t b0 = s_boff[i], b1 = s_boff[i + 1];
         int len = (int)(b1 - b0);
-        // byte position inside the partition of this row's end
-        int64_t part_byte0 = p.byte_start[(int64_t)j_bin * (P + 1) + pid];
-        int64_t rel_start = s_bbase[pid] - part_byte0 + (b0 - run0);
-        int64_t row_in_part = s_pbase[pid] - p.row_start[pid] + k;
-        ooff[p.row_start[pid] + pid + row_in_part + 1] = (int32_t)(rel_start + len);
-        copy_bytes(odata + part_byte0 + rel_start, sdata + s_src[i], len);
+        // offsets of partition pid start at index row_start + pid; this row is
+        // (s_pbase - row_start) + (i - toff) rows into the partition
+        ooff[s_pbase[pid] + pid + 1 + (i - (int)toff)] = (int32_t)(s_brel[pid] + (b0 - run0) + len);
+        copy_bytes(odata + s_bbase[pid] + (b0 - run0), src + s_src[i], len);
       }
       __syncthreads();
     }
@@ -505,15 +553,7 @@ __global__ __launch_bounds__(BLOCK) void split_scatter_kernel(SplitParams p) {
 }
 
 // Host-side LDS bytes of the scatter kernel.
-static size_t scatter_lds(int TILE, int BLOCK, int P) {
-  int NW = BLOCK / kWave;
-  size_t b = 8 * (NW + 1) + 8 * P + 8 * P + 4 * (P + 1) + 2 * NW * P;
-  b = (b + 15) & ~(size_t)15;
-  b += 2 * TILE;
-  b = (b + 15) & ~(size_t)15;
-  size_t stage = std::max<size_t>((size_t)TILE * 8, (size_t)TILE * 4 * 2 + 4);
-  return b + stage;
-}
+static size_t scatter_lds(int TILE, int BLOCK, int P) { return scatter_layout(TILE, BLOCK, P).total; }
 
 }  // namespace gz
 
@@ -633,7 +673,7 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
   h->types_in.assign(col_types, col_types + num_cols);
   if (mode == 0) h->keys.assign(key_cols, key_cols + num_keys);
   h->opts = o;
-  h->tile_rows = o.tile_rows > 0 ? o.tile_rows : 4096;
+  h->tile_rows = o.tile_rows > 0 ? o.tile_rows : 2048;
   if (h->tile_rows != 4096 && h->tile_rows != 2048 && h->tile_rows != 1024) h->tile_rows = 4096;
   SplitParams& b = h->base;
   memset(&b, 0, sizeof(b));
@@ -661,6 +701,7 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
     }
     b.out_col[c] = oc++;
     h->out_types.push_back(t);
+    if (type_is_binary(t) && nbin == 0) b.bin_col0 = c;
     b.bin[c] = type_is_binary(t) ? nbin++ : -1;
   }
   b.nbin = nbin;
@@ -836,10 +877,16 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     if (h->out_types[oc] == GZ_TYPE_BOOL) fill(seg->values[oc].p, nwords, 0xFFFFFFFFu);
   }
   const int64_t grid = (T + 7) / 8 * 8;
+  // hot path: hash of one int64 key that has no validity buffer in this call
+  const int keyk = (h->mode == 0 && prm.nkeys == 1 && prm.key_type[0] == GZ_TYPE_INT64 &&
+                    !has_validity[prm.key_col[0]]) ? 1 : 0;
   if (T > 0) {
     KernelTimer kt(ctx, "split_count");
     size_t lds = sizeof(uint32_t) * P * (1 + nbin);
-    hipLaunchKernelGGL(split_count_kernel<256>, dim3((unsigned)grid), dim3(256), lds, st, prm);
+    if (keyk)
+      hipLaunchKernelGGL((split_count_kernel<256, 1>), dim3((unsigned)grid), dim3(256), lds, st, prm);
+    else
+      hipLaunchKernelGGL((split_count_kernel<256, 0>), dim3((unsigned)grid), dim3(256), lds, st, prm);
   }
   {
     KernelTimer kt(ctx, "split_scan");
@@ -882,12 +929,12 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   if (T > 0) {
     KernelTimer kt(ctx, "split_scatter");
     size_t lds = scatter_lds(TILE, 256, P);
-    if (TILE == 4096)
-      hipLaunchKernelGGL((split_scatter_kernel<4096, 256>), dim3((unsigned)grid), dim3(256), lds, st, prm);
-    else if (TILE == 2048)
-      hipLaunchKernelGGL((split_scatter_kernel<2048, 256>), dim3((unsigned)grid), dim3(256), lds, st, prm);
-    else
-      hipLaunchKernelGGL((split_scatter_kernel<1024, 256>), dim3((unsigned)grid), dim3(256), lds, st, prm);
+#define GZ_SCATTER(TL, KK) \
+  hipLaunchKernelGGL((split_scatter_kernel<TL, 256, KK>), dim3((unsigned)grid), dim3(256), lds, st, prm)
+    if (TILE == 4096) { if (keyk) GZ_SCATTER(4096, 1); else GZ_SCATTER(4096, 0); }
+    else if (TILE == 2048) { if (keyk) GZ_SCATTER(2048, 1); else GZ_SCATTER(2048, 0); }
+    else { if (keyk) GZ_SCATTER(1024, 1); else GZ_SCATTER(1024, 0); }
+#undef GZ_SCATTER
   }
   GZ_HIP(hipGetLastError());
   h->segments.push_back(seg);

@@ -30,10 +30,20 @@ def host_values(batch, c, rows):
 
 
 def check_output(got, exp_cols):
-    for (t, vals, ok), (ev, eok) in zip(got, exp_cols):
-        assert list(ok) == eok
+    """Column-by-column comparison that reports the first difference (a plain
+    list assert makes pytest diff long lists for minutes)."""
+    for c, ((t, vals, ok), (ev, eok)) in enumerate(zip(got, exp_cols)):
+        ok = np.asarray(ok, bool)
+        eok = np.asarray(eok, bool)
+        if len(ok) != len(eok):
+            raise AssertionError(f"column {c}: {len(ok)} rows, expected {len(eok)}")
+        bad = np.nonzero(ok != eok)[0]
+        if len(bad):
+            raise AssertionError(f"column {c}: validity differs at {bad[:10].tolist()}")
         gv = [v.item() if hasattr(v, "item") else v for v in vals]
-        assert [g for g, o in zip(gv, eok) if o] == [e for e, o in zip(ev, eok) if o]
+        for i in np.nonzero(eok)[0]:
+            if gv[i] != ev[i]:
+                raise AssertionError(f"column {c} row {i}: {gv[i]!r} != {ev[i]!r}")
 
 
 def run_case(ctx, build, probe, btypes, ptypes, bkey, pkey, join_type, null_aware, result, batched,
