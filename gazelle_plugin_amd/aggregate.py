@@ -170,6 +170,7 @@ class HashAggregateResultIterator:
     def results(self, inst: int = 0) -> List[Tuple[int, np.ndarray, np.ndarray]]:
         """All result columns of an instance: (type, values, valid) per column."""
         self.finish()
+        self.k.ctx.synchronize()
         g = self.num_groups(inst)
         ncol = L.lib().gz_agg_num_result_columns(self._h)
         out = []

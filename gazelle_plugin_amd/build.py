@@ -14,7 +14,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # ordinary float code.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-Wno-unused-result", "-Wno-unused-value", "-D__HIP_PLATFORM_AMD__"]
+         "-Wno-unused-result", "-Wno-unused-value"]
 
 
 def build(verbose: bool = False) -> str:

@@ -59,9 +59,23 @@ __device__ __forceinline__ int pmod_hash(uint32_t h, const FastMod& f) {
   return r < 0 ? r + (int)f.P : r;
 }
 
+// ---- global memory ------------------------------------------------------------
+// Pointers read from LDS or from pointer tables are generic to the compiler, so
+// it emits flat instructions, which count against both the vector-memory and the
+// LDS counters and make LDS traffic wait on HBM loads.  These go through the
+// global address space instead.
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
+
 // ---- bits ---------------------------------------------------------------------
 __device__ __forceinline__ int bit_get(const uint8_t* bm, int64_t i) {
-  return (bm[i >> 3] >> (i & 7)) & 1;
+  return (gld(bm + (i >> 3)) >> (i & 7)) & 1;
 }
 
 // ---- wave helpers -------------------------------------------------------------

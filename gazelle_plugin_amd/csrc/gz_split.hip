@@ -5,17 +5,22 @@
 // of every partition contiguous and in input order, which is the order the
 // reference's DoSplit appends them in (splitter.cc:855-938).
 //
-// Three launches per segment:
-//   split_count   one workgroup per tile: partition id of every row (Spark
-//                 Murmur3 hashLong seed 42 + pmod, or rr / range / single), an
-//                 LDS histogram of rows and of utf8 bytes per partition, null
-//                 counts.  Reads the key (and utf8 offsets) only.
-//   split_scan    exclusive scan of the per-tile counts, partition-major, so each
-//                 tile knows where its run of partition p starts.
-//   split_scatter one workgroup per tile: a stable in-tile rank per partition from
-//                 wave ballots, an LDS permutation into partition order, then
-//                 coalesced writes of each column's runs; validity and boolean bits
-//                 are cleared with atomic AND in bitmaps pre-filled with ones.
+// Launches per segment:
+//   split_count    one workgroup per tile: partition id of every row (Spark
+//                  Murmur3 hashLong seed 42 + pmod, or rr / range / single), an
+//                  LDS histogram of rows and of utf8 bytes per partition, and the
+//                  null count of every nullable column.  Reads the key (and utf8
+//                  offsets, validity) only.  Writes one row of R counters per
+//                  tile (tile-major, so the row is one contiguous store).
+//   split_scan_*   exclusive scan over tiles of every counter (three small
+//                  launches: chunk sums, scan of chunk sums, per-tile prefixes).
+//   split_layout   partition starts (rows and utf8 bytes) from the totals.
+//   split_scatter  one 1024-thread workgroup per tile: a stable in-tile rank per
+//                  partition from wave ballots, kept in registers; each column is
+//                  loaded with independent coalesced loads, permuted through LDS
+//                  into partition order and written as contiguous runs; utf8
+//                  strings move with aligned dword loads; validity and boolean
+//                  bits are cleared with atomic AND in bitmaps pre-filled with ones.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +36,7 @@ namespace gz {
 constexpr int kMaxCols = 32;
 constexpr int kMaxKeys = 4;
 constexpr int kMaxPartitions = 2048;
+constexpr int kScanChunk = 64;  // tiles per chunk of the tile scan
 
 enum ColKind : int { K_NA = 0, K_FIXED = 1, K_BOOL = 2, K_BINARY = 3 };
 
@@ -59,31 +65,28 @@ struct SplitParams {
   int width[kMaxCols];
   int out_col[kMaxCols];     // output column of an input column, -1 if dropped
   int bin[kMaxCols];         // binary column ordinal, -1 otherwise
+  int nullc[kMaxCols];       // null-count slot of a nullable column, -1 otherwise
   int nbin;
   int bin_col0;              // input column of the first binary column
-  // per-segment buffers
-  uint32_t* counts;          // [P][T]
-  uint32_t* prefix;          // [P][T] exclusive within partition
-  uint32_t* bytes;           // [nbin][P][T]
-  uint32_t* bprefix;         // [nbin][P][T]
+  // per-segment buffers.  A tile's counter row has R = P * (1 + nbin) + nnull
+  // entries: rows per partition, utf8 bytes per (binary column, partition),
+  // nulls per nullable column.
+  int R;
+  uint32_t* counts;          // [T][R]
+  uint32_t* prefix;          // [T][R] exclusive over tiles
   const int64_t* row_start;  // [P+1]
   const int64_t* byte_start; // [nbin][P+1]
   void* out_values[kMaxCols];
   uint32_t* out_valid[kMaxCols];
   int32_t* out_off[kMaxCols];
   uint8_t* out_data[kMaxCols];
-  unsigned long long* null_counts;  // [ncols]
   int* err;
-};
-
-// Buffer pointers of the tile's batch, staged in LDS once per workgroup.
-struct TilePtrs {
-  const void* v[kMaxCols * 3];
+  int cbuf;                  // LDS bytes for partition-ordered utf8 staging
 };
 
 __device__ __forceinline__ void load_tile_ptrs(const SplitParams& p, int batch, const void** s_bp) {
   for (int i = threadIdx.x; i < p.ncols * 3; i += blockDim.x)
-    s_bp[i] = p.bufs[(int64_t)batch * p.bstride + i];
+    s_bp[i] = gld(&p.bufs[(int64_t)batch * p.bstride + i]);
 }
 
 // Spark hash of key column c at row r, chained through h (Murmur3Hash semantics:
@@ -93,21 +96,21 @@ __device__ __forceinline__ uint32_t hash_key(const void* const* bp, int c, int t
   if (vb && !bit_get(vb, r)) return h;
   const void* v = bp[c * 3 + 2];
   switch (t) {
-    case GZ_TYPE_INT64: return spark_hash_long((uint64_t)((const int64_t*)v)[r], h);
+    case GZ_TYPE_INT64: return spark_hash_long((uint64_t)gld((const int64_t*)v + r), h);
     case GZ_TYPE_INT32:
-    case GZ_TYPE_DATE32: return spark_hash_int((uint32_t)((const int32_t*)v)[r], h);
+    case GZ_TYPE_DATE32: return spark_hash_int((uint32_t)gld((const int32_t*)v + r), h);
     case GZ_TYPE_BOOL: return spark_hash_int(bit_get((const uint8_t*)v, r), h);
-    case GZ_TYPE_INT8: return spark_hash_int((uint32_t)(int32_t)((const int8_t*)v)[r], h);
-    case GZ_TYPE_INT16: return spark_hash_int((uint32_t)(int32_t)((const int16_t*)v)[r], h);
+    case GZ_TYPE_INT8: return spark_hash_int((uint32_t)(int32_t)gld((const int8_t*)v + r), h);
+    case GZ_TYPE_INT16: return spark_hash_int((uint32_t)(int32_t)gld((const int16_t*)v + r), h);
     case GZ_TYPE_FLOAT: {
-      float f = ((const float*)v)[r];
+      float f = gld((const float*)v + r);
       uint32_t bits;
       if (f == 0.0f) f = 0.0f;
       if (f != f) bits = 0x7fc00000u; else bits = __float_as_uint(f);
       return spark_hash_int(bits, h);
     }
     case GZ_TYPE_DOUBLE: {
-      double d = ((const double*)v)[r];
+      double d = gld((const double*)v + r);
       uint64_t bits;
       if (d == 0.0) d = 0.0;
       if (d != d) bits = 0x7ff8000000000000ull; else bits = (uint64_t)__double_as_longlong(d);
@@ -116,31 +119,26 @@ __device__ __forceinline__ uint32_t hash_key(const void* const* bp, int c, int t
     case GZ_TYPE_STRING:
     case GZ_TYPE_BINARY: {
       const int32_t* off = (const int32_t*)bp[c * 3 + 1];
-      const uint8_t* d = (const uint8_t*)v + off[r];
-      int len = off[r + 1] - off[r];
+      int32_t o0 = gld(off + r), o1 = gld(off + r + 1);
+      const uint8_t* d = (const uint8_t*)v + o0;
+      int len = o1 - o0;
       int aligned = len & ~3;
       for (int i = 0; i < aligned; i += 4) {
-        uint32_t w = (uint32_t)d[i] | ((uint32_t)d[i + 1] << 8) | ((uint32_t)d[i + 2] << 16) |
-                     ((uint32_t)d[i + 3] << 24);
+        uint32_t w = (uint32_t)gld(d + i) | ((uint32_t)gld(d + i + 1) << 8) |
+                     ((uint32_t)gld(d + i + 2) << 16) | ((uint32_t)gld(d + i + 3) << 24);
         h = mix_h1(h, mix_k1(w));
       }
-      for (int i = aligned; i < len; i++) h = mix_h1(h, mix_k1((uint32_t)(int32_t)(int8_t)d[i]));
+      for (int i = aligned; i < len; i++) h = mix_h1(h, mix_k1((uint32_t)(int32_t)(int8_t)gld(d + i)));
       return fmix32(h, (uint32_t)len);
     }
     default: return h;
   }
 }
 
-// Partition id of row lr of a tile.  KEYK = 1: one int64 key without
-// validity (the hot case), else the generic path.
-template <int KEYK>
+// Partition id of row lr of a tile (generic path: any keys, any mode).
 __device__ __forceinline__ int compute_pid(const SplitParams& p, const void* const* bp, const TileDesc& td,
                                            int lr) {
   int64_t r = (int64_t)td.row_begin + lr;
-  if (KEYK == 1) {
-    uint64_t k = (uint64_t)((const int64_t*)bp[p.key_col[0] * 3 + 2])[r];
-    return pmod_hash(spark_hash_long(k, p.seed), p.fm);
-  }
   if (p.mode == 0) {
     uint32_t h = p.seed;
     for (int k = 0; k < p.nkeys; k++) h = hash_key(bp, p.key_col[k], p.key_type[k], r, h);
@@ -148,7 +146,7 @@ __device__ __forceinline__ int compute_pid(const SplitParams& p, const void* con
   }
   if (p.mode == 1) return (int)fastmod_u32((uint32_t)(td.rr + lr), p.fm);
   if (p.mode == 2) {
-    int v = ((const int32_t*)bp[2])[r];
+    int v = gld((const int32_t*)bp[2] + r);
     if (v < 0 || v >= p.P) {
       atomicOr(p.err, 1);
       return 0;
@@ -159,22 +157,22 @@ __device__ __forceinline__ int compute_pid(const SplitParams& p, const void* con
 }
 
 // ---------------------------------------------------------------- pass 1
+// KEYK = 1: one int64 key without validity (the hot case), else generic.
 template <int BLOCK, int KEYK>
 __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
-  extern __shared__ uint32_t s_cnt[];  // [P] rows, then [nbin][P] bytes
+  extern __shared__ uint32_t s_cnt[];  // [R]
   __shared__ const void* s_bp[kMaxCols * 3];
   int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
   if (t < 0) return;
   const TileDesc td = p.tiles[t];
   const int P = p.P;
   load_tile_ptrs(p, td.batch, s_bp);
-  for (int i = threadIdx.x; i < P * (1 + p.nbin); i += BLOCK) s_cnt[i] = 0;
+  for (int i = threadIdx.x; i < p.R; i += BLOCK) s_cnt[i] = 0;
   __syncthreads();
-  if (KEYK == 1 && p.nbin == 1) {
-    // hot path: int64 key, one utf8 column; four independent rows per step
+  if (KEYK == 1) {
+    // four independent rows per step
     const int64_t* key = (const int64_t*)s_bp[p.key_col[0] * 3 + 2] + td.row_begin;
-    int bc = p.bin_col0;
-    const int32_t* off = (const int32_t*)s_bp[bc * 3 + 1] + td.row_begin;
+    const int32_t* off = p.nbin == 1 ? (const int32_t*)s_bp[p.bin_col0 * 3 + 1] + td.row_begin : nullptr;
     for (int lr0 = threadIdx.x; lr0 < td.nrows; lr0 += 4 * BLOCK) {
       uint64_t k[4];
       int32_t o0[4], o1[4];
@@ -182,22 +180,32 @@ __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
       for (int u = 0; u < 4; u++) {
         int lr = lr0 + u * BLOCK;
         bool a = lr < td.nrows;
-        k[u] = a ? (uint64_t)key[lr] : 0;
-        o0[u] = a ? off[lr] : 0;
-        o1[u] = a ? off[lr + 1] : 0;
+        k[u] = a ? (uint64_t)gld(key + lr) : 0;
+        o0[u] = a && off ? gld(off + lr) : 0;
+        o1[u] = a && off ? gld(off + lr + 1) : 0;
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        if (lr0 + u * BLOCK < td.nrows) {
+        int lr = lr0 + u * BLOCK;
+        if (lr < td.nrows) {
           int pid = pmod_hash(spark_hash_long(k[u], p.seed), p.fm);
           atomicAdd(&s_cnt[pid], 1u);
-          atomicAdd(&s_cnt[P + pid], (uint32_t)(o1[u] - o0[u]));
+          if (off) atomicAdd(&s_cnt[P + pid], (uint32_t)(o1[u] - o0[u]));
+          else if (p.nbin) {
+            int64_t r = (int64_t)td.row_begin + lr;
+            for (int c = 0; c < p.ncols; c++) {
+              int j = p.bin[c];
+              if (j < 0) continue;
+              const int32_t* of = (const int32_t*)s_bp[c * 3 + 1];
+              atomicAdd(&s_cnt[P * (1 + j) + pid], (uint32_t)(gld(of + r + 1) - gld(of + r)));
+            }
+          }
         }
       }
     }
   } else {
     for (int lr = threadIdx.x; lr < td.nrows; lr += BLOCK) {
-      int pid = compute_pid<KEYK>(p, s_bp, td, lr);
+      int pid = compute_pid(p, s_bp, td, lr);
       atomicAdd(&s_cnt[pid], 1u);
       if (p.nbin) {
         int64_t r = (int64_t)td.row_begin + lr;
@@ -205,7 +213,7 @@ __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
           int j = p.bin[c];
           if (j < 0) continue;
           const int32_t* off = (const int32_t*)s_bp[c * 3 + 1];
-          atomicAdd(&s_cnt[P * (1 + j) + pid], (uint32_t)(off[r + 1] - off[r]));
+          atomicAdd(&s_cnt[P * (1 + j) + pid], (uint32_t)(gld(off + r + 1) - gld(off + r)));
         }
       }
     }
@@ -213,60 +221,84 @@ __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
   // null counts: zero bits of each validity bitmap over the tile's rows
   // (tiles start on multiples of 8 rows inside a batch).
   for (int c = 0; c < p.ncols; c++) {
-    if (p.out_col[c] < 0 || p.kind[c] == K_NA) continue;
+    int slot = p.nullc[c];
+    if (slot < 0) continue;
     const uint8_t* vb = (const uint8_t*)s_bp[c * 3 + 0];
     if (!vb) continue;
     uint32_t nulls = 0;
     int nbytes = (td.nrows + 7) >> 3;
     const uint8_t* base = vb + (td.row_begin >> 3);
     for (int i = threadIdx.x; i < nbytes; i += BLOCK) {
-      uint32_t b = base[i];
+      uint32_t b = gld(base + i);
       int valid_bits = min(8, td.nrows - i * 8);
       uint32_t mask = (1u << valid_bits) - 1u;
       nulls += __popc((~b) & mask);
     }
     nulls = wave_reduce_sum(nulls);
-    if (__lane_id() == 0 && nulls) atomicAdd(&p.null_counts[p.out_col[c]], (unsigned long long)nulls);
+    if (__lane_id() == 0 && nulls) atomicAdd(&s_cnt[slot], nulls);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < P; i += BLOCK) p.counts[(int64_t)i * p.T + t] = s_cnt[i];
-  for (int j = 0; j < p.nbin; j++)
-    for (int i = threadIdx.x; i < P; i += BLOCK)
-      p.bytes[((int64_t)j * P + i) * p.T + t] = s_cnt[P * (1 + j) + i];
+  uint32_t* row = p.counts + t * p.R;
+  for (int i = threadIdx.x; i < p.R; i += BLOCK) gst(row + i, s_cnt[i]);
 }
 
 // ---------------------------------------------------------------- scan
-// One block per row (partition, or (binary column, partition)): exclusive scan
-// over T tiles; the row total goes to tot[row].
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void split_scan_kernel(const uint32_t* in, uint32_t* out,
-                                                            uint64_t* tot, int64_t T) {
-  __shared__ uint64_t s_scan[BLOCK / kWave + 1];
-  int64_t row = blockIdx.x;
-  const uint32_t* a = in + row * T;
-  uint32_t* o = out + row * T;
-  uint64_t carry = 0;
-  for (int64_t base = 0; base < T; base += (int64_t)BLOCK * 8) {
-    // each thread scans 8 consecutive entries
-    int64_t i0 = base + (int64_t)threadIdx.x * 8;
-    uint32_t v[8];
+// Exclusive scan over tiles of every counter of the [T][R] count matrix, in
+// three launches over chunks of kScanChunk tiles: chunk sums, a scan of the chunk
+// sums (which also gives the totals), then the per-tile prefixes.  Every access
+// walks tile rows, which are contiguous.
+__global__ __launch_bounds__(256) void split_scan_sum_kernel(const uint32_t* counts, int R, int64_t T,
+                                                               uint64_t* csum) {
+  int64_t c = blockIdx.x;
+  int64_t t0 = c * kScanChunk, t1 = min<int64_t>(T, t0 + kScanChunk);
+  for (int e = threadIdx.x; e < R; e += 256) {
     uint64_t s = 0;
+    for (int64_t tb = t0; tb < t1; tb += 16) {
+      uint32_t v[16];  // sixteen independent loads in flight
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      v[k] = (i0 + k < T) ? a[i0 + k] : 0u;
-      s += v[k];
+      for (int k = 0; k < 16; k++) v[k] = tb + k < t1 ? gld(counts + (tb + k) * R + e) : 0u;
+#pragma unroll
+      for (int k = 0; k < 16; k++) s += v[k];
     }
-    uint64_t total;
-    uint64_t ex = block_exclusive_scan<uint64_t>(s, s_scan, &total);
-    uint64_t run = carry + ex;
+    csum[c * R + e] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void split_scan_chunks_kernel(uint64_t* csum, int R, int64_t C,
+                                                                  uint64_t* tot) {
+  int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= R) return;
+  uint64_t run = 0;
+  for (int64_t cb = 0; cb < C; cb += 16) {
+    uint64_t v[16];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      if (i0 + k < T) o[i0 + k] = (uint32_t)run;
+    for (int k = 0; k < 16; k++) v[k] = cb + k < C ? csum[(cb + k) * R + e] : 0ull;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      if (cb + k < C) csum[(cb + k) * R + e] = run;
       run += v[k];
     }
-    carry += total;
   }
-  if (threadIdx.x == 0) tot[row] = carry;
+  tot[e] = run;
+}
+
+__global__ __launch_bounds__(256) void split_scan_write_kernel(const uint32_t* counts, const uint64_t* cpre,
+                                                                 int R, int64_t T, uint32_t* prefix) {
+  int64_t c = blockIdx.x;
+  int64_t t0 = c * kScanChunk, t1 = min<int64_t>(T, t0 + kScanChunk);
+  for (int e = threadIdx.x; e < R; e += 256) {
+    uint64_t run = cpre[c * R + e];
+    for (int64_t tb = t0; tb < t1; tb += 16) {
+      uint32_t v[16];
+#pragma unroll
+      for (int k = 0; k < 16; k++) v[k] = tb + k < t1 ? gld(counts + (tb + k) * R + e) : 0u;
+#pragma unroll
+      for (int k = 0; k < 16; k++) {
+        if (tb + k < t1) gst(prefix + (tb + k) * R + e, (uint32_t)run);
+        run += v[k];
+      }
+    }
+  }
 }
 
 // One block: exclusive scans of the partition totals into row_start / byte_start,
@@ -305,27 +337,101 @@ __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
 }
 
 // ---------------------------------------------------------------- pass 2
-// Copy len bytes: byte head to a 4-byte aligned destination, 4-byte stores in
-// the middle (unaligned 4-byte loads), byte tail.  Exactly [dst, dst+len) is
-// written, so neighbouring strings owned by other threads are untouched.
-__device__ __forceinline__ void copy_bytes(uint8_t* dst, const uint8_t* src, int len) {
-  int i = 0;
-  int head = (int)((4 - ((uintptr_t)dst & 3)) & 3);
-  if (head > len) head = len;
-  for (; i < head; i++) dst[i] = src[i];
-  for (; i + 4 <= len; i += 4) {
-    uint32_t w;
-    __builtin_memcpy(&w, src + i, 4);
-    *(uint32_t*)(dst + i) = w;
+__device__ __forceinline__ uint32_t funnel(uint32_t lo, uint32_t hi, uint32_t sh) {
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * sh));
+}
+
+// Byte / dword stores to HBM or to LDS.
+struct HbmBytes {
+  uint8_t* p;
+  __device__ __forceinline__ void b(int i, uint8_t v) const { gst(p + i, v); }
+  __device__ __forceinline__ void d(int i, uint32_t v) const { gst((uint32_t*)(p + i), v); }
+};
+struct LdsBytes {
+  uint8_t* p;
+  __device__ __forceinline__ void b(int i, uint8_t v) const { p[i] = v; }
+  __device__ __forceinline__ void d(int i, uint32_t v) const { *(uint32_t*)(p + i) = v; }
+};
+
+// Copy one string of len bytes from HBM.  Source bytes are read as the aligned
+// dwords that contain them (an aligned dword holding a byte of the buffer never
+// crosses a page boundary): up to nine independent loads for strings of at most
+// 32 bytes.  The destination gets byte stores up to its first 4-byte boundary,
+// dword stores, then byte stores for the tail, so exactly [dst, dst+len) is
+// written.
+template <class D>
+__device__ __forceinline__ void copy_string(D dst, const uint8_t* src, int len) {
+  if (len <= 0) return;
+  const int hb = min(len, (int)((4 - ((uintptr_t)dst.p & 3)) & 3));  // head bytes
+  if (len <= 32) {
+    // scalar registers rather than arrays, so nothing can end up in scratch
+    const uint32_t* a = (const uint32_t*)((uintptr_t)src & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)((uintptr_t)src & 3);
+    const int ndw = (int)((sh + len + 3) >> 2);  // <= 9
+#define GZ_W(k) const uint32_t w##k = (k) < ndw ? gld(a + (k)) : 0u;
+    GZ_W(0) GZ_W(1) GZ_W(2) GZ_W(3) GZ_W(4) GZ_W(5) GZ_W(6) GZ_W(7) GZ_W(8)
+#undef GZ_W
+    const uint32_t w9 = 0;
+    // head: string bytes [0, hb)
+    const uint32_t h0 = funnel(w0, w1, sh);
+    for (int q = 0; q < hb; q++) dst.b(q, (uint8_t)(h0 >> (8 * q)));
+    // body: dword m holds string bytes [hb + 4m, hb + 4m + 4), the source dword
+    // stream from byte sh + hb
+    const uint32_t s0 = sh + (uint32_t)hb;
+    const uint32_t r = s0 & 3, b = s0 >> 2;  // b <= 1
+#define GZ_U(k, k1) const uint32_t u##k = funnel(w##k, w##k1, r);
+    GZ_U(0, 1) GZ_U(1, 2) GZ_U(2, 3) GZ_U(3, 4) GZ_U(4, 5) GZ_U(5, 6) GZ_U(6, 7) GZ_U(7, 8) GZ_U(8, 9)
+#undef GZ_U
+    const int body = len - hb;
+    const int nfull = body >> 2;
+    uint32_t tail = 0;
+#define GZ_S(m, m1)                              \
+  {                                              \
+    const uint32_t x = b ? u##m1 : u##m;         \
+    if ((m) < nfull) dst.d(hb + 4 * (m), x);     \
+    if ((m) == nfull) tail = x;                  \
   }
-  for (; i < len; i++) dst[i] = src[i];
+    GZ_S(0, 1) GZ_S(1, 2) GZ_S(2, 3) GZ_S(3, 4) GZ_S(4, 5) GZ_S(5, 6) GZ_S(6, 7) GZ_S(7, 8)
+#undef GZ_S
+    const int tb = body & 3;
+    const int t0 = hb + 4 * nfull;
+    for (int q = 0; q < tb; q++) dst.b(t0 + q, (uint8_t)(tail >> (8 * q)));
+    return;
+  }
+  int i = 0;
+  for (; i < hb; i++) dst.b(i, gld(src + i));
+  for (; i + 4 <= len; i += 4) {
+    uint32_t x = (uint32_t)gld(src + i) | ((uint32_t)gld(src + i + 1) << 8) |
+                 ((uint32_t)gld(src + i + 2) << 16) | ((uint32_t)gld(src + i + 3) << 24);
+    dst.d(i, x);
+  }
+  for (; i < len; i++) dst.b(i, gld(src + i));
+}
+
+// One wave writes bytes [r0, r0 + len) of an LDS buffer to dst in HBM: byte
+// stores up to the first 4-byte boundary of dst, then one dword per lane per
+// step (contiguous across the wave), then the tail bytes.  The buffer holds 8
+// readable bytes past its end.
+__device__ __forceinline__ void write_run(uint8_t* dst, const uint8_t* buf, uint32_t r0, int len, int lane) {
+  const int head = min(len, (int)((4 - ((uintptr_t)dst & 3)) & 3));
+  if (lane < head) gst(dst + lane, buf[r0 + lane]);
+  const int body = len - head;
+  const int nd = body >> 2;
+  const uint32_t sb = r0 + (uint32_t)head;
+  const uint32_t sh = sb & 3;
+  const uint32_t* b32 = (const uint32_t*)buf + (sb >> 2);
+  uint32_t* d32 = (uint32_t*)(dst + head);
+  for (int m = lane; m < nd; m += kWave) gst(d32 + m, funnel(b32[m], b32[m + 1], sh));
+  const int tb = body & 3;
+  if (lane < tb) gst((uint8_t*)(d32 + nd) + lane, buf[sb + 4 * nd + lane]);
 }
 
 // LDS layout of the scatter kernel (bytes from the start of dynamic LDS).
 struct ScatterLds {
-  size_t scan, pbase, bbase, brel, toff, wcnt, bp, pid, pos, spid, stage, total;
+  size_t scan, pbase, bbase, brel, toff, wcnt, bp, spid, pos, stage, cbuf, total;
 };
-__host__ __device__ inline ScatterLds scatter_layout(int TILE, int BLOCK, int P) {
+// cbuf: bytes of the partition-ordered utf8 buffer (0 without binary columns).
+__host__ __device__ inline ScatterLds scatter_layout(int TILE, int BLOCK, int P, int cbuf) {
   ScatterLds L;
   size_t o = 0;
   auto take = [&](size_t bytes, size_t align) {
@@ -336,25 +442,29 @@ __host__ __device__ inline ScatterLds scatter_layout(int TILE, int BLOCK, int P)
   };
   int NW = BLOCK / 64;
   L.scan = take(8 * (NW + 1), 8);
-  L.pbase = take(8 * (size_t)P, 8);
+  L.pbase = take(4 * (size_t)P, 4);
   L.bbase = take(8 * (size_t)P, 8);
   L.brel = take(4 * (size_t)P, 4);
   L.toff = take(4 * (size_t)(P + 1), 4);
   L.wcnt = take(2 * (size_t)NW * P, 4);
   L.bp = take(8 * (size_t)kMaxCols * 3, 8);
-  L.pid = take(2 * (size_t)TILE, 4);
-  L.pos = take(2 * (size_t)TILE, 4);
   L.spid = take(2 * (size_t)TILE, 4);
-  L.stage = take((size_t)TILE * 8 + 4, 16);
+  L.pos = take(2 * (size_t)TILE, 4);
+  L.stage = take((size_t)TILE * 8 + 8, 16);
+  L.cbuf = take(cbuf > 0 ? (size_t)cbuf + 8 : 0, 16);
   L.total = o;
   return L;
 }
 
+// Rows of a tile are owned in the ranking, loading and validity steps by
+// thread (wave w, lane l) as rows w*RW + j*64 + l, j < RPT (coalesced per wave);
+// sorted positions are owned in the writing steps as j*BLOCK + tid.
 template <int TILE, int BLOCK, int KEYK>
-__global__ __launch_bounds__(BLOCK) void split_scatter_kernel(SplitParams p) {
+__global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatter_kernel(SplitParams p) {
   constexpr int NW = BLOCK / kWave;
-  constexpr int RW = TILE / NW;  // rows per wave in the ranking pass
-  static_assert(RW % kWave == 0, "tile must be a multiple of the block");
+  constexpr int RPT = TILE / BLOCK;  // rows per thread
+  constexpr int RW = TILE / NW;      // rows per wave = RPT * 64
+  static_assert(RW == RPT * kWave, "tile must be a multiple of the block");
   extern __shared__ __align__(16) uint8_t smem[];
   int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
   if (t < 0) return;
@@ -362,83 +472,118 @@ __global__ __launch_bounds__(BLOCK) void split_scatter_kernel(SplitParams p) {
   const int n = td.nrows;
   const int P = p.P;
   const int tid = threadIdx.x, w = tid / kWave, lane = __lane_id();
-  const ScatterLds Lo = scatter_layout(TILE, BLOCK, P);
+  const ScatterLds Lo = scatter_layout(TILE, BLOCK, P, p.cbuf);
   uint64_t* s_scan = (uint64_t*)(smem + Lo.scan);
-  int64_t* s_pbase = (int64_t*)(smem + Lo.pbase);
+  uint32_t* s_pbase = (uint32_t*)(smem + Lo.pbase);
   int64_t* s_bbase = (int64_t*)(smem + Lo.bbase);
   uint32_t* s_brel = (uint32_t*)(smem + Lo.brel);
   uint32_t* s_toff = (uint32_t*)(smem + Lo.toff);
   uint16_t* s_wcnt = (uint16_t*)(smem + Lo.wcnt);
   const void** s_bp = (const void**)(smem + Lo.bp);
-  uint16_t* s_pid = (uint16_t*)(smem + Lo.pid);
-  uint16_t* s_pos = (uint16_t*)(smem + Lo.pos);
   uint16_t* s_spid = (uint16_t*)(smem + Lo.spid);
+  uint16_t* s_pos = (uint16_t*)(smem + Lo.pos);  // sorted slot of every tile row
   uint8_t* s_stage = smem + Lo.stage;
+  uint8_t* s_cbuf = smem + Lo.cbuf;
+  const uint32_t* trow = p.prefix + t * p.R;  // this tile's exclusive prefixes
 
   load_tile_ptrs(p, td.batch, s_bp);
   for (int i = tid; i < NW * P; i += BLOCK) s_wcnt[i] = 0;
+  // first destination row of every partition run of this tile (issued early so
+  // the loads overlap the ranking)
+  for (int pp = tid; pp < P; pp += BLOCK) s_pbase[pp] = (uint32_t)(gld(p.row_start + pp) + gld(trow + pp));
   __syncthreads();
 
-  // ---- 1: partition id and stable rank of every row.  Wave w ranks rows
-  // [w*RW, (w+1)*RW) in chunks of 64 consecutive rows.
-  for (int j = 0; j < RW / kWave; j++) {
-    int lr = w * RW + j * kWave + lane;
-    bool act = lr < n;
-    int pid = act ? compute_pid<KEYK>(p, s_bp, td, lr) : 0;
-    uint64_t m = match_bits((uint32_t)pid, act, p.pbits);
+  const int64_t rb = td.row_begin;
+  auto lrow = [&](int j) { return w * RW + j * kWave + lane; };
+
+  // ---- 1: partition id and stable rank within the wave's rows
+  int pid[RPT], pos[RPT];
+  if (KEYK == 1) {
+    const int64_t* key = (const int64_t*)s_bp[p.key_col[0] * 3 + 2] + rb;
+    uint64_t kv[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; j++) kv[j] = lrow(j) < n ? (uint64_t)gld(key + lrow(j)) : 0;
+#pragma unroll
+    for (int j = 0; j < RPT; j++) pid[j] = pmod_hash(spark_hash_long(kv[j], p.seed), p.fm);
+  } else {
+#pragma unroll
+    for (int j = 0; j < RPT; j++) pid[j] = lrow(j) < n ? compute_pid(p, s_bp, td, lrow(j)) : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < RPT; j++) {
+    bool act = lrow(j) < n;
+    uint64_t m = match_bits((uint32_t)pid[j], act, p.pbits);
     uint64_t lt = m & lanemask_lt();
-    int before = act ? s_wcnt[w * P + pid] : 0;
-    if (act) {
-      s_pid[lr] = (uint16_t)pid;
-      s_pos[lr] = (uint16_t)(before + __popcll(lt));
-      if (lt == 0) s_wcnt[w * P + pid] = (uint16_t)(before + __popcll(m));
-    }
+    int before = act ? s_wcnt[w * P + pid[j]] : 0;
+    pos[j] = before + __popcll(lt);
+    if (act && lt == 0) s_wcnt[w * P + pid[j]] = (uint16_t)(before + __popcll(m));
   }
   __syncthreads();
 
-  // ---- 2: per partition: wave offsets (in place), tile count, tile offsets,
-  // destination bases
+  // ---- 2: per partition: exclusive scan of the wave counts (in place; a
+  // segment of NW lanes per partition, lane k holding wave k), the tile count,
+  // then the tile offsets (exclusive scan over partitions)
+  {
+    constexpr int GPB = BLOCK / NW;  // partitions per pass
+    const int k = tid % NW, g = tid / NW;
+    for (int base = 0; base < P; base += GPB) {
+      const int pp = base + g;
+      const uint32_t x = pp < P ? s_wcnt[k * P + pp] : 0u;
+      uint32_t inc = x;
+#pragma unroll
+      for (int o = 1; o < NW; o <<= 1) {
+        uint32_t y = __shfl_up(inc, o, NW);
+        if (k >= o) inc += y;
+      }
+      if (pp < P) {
+        s_wcnt[k * P + pp] = (uint16_t)(inc - x);
+        if (k == NW - 1) s_toff[pp] = inc;  // tile count for now
+      }
+    }
+  }
+  __syncthreads();
   {
     constexpr int PPT = (kMaxPartitions + BLOCK - 1) / BLOCK;
     uint32_t sum = 0;
+    uint32_t cnt[PPT];
+#pragma unroll
     for (int k = 0; k < PPT; k++) {
       int pp = tid * PPT + k;
-      if (pp >= P) break;
-      uint32_t c = 0;
-      for (int ww = 0; ww < NW; ww++) {
-        uint32_t x = s_wcnt[ww * P + pp];
-        s_wcnt[ww * P + pp] = (uint16_t)c;
-        c += x;
-      }
-      s_toff[pp] = c;  // count for now
-      sum += c;
+      cnt[k] = pp < P ? s_toff[pp] : 0u;
+      sum += cnt[k];
     }
     uint64_t total;
     uint64_t ex = block_exclusive_scan<uint64_t>(sum, s_scan, &total);
     uint32_t run = (uint32_t)ex;
+#pragma unroll
     for (int k = 0; k < PPT; k++) {
       int pp = tid * PPT + k;
-      if (pp >= P) break;
-      uint32_t c = s_toff[pp];
-      s_toff[pp] = run;
-      s_pbase[pp] = p.row_start[pp] + p.prefix[(int64_t)pp * p.T + t];
-      run += c;
+      if (pp < P) {
+        s_toff[pp] = run;
+        run += cnt[k];
+      }
     }
     if (tid == 0) s_toff[P] = (uint32_t)total;
   }
   __syncthreads();
 
-  // ---- 3: sorted position of every row; pid of every sorted position
-  for (int lr = tid; lr < n; lr += BLOCK) {
-    int pid = s_pid[lr];
-    int spos = (int)s_toff[pid] + s_wcnt[(lr / RW) * P + pid] + s_pos[lr];
-    s_pos[lr] = (uint16_t)spos;
-    s_spid[spos] = (uint16_t)pid;
+  // ---- 3: sorted position of every owned row; partition of every sorted slot
+#pragma unroll
+  for (int j = 0; j < RPT; j++) {
+    if (lrow(j) < n) {
+      int sp = pos[j] + (int)s_toff[pid[j]] + s_wcnt[w * P + pid[j]];
+      s_spid[sp] = (uint16_t)pid[j];
+      s_pos[lrow(j)] = (uint16_t)sp;
+    }
   }
   __syncthreads();
+  // destination row (a row index of the segment) of sorted slot i
+  auto drow = [&](int i) -> uint32_t {
+    int q = s_spid[i];
+    return s_pbase[q] + (uint32_t)i - s_toff[q];
+  };
 
   // ---- 4: columns
-  const int64_t rb = td.row_begin;
   for (int c = 0; c < p.ncols; c++) {
     int oc = p.out_col[c];
     if (oc < 0) continue;
@@ -450,10 +595,10 @@ __global__ __launch_bounds__(BLOCK) void split_scatter_kernel(SplitParams p) {
     // validity: clear the bit of every null row at its destination
     if (vb && p.out_valid[oc]) {
       uint32_t* ov = p.out_valid[oc];
-      for (int lr = tid; lr < n; lr += BLOCK) {
-        if (!bit_get(vb, rb + lr)) {
-          int pid = s_pid[lr];
-          int64_t d = s_pbase[pid] + ((int)s_pos[lr] - (int)s_toff[pid]);
+#pragma unroll
+      for (int j = 0; j < RPT; j++) {
+        if (lrow(j) < n && !bit_get(vb, rb + lrow(j))) {
+          uint32_t d = drow(s_pos[lrow(j)]);
           atomicAnd(&ov[d >> 5], ~(1u << (d & 31)));
         }
       }
@@ -461,10 +606,10 @@ __global__ __launch_bounds__(BLOCK) void split_scatter_kernel(SplitParams p) {
 
     if (kind == K_BOOL) {
       uint32_t* ov = (uint32_t*)p.out_values[oc];
-      for (int lr = tid; lr < n; lr += BLOCK) {
-        if (!bit_get(src, rb + lr)) {
-          int pid = s_pid[lr];
-          int64_t d = s_pbase[pid] + ((int)s_pos[lr] - (int)s_toff[pid]);
+#pragma unroll
+      for (int j = 0; j < RPT; j++) {
+        if (lrow(j) < n && !bit_get(src, rb + lrow(j))) {
+          uint32_t d = drow(s_pos[lrow(j)]);
           atomicAnd(&ov[d >> 5], ~(1u << (d & 31)));
         }
       }
@@ -476,55 +621,90 @@ __global__ __launch_bounds__(BLOCK) void split_scatter_kernel(SplitParams p) {
       for (int part = 0; part < parts; part++) {
         if (EW == 8) {
           const uint64_t* s8 = (const uint64_t*)src;
-          for (int lr = tid; lr < n; lr += BLOCK)
-            ((uint64_t*)s_stage)[s_pos[lr]] = s8[(rb + lr) * parts + part];
+          uint64_t v[RPT];
+#pragma unroll
+          for (int j = 0; j < RPT; j++) v[j] = lrow(j) < n ? gld(s8 + (rb + lrow(j)) * parts + part) : 0;
+#pragma unroll
+          for (int j = 0; j < RPT; j++)
+            if (lrow(j) < n) ((uint64_t*)s_stage)[s_pos[lrow(j)]] = v[j];
         } else if (EW == 4) {
           const uint32_t* s4 = (const uint32_t*)src;
-          for (int lr = tid; lr < n; lr += BLOCK) ((uint32_t*)s_stage)[s_pos[lr]] = s4[rb + lr];
+          uint32_t v[RPT];
+#pragma unroll
+          for (int j = 0; j < RPT; j++) v[j] = lrow(j) < n ? gld(s4 + rb + lrow(j)) : 0;
+#pragma unroll
+          for (int j = 0; j < RPT; j++)
+            if (lrow(j) < n) ((uint32_t*)s_stage)[s_pos[lrow(j)]] = v[j];
         } else if (EW == 2) {
           const uint16_t* s2 = (const uint16_t*)src;
-          for (int lr = tid; lr < n; lr += BLOCK) ((uint16_t*)s_stage)[s_pos[lr]] = s2[rb + lr];
+          uint16_t v[RPT];
+#pragma unroll
+          for (int j = 0; j < RPT; j++) v[j] = lrow(j) < n ? gld(s2 + rb + lrow(j)) : 0;
+#pragma unroll
+          for (int j = 0; j < RPT; j++)
+            if (lrow(j) < n) ((uint16_t*)s_stage)[s_pos[lrow(j)]] = v[j];
         } else {
-          for (int lr = tid; lr < n; lr += BLOCK) s_stage[s_pos[lr]] = src[rb + lr];
+          uint8_t v[RPT];
+#pragma unroll
+          for (int j = 0; j < RPT; j++) v[j] = lrow(j) < n ? gld(src + rb + lrow(j)) : 0;
+#pragma unroll
+          for (int j = 0; j < RPT; j++)
+            if (lrow(j) < n) s_stage[s_pos[lrow(j)]] = v[j];
         }
         __syncthreads();
-        for (int i = tid; i < n; i += BLOCK) {
-          int pid = s_spid[i];
-          int64_t d = s_pbase[pid] + (i - (int)s_toff[pid]);
-          if (EW == 8) ((uint64_t*)dst)[d * parts + part] = ((uint64_t*)s_stage)[i];
-          else if (EW == 4) ((uint32_t*)dst)[d] = ((uint32_t*)s_stage)[i];
-          else if (EW == 2) ((uint16_t*)dst)[d] = ((uint16_t*)s_stage)[i];
-          else dst[d] = s_stage[i];
+#pragma unroll 1
+        for (int j = 0; j < RPT; j++) {
+          int i = j * BLOCK + tid;
+          if (i >= n) continue;
+          uint64_t d = drow(i);
+          if (EW == 8) gst((uint64_t*)dst + d * parts + part, ((uint64_t*)s_stage)[i]);
+          else if (EW == 4) gst((uint32_t*)dst + d, ((uint32_t*)s_stage)[i]);
+          else if (EW == 2) gst((uint16_t*)dst + d, ((uint16_t*)s_stage)[i]);
+          else gst(dst + d, s_stage[i]);
         }
         __syncthreads();
       }
     } else if (kind == K_BINARY) {
       const int j_bin = p.bin[c];
-      const int32_t* soff = (const int32_t*)s_bp[c * 3 + 1];
-      uint32_t* s_src = (uint32_t*)s_stage;  // [TILE] source byte offset
+      const int32_t* soff = (const int32_t*)s_bp[c * 3 + 1] + rb;
+      uint32_t* s_src = (uint32_t*)s_stage;  // [TILE] source byte offset by sorted slot
       uint32_t* s_boff = s_src + TILE;       // [TILE+1] sorted byte prefix
-      for (int lr = tid; lr < n; lr += BLOCK) {
-        int o0 = soff[rb + lr], o1 = soff[rb + lr + 1];
-        int sp = s_pos[lr];
-        s_src[sp] = (uint32_t)o0;
-        s_boff[sp] = (uint32_t)(o1 - o0);
+      {
+        int32_t o0[RPT], o1[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+          bool a = lrow(j) < n;
+          o0[j] = a ? gld(soff + lrow(j)) : 0;
+          o1[j] = a ? gld(soff + lrow(j) + 1) : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+          if (lrow(j) < n) {
+            const int sp = s_pos[lrow(j)];
+            s_src[sp] = (uint32_t)o0[j];
+            s_boff[sp] = (uint32_t)(o1[j] - o0[j]);
+          }
+        }
       }
+      const uint32_t* brow = trow + P * (1 + j_bin);
       for (int pp = tid; pp < P; pp += BLOCK) {
-        uint32_t rel = p.bprefix[((int64_t)j_bin * P + pp) * p.T + t];
+        uint32_t rel = gld(brow + pp);
         s_brel[pp] = rel;
-        s_bbase[pp] = p.byte_start[(int64_t)j_bin * (P + 1) + pp] + rel;
+        s_bbase[pp] = gld(p.byte_start + (int64_t)j_bin * (P + 1) + pp) + rel;
       }
       __syncthreads();
       {
-        constexpr int EPT = TILE / BLOCK;
+        // exclusive scan of the sorted lengths; thread tid owns RPT consecutive slots
         uint32_t s = 0;
-        int i0 = tid * EPT;
-        for (int k = 0; k < EPT; k++)
+        int i0 = tid * RPT;
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
           if (i0 + k < n) s += s_boff[i0 + k];
         uint64_t total;
         uint64_t ex = block_exclusive_scan<uint64_t>(s, s_scan, &total);
         uint32_t run = (uint32_t)ex;
-        for (int k = 0; k < EPT; k++) {
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
           if (i0 + k < n) {
             uint32_t v = s_boff[i0 + k];
             s_boff[i0 + k] = run;
@@ -536,24 +716,56 @@ __global__ __launch_bounds__(BLOCK) void split_scatter_kernel(SplitParams p) {
       __syncthreads();
       uint8_t* odata = p.out_data[oc];
       int32_t* ooff = p.out_off[oc];
+      // offsets, in sorted-slot order (contiguous per partition run)
+#pragma unroll 1
       for (int i = tid; i < n; i += BLOCK) {
-        int pid = s_spid[i];
-        uint32_t toff = s_toff[pid];
+        int q = s_spid[i];
+        uint32_t toff = s_toff[q];
         uint32_t run0 = s_boff[toff];
-        uint32_t b0 = s_boff[i], b1 = s_boff[i + 1];
-        int len = (int)(b1 - b0);
-        // offsets of partition pid start at index row_start + pid; this row is
-        // (s_pbase - row_start) + (i - toff) rows into the partition
-        ooff[s_pbase[pid] + pid + 1 + (i - (int)toff)] = (int32_t)(s_brel[pid] + (b0 - run0) + len);
-        copy_bytes(odata + s_bbase[pid] + (b0 - run0), src + s_src[i], len);
+        uint32_t b1 = s_boff[i + 1];
+        uint32_t d = s_pbase[q] + (uint32_t)i - toff;
+        // offsets of partition q start at index row_start + q
+        gst(ooff + (int64_t)d + q + 1, (int32_t)(s_brel[q] + (b1 - run0)));
+      }
+      if (s_boff[n] <= (uint32_t)p.cbuf) {
+        // strings of the owned rows (source order: a wave reads one contiguous
+        // span) into the partition-ordered LDS buffer ...
+#pragma unroll 1
+        for (int j = 0; j < RPT; j++) {
+          if (lrow(j) < n) {
+            const int sp = s_pos[lrow(j)];
+            const uint32_t u = s_boff[sp];
+            copy_string(LdsBytes{s_cbuf + u}, src + s_src[sp], (int)(s_boff[sp + 1] - u));
+          }
+        }
+        __syncthreads();
+        // ... then every partition run leaves as one contiguous span per wave
+#pragma unroll 1
+        for (int q = w; q < P; q += NW) {
+          const uint32_t r0 = s_boff[s_toff[q]], r1 = s_boff[s_toff[q + 1]];
+          if (r1 > r0) write_run(odata + s_bbase[q], s_cbuf, r0, (int)(r1 - r0), lane);
+        }
+      } else {
+        // tile bytes exceed the buffer: every sorted slot copies its string
+#pragma unroll 1
+        for (int i = tid; i < n; i += BLOCK) {
+          int q = s_spid[i];
+          uint32_t run0 = s_boff[s_toff[q]];
+          uint32_t b0 = s_boff[i];
+          copy_string(HbmBytes{odata + s_bbase[q] + (b0 - run0)}, src + s_src[i], (int)(s_boff[i + 1] - b0));
+        }
       }
       __syncthreads();
     }
   }
 }
 
-// Host-side LDS bytes of the scatter kernel.
-static size_t scatter_lds(int TILE, int BLOCK, int P) { return scatter_layout(TILE, BLOCK, P).total; }
+// Host-side LDS bytes and block size of the scatter kernel.
+static size_t scatter_lds(int TILE, int BLOCK, int P, int cbuf) { return scatter_layout(TILE, BLOCK, P, cbuf).total; }
+// Partition-ordered utf8 staging: about 22 bytes per row, so tiles of strings
+// up to that average length go through LDS.
+static int scatter_cbuf(int TILE, int nbin) { return nbin > 0 ? 22 * TILE : 0; }
+static int scatter_block(int P) { return P > 1024 ? 512 : 1024; }
 
 }  // namespace gz
 
@@ -564,7 +776,7 @@ struct SplitSegment {
   int64_t N = 0;
   int32_t nbatches = 0;
   int64_t T = 0;
-  DBuf counts, prefix, bytes, bprefix, tot, row_start, byte_start, null_counts, err, offtab;
+  DBuf counts, prefix, csum, tot, row_start, byte_start, err, offtab;
   std::vector<DBuf> values, validity, offsets, data;
   std::vector<int32_t> tile_batch;  // batch of each tile
   // host copies, valid after gz_split_stop
@@ -592,8 +804,8 @@ struct gz_splitter {
 };
 
 static int split_release_segment(gz_context* ctx, SplitSegment* s) {
-  for (DBuf* b : {&s->counts, &s->prefix, &s->bytes, &s->bprefix, &s->tot, &s->row_start,
-                  &s->byte_start, &s->null_counts, &s->err, &s->offtab})
+  for (DBuf* b : {&s->counts, &s->prefix, &s->csum, &s->tot, &s->row_start, &s->byte_start, &s->err,
+                  &s->offtab})
     ctx_free(ctx, b);
   for (auto* v : {&s->values, &s->validity, &s->offsets, &s->data})
     for (auto& b : *v) ctx_free(ctx, &b);
@@ -673,8 +885,8 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
   h->types_in.assign(col_types, col_types + num_cols);
   if (mode == 0) h->keys.assign(key_cols, key_cols + num_keys);
   h->opts = o;
-  h->tile_rows = o.tile_rows > 0 ? o.tile_rows : 2048;
-  if (h->tile_rows != 4096 && h->tile_rows != 2048 && h->tile_rows != 1024) h->tile_rows = 4096;
+  h->tile_rows = o.tile_rows;
+  if (h->tile_rows != 4096 && h->tile_rows != 2048 && h->tile_rows != 1024) h->tile_rows = 0;
   SplitParams& b = h->base;
   memset(&b, 0, sizeof(b));
   b.mode = mode;
@@ -705,7 +917,10 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
     b.bin[c] = type_is_binary(t) ? nbin++ : -1;
   }
   b.nbin = nbin;
-  if (scatter_lds(h->tile_rows, 256, P) > 160 * 1024) {
+  // default tile: 2048 rows when utf8 strings are staged through LDS, else 4096
+  if (h->tile_rows == 0) h->tile_rows = nbin > 0 ? 2048 : 4096;
+  if (P > 1024) h->tile_rows = 2048;  // 512-thread scatter for large fan-out
+  if (scatter_lds(h->tile_rows, scatter_block(P), P, scatter_cbuf(h->tile_rows, nbin)) > 160 * 1024) {
     delete h;
     return fail(GZ_NOT_IMPLEMENTED, "partition count too large for the LDS-staged scatter");
   }
@@ -810,17 +1025,18 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     split_release_segment(ctx, seg);
     return code;
   };
-  // ---- allocate the segment
-  if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * P * std::max<int64_t>(T, 1), &seg->counts))) return bail(rc);
-  if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * P * std::max<int64_t>(T, 1), &seg->prefix))) return bail(rc);
+  // ---- allocate the segment.  Counter row of a tile: rows per partition, utf8
+  // bytes per (binary column, partition), nulls per output column.
+  const int R = P * (1 + nbin) + std::max(nout, 1);
+  const int64_t C = (std::max<int64_t>(T, 1) + kScanChunk - 1) / kScanChunk;
+  if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * R * std::max<int64_t>(T, 1), &seg->counts))) return bail(rc);
+  if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * R * std::max<int64_t>(T, 1), &seg->prefix))) return bail(rc);
+  if ((rc = ctx_alloc(ctx, sizeof(uint64_t) * R * C, &seg->csum))) return bail(rc);
   if (nbin) {
-    if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * nbin * P * std::max<int64_t>(T, 1), &seg->bytes))) return bail(rc);
-    if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * nbin * P * std::max<int64_t>(T, 1), &seg->bprefix))) return bail(rc);
     if ((rc = ctx_alloc(ctx, sizeof(int64_t) * nbin * (P + 1), &seg->byte_start))) return bail(rc);
   }
-  if ((rc = ctx_alloc(ctx, sizeof(uint64_t) * (1 + nbin) * P, &seg->tot))) return bail(rc);
+  if ((rc = ctx_alloc(ctx, sizeof(uint64_t) * R, &seg->tot))) return bail(rc);
   if ((rc = ctx_alloc(ctx, sizeof(int64_t) * (P + 1), &seg->row_start))) return bail(rc);
-  if ((rc = ctx_alloc(ctx, sizeof(unsigned long long) * std::max(nout, 1), &seg->null_counts))) return bail(rc);
   if ((rc = ctx_alloc(ctx, sizeof(int), &seg->err))) return bail(rc);
   seg->values.resize(nout);
   seg->validity.resize(nout);
@@ -855,13 +1071,15 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   prm.T = T;
   prm.bufs = (const void* const*)h->d_bufs.p;
   prm.bstride = nc * 3;
+  prm.R = R;
+  for (int c = 0; c < nc; c++) {
+    int oc = prm.out_col[c];
+    prm.nullc[c] = (oc >= 0 && has_validity[c] && h->types_in[c] != GZ_TYPE_NA) ? P * (1 + nbin) + oc : -1;
+  }
   prm.counts = (uint32_t*)seg->counts.p;
   prm.prefix = (uint32_t*)seg->prefix.p;
-  prm.bytes = (uint32_t*)seg->bytes.p;
-  prm.bprefix = (uint32_t*)seg->bprefix.p;
   prm.row_start = (const int64_t*)seg->row_start.p;
   prm.byte_start = (const int64_t*)seg->byte_start.p;
-  prm.null_counts = (unsigned long long*)seg->null_counts.p;
   prm.err = (int*)seg->err.p;
   hipStream_t st = ctx->stream;
   // ---- fills (kernels, not memsets, so the sequence can be graph-captured)
@@ -870,7 +1088,6 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     int blocks = (int)std::min<int64_t>((n32 + 255) / 256, 4096);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, st, (uint32_t*)p, n32, v);
   };
-  fill(seg->null_counts.p, 2 * std::max(nout, 1), 0);
   fill(seg->err.p, 1, 0);
   for (int oc = 0; oc < nout; oc++) {
     if (seg->validity[oc].p) fill(seg->validity[oc].p, nwords, 0xFFFFFFFFu);
@@ -882,7 +1099,7 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
                     !has_validity[prm.key_col[0]]) ? 1 : 0;
   if (T > 0) {
     KernelTimer kt(ctx, "split_count");
-    size_t lds = sizeof(uint32_t) * P * (1 + nbin);
+    size_t lds = sizeof(uint32_t) * R;
     if (keyk)
       hipLaunchKernelGGL((split_count_kernel<256, 1>), dim3((unsigned)grid), dim3(256), lds, st, prm);
     else
@@ -891,15 +1108,15 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   {
     KernelTimer kt(ctx, "split_scan");
     if (T > 0) {
-      hipLaunchKernelGGL(split_scan_kernel<256>, dim3((unsigned)P), dim3(256), 0, st,
-                         (const uint32_t*)seg->counts.p, (uint32_t*)seg->prefix.p,
-                         (uint64_t*)seg->tot.p, T);
-      if (nbin)
-        hipLaunchKernelGGL(split_scan_kernel<256>, dim3((unsigned)(P * nbin)), dim3(256), 0, st,
-                           (const uint32_t*)seg->bytes.p, (uint32_t*)seg->bprefix.p,
-                           (uint64_t*)seg->tot.p + P, T);
+      hipLaunchKernelGGL(split_scan_sum_kernel, dim3((unsigned)C), dim3(256), 0, st,
+                         (const uint32_t*)seg->counts.p, R, T, (uint64_t*)seg->csum.p);
+      hipLaunchKernelGGL(split_scan_chunks_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st,
+                         (uint64_t*)seg->csum.p, R, C, (uint64_t*)seg->tot.p);
+      hipLaunchKernelGGL(split_scan_write_kernel, dim3((unsigned)C), dim3(256), 0, st,
+                         (const uint32_t*)seg->counts.p, (const uint64_t*)seg->csum.p, R, T,
+                         (uint32_t*)seg->prefix.p);
     } else {
-      fill(seg->tot.p, 2 * (int64_t)(1 + nbin) * P, 0);
+      fill(seg->tot.p, 2 * (int64_t)R, 0);
     }
     std::vector<int32_t*> offp(std::max(nbin, 1), nullptr);
     for (int c = 0; c < nc; c++)
@@ -928,12 +1145,15 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   }
   if (T > 0) {
     KernelTimer kt(ctx, "split_scatter");
-    size_t lds = scatter_lds(TILE, 256, P);
-#define GZ_SCATTER(TL, KK) \
-  hipLaunchKernelGGL((split_scatter_kernel<TL, 256, KK>), dim3((unsigned)grid), dim3(256), lds, st, prm)
-    if (TILE == 4096) { if (keyk) GZ_SCATTER(4096, 1); else GZ_SCATTER(4096, 0); }
-    else if (TILE == 2048) { if (keyk) GZ_SCATTER(2048, 1); else GZ_SCATTER(2048, 0); }
-    else { if (keyk) GZ_SCATTER(1024, 1); else GZ_SCATTER(1024, 0); }
+    const int BLOCK = scatter_block(P);
+    prm.cbuf = scatter_cbuf(TILE, nbin);
+    size_t lds = scatter_lds(TILE, BLOCK, P, prm.cbuf);
+#define GZ_SCATTER(TL, BL, KK) \
+  hipLaunchKernelGGL((split_scatter_kernel<TL, BL, KK>), dim3((unsigned)grid), dim3(BL), lds, st, prm)
+    if (BLOCK == 512) { if (keyk) GZ_SCATTER(2048, 512, 1); else GZ_SCATTER(2048, 512, 0); }
+    else if (TILE == 4096) { if (keyk) GZ_SCATTER(4096, 1024, 1); else GZ_SCATTER(4096, 1024, 0); }
+    else if (TILE == 2048) { if (keyk) GZ_SCATTER(2048, 1024, 1); else GZ_SCATTER(2048, 1024, 0); }
+    else { if (keyk) GZ_SCATTER(1024, 1024, 1); else GZ_SCATTER(1024, 1024, 0); }
 #undef GZ_SCATTER
   }
   GZ_HIP(hipGetLastError());
@@ -961,8 +1181,8 @@ static int resolve_segment(gz_splitter* h, SplitSegment* s) {
   if (nbin)
     GZ_HIP(hipMemcpyAsync(s->h_byte_start.data(), s->byte_start.p,
                           sizeof(int64_t) * nbin * (P + 1), hipMemcpyDeviceToHost, ctx->stream));
-  GZ_HIP(hipMemcpyAsync(s->h_null.data(), s->null_counts.p, sizeof(unsigned long long) * std::max(nout, 1),
-                        hipMemcpyDeviceToHost, ctx->stream));
+  GZ_HIP(hipMemcpyAsync(s->h_null.data(), (uint64_t*)s->tot.p + (int64_t)P * (1 + nbin),
+                        sizeof(unsigned long long) * std::max(nout, 1), hipMemcpyDeviceToHost, ctx->stream));
   GZ_HIP(hipMemcpyAsync(&err, s->err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   GZ_HIP(hipStreamSynchronize(ctx->stream));
   if (err & 1) return fail(GZ_INVALID, "Partition id is equal or greater than num_partitions");

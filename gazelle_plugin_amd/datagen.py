@@ -144,3 +144,81 @@ def agg_host(num_rows: int, num_keys: int = 10000, batch_rows: int = BATCH_ROWS,
 
 def device_from_host(batches: List[HostBatch], device: str = "cuda") -> List[DeviceBatch]:
     return [DeviceBatch.from_host(b, device) for b in batches]
+
+
+def _agg_columns_device(g, n: int, key, device: str):
+    """price f64 ~ N(100, 25) with 1% nulls, qty int32 ~ U[1, 50] next to a key."""
+    import torch
+
+    price = torch.randn(n, dtype=torch.float64, device=device, generator=g) * 25.0 + 100.0
+    valid = torch.rand(n, device=device, generator=g) >= 0.01
+    qty = torch.randint(1, 51, (n,), dtype=torch.int32, device=device, generator=g)
+    return key, price, valid, qty
+
+
+def agg_device(num_rows: int, num_keys: int = 10000, batch_rows: int = BATCH_ROWS, seed: int = 42,
+               device: str = "cuda") -> DeviceTable:
+    """The plain aggregate workload in HBM: int64 key over num_keys distinct
+    values, price f64 ~ N(100, 25) with 1% nulls, qty int32 ~ U[1, 50]."""
+    import torch
+
+    assert num_rows % batch_rows == 0 and batch_rows % 64 == 0
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    keyset = torch.randint(-(1 << 62), 1 << 62, (num_keys,), dtype=torch.int64, device=device, generator=g)
+    key = keyset[torch.randint(0, num_keys, (num_rows,), device=device, generator=g)]
+    key, price, valid, qty = _agg_columns_device(g, num_rows, key, device)
+    vbits = _bitmap_device(valid)
+    kb, pb, vb, qb = (t.view(torch.uint8) for t in (key, price, vbits, qty))
+    batches = []
+    for b0 in range(0, num_rows, batch_rows):
+        n = batch_rows
+        bufs = [None, kb[b0 * 8:(b0 + n) * 8], vb[b0 // 8:(b0 + n) // 8], pb[b0 * 8:(b0 + n) * 8],
+                None, qb[b0 * 4:(b0 + n) * 4]]
+        batches.append(DeviceBatch(n, AGG_TYPES, bufs))
+    nbytes = num_rows * (8 + 8 + 4) + num_rows // 8
+    return DeviceTable((key, price, vbits, qty), batches, num_rows, nbytes)
+
+
+def batched_agg_shapes(num_batches: int = 512, seed: int = 42, min_rows: int = 1000,
+                       max_rows: int = 1000000, min_groups: int = 10, max_groups: int = 10 ** 7):
+    """Row counts and group cardinalities, both log-uniform; groups capped at
+    rows; row counts rounded up to multiples of 64 (whole validity words)."""
+    r = _rng(20, seed)
+    rows = np.exp(r.uniform(np.log(min_rows), np.log(max_rows), size=num_batches))
+    rows = ((np.ceil(rows).astype(np.int64) + 63) // 64) * 64
+    groups = np.exp(r.uniform(np.log(min_groups), np.log(max_groups), size=num_batches))
+    groups = np.minimum(np.ceil(groups).astype(np.int64), rows)
+    return rows, groups
+
+
+def batched_agg_device(num_batches: int = 512, seed: int = 42, device: str = "cuda"):
+    """The batched aggregate workload in HBM: one RecordBatch per independent
+    GROUP BY (instance b), rows and cardinality log-uniform.  Returns the
+    DeviceTable, the rows and the groups of every batch."""
+    import torch
+
+    rows, groups = batched_agg_shapes(num_batches, seed)
+    total = int(rows.sum())
+    g = torch.Generator(device=device)
+    g.manual_seed(seed + 1)
+    # key of row r of batch b: a mixed value of (b, u), u ~ U[0, groups_b)
+    starts = np.zeros(num_batches + 1, np.int64)
+    starts[1:] = np.cumsum(rows)
+    gb = torch.from_numpy(np.repeat(groups, rows)).to(device)
+    u = (torch.rand(total, dtype=torch.float64, device=device, generator=g) * gb).to(torch.int64)
+    u = torch.minimum(u, gb - 1)
+    bid = torch.from_numpy(np.repeat(np.arange(num_batches, dtype=np.int64), rows)).to(device)
+    key = (u * 0x1E3779B97F4A7C15 + bid * 0x232BE59BD9B4E019) & 0x7FFFFFFFFFFFFFFF
+    del u, bid, gb
+    key, price, valid, qty = _agg_columns_device(g, total, key, device)
+    vbits = _bitmap_device(valid)
+    kb, pb, vb, qb = (t.view(torch.uint8) for t in (key, price, vbits, qty))
+    batches = []
+    for b in range(num_batches):
+        b0, n = int(starts[b]), int(rows[b])
+        bufs = [None, kb[b0 * 8:(b0 + n) * 8], vb[b0 // 8:(b0 + n) // 8], pb[b0 * 8:(b0 + n) * 8],
+                None, qb[b0 * 4:(b0 + n) * 4]]
+        batches.append(DeviceBatch(n, AGG_TYPES, bufs))
+    nbytes = total * (8 + 8 + 4) + total // 8
+    return DeviceTable((key, price, vbits, qty), batches, total, nbytes), rows, groups

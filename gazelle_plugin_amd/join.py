@@ -118,7 +118,10 @@ class ConditionedProbeKernel:
         return n.value, per[: len(batches)]
 
     def result_columns(self, n: int) -> List:
-        """Host copy of the last output: per column (type, values, valid[, offsets, data])."""
+        """Host copy of the last output: per column (type, values, valid[, offsets, data]).
+        The kernels run on the context's non-blocking stream, so the stream is
+        drained before the null-stream copies read their output."""
+        self.rel.ctx.synchronize()
         out = []
         for c, t in enumerate(self.result_types):
             vals, valid, offs, data = (ctypes.c_void_p() for _ in range(4))

@@ -97,7 +97,9 @@ class Splitter:
 
     # -- readback
     def segment(self, seg: int) -> Dict:
-        """Host copy of one segment's partitioned layout."""
+        """Host copy of one segment's partitioned layout (drains the context's
+        stream first: the copies run on the null stream)."""
+        self.ctx.synchronize()
         P = self.num_partitions
         rs = np.zeros(P + 1, np.int64)
         L.check(L.lib().gz_split_segment_rows(self._h, seg, rs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))

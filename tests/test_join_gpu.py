@@ -1,6 +1,8 @@
 """Parity of the HIP hash join with the oracle through the C ABI: output rows
 bit-exact and in the reference's order (probe order, matches in build insertion
 order)."""
+import struct
+
 import numpy as np
 import pytest
 
@@ -42,7 +44,11 @@ def check_output(got, exp_cols):
             raise AssertionError(f"column {c}: validity differs at {bad[:10].tolist()}")
         gv = [v.item() if hasattr(v, "item") else v for v in vals]
         for i in np.nonzero(eok)[0]:
-            if gv[i] != ev[i]:
+            if isinstance(gv[i], float) and isinstance(ev[i], float):
+                same = struct.pack("<d", gv[i]) == struct.pack("<d", ev[i])  # bit-exact, NaN included
+            else:
+                same = gv[i] == ev[i]
+            if not same:
                 raise AssertionError(f"column {c} row {i}: {gv[i]!r} != {ev[i]!r}")
 
 
