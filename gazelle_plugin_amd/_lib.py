@@ -113,6 +113,17 @@ _OPTIONAL_SIGS = {
     "gz_agg_finish": (ctypes.c_int, [_P]),
     "gz_agg_result_column": (ctypes.c_int, [_P, _I32, _PP, _PP]),
     "gz_agg_release": (ctypes.c_int, [_P]),
+    "gz_agg_set_make": (ctypes.c_int, [_P, _I32, _PI32, _I32, ctypes.POINTER(GzAggAction),
+                                       ctypes.POINTER(GzAggOptions), _I32, _PP]),
+    "gz_agg_set_process": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch), _PI32]),
+    "gz_agg_set_num_groups": (ctypes.c_int, [_P, _I32, _PI64]),
+    "gz_agg_set_result_column": (ctypes.c_int, [_P, _I32, _I32, _PP, _PP]),
+    "gz_agg_set_placement": (ctypes.c_int, [_P, _I32, _PI32]),
+    "gz_agg_set_prepare": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch), _PI32]),
+    "gz_agg_set_run": (ctypes.c_int, [_P]),
+    "gz_agg_set_capture": (ctypes.c_int, [_P]),
+    "gz_agg_set_launch": (ctypes.c_int, [_P]),
+    "gz_agg_set_autotune": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch), _PI32, _PI32]),
     "gz_relation_make": (ctypes.c_int, [_P, _I32, _PI32, _I32, _I32, _PP]),
     "gz_relation_append": (ctypes.c_int, [_P, _I64, _PI64, _PI64, _I32]),
     "gz_relation_append_batches": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch)]),

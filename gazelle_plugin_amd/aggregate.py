@@ -117,12 +117,8 @@ class HashAggregateResultIterator:
             L.check(L.lib().gz_agg_make(kernel.ctx.handle, len(kernel.input_types), types, len(kernel.actions),
                                         acts, ctypes.byref(opts), ctypes.byref(self._h)))
         else:
-            fn = L.lib().gz_agg_set_make
-            fn.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
-                           ctypes.POINTER(L.GzAggAction), ctypes.POINTER(L.GzAggOptions), ctypes.c_int32,
-                           ctypes.POINTER(ctypes.c_void_p)]
-            L.check(fn(kernel.ctx.handle, len(kernel.input_types), types, len(kernel.actions), acts,
-                       ctypes.byref(opts), num_instances, ctypes.byref(self._h)))
+            L.check(L.lib().gz_agg_set_make(kernel.ctx.handle, len(kernel.input_types), types, len(kernel.actions),
+                                            acts, ctypes.byref(opts), num_instances, ctypes.byref(self._h)))
         self._offset = 0
         self._finished = False
         self._results = None
@@ -139,9 +135,36 @@ class HashAggregateResultIterator:
     def ProcessInstances(self, batches: Sequence[DeviceBatch], instance_of: Sequence[int], arr=None):
         if arr is None:
             arr = make_batch_array(batches)
-        fn = L.lib().gz_agg_set_process
-        fn.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(L.GzBatch), ctypes.POINTER(ctypes.c_int32)]
-        L.check(fn(self._h, len(batches), arr, L.i32_array(instance_of)))
+        L.check(L.lib().gz_agg_set_process(self._h, len(batches), arr, L.i32_array(instance_of)))
+
+    # -- planned form: prepare once, then run / capture + launch (HIP graph)
+    def Prepare(self, batches: Sequence[DeviceBatch], instance_of: Sequence[int], arr=None):
+        self._plan_arr = arr if arr is not None else make_batch_array(batches)  # must outlive the plan
+        self._plan_inst = L.i32_array(instance_of)
+        L.check(L.lib().gz_agg_set_prepare(self._h, len(batches), self._plan_arr, self._plan_inst))
+        self._finished = False
+
+    def Autotune(self, batches: Sequence[DeviceBatch], instance_of: Sequence[int], arr=None) -> int:
+        self._plan_arr = arr if arr is not None else make_batch_array(batches)
+        self._plan_inst = L.i32_array(instance_of)
+        chosen = ctypes.c_int32()
+        L.check(L.lib().gz_agg_set_autotune(self._h, len(batches), self._plan_arr, self._plan_inst,
+                                            ctypes.byref(chosen)))
+        self._finished = False
+        return chosen.value
+
+    def Run(self):
+        L.check(L.lib().gz_agg_set_run(self._h))
+        self._finished = False
+        self._results = None
+
+    def Capture(self):
+        L.check(L.lib().gz_agg_set_capture(self._h))
+
+    def Launch(self):
+        L.check(L.lib().gz_agg_set_launch(self._h))
+        self._finished = False
+        self._results = None
 
     # -- output
     def finish(self):
@@ -155,16 +178,12 @@ class HashAggregateResultIterator:
         if self.num_instances == 1:
             L.check(L.lib().gz_agg_num_groups(self._h, ctypes.byref(n)))
         else:
-            fn = L.lib().gz_agg_set_num_groups
-            fn.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]
-            L.check(fn(self._h, inst, ctypes.byref(n)))
+            L.check(L.lib().gz_agg_set_num_groups(self._h, inst, ctypes.byref(n)))
         return n.value
 
     def placement(self, inst: int = 0) -> int:
         out = ctypes.c_int32()
-        fn = L.lib().gz_agg_set_placement
-        fn.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
-        L.check(fn(self._h, inst, ctypes.byref(out)))
+        L.check(L.lib().gz_agg_set_placement(self._h, inst, ctypes.byref(out)))
         return out.value
 
     def results(self, inst: int = 0) -> List[Tuple[int, np.ndarray, np.ndarray]]:
@@ -180,10 +199,7 @@ class HashAggregateResultIterator:
             if self.num_instances == 1:
                 L.check(L.lib().gz_agg_result_column(self._h, c, ctypes.byref(vals), ctypes.byref(valid)))
             else:
-                fn = L.lib().gz_agg_set_result_column
-                fn.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p),
-                               ctypes.POINTER(ctypes.c_void_p)]
-                L.check(fn(self._h, inst, c, ctypes.byref(vals), ctypes.byref(valid)))
+                L.check(L.lib().gz_agg_set_result_column(self._h, inst, c, ctypes.byref(vals), ctypes.byref(valid)))
             dt = np.dtype(_NP[t])
             v = to_host(vals.value, g * dt.itemsize, dt) if g else np.zeros(0, dt)
             ok = to_host(valid.value, g, np.uint8).astype(bool) if g else np.zeros(0, bool)

@@ -306,6 +306,31 @@ __global__ void agg_init_kernel(const AggInst* inst, int ninst, int nl, const in
   }
 }
 
+// Reset of every instance's table in one launch: slot g of the concatenated
+// slot space belongs to the instance whose range [prefix[i], prefix[i+1])
+// holds it.
+__global__ void agg_init_all_kernel(const AggInst* inst, int ninst, const int64_t* prefix, int64_t total,
+                                    int nl, const int* lane_ops) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const AggInst I = inst[lo];
+    const int64_t S = (int64_t)I.cap + 2;
+    const int64_t s = g - prefix[lo];
+    I.base[s] = kEmpty;
+    I.base[S + s] = INT64_MAX;
+    for (int l = 0; l < nl; l++) I.base[(2 + l) * S + s] = lane_init(lane_ops[l]);
+    if (s == 0) {
+      *I.counter = 0;
+      *I.overflow = 0;
+    }
+  }
+}
+
 // HBM path: one workgroup per tile.
 __global__ __launch_bounds__(256) void agg_update_kernel(AggParams p) {
   int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
@@ -653,6 +678,19 @@ struct gz_agg {
   std::vector<DBuf> res_values, res_valid;
   int64_t total_groups = 0;
   bool inst_dirty = true;
+  // planned form (gz_agg_set_prepare / run / capture / launch)
+  struct Launch {
+    bool ready = false;
+    AggParams p_hbm, p_lds;
+    unsigned grid_hbm = 0, blocks_lds = 0;
+    int lds = 0;
+    int64_t total_slots = 0;
+    DBuf d_hbm, d_lds, d_slot_prefix;
+    std::vector<int64_t> rows;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } L;
+  int lds_threshold = 256;  // groups up to which an instance gets the LDS first level
 };
 
 static int add_lane(AggPlan& P, int op, int col, int col2) {
@@ -994,7 +1032,7 @@ static int agg_process(gz_agg* h, int32_t nb, const gz_batch* batches, const int
       h->g_bound[i] = 2 + (int64_t)std::min<double>(g, (double)rows[i]);
       // placement: LDS first-level table when the groups fit in LDS
       if (h->opts.placement == 0)
-        h->placement[i] = (g <= 256.0 || h->plan.key_col < 0) ? 1 : 2;
+        h->placement[i] = (g <= (double)h->lds_threshold || h->plan.key_col < 0) ? 1 : 2;
       if (!estimated) h->g_bound[i] = 2 + rows[i];
     } else if ((double)(h->g_bound[i] + rows[i]) > lf * h->inst[i].cap) {
       uint64_t cap = pow2_at_least((uint64_t)((double)(h->g_bound[i] + rows[i]) * 2 / lf));
@@ -1062,6 +1100,14 @@ static int agg_finish(gz_agg* h) {
   if (h->finished) return GZ_OK;
   gz_context* ctx = h->ctx;
   GZ_HIP(hipSetDevice(ctx->device));
+  if (h->L.ready) {
+    // a planned run sized its tables from estimates: verify none overflowed
+    std::vector<int64_t> g;
+    int ovf = 0;
+    GZ_TRY(read_counts(h, g, &ovf));
+    if (ovf) return fail(GZ_CAPACITY_ERROR, "aggregate hash table overflowed (cardinality estimate); "
+                                            "prepare again with expected_groups");
+  }
   // instances never fed get an empty table so the collect kernel can run
   for (int i = 0; i < h->ninst; i++)
     if (!h->tables[i].p) GZ_TRY(inst_alloc(h, i, 16));
@@ -1122,8 +1168,10 @@ static int agg_finish(gz_agg* h) {
                                      (unsigned long long*)h->svals.p, (size_t)n, 0, end_bit, ctx->stream));
   }
   const int nres = (int)h->plan.results.size();
-  h->res_values.resize(nres);
-  h->res_valid.resize(nres);
+  for (auto& b : h->res_values) ctx_free(ctx, &b);
+  for (auto& b : h->res_valid) ctx_free(ctx, &b);
+  h->res_values.assign(nres, DBuf());
+  h->res_valid.assign(nres, DBuf());
   FinishParams f;
   memset(&f, 0, sizeof(f));
   f.inst = (const AggInst*)h->d_inst.p;
@@ -1148,6 +1196,246 @@ static int agg_finish(gz_agg* h) {
   GZ_HIP(hipGetLastError());
   GZ_HIP(hipStreamSynchronize(ctx->stream));
   h->finished = true;
+  return GZ_OK;
+}
+
+// ---- planned form: prepare (host decisions, allocation, uploads), then a
+// kernel-only run that can be captured into a HIP graph and replayed.
+static void drop_graph(gz_agg* h) {
+  if (h->L.exec) hipGraphExecDestroy(h->L.exec);
+  if (h->L.graph) hipGraphDestroy(h->L.graph);
+  h->L.exec = nullptr;
+  h->L.graph = nullptr;
+}
+
+static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of) {
+  gz_context* ctx = h->ctx;
+  GZ_HIP(hipSetDevice(ctx->device));
+  drop_graph(h);
+  h->L.ready = false;
+  const int tile_rows = h->opts.tile_rows > 0 ? h->opts.tile_rows : 4096;
+  std::vector<AggTile> tiles;
+  std::vector<int64_t> rows;
+  // a prepared run aggregates its batches from scratch: instance rows start at 0
+  std::fill(h->rows_seen.begin(), h->rows_seen.end(), 0);
+  GZ_TRY(agg_stage(h, nb, batches, inst_of, tiles, rows, tile_rows));
+  AggParams p = base_params(h);
+  p.bufs = (const void* const*)h->d_bufs.p;
+  p.bstride = h->plan.ncols * 3;
+  const double lf = h->opts.load_factor;
+  // cardinality per instance: the hint, else HyperLogLog over its rows
+  std::vector<double> est(h->ninst, 0.0);
+  bool need_est = false;
+  for (int i = 0; i < h->ninst; i++)
+    if (rows[i] > 4096 && h->opts.expected_groups <= 0) need_est = true;
+  if (need_est && !tiles.empty()) {
+    GZ_TRY(upload(ctx, &h->d_tiles, tiles.data(), sizeof(AggTile) * tiles.size()));
+    p.tiles = (const AggTile*)h->d_tiles.p;
+    p.T = (int64_t)tiles.size();
+    GZ_TRY(ctx_ensure(ctx, &h->d_regs, sizeof(unsigned) * kHllRegs * h->ninst));
+    GZ_TRY(ctx_ensure(ctx, &h->d_est, sizeof(double) * h->ninst));
+    int64_t n32 = (int64_t)kHllRegs * h->ninst;
+    hipLaunchKernelGGL(fill_u32_kernel_agg, dim3((unsigned)std::min<int64_t>((n32 + 255) / 256, 4096)), dim3(256), 0,
+                       ctx->stream, (uint32_t*)h->d_regs.p, n32);
+    int64_t blocks = std::min<int64_t>(p.T, (int64_t)ctx->num_cus * 4);
+    p.tiles_per_block = (int)((p.T + blocks - 1) / blocks);
+    blocks = (p.T + p.tiles_per_block - 1) / p.tiles_per_block;
+    hipLaunchKernelGGL(agg_hll_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, p,
+                       (unsigned int*)h->d_regs.p);
+    hipLaunchKernelGGL(agg_hll_estimate_kernel, dim3((unsigned)h->ninst), dim3(256), 0, ctx->stream,
+                       (const unsigned int*)h->d_regs.p, (double*)h->d_est.p);
+    GZ_HIP(hipMemcpyAsync(est.data(), h->d_est.p, sizeof(double) * h->ninst, hipMemcpyDeviceToHost, ctx->stream));
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  // tables (reused when large enough: the run resets them) and placement
+  std::vector<int64_t> prefix(h->ninst + 1, 0);
+  for (int i = 0; i < h->ninst; i++) {
+    double g;
+    if (rows[i] == 0) g = 1;
+    else if (h->opts.expected_groups > 0) g = std::min<double>((double)h->opts.expected_groups, (double)rows[i]);
+    else if (rows[i] > 4096) g = std::min<double>(est[i] * 1.3 + 64, (double)rows[i]);
+    else g = (double)rows[i];
+    uint64_t cap = h->plan.key_col < 0 ? 16 : pow2_at_least((uint64_t)(g / lf) + 16);
+    if (!h->tables[i].p || h->inst[i].cap < cap || h->inst[i].cap > 4 * cap) {
+      if (h->tables[i].p) {
+        ctx_free(ctx, &h->tables[i]);
+        ctx_free(ctx, &h->counters[i]);
+      }
+      GZ_TRY(inst_alloc(h, i, cap));
+    }
+    h->g_bound[i] = 2 + (int64_t)g;
+    if (h->opts.placement == 0)
+      h->placement[i] = (g <= (double)h->lds_threshold || h->plan.key_col < 0) ? 1 : 2;
+    prefix[i + 1] = prefix[i] + (int64_t)h->inst[i].cap + 2;
+  }
+  GZ_TRY(upload(ctx, &h->d_inst, h->inst.data(), sizeof(AggInst) * h->inst.size()));
+  h->inst_dirty = false;
+  GZ_TRY(upload(ctx, &h->L.d_slot_prefix, prefix.data(), sizeof(int64_t) * prefix.size()));
+  h->L.total_slots = prefix[h->ninst];
+  std::vector<int> ops(std::max<size_t>(h->plan.lanes.size(), 1));
+  for (size_t l = 0; l < h->plan.lanes.size(); l++) ops[l] = h->plan.lanes[l].op;
+  GZ_TRY(upload(ctx, &h->d_lane_ops, ops.data(), sizeof(int) * ops.size()));
+  p.inst = (const AggInst*)h->d_inst.p;
+  // launch plan
+  std::vector<AggTile> lds_tiles, hbm_tiles;
+  for (auto& t : tiles) (h->placement[t.inst] == 1 ? lds_tiles : hbm_tiles).push_back(t);
+  h->L.p_hbm = p;
+  h->L.p_lds = p;
+  h->L.grid_hbm = 0;
+  h->L.blocks_lds = 0;
+  if (!hbm_tiles.empty()) {
+    GZ_TRY(upload(ctx, &h->L.d_hbm, hbm_tiles.data(), sizeof(AggTile) * hbm_tiles.size()));
+    h->L.p_hbm.tiles = (const AggTile*)h->L.d_hbm.p;
+    h->L.p_hbm.T = (int64_t)hbm_tiles.size();
+    h->L.grid_hbm = (unsigned)((h->L.p_hbm.T + 7) / 8 * 8);
+  }
+  if (!lds_tiles.empty()) {
+    GZ_TRY(upload(ctx, &h->L.d_lds, lds_tiles.data(), sizeof(AggTile) * lds_tiles.size()));
+    AggParams& q = h->L.p_lds;
+    q.tiles = (const AggTile*)h->L.d_lds.p;
+    q.T = (int64_t)lds_tiles.size();
+    int64_t blocks = std::min<int64_t>(q.T, (int64_t)ctx->num_cus * 2);
+    q.tiles_per_block = (int)((q.T + blocks - 1) / blocks);
+    h->L.blocks_lds = (unsigned)((q.T + q.tiles_per_block - 1) / q.tiles_per_block);
+    q.lcap = 1024;
+    int lds = lds_bytes(h, q.lcap);
+    while (lds > 96 * 1024 && q.lcap > 64) {
+      q.lcap >>= 1;
+      lds = lds_bytes(h, q.lcap);
+    }
+    h->L.lds = lds;
+  }
+  h->L.rows = rows;
+  h->L.ready = true;
+  GZ_HIP(hipGetLastError());
+  return GZ_OK;
+}
+
+// Kernels only: no allocation, no host synchronization, no memset nodes.
+static int agg_run_kernels(gz_agg* h) {
+  gz_context* ctx = h->ctx;
+  hipStream_t st = ctx->stream;
+  const int nl = (int)h->plan.lanes.size();
+  unsigned gi = (unsigned)std::min<int64_t>((h->L.total_slots + 255) / 256, 16384);
+  hipLaunchKernelGGL(agg_init_all_kernel, dim3(gi), dim3(256), 0, st, (const AggInst*)h->d_inst.p, h->ninst,
+                     (const int64_t*)h->L.d_slot_prefix.p, h->L.total_slots, nl, (const int*)h->d_lane_ops.p);
+  if (h->L.grid_hbm)
+    hipLaunchKernelGGL(agg_update_kernel, dim3(h->L.grid_hbm), dim3(256), 0, st, h->L.p_hbm);
+  if (h->L.blocks_lds)
+    hipLaunchKernelGGL(agg_update_lds_kernel, dim3(h->L.blocks_lds), dim3(256), h->L.lds, st, h->L.p_lds);
+  GZ_HIP(hipGetLastError());
+  return GZ_OK;
+}
+
+static void mark_run(gz_agg* h) {
+  for (int i = 0; i < h->ninst; i++) h->rows_seen[i] = h->L.rows[i];
+  h->finished = false;
+}
+
+static int agg_run(gz_agg* h) {
+  if (!h->L.ready) return fail(GZ_INVALID, "gz_agg_set_prepare first");
+  GZ_HIP(hipSetDevice(h->ctx->device));
+  {
+    KernelTimer kt(h->ctx, "agg_run");
+    GZ_TRY(agg_run_kernels(h));
+  }
+  mark_run(h);
+  return GZ_OK;
+}
+
+static int agg_capture(gz_agg* h) {
+  if (!h->L.ready) return fail(GZ_INVALID, "gz_agg_set_prepare first");
+  GZ_HIP(hipSetDevice(h->ctx->device));
+  drop_graph(h);
+  hipStream_t st = h->ctx->stream;
+  GZ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = agg_run_kernels(h);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(GZ_HIP_ERROR, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  h->L.graph = g;
+  e = hipGraphInstantiate(&h->L.exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    drop_graph(h);
+    return fail(GZ_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return GZ_OK;
+}
+
+static int agg_launch(gz_agg* h) {
+  if (!h->L.exec) return agg_run(h);
+  GZ_HIP(hipSetDevice(h->ctx->device));
+  {
+    KernelTimer kt(h->ctx, "agg_run");
+    GZ_HIP(hipGraphLaunch(h->L.exec, h->ctx->stream));
+  }
+  mark_run(h);
+  return GZ_OK;
+}
+
+// Autotuner: candidate plans (update tile rows x LDS placement threshold x load
+// factor), each prepared and timed over two runs on the given batches; the
+// fastest stays prepared.
+static int agg_autotune(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of, int32_t* chosen) {
+  struct Cand {
+    int tile_rows, threshold;
+    double lf;
+  };
+  std::vector<Cand> cands;
+  for (int tr : {2048, 4096, 8192})
+    for (int th : {0, 256, 1024})
+      for (double lf : {0.5, 0.7}) cands.push_back({tr, th, lf});
+  gz_context* ctx = h->ctx;
+  const gz_agg_options saved = h->opts;
+  const int saved_th = h->lds_threshold;
+  hipEvent_t e0, e1;
+  GZ_HIP(hipEventCreate(&e0));
+  GZ_HIP(hipEventCreate(&e1));
+  int best = -1;
+  float best_ms = 0;
+  int rc = GZ_OK;
+  for (size_t c = 0; c < cands.size() && rc == GZ_OK; c++) {
+    h->opts.tile_rows = cands[c].tile_rows;
+    h->opts.load_factor = cands[c].lf;
+    if (saved.placement == 0) h->lds_threshold = cands[c].threshold;
+    else if (c > 0 && cands[c].threshold != cands[0].threshold) continue;  // placement fixed by the caller
+    if ((rc = agg_prepare(h, nb, batches, inst_of))) break;
+    if ((rc = agg_run_kernels(h))) break;  // warm
+    hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < 2 && rc == GZ_OK; r++) rc = agg_run_kernels(h);
+    hipEventRecord(e1, ctx->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = fail(GZ_HIP_ERROR, "autotune timing");
+      break;
+    }
+    // a candidate whose table overflowed is not eligible
+    std::vector<int64_t> g;
+    int ovf = 0;
+    if ((rc = read_counts(h, g, &ovf))) break;
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (!ovf && (best < 0 || ms < best_ms)) {
+      best = (int)c;
+      best_ms = ms;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) {
+    h->opts = saved;
+    h->lds_threshold = saved_th;
+    return rc;
+  }
+  if (best < 0) return fail(GZ_CAPACITY_ERROR, "no candidate plan fits");
+  h->opts.tile_rows = cands[best].tile_rows;
+  h->opts.load_factor = cands[best].lf;
+  if (saved.placement == 0) h->lds_threshold = cands[best].threshold;
+  GZ_TRY(agg_prepare(h, nb, batches, inst_of));
+  if (chosen) *chosen = best;
   return GZ_OK;
 }
 
@@ -1230,8 +1518,11 @@ int gz_agg_release(gz_agg* h) {
   for (auto& b : h->res_values) ctx_free(ctx, &b);
   for (auto& b : h->res_valid) ctx_free(ctx, &b);
   for (DBuf* b : {&h->d_inst, &h->d_lane_ops, &h->d_tiles, &h->d_bufs, &h->d_regs, &h->d_est, &h->skeys,
-                  &h->svals, &h->tkeys, &h->tvals, &h->cursor, &h->counts, &h->sort_tmp})
+                  &h->svals, &h->tkeys, &h->tvals, &h->cursor, &h->counts, &h->sort_tmp, &h->L.d_hbm,
+                  &h->L.d_lds, &h->L.d_slot_prefix})
     ctx_free(ctx, b);
+  if (h->L.exec) hipGraphExecDestroy(h->L.exec);
+  if (h->L.graph) hipGraphDestroy(h->L.graph);
   delete h;
   return GZ_OK;
 }
@@ -1272,6 +1563,34 @@ int gz_agg_set_placement(gz_agg* h, int32_t inst, int32_t* out) {
   if (!h || !out || inst < 0 || inst >= h->ninst) return fail(GZ_INVALID, "bad argument");
   *out = h->placement[inst];
   return GZ_OK;
+}
+
+int gz_agg_set_prepare(gz_agg* h, int32_t num_batches, const gz_batch* batches, const int32_t* instance_of) {
+  if (!h) return fail(GZ_INVALID, "null aggregate set");
+  if (num_batches < 0 || (num_batches > 0 && (!batches || !instance_of))) return fail(GZ_INVALID, "bad batch list");
+  return agg_prepare(h, num_batches, batches, instance_of);
+}
+
+int gz_agg_set_run(gz_agg* h) {
+  if (!h) return fail(GZ_INVALID, "null aggregate set");
+  return agg_run(h);
+}
+
+int gz_agg_set_capture(gz_agg* h) {
+  if (!h) return fail(GZ_INVALID, "null aggregate set");
+  return agg_capture(h);
+}
+
+int gz_agg_set_launch(gz_agg* h) {
+  if (!h) return fail(GZ_INVALID, "null aggregate set");
+  return agg_launch(h);
+}
+
+int gz_agg_set_autotune(gz_agg* h, int32_t num_batches, const gz_batch* batches, const int32_t* instance_of,
+                        int32_t* chosen) {
+  if (!h) return fail(GZ_INVALID, "null aggregate set");
+  if (num_batches < 0 || (num_batches > 0 && (!batches || !instance_of))) return fail(GZ_INVALID, "bad batch list");
+  return agg_autotune(h, num_batches, batches, instance_of, chosen);
 }
 
 }  // extern "C"

@@ -59,6 +59,8 @@ __device__ __forceinline__ int pmod_hash(uint32_t h, const FastMod& f) {
   return r < 0 ? r + (int)f.P : r;
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // ---- global memory ------------------------------------------------------------
 // Pointers read from LDS or from pointer tables are generic to the compiler, so
 // it emits flat instructions, which count against both the vector-memory and the

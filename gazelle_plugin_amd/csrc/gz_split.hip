@@ -37,6 +37,11 @@ constexpr int kMaxCols = 32;
 constexpr int kMaxKeys = 4;
 constexpr int kMaxPartitions = 2048;
 constexpr int kScanChunk = 64;  // tiles per chunk of the tile scan
+constexpr int kPre = 4;         // fixed-width columns the scatter loads up front
+// ... fewer when a thread owns more rows (registers)
+__host__ __device__ constexpr int scatter_max_pre(int rows_per_thread) {
+  return rows_per_thread <= 2 ? kPre : rows_per_thread <= 4 ? 2 : 1;
+}
 
 enum ColKind : int { K_NA = 0, K_FIXED = 1, K_BOOL = 2, K_BINARY = 3 };
 
@@ -66,6 +71,9 @@ struct SplitParams {
   int out_col[kMaxCols];     // output column of an input column, -1 if dropped
   int bin[kMaxCols];         // binary column ordinal, -1 otherwise
   int nullc[kMaxCols];       // null-count slot of a nullable column, -1 otherwise
+  int npre;                  // fixed-width columns loaded up front by the scatter
+  int pre_col[kPre];         // (the key column first on the int64-key path)
+  int is_pre[kMaxCols];
   int nbin;
   int bin_col0;              // input column of the first binary column
   // per-segment buffers.  A tile's counter row has R = P * (1 + nbin) + nnull
@@ -341,94 +349,50 @@ __device__ __forceinline__ uint32_t funnel(uint32_t lo, uint32_t hi, uint32_t sh
   return (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * sh));
 }
 
-// Byte / dword stores to HBM or to LDS.
-struct HbmBytes {
-  uint8_t* p;
-  __device__ __forceinline__ void b(int i, uint8_t v) const { gst(p + i, v); }
-  __device__ __forceinline__ void d(int i, uint32_t v) const { gst((uint32_t*)(p + i), v); }
-};
-struct LdsBytes {
-  uint8_t* p;
-  __device__ __forceinline__ void b(int i, uint8_t v) const { p[i] = v; }
-  __device__ __forceinline__ void d(int i, uint32_t v) const { *(uint32_t*)(p + i) = v; }
-};
-
-// Copy one string of len bytes from HBM.  Source bytes are read as the aligned
-// dwords that contain them (an aligned dword holding a byte of the buffer never
-// crosses a page boundary): up to nine independent loads for strings of at most
-// 32 bytes.  The destination gets byte stores up to its first 4-byte boundary,
-// dword stores, then byte stores for the tail, so exactly [dst, dst+len) is
-// written.
-template <class D>
-__device__ __forceinline__ void copy_string(D dst, const uint8_t* src, int len) {
+// Copy one string of len bytes HBM to HBM (the path for tiles whose utf8 bytes
+// do not fit the LDS buffer): byte stores up to the first 4-byte boundary of
+// dst, dwords assembled from byte loads, then the tail, so exactly
+// [dst, dst+len) is written.
+__device__ __forceinline__ void copy_string(uint8_t* dst, const uint8_t* src, int len) {
   if (len <= 0) return;
-  const int hb = min(len, (int)((4 - ((uintptr_t)dst.p & 3)) & 3));  // head bytes
-  if (len <= 32) {
-    // scalar registers rather than arrays, so nothing can end up in scratch
-    const uint32_t* a = (const uint32_t*)((uintptr_t)src & ~(uintptr_t)3);
-    const uint32_t sh = (uint32_t)((uintptr_t)src & 3);
-    const int ndw = (int)((sh + len + 3) >> 2);  // <= 9
-#define GZ_W(k) const uint32_t w##k = (k) < ndw ? gld(a + (k)) : 0u;
-    GZ_W(0) GZ_W(1) GZ_W(2) GZ_W(3) GZ_W(4) GZ_W(5) GZ_W(6) GZ_W(7) GZ_W(8)
-#undef GZ_W
-    const uint32_t w9 = 0;
-    // head: string bytes [0, hb)
-    const uint32_t h0 = funnel(w0, w1, sh);
-    for (int q = 0; q < hb; q++) dst.b(q, (uint8_t)(h0 >> (8 * q)));
-    // body: dword m holds string bytes [hb + 4m, hb + 4m + 4), the source dword
-    // stream from byte sh + hb
-    const uint32_t s0 = sh + (uint32_t)hb;
-    const uint32_t r = s0 & 3, b = s0 >> 2;  // b <= 1
-#define GZ_U(k, k1) const uint32_t u##k = funnel(w##k, w##k1, r);
-    GZ_U(0, 1) GZ_U(1, 2) GZ_U(2, 3) GZ_U(3, 4) GZ_U(4, 5) GZ_U(5, 6) GZ_U(6, 7) GZ_U(7, 8) GZ_U(8, 9)
-#undef GZ_U
-    const int body = len - hb;
-    const int nfull = body >> 2;
-    uint32_t tail = 0;
-#define GZ_S(m, m1)                              \
-  {                                              \
-    const uint32_t x = b ? u##m1 : u##m;         \
-    if ((m) < nfull) dst.d(hb + 4 * (m), x);     \
-    if ((m) == nfull) tail = x;                  \
-  }
-    GZ_S(0, 1) GZ_S(1, 2) GZ_S(2, 3) GZ_S(3, 4) GZ_S(4, 5) GZ_S(5, 6) GZ_S(6, 7) GZ_S(7, 8)
-#undef GZ_S
-    const int tb = body & 3;
-    const int t0 = hb + 4 * nfull;
-    for (int q = 0; q < tb; q++) dst.b(t0 + q, (uint8_t)(tail >> (8 * q)));
-    return;
-  }
+  const int hb = min(len, (int)((4 - ((uintptr_t)dst & 3)) & 3));
   int i = 0;
-  for (; i < hb; i++) dst.b(i, gld(src + i));
+  for (; i < hb; i++) gst(dst + i, gld(src + i));
   for (; i + 4 <= len; i += 4) {
     uint32_t x = (uint32_t)gld(src + i) | ((uint32_t)gld(src + i + 1) << 8) |
                  ((uint32_t)gld(src + i + 2) << 16) | ((uint32_t)gld(src + i + 3) << 24);
-    dst.d(i, x);
+    gst((uint32_t*)(dst + i), x);
   }
-  for (; i < len; i++) dst.b(i, gld(src + i));
+  for (; i < len; i++) gst(dst + i, gld(src + i));
 }
 
-// One wave writes bytes [r0, r0 + len) of an LDS buffer to dst in HBM: byte
-// stores up to the first 4-byte boundary of dst, then one dword per lane per
-// step (contiguous across the wave), then the tail bytes.  The buffer holds 8
-// readable bytes past its end.
-__device__ __forceinline__ void write_run(uint8_t* dst, const uint8_t* buf, uint32_t r0, int len, int lane) {
+// Bytes [b0, b0 + len) of an LDS buffer to dst in HBM: byte stores up to the
+// first 4-byte boundary of dst, dword stores (each assembled from two aligned LDS
+// dwords), then the tail bytes.  The buffer holds 8 readable bytes past its end.
+__device__ __forceinline__ void lds_to_hbm(uint8_t* dst, const uint8_t* buf, uint32_t b0, int len) {
+  if (len <= 0) return;
   const int head = min(len, (int)((4 - ((uintptr_t)dst & 3)) & 3));
-  if (lane < head) gst(dst + lane, buf[r0 + lane]);
+  for (int q = 0; q < head; q++) gst(dst + q, buf[b0 + q]);
   const int body = len - head;
   const int nd = body >> 2;
-  const uint32_t sb = r0 + (uint32_t)head;
+  const uint32_t sb = b0 + (uint32_t)head;
   const uint32_t sh = sb & 3;
   const uint32_t* b32 = (const uint32_t*)buf + (sb >> 2);
   uint32_t* d32 = (uint32_t*)(dst + head);
-  for (int m = lane; m < nd; m += kWave) gst(d32 + m, funnel(b32[m], b32[m + 1], sh));
+  uint32_t lo = b32[0];
+  for (int m = 0; m < nd; m++) {
+    const uint32_t hi = b32[m + 1];
+    gst(d32 + m, funnel(lo, hi, sh));
+    lo = hi;
+  }
   const int tb = body & 3;
-  if (lane < tb) gst((uint8_t*)(d32 + nd) + lane, buf[sb + 4 * nd + lane]);
+  uint8_t* dt = (uint8_t*)(d32 + nd);
+  for (int q = 0; q < tb; q++) gst(dt + q, buf[sb + 4 * nd + q]);
 }
 
 // LDS layout of the scatter kernel (bytes from the start of dynamic LDS).
 struct ScatterLds {
-  size_t scan, pbase, bbase, brel, toff, wcnt, bp, spid, pos, stage, cbuf, total;
+  size_t scan, pbase, bbase, brel, toff, wcnt, spid, pos, stage, cbuf, total;
 };
 // cbuf: bytes of the partition-ordered utf8 buffer (0 without binary columns).
 __host__ __device__ inline ScatterLds scatter_layout(int TILE, int BLOCK, int P, int cbuf) {
@@ -447,7 +411,6 @@ __host__ __device__ inline ScatterLds scatter_layout(int TILE, int BLOCK, int P,
   L.brel = take(4 * (size_t)P, 4);
   L.toff = take(4 * (size_t)(P + 1), 4);
   L.wcnt = take(2 * (size_t)NW * P, 4);
-  L.bp = take(8 * (size_t)kMaxCols * 3, 8);
   L.spid = take(2 * (size_t)TILE, 4);
   L.pos = take(2 * (size_t)TILE, 4);
   L.stage = take((size_t)TILE * 8 + 8, 16);
@@ -456,9 +419,16 @@ __host__ __device__ inline ScatterLds scatter_layout(int TILE, int BLOCK, int P,
   return L;
 }
 
-// Rows of a tile are owned in the ranking, loading and validity steps by
+// Rows of a tile are owned in the loading, ranking and validity steps by
 // thread (wave w, lane l) as rows w*RW + j*64 + l, j < RPT (coalesced per wave);
-// sorted positions are owned in the writing steps as j*BLOCK + tid.
+// sorted slots are owned in the writing steps as j*BLOCK + tid.
+//
+// Every load that does not depend on the ranking is issued first: the values of
+// up to kPre fixed-width columns and their validity, the offsets of the first
+// utf8 column and the tile's first-row table, so one HBM round trip overlaps the
+// hashing and ranking instead of one round trip per column.  The tile's utf8
+// bytes are copied into LDS as one contiguous block (coalesced 16-byte loads);
+// each sorted slot then writes its string from LDS.
 template <int TILE, int BLOCK, int KEYK>
 __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatter_kernel(SplitParams p) {
   constexpr int NW = BLOCK / kWave;
@@ -479,35 +449,66 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
   uint32_t* s_brel = (uint32_t*)(smem + Lo.brel);
   uint32_t* s_toff = (uint32_t*)(smem + Lo.toff);
   uint16_t* s_wcnt = (uint16_t*)(smem + Lo.wcnt);
-  const void** s_bp = (const void**)(smem + Lo.bp);
   uint16_t* s_spid = (uint16_t*)(smem + Lo.spid);
   uint16_t* s_pos = (uint16_t*)(smem + Lo.pos);  // sorted slot of every tile row
   uint8_t* s_stage = smem + Lo.stage;
   uint8_t* s_cbuf = smem + Lo.cbuf;
   const uint32_t* trow = p.prefix + t * p.R;  // this tile's exclusive prefixes
-
-  load_tile_ptrs(p, td.batch, s_bp);
-  for (int i = tid; i < NW * P; i += BLOCK) s_wcnt[i] = 0;
-  // first destination row of every partition run of this tile (issued early so
-  // the loads overlap the ranking)
-  for (int pp = tid; pp < P; pp += BLOCK) s_pbase[pp] = (uint32_t)(gld(p.row_start + pp) + gld(trow + pp));
-  __syncthreads();
-
+  // the batch's buffer table (uniform: scalar loads)
+  const void* const* bp = p.bufs + (int64_t)td.batch * p.bstride;
+  auto colp = [&](int c, int k) -> const uint8_t* { return (const uint8_t*)bp[c * 3 + k]; };
   const int64_t rb = td.row_begin;
   auto lrow = [&](int j) { return w * RW + j * kWave + lane; };
 
+  // ---- 0: loads that do not depend on the ranking
+  for (int i = tid; i < NW * P; i += BLOCK) s_wcnt[i] = 0;
+  for (int pp = tid; pp < P; pp += BLOCK) s_pbase[pp] = (uint32_t)(gld(p.row_start + pp) + gld(trow + pp));
+  constexpr int NPRE = scatter_max_pre(RPT);
+  uint64_t pv[NPRE][RPT];
+  uint32_t nullm[RPT];  // bit k: the row is null in prefetched column k
+#pragma unroll
+  for (int j = 0; j < RPT; j++) nullm[j] = 0;
+#pragma unroll
+  for (int k = 0; k < NPRE; k++) {
+#pragma unroll
+    for (int j = 0; j < RPT; j++) pv[k][j] = 0;
+    if (k < p.npre) {
+      const int c = p.pre_col[k];
+      const uint8_t* v = colp(c, 2);
+      const uint8_t* vbm = colp(c, 0);
+      const int W = p.width[c];
+#pragma unroll
+      for (int j = 0; j < RPT; j++) {
+        if (lrow(j) < n) {
+          const int64_t r = rb + lrow(j);
+          if (W == 8) pv[k][j] = gld((const uint64_t*)v + r);
+          else if (W == 4) pv[k][j] = gld((const uint32_t*)v + r);
+          else if (W == 2) pv[k][j] = gld((const uint16_t*)v + r);
+          else pv[k][j] = gld(v + r);
+          if (vbm && !bit_get(vbm, r)) nullm[j] |= 1u << k;
+        }
+      }
+    }
+  }
+  int32_t o0[RPT], o1[RPT];
+  const int bc = p.bin_col0;
+  if (p.nbin) {
+    const int32_t* off = (const int32_t*)colp(bc, 1) + rb;
+#pragma unroll
+    for (int j = 0; j < RPT; j++) {
+      const bool a = lrow(j) < n;
+      o0[j] = a ? gld(off + lrow(j)) : 0;
+      o1[j] = a ? gld(off + lrow(j) + 1) : 0;
+    }
+  }
+  __syncthreads();
+
   // ---- 1: partition id and stable rank within the wave's rows
   int pid[RPT], pos[RPT];
-  if (KEYK == 1) {
-    const int64_t* key = (const int64_t*)s_bp[p.key_col[0] * 3 + 2] + rb;
-    uint64_t kv[RPT];
 #pragma unroll
-    for (int j = 0; j < RPT; j++) kv[j] = lrow(j) < n ? (uint64_t)gld(key + lrow(j)) : 0;
-#pragma unroll
-    for (int j = 0; j < RPT; j++) pid[j] = pmod_hash(spark_hash_long(kv[j], p.seed), p.fm);
-  } else {
-#pragma unroll
-    for (int j = 0; j < RPT; j++) pid[j] = lrow(j) < n ? compute_pid(p, s_bp, td, lrow(j)) : 0;
+  for (int j = 0; j < RPT; j++) {
+    if (KEYK == 1) pid[j] = pmod_hash(spark_hash_long(pv[0][j], p.seed), p.fm);
+    else pid[j] = lrow(j) < n ? compute_pid(p, bp, td, lrow(j)) : 0;
   }
 #pragma unroll
   for (int j = 0; j < RPT; j++) {
@@ -518,11 +519,27 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
     pos[j] = before + __popcll(lt);
     if (act && lt == 0) s_wcnt[w * P + pid[j]] = (uint16_t)(before + __popcll(m));
   }
+  // utf8 bytes of the first binary column: one contiguous block in LDS, from
+  // the 16-byte boundary at or before the tile's first byte
+  const uint8_t* cbase = nullptr;
+  int64_t clen = 0;
+  uint32_t cshift = 0;
+  bool chars_lds = false;
+  if (p.nbin) {
+    const int32_t* off = (const int32_t*)colp(bc, 1);
+    const int64_t cb0 = gld(off + rb), cbn = gld(off + rb + n);
+    const uint8_t* first = colp(bc, 2) + cb0;
+    cbase = (const uint8_t*)((uintptr_t)first & ~(uintptr_t)15);
+    cshift = (uint32_t)(first - cbase) - (uint32_t)cb0;  // LDS offset of a source offset o: o + cshift
+    clen = (int64_t)(first - cbase) + (cbn - cb0);
+    chars_lds = clen + 16 <= (int64_t)p.cbuf;
+    if (chars_lds)
+      for (int64_t k = tid; k * 16 < clen; k += BLOCK) *(u32x4*)(s_cbuf + k * 16) = gld((const u32x4*)cbase + k);
+  }
   __syncthreads();
 
   // ---- 2: per partition: exclusive scan of the wave counts (in place; a
-  // segment of NW lanes per partition, lane k holding wave k), the tile count,
-  // then the tile offsets (exclusive scan over partitions)
+  // segment of NW lanes per partition, lane k holding wave k) and the tile count
   {
     constexpr int GPB = BLOCK / NW;  // partitions per pass
     const int k = tid % NW, g = tid / NW;
@@ -542,55 +559,89 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
     }
   }
   __syncthreads();
-  {
-    constexpr int PPT = (kMaxPartitions + BLOCK - 1) / BLOCK;
-    uint32_t sum = 0;
-    uint32_t cnt[PPT];
-#pragma unroll
-    for (int k = 0; k < PPT; k++) {
-      int pp = tid * PPT + k;
-      cnt[k] = pp < P ? s_toff[pp] : 0u;
-      sum += cnt[k];
+  // tile offsets: one wave scans the partition counts (lane l owns a contiguous
+  // range of partitions)
+  if (w == 0) {
+    const int per = (P + kWave - 1) / kWave;
+    const int p0 = min(P, lane * per), p1 = min(P, p0 + per);
+    uint32_t s = 0;
+    for (int pp = p0; pp < p1; pp++) s += s_toff[pp];
+    const uint32_t inc = wave_inclusive_scan(s);
+    uint32_t run = inc - s;
+    for (int pp = p0; pp < p1; pp++) {
+      const uint32_t c = s_toff[pp];
+      s_toff[pp] = run;
+      run += c;
     }
-    uint64_t total;
-    uint64_t ex = block_exclusive_scan<uint64_t>(sum, s_scan, &total);
-    uint32_t run = (uint32_t)ex;
-#pragma unroll
-    for (int k = 0; k < PPT; k++) {
-      int pp = tid * PPT + k;
-      if (pp < P) {
-        s_toff[pp] = run;
-        run += cnt[k];
-      }
-    }
-    if (tid == 0) s_toff[P] = (uint32_t)total;
+    if (lane == kWave - 1) s_toff[P] = inc;
   }
   __syncthreads();
 
-  // ---- 3: sorted position of every owned row; partition of every sorted slot
+  // ---- 3: sorted slot of every owned row; partition of every sorted slot;
+  // s_pbase[q] becomes the first segment row of partition q's run minus its
+  // tile offset, so sorted slot i goes to segment row s_pbase[q] + i
 #pragma unroll
   for (int j = 0; j < RPT; j++) {
     if (lrow(j) < n) {
-      int sp = pos[j] + (int)s_toff[pid[j]] + s_wcnt[w * P + pid[j]];
-      s_spid[sp] = (uint16_t)pid[j];
-      s_pos[lrow(j)] = (uint16_t)sp;
+      pos[j] += (int)s_toff[pid[j]] + s_wcnt[w * P + pid[j]];
+      s_spid[pos[j]] = (uint16_t)pid[j];
+      s_pos[lrow(j)] = (uint16_t)pos[j];
     }
   }
+  for (int q = tid; q < P; q += BLOCK) s_pbase[q] -= s_toff[q];
   __syncthreads();
-  // destination row (a row index of the segment) of sorted slot i
-  auto drow = [&](int i) -> uint32_t {
-    int q = s_spid[i];
-    return s_pbase[q] + (uint32_t)i - s_toff[q];
-  };
+  auto drow = [&](int i) -> uint32_t { return s_pbase[s_spid[i]] + (uint32_t)i; };
 
-  // ---- 4: columns
+  // ---- 4a: prefetched fixed-width columns, from registers
+#pragma unroll
+  for (int k = 0; k < NPRE; k++) {
+    if (k >= p.npre) break;
+    const int c = p.pre_col[k];
+    const int oc = p.out_col[c];
+    if (p.out_valid[oc]) {
+      uint32_t* ov = p.out_valid[oc];
+#pragma unroll
+      for (int j = 0; j < RPT; j++) {
+        if (lrow(j) < n && (nullm[j] >> k & 1)) {
+          const uint32_t d = drow(pos[j]);
+          atomicAnd(&ov[d >> 5], ~(1u << (d & 31)));
+        }
+      }
+    }
+    const int W = p.width[c];
+    uint8_t* dst = (uint8_t*)p.out_values[oc];
+#pragma unroll
+    for (int j = 0; j < RPT; j++) {
+      if (lrow(j) < n) {
+        if (W == 8) ((uint64_t*)s_stage)[pos[j]] = pv[k][j];
+        else if (W == 4) ((uint32_t*)s_stage)[pos[j]] = (uint32_t)pv[k][j];
+        else if (W == 2) ((uint16_t*)s_stage)[pos[j]] = (uint16_t)pv[k][j];
+        else s_stage[pos[j]] = (uint8_t)pv[k][j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RPT; j++) {
+      const int i = j * BLOCK + tid;
+      if (i < n) {
+        const uint64_t d = drow(i);
+        if (W == 8) gst((uint64_t*)dst + d, ((uint64_t*)s_stage)[i]);
+        else if (W == 4) gst((uint32_t*)dst + d, ((uint32_t*)s_stage)[i]);
+        else if (W == 2) gst((uint16_t*)dst + d, ((uint16_t*)s_stage)[i]);
+        else gst(dst + d, s_stage[i]);
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- 4b: every other column
   for (int c = 0; c < p.ncols; c++) {
     int oc = p.out_col[c];
-    if (oc < 0) continue;
+    if (oc < 0 || p.is_pre[c]) continue;
     int kind = p.kind[c];
     if (kind == K_NA) continue;
-    const uint8_t* vb = (const uint8_t*)s_bp[c * 3 + 0];
-    const uint8_t* src = (const uint8_t*)s_bp[c * 3 + 2];
+    const uint8_t* vb = colp(c, 0);
+    const uint8_t* src = colp(c, 2);
 
     // validity: clear the bit of every null row at its destination
     if (vb && p.out_valid[oc]) {
@@ -598,7 +649,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
 #pragma unroll
       for (int j = 0; j < RPT; j++) {
         if (lrow(j) < n && !bit_get(vb, rb + lrow(j))) {
-          uint32_t d = drow(s_pos[lrow(j)]);
+          uint32_t d = drow(pos[j]);
           atomicAnd(&ov[d >> 5], ~(1u << (d & 31)));
         }
       }
@@ -609,7 +660,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
 #pragma unroll
       for (int j = 0; j < RPT; j++) {
         if (lrow(j) < n && !bit_get(src, rb + lrow(j))) {
-          uint32_t d = drow(s_pos[lrow(j)]);
+          uint32_t d = drow(pos[j]);
           atomicAnd(&ov[d >> 5], ~(1u << (d & 31)));
         }
       }
@@ -626,7 +677,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
           for (int j = 0; j < RPT; j++) v[j] = lrow(j) < n ? gld(s8 + (rb + lrow(j)) * parts + part) : 0;
 #pragma unroll
           for (int j = 0; j < RPT; j++)
-            if (lrow(j) < n) ((uint64_t*)s_stage)[s_pos[lrow(j)]] = v[j];
+            if (lrow(j) < n) ((uint64_t*)s_stage)[pos[j]] = v[j];
         } else if (EW == 4) {
           const uint32_t* s4 = (const uint32_t*)src;
           uint32_t v[RPT];
@@ -634,7 +685,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
           for (int j = 0; j < RPT; j++) v[j] = lrow(j) < n ? gld(s4 + rb + lrow(j)) : 0;
 #pragma unroll
           for (int j = 0; j < RPT; j++)
-            if (lrow(j) < n) ((uint32_t*)s_stage)[s_pos[lrow(j)]] = v[j];
+            if (lrow(j) < n) ((uint32_t*)s_stage)[pos[j]] = v[j];
         } else if (EW == 2) {
           const uint16_t* s2 = (const uint16_t*)src;
           uint16_t v[RPT];
@@ -642,14 +693,14 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
           for (int j = 0; j < RPT; j++) v[j] = lrow(j) < n ? gld(s2 + rb + lrow(j)) : 0;
 #pragma unroll
           for (int j = 0; j < RPT; j++)
-            if (lrow(j) < n) ((uint16_t*)s_stage)[s_pos[lrow(j)]] = v[j];
+            if (lrow(j) < n) ((uint16_t*)s_stage)[pos[j]] = v[j];
         } else {
           uint8_t v[RPT];
 #pragma unroll
           for (int j = 0; j < RPT; j++) v[j] = lrow(j) < n ? gld(src + rb + lrow(j)) : 0;
 #pragma unroll
           for (int j = 0; j < RPT; j++)
-            if (lrow(j) < n) s_stage[s_pos[lrow(j)]] = v[j];
+            if (lrow(j) < n) s_stage[pos[j]] = v[j];
         }
         __syncthreads();
 #pragma unroll 1
@@ -666,23 +717,23 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
       }
     } else if (kind == K_BINARY) {
       const int j_bin = p.bin[c];
-      const int32_t* soff = (const int32_t*)s_bp[c * 3 + 1] + rb;
+      const bool first = c == bc;  // offsets prefetched, bytes possibly in LDS
       uint32_t* s_src = (uint32_t*)s_stage;  // [TILE] source byte offset by sorted slot
       uint32_t* s_boff = s_src + TILE;       // [TILE+1] sorted byte prefix
       {
-        int32_t o0[RPT], o1[RPT];
+        int32_t a0[RPT], a1[RPT];
+        const int32_t* soff = (const int32_t*)colp(c, 1) + rb;
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
-          bool a = lrow(j) < n;
-          o0[j] = a ? gld(soff + lrow(j)) : 0;
-          o1[j] = a ? gld(soff + lrow(j) + 1) : 0;
+          const bool a = lrow(j) < n;
+          a0[j] = first ? o0[j] : (a ? gld(soff + lrow(j)) : 0);
+          a1[j] = first ? o1[j] : (a ? gld(soff + lrow(j) + 1) : 0);
         }
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
           if (lrow(j) < n) {
-            const int sp = s_pos[lrow(j)];
-            s_src[sp] = (uint32_t)o0[j];
-            s_boff[sp] = (uint32_t)(o1[j] - o0[j]);
+            s_src[pos[j]] = (uint32_t)a0[j];
+            s_boff[pos[j]] = (uint32_t)(a1[j] - a0[j]);
           }
         }
       }
@@ -714,45 +765,38 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
         if (tid == 0) s_boff[n] = (uint32_t)total;
       }
       __syncthreads();
+      // per partition: offset value and data address adjustments relative to
+      // the tile-sorted byte prefix
+      uint32_t* s_badj = s_brel;  // brel - run start
+      int64_t* s_dadj = s_bbase;  // bbase - run start
+      for (int q = tid; q < P; q += BLOCK) {
+        const uint32_t run0 = s_boff[s_toff[q]];
+        s_badj[q] -= run0;
+        s_dadj[q] -= (int64_t)run0;
+      }
+      __syncthreads();
       uint8_t* odata = p.out_data[oc];
       int32_t* ooff = p.out_off[oc];
-      // offsets, in sorted-slot order (contiguous per partition run)
+      // offsets, in sorted-slot order (contiguous per partition run); offsets of
+      // partition q start at index row_start + q
 #pragma unroll 1
       for (int i = tid; i < n; i += BLOCK) {
-        int q = s_spid[i];
-        uint32_t toff = s_toff[q];
-        uint32_t run0 = s_boff[toff];
-        uint32_t b1 = s_boff[i + 1];
-        uint32_t d = s_pbase[q] + (uint32_t)i - toff;
-        // offsets of partition q start at index row_start + q
-        gst(ooff + (int64_t)d + q + 1, (int32_t)(s_brel[q] + (b1 - run0)));
+        const int q = s_spid[i];
+        gst(ooff + (int64_t)(s_pbase[q] + (uint32_t)i) + q + 1, (int32_t)(s_badj[q] + s_boff[i + 1]));
       }
-      if (s_boff[n] <= (uint32_t)p.cbuf) {
-        // strings of the owned rows (source order: a wave reads one contiguous
-        // span) into the partition-ordered LDS buffer ...
-#pragma unroll 1
-        for (int j = 0; j < RPT; j++) {
-          if (lrow(j) < n) {
-            const int sp = s_pos[lrow(j)];
-            const uint32_t u = s_boff[sp];
-            copy_string(LdsBytes{s_cbuf + u}, src + s_src[sp], (int)(s_boff[sp + 1] - u));
-          }
-        }
-        __syncthreads();
-        // ... then every partition run leaves as one contiguous span per wave
-#pragma unroll 1
-        for (int q = w; q < P; q += NW) {
-          const uint32_t r0 = s_boff[s_toff[q]], r1 = s_boff[s_toff[q + 1]];
-          if (r1 > r0) write_run(odata + s_bbase[q], s_cbuf, r0, (int)(r1 - r0), lane);
-        }
-      } else {
-        // tile bytes exceed the buffer: every sorted slot copies its string
+      if (first && chars_lds) {
+        // every sorted slot writes its string from the LDS copy of the tile's
+        // bytes; neighbouring lanes write neighbouring bytes of a partition run
 #pragma unroll 1
         for (int i = tid; i < n; i += BLOCK) {
-          int q = s_spid[i];
-          uint32_t run0 = s_boff[s_toff[q]];
-          uint32_t b0 = s_boff[i];
-          copy_string(HbmBytes{odata + s_bbase[q] + (b0 - run0)}, src + s_src[i], (int)(s_boff[i + 1] - b0));
+          const uint32_t b0 = s_boff[i];
+          lds_to_hbm(odata + s_dadj[s_spid[i]] + b0, s_cbuf, s_src[i] + cshift, (int)(s_boff[i + 1] - b0));
+        }
+      } else {
+#pragma unroll 1
+        for (int i = tid; i < n; i += BLOCK) {
+          const uint32_t b0 = s_boff[i];
+          copy_string(odata + s_dadj[s_spid[i]] + b0, src + s_src[i], (int)(s_boff[i + 1] - b0));
         }
       }
       __syncthreads();
@@ -762,9 +806,9 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
 
 // Host-side LDS bytes and block size of the scatter kernel.
 static size_t scatter_lds(int TILE, int BLOCK, int P, int cbuf) { return scatter_layout(TILE, BLOCK, P, cbuf).total; }
-// Partition-ordered utf8 staging: about 22 bytes per row, so tiles of strings
-// up to that average length go through LDS.
-static int scatter_cbuf(int TILE, int nbin) { return nbin > 0 ? 22 * TILE : 0; }
+// Partition-ordered utf8 staging: 21 bytes per row, so tiles of strings up to
+// that average length go through LDS (and two 2048-row workgroups fit a CU).
+static int scatter_cbuf(int TILE, int nbin) { return nbin > 0 ? 21 * TILE : 0; }
 static int scatter_block(int P) { return P > 1024 ? 512 : 1024; }
 
 }  // namespace gz
@@ -917,8 +961,9 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
     b.bin[c] = type_is_binary(t) ? nbin++ : -1;
   }
   b.nbin = nbin;
-  // default tile: 2048 rows when utf8 strings are staged through LDS, else 4096
-  if (h->tile_rows == 0) h->tile_rows = nbin > 0 ? 2048 : 4096;
+  // default tile: 2048 rows (two 1024-thread workgroups per CU, also with the
+  // utf8 staging buffer)
+  if (h->tile_rows == 0) h->tile_rows = 2048;
   if (P > 1024) h->tile_rows = 2048;  // 512-thread scatter for large fan-out
   if (scatter_lds(h->tile_rows, scatter_block(P), P, scatter_cbuf(h->tile_rows, nbin)) > 160 * 1024) {
     delete h;
@@ -1097,6 +1142,19 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   // hot path: hash of one int64 key that has no validity buffer in this call
   const int keyk = (h->mode == 0 && prm.nkeys == 1 && prm.key_type[0] == GZ_TYPE_INT64 &&
                     !has_validity[prm.key_col[0]]) ? 1 : 0;
+  // fixed-width columns the scatter loads before ranking (the key first when the
+  // int64-key path hashes it from those registers)
+  prm.npre = 0;
+  for (int c = 0; c < nc; c++) prm.is_pre[c] = 0;
+  const int max_pre = scatter_max_pre(TILE / scatter_block(P));
+  auto add_pre = [&](int c) {
+    if (prm.npre >= max_pre || prm.is_pre[c] || prm.out_col[c] < 0 || prm.kind[c] != K_FIXED || prm.width[c] > 8)
+      return;
+    prm.pre_col[prm.npre++] = c;
+    prm.is_pre[c] = 1;
+  };
+  if (keyk) add_pre(prm.key_col[0]);
+  for (int c = 0; c < nc; c++) add_pre(c);
   if (T > 0) {
     KernelTimer kt(ctx, "split_count");
     size_t lds = sizeof(uint32_t) * R;

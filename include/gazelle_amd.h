@@ -242,6 +242,27 @@ int gz_agg_set_result_column(gz_agg* h, int32_t inst, int32_t col, void** values
 /* Placement the autotuner chose for an instance: 1 = LDS first level, 2 = HBM. */
 int gz_agg_set_placement(gz_agg* h, int32_t inst, int32_t* out);
 
+/* Planned, replayable form of a set.  gz_agg_set_prepare sizes every
+ * instance's table for the given batches (cardinality estimate, allocation,
+ * placement) and records the launch plan; gz_agg_set_run executes it (table
+ * reset and update kernels only: no allocation, no host synchronization, no
+ * memset); gz_agg_set_capture records that sequence into a HIP graph and
+ * gz_agg_set_launch replays it (or runs it when nothing was captured).  After a
+ * run or a launch, gz_agg_finish produces that run's results (and reports
+ * GZ_CAPACITY_ERROR if an estimated table overflowed).  The batches must stay
+ * valid while the plan is used.  No counterpart in the reference. */
+int gz_agg_set_prepare(gz_agg* h, int32_t num_batches, const gz_batch* batches,
+                       const int32_t* instance_of_batch);
+int gz_agg_set_run(gz_agg* h);
+int gz_agg_set_capture(gz_agg* h);
+int gz_agg_set_launch(gz_agg* h);
+/* Autotuner: prepares and times candidate plans (update tile rows, LDS
+ * placement threshold, table load factor) on the given batches and keeps the
+ * fastest prepared; *chosen receives its index.  No counterpart in the
+ * reference. */
+int gz_agg_set_autotune(gz_agg* h, int32_t num_batches, const gz_batch* batches,
+                        const int32_t* instance_of_batch, int32_t* chosen);
+
 /* ---- hash join -----------------------------------------------------------------
  * Build replaces HashRelationKernel (ext/hash_relation_kernel.cc:50-344) and
  * HashRelation (codegen/common/hash_relation.h:123-645); probe replaces

@@ -123,3 +123,74 @@ def test_aggregate_set_instances_independent(ctx):
     for i, bl in enumerate(inst_batches):
         assert_results_equal(it.results(i), expected(bl, datagen.AGG_TYPES, CONFIG0_ACTIONS), RTOL)
     it.close()
+
+
+def _instances(seed, count=10):
+    rng = np.random.default_rng(seed)
+    inst_batches = []
+    for i in range(count):
+        rows = int(rng.integers(500, 30000))
+        keys = int(rng.integers(3, 20000))
+        inst_batches.append(datagen.agg_host(rows, num_keys=keys, batch_rows=4096, seed=seed * 100 + i))
+    dev, inst_of = [], []
+    for i, bl in enumerate(inst_batches):
+        for b in bl:
+            dev.append(DeviceBatch.from_host(b))
+            inst_of.append(i)
+    return inst_batches, dev, inst_of
+
+
+def test_planned_set_run_matches_oracle(ctx):
+    """prepare + run (kernels only) gives each instance's oracle result, twice in a row."""
+    from gazelle_plugin_amd.aggregate import HashAggregateResultIterator
+
+    inst_batches, dev, inst_of = _instances(5)
+    k = HashAggregateKernel.Make(ctx, datagen.AGG_TYPES, CONFIG0_ACTIONS)
+    it = HashAggregateResultIterator(k, num_instances=len(inst_batches))
+    it.Prepare(dev, inst_of)
+    exp = [expected(bl, datagen.AGG_TYPES, CONFIG0_ACTIONS) for bl in inst_batches]
+    for _ in range(2):
+        it.Run()
+        for i in range(len(inst_batches)):
+            assert_results_equal(it.results(i), exp[i], RTOL)
+    it.close()
+
+
+def test_graph_replays_match_eager(ctx):
+    """The captured graph replayed several times, with eager work and host
+    synchronization between replays, gives the eager run's results every time."""
+    import torch
+
+    from gazelle_plugin_amd.aggregate import HashAggregateResultIterator
+
+    inst_batches, dev, inst_of = _instances(6)
+    k = HashAggregateKernel.Make(ctx, datagen.AGG_TYPES, CONFIG0_ACTIONS)
+    it = HashAggregateResultIterator(k, num_instances=len(inst_batches))
+    it.Prepare(dev, inst_of)
+    it.Run()
+    eager = [it.results(i) for i in range(len(inst_batches))]
+    it.Capture()
+    for rep in range(3):
+        x = torch.ones(1 << 20, device="cuda") * rep  # unrelated eager work between replays
+        it.Launch()
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        for i in range(len(inst_batches)):
+            assert_results_equal(it.results(i), eager[i], RTOL)
+        del x
+    it.close()
+
+
+def test_autotuned_plan_matches_oracle(ctx):
+    from gazelle_plugin_amd.aggregate import HashAggregateResultIterator
+
+    inst_batches, dev, inst_of = _instances(7, count=6)
+    k = HashAggregateKernel.Make(ctx, datagen.AGG_TYPES, CONFIG0_ACTIONS)
+    it = HashAggregateResultIterator(k, num_instances=len(inst_batches))
+    chosen = it.Autotune(dev, inst_of)
+    assert 0 <= chosen < 18
+    it.Capture()
+    it.Launch()
+    for i, bl in enumerate(inst_batches):
+        assert_results_equal(it.results(i), expected(bl, datagen.AGG_TYPES, CONFIG0_ACTIONS), RTOL)
+    it.close()
