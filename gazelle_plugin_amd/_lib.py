@@ -49,7 +49,7 @@ class GzBatch(ctypes.Structure):
 class GzSplitOptions(ctypes.Structure):
     _fields_ = [("buffer_size", ctypes.c_int32), ("prefer_spill", ctypes.c_int32),
                 ("offheap_per_task", ctypes.c_int64), ("tile_rows", ctypes.c_int32),
-                ("hash_seed", ctypes.c_int32)]
+                ("hash_seed", ctypes.c_int32), ("append_row_ids", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class GzAggAction(ctypes.Structure):
@@ -60,6 +60,35 @@ class GzAggAction(ctypes.Structure):
 class GzAggOptions(ctypes.Structure):
     _fields_ = [("placement", ctypes.c_int32), ("tile_rows", ctypes.c_int32),
                 ("load_factor", ctypes.c_double), ("expected_groups", ctypes.c_int64)]
+
+
+class GzAggPlan(ctypes.Structure):
+    _fields_ = [("num_cols", ctypes.c_int32), ("col_types", ctypes.POINTER(ctypes.c_int32)),
+                ("num_actions", ctypes.c_int32), ("actions", ctypes.POINTER(GzAggAction))]
+
+
+class GzSplitPlan(ctypes.Structure):
+    _fields_ = [("num_cols", ctypes.c_int32), ("col_types", ctypes.POINTER(ctypes.c_int32)),
+                ("num_partitions", ctypes.c_int32), ("key_cols", ctypes.POINTER(ctypes.c_int32)),
+                ("num_keys", ctypes.c_int32)]
+
+
+class GzProbePlan(ctypes.Structure):
+    _fields_ = [("num_probe_cols", ctypes.c_int32), ("probe_types", ctypes.POINTER(ctypes.c_int32)),
+                ("probe_key_col", ctypes.c_int32), ("join_type", ctypes.c_int32),
+                ("num_result_cols", ctypes.c_int32), ("result_sides", ctypes.POINTER(ctypes.c_int32)),
+                ("result_indices", ctypes.POINTER(ctypes.c_int32))]
+
+
+class GzFragment(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("relation", ctypes.c_int32), ("num_batches", ctypes.c_int32),
+                ("batches", ctypes.POINTER(GzBatch)), ("arrival_s", ctypes.c_double)]
+
+
+class GzFragmentStats(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("admitted_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("rows", ctypes.c_int64), ("out_rows", ctypes.c_int64), ("status", ctypes.c_int32),
+                ("slot", ctypes.c_int32)]
 
 
 _lib = None
@@ -124,6 +153,14 @@ _OPTIONAL_SIGS = {
     "gz_agg_set_capture": (ctypes.c_int, [_P]),
     "gz_agg_set_launch": (ctypes.c_int, [_P]),
     "gz_agg_set_autotune": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch), _PI32, _PI32]),
+    "gz_server_create": (ctypes.c_int, [_I32, _I64, _I32, ctypes.POINTER(GzAggPlan), ctypes.POINTER(GzSplitPlan),
+                                        ctypes.POINTER(GzProbePlan), ctypes.POINTER(GzAggPlan), _PP]),
+    "gz_server_add_relation": (ctypes.c_int, [_P, _P, _PI32]),
+    "gz_server_run": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzFragment), _I32, ctypes.POINTER(GzFragmentStats)]),
+    "gz_server_fragment_agg": (ctypes.c_int, [_P, _I32, _PP]),
+    "gz_server_fragment_split": (ctypes.c_int, [_P, _I32, _PP]),
+    "gz_server_release_results": (ctypes.c_int, [_P]),
+    "gz_server_destroy": (ctypes.c_int, [_P]),
     "gz_relation_make": (ctypes.c_int, [_P, _I32, _PI32, _I32, _I32, _PP]),
     "gz_relation_append": (ctypes.c_int, [_P, _I64, _PI64, _PI64, _I32]),
     "gz_relation_append_batches": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch)]),

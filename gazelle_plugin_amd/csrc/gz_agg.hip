@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,8 @@ constexpr int kMaxActions = 32;
 constexpr int64_t kEmpty = INT64_MIN;
 constexpr int kHllBits = 11;
 constexpr int kHllRegs = 1 << kHllBits;
+constexpr int kMaxUsed = 8;        // used input columns of a partition record
+constexpr int kMaxFanout = 256;    // partitions per instance on the partitioned path
 
 enum LaneOp : int {
   L_CNT = 0, L_NN, L_NN2, L_SUMI, L_SUMF, L_SUMCNT, L_MINI, L_MAXI, L_MINU, L_MAXU,
@@ -55,6 +58,8 @@ struct AggInst {
   uint64_t cap;           // power of two; slot cap = null key, cap + 1 = key equal to kEmpty
   unsigned int* counter;  // keys inserted
   int* overflow;
+  int32_t fan_log2;       // partitioned placement: log2 of the partition count (0 = off)
+  int32_t bucket_base;    // first global partition of the instance
 };
 
 struct AggTile {
@@ -77,6 +82,10 @@ struct AggParams {
   const AggInst* inst;
   int tiles_per_block;
   int lcap;
+  // partitioned placement: the used input columns of a record
+  int nu;                     // used columns (lane inputs other than the key)
+  int ucol[kMaxUsed];         // column of used slot k
+  int uslot[kAggMaxCols];     // used slot of a column
 };
 
 __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
@@ -178,10 +187,46 @@ __device__ __forceinline__ void max_u64(int64_t* p, uint64_t v) {
   if (v > (uint64_t)*p) atomicMax((unsigned long long*)p, (unsigned long long)v);
 }
 
+// Row accessors for apply_row: a row of the input Arrow buffers, or a partition
+// record (the row's used inputs widened to 8 bytes: integers as read_i64,
+// floats as double bits, uint64 raw, and a null mask).
+struct ArrowRow {
+  const AggParams& p;
+  int batch;
+  int64_t r;
+  __device__ __forceinline__ bool valid(int col) const { return is_valid(p, batch, col, r); }
+  __device__ __forceinline__ int64_t i64(int col) const {
+    return read_i64(abuf(p, batch, col, 2), p.col_type[col], r);
+  }
+  __device__ __forceinline__ double f64(int col) const {
+    return read_f64(abuf(p, batch, col, 2), p.col_type[col], r);
+  }
+  __device__ __forceinline__ uint64_t u64(int col) const { return ((const uint64_t*)abuf(p, batch, col, 2))[r]; }
+  __device__ __forceinline__ int64_t raw64(int col) const { return ((const int64_t*)abuf(p, batch, col, 2))[r]; }
+};
+
+struct RecRow {
+  const AggParams& p;
+  const int64_t* v;  // record values, by used-column slot
+  uint32_t nulls;    // bit k: used column k is null
+  __device__ __forceinline__ bool valid(int col) const { return !((nulls >> p.uslot[col]) & 1); }
+  __device__ __forceinline__ int64_t i64(int col) const { return v[p.uslot[col]]; }
+  __device__ __forceinline__ double f64(int col) const {
+    const int64_t x = v[p.uslot[col]];
+    const int t = p.col_type[col];
+    if (t == GZ_TYPE_DOUBLE || t == GZ_TYPE_FLOAT) return __longlong_as_double(x);
+    if (t == GZ_TYPE_UINT64) return (double)(uint64_t)x;
+    return (double)x;
+  }
+  __device__ __forceinline__ uint64_t u64(int col) const { return (uint64_t)v[p.uslot[col]]; }
+  __device__ __forceinline__ int64_t raw64(int col) const { return v[p.uslot[col]]; }
+};
+
 // Apply one row to the lanes of `slot` in a table laid out as base + lane * S.
 // Works on HBM and on LDS (the atomics are generic-address).
+template <class Row>
 __device__ __forceinline__ void apply_row(const AggParams& p, int64_t* base, int64_t S, int64_t slot,
-                                          int batch, int64_t r, int64_t grow) {
+                                          const Row& row, int64_t grow) {
   int64_t* fr = base + S + slot;
   if (grow < *fr) atomicMin((long long*)fr, (long long)grow);
   for (int l = 0; l < p.nl; l++) {
@@ -190,60 +235,54 @@ __device__ __forceinline__ void apply_row(const AggParams& p, int64_t* base, int
     switch (ln.op) {
       case L_CNT: atomicAdd((unsigned long long*)ptr, 1ull); break;
       case L_NN:
-        if (is_valid(p, batch, ln.col, r)) atomicAdd((unsigned long long*)ptr, 1ull);
+        if (row.valid(ln.col)) atomicAdd((unsigned long long*)ptr, 1ull);
         break;
       case L_NN2:
-        if (is_valid(p, batch, ln.col, r) && is_valid(p, batch, ln.col2, r))
-          atomicAdd((unsigned long long*)ptr, 1ull);
+        if (row.valid(ln.col) && row.valid(ln.col2)) atomicAdd((unsigned long long*)ptr, 1ull);
         break;
       case L_SUMI:
-        if (is_valid(p, batch, ln.col, r))
-          atomicAdd((unsigned long long*)ptr,
-                    (unsigned long long)read_i64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r));
+        if (row.valid(ln.col)) atomicAdd((unsigned long long*)ptr, (unsigned long long)row.i64(ln.col));
         break;
       case L_SUMF:
-        if (is_valid(p, batch, ln.col, r))
-          unsafeAtomicAdd((double*)ptr, read_f64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r));
+        if (row.valid(ln.col)) unsafeAtomicAdd((double*)ptr, row.f64(ln.col));
         break;
       case L_SUMCNT:
-        if (is_valid(p, batch, ln.col, r))
-          atomicAdd((unsigned long long*)ptr,
-                    (unsigned long long)((const int64_t*)abuf(p, batch, ln.col2, 2))[r]);
+        if (row.valid(ln.col)) atomicAdd((unsigned long long*)ptr, (unsigned long long)row.raw64(ln.col2));
         break;
       case L_MINI:
       case L_MAXI: {
-        if (!is_valid(p, batch, ln.col, r)) break;
-        int64_t v = read_i64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r);
+        if (!row.valid(ln.col)) break;
+        int64_t v = row.i64(ln.col);
         if (ln.op == L_MINI) min_i64(ptr, v); else max_i64(ptr, v);
         break;
       }
       case L_MINU:
       case L_MAXU: {
-        if (!is_valid(p, batch, ln.col, r)) break;
-        uint64_t v = ((const uint64_t*)abuf(p, batch, ln.col, 2))[r];
+        if (!row.valid(ln.col)) break;
+        uint64_t v = row.u64(ln.col);
         if (ln.op == L_MINU) min_u64(ptr, v); else max_u64(ptr, v);
         break;
       }
       case L_MINF:
       case L_MAXF: {
-        if (!is_valid(p, batch, ln.col, r)) break;
-        double d = read_f64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r);
+        if (!row.valid(ln.col)) break;
+        double d = row.f64(ln.col);
         if (d != d) break;
         if (ln.op == L_MINF) min_u64(ptr, ordered(d)); else max_u64(ptr, ordered(d));
         break;
       }
       case L_HASNAN: {
-        if (!is_valid(p, batch, ln.col, r)) break;
-        double d = read_f64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r);
+        if (!row.valid(ln.col)) break;
+        double d = row.f64(ln.col);
         if (d != d && *ptr == 0) atomicOr((unsigned long long*)ptr, 1ull);
         break;
       }
       case L_FIRSTNN:
-        if (is_valid(p, batch, ln.col, r) && grow < *ptr) atomicMin((long long*)ptr, (long long)grow);
+        if (row.valid(ln.col) && grow < *ptr) atomicMin((long long*)ptr, (long long)grow);
         break;
       case L_FIRSTNAN: {
-        if (!is_valid(p, batch, ln.col, r)) break;
-        double d = read_f64(abuf(p, batch, ln.col, 2), p.col_type[ln.col], r);
+        if (!row.valid(ln.col)) break;
+        double d = row.f64(ln.col);
         if (d != d && grow < *ptr) atomicMin((long long*)ptr, (long long)grow);
         break;
       }
@@ -344,7 +383,7 @@ __global__ __launch_bounds__(256) void agg_update_kernel(AggParams p) {
     bool valid = row_key(p, td.batch, r, &key);
     int64_t slot = table_slot(I, valid, key);
     if (slot < 0) continue;
-    apply_row(p, I.base, S, slot, td.batch, r, td.row_base + lr);
+    apply_row(p, I.base, S, slot, ArrowRow{p, td.batch, r}, td.row_base + lr);
   }
 }
 
@@ -417,15 +456,189 @@ __global__ __launch_bounds__(256) void agg_update_lds_kernel(AggParams p) {
         }
       }
       if (ls >= 0) {
-        apply_row(p, s_tab, LS, ls, td.batch, r, grow);
+        apply_row(p, s_tab, LS, ls, ArrowRow{p, td.batch, r}, grow);
       } else {
         int64_t g = table_slot(I, valid, key);
-        if (g >= 0) apply_row(p, I.base, (int64_t)I.cap + 2, g, td.batch, r, grow);
+        if (g >= 0) apply_row(p, I.base, (int64_t)I.cap + 2, g, ArrowRow{p, td.batch, r}, grow);
       }
     }
   }
   __syncthreads();
   lds_merge(p, p.inst[cur], s_tab, LS);
+}
+
+// ---------------------------------------------------------------- partitioned
+// Partitioned placement for medium cardinalities: the rows of an instance are
+// scattered into 2^fan_log2 partitions by the high bits of the key hash (the LDS
+// table indexes by the low bits), as fixed-size records carrying the row's
+// instance-global index, so the partitions need no stable order.  Each
+// partition's groups then fit an LDS table; workgroups aggregate chunks of
+// records in LDS and merge into the instance's HBM table, which partitions of
+// different keys never share.
+//
+// Record (8-byte words): key, first-row index | null mask << 48 (bit 15 = key
+// null), then the used input columns widened to 8 bytes.
+constexpr int kRecKeyNull = 15;
+
+__device__ __forceinline__ int part_of(const AggInst& I, bool valid, int64_t key) {
+  if (!valid || I.fan_log2 == 0) return 0;
+  return (int)(fmix64((uint64_t)key) >> (64 - I.fan_log2));
+}
+
+// Rows per partition (tile histogram in LDS, one global atomic per non-empty
+// partition of the tile).
+__global__ __launch_bounds__(256) void agg_part_hist_kernel(AggParams p, unsigned int* pcount) {
+  __shared__ unsigned int s_h[kMaxFanout];
+  int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
+  if (t < 0) return;
+  const AggTile td = p.tiles[t];
+  const AggInst I = p.inst[td.inst];
+  const int F = 1 << I.fan_log2;
+  for (int b = threadIdx.x; b < F; b += blockDim.x) s_h[b] = 0;
+  __syncthreads();
+  for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
+    int64_t key;
+    bool valid = row_key(p, td.batch, (int64_t)td.row_begin + lr, &key);
+    atomicAdd(&s_h[part_of(I, valid, key)], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < F; b += blockDim.x)
+    if (s_h[b]) atomicAdd(&pcount[I.bucket_base + b], s_h[b]);
+}
+
+// One block: partition starts (exclusive scan of the counts, total at [B]) and
+// the write cursors.
+__global__ __launch_bounds__(256) void agg_part_scan_kernel(const unsigned int* pcount, int B, int64_t* pstart,
+                                                             unsigned long long* pcursor) {
+  __shared__ uint64_t s_scan[256 / kWave + 1];
+  uint64_t carry = 0;
+  for (int base = 0; base < B; base += 256) {
+    int i = base + threadIdx.x;
+    uint64_t v = i < B ? pcount[i] : 0;
+    uint64_t total;
+    uint64_t ex = block_exclusive_scan<uint64_t>(v, s_scan, &total);
+    if (i < B) {
+      pstart[i] = (int64_t)(carry + ex);
+      pcursor[i] = carry + ex;
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) pstart[B] = (int64_t)carry;
+}
+
+// Records of a tile: partition counts in LDS, one range reserved per non-empty
+// partition, then each row takes the next place of its partition's range.
+__global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, unsigned long long* pcursor,
+                                                               int64_t* recs) {
+  __shared__ unsigned int s_h[kMaxFanout];
+  __shared__ unsigned long long s_base[kMaxFanout];
+  int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
+  if (t < 0) return;
+  const AggTile td = p.tiles[t];
+  const AggInst I = p.inst[td.inst];
+  const int F = 1 << I.fan_log2;
+  const int W = 2 + p.nu;
+  for (int b = threadIdx.x; b < F; b += blockDim.x) s_h[b] = 0;
+  __syncthreads();
+  for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
+    int64_t key;
+    bool valid = row_key(p, td.batch, (int64_t)td.row_begin + lr, &key);
+    atomicAdd(&s_h[part_of(I, valid, key)], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < F; b += blockDim.x) {
+    s_base[b] = s_h[b] ? atomicAdd(&pcursor[I.bucket_base + b], (unsigned long long)s_h[b]) : 0ull;
+    s_h[b] = 0;
+  }
+  __syncthreads();
+  for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
+    const int64_t r = (int64_t)td.row_begin + lr;
+    int64_t key;
+    bool valid = row_key(p, td.batch, r, &key);
+    const int b = part_of(I, valid, key);
+    const unsigned long long pos = s_base[b] + atomicAdd(&s_h[b], 1u);
+    int64_t* rec = recs + (int64_t)pos * W;
+    uint64_t nulls = valid ? 0ull : (1ull << kRecKeyNull);
+    for (int k = 0; k < p.nu; k++) {
+      const int c = p.ucol[k];
+      int64_t v = 0;
+      if (!is_valid(p, td.batch, c, r)) {
+        nulls |= 1ull << k;
+      } else {
+        const int ty = p.col_type[c];
+        const void* vb = abuf(p, td.batch, c, 2);
+        if (ty == GZ_TYPE_DOUBLE || ty == GZ_TYPE_FLOAT) v = __double_as_longlong(read_f64(vb, ty, r));
+        else if (ty == GZ_TYPE_UINT64) v = ((const int64_t*)vb)[r];
+        else v = read_i64(vb, ty, r);
+      }
+      rec[2 + k] = v;
+    }
+    rec[0] = valid ? key : 0;
+    rec[1] = (int64_t)((uint64_t)(td.row_base + lr) | (nulls << 48));
+  }
+}
+
+// Workgroups walk fixed chunks of the record array; inside a chunk, each
+// partition segment is aggregated in an LDS table and merged into the HBM table
+// of the segment's instance.
+__global__ __launch_bounds__(256) void agg_part_update_kernel(AggParams p, const int64_t* recs, const int64_t* pstart,
+                                                              const int* pinst, int B, int64_t chunk) {
+  extern __shared__ int64_t s_tab[];
+  const int64_t LS = p.lcap + 2;
+  const uint64_t lmask = (uint64_t)p.lcap - 1;
+  const int W = 2 + p.nu;
+  const int64_t total = pstart[B];
+  for (int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < total; c0 += (int64_t)gridDim.x * chunk) {
+    const int64_t c1 = min(total, c0 + chunk);
+    // partition holding record c0
+    int lo = 0, hi = B - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (pstart[mid] <= c0) lo = mid; else hi = mid - 1;
+    }
+    for (int b = lo; b < B; b++) {
+      const int64_t a = max(c0, pstart[b]), e = min(c1, pstart[b + 1]);
+      if (a >= c1) break;
+      if (a >= e) continue;
+      const AggInst I = p.inst[pinst[b]];
+      __syncthreads();
+      lds_reset(p, s_tab, LS);
+      __syncthreads();
+      for (int64_t q = a + threadIdx.x; q < e; q += blockDim.x) {
+        const int64_t* rec = recs + q * W;
+        const uint64_t w1 = (uint64_t)rec[1];
+        const int64_t grow = (int64_t)(w1 & ((1ull << 48) - 1));
+        const uint32_t nulls = (uint32_t)(w1 >> 48);
+        const bool valid = !((nulls >> kRecKeyNull) & 1);
+        const int64_t key = rec[0];
+        int64_t ls = -1;
+        if (!valid) ls = p.lcap;
+        else if (key == kEmpty) ls = p.lcap + 1;
+        else {
+          uint64_t s = fmix64((uint64_t)key) & lmask;
+          for (int i = 0; i < 32; i++) {  // short probe: a full LDS table spills to HBM
+            int64_t k = s_tab[s];
+            if (k == key) { ls = (int64_t)s; break; }
+            if (k == kEmpty) {
+              unsigned long long prev = atomicCAS((unsigned long long*)&s_tab[s], (unsigned long long)kEmpty,
+                                                  (unsigned long long)key);
+              if ((int64_t)prev == kEmpty || (int64_t)prev == key) { ls = (int64_t)s; break; }
+            }
+            s = (s + 1) & lmask;
+          }
+        }
+        const RecRow row{p, rec + 2, nulls};
+        if (ls >= 0) {
+          apply_row(p, s_tab, LS, ls, row, grow);
+        } else {
+          int64_t g = table_slot(I, valid, key);
+          if (g >= 0) apply_row(p, I.base, (int64_t)I.cap + 2, g, row, grow);
+        }
+      }
+      __syncthreads();
+      lds_merge(p, I, s_tab, LS);
+    }
+  }
 }
 
 // HyperLogLog registers per instance (2^11 registers), persistent static chunks
@@ -514,22 +727,67 @@ __global__ void agg_rehash_kernel(AggInst from, AggInst to, int nl) {
 }
 
 // Occupied slots of all instances -> (instance << 44 | first_row, slot) pairs.
-__global__ void agg_collect_kernel(const AggInst* inst, int ninst, unsigned long long* keys,
-                                   unsigned long long* vals, unsigned long long* cursor,
-                                   unsigned long long* counts) {
-  int i = blockIdx.y;
-  if (i >= ninst) return;
+// Occupied slots of every instance, compacted without atomics: blocks own
+// chunks of kCollectChunk slots of one instance (grid.y = instance); a first
+// pass counts each chunk, a scan over (instance, chunk) gives every chunk its
+// output position, a second pass writes (instance, first row) sort keys and slot
+// numbers in slot order.  A slot is occupied iff a row reached it.
+constexpr int kCollectChunk = 2048;  // 256 threads x 8 slots
+
+__global__ __launch_bounds__(256) void agg_occupancy_kernel(const AggInst* inst, int64_t X,
+                                                            unsigned long long* cc) {
+  __shared__ uint64_t s_scan[256 / kWave + 1];
+  const int i = blockIdx.y;
+  const int64_t x = blockIdx.x;
   const AggInst I = inst[i];
-  int64_t S = (int64_t)I.cap + 2;
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < S;
-       s += (int64_t)gridDim.x * blockDim.x) {
-    int64_t fr = I.base[S + s];
-    if (fr == INT64_MAX) continue;  // a slot is occupied iff a row reached it
-    unsigned long long pos = atomicAdd(cursor, 1ull);
-    keys[pos] = ((unsigned long long)i << 44) | (unsigned long long)fr;
-    vals[pos] = (unsigned long long)s;
-    atomicAdd(&counts[i], 1ull);
+  const int64_t S = (int64_t)I.cap + 2;
+  const int64_t s0 = x * kCollectChunk + threadIdx.x * 8;
+  uint64_t c = 0;
+  if (x * kCollectChunk < S) {
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (s0 + k < S && I.base[S + s0 + k] != INT64_MAX) c++;
   }
+  uint64_t total;
+  block_exclusive_scan<uint64_t>(c, s_scan, &total);
+  if (threadIdx.x == 0) cc[(int64_t)i * X + x] = total;
+}
+
+__global__ __launch_bounds__(256) void agg_collect_kernel(const AggInst* inst, int64_t X,
+                                                          const unsigned long long* ccpre,
+                                                          unsigned long long* keys, unsigned long long* vals) {
+  __shared__ uint64_t s_scan[256 / kWave + 1];
+  const int i = blockIdx.y;
+  const int64_t x = blockIdx.x;
+  const AggInst I = inst[i];
+  const int64_t S = (int64_t)I.cap + 2;
+  if (x * kCollectChunk >= S) return;
+  const int64_t s0 = x * kCollectChunk + threadIdx.x * 8;
+  int64_t fr[8];
+  uint64_t c = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    fr[k] = s0 + k < S ? I.base[S + s0 + k] : INT64_MAX;
+    c += fr[k] != INT64_MAX;
+  }
+  uint64_t pos = ccpre[(int64_t)i * X + x] + block_exclusive_scan<uint64_t>(c, s_scan, nullptr);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (fr[k] != INT64_MAX) {
+      keys[pos] = ((unsigned long long)i << 44) | (unsigned long long)fr[k];
+      vals[pos] = (unsigned long long)(s0 + k);
+      pos++;
+    }
+  }
+}
+
+// Groups per instance from the chunk scan.
+__global__ void agg_inst_counts_kernel(const unsigned long long* cc, const unsigned long long* ccpre, int ninst,
+                                       int64_t X, unsigned long long* counts) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst) return;
+  const int64_t last = (int64_t)i * X + X - 1;
+  counts[i] = ccpre[last] + cc[last] - ccpre[(int64_t)i * X];
 }
 
 struct ResultSpec {
@@ -674,7 +932,7 @@ struct gz_agg {
   // finish
   bool finished = false;
   std::vector<int64_t> ngroups, group_off;
-  DBuf skeys, svals, tkeys, tvals, cursor, counts, sort_tmp;
+  DBuf skeys, svals, tkeys, tvals, cursor, ccpre, counts, sort_tmp;
   std::vector<DBuf> res_values, res_valid;
   int64_t total_groups = 0;
   bool inst_dirty = true;
@@ -685,12 +943,21 @@ struct gz_agg {
     unsigned grid_hbm = 0, blocks_lds = 0;
     int lds = 0;
     int64_t total_slots = 0;
-    DBuf d_hbm, d_lds, d_slot_prefix;
+    DBuf d_hbm, d_lds, d_slot_prefix, d_bufs;
     std::vector<int64_t> rows;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } L;
   int lds_threshold = 256;  // groups up to which an instance gets the LDS first level
+  int part_groups = 512;    // groups per partition on the partitioned placement (0 = off)
+  // partitioned placement buffers of the plan
+  DBuf d_part, pcount, pstart, pcursor, pinst, recs;
+  int B = 0;                // partitions over all partitioned instances
+  int64_t nrec = 0;
+  AggParams p_part;
+  unsigned grid_part = 0, blocks_part_upd = 0;
+  int64_t part_chunk = 0;
+  int lds_part = 0;
 };
 
 static int add_lane(AggPlan& P, int op, int col, int col2) {
@@ -824,6 +1091,20 @@ static AggParams base_params(gz_agg* h) {
   for (int l = 0; l < p.nl; l++) p.lanes[l] = h->plan.lanes[l];
   for (int c = 0; c < h->plan.ncols; c++) p.col_type[c] = h->plan.types[c];
   p.lcap = 1024;
+  // used input columns of a partition record
+  for (int c = 0; c < kAggMaxCols; c++) p.uslot[c] = -1;
+  p.nu = 0;
+  for (int l = 0; l < p.nl; l++) {
+    for (int c : {p.lanes[l].col, p.lanes[l].col2}) {
+      if (c < 0 || p.uslot[c] >= 0) continue;
+      if (p.nu == kMaxUsed) {
+        p.nu = -1;  // too many for a record: no partitioned placement
+        return p;
+      }
+      p.uslot[c] = p.nu;
+      p.ucol[p.nu++] = c;
+    }
+  }
   return p;
 }
 
@@ -839,7 +1120,7 @@ static int inst_alloc(gz_agg* h, int i, uint64_t cap) {
   GZ_TRY(ctx_alloc(ctx, sizeof(int64_t) * (cap + 2) * (2 + nl), &nb));
   DBuf nc;
   GZ_TRY(ctx_alloc(ctx, 16, &nc));
-  AggInst ni;
+  AggInst ni{};
   ni.base = (int64_t*)nb.p;
   ni.cap = cap;
   ni.counter = (unsigned int*)nc.p;
@@ -913,7 +1194,8 @@ static int agg_set_make(gz_context* ctx, int32_t ncols, const int32_t* types, in
 
 // Tile table + buffer table of a call.
 static int agg_stage(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of,
-                     std::vector<AggTile>& tiles, std::vector<int64_t>& rows_per_inst, int tile_rows) {
+                     std::vector<AggTile>& tiles, std::vector<int64_t>& rows_per_inst, int tile_rows,
+                     DBuf* bufs_dst = nullptr) {
   const int nc = h->plan.ncols;
   std::vector<const void*> bufs((size_t)nb * nc * 3, nullptr);
   rows_per_inst.assign(h->ninst, 0);
@@ -953,14 +1235,26 @@ static int agg_stage(gz_agg* h, int32_t nb, const gz_batch* batches, const int32
     base[ii] += bt.num_rows;
     rows_per_inst[ii] += bt.num_rows;
   }
-  GZ_TRY(upload(h->ctx, &h->d_bufs, bufs.data(), sizeof(void*) * std::max<size_t>(bufs.size(), 1)));
+  GZ_TRY(upload(h->ctx, bufs_dst ? bufs_dst : &h->d_bufs, bufs.data(), sizeof(void*) * std::max<size_t>(bufs.size(), 1)));
   return GZ_OK;
 }
+
+static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of);
+static int agg_run(gz_agg* h);
 
 static int agg_process(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of) {
   gz_context* ctx = h->ctx;
   if (h->finished) return fail(GZ_INVALID, "aggregate already finished");
   GZ_HIP(hipSetDevice(ctx->device));
+  // the first call of a fresh aggregate goes through the planned path (all
+  // placements, the partitioned one included); later calls add to its tables
+  bool fresh = !h->L.ready;
+  for (int i = 0; i < h->ninst && fresh; i++)
+    if (h->tables[i].p) fresh = false;
+  if (fresh && nb > 0) {
+    GZ_TRY(agg_prepare(h, nb, batches, inst_of));
+    return agg_run(h);
+  }
   const int tile_rows = h->opts.tile_rows > 0 ? h->opts.tile_rows : 4096;
   std::vector<AggTile> tiles;
   std::vector<int64_t> rows;
@@ -1125,24 +1419,35 @@ static int agg_finish(gz_agg* h) {
   GZ_TRY(ctx_ensure(ctx, &h->svals, sizeof(uint64_t) * maxg));
   GZ_TRY(ctx_ensure(ctx, &h->tkeys, sizeof(uint64_t) * maxg));
   GZ_TRY(ctx_ensure(ctx, &h->tvals, sizeof(uint64_t) * maxg));
-  GZ_TRY(ctx_ensure(ctx, &h->cursor, sizeof(uint64_t)));
+  // compaction of the occupied slots: chunk counts, scan, ordered writes
+  const int64_t X = (int64_t)((maxcap + kCollectChunk - 1) / kCollectChunk);
+  const int64_t NC = X * h->ninst;
+  GZ_TRY(ctx_ensure(ctx, &h->cursor, sizeof(uint64_t) * NC));  // chunk counts
+  GZ_TRY(ctx_ensure(ctx, &h->ccpre, sizeof(uint64_t) * NC));   // their exclusive scan
   GZ_TRY(ctx_ensure(ctx, &h->counts, sizeof(uint64_t) * h->ninst));
-  hipLaunchKernelGGL(fill_u32_kernel_agg, dim3(1), dim3(256), 0, ctx->stream, (uint32_t*)h->cursor.p, (int64_t)2);
-  int64_t nc32 = 2 * (int64_t)h->ninst;
-  hipLaunchKernelGGL(fill_u32_kernel_agg, dim3((unsigned)std::min<int64_t>((nc32 + 255) / 256, 4096)), dim3(256), 0,
-                     ctx->stream, (uint32_t*)h->counts.p, nc32);
   {
     KernelTimer kt(ctx, "agg_collect");
-    unsigned gx = (unsigned)std::min<uint64_t>((maxcap + 255) / 256, 4096);
-    hipLaunchKernelGGL(agg_collect_kernel, dim3(gx, (unsigned)h->ninst), dim3(256), 0, ctx->stream,
-                       (const AggInst*)h->d_inst.p, h->ninst, (unsigned long long*)h->tkeys.p,
-                       (unsigned long long*)h->tvals.p, (unsigned long long*)h->cursor.p,
+    hipLaunchKernelGGL(agg_occupancy_kernel, dim3((unsigned)X, (unsigned)h->ninst), dim3(256), 0, ctx->stream,
+                       (const AggInst*)h->d_inst.p, X, (unsigned long long*)h->cursor.p);
+    size_t tmp = 0;
+    GZ_HIP(rocprim::exclusive_scan(nullptr, tmp, (const unsigned long long*)h->cursor.p,
+                                   (unsigned long long*)h->ccpre.p, 0ull, (size_t)NC,
+                                   rocprim::plus<unsigned long long>(), ctx->stream));
+    GZ_TRY(ctx_ensure(ctx, &h->sort_tmp, tmp));
+    GZ_HIP(rocprim::exclusive_scan(h->sort_tmp.p, tmp, (const unsigned long long*)h->cursor.p,
+                                   (unsigned long long*)h->ccpre.p, 0ull, (size_t)NC,
+                                   rocprim::plus<unsigned long long>(), ctx->stream));
+    hipLaunchKernelGGL(agg_collect_kernel, dim3((unsigned)X, (unsigned)h->ninst), dim3(256), 0, ctx->stream,
+                       (const AggInst*)h->d_inst.p, X, (const unsigned long long*)h->ccpre.p,
+                       (unsigned long long*)h->tkeys.p, (unsigned long long*)h->tvals.p);
+    hipLaunchKernelGGL(agg_inst_counts_kernel, dim3((unsigned)((h->ninst + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const unsigned long long*)h->cursor.p, (const unsigned long long*)h->ccpre.p, h->ninst, X,
                        (unsigned long long*)h->counts.p);
   }
-  std::vector<unsigned long long> cnt(h->ninst + 1);
+  std::vector<unsigned long long> cnt(h->ninst + 1, 0);
   GZ_HIP(hipMemcpyAsync(cnt.data(), h->counts.p, sizeof(uint64_t) * h->ninst, hipMemcpyDeviceToHost, ctx->stream));
-  GZ_HIP(hipMemcpyAsync(&cnt[h->ninst], h->cursor.p, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
   GZ_HIP(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < h->ninst; i++) cnt[h->ninst] += cnt[i];
   const int64_t n = (int64_t)cnt[h->ninst];
   h->ngroups.assign(h->ninst, 0);
   h->group_off.assign(h->ninst + 1, 0);
@@ -1218,9 +1523,9 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   std::vector<int64_t> rows;
   // a prepared run aggregates its batches from scratch: instance rows start at 0
   std::fill(h->rows_seen.begin(), h->rows_seen.end(), 0);
-  GZ_TRY(agg_stage(h, nb, batches, inst_of, tiles, rows, tile_rows));
+  GZ_TRY(agg_stage(h, nb, batches, inst_of, tiles, rows, tile_rows, &h->L.d_bufs));
   AggParams p = base_params(h);
-  p.bufs = (const void* const*)h->d_bufs.p;
+  p.bufs = (const void* const*)h->L.d_bufs.p;  // the plan's own batch table
   p.bstride = h->plan.ncols * 3;
   const double lf = h->opts.load_factor;
   // cardinality per instance: the hint, else HyperLogLog over its rows
@@ -1264,10 +1569,31 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
       GZ_TRY(inst_alloc(h, i, cap));
     }
     h->g_bound[i] = 2 + (int64_t)g;
+    const bool part_ok = p.nu >= 0 && h->plan.key_col >= 0 && h->part_groups > 0 &&
+                         g <= (double)kMaxFanout * h->part_groups;
     if (h->opts.placement == 0)
-      h->placement[i] = (g <= (double)h->lds_threshold || h->plan.key_col < 0) ? 1 : 2;
+      h->placement[i] = (g <= (double)h->lds_threshold || h->plan.key_col < 0) ? 1 : part_ok ? 3 : 2;
+    else if (h->opts.placement == 3 && !part_ok)
+      h->placement[i] = 2;
+    h->inst[i].fan_log2 = 0;
+    h->inst[i].bucket_base = 0;
+    if (h->placement[i] == 3) {
+      int lg = 1;
+      while (lg < 8 && (double)(1 << lg) * h->part_groups < g) lg++;
+      h->inst[i].fan_log2 = lg;
+    }
     prefix[i + 1] = prefix[i] + (int64_t)h->inst[i].cap + 2;
   }
+  // partition numbering over the partitioned instances
+  int B = 0;
+  std::vector<int> pinst;
+  for (int i = 0; i < h->ninst; i++) {
+    if (h->placement[i] != 3) continue;
+    h->inst[i].bucket_base = B;
+    for (int b = 0; b < (1 << h->inst[i].fan_log2); b++) pinst.push_back(i);
+    B += 1 << h->inst[i].fan_log2;
+  }
+  h->B = B;
   GZ_TRY(upload(ctx, &h->d_inst, h->inst.data(), sizeof(AggInst) * h->inst.size()));
   h->inst_dirty = false;
   GZ_TRY(upload(ctx, &h->L.d_slot_prefix, prefix.data(), sizeof(int64_t) * prefix.size()));
@@ -1277,8 +1603,35 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   GZ_TRY(upload(ctx, &h->d_lane_ops, ops.data(), sizeof(int) * ops.size()));
   p.inst = (const AggInst*)h->d_inst.p;
   // launch plan
-  std::vector<AggTile> lds_tiles, hbm_tiles;
-  for (auto& t : tiles) (h->placement[t.inst] == 1 ? lds_tiles : hbm_tiles).push_back(t);
+  std::vector<AggTile> lds_tiles, hbm_tiles, part_tiles;
+  for (auto& t : tiles)
+    (h->placement[t.inst] == 1 ? lds_tiles : h->placement[t.inst] == 3 ? part_tiles : hbm_tiles).push_back(t);
+  h->grid_part = 0;
+  h->nrec = 0;
+  if (!part_tiles.empty()) {
+    for (auto& t : part_tiles) h->nrec += t.nrows;
+    GZ_TRY(upload(ctx, &h->d_part, part_tiles.data(), sizeof(AggTile) * part_tiles.size()));
+    GZ_TRY(upload(ctx, &h->pinst, pinst.data(), sizeof(int) * pinst.size()));
+    GZ_TRY(ctx_ensure(ctx, &h->pcount, sizeof(unsigned) * B));
+    GZ_TRY(ctx_ensure(ctx, &h->pstart, sizeof(int64_t) * (B + 1)));
+    GZ_TRY(ctx_ensure(ctx, &h->pcursor, sizeof(uint64_t) * B));
+    GZ_TRY(ctx_ensure(ctx, &h->recs, sizeof(int64_t) * (2 + p.nu) * h->nrec));
+    AggParams& q = h->p_part;
+    q = p;
+    q.tiles = (const AggTile*)h->d_part.p;
+    q.T = (int64_t)part_tiles.size();
+    h->grid_part = (unsigned)((q.T + 7) / 8 * 8);
+    q.lcap = 1024;
+    int lds = lds_bytes(h, q.lcap);
+    while (lds > 96 * 1024 && q.lcap > 64) {
+      q.lcap >>= 1;
+      lds = lds_bytes(h, q.lcap);
+    }
+    h->lds_part = lds;
+    h->blocks_part_upd = (unsigned)std::max<int64_t>(1, std::min<int64_t>((h->nrec + 4095) / 4096,
+                                                                          (int64_t)ctx->num_cus * 2));
+    h->part_chunk = (h->nrec + h->blocks_part_upd - 1) / h->blocks_part_upd;
+  }
   h->L.p_hbm = p;
   h->L.p_lds = p;
   h->L.grid_hbm = 0;
@@ -1323,6 +1676,19 @@ static int agg_run_kernels(gz_agg* h) {
     hipLaunchKernelGGL(agg_update_kernel, dim3(h->L.grid_hbm), dim3(256), 0, st, h->L.p_hbm);
   if (h->L.blocks_lds)
     hipLaunchKernelGGL(agg_update_lds_kernel, dim3(h->L.blocks_lds), dim3(256), h->L.lds, st, h->L.p_lds);
+  if (h->grid_part) {
+    hipLaunchKernelGGL(fill_u32_kernel_agg, dim3((unsigned)std::min<int64_t>((h->B + 255) / 256, 4096)), dim3(256), 0,
+                       st, (uint32_t*)h->pcount.p, (int64_t)h->B);
+    hipLaunchKernelGGL(agg_part_hist_kernel, dim3(h->grid_part), dim3(256), 0, st, h->p_part,
+                       (unsigned int*)h->pcount.p);
+    hipLaunchKernelGGL(agg_part_scan_kernel, dim3(1), dim3(256), 0, st, (const unsigned int*)h->pcount.p, h->B,
+                       (int64_t*)h->pstart.p, (unsigned long long*)h->pcursor.p);
+    hipLaunchKernelGGL(agg_part_scatter_kernel, dim3(h->grid_part), dim3(256), 0, st, h->p_part,
+                       (unsigned long long*)h->pcursor.p, (int64_t*)h->recs.p);
+    hipLaunchKernelGGL(agg_part_update_kernel, dim3(h->blocks_part_upd), dim3(256), h->lds_part, st, h->p_part,
+                       (const int64_t*)h->recs.p, (const int64_t*)h->pstart.p, (const int*)h->pinst.p, h->B,
+                       h->part_chunk);
+  }
   GZ_HIP(hipGetLastError());
   return GZ_OK;
 }
@@ -1382,16 +1748,16 @@ static int agg_launch(gz_agg* h) {
 // fastest stays prepared.
 static int agg_autotune(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of, int32_t* chosen) {
   struct Cand {
-    int tile_rows, threshold;
+    int tile_rows, threshold, part;
     double lf;
   };
   std::vector<Cand> cands;
-  for (int tr : {2048, 4096, 8192})
+  for (int tr : {2048, 8192})
     for (int th : {0, 256, 1024})
-      for (double lf : {0.5, 0.7}) cands.push_back({tr, th, lf});
+      for (int part : {0, 512}) cands.push_back({tr, th, part, 0.5});
   gz_context* ctx = h->ctx;
   const gz_agg_options saved = h->opts;
-  const int saved_th = h->lds_threshold;
+  const int saved_th = h->lds_threshold, saved_part = h->part_groups;
   hipEvent_t e0, e1;
   GZ_HIP(hipEventCreate(&e0));
   GZ_HIP(hipEventCreate(&e1));
@@ -1401,8 +1767,12 @@ static int agg_autotune(gz_agg* h, int32_t nb, const gz_batch* batches, const in
   for (size_t c = 0; c < cands.size() && rc == GZ_OK; c++) {
     h->opts.tile_rows = cands[c].tile_rows;
     h->opts.load_factor = cands[c].lf;
-    if (saved.placement == 0) h->lds_threshold = cands[c].threshold;
-    else if (c > 0 && cands[c].threshold != cands[0].threshold) continue;  // placement fixed by the caller
+    if (saved.placement == 0) {
+      h->lds_threshold = cands[c].threshold;
+      h->part_groups = cands[c].part;
+    } else if (c > 0 && (cands[c].threshold != cands[0].threshold || cands[c].part != cands[0].part)) {
+      continue;  // placement fixed by the caller
+    }
     if ((rc = agg_prepare(h, nb, batches, inst_of))) break;
     if ((rc = agg_run_kernels(h))) break;  // warm
     hipEventRecord(e0, ctx->stream);
@@ -1428,12 +1798,16 @@ static int agg_autotune(gz_agg* h, int32_t nb, const gz_batch* batches, const in
   if (rc) {
     h->opts = saved;
     h->lds_threshold = saved_th;
+    h->part_groups = saved_part;
     return rc;
   }
   if (best < 0) return fail(GZ_CAPACITY_ERROR, "no candidate plan fits");
   h->opts.tile_rows = cands[best].tile_rows;
   h->opts.load_factor = cands[best].lf;
-  if (saved.placement == 0) h->lds_threshold = cands[best].threshold;
+  if (saved.placement == 0) {
+    h->lds_threshold = cands[best].threshold;
+    h->part_groups = cands[best].part;
+  }
   GZ_TRY(agg_prepare(h, nb, batches, inst_of));
   if (chosen) *chosen = best;
   return GZ_OK;
@@ -1518,8 +1892,9 @@ int gz_agg_release(gz_agg* h) {
   for (auto& b : h->res_values) ctx_free(ctx, &b);
   for (auto& b : h->res_valid) ctx_free(ctx, &b);
   for (DBuf* b : {&h->d_inst, &h->d_lane_ops, &h->d_tiles, &h->d_bufs, &h->d_regs, &h->d_est, &h->skeys,
-                  &h->svals, &h->tkeys, &h->tvals, &h->cursor, &h->counts, &h->sort_tmp, &h->L.d_hbm,
-                  &h->L.d_lds, &h->L.d_slot_prefix})
+                  &h->svals, &h->tkeys, &h->tvals, &h->cursor, &h->ccpre, &h->counts, &h->sort_tmp, &h->L.d_hbm,
+                  &h->L.d_lds, &h->L.d_slot_prefix, &h->L.d_bufs, &h->d_part, &h->pcount, &h->pstart, &h->pcursor, &h->pinst,
+                  &h->recs})
     ctx_free(ctx, b);
   if (h->L.exec) hipGraphExecDestroy(h->L.exec);
   if (h->L.graph) hipGraphDestroy(h->L.graph);

@@ -140,4 +140,9 @@ void ctx_resolve_timings(gz_context* ctx, bool wait);
 // buffer: it may be reused as soon as this returns).
 int upload(gz_context* ctx, DBuf* dst, const void* src, size_t bytes);
 
+// Probe on another context than the relation's (resident relations).
+void probe_set_context(gz_probe* h, gz_context* ctx);
+// Wait for the relation's build (its context's stream).
+int relation_synchronize(gz_relation* r);
+
 }  // namespace gz

@@ -764,6 +764,7 @@ int gz_relation_release(gz_relation* r) {
 
 struct gz_probe {
   gz_relation* rel = nullptr;
+  gz_context* ctx = nullptr;  // stream and memory of the probe (the relation's by default)
   int nprobe = 0;
   std::vector<int> ptypes;
   int key_col = 0, join_type = 0, null_aware = 0;
@@ -781,7 +782,7 @@ struct gz_probe {
 static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* out_rows,
                      int64_t* out_batch_rows) {
   gz_relation* r = h->rel;
-  gz_context* ctx = r->ctx;
+  gz_context* ctx = h->ctx;
   hipStream_t st = ctx->stream;
   GZ_HIP(hipSetDevice(ctx->device));
   const int TILE = h->tile_rows;
@@ -996,6 +997,7 @@ int gz_probe_make(gz_relation* r, int32_t num_probe_cols, const int32_t* probe_t
   if (pk != bk) return fail(GZ_TYPE_ERROR, "probe and build key types differ");
   auto* h = new gz_probe();
   h->rel = r;
+  h->ctx = r->ctx;
   h->nprobe = num_probe_cols;
   h->ptypes.assign(probe_types, probe_types + num_probe_cols);
   h->key_col = probe_key_col;
@@ -1050,7 +1052,7 @@ int gz_probe_result_column(gz_probe* h, int32_t col, void** values, void** valid
 
 int gz_probe_release(gz_probe* h) {
   if (!h) return GZ_OK;
-  gz_context* ctx = h->rel->ctx;
+  gz_context* ctx = h->ctx;
   for (auto& o : h->outs)
     for (DBuf* b : {&o.values, &o.valid, &o.offsets, &o.data, &o.lens}) ctx_free(ctx, b);
   for (DBuf* b : {&h->d_tiles, &h->d_bufs, &h->match, &h->tile_count, &h->tile_off, &h->scan_tmp,
@@ -1061,3 +1063,13 @@ int gz_probe_release(gz_probe* h) {
 }
 
 }  // extern "C"
+
+namespace gz {
+// Probe on another context (stream and memory) than the relation's: a resident
+// relation serves probes from many streams; its build must be complete.
+void probe_set_context(gz_probe* h, gz_context* ctx) { h->ctx = ctx; }
+int relation_synchronize(gz_relation* r) {
+  GZ_HIP(hipStreamSynchronize(r->ctx->stream));
+  return GZ_OK;
+}
+}  // namespace gz

@@ -49,7 +49,9 @@ struct TileDesc {
   int32_t batch;
   int32_t row_begin;
   int32_t nrows;
-  int32_t rr;  // round-robin partition of the tile's first row
+  int32_t rr;        // round-robin partition of the tile's first row
+  int32_t call_row;  // index of the tile's first row among the rows of the call
+  int32_t pad;
 };
 
 struct SplitParams {
@@ -90,6 +92,7 @@ struct SplitParams {
   uint8_t* out_data[kMaxCols];
   int* err;
   int cbuf;                  // LDS bytes for partition-ordered utf8 staging
+  int rowid_oc;              // output column of the row indices, -1 if none
 };
 
 __device__ __forceinline__ void load_tile_ptrs(const SplitParams& p, int batch, const void** s_bp) {
@@ -802,6 +805,16 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
       __syncthreads();
     }
   }
+
+  // ---- 4c: row indices (an output column without an input)
+  if (p.rowid_oc >= 0) {
+#pragma unroll
+    for (int j = 0; j < RPT; j++)
+      if (lrow(j) < n) ((uint32_t*)s_stage)[pos[j]] = (uint32_t)(td.call_row + lrow(j));
+    __syncthreads();
+    uint32_t* dst = (uint32_t*)p.out_values[p.rowid_oc];
+    for (int i = tid; i < n; i += BLOCK) gst(dst + drow(i), ((uint32_t*)s_stage)[i]);
+  }
 }
 
 // Host-side LDS bytes and block size of the scatter kernel.
@@ -874,6 +887,8 @@ void gz_split_options_defaults(gz_split_options* o) {
   o->offheap_per_task = 0;
   o->tile_rows = 0;
   o->hash_seed = 42;
+  o->append_row_ids = 0;
+  o->reserved = 0;
 }
 
 int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
@@ -961,6 +976,15 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
     b.bin[c] = type_is_binary(t) ? nbin++ : -1;
   }
   b.nbin = nbin;
+  b.rowid_oc = -1;
+  if (o.append_row_ids) {
+    if (oc >= kMaxCols) {
+      delete h;
+      return fail(GZ_NOT_IMPLEMENTED, "more than 32 output columns");
+    }
+    b.rowid_oc = oc++;
+    h->out_types.push_back(GZ_TYPE_INT32);
+  }
   // default tile: 2048 rows (two 1024-thread workgroups per CU, also with the
   // utf8 staging buffer)
   if (h->tile_rows == 0) h->tile_rows = 2048;
@@ -1038,6 +1062,8 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
       td.row_begin = (int32_t)r0;
       td.nrows = (int32_t)std::min<int64_t>(TILE, bt.num_rows - r0);
       td.rr = (int32_t)((rr + r0) % P);
+      td.call_row = (int32_t)(N + r0);
+      td.pad = 0;
       tiles.push_back(td);
       tile_batch.push_back(bi);
     }
@@ -1108,6 +1134,10 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     prm.out_valid[oc] = (uint32_t*)seg->validity[oc].p;
     prm.out_off[oc] = (int32_t*)seg->offsets[oc].p;
     prm.out_data[oc] = (uint8_t*)seg->data[oc].p;
+  }
+  if (prm.rowid_oc >= 0) {
+    if ((rc = ctx_alloc(ctx, (size_t)std::max<int64_t>(N, 1) * 4, &seg->values[prm.rowid_oc]))) return bail(rc);
+    prm.out_values[prm.rowid_oc] = seg->values[prm.rowid_oc].p;
   }
   // output columns of an input column are indexed by the output index
   if ((rc = upload(ctx, &h->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size()))) return bail(rc);

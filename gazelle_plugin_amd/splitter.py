@@ -28,6 +28,7 @@ class SplitOptions:
     offheap_per_task: int = 0
     tile_rows: int = 0
     hash_seed: int = 42
+    append_row_ids: bool = False  # last output column: int32 row index within the split call
 
     def to_c(self) -> L.GzSplitOptions:
         o = L.GzSplitOptions()
@@ -36,6 +37,7 @@ class SplitOptions:
         o.offheap_per_task = self.offheap_per_task
         o.tile_rows = self.tile_rows
         o.hash_seed = self.hash_seed
+        o.append_row_ids = int(self.append_row_ids)
         return o
 
 
@@ -57,7 +59,10 @@ class Splitter:
                                       num_partitions, L.i32_array(key_cols), len(key_cols),
                                       ctypes.byref(opts), ctypes.byref(h)))
         P = 1 if short_name == "single" else num_partitions
-        return cls(ctx, h, types, P, short_name)
+        sp = cls(ctx, h, types, P, short_name)
+        if options is not None and options.append_row_ids:
+            sp.output_types.append(T.INT32)
+        return sp
 
     # -- splitting
     def Split(self, batch: DeviceBatch):
@@ -99,7 +104,8 @@ class Splitter:
     def segment(self, seg: int) -> Dict:
         """Host copy of one segment's partitioned layout (drains the context's
         stream first: the copies run on the null stream)."""
-        self.ctx.synchronize()
+        if self.ctx is not None:
+            self.ctx.synchronize()
         P = self.num_partitions
         rs = np.zeros(P + 1, np.int64)
         L.check(L.lib().gz_split_segment_rows(self._h, seg, rs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))

@@ -109,6 +109,9 @@ typedef struct {
   int64_t offheap_per_task;   /* SplitOptions.offheap_per_task (type.h:40) */
   int32_t tile_rows;          /* rows per workgroup tile: 0 = autotuned */
   int32_t hash_seed;          /* 42, the seed Spark uses (splitter.cc:1405) */
+  int32_t append_row_ids;     /* 1 = add a last output column, int32, holding each
+                                 row's index among the rows of its split call */
+  int32_t reserved;
 } gz_split_options;
 
 void gz_split_options_defaults(gz_split_options* opts);
@@ -313,6 +316,83 @@ int gz_probe_process_batches(gz_probe* p, int32_t num_batches, const gz_batch* b
 int gz_probe_result_column(gz_probe* p, int32_t col, void** values, void** valid_bytes,
                            void** offsets, void** data);
 int gz_probe_release(gz_probe* p);
+
+/* ---- serving -----------------------------------------------------------------
+ * A fragment server: query fragments (an aggregate, a join followed by an
+ * aggregate, a shuffle split) arrive on a timeline and are admitted, first come
+ * first served, when one of the server's slots is idle and the fragment's HBM
+ * estimate fits what the admitted fragments leave of the budget.  Each slot has
+ * its own context (HIP stream and memory pool), so admitted fragments overlap
+ * on the device; registered hash relations stay resident and serve the probes
+ * of every join fragment.  No counterpart in the reference, where one Spark
+ * task drives one kernel instance; the operators are the ones above.
+ */
+typedef struct gz_server gz_server;
+
+typedef struct {
+  int32_t num_cols;
+  const int32_t* col_types;
+  int32_t num_actions;
+  const gz_agg_action* actions;
+} gz_agg_plan;
+
+typedef struct {
+  int32_t num_cols;
+  const int32_t* col_types;
+  int32_t num_partitions;
+  const int32_t* key_cols;
+  int32_t num_keys;
+} gz_split_plan;
+
+typedef struct {
+  int32_t num_probe_cols;
+  const int32_t* probe_types;
+  int32_t probe_key_col;
+  int32_t join_type;
+  int32_t num_result_cols;
+  const int32_t* result_sides;
+  const int32_t* result_indices;
+} gz_probe_plan;
+
+#define GZ_FRAG_AGGREGATE 0
+#define GZ_FRAG_JOIN_AGGREGATE 1
+#define GZ_FRAG_SPLIT 2
+
+typedef struct {
+  int32_t kind;             /* GZ_FRAG_* */
+  int32_t relation;         /* join fragments: id from gz_server_add_relation */
+  int32_t num_batches;
+  const gz_batch* batches;  /* device buffers (the probe side for joins) */
+  double arrival_s;         /* arrival, seconds after the run starts */
+} gz_fragment;
+
+typedef struct {
+  double arrival_s, admitted_s, done_s;  /* seconds after the run starts */
+  int64_t rows;                          /* input rows */
+  int64_t out_rows;                      /* groups, or rows split */
+  int32_t status;                        /* GZ_OK or the failure code */
+  int32_t slot;
+} gz_fragment_stats;
+
+/* Plans may be null for fragment kinds the server will not see.  The join
+ * aggregate reads the join result columns, in result order (fixed-width). */
+int gz_server_create(int32_t device, int64_t hbm_budget, int32_t num_slots, const gz_agg_plan* agg,
+                     const gz_split_plan* split, const gz_probe_plan* probe, const gz_agg_plan* join_agg,
+                     gz_server** out);
+/* A finished relation, resident from now on (the caller keeps ownership and
+ * releases it after the server). */
+int gz_server_add_relation(gz_server* s, gz_relation* r, int32_t* id);
+/* Runs fragments sorted by arrival; returns when all are done, with per-fragment
+ * times in stats.  keep_results != 0 keeps each output for the accessors below
+ * until the next run or gz_server_release_results. */
+int gz_server_run(gz_server* s, int32_t num_fragments, const gz_fragment* frags, int32_t keep_results,
+                  gz_fragment_stats* stats);
+/* Finished aggregate of an aggregate or join fragment / stopped splitter of a
+ * split fragment; owned by the server. */
+int gz_server_fragment_agg(gz_server* s, int32_t i, gz_agg** out);
+int gz_server_fragment_split(gz_server* s, int32_t i, gz_splitter** out);
+int gz_server_release_results(gz_server* s);
+int gz_server_destroy(gz_server* s);
 
 #ifdef __cplusplus
 }

@@ -37,7 +37,7 @@ def expected(batches, types, actions):
 
 
 @pytest.mark.parametrize("batched", [True, False])
-@pytest.mark.parametrize("placement", [0, 1, 2])
+@pytest.mark.parametrize("placement", [0, 1, 2, 3])
 def test_config0_query_small(ctx, batched, placement):
     batches = datagen.agg_host(50000, num_keys=10000)
     got = run_gpu(ctx, batches, datagen.AGG_TYPES, CONFIG0_ACTIONS, batched, placement=placement)
@@ -76,7 +76,7 @@ def test_nulls_key_types_and_all_actions(ctx):
                ("action_max", [3]), ("action_sum_count", [4]), ("action_min", [5]), ("action_max", [2]),
                ("action_avgByCount", [1, 2])]
     exp = expected(batches, types, actions)
-    for placement in (1, 2):
+    for placement in (1, 2, 3):
         got = run_gpu(ctx, batches, types, actions, placement=placement)
         assert_results_equal(got, exp, RTOL)
 
@@ -188,7 +188,7 @@ def test_autotuned_plan_matches_oracle(ctx):
     k = HashAggregateKernel.Make(ctx, datagen.AGG_TYPES, CONFIG0_ACTIONS)
     it = HashAggregateResultIterator(k, num_instances=len(inst_batches))
     chosen = it.Autotune(dev, inst_of)
-    assert 0 <= chosen < 18
+    assert 0 <= chosen < 12
     it.Capture()
     it.Launch()
     for i, bl in enumerate(inst_batches):

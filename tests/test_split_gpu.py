@@ -135,3 +135,27 @@ def test_device_generated_full_path_properties(ctx):
     assert seg["columns"][2]["null_count"] == nulls
     sp.close()
     torch.cuda.synchronize()
+
+
+def test_row_ids_follow_rows(ctx):
+    """The optional row-index column lands next to its row: within every
+    partition the indices ascend (stable split) and index the input rows."""
+    from gazelle_plugin_amd.splitter import SplitOptions, Splitter
+
+    types = [T.INT64, T.DOUBLE]
+    batches = random_batches(types, [3000, 4096, 17, 5000], seed=17, null_frac=0.1)
+    sp = Splitter.Make(ctx, "hash", types, 37, [0], SplitOptions(append_row_ids=True))
+    sp.SplitBatches([DeviceBatch.from_host(b) for b in batches])
+    sp.Stop()
+    lay = sp.read_partitions()
+    exp = oracle.split(batches, types, 37, [0], "hash", 42)
+    keys_in = np.concatenate([b.columns[0].values for b in batches])
+    rid = lay["columns"][2]["values"].view(np.int32)
+    assert len(rid) == sum(b.num_rows for b in batches)
+    for p in range(37):
+        a, b = lay["row_start"][p], lay["row_start"][p + 1]
+        assert np.all(np.diff(rid[a:b]) > 0)
+        got_keys = lay["columns"][0]["values"].view(np.int64)[a:b]
+        assert np.array_equal(keys_in[rid[a:b]], got_keys)
+    assert np.array_equal(lay["row_start"], exp["row_start"])
+    sp.close()
