@@ -1,25 +1,40 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X shuffle hash split (and, with --workload, the other
-hot-path operators).
+"""Benchmark of the MI355X columnar hot path: shuffle hash split (the headline
+line), hash aggregate (plain and batched), hash join, and the fragment server.
 
-Default workload: shuffle hash split of 64M rows in 4096-row batches
-(int64 key U[0,2^40), int32, float64 N(0,1) with 2% nulls, utf8 of length
-U[8,32] over [a-z]) into 200 partitions by Spark Murmur3 (seed 42) pmod.  A
-step splits all 16384 batches in one batched launch sequence; inputs are
-resident in HBM before timing starts.  Prints one JSON line.
+Headline workload: shuffle hash split of 64M rows in 4096-row batches (int64 key
+U[0,2^40), int32, float64 N(0,1) with 2% nulls, utf8 of length U[8,32] over
+[a-z]) into 200 partitions by Spark Murmur3 (seed 42) pmod.  A step splits all
+16384 batches in one batched launch sequence; inputs are resident in HBM before
+timing starts.  The other workloads report in sub-objects of the same JSON line:
+  aggregate.plain_query  8M rows, 10k groups, GROUP BY key: sum/avg price,
+                         count(*), min/max/sum qty; batched vs per-batch kernels
+  aggregate.batched_set  512 independent GROUP BYs, rows log-uniform 1k-1M,
+                         groups log-uniform 10-1e7; one autotuned, graph-captured
+                         launch vs per-batch kernels
+  join                   4M-row build, 256M-row probe (Zipf 1.1 foreign keys, 5%
+                         absent): inner / semi / anti
+  serving                fragment stream (40% aggregate, 40% join+aggregate, 20%
+                         split; rows log-uniform 1e4-5e7) under a 200 GB budget,
+                         Poisson arrivals at 70% of measured capacity; p50/p99
+Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
+import traceback
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+AGG_ACTIONS = [("action_groupby", [0]), ("action_sum", [1]), ("action_avg", [1]), ("action_countLiteral_1", []),
+               ("action_min", [2]), ("action_max", [2]), ("action_sum", [2])]
 
 
 def parse():
@@ -29,11 +44,16 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=64 * 1024 * 1024)
     ap.add_argument("--plain-batches", type=int, default=2048,
-                    help="batches timed on the plain per-batch path")
+                    help="batches timed on the plain per-batch split path")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=0, help="split tile rows (0 = default)")
+    ap.add_argument("--sections", default="split,agg,join,serve",
+                    help="comma list of split, agg, join, serve (split always runs)")
+    ap.add_argument("--join-probe-rows", type=int, default=256 << 20)
+    ap.add_argument("--serve-fragments", type=int, default=1000)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run a device copy of this many MiB (PMC calibration)")
     return ap.parse_args()
@@ -50,70 +70,83 @@ def load_traffic(name):
         return None
 
 
-def cpu_baseline_split(target_s: float):
-    """The oracle's per-batch reference split loop on one host core."""
-    import numpy as np
+def timed(fn, stream, reps):
+    """Mean and min milliseconds of fn over reps runs, bracketed by events on
+    the library's stream after a synchronization."""
+    import torch
 
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return sum(ts) / len(ts), min(ts)
+
+
+# ---------------------------------------------------------------- CPU baseline
+def cpu_baseline_split(target_s: float):
+    """The oracle's per-batch split loop (splitter.cc DoSplit restated) on one
+    host core, over a bounded sample of the same schema."""
     import oracle
     from gazelle_plugin_amd import datagen
 
     rows = 256 * 4096
     batches = datagen.split_host(rows)
     cb = oracle._CBatches(batches)
-    sp = oracle.CpuSplitter(datagen.SPLIT_TYPES, 200, [0])
-    t0 = time.perf_counter()
-    for i in range(len(batches)):
-        sp.split(cb, i)
-    t1 = time.perf_counter()
-    sp.close()
-    per_row = (t1 - t0) / rows
-    # extend the sample to about target_s of work by repeating the pass
-    reps = max(1, int(target_s / max(1e-9, t1 - t0)))
-    total_rows = rows
-    total_t = t1 - t0
-    for _ in range(reps - 1):
+    total_rows, total_t, passes = 0, 0.0, 0
+    while total_t < target_s or passes == 0:
         sp = oracle.CpuSplitter(datagen.SPLIT_TYPES, 200, [0])
         t0 = time.perf_counter()
         for i in range(len(batches)):
             sp.split(cb, i)
         total_t += time.perf_counter() - t0
-        total_rows += rows
         sp.close()
+        total_rows += rows
+        passes += 1
     return {"value": total_rows / total_t, "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": f"{total_rows} rows ({reps} passes over 256 batches x 4096 rows) of the "
-                      f"same schema, oracle per-batch split loop (splitter.cc DoSplit restated), "
+            "sample": f"{total_rows} rows ({passes} passes over 256 batches x 4096 rows) of the split "
+                      f"schema, oracle per-batch split loop, {total_t:.1f} s on {os.cpu_count()} visible cores "
+                      f"(1 used)"}
+
+
+def cpu_baseline_agg(target_s: float):
+    import oracle
+    from gazelle_plugin_amd import datagen
+    from gazelle_plugin_amd.aggregate import parse_action
+
+    batches = datagen.agg_host(1 << 20, num_keys=10000)
+    acts = [parse_action(n, i) for n, i in AGG_ACTIONS]
+    total_rows, total_t = 0, 0.0
+    while total_t < target_s or total_rows == 0:
+        t0 = time.perf_counter()
+        oracle.aggregate(batches, datagen.AGG_TYPES, acts)
+        total_t += time.perf_counter() - t0
+        total_rows += 1 << 20
+    return {"value": total_rows / total_t, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"{total_rows} rows of the plain-query schema (10k groups), oracle hash aggregate, "
                       f"{total_t:.1f} s"}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
-                                   "described in DESIGN.md but not launched", "n_gpus": args.gpus}))
-        sys.exit(2)
+# ---------------------------------------------------------------- sections
+def bench_split(args, ctx, stream):
     import torch
 
     from gazelle_plugin_amd import datagen
-    from gazelle_plugin_amd.context import Context
-    from gazelle_plugin_amd.splitter import Splitter
+    from gazelle_plugin_amd.splitter import SplitOptions, Splitter, make_batch_array
 
-    torch.cuda.init()
-    stream = torch.cuda.current_stream()
-    ctx = Context(0, stream=stream.cuda_stream)
     tab = datagen.split_device(args.rows)
     torch.cuda.synchronize()
     N = tab.num_rows
-    in_bytes = tab.nbytes
     P = 200
-    out_bytes = in_bytes + 4 * P  # same columns out, offsets gain one entry per partition
+    in_bytes = tab.nbytes
+    out_bytes = in_bytes + 4 * P  # same columns out; offsets gain one entry per partition
     algo_bytes = in_bytes + out_bytes
-
-    from gazelle_plugin_amd.splitter import make_batch_array
-
     arr = make_batch_array(tab.batches)
-
-    from gazelle_plugin_amd.splitter import SplitOptions
-
     opts = SplitOptions(tile_rows=args.tile_rows)
 
     def step():
@@ -136,14 +169,10 @@ def main():
     ev1.record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    gpu_ms = ev0.elapsed_time(ev1)
-    ms_per_step = gpu_ms / args.steps
-    scat_ms, scat_n = ctx.kernel_time("split_scatter")
-    cnt_ms, cnt_n = ctx.kernel_time("split_count")
-    scan_ms, scan_n = ctx.kernel_time("split_scan")
+    ms_per_step = ev0.elapsed_time(ev1) / args.steps
+    kt = {k: ctx.kernel_time(k) for k in ("split_count", "split_scan", "split_scatter")}
     ctx.set_profiling(False)
-    scat_avg = scat_ms / max(1, scat_n)
-    rows_per_s = N / (ms_per_step / 1e3)
+    scat_ms = kt["split_scatter"][0] / max(1, kt["split_scatter"][1])
 
     # plain per-batch path on a prefix of the batches
     nb = min(args.plain_batches, len(tab.batches))
@@ -159,16 +188,248 @@ def main():
     plain_ms = e0.elapsed_time(e1)
     sp.close()
     plain_rows_s = nb * datagen.BATCH_ROWS / (plain_ms / 1e3)
+    rows_per_s = N / (ms_per_step / 1e3)
 
     traffic = load_traffic("split_scatter_traffic.json")
-    roof = {"bound": "hbm", "achieved": algo_bytes / (scat_avg / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s"}
+    roof = {"bound": "hbm", "achieved": algo_bytes / (scat_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = traffic.get("bytes_per_launch") if traffic else None
     roof["kernel"] = "split_scatter"
     roof["algorithmic_bytes"] = algo_bytes
-    roof["kernel_ms"] = scat_avg
+    roof["kernel_ms"] = scat_ms
+    res = {"rows": N, "batches": len(tab.batches), "partitions": P, "ms_per_step": ms_per_step,
+           "rows_per_s": rows_per_s, "wall_s": t1 - t0,
+           "step_GBps": algo_bytes / (ms_per_step / 1e3) / 1e9,
+           "kernels_ms_avg": {k: v[0] / max(1, v[1]) for k, v in kt.items()},
+           "plain": {"rows_per_s": plain_rows_s, "batches": nb, "ms": plain_ms},
+           "batched_over_plain": rows_per_s / plain_rows_s, "roofline": roof}
+    del tab, arr
+    return res
 
+
+def bench_agg(args, ctx, stream):
+    import torch
+
+    from gazelle_plugin_amd import datagen
+    from gazelle_plugin_amd.aggregate import HashAggregateKernel, HashAggregateResultIterator
+    from gazelle_plugin_amd.splitter import make_batch_array
+
+    out = {}
+    k = HashAggregateKernel.Make(ctx, datagen.AGG_TYPES, AGG_ACTIONS)
+    # plain query: 8M rows, 10k groups
+    tab = datagen.agg_device(8 << 20)
+    arr = make_batch_array(tab.batches)
+    torch.cuda.synchronize()
+
+    def batched():
+        it = k.MakeResultIterator()
+        it.ProcessBatches(tab.batches, arr)
+        it.finish()
+        it.close()
+
+    def plain():
+        it = k.MakeResultIterator()
+        for b in tab.batches:
+            it.ProcessAndCacheOne(b)
+        it.finish()
+        it.close()
+
+    batched()
+    bm, bmin = timed(batched, stream, 5)
+    pm, _ = timed(plain, stream, 1)
+    out["plain_query"] = {"rows": tab.num_rows, "groups": 10000, "batches": len(tab.batches),
+                          "batched_ms": bm, "batched_min_ms": bmin, "batched_rows_per_s": tab.num_rows / (bm / 1e3),
+                          "batched_GBps": tab.nbytes / (bm / 1e3) / 1e9,
+                          "plain_ms": pm, "plain_rows_per_s": tab.num_rows / (pm / 1e3),
+                          "batched_over_plain": pm / bm}
+    out["plain_query"]["frac_of_hbm_peak"] = out["plain_query"]["batched_GBps"] / HBM_PEAK_GBS
+    del tab, arr
+    # batched set: 512 independent GROUP BYs
+    tab, rows, groups = datagen.batched_agg_device(512)
+    arr = make_batch_array(tab.batches)
+    inst = list(range(len(tab.batches)))
+    torch.cuda.synchronize()
+    it = HashAggregateResultIterator(k, num_instances=len(tab.batches))
+    t0 = time.perf_counter()
+    chosen = it.Autotune(tab.batches, inst, arr)
+    tune_s = time.perf_counter() - t0
+    it.Capture()
+
+    def graph_step():
+        it.Launch()
+        it.finish()
+
+    graph_step()
+    gm, gmin = timed(graph_step, stream, 3)
+
+    def plain_set():
+        for b in tab.batches:
+            j = k.MakeResultIterator()
+            j.ProcessAndCacheOne(b)
+            j.finish()
+            j.close()
+
+    pm, _ = timed(plain_set, stream, 1)
+    out["batched_set"] = {"instances": len(tab.batches), "rows": tab.num_rows, "groups_total": int(groups.sum()),
+                          "graph_ms": gm, "graph_min_ms": gmin, "graph_rows_per_s": tab.num_rows / (gm / 1e3),
+                          "graph_GBps": tab.nbytes / (gm / 1e3) / 1e9, "plain_ms": pm,
+                          "plain_rows_per_s": tab.num_rows / (pm / 1e3), "batched_over_plain": pm / gm,
+                          "autotune_candidate": chosen, "autotune_s": tune_s,
+                          "note": "graph = one captured launch (table reset, partition and update kernels) "
+                                  "+ finish (collect, sort, results); plain = one aggregate per batch"}
+    it.close()
+    del tab, arr
+    return out
+
+
+def bench_join(args, ctx, stream):
+    import torch
+
+    from gazelle_plugin_amd import datagen
+    from gazelle_plugin_amd import types as T
+    from gazelle_plugin_amd.join import ConditionedProbeKernel, HashRelationKernel
+    from gazelle_plugin_amd.splitter import make_batch_array
+
+    build, probe = datagen.join_device(probe_rows=args.join_probe_rows)
+    barr = make_batch_array(build.batches)
+    parr = make_batch_array(probe.batches)
+    torch.cuda.synchronize()
+
+    def do_build():
+        r = HashRelationKernel.Make(ctx, [T.INT64, T.INT32, T.DOUBLE], 0)
+        r.EvaluateBatches(build.batches)
+        r.Finish()
+        return r
+
+    rel = do_build()
+    torch.cuda.synchronize()
+    holder = {}
+
+    def rebuild():
+        holder["r"] = do_build()
+        holder["r"].close()
+
+    build_ms, _ = timed(rebuild, stream, 2)
+    out = {"build_rows": build.num_rows, "probe_rows": probe.num_rows, "build_ms": build_ms,
+           "build_rows_per_s": build.num_rows / (build_ms / 1e3)}
+    for jt, result in (("inner", [(1, 0), (0, 1), (0, 2)]), ("semi", [(1, 0)]), ("anti", [(1, 0)])):
+        pk = ConditionedProbeKernel.Make(rel, [T.INT64], 0, jt, False, result)
+        res = {}
+
+        def run():
+            res["n"], _ = pk.ProcessBatches(probe.batches, parr)
+
+        run()
+        ms, mn = timed(run, stream, 3)
+        n = res["n"]
+        # algorithmic bytes: probe key + table entry per probe row, plus the
+        # gathered build payload and the output per output row
+        per_out = 12 + 20 if jt == "inner" else 8
+        algo = probe.num_rows * (8 + 12) + n * per_out
+        out[jt] = {"ms": ms, "min_ms": mn, "out_rows": n, "probe_rows_per_s": probe.num_rows / (ms / 1e3),
+                   "GBps": algo / (ms / 1e3) / 1e9, "frac_of_hbm_peak": algo / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+        pk.close()
+    rel.close()
+    del build, probe, barr, parr
+    return out
+
+
+def bench_serve(args):
+    import numpy as np
+    import torch
+
+    from gazelle_plugin_amd import datagen
+    from gazelle_plugin_amd import types as T
+    from gazelle_plugin_amd.context import Context
+    from gazelle_plugin_amd.join import HashRelationKernel
+    from gazelle_plugin_amd.serving import AGGREGATE, JOIN_AGGREGATE, SPLIT, FragmentServer, latency_summary
+    from gazelle_plugin_amd.splitter import make_batch_array
+
+    B = datagen.BATCH_ROWS
+    max_rows = 50_000_000
+    nb_max = (max_rows + B - 1) // B
+    agg = datagen.agg_device(nb_max * B, num_keys=10000)
+    spl = datagen.split_device(nb_max * B)
+    build, probe = datagen.join_device(probe_rows=nb_max * B)
+    arrays = {AGGREGATE: make_batch_array(agg.batches), SPLIT: make_batch_array(spl.batches),
+              JOIN_AGGREGATE: make_batch_array(probe.batches)}
+    torch.cuda.synchronize()
+    bctx = Context(0)
+    rel = HashRelationKernel.Make(bctx, [T.INT64, T.INT32, T.DOUBLE], 0)
+    rel.EvaluateBatches(build.batches)
+    rel.Finish()
+    budget = 200 << 30
+    srv = FragmentServer(0, budget, args.serve_slots, agg=(datagen.AGG_TYPES, AGG_ACTIONS),
+                         split=(datagen.SPLIT_TYPES, 200, [0]),
+                         probe=([T.INT64], 0, "inner", [(1, 0), (0, 2), (0, 1)]),
+                         join_agg=([T.INT64, T.DOUBLE, T.INT32], AGG_ACTIONS))
+    rid = srv.add_relation(rel)
+    rng = np.random.default_rng(4242)
+    n = args.serve_fragments
+    kinds = rng.choice([AGGREGATE, JOIN_AGGREGATE, SPLIT], size=n, p=[0.4, 0.4, 0.2])
+    rows = np.exp(rng.uniform(math.log(1e4), math.log(5e7), size=n))
+    nbs = np.clip(np.ceil(rows / B).astype(np.int64), 1, nb_max)
+    starts = (rng.random(n) * (nb_max - nbs + 1)).astype(np.int64)
+
+    def frags(arrivals):
+        return [(int(k), rid, arrays[int(k)], int(s), int(m), float(t))
+                for k, s, m, t in zip(kinds, starts, nbs, arrivals)]
+
+    # capacity: everything arrives at once
+    cap = srv.run(frags(np.zeros(n)))
+    cap_s = latency_summary(cap)
+    qps_cap = n / cap_s["span_s"]
+    qps = 0.7 * qps_cap
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n))
+    arrivals -= arrivals[0]
+    st = srv.run(frags(arrivals))
+    summ = latency_summary(st)
+    by_kind = {}
+    for k, name in ((AGGREGATE, "aggregate"), (JOIN_AGGREGATE, "join_aggregate"), (SPLIT, "split")):
+        m = kinds == k
+        if m.any():
+            lat = (st["done_s"] - st["arrival_s"])[m]
+            by_kind[name] = {"count": int(m.sum()), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                             "p99_ms": float(np.percentile(lat, 99) * 1e3)}
+    srv.close()
+    rel.close()
+    bctx.close()
+    return {"fragments": n, "slots": args.serve_slots, "hbm_budget_bytes": budget,
+            "capacity_fragments_per_s": qps_cap, "capacity_rows_per_s": cap_s["rows_per_s"],
+            "qps": qps, "p50_ms": summ["p50_ms"], "p99_ms": summ["p99_ms"], "rows_per_s": summ["rows_per_s"],
+            "rows": summ["rows"], "span_s": summ["span_s"], "by_kind": by_kind,
+            "note": "Poisson arrivals at 70% of the measured capacity; latency = completion - arrival"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; how the path would shard over several is "
+                                   "described in DESIGN.md, multi-GPU runs are not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from gazelle_plugin_amd.context import Context
+
+    torch.cuda.init()
+    stream = torch.cuda.current_stream()
+    ctx = Context(0, stream=stream.cuda_stream)
+    sections = set(args.sections.split(","))
+    split = bench_split(args, ctx, stream)
+    torch.cuda.empty_cache()
+    extra = {}
+    for name, fn in (("aggregate", lambda: bench_agg(args, ctx, stream)),
+                     ("join", lambda: bench_join(args, ctx, stream)),
+                     ("serving", lambda: bench_serve(args))):
+        key = {"aggregate": "agg", "join": "join", "serving": "serve"}[name]
+        if key not in sections:
+            continue
+        try:
+            extra[name] = fn()
+        except Exception as e:  # reported in the line, the headline stays valid
+            extra[name] = {"error": f"{type(e).__name__}: {e}", "trace": traceback.format_exc()[-800:]}
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
     if args.calibrate_copy:
         # a plain copy of known size: the PMC FETCH_SIZE / WRITE_SIZE it reads
         # scale the split kernel's counters to bytes
@@ -178,33 +439,34 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
     cpu = None if args.no_cpu else cpu_baseline_split(args.cpu_seconds)
+    if not args.no_cpu and "aggregate" in extra and "error" not in extra["aggregate"]:
+        extra["aggregate"]["cpu_baseline"] = cpu_baseline_agg(min(5.0, args.cpu_seconds))
     out = {
         "metric": "rows/s + HBM GB/s (% peak), batched vs plain GPU path; p50/p99 fragment latency",
-        "value": rows_per_s,
+        "value": split["rows_per_s"],
         "unit": "rows/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": split["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic (seeded, generated in HBM)",
-        "config": {"workload": "shuffle hash split: 64M rows in 4096-row batches, int64 key "
-                               "U[0,2^40) + int32 + float64 (2% nulls) + utf8 len U[8,32], "
-                               "200 partitions, Spark Murmur3 seed 42 pmod",
-                   "rows": N, "batches": len(tab.batches), "partitions": P,
+        "config": {"workload": "shuffle hash split: 64M rows in 4096-row batches, int64 key U[0,2^40) + int32 + "
+                               "float64 (2% nulls) + utf8 len U[8,32], 200 partitions, Spark Murmur3 seed 42 pmod",
+                   "rows": split["rows"], "batches": split["batches"], "partitions": split["partitions"],
                    "path": "batched (all batches in one launch sequence)"},
-        "step_GBps": algo_bytes / (ms_per_step / 1e3) / 1e9,
-        "kernels_ms_avg": {"split_count": cnt_ms / max(1, cnt_n), "split_scan": scan_ms / max(1, scan_n),
-                           "split_scatter": scat_avg},
-        "plain": {"rows_per_s": plain_rows_s, "batches": nb, "ms": plain_ms},
-        "batched_over_plain": rows_per_s / plain_rows_s,
-        "wall_s": t1 - t0,
-        "roofline": roof,
+        "step_GBps": split["step_GBps"],
+        "kernels_ms_avg": split["kernels_ms_avg"],
+        "plain": split["plain"],
+        "batched_over_plain": split["batched_over_plain"],
+        "wall_s": split["wall_s"],
+        "roofline": split["roofline"],
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
     ctx.close()
 

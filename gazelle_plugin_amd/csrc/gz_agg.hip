@@ -527,43 +527,71 @@ __global__ __launch_bounds__(256) void agg_part_scan_kernel(const unsigned int* 
 }
 
 // Records of a tile: partition counts in LDS, one range reserved per non-empty
-// partition, then each row takes the next place of its partition's range.
+// partition of the tile, the tile's records laid out in LDS in partition order,
+// then written out as one contiguous run per partition (consecutive threads,
+// consecutive records).  Tiles of this kernel are at most kPartTile rows.
+constexpr int kPartTile = 1024;
+
 __global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, unsigned long long* pcursor,
                                                                int64_t* recs) {
+  extern __shared__ int64_t s_rec[];  // [kPartTile][W]
   __shared__ unsigned int s_h[kMaxFanout];
+  __shared__ unsigned int s_off[kMaxFanout + 1];
   __shared__ unsigned long long s_base[kMaxFanout];
+  __shared__ unsigned char s_pt[kPartTile];
+  __shared__ uint64_t s_scan[256 / kWave + 1];
   int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
   if (t < 0) return;
   const AggTile td = p.tiles[t];
   const AggInst I = p.inst[td.inst];
   const int F = 1 << I.fan_log2;
   const int W = 2 + p.nu;
+  constexpr int RPT = kPartTile / 256;
   for (int b = threadIdx.x; b < F; b += blockDim.x) s_h[b] = 0;
   __syncthreads();
-  for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
-    int64_t key;
-    bool valid = row_key(p, td.batch, (int64_t)td.row_begin + lr, &key);
-    atomicAdd(&s_h[part_of(I, valid, key)], 1u);
+  // partition and in-partition rank of the owned rows (rank order is free: the
+  // record carries the row index)
+  int part[RPT], rank[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    const int lr = k * 256 + threadIdx.x;
+    part[k] = 0;
+    rank[k] = 0;
+    if (lr < td.nrows) {
+      int64_t key;
+      bool valid = row_key(p, td.batch, (int64_t)td.row_begin + lr, &key);
+      part[k] = part_of(I, valid, key);
+      rank[k] = (int)atomicAdd(&s_h[part[k]], 1u);
+    }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < F; b += blockDim.x) {
-    s_base[b] = s_h[b] ? atomicAdd(&pcursor[I.bucket_base + b], (unsigned long long)s_h[b]) : 0ull;
-    s_h[b] = 0;
+  {
+    // tile offsets of the partitions (F <= 256: one per thread)
+    const uint64_t c = threadIdx.x < F ? s_h[threadIdx.x] : 0;
+    uint64_t total;
+    const uint64_t ex = block_exclusive_scan<uint64_t>(c, s_scan, &total);
+    if (threadIdx.x < F) {
+      s_off[threadIdx.x] = (unsigned)ex;
+      s_base[threadIdx.x] = c ? atomicAdd(&pcursor[I.bucket_base + threadIdx.x], (unsigned long long)c) : 0ull;
+    }
+    if (threadIdx.x == 0) s_off[F] = (unsigned)total;
   }
   __syncthreads();
-  for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    const int lr = k * 256 + threadIdx.x;
+    if (lr >= td.nrows) continue;
     const int64_t r = (int64_t)td.row_begin + lr;
+    const int slot = (int)s_off[part[k]] + rank[k];
+    int64_t* rec = s_rec + (int64_t)slot * W;
     int64_t key;
     bool valid = row_key(p, td.batch, r, &key);
-    const int b = part_of(I, valid, key);
-    const unsigned long long pos = s_base[b] + atomicAdd(&s_h[b], 1u);
-    int64_t* rec = recs + (int64_t)pos * W;
     uint64_t nulls = valid ? 0ull : (1ull << kRecKeyNull);
-    for (int k = 0; k < p.nu; k++) {
-      const int c = p.ucol[k];
+    for (int u = 0; u < p.nu; u++) {
+      const int c = p.ucol[u];
       int64_t v = 0;
       if (!is_valid(p, td.batch, c, r)) {
-        nulls |= 1ull << k;
+        nulls |= 1ull << u;
       } else {
         const int ty = p.col_type[c];
         const void* vb = abuf(p, td.batch, c, 2);
@@ -571,17 +599,27 @@ __global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, unsi
         else if (ty == GZ_TYPE_UINT64) v = ((const int64_t*)vb)[r];
         else v = read_i64(vb, ty, r);
       }
-      rec[2 + k] = v;
+      rec[2 + u] = v;
     }
     rec[0] = valid ? key : 0;
     rec[1] = (int64_t)((uint64_t)(td.row_base + lr) | (nulls << 48));
+    s_pt[slot] = (unsigned char)part[k];
+  }
+  __syncthreads();
+  // one 8-byte word per thread per step over the tile's records in partition order
+  const int n = td.nrows;
+  for (int x = threadIdx.x; x < n * W; x += blockDim.x) {
+    const int slot = x / W, w = x - slot * W;
+    const int b = s_pt[slot];
+    const unsigned long long pos = s_base[b] + (unsigned long long)(slot - (int)s_off[b]);
+    recs[(int64_t)pos * W + w] = s_rec[x];
   }
 }
 
 // Workgroups walk fixed chunks of the record array; inside a chunk, each
 // partition segment is aggregated in an LDS table and merged into the HBM table
 // of the segment's instance.
-__global__ __launch_bounds__(256) void agg_part_update_kernel(AggParams p, const int64_t* recs, const int64_t* pstart,
+__global__ __launch_bounds__(1024) void agg_part_update_kernel(AggParams p, const int64_t* recs, const int64_t* pstart,
                                                               const int* pinst, int B, int64_t chunk) {
   extern __shared__ int64_t s_tab[];
   const int64_t LS = p.lcap + 2;
@@ -1255,7 +1293,11 @@ static int agg_process(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     GZ_TRY(agg_prepare(h, nb, batches, inst_of));
     return agg_run(h);
   }
-  const int tile_rows = h->opts.tile_rows > 0 ? h->opts.tile_rows : 4096;
+  // small calls (one batch at a time) get small tiles so a call still spreads
+  // over many workgroups
+  int64_t call_rows = 0;
+  for (int32_t b = 0; b < nb; b++) call_rows += std::max<int64_t>(0, batches[b].num_rows);
+  const int tile_rows = h->opts.tile_rows > 0 ? h->opts.tile_rows : (call_rows < (1 << 20) ? 512 : 4096);
   std::vector<AggTile> tiles;
   std::vector<int64_t> rows;
   GZ_TRY(agg_stage(h, nb, batches, inst_of, tiles, rows, tile_rows));
@@ -1604,8 +1646,19 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   p.inst = (const AggInst*)h->d_inst.p;
   // launch plan
   std::vector<AggTile> lds_tiles, hbm_tiles, part_tiles;
-  for (auto& t : tiles)
-    (h->placement[t.inst] == 1 ? lds_tiles : h->placement[t.inst] == 3 ? part_tiles : hbm_tiles).push_back(t);
+  for (auto& t : tiles) {
+    if (h->placement[t.inst] != 3) {
+      (h->placement[t.inst] == 1 ? lds_tiles : hbm_tiles).push_back(t);
+      continue;
+    }
+    for (int32_t r0 = 0; r0 < t.nrows; r0 += kPartTile) {  // partition tiles are kPartTile rows
+      AggTile u = t;
+      u.row_begin = t.row_begin + r0;
+      u.nrows = std::min<int32_t>(kPartTile, t.nrows - r0);
+      u.row_base = t.row_base + r0;
+      part_tiles.push_back(u);
+    }
+  }
   h->grid_part = 0;
   h->nrec = 0;
   if (!part_tiles.empty()) {
@@ -1683,9 +1736,10 @@ static int agg_run_kernels(gz_agg* h) {
                        (unsigned int*)h->pcount.p);
     hipLaunchKernelGGL(agg_part_scan_kernel, dim3(1), dim3(256), 0, st, (const unsigned int*)h->pcount.p, h->B,
                        (int64_t*)h->pstart.p, (unsigned long long*)h->pcursor.p);
-    hipLaunchKernelGGL(agg_part_scatter_kernel, dim3(h->grid_part), dim3(256), 0, st, h->p_part,
+    hipLaunchKernelGGL(agg_part_scatter_kernel, dim3(h->grid_part), dim3(256),
+                       sizeof(int64_t) * kPartTile * (2 + h->p_part.nu), st, h->p_part,
                        (unsigned long long*)h->pcursor.p, (int64_t*)h->recs.p);
-    hipLaunchKernelGGL(agg_part_update_kernel, dim3(h->blocks_part_upd), dim3(256), h->lds_part, st, h->p_part,
+    hipLaunchKernelGGL(agg_part_update_kernel, dim3(h->blocks_part_upd), dim3(1024), h->lds_part, st, h->p_part,
                        (const int64_t*)h->recs.p, (const int64_t*)h->pstart.p, (const int*)h->pinst.p, h->B,
                        h->part_chunk);
   }

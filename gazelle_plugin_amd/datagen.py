@@ -222,3 +222,55 @@ def batched_agg_device(num_batches: int = 512, seed: int = 42, device: str = "cu
         batches.append(DeviceBatch(n, AGG_TYPES, bufs))
     nbytes = total * (8 + 8 + 4) + total // 8
     return DeviceTable((key, price, vbits, qty), batches, total, nbytes), rows, groups
+
+
+# ------------------------------------------------------------------ join workload
+
+def join_device(build_rows: int = 4 << 20, probe_rows: int = 256 << 20, zipf_s: float = 1.1,
+                miss_frac: float = 0.05, batch_rows: int = BATCH_ROWS, seed: int = 42, device: str = "cuda"):
+    """The hash-join workload in HBM.  Build: unique int64 keys in [0, 2^40)
+    (an odd multiplier permutes that range), int32 and float64 payloads.  Probe:
+    foreign keys build_key[perm[r]] with r ~ Zipf(s) over [1, build_rows]
+    (inverse CDF), and miss_frac of them replaced by keys >= 2^40 that are absent.
+    Returns (build DeviceTable, probe DeviceTable)."""
+    import torch
+
+    assert build_rows % batch_rows == 0 and probe_rows % batch_rows == 0
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    idx = torch.randperm(build_rows, device=device, generator=g).to(torch.int64)
+    bkey = (idx * 0x9E3779B1 + 12345) & ((1 << 40) - 1)
+    bi32 = torch.randint(-(1 << 31), (1 << 31) - 1, (build_rows,), dtype=torch.int64, device=device,
+                         generator=g).to(torch.int32)
+    bf64 = torch.randn(build_rows, dtype=torch.float64, device=device, generator=g)
+    # Zipf ranks by inverse CDF
+    w = torch.arange(1, build_rows + 1, dtype=torch.float64, device=device).pow(-zipf_s)
+    cdf = torch.cumsum(w, 0)
+    cdf /= cdf[-1].clone()
+    del w
+    perm = torch.randperm(build_rows, device=device, generator=g)
+    fk = torch.empty(probe_rows, dtype=torch.int64, device=device)
+    step = 32 << 20
+    for a in range(0, probe_rows, step):
+        b = min(probe_rows, a + step)
+        u = torch.rand(b - a, dtype=torch.float64, device=device, generator=g)
+        rank = torch.searchsorted(cdf, u).clamp_(max=build_rows - 1)
+        k = bkey[perm[rank]]
+        miss = torch.rand(b - a, device=device, generator=g) < miss_frac
+        absent = (1 << 40) + torch.randint(0, 1 << 40, (b - a,), dtype=torch.int64, device=device, generator=g)
+        fk[a:b] = torch.where(miss, absent, k)
+        del u, rank, k, miss, absent
+    del cdf, perm, idx
+    build_types = [T.INT64, T.INT32, T.DOUBLE]
+    kb, ib, fb = (t.view(torch.uint8) for t in (bkey, bi32, bf64))
+    bb = []
+    for b0 in range(0, build_rows, batch_rows):
+        n = batch_rows
+        bb.append(DeviceBatch(n, build_types, [None, kb[b0 * 8:(b0 + n) * 8], None, ib[b0 * 4:(b0 + n) * 4],
+                                               None, fb[b0 * 8:(b0 + n) * 8]]))
+    pk = fk.view(torch.uint8)
+    pb = [DeviceBatch(batch_rows, [T.INT64], [None, pk[b0 * 8:(b0 + batch_rows) * 8]])
+          for b0 in range(0, probe_rows, batch_rows)]
+    build = DeviceTable((bkey, bi32, bf64), bb, build_rows, build_rows * 20)
+    probe = DeviceTable((fk,), pb, probe_rows, probe_rows * 8)
+    return build, probe
