@@ -36,7 +36,7 @@ namespace gz {
 constexpr int kMaxCols = 32;
 constexpr int kMaxKeys = 4;
 constexpr int kMaxPartitions = 2048;
-constexpr int kScanChunk = 64;  // tiles per chunk of the tile scan
+constexpr int kScanChunk = 16;  // tiles per chunk of the tile scan
 constexpr int kPre = 4;         // fixed-width columns the scatter loads up front
 // ... fewer when a thread owns more rows (registers)
 __host__ __device__ constexpr int scatter_max_pre(int rows_per_thread) {
