@@ -32,6 +32,7 @@
 
 #include "gz_device.h"
 #include "gz_internal.h"
+#include "gz_scan.h"
 
 namespace gz {
 
@@ -479,37 +480,56 @@ __global__ __launch_bounds__(256) void agg_update_lds_kernel(AggParams p) {
 // Record (8-byte words): key, first-row index | null mask << 48 (bit 15 = key
 // null), then the used input columns widened to 8 bytes.
 constexpr int kRecKeyNull = 15;
+constexpr int kPartTile = 1024;       // rows per tile of the partition kernels
+constexpr int kPartScanChunk = 16;    // tiles per chunk of their scan
 
 __device__ __forceinline__ int part_of(const AggInst& I, bool valid, int64_t key) {
   if (!valid || I.fan_log2 == 0) return 0;
   return (int)(fmix64((uint64_t)key) >> (64 - I.fan_log2));
 }
 
-// Rows per partition (tile histogram in LDS, one global atomic per non-empty
-// partition of the tile).
-__global__ __launch_bounds__(256) void agg_part_hist_kernel(AggParams p, unsigned int* pcount) {
+// Rows per partition of every tile (LDS histogram), written as the tile's row
+// of a [T][R] count matrix (R = the largest fan-out; columns past the tile's
+// instance's fan-out are zero).  No global atomics: partition starts come from a
+// scan of this matrix (tiles of one instance are contiguous).
+__global__ __launch_bounds__(256) void agg_part_hist_kernel(AggParams p, uint32_t* tcount, int R) {
   __shared__ unsigned int s_h[kMaxFanout];
   int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
   if (t < 0) return;
   const AggTile td = p.tiles[t];
   const AggInst I = p.inst[td.inst];
-  const int F = 1 << I.fan_log2;
-  for (int b = threadIdx.x; b < F; b += blockDim.x) s_h[b] = 0;
+  for (int b = threadIdx.x; b < R; b += blockDim.x) s_h[b] = 0;
   __syncthreads();
-  for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
-    int64_t key;
-    bool valid = row_key(p, td.batch, (int64_t)td.row_begin + lr, &key);
-    atomicAdd(&s_h[part_of(I, valid, key)], 1u);
+  constexpr int RPT = kPartTile / 256;
+  int64_t key[RPT];
+  bool valid[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    const int lr = k * 256 + threadIdx.x;
+    valid[k] = lr < td.nrows && row_key(p, td.batch, (int64_t)td.row_begin + lr, &key[k]);
   }
+#pragma unroll
+  for (int k = 0; k < RPT; k++)
+    if (k * 256 + threadIdx.x < td.nrows) atomicAdd(&s_h[part_of(I, valid[k], key[k])], 1u);
   __syncthreads();
-  for (int b = threadIdx.x; b < F; b += blockDim.x)
-    if (s_h[b]) atomicAdd(&pcount[I.bucket_base + b], s_h[b]);
+  for (int b = threadIdx.x; b < R; b += blockDim.x) gst(tcount + t * R + b, s_h[b]);
 }
 
-// One block: partition starts (exclusive scan of the counts, total at [B]) and
-// the write cursors.
-__global__ __launch_bounds__(256) void agg_part_scan_kernel(const unsigned int* pcount, int B, int64_t* pstart,
-                                                             unsigned long long* pcursor) {
+// Rows of every global partition g = (instance, b), from the tile prefixes at
+// the instance's first and one-past-last tile.
+__global__ void agg_part_totals_kernel(const uint32_t* tprefix, const uint64_t* tot, int R, int64_t T,
+                                       const AggInst* inst, const int* pinst, const int* it0, const int* it1, int B,
+                                       unsigned int* pcount) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= B) return;
+  const int i = pinst[g];
+  const int b = g - inst[i].bucket_base;
+  const uint64_t hi = it1[i] < T ? (uint64_t)tprefix[(int64_t)it1[i] * R + b] : tot[b];
+  pcount[g] = (unsigned int)(hi - tprefix[(int64_t)it0[i] * R + b]);
+}
+
+// One block: partition starts (exclusive scan of the counts, total at [B]).
+__global__ __launch_bounds__(256) void agg_part_scan_kernel(const unsigned int* pcount, int B, int64_t* pstart) {
   __shared__ uint64_t s_scan[256 / kWave + 1];
   uint64_t carry = 0;
   for (int base = 0; base < B; base += 256) {
@@ -517,10 +537,7 @@ __global__ __launch_bounds__(256) void agg_part_scan_kernel(const unsigned int* 
     uint64_t v = i < B ? pcount[i] : 0;
     uint64_t total;
     uint64_t ex = block_exclusive_scan<uint64_t>(v, s_scan, &total);
-    if (i < B) {
-      pstart[i] = (int64_t)(carry + ex);
-      pcursor[i] = carry + ex;
-    }
+    if (i < B) pstart[i] = (int64_t)(carry + ex);
     carry += total;
   }
   if (threadIdx.x == 0) pstart[B] = (int64_t)carry;
@@ -530,9 +547,9 @@ __global__ __launch_bounds__(256) void agg_part_scan_kernel(const unsigned int* 
 // partition of the tile, the tile's records laid out in LDS in partition order,
 // then written out as one contiguous run per partition (consecutive threads,
 // consecutive records).  Tiles of this kernel are at most kPartTile rows.
-constexpr int kPartTile = 1024;
 
-__global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, unsigned long long* pcursor,
+__global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, const uint32_t* tprefix, int R,
+                                                               const int64_t* pstart, const int* it0,
                                                                int64_t* recs) {
   extern __shared__ int64_t s_rec[];  // [kPartTile][W]
   __shared__ unsigned int s_h[kMaxFanout];
@@ -572,7 +589,9 @@ __global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, unsi
     const uint64_t ex = block_exclusive_scan<uint64_t>(c, s_scan, &total);
     if (threadIdx.x < F) {
       s_off[threadIdx.x] = (unsigned)ex;
-      s_base[threadIdx.x] = c ? atomicAdd(&pcursor[I.bucket_base + threadIdx.x], (unsigned long long)c) : 0ull;
+      const int b = threadIdx.x;
+      s_base[b] = (unsigned long long)pstart[I.bucket_base + b] + tprefix[t * R + b] -
+                  tprefix[(int64_t)it0[td.inst] * R + b];
     }
     if (threadIdx.x == 0) s_off[F] = (unsigned)total;
   }
@@ -989,7 +1008,8 @@ struct gz_agg {
   int lds_threshold = 256;  // groups up to which an instance gets the LDS first level
   int part_groups = 512;    // groups per partition on the partitioned placement (0 = off)
   // partitioned placement buffers of the plan
-  DBuf d_part, pcount, pstart, pcursor, pinst, recs;
+  DBuf d_part, pcount, pstart, pinst, recs, tcount, tprefix, tcsum, ttot, it0, it1;
+  int R = 0;                // largest fan-out (columns of the tile count matrix)
   int B = 0;                // partitions over all partitioned instances
   int64_t nrec = 0;
   AggParams p_part;
@@ -1662,12 +1682,28 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   h->grid_part = 0;
   h->nrec = 0;
   if (!part_tiles.empty()) {
+    // tiles of one instance contiguous (the partition scan relies on it)
+    std::stable_sort(part_tiles.begin(), part_tiles.end(),
+                     [](const AggTile& a, const AggTile& b) { return a.inst < b.inst; });
+    std::vector<int> t0(h->ninst, 0), t1(h->ninst, 0);
+    for (int64_t t = (int64_t)part_tiles.size() - 1; t >= 0; t--) t0[part_tiles[t].inst] = (int)t;
+    for (int64_t t = 0; t < (int64_t)part_tiles.size(); t++) t1[part_tiles[t].inst] = (int)t + 1;
+    h->R = 0;
+    for (int i = 0; i < h->ninst; i++)
+      if (h->placement[i] == 3) h->R = std::max(h->R, 1 << h->inst[i].fan_log2);
+    const int64_t TP = (int64_t)part_tiles.size();
+    const int64_t C = (TP + kPartScanChunk - 1) / kPartScanChunk;
+    GZ_TRY(upload(ctx, &h->it0, t0.data(), sizeof(int) * t0.size()));
+    GZ_TRY(upload(ctx, &h->it1, t1.data(), sizeof(int) * t1.size()));
+    GZ_TRY(ctx_ensure(ctx, &h->tcount, sizeof(uint32_t) * TP * h->R));
+    GZ_TRY(ctx_ensure(ctx, &h->tprefix, sizeof(uint32_t) * TP * h->R));
+    GZ_TRY(ctx_ensure(ctx, &h->tcsum, sizeof(uint64_t) * C * h->R));
+    GZ_TRY(ctx_ensure(ctx, &h->ttot, sizeof(uint64_t) * h->R));
     for (auto& t : part_tiles) h->nrec += t.nrows;
     GZ_TRY(upload(ctx, &h->d_part, part_tiles.data(), sizeof(AggTile) * part_tiles.size()));
     GZ_TRY(upload(ctx, &h->pinst, pinst.data(), sizeof(int) * pinst.size()));
     GZ_TRY(ctx_ensure(ctx, &h->pcount, sizeof(unsigned) * B));
     GZ_TRY(ctx_ensure(ctx, &h->pstart, sizeof(int64_t) * (B + 1)));
-    GZ_TRY(ctx_ensure(ctx, &h->pcursor, sizeof(uint64_t) * B));
     GZ_TRY(ctx_ensure(ctx, &h->recs, sizeof(int64_t) * (2 + p.nu) * h->nrec));
     AggParams& q = h->p_part;
     q = p;
@@ -1730,15 +1766,27 @@ static int agg_run_kernels(gz_agg* h) {
   if (h->L.blocks_lds)
     hipLaunchKernelGGL(agg_update_lds_kernel, dim3(h->L.blocks_lds), dim3(256), h->L.lds, st, h->L.p_lds);
   if (h->grid_part) {
-    hipLaunchKernelGGL(fill_u32_kernel_agg, dim3((unsigned)std::min<int64_t>((h->B + 255) / 256, 4096)), dim3(256), 0,
-                       st, (uint32_t*)h->pcount.p, (int64_t)h->B);
+    const int64_t TP = h->p_part.T;
+    const int64_t C = (TP + kPartScanChunk - 1) / kPartScanChunk;
     hipLaunchKernelGGL(agg_part_hist_kernel, dim3(h->grid_part), dim3(256), 0, st, h->p_part,
-                       (unsigned int*)h->pcount.p);
+                       (uint32_t*)h->tcount.p, h->R);
+    hipLaunchKernelGGL(tile_scan_sum_kernel<kPartScanChunk>, dim3((unsigned)C), dim3(256), 0, st,
+                       (const uint32_t*)h->tcount.p, h->R, TP, (uint64_t*)h->tcsum.p);
+    hipLaunchKernelGGL(tile_scan_chunks_kernel, dim3((unsigned)((h->R + 255) / 256)), dim3(256), 0, st,
+                       (uint64_t*)h->tcsum.p, h->R, C, (uint64_t*)h->ttot.p);
+    hipLaunchKernelGGL(tile_scan_write_kernel<kPartScanChunk>, dim3((unsigned)C), dim3(256), 0, st,
+                       (const uint32_t*)h->tcount.p, (const uint64_t*)h->tcsum.p, h->R, TP,
+                       (uint32_t*)h->tprefix.p);
+    hipLaunchKernelGGL(agg_part_totals_kernel, dim3((unsigned)((h->B + 255) / 256)), dim3(256), 0, st,
+                       (const uint32_t*)h->tprefix.p, (const uint64_t*)h->ttot.p, h->R, TP,
+                       (const AggInst*)h->d_inst.p, (const int*)h->pinst.p, (const int*)h->it0.p,
+                       (const int*)h->it1.p, h->B, (unsigned int*)h->pcount.p);
     hipLaunchKernelGGL(agg_part_scan_kernel, dim3(1), dim3(256), 0, st, (const unsigned int*)h->pcount.p, h->B,
-                       (int64_t*)h->pstart.p, (unsigned long long*)h->pcursor.p);
+                       (int64_t*)h->pstart.p);
     hipLaunchKernelGGL(agg_part_scatter_kernel, dim3(h->grid_part), dim3(256),
                        sizeof(int64_t) * kPartTile * (2 + h->p_part.nu), st, h->p_part,
-                       (unsigned long long*)h->pcursor.p, (int64_t*)h->recs.p);
+                       (const uint32_t*)h->tprefix.p, h->R, (const int64_t*)h->pstart.p, (const int*)h->it0.p,
+                       (int64_t*)h->recs.p);
     hipLaunchKernelGGL(agg_part_update_kernel, dim3(h->blocks_part_upd), dim3(1024), h->lds_part, st, h->p_part,
                        (const int64_t*)h->recs.p, (const int64_t*)h->pstart.p, (const int*)h->pinst.p, h->B,
                        h->part_chunk);
@@ -1947,8 +1995,8 @@ int gz_agg_release(gz_agg* h) {
   for (auto& b : h->res_valid) ctx_free(ctx, &b);
   for (DBuf* b : {&h->d_inst, &h->d_lane_ops, &h->d_tiles, &h->d_bufs, &h->d_regs, &h->d_est, &h->skeys,
                   &h->svals, &h->tkeys, &h->tvals, &h->cursor, &h->ccpre, &h->counts, &h->sort_tmp, &h->L.d_hbm,
-                  &h->L.d_lds, &h->L.d_slot_prefix, &h->L.d_bufs, &h->d_part, &h->pcount, &h->pstart, &h->pcursor, &h->pinst,
-                  &h->recs})
+                  &h->L.d_lds, &h->L.d_slot_prefix, &h->L.d_bufs, &h->d_part, &h->pcount, &h->pstart, &h->pinst,
+                  &h->recs, &h->tcount, &h->tprefix, &h->tcsum, &h->ttot, &h->it0, &h->it1})
     ctx_free(ctx, b);
   if (h->L.exec) hipGraphExecDestroy(h->L.exec);
   if (h->L.graph) hipGraphDestroy(h->L.graph);

@@ -30,6 +30,7 @@
 
 #include "gz_device.h"
 #include "gz_internal.h"
+#include "gz_scan.h"
 
 namespace gz {
 
@@ -254,63 +255,7 @@ __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
 }
 
 // ---------------------------------------------------------------- scan
-// Exclusive scan over tiles of every counter of the [T][R] count matrix, in
-// three launches over chunks of kScanChunk tiles: chunk sums, a scan of the chunk
-// sums (which also gives the totals), then the per-tile prefixes.  Every access
-// walks tile rows, which are contiguous.
-__global__ __launch_bounds__(256) void split_scan_sum_kernel(const uint32_t* counts, int R, int64_t T,
-                                                               uint64_t* csum) {
-  int64_t c = blockIdx.x;
-  int64_t t0 = c * kScanChunk, t1 = min<int64_t>(T, t0 + kScanChunk);
-  for (int e = threadIdx.x; e < R; e += 256) {
-    uint64_t s = 0;
-    for (int64_t tb = t0; tb < t1; tb += 16) {
-      uint32_t v[16];  // sixteen independent loads in flight
-#pragma unroll
-      for (int k = 0; k < 16; k++) v[k] = tb + k < t1 ? gld(counts + (tb + k) * R + e) : 0u;
-#pragma unroll
-      for (int k = 0; k < 16; k++) s += v[k];
-    }
-    csum[c * R + e] = s;
-  }
-}
-
-__global__ __launch_bounds__(256) void split_scan_chunks_kernel(uint64_t* csum, int R, int64_t C,
-                                                                  uint64_t* tot) {
-  int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= R) return;
-  uint64_t run = 0;
-  for (int64_t cb = 0; cb < C; cb += 16) {
-    uint64_t v[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) v[k] = cb + k < C ? csum[(cb + k) * R + e] : 0ull;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-      if (cb + k < C) csum[(cb + k) * R + e] = run;
-      run += v[k];
-    }
-  }
-  tot[e] = run;
-}
-
-__global__ __launch_bounds__(256) void split_scan_write_kernel(const uint32_t* counts, const uint64_t* cpre,
-                                                                 int R, int64_t T, uint32_t* prefix) {
-  int64_t c = blockIdx.x;
-  int64_t t0 = c * kScanChunk, t1 = min<int64_t>(T, t0 + kScanChunk);
-  for (int e = threadIdx.x; e < R; e += 256) {
-    uint64_t run = cpre[c * R + e];
-    for (int64_t tb = t0; tb < t1; tb += 16) {
-      uint32_t v[16];
-#pragma unroll
-      for (int k = 0; k < 16; k++) v[k] = tb + k < t1 ? gld(counts + (tb + k) * R + e) : 0u;
-#pragma unroll
-      for (int k = 0; k < 16; k++) {
-        if (tb + k < t1) gst(prefix + (tb + k) * R + e, (uint32_t)run);
-        run += v[k];
-      }
-    }
-  }
-}
+// (tile_scan_* in gz_scan.h)
 
 // One block: exclusive scans of the partition totals into row_start / byte_start,
 // first utf8 offset of every partition run set to 0, error on a partition whose
@@ -833,6 +778,7 @@ struct SplitSegment {
   int64_t N = 0;
   int32_t nbatches = 0;
   int64_t T = 0;
+  int R = 0;  // counters per tile row
   DBuf counts, prefix, csum, tot, row_start, byte_start, err, offtab;
   std::vector<DBuf> values, validity, offsets, data;
   std::vector<int32_t> tile_batch;  // batch of each tile
@@ -1099,6 +1045,7 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   // ---- allocate the segment.  Counter row of a tile: rows per partition, utf8
   // bytes per (binary column, partition), nulls per output column.
   const int R = P * (1 + nbin) + std::max(nout, 1);
+  seg->R = R;
   const int64_t C = (std::max<int64_t>(T, 1) + kScanChunk - 1) / kScanChunk;
   if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * R * std::max<int64_t>(T, 1), &seg->counts))) return bail(rc);
   if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * R * std::max<int64_t>(T, 1), &seg->prefix))) return bail(rc);
@@ -1196,11 +1143,11 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   {
     KernelTimer kt(ctx, "split_scan");
     if (T > 0) {
-      hipLaunchKernelGGL(split_scan_sum_kernel, dim3((unsigned)C), dim3(256), 0, st,
+      hipLaunchKernelGGL(tile_scan_sum_kernel<kScanChunk>, dim3((unsigned)C), dim3(256), 0, st,
                          (const uint32_t*)seg->counts.p, R, T, (uint64_t*)seg->csum.p);
-      hipLaunchKernelGGL(split_scan_chunks_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st,
+      hipLaunchKernelGGL(tile_scan_chunks_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st,
                          (uint64_t*)seg->csum.p, R, C, (uint64_t*)seg->tot.p);
-      hipLaunchKernelGGL(split_scan_write_kernel, dim3((unsigned)C), dim3(256), 0, st,
+      hipLaunchKernelGGL(tile_scan_write_kernel<kScanChunk>, dim3((unsigned)C), dim3(256), 0, st,
                          (const uint32_t*)seg->counts.p, (const uint64_t*)seg->csum.p, R, T,
                          (uint32_t*)seg->prefix.p);
     } else {
@@ -1364,16 +1311,17 @@ int gz_split_segment_batch_counts(gz_splitter* h, int32_t seg, int32_t* num_batc
   if (num_batches) *num_batches = s->nbatches;
   if (!counts) return GZ_OK;
   const int P = h->P;
-  std::vector<uint32_t> tc((size_t)P * std::max<int64_t>(s->T, 1));
+  // tile-major counters: row t holds the tile's rows per partition first
+  std::vector<uint32_t> tc((size_t)s->R * std::max<int64_t>(s->T, 1));
   if (s->T > 0) {
-    GZ_HIP(hipMemcpyAsync(tc.data(), s->counts.p, sizeof(uint32_t) * P * s->T,
+    GZ_HIP(hipMemcpyAsync(tc.data(), s->counts.p, sizeof(uint32_t) * s->R * s->T,
                           hipMemcpyDeviceToHost, h->ctx->stream));
     GZ_HIP(hipStreamSynchronize(h->ctx->stream));
   }
   memset(counts, 0, sizeof(int32_t) * (size_t)s->nbatches * P);
   for (int64_t t = 0; t < s->T; t++) {
     int b = s->tile_batch[t];
-    for (int p = 0; p < P; p++) counts[(size_t)b * P + p] += (int32_t)tc[(size_t)p * s->T + t];
+    for (int p = 0; p < P; p++) counts[(size_t)b * P + p] += (int32_t)tc[(size_t)t * s->R + p];
   }
   return GZ_OK;
 }

@@ -159,3 +159,17 @@ def test_row_ids_follow_rows(ctx):
         assert np.array_equal(keys_in[rid[a:b]], got_keys)
     assert np.array_equal(lay["row_start"], exp["row_start"])
     sp.close()
+
+
+def test_segment_batch_counts_follow_the_input_batches(ctx):
+    """Rows per (batch, partition) of a batched call, from the tile counters."""
+    types = [T.INT64, T.STRING]
+    batches = random_batches(types, [5000, 100, 4096, 3000], seed=23, null_frac=0.05)
+    sp = Splitter.Make(ctx, "hash", types, 31, [0])
+    sp.SplitBatches([DeviceBatch.from_host(b) for b in batches])
+    sp.Stop()
+    got = np.asarray(sp.segment_batch_counts(0)).reshape(len(batches), 31)
+    for bi, b in enumerate(batches):
+        pids = np.asarray(oracle.split_pids([b], types, 31, [0]))
+        assert np.array_equal(got[bi], np.bincount(pids, minlength=31)), bi
+    sp.close()
