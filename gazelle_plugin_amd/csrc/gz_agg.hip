@@ -87,6 +87,7 @@ struct AggParams {
   int nu;                     // used columns (lane inputs other than the key)
   int ucol[kMaxUsed];         // column of used slot k
   int uslot[kAggMaxCols];     // used slot of a column
+  int64_t rstride;            // records: word w of record q at recs[w * rstride + q]
 };
 
 __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
@@ -150,6 +151,83 @@ __device__ __forceinline__ bool row_key(const AggParams& p, int batch, int64_t r
   return true;
 }
 
+// ---- batched row loads: RPT rows lr0 + k * S (k < RPT) of a tile starting at
+// input row r0, rows with lr0 + k * S >= n inactive.  Every load is issued
+// before any value is used (one HBM round trip for all RPT rows); the type
+// switch is uniform.
+//
+// Record word of a column: integers widened as read_i64, floats as double bits,
+// 8-byte types raw (the same words the scatter writes and RecRow decodes).
+template <int RPT, int S>
+__device__ __forceinline__ void load_words(const void* vb, int ty, int64_t r0, int lr0, int n, int64_t (&v)[RPT]) {
+#define GZ_LOADW(T, CONV)                                                        \
+  {                                                                              \
+    const T* s = (const T*)vb + r0 + lr0;                                        \
+    T x[RPT];                                                                    \
+    _Pragma("unroll") for (int k = 0; k < RPT; k++) x[k] = lr0 + k * S < n ? gld(s + k * S) : (T)0; \
+    _Pragma("unroll") for (int k = 0; k < RPT; k++) v[k] = CONV(x[k]);          \
+    break;                                                                       \
+  }
+#define GZ_SEXT(x) ((int64_t)(x))
+#define GZ_F2W(x) (__double_as_longlong((double)(x)))
+  switch (ty) {
+    case GZ_TYPE_BOOL: {
+      const uint8_t* s = (const uint8_t*)vb;
+      uint8_t x[RPT];
+#pragma unroll
+      for (int k = 0; k < RPT; k++) x[k] = lr0 + k * S < n ? gld(s + ((r0 + lr0 + k * S) >> 3)) : (uint8_t)0;
+#pragma unroll
+      for (int k = 0; k < RPT; k++) v[k] = (x[k] >> ((r0 + lr0 + k * S) & 7)) & 1;
+      break;
+    }
+    case GZ_TYPE_INT8: GZ_LOADW(int8_t, GZ_SEXT)
+    case GZ_TYPE_UINT8: GZ_LOADW(uint8_t, GZ_SEXT)
+    case GZ_TYPE_INT16: GZ_LOADW(int16_t, GZ_SEXT)
+    case GZ_TYPE_UINT16: GZ_LOADW(uint16_t, GZ_SEXT)
+    case GZ_TYPE_INT32: case GZ_TYPE_DATE32: GZ_LOADW(int32_t, GZ_SEXT)
+    case GZ_TYPE_UINT32: GZ_LOADW(uint32_t, GZ_SEXT)
+    case GZ_TYPE_FLOAT: GZ_LOADW(float, GZ_F2W)
+    default: GZ_LOADW(int64_t, GZ_SEXT)
+  }
+#undef GZ_LOADW
+#undef GZ_SEXT
+#undef GZ_F2W
+}
+
+// Validity of RPT rows in two steps, so the bitmap loads can be issued before
+// other loads and used after them: valid_bytes issues the loads, valid_mask
+// gives bit k set when row k is active and valid (no bitmap: every active row).
+template <int RPT, int S>
+__device__ __forceinline__ void valid_bytes(const uint8_t* vb, int64_t r0, int lr0, int n, uint8_t (&x)[RPT]) {
+#pragma unroll
+  for (int k = 0; k < RPT; k++) x[k] = vb && lr0 + k * S < n ? gld(vb + ((r0 + lr0 + k * S) >> 3)) : (uint8_t)0xff;
+}
+template <int RPT, int S>
+__device__ __forceinline__ uint32_t valid_mask(const uint8_t (&x)[RPT], int64_t r0, int lr0, int n) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < RPT; k++)
+    if (lr0 + k * S < n && ((x[k] >> ((r0 + lr0 + k * S) & 7)) & 1)) m |= 1u << k;
+  return m;
+}
+
+// Group keys of RPT rows and their validity mask (a null key: bit clear).
+template <int RPT, int S>
+__device__ __forceinline__ uint32_t load_keys(const AggParams& p, int batch, int64_t r0, int lr0, int n,
+                                              int64_t (&key)[RPT]) {
+  uint8_t vx[RPT];
+  if (p.key_col < 0) {
+#pragma unroll
+    for (int k = 0; k < RPT; k++) key[k] = 0;
+    valid_bytes<RPT, S>(nullptr, r0, lr0, n, vx);
+    return valid_mask<RPT, S>(vx, r0, lr0, n);
+  }
+  const void* const* bt = p.bufs + (int64_t)batch * p.bstride + p.key_col * 3;
+  valid_bytes<RPT, S>((const uint8_t*)bt[0], r0, lr0, n, vx);
+  load_words<RPT, S>(bt[2], p.key_type, r0, lr0, n, key);
+  return valid_mask<RPT, S>(vx, r0, lr0, n);
+}
+
 // Slot of a key in an HBM table (insert if new); -1 when the table is full.
 __device__ __forceinline__ int64_t table_slot(const AggInst& I, bool valid, int64_t key) {
   if (!valid) return (int64_t)I.cap;
@@ -206,22 +284,89 @@ struct ArrowRow {
   __device__ __forceinline__ int64_t raw64(int col) const { return ((const int64_t*)abuf(p, batch, col, 2))[r]; }
 };
 
+// Records are stored word-major (word w of record q at rec[w * rstride]); the
+// first two used words are loaded ahead with the key, the others on use.
 struct RecRow {
   const AggParams& p;
-  const int64_t* v;  // record values, by used-column slot
-  uint32_t nulls;    // bit k: used column k is null
+  int64_t v0, v1;      // used words 0 and 1
+  const int64_t* rec;  // &recs[q]
+  uint32_t nulls;      // bit k: used column k is null
+  __device__ __forceinline__ int64_t word(int col) const {
+    const int u = p.uslot[col];
+    return u == 0 ? v0 : u == 1 ? v1 : gld(rec + (int64_t)(2 + u) * p.rstride);
+  }
   __device__ __forceinline__ bool valid(int col) const { return !((nulls >> p.uslot[col]) & 1); }
-  __device__ __forceinline__ int64_t i64(int col) const { return v[p.uslot[col]]; }
+  __device__ __forceinline__ int64_t i64(int col) const { return word(col); }
   __device__ __forceinline__ double f64(int col) const {
-    const int64_t x = v[p.uslot[col]];
+    const int64_t x = word(col);
     const int t = p.col_type[col];
     if (t == GZ_TYPE_DOUBLE || t == GZ_TYPE_FLOAT) return __longlong_as_double(x);
     if (t == GZ_TYPE_UINT64) return (double)(uint64_t)x;
     return (double)x;
   }
-  __device__ __forceinline__ uint64_t u64(int col) const { return (uint64_t)v[p.uslot[col]]; }
-  __device__ __forceinline__ int64_t raw64(int col) const { return v[p.uslot[col]]; }
+  __device__ __forceinline__ uint64_t u64(int col) const { return (uint64_t)word(col); }
+  __device__ __forceinline__ int64_t raw64(int col) const { return word(col); }
 };
+
+// Apply one row to one lane's state at ptr.
+template <class Row>
+__device__ __forceinline__ void apply_lane(const Lane ln, int64_t* ptr, const Row& row, int64_t grow) {
+  switch (ln.op) {
+    case L_CNT: atomicAdd((unsigned long long*)ptr, 1ull); break;
+    case L_NN:
+      if (row.valid(ln.col)) atomicAdd((unsigned long long*)ptr, 1ull);
+      break;
+    case L_NN2:
+      if (row.valid(ln.col) && row.valid(ln.col2)) atomicAdd((unsigned long long*)ptr, 1ull);
+      break;
+    case L_SUMI:
+      if (row.valid(ln.col)) atomicAdd((unsigned long long*)ptr, (unsigned long long)row.i64(ln.col));
+      break;
+    case L_SUMF:
+      if (row.valid(ln.col)) unsafeAtomicAdd((double*)ptr, row.f64(ln.col));
+      break;
+    case L_SUMCNT:
+      if (row.valid(ln.col)) atomicAdd((unsigned long long*)ptr, (unsigned long long)row.raw64(ln.col2));
+      break;
+    case L_MINI:
+    case L_MAXI: {
+      if (!row.valid(ln.col)) break;
+      int64_t v = row.i64(ln.col);
+      if (ln.op == L_MINI) min_i64(ptr, v); else max_i64(ptr, v);
+      break;
+    }
+    case L_MINU:
+    case L_MAXU: {
+      if (!row.valid(ln.col)) break;
+      uint64_t v = row.u64(ln.col);
+      if (ln.op == L_MINU) min_u64(ptr, v); else max_u64(ptr, v);
+      break;
+    }
+    case L_MINF:
+    case L_MAXF: {
+      if (!row.valid(ln.col)) break;
+      double d = row.f64(ln.col);
+      if (d != d) break;
+      if (ln.op == L_MINF) min_u64(ptr, ordered(d)); else max_u64(ptr, ordered(d));
+      break;
+    }
+    case L_HASNAN: {
+      if (!row.valid(ln.col)) break;
+      double d = row.f64(ln.col);
+      if (d != d && *ptr == 0) atomicOr((unsigned long long*)ptr, 1ull);
+      break;
+    }
+    case L_FIRSTNN:
+      if (row.valid(ln.col) && grow < *ptr) atomicMin((long long*)ptr, (long long)grow);
+      break;
+    case L_FIRSTNAN: {
+      if (!row.valid(ln.col)) break;
+      double d = row.f64(ln.col);
+      if (d != d && grow < *ptr) atomicMin((long long*)ptr, (long long)grow);
+      break;
+    }
+  }
+}
 
 // Apply one row to the lanes of `slot` in a table laid out as base + lane * S.
 // Works on HBM and on LDS (the atomics are generic-address).
@@ -230,73 +375,15 @@ __device__ __forceinline__ void apply_row(const AggParams& p, int64_t* base, int
                                           const Row& row, int64_t grow) {
   int64_t* fr = base + S + slot;
   if (grow < *fr) atomicMin((long long*)fr, (long long)grow);
-  for (int l = 0; l < p.nl; l++) {
-    const Lane ln = p.lanes[l];
-    int64_t* ptr = base + (2 + l) * S + slot;
-    switch (ln.op) {
-      case L_CNT: atomicAdd((unsigned long long*)ptr, 1ull); break;
-      case L_NN:
-        if (row.valid(ln.col)) atomicAdd((unsigned long long*)ptr, 1ull);
-        break;
-      case L_NN2:
-        if (row.valid(ln.col) && row.valid(ln.col2)) atomicAdd((unsigned long long*)ptr, 1ull);
-        break;
-      case L_SUMI:
-        if (row.valid(ln.col)) atomicAdd((unsigned long long*)ptr, (unsigned long long)row.i64(ln.col));
-        break;
-      case L_SUMF:
-        if (row.valid(ln.col)) unsafeAtomicAdd((double*)ptr, row.f64(ln.col));
-        break;
-      case L_SUMCNT:
-        if (row.valid(ln.col)) atomicAdd((unsigned long long*)ptr, (unsigned long long)row.raw64(ln.col2));
-        break;
-      case L_MINI:
-      case L_MAXI: {
-        if (!row.valid(ln.col)) break;
-        int64_t v = row.i64(ln.col);
-        if (ln.op == L_MINI) min_i64(ptr, v); else max_i64(ptr, v);
-        break;
-      }
-      case L_MINU:
-      case L_MAXU: {
-        if (!row.valid(ln.col)) break;
-        uint64_t v = row.u64(ln.col);
-        if (ln.op == L_MINU) min_u64(ptr, v); else max_u64(ptr, v);
-        break;
-      }
-      case L_MINF:
-      case L_MAXF: {
-        if (!row.valid(ln.col)) break;
-        double d = row.f64(ln.col);
-        if (d != d) break;
-        if (ln.op == L_MINF) min_u64(ptr, ordered(d)); else max_u64(ptr, ordered(d));
-        break;
-      }
-      case L_HASNAN: {
-        if (!row.valid(ln.col)) break;
-        double d = row.f64(ln.col);
-        if (d != d && *ptr == 0) atomicOr((unsigned long long*)ptr, 1ull);
-        break;
-      }
-      case L_FIRSTNN:
-        if (row.valid(ln.col) && grow < *ptr) atomicMin((long long*)ptr, (long long)grow);
-        break;
-      case L_FIRSTNAN: {
-        if (!row.valid(ln.col)) break;
-        double d = row.f64(ln.col);
-        if (d != d && grow < *ptr) atomicMin((long long*)ptr, (long long)grow);
-        break;
-      }
-    }
-  }
+  for (int l = 0; l < p.nl; l++) apply_lane(p.lanes[l], base + (2 + l) * S + slot, row, grow);
 }
 
-// Merge one LDS slot's lanes into an HBM slot.
+// Merge one LDS slot's lanes into an HBM slot.  The atomics return nothing
+// and nothing waits on them: no HBM round trip per lane.
 __device__ __forceinline__ void merge_slot(const AggParams& p, int64_t* g, int64_t GS, int64_t gslot,
                                            const int64_t* l, int64_t LS, int64_t lslot) {
-  int64_t fr = l[LS + lslot];
-  int64_t* gfr = g + GS + gslot;
-  if (fr < *gfr) atomicMin((long long*)gfr, (long long)fr);
+  const int64_t fr = l[LS + lslot];
+  if (fr != INT64_MAX) atomicMin((long long*)(g + GS + gslot), (long long)fr);
   for (int k = 0; k < p.nl; k++) {
     int64_t v = l[(2 + k) * LS + lslot];
     int64_t* ptr = g + (2 + k) * GS + gslot;
@@ -309,15 +396,80 @@ __device__ __forceinline__ void merge_slot(const AggParams& p, int64_t* g, int64
         if (d != 0.0) unsafeAtomicAdd((double*)ptr, d);
         break;
       }
-      case L_MINI: if (v != INT64_MAX) min_i64(ptr, v); break;
-      case L_MAXI: if (v != INT64_MIN) max_i64(ptr, v); break;
-      case L_MINU: case L_MINF: if (v != -1) min_u64(ptr, (uint64_t)v); break;
-      case L_MAXU: case L_MAXF: if (v != 0) max_u64(ptr, (uint64_t)v); break;
-      case L_HASNAN: if (v && *ptr == 0) atomicOr((unsigned long long*)ptr, 1ull); break;
+      case L_MINI: if (v != INT64_MAX) atomicMin((long long*)ptr, (long long)v); break;
+      case L_MAXI: if (v != INT64_MIN) atomicMax((long long*)ptr, (long long)v); break;
+      case L_MINU: case L_MINF: if (v != -1) atomicMin((unsigned long long*)ptr, (unsigned long long)v); break;
+      case L_MAXU: case L_MAXF: if (v != 0) atomicMax((unsigned long long*)ptr, (unsigned long long)v); break;
+      case L_HASNAN: if (v) atomicOr((unsigned long long*)ptr, 1ull); break;
       case L_FIRSTNN: case L_FIRSTNAN:
-        if (v < *ptr) atomicMin((long long*)ptr, (long long)v);
+        if (v != INT64_MAX) atomicMin((long long*)ptr, (long long)v);
         break;
     }
+  }
+}
+
+// Branch-free application of RPT records to the lanes of an LDS table: one
+// switch per lane per call, and every record updates every lane (a record
+// without a slot of its own updates the trash slot; a null input contributes
+// the lane's neutral value), so no per-record branches or exec-mask changes.
+// Results equal apply_row's: adding 0 or 0.0 and min/max/or with the neutral
+// value change nothing (a sum lane is never -0.0: it starts at +0.0).
+// Records of at most two used columns (every operand in registers, no memory
+// access but the LDS atomics).  w1: record word 1 (first row | nulls << 48);
+// u0/u1: used words 0 and 1.
+template <int RPT>
+__device__ __forceinline__ void lds_apply_records(const AggParams& p, int64_t* tab, int64_t LS, const int (&slot)[RPT],
+                                                  const int64_t (&w1)[RPT], const int64_t (&u0)[RPT],
+                                                  const int64_t (&u1)[RPT]) {
+  int64_t grow[RPT];
+  uint32_t nulls[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    grow[k] = (int64_t)((uint64_t)w1[k] & ((1ull << 48) - 1));
+    nulls[k] = (uint32_t)((uint64_t)w1[k] >> 48);
+    atomicMin((long long*)(tab + LS + slot[k]), (long long)grow[k]);
+  }
+  for (int l = 0; l < p.nl; l++) {
+    const Lane ln = p.lanes[l];
+    int64_t* lp = tab + (2 + l) * LS;
+    const int u = ln.col >= 0 ? p.uslot[ln.col] : 0;
+    const int u2 = ln.col2 >= 0 ? p.uslot[ln.col2] : 0;
+    const int ty = ln.col >= 0 ? p.col_type[ln.col] : 0;
+    auto word = [&](int uu, int k) -> int64_t { return uu == 0 ? u0[k] : u1[k]; };
+    auto ok = [&](int uu, int k) -> bool { return !((nulls[k] >> uu) & 1); };
+    // the input as double, converted for all RPT records before any use (a
+    // uniform branch on the type, no per-record branch)
+    double dv[RPT];
+    if (ty == GZ_TYPE_DOUBLE || ty == GZ_TYPE_FLOAT) {
+#pragma unroll
+      for (int k = 0; k < RPT; k++) dv[k] = __longlong_as_double(word(u, k));
+    } else if (ty == GZ_TYPE_UINT64) {
+#pragma unroll
+      for (int k = 0; k < RPT; k++) dv[k] = (double)(uint64_t)word(u, k);
+    } else {
+#pragma unroll
+      for (int k = 0; k < RPT; k++) dv[k] = (double)word(u, k);
+    }
+    auto f64 = [&](int k) -> double { return dv[k]; };
+#define GZ_EACH(STMT) _Pragma("unroll") for (int k = 0; k < RPT; k++) { unsigned long long* up = (unsigned long long*)(lp + slot[k]); long long* sp = (long long*)(lp + slot[k]); (void)up; (void)sp; STMT; }
+    switch (ln.op) {
+      case L_CNT: GZ_EACH(atomicAdd(up, 1ull)) break;
+      case L_NN: GZ_EACH(atomicAdd(up, ok(u, k) ? 1ull : 0ull)) break;
+      case L_NN2: GZ_EACH(atomicAdd(up, ok(u, k) && ok(u2, k) ? 1ull : 0ull)) break;
+      case L_SUMI: GZ_EACH(atomicAdd(up, ok(u, k) ? (unsigned long long)word(u, k) : 0ull)) break;
+      case L_SUMF: GZ_EACH(unsafeAtomicAdd((double*)up, ok(u, k) ? f64(k) : 0.0)) break;
+      case L_SUMCNT: GZ_EACH(atomicAdd(up, ok(u, k) ? (unsigned long long)word(u2, k) : 0ull)) break;
+      case L_MINI: GZ_EACH(atomicMin(sp, ok(u, k) ? (long long)word(u, k) : INT64_MAX)) break;
+      case L_MAXI: GZ_EACH(atomicMax(sp, ok(u, k) ? (long long)word(u, k) : INT64_MIN)) break;
+      case L_MINU: GZ_EACH(atomicMin(up, ok(u, k) ? (unsigned long long)word(u, k) : ~0ull)) break;
+      case L_MAXU: GZ_EACH(atomicMax(up, ok(u, k) ? (unsigned long long)word(u, k) : 0ull)) break;
+      case L_MINF: GZ_EACH(const double d = f64(k); atomicMin(up, ok(u, k) && d == d ? ordered(d) : ~0ull)) break;
+      case L_MAXF: GZ_EACH(const double d = f64(k); atomicMax(up, ok(u, k) && d == d ? ordered(d) : 0ull)) break;
+      case L_HASNAN: GZ_EACH(const double d = f64(k); atomicOr(up, ok(u, k) && d != d ? 1ull : 0ull)) break;
+      case L_FIRSTNN: GZ_EACH(atomicMin(sp, ok(u, k) ? (long long)grow[k] : INT64_MAX)) break;
+      case L_FIRSTNAN: GZ_EACH(const double d = f64(k); atomicMin(sp, ok(u, k) && d != d ? (long long)grow[k] : INT64_MAX)) break;
+    }
+#undef GZ_EACH
   }
 }
 
@@ -393,7 +545,7 @@ __global__ __launch_bounds__(256) void agg_update_kernel(AggParams p) {
 // the instance changes and at the end.
 __device__ void lds_merge(const AggParams& p, const AggInst& I, int64_t* l, int64_t LS) {
   const int64_t GS = (int64_t)I.cap + 2;
-  for (int64_t s = threadIdx.x; s < LS; s += blockDim.x) {
+  for (int64_t s = threadIdx.x; s < (int64_t)p.lcap + 2; s += blockDim.x) {
     bool special = s >= p.lcap;
     int64_t key = l[s];
     if (!special && key == kEmpty) continue;
@@ -416,7 +568,7 @@ __device__ void lds_reset(const AggParams& p, int64_t* l, int64_t LS) {
 
 __global__ __launch_bounds__(256) void agg_update_lds_kernel(AggParams p) {
   extern __shared__ int64_t s_tab[];
-  const int64_t LS = p.lcap + 2;
+  const int64_t LS = p.lcap + 3;  // stride: lcap slots, 2 special, 1 trash
   const uint64_t lmask = (uint64_t)p.lcap - 1;
   int64_t t0 = (int64_t)blockIdx.x * p.tiles_per_block;
   int64_t t1 = min(p.T, t0 + p.tiles_per_block);
@@ -498,19 +650,14 @@ __global__ __launch_bounds__(256) void agg_part_hist_kernel(AggParams p, uint32_
   if (t < 0) return;
   const AggTile td = p.tiles[t];
   const AggInst I = p.inst[td.inst];
-  for (int b = threadIdx.x; b < R; b += blockDim.x) s_h[b] = 0;
-  __syncthreads();
   constexpr int RPT = kPartTile / 256;
   int64_t key[RPT];
-  bool valid[RPT];
-#pragma unroll
-  for (int k = 0; k < RPT; k++) {
-    const int lr = k * 256 + threadIdx.x;
-    valid[k] = lr < td.nrows && row_key(p, td.batch, (int64_t)td.row_begin + lr, &key[k]);
-  }
+  const uint32_t vm = load_keys<RPT, 256>(p, td.batch, td.row_begin, threadIdx.x, td.nrows, key);
+  for (int b = threadIdx.x; b < R; b += blockDim.x) s_h[b] = 0;
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < RPT; k++)
-    if (k * 256 + threadIdx.x < td.nrows) atomicAdd(&s_h[part_of(I, valid[k], key[k])], 1u);
+    if (k * 256 + (int)threadIdx.x < td.nrows) atomicAdd(&s_h[part_of(I, (vm >> k) & 1, key[k])], 1u);
   __syncthreads();
   for (int b = threadIdx.x; b < R; b += blockDim.x) gst(tcount + t * R + b, s_h[b]);
 }
@@ -544,14 +691,17 @@ __global__ __launch_bounds__(256) void agg_part_scan_kernel(const unsigned int* 
 }
 
 // Records of a tile: partition counts in LDS, one range reserved per non-empty
-// partition of the tile, the tile's records laid out in LDS in partition order,
-// then written out as one contiguous run per partition (consecutive threads,
-// consecutive records).  Tiles of this kernel are at most kPartTile rows.
+// partition of the tile, the tile's records laid out in LDS in partition order
+// (word-major), then written out as one contiguous run per partition and word
+// (consecutive threads, consecutive records).  Tiles of this kernel are at most
+// kPartTile rows.  The key and the first kPreU used columns are loaded before
+// the ranking; the other used columns after it.
+constexpr int kPreU = 2;
 
 __global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, const uint32_t* tprefix, int R,
                                                                const int64_t* pstart, const int* it0,
                                                                int64_t* recs) {
-  extern __shared__ int64_t s_rec[];  // [kPartTile][W]
+  extern __shared__ int64_t s_rec[];  // [W][kPartTile]
   __shared__ unsigned int s_h[kMaxFanout];
   __shared__ unsigned int s_off[kMaxFanout + 1];
   __shared__ unsigned long long s_base[kMaxFanout];
@@ -564,86 +714,128 @@ __global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, cons
   const int F = 1 << I.fan_log2;
   const int W = 2 + p.nu;
   constexpr int RPT = kPartTile / 256;
-  for (int b = threadIdx.x; b < F; b += blockDim.x) s_h[b] = 0;
+  const int tid = threadIdx.x, n = td.nrows;
+  const int64_t rb = td.row_begin;
+  const void* const* bt = p.bufs + (int64_t)td.batch * p.bstride;
+  // ---- loads that do not depend on the partition
+  int64_t key[RPT];
+  const uint32_t kv = load_keys<RPT, 256>(p, td.batch, rb, tid, n, key);
+  int64_t pre[kPreU][RPT];
+  uint32_t pvalid[kPreU];
+#pragma unroll
+  for (int u = 0; u < kPreU; u++) {
+    pvalid[u] = 0;
+#pragma unroll
+    for (int k = 0; k < RPT; k++) pre[u][k] = 0;
+    if (u < p.nu) {
+      const int c = p.ucol[u];
+      uint8_t vx[RPT];
+      valid_bytes<RPT, 256>((const uint8_t*)bt[c * 3], rb, tid, n, vx);
+      load_words<RPT, 256>(bt[c * 3 + 2], p.col_type[c], rb, tid, n, pre[u]);
+      pvalid[u] = valid_mask<RPT, 256>(vx, rb, tid, n);
+    }
+  }
+  for (int b = tid; b < F; b += blockDim.x) s_h[b] = 0;
   __syncthreads();
   // partition and in-partition rank of the owned rows (rank order is free: the
   // record carries the row index)
   int part[RPT], rank[RPT];
 #pragma unroll
   for (int k = 0; k < RPT; k++) {
-    const int lr = k * 256 + threadIdx.x;
     part[k] = 0;
     rank[k] = 0;
-    if (lr < td.nrows) {
-      int64_t key;
-      bool valid = row_key(p, td.batch, (int64_t)td.row_begin + lr, &key);
-      part[k] = part_of(I, valid, key);
+    if (k * 256 + tid < n) {
+      part[k] = part_of(I, (kv >> k) & 1, key[k]);
       rank[k] = (int)atomicAdd(&s_h[part[k]], 1u);
     }
   }
   __syncthreads();
   {
     // tile offsets of the partitions (F <= 256: one per thread)
-    const uint64_t c = threadIdx.x < F ? s_h[threadIdx.x] : 0;
+    const uint64_t c = tid < F ? s_h[tid] : 0;
     uint64_t total;
     const uint64_t ex = block_exclusive_scan<uint64_t>(c, s_scan, &total);
-    if (threadIdx.x < F) {
-      s_off[threadIdx.x] = (unsigned)ex;
-      const int b = threadIdx.x;
-      s_base[b] = (unsigned long long)pstart[I.bucket_base + b] + tprefix[t * R + b] -
-                  tprefix[(int64_t)it0[td.inst] * R + b];
+    if (tid < F) {
+      s_off[tid] = (unsigned)ex;
+      s_base[tid] = (unsigned long long)pstart[I.bucket_base + tid] + tprefix[t * R + tid] -
+                    tprefix[(int64_t)it0[td.inst] * R + tid];
     }
-    if (threadIdx.x == 0) s_off[F] = (unsigned)total;
+    if (tid == 0) s_off[F] = (unsigned)total;
   }
   __syncthreads();
+  int slot[RPT];
+  uint32_t nulls[RPT];
 #pragma unroll
   for (int k = 0; k < RPT; k++) {
-    const int lr = k * 256 + threadIdx.x;
-    if (lr >= td.nrows) continue;
-    const int64_t r = (int64_t)td.row_begin + lr;
-    const int slot = (int)s_off[part[k]] + rank[k];
-    int64_t* rec = s_rec + (int64_t)slot * W;
-    int64_t key;
-    bool valid = row_key(p, td.batch, r, &key);
-    uint64_t nulls = valid ? 0ull : (1ull << kRecKeyNull);
-    for (int u = 0; u < p.nu; u++) {
-      const int c = p.ucol[u];
-      int64_t v = 0;
-      if (!is_valid(p, td.batch, c, r)) {
-        nulls |= 1ull << u;
-      } else {
-        const int ty = p.col_type[c];
-        const void* vb = abuf(p, td.batch, c, 2);
-        if (ty == GZ_TYPE_DOUBLE || ty == GZ_TYPE_FLOAT) v = __double_as_longlong(read_f64(vb, ty, r));
-        else if (ty == GZ_TYPE_UINT64) v = ((const int64_t*)vb)[r];
-        else v = read_i64(vb, ty, r);
-      }
-      rec[2 + u] = v;
+    slot[k] = (int)s_off[part[k]] + rank[k];
+    const bool kval = (kv >> k) & 1;
+    nulls[k] = kval ? 0u : (1u << kRecKeyNull);
+#pragma unroll
+    for (int u = 0; u < kPreU; u++)
+      if (u < p.nu && !((pvalid[u] >> k) & 1)) nulls[k] |= 1u << u;
+    if (k * 256 + tid < n) {
+      s_rec[slot[k]] = kval ? key[k] : 0;
+#pragma unroll
+      for (int u = 0; u < kPreU; u++)
+        if (u < p.nu) s_rec[(2 + u) * kPartTile + slot[k]] = ((pvalid[u] >> k) & 1) ? pre[u][k] : 0;
+      s_pt[slot[k]] = (unsigned char)part[k];
     }
-    rec[0] = valid ? key : 0;
-    rec[1] = (int64_t)((uint64_t)(td.row_base + lr) | (nulls << 48));
-    s_pt[slot] = (unsigned char)part[k];
   }
+  for (int u = kPreU; u < p.nu; u++) {
+    const int c = p.ucol[u];
+    uint8_t vx[RPT];
+    int64_t v[RPT];
+    valid_bytes<RPT, 256>((const uint8_t*)bt[c * 3], rb, tid, n, vx);
+    load_words<RPT, 256>(bt[c * 3 + 2], p.col_type[c], rb, tid, n, v);
+    const uint32_t vm = valid_mask<RPT, 256>(vx, rb, tid, n);
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+      if (!((vm >> k) & 1)) nulls[k] |= 1u << u;
+      if (k * 256 + tid < n) s_rec[(2 + u) * kPartTile + slot[k]] = ((vm >> k) & 1) ? v[k] : 0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < RPT; k++)
+    if (k * 256 + tid < n)
+      s_rec[kPartTile + slot[k]] = (int64_t)((uint64_t)(td.row_base + k * 256 + tid) | ((uint64_t)nulls[k] << 48));
   __syncthreads();
-  // one 8-byte word per thread per step over the tile's records in partition order
-  const int n = td.nrows;
-  for (int x = threadIdx.x; x < n * W; x += blockDim.x) {
-    const int slot = x / W, w = x - slot * W;
-    const int b = s_pt[slot];
-    const unsigned long long pos = s_base[b] + (unsigned long long)(slot - (int)s_off[b]);
-    recs[(int64_t)pos * W + w] = s_rec[x];
+  // word-major runs: thread tid writes sorted slots tid + k * 256 of every word
+  int64_t pos[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    const int sl = k * 256 + tid;
+    pos[k] = 0;
+    if (sl < n) {
+      const int b = s_pt[sl];
+      pos[k] = (int64_t)(s_base[b] + (unsigned long long)(sl - (int)s_off[b]));
+    }
+  }
+  for (int w = 0; w < W; w++) {
+    int64_t* dst = recs + (int64_t)w * p.rstride;
+#pragma unroll
+    for (int k = 0; k < RPT; k++)
+      if (k * 256 + tid < n) gst(dst + pos[k], s_rec[w * kPartTile + k * 256 + tid]);
   }
 }
+
+#ifndef GZ_UPD_RPT
+#define GZ_UPD_RPT 4
+#endif
+#ifndef GZ_UPD_BLOCK
+#define GZ_UPD_BLOCK 512
+#endif
+constexpr int kUpdRPT = GZ_UPD_RPT;      // records per thread per step of the update kernel
+constexpr int kUpdBlock = GZ_UPD_BLOCK;  // its workgroup size (two per CU)
 
 // Workgroups walk fixed chunks of the record array; inside a chunk, each
 // partition segment is aggregated in an LDS table and merged into the HBM table
 // of the segment's instance.
-__global__ __launch_bounds__(1024) void agg_part_update_kernel(AggParams p, const int64_t* recs, const int64_t* pstart,
+__global__ __launch_bounds__(kUpdBlock, 2 * kUpdBlock / 256) void agg_part_update_kernel(AggParams p, const int64_t* recs, const int64_t* pstart,
                                                               const int* pinst, int B, int64_t chunk) {
   extern __shared__ int64_t s_tab[];
-  const int64_t LS = p.lcap + 2;
+  const int64_t LS = p.lcap + 3;  // stride: lcap slots, 2 special, 1 trash
   const uint64_t lmask = (uint64_t)p.lcap - 1;
-  const int W = 2 + p.nu;
+  const int64_t NR = p.rstride;
   const int64_t total = pstart[B];
   for (int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < total; c0 += (int64_t)gridDim.x * chunk) {
     const int64_t c1 = min(total, c0 + chunk);
@@ -661,35 +853,65 @@ __global__ __launch_bounds__(1024) void agg_part_update_kernel(AggParams p, cons
       __syncthreads();
       lds_reset(p, s_tab, LS);
       __syncthreads();
-      for (int64_t q = a + threadIdx.x; q < e; q += blockDim.x) {
-        const int64_t* rec = recs + q * W;
-        const uint64_t w1 = (uint64_t)rec[1];
-        const int64_t grow = (int64_t)(w1 & ((1ull << 48) - 1));
-        const uint32_t nulls = (uint32_t)(w1 >> 48);
-        const bool valid = !((nulls >> kRecKeyNull) & 1);
-        const int64_t key = rec[0];
-        int64_t ls = -1;
-        if (!valid) ls = p.lcap;
-        else if (key == kEmpty) ls = p.lcap + 1;
-        else {
-          uint64_t s = fmix64((uint64_t)key) & lmask;
-          for (int i = 0; i < 32; i++) {  // short probe: a full LDS table spills to HBM
-            int64_t k = s_tab[s];
-            if (k == key) { ls = (int64_t)s; break; }
-            if (k == kEmpty) {
-              unsigned long long prev = atomicCAS((unsigned long long*)&s_tab[s], (unsigned long long)kEmpty,
-                                                  (unsigned long long)key);
-              if ((int64_t)prev == kEmpty || (int64_t)prev == key) { ls = (int64_t)s; break; }
+      // kUpdRPT records per thread per step, their key, row and first two
+      // used words loaded together
+      for (int64_t q0 = a + threadIdx.x; q0 < e; q0 += (int64_t)blockDim.x * kUpdRPT) {
+        int64_t kw[kUpdRPT], w1w[kUpdRPT], u0[kUpdRPT], u1[kUpdRPT];
+#pragma unroll
+        for (int k = 0; k < kUpdRPT; k++) {
+          const int64_t q = q0 + (int64_t)k * blockDim.x;
+          const bool act = q < e;
+          kw[k] = act ? gld(recs + q) : 0;
+          w1w[k] = act ? gld(recs + NR + q) : 0;
+          u0[k] = act && p.nu > 0 ? gld(recs + 2 * NR + q) : 0;
+          u1[k] = act && p.nu > 1 ? gld(recs + 3 * NR + q) : 0;
+        }
+        // LDS slot of every record (-1: none, -2: the LDS probe failed)
+        int ls[kUpdRPT];
+#pragma unroll
+        for (int k = 0; k < kUpdRPT; k++) {
+          const int64_t q = q0 + (int64_t)k * blockDim.x;
+          ls[k] = -1;
+          if (q >= e) continue;
+          const uint32_t nulls = (uint32_t)((uint64_t)w1w[k] >> 48);
+          const int64_t key = kw[k];
+          if ((nulls >> kRecKeyNull) & 1) ls[k] = p.lcap;
+          else if (key == kEmpty) ls[k] = p.lcap + 1;
+          else {
+            ls[k] = -2;
+            uint64_t s = fmix64((uint64_t)key) & lmask;
+            for (int i = 0; i < 32; i++) {  // short probe: a full LDS table spills to HBM
+              int64_t kk = s_tab[s];
+              if (kk == key) { ls[k] = (int)s; break; }
+              if (kk == kEmpty) {
+                unsigned long long prev = atomicCAS((unsigned long long*)&s_tab[s], (unsigned long long)kEmpty,
+                                                    (unsigned long long)key);
+                if ((int64_t)prev == kEmpty || (int64_t)prev == key) { ls[k] = (int)s; break; }
+              }
+              s = (s + 1) & lmask;
             }
-            s = (s + 1) & lmask;
           }
         }
-        const RecRow row{p, rec + 2, nulls};
-        if (ls >= 0) {
-          apply_row(p, s_tab, LS, ls, row, grow);
+        auto grow_of = [&](int k) { return (int64_t)((uint64_t)w1w[k] & ((1ull << 48) - 1)); };
+        auto row_of = [&](int k) {
+          return RecRow{p, u0[k], u1[k], recs + q0 + (int64_t)k * blockDim.x, (uint32_t)((uint64_t)w1w[k] >> 48)};
+        };
+        if (p.nu <= 2) {
+          int slot[kUpdRPT];
+#pragma unroll
+          for (int k = 0; k < kUpdRPT; k++) slot[k] = ls[k] >= 0 ? ls[k] : p.lcap + 2;  // trash slot
+          lds_apply_records<kUpdRPT>(p, s_tab, LS, slot, w1w, u0, u1);
         } else {
-          int64_t g = table_slot(I, valid, key);
-          if (g >= 0) apply_row(p, I.base, (int64_t)I.cap + 2, g, row, grow);
+#pragma unroll
+          for (int k = 0; k < kUpdRPT; k++)
+            if (ls[k] >= 0) apply_row(p, s_tab, LS, ls[k], row_of(k), grow_of(k));
+        }
+        // records whose LDS probe failed go to the instance's HBM table
+#pragma unroll
+        for (int k = 0; k < kUpdRPT; k++) {
+          if (ls[k] != -2) continue;
+          const int64_t g = table_slot(I, true, kw[k]);
+          if (g >= 0) apply_row(p, I.base, (int64_t)I.cap + 2, g, row_of(k), grow_of(k));
         }
       }
       __syncthreads();
@@ -718,14 +940,19 @@ __global__ __launch_bounds__(256) void agg_hll_kernel(AggParams p, unsigned int*
       __syncthreads();
       cur = td.inst;
     }
-    for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
-      int64_t key;
-      if (!row_key(p, td.batch, (int64_t)td.row_begin + lr, &key)) continue;
-      uint64_t h = fmix64((uint64_t)key ^ 0x9e3779b97f4a7c15ull);
-      unsigned idx = (unsigned)(h >> (64 - kHllBits));
-      uint64_t w = (h << kHllBits) | (1ull << (kHllBits - 1));
-      unsigned rank = (unsigned)__clzll((long long)w) + 1;
-      if (rank > s_reg[idx]) atomicMax(&s_reg[idx], rank);
+    constexpr int RPT = 8;  // rows per thread per step, loaded together
+    for (int base = 0; base < td.nrows; base += 256 * RPT) {
+      int64_t key[RPT];
+      const uint32_t vm = load_keys<RPT, 256>(p, td.batch, td.row_begin, base + (int)threadIdx.x, td.nrows, key);
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        if (!((vm >> k) & 1)) continue;
+        uint64_t h = fmix64((uint64_t)key[k] ^ 0x9e3779b97f4a7c15ull);
+        unsigned idx = (unsigned)(h >> (64 - kHllBits));
+        uint64_t w = (h << kHllBits) | (1ull << (kHllBits - 1));
+        unsigned rank = (unsigned)__clzll((long long)w) + 1;
+        if (rank > s_reg[idx]) atomicMax(&s_reg[idx], rank);
+      }
     }
   }
   __syncthreads();
@@ -1166,8 +1393,9 @@ static AggParams base_params(gz_agg* h) {
   return p;
 }
 
+// LDS table: lcap slots, the null-key and kEmpty-key slots, and a trash slot.
 static int lds_bytes(const gz_agg* h, int lcap) {
-  return (int)((lcap + 2) * (2 + h->plan.lanes.size()) * 8);
+  return (int)((lcap + 3) * (2 + h->plan.lanes.size()) * 8);
 }
 
 // (Re)allocate an instance's table with capacity cap (copying existing groups).
@@ -1707,6 +1935,7 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     GZ_TRY(ctx_ensure(ctx, &h->recs, sizeof(int64_t) * (2 + p.nu) * h->nrec));
     AggParams& q = h->p_part;
     q = p;
+    q.rstride = h->nrec;
     q.tiles = (const AggTile*)h->d_part.p;
     q.T = (int64_t)part_tiles.size();
     h->grid_part = (unsigned)((q.T + 7) / 8 * 8);
@@ -1772,7 +2001,7 @@ static int agg_run_kernels(gz_agg* h) {
                        (uint32_t*)h->tcount.p, h->R);
     hipLaunchKernelGGL(tile_scan_sum_kernel<kPartScanChunk>, dim3((unsigned)C), dim3(256), 0, st,
                        (const uint32_t*)h->tcount.p, h->R, TP, (uint64_t*)h->tcsum.p);
-    hipLaunchKernelGGL(tile_scan_chunks_kernel, dim3((unsigned)((h->R + 255) / 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(tile_scan_chunks_kernel, dim3((unsigned)h->R), dim3(256), 0, st,
                        (uint64_t*)h->tcsum.p, h->R, C, (uint64_t*)h->ttot.p);
     hipLaunchKernelGGL(tile_scan_write_kernel<kPartScanChunk>, dim3((unsigned)C), dim3(256), 0, st,
                        (const uint32_t*)h->tcount.p, (const uint64_t*)h->tcsum.p, h->R, TP,
@@ -1787,7 +2016,7 @@ static int agg_run_kernels(gz_agg* h) {
                        sizeof(int64_t) * kPartTile * (2 + h->p_part.nu), st, h->p_part,
                        (const uint32_t*)h->tprefix.p, h->R, (const int64_t*)h->pstart.p, (const int*)h->it0.p,
                        (int64_t*)h->recs.p);
-    hipLaunchKernelGGL(agg_part_update_kernel, dim3(h->blocks_part_upd), dim3(1024), h->lds_part, st, h->p_part,
+    hipLaunchKernelGGL(agg_part_update_kernel, dim3(h->blocks_part_upd), dim3(kUpdBlock), h->lds_part, st, h->p_part,
                        (const int64_t*)h->recs.p, (const int64_t*)h->pstart.p, (const int*)h->pinst.p, h->B,
                        h->part_chunk);
   }

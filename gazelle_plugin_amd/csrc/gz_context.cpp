@@ -1,4 +1,5 @@
 // Context, error reporting, the HBM pool and kernel timing of the C ABI.
+#include <algorithm>
 #include <cstring>
 
 #include "gz_internal.h"
@@ -117,10 +118,13 @@ int upload(gz_context* ctx, DBuf* dst, const void* src, size_t bytes) {
     GZ_HIP(hipEventSynchronize(s.done));
     s.pending = false;
   }
-  if (s.cap < bytes) {
+  if (s.cap < bytes || s.cap < ctx->staging_hwm) {
     if (s.host) hipHostFree(s.host);
     s.host = nullptr;
-    size_t want = bytes < 65536 ? 65536 : bytes + bytes / 2;
+    // pinned allocations cost ~100 us each: size slots generously and to the
+    // largest size any slot has needed, so the ring stops reallocating
+    size_t want = std::max<size_t>({bytes + bytes / 2, (size_t)1 << 20, ctx->staging_hwm});
+    ctx->staging_hwm = want;
     GZ_HIP(hipHostMalloc(&s.host, want, hipHostMallocDefault));
     s.cap = want;
   }

@@ -114,6 +114,7 @@ struct gz_context {
   };
   Staging staging[8];
   int staging_next = 0;
+  size_t staging_hwm = 0;  // largest slot size so far: every slot grows to it at once
   std::mutex staging_mu;
 };
 

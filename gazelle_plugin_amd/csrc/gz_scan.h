@@ -30,24 +30,36 @@ __global__ __launch_bounds__(256) void tile_scan_sum_kernel(const uint32_t* coun
   }
 }
 
-// One thread per column: exclusive scan of the chunk sums in place; the column
-// total goes to tot[e].
+// One block per column: exclusive scan of the column's chunk sums in place
+// (thread i owns a contiguous run of chunks, block scan of the run sums); the
+// column total goes to tot[e].  Grid = R.
 __global__ __launch_bounds__(256) inline void tile_scan_chunks_kernel(uint64_t* csum, int R, int64_t C,
                                                                       uint64_t* tot) {
-  int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= R) return;
-  uint64_t run = 0;
-  for (int64_t cb = 0; cb < C; cb += 16) {
+  __shared__ uint64_t s_scan[256 / kWave + 1];
+  const int e = blockIdx.x;
+  const int64_t per = (C + 255) / 256;
+  const int64_t c0 = min<int64_t>(C, (int64_t)threadIdx.x * per), c1 = min<int64_t>(C, c0 + per);
+  uint64_t s = 0;
+  for (int64_t cb = c0; cb < c1; cb += 16) {
     uint64_t v[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) v[k] = cb + k < C ? csum[(cb + k) * R + e] : 0ull;
+    for (int k = 0; k < 16; k++) v[k] = cb + k < c1 ? csum[(cb + k) * R + e] : 0ull;
+#pragma unroll
+    for (int k = 0; k < 16; k++) s += v[k];
+  }
+  uint64_t total;
+  uint64_t run = block_exclusive_scan<uint64_t>(s, s_scan, &total);
+  for (int64_t cb = c0; cb < c1; cb += 16) {
+    uint64_t v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = cb + k < c1 ? csum[(cb + k) * R + e] : 0ull;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-      if (cb + k < C) csum[(cb + k) * R + e] = run;
+      if (cb + k < c1) csum[(cb + k) * R + e] = run;
       run += v[k];
     }
   }
-  tot[e] = run;
+  if (threadIdx.x == 0) tot[e] = total;
 }
 
 template <int CH>

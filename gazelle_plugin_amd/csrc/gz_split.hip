@@ -1145,7 +1145,7 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     if (T > 0) {
       hipLaunchKernelGGL(tile_scan_sum_kernel<kScanChunk>, dim3((unsigned)C), dim3(256), 0, st,
                          (const uint32_t*)seg->counts.p, R, T, (uint64_t*)seg->csum.p);
-      hipLaunchKernelGGL(tile_scan_chunks_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st,
+      hipLaunchKernelGGL(tile_scan_chunks_kernel, dim3((unsigned)R), dim3(256), 0, st,
                          (uint64_t*)seg->csum.p, R, C, (uint64_t*)seg->tot.p);
       hipLaunchKernelGGL(tile_scan_write_kernel<kScanChunk>, dim3((unsigned)C), dim3(256), 0, st,
                          (const uint32_t*)seg->counts.p, (const uint64_t*)seg->csum.p, R, T,
