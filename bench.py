@@ -412,7 +412,11 @@ def main():
     from gazelle_plugin_amd.context import Context
 
     torch.cuda.init()
-    stream = torch.cuda.current_stream()
+    # one stream (default priority) for torch and the library: the timing events
+    # are recorded on the stream every timed kernel runs on.  (The null stream's
+    # handle is 0, which the library reads as "use the context's own stream".)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     ctx = Context(0, stream=stream.cuda_stream)
     sections = set(args.sections.split(","))
     split = bench_split(args, ctx, stream)

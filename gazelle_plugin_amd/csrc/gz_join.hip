@@ -34,6 +34,10 @@
 namespace gz {
 
 constexpr int kJoinMaxCols = 32;
+#ifndef GZ_WRITE01_BLOCK
+#define GZ_WRITE01_BLOCK 1024
+#endif
+constexpr int kWrite01Block = GZ_WRITE01_BLOCK;  // threads of the at-most-one-row write pass (4096-row tiles)
 constexpr int64_t kJEmpty = INT64_MIN;
 
 __device__ __forceinline__ uint64_t jmix64(uint64_t k) {
@@ -72,26 +76,16 @@ __device__ __forceinline__ int64_t join_key(const void* v, int t, int64_t r, boo
   }
 }
 
+// The probe reads only the 8-byte keys (the smallest table for the random
+// accesses); starts and counts are read for matched rows that need them.
 struct JTable {
-  int64_t* keys;    // [cap]
-  int32_t* start;   // [cap] first row (unique keys) or rowlist start
-  int32_t* count;   // [cap]
-  int32_t* rowlist; // rows grouped by key in insertion order, or null (unique keys)
+  const int64_t* keys;     // [cap]
+  const int32_t* start;    // [cap] first row (unique keys) or rowlist start
+  const int32_t* count;    // [cap]
+  const int32_t* rowlist;  // rows grouped by key in insertion order, or null (unique keys)
   uint64_t cap;
   int64_t special_start, special_count;  // the key equal to kJEmpty
 };
-
-__device__ __forceinline__ int64_t jt_find(const JTable& J, int64_t key) {
-  uint64_t mask = J.cap - 1;
-  uint64_t s = jmix64((uint64_t)key) & mask;
-  for (uint64_t i = 0; i < J.cap; i++) {
-    int64_t k = J.keys[s];
-    if (k == key) return (int64_t)s;
-    if (k == kJEmpty) return -1;
-    s = (s + 1) & mask;
-  }
-  return -1;
-}
 
 // ---------------------------------------------------------------- build store
 // One appended batch column: bits -> bytes for validity and booleans, values
@@ -233,6 +227,41 @@ __global__ void join_rowlist_kernel(const unsigned long long* sorted, int64_t n,
     rowlist[i] = (int32_t)(sorted[i] & 0xffffffffull);
 }
 
+// Row-major copy of the fixed-width build columns a probe outputs: per build row
+// one 16-byte record, the values at their offsets and a validity bit per
+// column in the last dword, so a matched row costs one random 16-byte load.
+struct PayloadParams {
+  int ncol;
+  const uint8_t* bvalid[kJoinMaxCols];   // store validity bytes
+  const uint8_t* bvalues[kJoinMaxCols];  // store values (width bytes per row)
+  int width[kJoinMaxCols];
+  int off[kJoinMaxCols];                 // byte offset in the record
+  int bit[kJoinMaxCols];                 // validity bit: the output column index
+  int64_t n;
+  u32x4* out;
+};
+
+__global__ void join_payload_kernel(PayloadParams q) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < q.n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t lo = 0, hi = 0;
+    uint32_t vm = 0;
+    for (int c = 0; c < q.ncol; c++) {
+      const int w = q.width[c], off = q.off[c];
+      uint64_t v;
+      const uint8_t* src = q.bvalues[c] + r * w;
+      if (w == 8) v = gld((const uint64_t*)src);
+      else if (w == 4) v = gld((const uint32_t*)src);
+      else if (w == 2) v = gld((const uint16_t*)src);
+      else v = gld(src);
+      if (off < 8) lo |= v << (8 * off);
+      else hi |= v << (8 * (off - 8));
+      if (gld(q.bvalid[c] + r)) vm |= 1u << q.bit[c];
+    }
+    gst(q.out + r, u32x4{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, vm});
+  }
+}
+
 // ---------------------------------------------------------------- probe
 struct PTile {
   int32_t batch;
@@ -264,7 +293,9 @@ struct ProbeParams {
   int build_has_null;
   int64_t build_keys;       // keys in the table (null keys excluded)
   JTable J;
-  int32_t* match;           // per probe row (tile-major): slot, -1 none, -2 the kJEmpty key
+  // per probe row (tile-major): the matched slot, -1 no match, -2 the kJEmpty
+  // key, -3 a null key
+  int32_t* match;
   uint32_t* tile_count;
   const uint64_t* tile_off;
   int ncols;
@@ -273,19 +304,25 @@ struct ProbeParams {
   int32_t* pair_build;
   int write_pairs;
   int tile_rows;
+  // build columns gathered from the row-major payload (null: per-column gathers)
+  const u32x4* payload;
+  int pay_off[kJoinMaxCols];  // byte offset of output column c in a payload record
 };
 
 __device__ __forceinline__ const void* pbuf(const ProbeParams& p, int batch, int col, int k) {
   return p.bufs[(int64_t)batch * p.bstride + col * 3 + k];
 }
 
-__device__ __forceinline__ int out_count(const ProbeParams& p, int32_t m, bool key_null) {
+// Output rows of a probe row given its match code (see ProbeParams::match) and
+// the matched slot's row count (tables with repeated keys).
+__device__ __forceinline__ int out_count(const ProbeParams& p, int32_t m, int32_t slot_count) {
+  const bool key_null = m == -3;
   if (p.join_type == GZ_JOIN_INNER) {
-    if (m >= 0) return p.J.count[m];
+    if (m >= 0) return p.J.rowlist ? slot_count : 1;
     if (m == -2) return (int)p.J.special_count;
     return 0;
   }
-  bool found = m != -1;
+  bool found = m >= 0 || m == -2;
   if (p.join_type == GZ_JOIN_SEMI) return found ? 1 : 0;
   // anti
   if (!p.null_aware) return found ? 0 : 1;
@@ -294,35 +331,282 @@ __device__ __forceinline__ int out_count(const ProbeParams& p, int32_t m, bool k
   return (!key_null && !found) ? 1 : 0;
 }
 
-template <int BLOCK>
+// Probe keys of RPT lane-consecutive rows (row lr0 + k * S), all loads issued
+// before any use; *nan marks NaN keys (never equal), bit k of the returned mask
+// a non-null active row.
+template <int RPT, int S>
+__device__ __forceinline__ uint32_t probe_keys(const ProbeParams& p, const PTile& td, int lr0, int64_t (&key)[RPT],
+                                               uint32_t* nanm) {
+  const uint8_t* kv = (const uint8_t*)pbuf(p, td.batch, p.key_col, 0);
+  const void* kd = pbuf(p, td.batch, p.key_col, 2);
+  const int64_t rb = td.row_begin;
+  const int n = td.nrows;
+  uint8_t vx[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; k++) vx[k] = kv && lr0 + k * S < n ? gld(kv + ((rb + lr0 + k * S) >> 3)) : (uint8_t)0xff;
+  *nanm = 0;
+#define GZ_PK(T, CONV)                                                                              \
+  {                                                                                                 \
+    T x[RPT];                                                                                       \
+    _Pragma("unroll") for (int k = 0; k < RPT; k++) x[k] = lr0 + k * S < n ? gld((const T*)kd + rb + lr0 + k * S) : (T)0; \
+    _Pragma("unroll") for (int k = 0; k < RPT; k++) key[k] = CONV(x[k]);                            \
+    break;                                                                                          \
+  }
+#define GZ_PK_I(x) ((int64_t)(x))
+  switch (p.key_type) {
+    case GZ_TYPE_BOOL: {
+      uint8_t x[RPT];
+#pragma unroll
+      for (int k = 0; k < RPT; k++) x[k] = lr0 + k * S < n ? gld((const uint8_t*)kd + ((rb + lr0 + k * S) >> 3)) : 0;
+#pragma unroll
+      for (int k = 0; k < RPT; k++) key[k] = (x[k] >> ((rb + lr0 + k * S) & 7)) & 1;
+      break;
+    }
+    case GZ_TYPE_INT8: GZ_PK(int8_t, GZ_PK_I)
+    case GZ_TYPE_UINT8: GZ_PK(uint8_t, GZ_PK_I)
+    case GZ_TYPE_INT16: GZ_PK(int16_t, GZ_PK_I)
+    case GZ_TYPE_UINT16: GZ_PK(uint16_t, GZ_PK_I)
+    case GZ_TYPE_INT32: case GZ_TYPE_DATE32: GZ_PK(int32_t, GZ_PK_I)
+    case GZ_TYPE_UINT32: GZ_PK(uint32_t, GZ_PK_I)
+    case GZ_TYPE_FLOAT:
+    case GZ_TYPE_DOUBLE: {
+      // the bits of the double value; NaN keys never match
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        bool nan = false;
+        key[k] = lr0 + k * S < n ? join_key(kd, p.key_type, rb + lr0 + k * S, &nan) : 0;
+        if (nan) *nanm |= 1u << k;
+      }
+      break;
+    }
+    default: GZ_PK(int64_t, GZ_PK_I)
+  }
+#undef GZ_PK
+#undef GZ_PK_I
+  uint32_t vm = 0;
+#pragma unroll
+  for (int k = 0; k < RPT; k++)
+    if (lr0 + k * S < n && ((vx[k] >> ((rb + lr0 + k * S) & 7)) & 1)) vm |= 1u << k;
+  return vm;
+}
+
+// Per probe row: the match code and the number of output rows; per tile the
+// total.  Rows are lane-consecutive (coalesced key loads and match stores), the
+// home slots of all RPT rows are loaded together and collisions walk on.
+template <int BLOCK, int RPT>
 __global__ __launch_bounds__(BLOCK) void join_probe_count_kernel(ProbeParams p) {
   __shared__ uint64_t s_scan[BLOCK / kWave + 1];
   int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
   if (t < 0) return;
   const PTile td = p.tiles[t];
-  const uint8_t* kv = (const uint8_t*)pbuf(p, td.batch, p.key_col, 0);
-  const void* kd = pbuf(p, td.batch, p.key_col, 2);
+  const int tid = threadIdx.x;
+  const uint64_t mask = p.J.cap - 1;
   uint64_t sum = 0;
-  for (int lr = threadIdx.x; lr < td.nrows; lr += BLOCK) {
-    int64_t r = (int64_t)td.row_begin + lr;
-    bool knull = kv && !bit_get(kv, r);
-    int32_t m = -1;
-    if (!knull) {
-      bool nan;
-      int64_t key = join_key(kd, p.key_type, r, &nan);
-      if (!nan) {
-        if (key == kJEmpty) m = p.J.special_count > 0 ? -2 : -1;
-        else m = (int32_t)jt_find(p.J, key);
-      }
+  for (int base = 0; base < td.nrows; base += BLOCK * RPT) {
+    int64_t key[RPT];
+    uint32_t nanm;
+    const uint32_t vm = probe_keys<RPT, BLOCK>(p, td, base + tid, key, &nanm);
+    uint64_t s[RPT];
+    int64_t sk[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+      s[k] = jmix64((uint64_t)key[k]) & mask;
+      const bool look = ((vm >> k) & 1) && !((nanm >> k) & 1) && key[k] != kJEmpty;
+      sk[k] = look ? gld(p.J.keys + s[k]) : kJEmpty;
     }
-    p.match[t * p.tile_rows + lr] = knull ? -3 : m;
-    sum += (uint64_t)out_count(p, m, knull);
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+      const int lr = base + k * BLOCK + tid;
+      if (lr >= td.nrows) continue;
+      int32_t m;
+      if (!((vm >> k) & 1)) m = -3;
+      else if ((nanm >> k) & 1) m = -1;
+      else if (key[k] == kJEmpty) m = p.J.special_count > 0 ? -2 : -1;
+      else {
+        m = -1;
+        int64_t v = sk[k];
+        uint64_t q = s[k];
+        for (uint64_t i = 0; i < p.J.cap; i++) {
+          if (v == key[k]) {
+            m = (int32_t)q;
+            break;
+          }
+          if (v == kJEmpty) break;
+          q = (q + 1) & mask;
+          v = gld(p.J.keys + q);
+        }
+      }
+      gst(p.match + t * p.tile_rows + lr, m);
+      // repeated keys: the slot's row count (inner joins only read it)
+      const int32_t cs = m >= 0 && p.J.rowlist && p.join_type == GZ_JOIN_INNER ? gld(p.J.count + m) : 1;
+      sum += (uint64_t)out_count(p, m, cs);
+    }
   }
   uint64_t total;
   block_exclusive_scan<uint64_t>(sum, s_scan, &total);
   if (threadIdx.x == 0) p.tile_count[t] = (uint32_t)total;
 }
 
+// Value of a fixed-width output column for output row o (probe or build side).
+__device__ __forceinline__ void store_value(const OutCol& oc, uint64_t o, uint64_t v, uint64_t v_hi) {
+  const int w = oc.width;
+  uint8_t* d = oc.ovalues + o * w;
+  if (w == 8) gst((uint64_t*)d, v);
+  else if (w == 4) gst((uint32_t*)d, (uint32_t)v);
+  else if (w == 2) gst((uint16_t*)d, (uint16_t)v);
+  else if (w == 1) gst(d, (uint8_t)v);
+  else {
+    gst((uint64_t*)d, v);
+    gst((uint64_t*)d + 1, v_hi);
+  }
+}
+
+// Write pass for tables whose probe rows emit at most one row each (inner
+// joins without repeated keys, semi and anti joins).  Rows are lane-
+// consecutive: wave ballots give each emitting row its place, so the stores of
+// a wave land on consecutive output rows.  Build columns come from the payload
+// (one 16-byte load per emitted row) when it holds them.
+template <int TILE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void join_probe_write01_kernel(ProbeParams p) {
+  constexpr int RPT = TILE / BLOCK;
+  constexpr int NW = BLOCK / kWave;
+  constexpr int G = 4;  // rows per thread whose gathers are in flight together
+  static_assert(RPT % G == 0, "groups of G rounds");
+  __shared__ uint32_t s_wt[RPT][NW];
+  int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
+  if (t < 0) return;
+  const PTile td = p.tiles[t];
+  const int tid = threadIdx.x, w = tid / kWave, lane = __lane_id();
+  const int n = td.nrows;
+  const int64_t rb = td.row_begin;
+  int32_t m[RPT];
+  uint32_t pre[RPT];  // bit 31: the row emits; low bits: emitting rows before it in its wave and round
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    const int lr = k * BLOCK + tid;
+    m[k] = lr < n ? gld(p.match + t * p.tile_rows + lr) : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    const int lr = k * BLOCK + tid;
+    const bool e = lr < n && out_count(p, m[k], 1) > 0;
+    const uint64_t b = __ballot(e);
+    pre[k] = (uint32_t)__popcll(b & lanemask_lt()) | (e ? 0x80000000u : 0u);
+    if (lane == 0) s_wt[k][w] = (uint32_t)__popcll(b);
+  }
+  __syncthreads();
+  const uint64_t obase = p.tile_off[t];
+  uint32_t acc = 0;  // rows emitted by earlier rounds
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    uint32_t before = acc;
+#pragma unroll
+    for (int x = 0; x < NW; x++) {
+      const uint32_t c = s_wt[k][x];
+      if (x < w) before += c;
+      acc += c;
+    }
+    pre[k] = (pre[k] & 0x80000000u) | (before + (pre[k] & 0x7fffffffu));
+  }
+  const bool inner = p.join_type == GZ_JOIN_INNER;
+#pragma unroll
+  for (int k0 = 0; k0 < RPT; k0 += G) {
+    int32_t b[G];
+    u32x4 pl[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const int k = k0 + g;
+      b[g] = -1;
+      if (inner && (pre[k] >> 31)) b[g] = m[k] >= 0 ? gld(p.J.start + m[k]) : (int32_t)p.J.special_start;
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) pl[g] = p.payload && b[g] >= 0 ? gld(p.payload + b[g]) : u32x4{0u, 0u, 0u, 0u};
+    if (p.write_pairs) {
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const int k = k0 + g;
+        if (!(pre[k] >> 31)) continue;
+        const uint64_t o = obase + (pre[k] & 0x7fffffffu);
+        gst(p.pair_probe + o, (int64_t)((uint64_t)t << 32 | (uint32_t)(k * BLOCK + tid)));
+        gst(p.pair_build + o, b[g]);
+      }
+    }
+    for (int c = 0; c < p.ncols; c++) {
+      const OutCol& oc = p.out[c];
+      if (oc.binary) continue;
+      if (oc.type == GZ_TYPE_NA) {
+#pragma unroll
+        for (int g = 0; g < G; g++)
+          if (pre[k0 + g] >> 31) gst(oc.ovalid + obase + (pre[k0 + g] & 0x7fffffffu), (uint8_t)0);
+        continue;
+      }
+      const int wd = oc.width;
+      uint64_t v[G], vh[G];
+      uint8_t ok[G];
+      if (oc.side == 1) {
+        const uint8_t* vb = (const uint8_t*)pbuf(p, td.batch, oc.col, 0);
+        const uint8_t* vv = (const uint8_t*)pbuf(p, td.batch, oc.col, 2);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+          const int k = k0 + g;
+          const int64_t r = rb + k * BLOCK + tid;
+          const bool e = pre[k] >> 31;
+          vh[g] = 0;
+          ok[g] = e && vb ? (uint8_t)bit_get(vb, r) : (uint8_t)1;
+          if (!e) v[g] = 0;
+          else if (oc.type == GZ_TYPE_BOOL) v[g] = (uint64_t)bit_get(vv, r);
+          else if (wd == 8) v[g] = gld((const uint64_t*)vv + r);
+          else if (wd == 4) v[g] = gld((const uint32_t*)vv + r);
+          else if (wd == 2) v[g] = gld((const uint16_t*)vv + r);
+          else if (wd == 1) v[g] = gld(vv + r);
+          else {
+            v[g] = gld((const uint64_t*)vv + 2 * r);
+            vh[g] = gld((const uint64_t*)vv + 2 * r + 1);
+          }
+        }
+      } else {
+        const int off = p.pay_off[c];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+          vh[g] = 0;
+          v[g] = 0;
+          ok[g] = 0;
+          if (b[g] < 0) continue;  // semi / anti: build columns are null (AppendNull)
+          if (off >= 0) {
+            const uint64_t lo = (uint64_t)pl[g].x | ((uint64_t)pl[g].y << 32);
+            const uint64_t hi = (uint64_t)pl[g].z;
+            const uint64_t x = off < 8 ? (lo >> (8 * off)) | (off ? hi << (64 - 8 * off) : 0) : hi >> (8 * (off - 8));
+            v[g] = wd == 8 ? x : x & ((1ull << (8 * wd)) - 1);
+            ok[g] = (uint8_t)((pl[g].w >> c) & 1);
+          } else {
+            ok[g] = gld(oc.bvalid + b[g]);
+            const uint8_t* sp = oc.bvalues + (int64_t)b[g] * wd;
+            if (wd == 8) v[g] = gld((const uint64_t*)sp);
+            else if (wd == 4) v[g] = gld((const uint32_t*)sp);
+            else if (wd == 2) v[g] = gld((const uint16_t*)sp);
+            else if (wd == 1) v[g] = gld(sp);
+            else {
+              v[g] = gld((const uint64_t*)sp);
+              vh[g] = gld((const uint64_t*)sp + 1);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const int k = k0 + g;
+        if (!(pre[k] >> 31)) continue;
+        const uint64_t o = obase + (pre[k] & 0x7fffffffu);
+        gst(oc.ovalid + o, ok[g]);
+        store_value(oc, o, v[g], vh[g]);
+      }
+    }
+  }
+}
+
+// Write pass for inner joins over tables with repeated keys: a probe row emits
+// all rows of its key in insertion order (rowlist).  Consecutive rows per
+// thread, so one scan gives every output position.
 template <int TILE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void join_probe_write_kernel(ProbeParams p) {
   constexpr int RPT = TILE / BLOCK;  // consecutive rows per thread
@@ -339,7 +623,7 @@ __global__ __launch_bounds__(BLOCK) void join_probe_write_kernel(ProbeParams p) 
     int lr = lr0 + k;
     int32_t m = lr < td.nrows ? p.match[t * p.tile_rows + lr] : -1;
     mm[k] = m;
-    cnt[k] = lr < td.nrows ? out_count(p, m == -3 ? -1 : m, m == -3) : 0;
+    cnt[k] = lr < td.nrows ? out_count(p, m, m >= 0 ? p.J.count[m] : 0) : 0;
     sum += (uint64_t)cnt[k];
   }
   uint64_t ex = block_exclusive_scan<uint64_t>(sum, s_scan, nullptr);
@@ -349,12 +633,9 @@ __global__ __launch_bounds__(BLOCK) void join_probe_write_kernel(ProbeParams p) 
     int lr = lr0 + k;
     int64_t r = (int64_t)td.row_begin + lr;
     int32_t m = mm[k];
+    const int64_t st = m >= 0 ? p.J.start[m] : p.J.special_start;
     for (int j = 0; j < cnt[k]; j++, o++) {
-      int32_t b = -1;  // build row (inner)
-      if (p.join_type == GZ_JOIN_INNER) {
-        int64_t st = m >= 0 ? p.J.start[m] : p.J.special_start;
-        b = p.J.rowlist ? p.J.rowlist[st + j] : (int32_t)(st + j);
-      }
+      const int32_t b = p.J.rowlist[st + j];  // build row
       if (p.write_pairs) {
         p.pair_probe[o] = ((int64_t)t << 32) | (int64_t)lr;
         p.pair_build[o] = b;
@@ -367,11 +648,6 @@ __global__ __launch_bounds__(BLOCK) void join_probe_write_kernel(ProbeParams p) 
         }
         const int w = oc.width;
         if (oc.side == 0) {
-          if (b < 0) {  // semi / anti: build columns are null (AppendNull)
-            oc.ovalid[o] = 0;
-            for (int x = 0; x < w; x++) oc.ovalues[o * w + x] = 0;
-            continue;
-          }
           oc.ovalid[o] = oc.bvalid[b];
           const uint8_t* s = oc.bvalues + (int64_t)b * w;
           uint8_t* d = oc.ovalues + o * w;
@@ -771,6 +1047,11 @@ struct gz_probe {
   std::vector<int> sides, idx;
   int tile_rows = 4096;
   DBuf d_tiles, d_bufs, match, tile_count, tile_off, scan_tmp, pair_probe, pair_build;
+  // row-major payload of the fixed-width build output columns (built at the
+  // first probe call when they fit a 16-byte record)
+  DBuf payload;
+  bool payload_ready = false;
+  std::vector<int> pay_off;  // per output column: byte offset in the record, -1 not in it
   struct Out {
     int type = 0, width = 0;
     DBuf values, valid, offsets, data, lens;
@@ -834,10 +1115,10 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   p.build_has_null = r->has_null;
   p.build_keys = r->nkeys;
   p.tile_rows = TILE;
-  p.J.keys = (int64_t*)r->keys.p;
-  p.J.start = (int32_t*)r->start.p;
-  p.J.count = (int32_t*)r->count.p;
-  p.J.rowlist = r->has_dups ? (int32_t*)r->rowlist.p : nullptr;
+  p.J.keys = (const int64_t*)r->keys.p;
+  p.J.start = (const int32_t*)r->start.p;
+  p.J.count = (const int32_t*)r->count.p;
+  p.J.rowlist = r->has_dups ? (const int32_t*)r->rowlist.p : nullptr;
   p.J.cap = r->cap;
   p.J.special_start = r->special_start;
   p.J.special_count = r->special_count;
@@ -857,7 +1138,7 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
     int64_t grid = (T + 7) / 8 * 8;
     {
       KernelTimer kt(ctx, "join_probe_count");
-      hipLaunchKernelGGL(join_probe_count_kernel<256>, dim3((unsigned)grid), dim3(256), 0, st, p);
+      hipLaunchKernelGGL((join_probe_count_kernel<256, 8>), dim3((unsigned)grid), dim3(256), 0, st, p);
     }
     // tile offsets: exclusive scan (uint32 counts -> uint64 offsets)
     size_t tmp = 0;
@@ -906,13 +1187,65 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
     p.pair_build = (int32_t*)h->pair_build.p;
     p.write_pairs = 1;
   }
+  // build columns from the row-major payload: fixed-width columns packed by
+  // decreasing width into 12 bytes (a validity bit each in the last dword)
+  if (!h->payload_ready && r->n > 0) {
+    h->pay_off.assign(nres, -1);
+    PayloadParams q;
+    memset(&q, 0, sizeof(q));
+    std::vector<int> order;
+    for (int c = 0; c < nres; c++)
+      if (p.out[c].side == 0 && !p.out[c].binary && p.out[c].type != GZ_TYPE_NA && p.out[c].width <= 8)
+        order.push_back(c);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return p.out[a].width > p.out[b].width; });
+    int off = 0;
+    bool fits = !order.empty();
+    for (int c : order) {
+      const int wd = p.out[c].width;
+      off = (off + wd - 1) / wd * wd;
+      if (off + wd > 12) { fits = false; break; }
+      q.bvalid[q.ncol] = p.out[c].bvalid;
+      q.bvalues[q.ncol] = p.out[c].bvalues;
+      q.width[q.ncol] = wd;
+      q.off[q.ncol] = off;
+      q.bit[q.ncol] = c;
+      h->pay_off[c] = off;
+      q.ncol++;
+      off += wd;
+    }
+    if (fits) {
+      GZ_TRY(ctx_ensure(ctx, &h->payload, sizeof(u32x4) * r->n));
+      q.n = r->n;
+      q.out = (u32x4*)h->payload.p;
+      unsigned gb = (unsigned)std::min<int64_t>((r->n + 255) / 256, 8192);
+      hipLaunchKernelGGL(join_payload_kernel, dim3(gb), dim3(256), 0, st, q);
+    } else {
+      h->pay_off.assign(nres, -1);
+    }
+    h->payload_ready = true;
+  }
+  bool use_payload = false;
+  for (int c = 0; c < nres && c < (int)h->pay_off.size(); c++) {
+    p.pay_off[c] = h->pay_off[c];
+    use_payload |= h->pay_off[c] >= 0;
+  }
+  for (int c = (int)h->pay_off.size(); c < nres; c++) p.pay_off[c] = -1;
+  p.payload = use_payload ? (const u32x4*)h->payload.p : nullptr;
+  const bool at_most_one = p.join_type != GZ_JOIN_INNER || !p.J.rowlist;
   if (T > 0 && total > 0) {
     int64_t grid = (T + 7) / 8 * 8;
     KernelTimer kt(ctx, "join_probe_write");
-    if (TILE == 4096)
+    if (at_most_one) {
+      if (TILE == 4096)
+        hipLaunchKernelGGL((join_probe_write01_kernel<4096, kWrite01Block>), dim3((unsigned)grid),
+                           dim3(kWrite01Block), 0, st, p);
+      else
+        hipLaunchKernelGGL((join_probe_write01_kernel<1024, 256>), dim3((unsigned)grid), dim3(256), 0, st, p);
+    } else if (TILE == 4096) {
       hipLaunchKernelGGL((join_probe_write_kernel<4096, 256>), dim3((unsigned)grid), dim3(256), 0, st, p);
-    else
+    } else {
       hipLaunchKernelGGL((join_probe_write_kernel<1024, 256>), dim3((unsigned)grid), dim3(256), 0, st, p);
+    }
   }
   if (any_binary && total > 0) {
     for (int c = 0; c < nres; c++) {
@@ -1056,7 +1389,7 @@ int gz_probe_release(gz_probe* h) {
   for (auto& o : h->outs)
     for (DBuf* b : {&o.values, &o.valid, &o.offsets, &o.data, &o.lens}) ctx_free(ctx, b);
   for (DBuf* b : {&h->d_tiles, &h->d_bufs, &h->match, &h->tile_count, &h->tile_off, &h->scan_tmp,
-                  &h->pair_probe, &h->pair_build})
+                  &h->pair_probe, &h->pair_build, &h->payload})
     ctx_free(ctx, b);
   delete h;
   return GZ_OK;

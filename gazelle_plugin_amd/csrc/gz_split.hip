@@ -958,6 +958,8 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   std::vector<const void*> bufs((size_t)nb * nc * 3, nullptr);
   int64_t N = 0;
   std::vector<size_t> data_bytes(h->base.nbin, 0);
+  std::vector<bool> data_shared(h->base.nbin, false);  // some data buffer backs several batches
+  std::vector<std::vector<const void*>> data_ptrs(h->base.nbin);
   std::vector<bool> has_validity(nc, false);
   std::vector<int32_t> tile_batch;
   std::vector<const int32_t*> empty_data;
@@ -997,7 +999,10 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
         slot[2] = slot[1];  // any valid address; no byte is read from it
       }
       if (bt.num_rows > 0 && !slot[2]) return fail(GZ_INVALID, "missing values buffer");
-      if (type_is_binary(t)) data_bytes[h->base.bin[c]] += (size_t)vsz;
+      if (type_is_binary(t)) {
+        data_bytes[h->base.bin[c]] += (size_t)vsz;
+        data_ptrs[h->base.bin[c]].push_back(slot[2]);
+      }
       else if (t == GZ_TYPE_BOOL) {
         if (vsz < (bt.num_rows + 7) / 8) return fail(GZ_INVALID, "values buffer too small");
       } else if (vsz < bt.num_rows * type_width(t)) return fail(GZ_INVALID, "values buffer too small");
@@ -1161,14 +1166,20 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
                        P, nbin, (int64_t*)seg->row_start.p, (int64_t*)seg->byte_start.p,
                        (int32_t**)seg->offtab.p, (int*)seg->err.p);
   }
-  // utf8 data outputs: the sum of the input data buffers bounds them; when that
-  // bound is loose (buffers shared by many batches), size them exactly from the
-  // scanned byte totals instead.
+  // utf8 data outputs: the sum of the input data buffers bounds them.  When
+  // batches share data buffers (slices of one buffer) that bound may be many
+  // times the real size, so those are sized exactly from the scanned byte
+  // totals (one host round trip); so is a bound above 16 GiB.
+  for (int b = 0; b < h->base.nbin; b++) {
+    auto& v = data_ptrs[b];
+    std::sort(v.begin(), v.end());
+    data_shared[b] = std::adjacent_find(v.begin(), v.end()) != v.end();
+  }
   for (int c = 0; c < nc; c++) {
     int oc = prm.out_col[c];
     if (oc < 0 || !type_is_binary(h->types_in[c])) continue;
     size_t bytes = data_bytes[prm.bin[c]];
-    if (bytes > ((size_t)1 << 30)) {
+    if (data_shared[prm.bin[c]] || bytes > ((size_t)16 << 30)) {
       int64_t total = 0;
       GZ_HIP(hipMemcpyAsync(&total, (int64_t*)seg->byte_start.p + (int64_t)prm.bin[c] * (P + 1) + P,
                             sizeof(int64_t), hipMemcpyDeviceToHost, st));

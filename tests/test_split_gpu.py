@@ -173,3 +173,31 @@ def test_segment_batch_counts_follow_the_input_batches(ctx):
         pids = np.asarray(oracle.split_pids([b], types, 31, [0]))
         assert np.array_equal(got[bi], np.bincount(pids, minlength=31)), bi
     sp.close()
+
+
+def test_batches_sharing_one_utf8_buffer(ctx):
+    """Batches whose utf8 bytes are slices of one device buffer (offsets that do
+    not start at 0, the same data address in every batch): the output bytes are
+    sized from the scanned totals, and the layout equals the oracle's."""
+    import torch
+
+    types = [T.INT64, T.STRING]
+    batches = random_batches(types, [3000, 4096, 17, 5000, 1], seed=5, null_frac=0.1)
+    exp = oracle.split(batches, types, 64, [0])
+    shared = torch.from_numpy(np.concatenate([b.columns[1].values for b in batches]).astype(np.uint8)).to("cuda")
+    dev, base = [], 0
+    for b in batches:
+        col = b.columns[1]
+        offs = (col.offsets.astype(np.int64) + base).astype(np.int32)
+        base += len(col.values)
+        host = DeviceBatch.from_host(b)
+        tensors = list(host.tensors)
+        tensors[3] = torch.from_numpy(offs.view(np.uint8).copy()).to("cuda")
+        tensors[4] = shared
+        dev.append(DeviceBatch(b.num_rows, types, tensors))
+    sp = Splitter.Make(ctx, "hash", types, 64, [0])
+    sp.SplitBatches(dev)
+    sp.Stop()
+    got = sp.read_partitions()
+    sp.close()
+    assert_layout_equal(got, exp, types)
