@@ -127,6 +127,10 @@ _SIGS = {
     "gz_split_segment_batch_counts": (ctypes.c_int, [_P, _I32, _PI32, _PI32]),
     "gz_split_compute_pid_time": (_I64, [_P]),
     "gz_split_release": (ctypes.c_int, [_P]),
+    "gz_split_ipc_prepare": (ctypes.c_int, [_P, ctypes.c_void_p, _I64, _I32, _PI64, _PI64]),
+    "gz_split_ipc_write": (ctypes.c_int, [_P, ctypes.c_void_p, _I64]),
+    "gz_ipc_record_batch_message": (ctypes.c_int, [_I64, _I32, _PI64, _I32, _PI64, _I64, ctypes.c_void_p, _I64,
+                                                   _PI64]),
 }
 
 # symbols added by later operators are bound when present

@@ -788,8 +788,11 @@ struct SplitSegment {
   bool resolved = false;
 };
 
+struct gz_ipc_plan;
+static void ipc_plan_free(gz_context* ctx, gz_ipc_plan* pl);
 struct gz_splitter {
   gz_context* ctx = nullptr;
+  gz_ipc_plan* ipc = nullptr;  // partition writer layout (gz_split_ipc_prepare)
   int mode = 0;
   int P = 0;
   int ncols_in = 0;
@@ -1345,9 +1348,687 @@ int gz_split_release(gz_splitter* h) {
   // No synchronization: the pool hands a freed block only to work enqueued
   // later on the context's stream, which runs after everything queued here.
   for (auto* s : h->segments) split_release_segment(h->ctx, s);
+  if (h->ipc) ipc_plan_free(h->ctx, h->ipc);
   ctx_free(h->ctx, &h->d_tiles);
   ctx_free(h->ctx, &h->d_bufs);
   delete h;
+  return GZ_OK;
+}
+
+}  // extern "C"
+
+// ======================================================= partition writer (IPC)
+// Splitter::Stop's data file (PartitionWriter, splitter.cc:70-185, :395-437) with
+// the default compression_type UNCOMPRESSED (shuffle/type.h:44): partition after
+// partition, each non-empty partition's Arrow IPC stream (the caller's schema
+// message when write_schema, the record batches cut as CacheRecordBatch cuts them
+// (:462-608, the cut rule of DoSplit :889-925), the end-of-stream marker
+// 0xFFFFFFFF 0x00000000); an empty partition writes nothing (length 0).
+//
+// The record-batch metadata (an Arrow `Message` flatbuffer with a `RecordBatch`
+// header) is built on the host; the bodies are assembled on the device from the
+// split segments by one kernel: byte copies (fixed-width values, utf8 bytes, the
+// metadata), bit-level concatenation (validity and boolean bitmaps, which start
+// at arbitrary bit positions in the segments), and rebased int32 offsets.
+// Buffers follow Arrow's IPC writer: a validity buffer only when the batch has
+// nulls (else a zero-length buffer entry), every buffer padded to 8 bytes.
+namespace gz {
+
+enum IpcKind : int32_t { IPC_COPY = 0, IPC_BITS = 1, IPC_OFFS = 2 };
+
+struct IpcPiece {
+  const uint8_t* src;
+  int64_t a;     // BITS: source bit; OFFS: unused
+  int64_t b;     // BITS: first bit in the buffer; OFFS: first row in the buffer
+  int64_t n;     // BITS: bits; OFFS: rows
+  int64_t base;  // OFFS: bytes of the batch before this piece
+};
+
+struct IpcTask {
+  int32_t kind;
+  int32_t pad;
+  int64_t p0, np;  // pieces [p0, p0 + np) (COPY: one)
+  int64_t dst;     // output byte offset
+  int64_t n;       // COPY: bytes; BITS: bits; OFFS: rows (n + 1 entries)
+  int64_t out;     // bytes written including the zero padding to a multiple of 8
+  int64_t total;   // OFFS: the last entry (bytes of the batch)
+};
+
+__device__ __forceinline__ uint32_t ipc_bits32(const uint8_t* src, int64_t bit) {
+  // 32 bits of a bitmap from an arbitrary bit position (5 byte loads)
+  const uint8_t* p = src + (bit >> 3);
+  uint64_t w = 0;
+#pragma unroll
+  for (int k = 0; k < 5; k++) w |= (uint64_t)gld(p + k) << (8 * k);
+  return (uint32_t)(w >> (bit & 7));
+}
+
+__global__ __launch_bounds__(256) void ipc_assemble_kernel(const IpcTask* tasks, int64_t ntasks,
+                                                           const IpcPiece* pieces, uint8_t* out) {
+  for (int64_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+    const IpcTask t = tasks[ti];
+    uint8_t* dst = out + t.dst;
+    if (t.kind == IPC_COPY) {
+      const uint8_t* src = pieces[t.p0].src;
+      for (int64_t i = threadIdx.x; i < t.out; i += blockDim.x) gst(dst + i, i < t.n ? gld(src + i) : (uint8_t)0);
+    } else if (t.kind == IPC_BITS) {
+      // output word w holds bits [32w, 32w + 32); pieces are ordered by b
+      const IpcPiece* pc = pieces + t.p0;
+      for (int64_t w = threadIdx.x; w * 4 < t.out; w += blockDim.x) {
+        const int64_t lo = 32 * w, hi = lo + 32;
+        uint32_t word = 0;
+        if (lo < t.n) {
+          int64_t l = 0, r = t.np - 1;  // last piece starting at or before lo
+          while (l < r) {
+            const int64_t m = (l + r + 1) >> 1;
+            if (pc[m].b <= lo) l = m; else r = m - 1;
+          }
+          for (int64_t k = l; k < t.np && pc[k].b < hi; k++) {
+            const int64_t s0 = max(lo, pc[k].b), s1 = min(hi, pc[k].b + pc[k].n);
+            if (s1 <= s0) continue;
+            const int cnt = (int)(s1 - s0);
+            // a segment without a validity buffer: every row valid
+            uint32_t v = pc[k].src ? ipc_bits32(pc[k].src, pc[k].a + (s0 - pc[k].b)) : 0xffffffffu;
+            if (cnt < 32) v &= (1u << cnt) - 1;
+            word |= v << (s0 - lo);
+          }
+          if (t.n < hi) word &= (1u << (t.n - lo)) - 1;
+        }
+        // the buffer's length is a multiple of 8 bytes: whole words
+        gst((uint32_t*)dst + w, word);
+      }
+    } else {
+      const IpcPiece* pc = pieces + t.p0;
+      for (int64_t j = threadIdx.x; j * 4 < t.out; j += blockDim.x) {
+        int32_t v = 0;
+        if (j == t.n) {
+          v = (int32_t)t.total;
+        } else if (j < t.n) {
+          int64_t l = 0, r = t.np - 1;
+          while (l < r) {
+            const int64_t m = (l + r + 1) >> 1;
+            if (pc[m].b <= j) l = m; else r = m - 1;
+          }
+          const int32_t* so = (const int32_t*)pc[l].src;
+          v = (int32_t)(gld(so + (j - pc[l].b)) - gld(so) + pc[l].base);
+        }
+        gst((int32_t*)dst + j, v);
+      }
+    }
+  }
+}
+
+// Zero bits (nulls) among n bits of a bitmap from bit a, per piece; grid-stride
+// over pieces with a wave per piece.
+__global__ __launch_bounds__(256) void ipc_null_count_kernel(const IpcPiece* pieces, int64_t np,
+                                                             unsigned long long* out) {
+  const int lane = __lane_id();
+  const int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t i = wv; i < np; i += nw) {
+    const IpcPiece pc = pieces[i];
+    unsigned long long ones = 0;
+    for (int64_t o = (int64_t)lane * 32; o < pc.n; o += 32 * kWave) {
+      uint32_t v = ipc_bits32(pc.src, pc.a + o);
+      const int64_t cnt = min<int64_t>(32, pc.n - o);
+      if (cnt < 32) v &= (1u << cnt) - 1;
+      ones += __popc(v);
+    }
+    ones = wave_reduce_sum(ones);
+    if (lane == 0) out[i] = (unsigned long long)pc.n - ones;
+  }
+}
+
+// A flatbuffer built back to front (the layout the format's own builders
+// produce): objects are prepended, an object is referred to by its distance
+// from the end of the buffer, and every offset points forward.
+class FlatBuilder {
+ public:
+  size_t size() const { return b_.size(); }
+  void align(size_t a) {
+    while (b_.size() % a) prepend_bytes("\0", 1);
+  }
+  void prepend_bytes(const void* p, size_t n) { b_.insert(b_.begin(), (const uint8_t*)p, (const uint8_t*)p + n); }
+  template <typename T>
+  void scalar(T v) {
+    align(sizeof(T));
+    prepend_bytes(&v, sizeof(T));
+  }
+  // uoffset to an object referred to by ref; returns the field's ref
+  size_t offset_to(size_t ref) {
+    align(4);
+    const uint32_t v = (uint32_t)(b_.size() + 4 - ref);
+    prepend_bytes(&v, 4);
+    return b_.size();
+  }
+  // vector of 16-byte structs of two int64 (FieldNode, Buffer); returns its ref
+  size_t vector_of_pairs(const std::vector<std::pair<int64_t, int64_t>>& v) {
+    // elements 8-aligned and preceded by the uint32 length
+    const size_t body = 16 * v.size();
+    while ((b_.size() + body) % 8) prepend_bytes("\0", 1);
+    for (size_t i = v.size(); i-- > 0;) {
+      int64_t s[2] = {v[i].first, v[i].second};
+      prepend_bytes(s, 16);
+    }
+    const uint32_t n = (uint32_t)v.size();
+    prepend_bytes(&n, 4);
+    return b_.size();
+  }
+  // tables: start, add fields (in any order), end
+  void start_table() {
+    fields_.clear();
+    table_end_ = b_.size();
+  }
+  template <typename T>
+  void field(int id, T v) {
+    scalar(v);
+    fields_.push_back({id, b_.size()});
+  }
+  void field_offset(int id, size_t ref) { fields_.push_back({id, offset_to(ref)}); }
+  size_t end_table() {
+    align(4);
+    int32_t placeholder = 0;
+    prepend_bytes(&placeholder, 4);
+    const size_t table_ref = b_.size();
+    int nf = 0;
+    for (auto& f : fields_) nf = std::max(nf, f.first + 1);
+    std::vector<uint16_t> vt(2 + nf, 0);
+    vt[0] = (uint16_t)(2 * (2 + nf));
+    vt[1] = (uint16_t)(table_ref - table_end_);
+    for (auto& f : fields_) vt[2 + f.first] = (uint16_t)(table_ref - f.second);
+    for (size_t i = vt.size(); i-- > 0;) prepend_bytes(&vt[i], 2);
+    const size_t vt_ref = b_.size();
+    const int32_t soff = (int32_t)(vt_ref - table_ref);  // vtable precedes the table
+    memcpy(&b_[b_.size() - table_ref], &soff, 4);
+    return table_ref;
+  }
+  std::vector<uint8_t> finish(size_t root) {
+    // the root offset sits at the start of a buffer whose size is a multiple of 8
+    while ((b_.size() + 4) % 8) prepend_bytes("\0", 1);
+    offset_to(root);
+    return b_;
+  }
+
+ private:
+  std::vector<uint8_t> b_;
+  std::vector<std::pair<int, size_t>> fields_;
+  size_t table_end_ = 0;
+};
+
+// Encapsulated IPC message of a record batch: continuation token, metadata
+// length, Message flatbuffer (version V5, header RecordBatch) padded so the body
+// starts on a multiple of 8.
+static std::vector<uint8_t> ipc_record_batch_message(int64_t rows,
+                                                     const std::vector<std::pair<int64_t, int64_t>>& nodes,
+                                                     const std::vector<std::pair<int64_t, int64_t>>& buffers,
+                                                     int64_t body_len) {
+  FlatBuilder fb;
+  const size_t vbuf = fb.vector_of_pairs(buffers);
+  const size_t vnode = fb.vector_of_pairs(nodes);
+  fb.start_table();  // RecordBatch
+  fb.field<int64_t>(0, rows);
+  fb.field_offset(1, vnode);
+  fb.field_offset(2, vbuf);
+  const size_t rb = fb.end_table();
+  fb.start_table();  // Message
+  fb.field<int64_t>(3, body_len);
+  fb.field_offset(2, rb);
+  fb.field<uint8_t>(1, 3);   // MessageHeader.RecordBatch
+  fb.field<int16_t>(0, 4);   // MetadataVersion.V5
+  const size_t msg = fb.end_table();
+  std::vector<uint8_t> meta = fb.finish(msg);
+  while ((meta.size() + 8) % 8) meta.push_back(0);
+  std::vector<uint8_t> out(8);
+  const int32_t cont = -1, len = (int32_t)meta.size();
+  memcpy(out.data(), &cont, 4);
+  memcpy(out.data() + 4, &len, 4);
+  out.insert(out.end(), meta.begin(), meta.end());
+  return out;
+}
+
+static int64_t pad8(int64_t n) { return (n + 7) / 8 * 8; }
+
+}  // namespace gz
+
+struct gz_ipc_plan {
+  int64_t total = 0;
+  std::vector<int64_t> part_len;
+  std::vector<IpcTask> tasks;
+  std::vector<IpcPiece> pieces;
+  std::vector<uint8_t> blob;  // metadata and schema bytes, copied by COPY tasks
+  DBuf d_tasks, d_pieces, d_blob;
+};
+
+static void ipc_plan_free(gz_context* ctx, gz_ipc_plan* pl) {
+  if (!pl) return;
+  ctx_free(ctx, &pl->d_tasks);
+  ctx_free(ctx, &pl->d_pieces);
+  ctx_free(ctx, &pl->d_blob);
+  delete pl;
+}
+
+// Rows of each record batch of one partition, from its rows per input batch in
+// order (the reference's buffer policy; partition_writer.py restates it).
+static std::vector<int64_t> ipc_batch_rows(const std::vector<int32_t>& counts, int64_t bs, bool prefer_spill) {
+  std::vector<int64_t> out;
+  int64_t size = 0, base = 0;
+  for (int32_t cnt : counts) {
+    if (cnt <= 0) continue;
+    const int64_t new_size = std::max<int64_t>(bs, cnt);
+    if (size == 0) {
+      size = new_size;
+    } else if (base + cnt > size) {
+      if (base > 0) out.push_back(base);
+      base = 0;
+      if (prefer_spill) {
+        if (new_size > size) size = new_size;
+      } else {
+        size = new_size;
+      }
+    }
+    base += cnt;
+  }
+  if (base > 0) out.push_back(base);
+  return out;
+}
+
+static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len, bool write_schema,
+                       gz_ipc_plan* plan) {
+  gz_context* ctx = h->ctx;
+  hipStream_t st = ctx->stream;
+  const int P = h->P;
+  const int nout = (int)h->out_types.size();
+  if (h->opts.offheap_per_task > 0)
+    return fail(GZ_NOT_IMPLEMENTED, "partition writer: offheap_per_task sizing is not supported");
+  const int64_t bs = h->opts.buffer_size > 0 ? h->opts.buffer_size : 4096;
+  const int nseg = (int)h->segments.size();
+  for (auto* s : h->segments) GZ_TRY(resolve_segment(h, s));
+  // rows per input batch per partition, segment by segment (counts[s][b * P + p])
+  std::vector<std::vector<int32_t>> cnt(nseg);
+  for (int s = 0; s < nseg; s++) {
+    cnt[s].assign((size_t)h->segments[s]->nbatches * P, 0);
+    int32_t nb = 0;
+    GZ_TRY(gz_split_segment_batch_counts(h, s, &nb, cnt[s].data()));
+  }
+  // input column of each output column (range mode drops the pid column)
+  std::vector<int> in_of(nout, -1);
+  for (int c = 0; c < h->ncols_in; c++)
+    if (h->base.out_col[c] >= 0) in_of[h->base.out_col[c]] = c;
+  // ---- pass 1: batches, their pieces per segment, validity pieces for null counts
+  struct Span {
+    int seg;
+    int64_t row0;  // partition-local row in the segment
+    int64_t rows;
+  };
+  struct Batch {
+    int p;
+    int64_t rows;
+    std::vector<Span> spans;
+  };
+  std::vector<Batch> batches;
+  for (int p = 0; p < P; p++) {
+    std::vector<int32_t> seq;
+    std::vector<std::pair<int, int64_t>> seg_rows;  // (segment, rows of p in it)
+    for (int s = 0; s < nseg; s++) {
+      int64_t r = 0;
+      for (int32_t b = 0; b < h->segments[s]->nbatches; b++) {
+        seq.push_back(cnt[s][(size_t)b * P + p]);
+        r += cnt[s][(size_t)b * P + p];
+      }
+      seg_rows.push_back({s, r});
+    }
+    std::vector<int64_t> rows = ipc_batch_rows(seq, bs, h->opts.prefer_spill != 0);
+    size_t si = 0;
+    int64_t used = 0;  // rows of segment seg_rows[si] already taken
+    for (int64_t n : rows) {
+      Batch bt;
+      bt.p = p;
+      bt.rows = n;
+      int64_t left = n;
+      while (left > 0) {
+        while (si < seg_rows.size() && used >= seg_rows[si].second) {
+          si++;
+          used = 0;
+        }
+        if (si >= seg_rows.size()) return fail(GZ_UNKNOWN_ERROR, "partition writer: row accounting");
+        const int64_t take = std::min(left, seg_rows[si].second - used);
+        bt.spans.push_back({seg_rows[si].first, used, take});
+        used += take;
+        left -= take;
+      }
+      batches.push_back(std::move(bt));
+    }
+  }
+  // validity pieces of every (batch, column with validity): null counts on the device
+  std::vector<IpcPiece> vp;
+  std::vector<int64_t> vp_first((size_t)batches.size() * nout + 1, 0);
+  for (size_t bi = 0; bi < batches.size(); bi++) {
+    const Batch& bt = batches[bi];
+    for (int c = 0; c < nout; c++) {
+      vp_first[bi * nout + c] = (int64_t)vp.size();
+      if (h->out_types[c] == GZ_TYPE_NA) continue;
+      int64_t dstbit = 0;
+      for (const Span& sp : bt.spans) {
+        SplitSegment* sg = h->segments[sp.seg];
+        if (sg->validity[c].p)
+          vp.push_back({(const uint8_t*)sg->validity[c].p, sg->h_row_start[bt.p] + sp.row0, dstbit, sp.rows, 0});
+        dstbit += sp.rows;
+      }
+    }
+  }
+  vp_first[batches.size() * nout] = (int64_t)vp.size();
+  std::vector<unsigned long long> vnull(std::max<size_t>(vp.size(), 1), 0);
+  if (!vp.empty()) {
+    DBuf dvp, dnull;
+    GZ_TRY(ctx_alloc(ctx, sizeof(IpcPiece) * vp.size(), &dvp));
+    GZ_TRY(ctx_alloc(ctx, sizeof(unsigned long long) * vp.size(), &dnull));
+    GZ_HIP(hipMemcpyAsync(dvp.p, vp.data(), sizeof(IpcPiece) * vp.size(), hipMemcpyHostToDevice, st));
+    const unsigned g = (unsigned)std::min<size_t>((vp.size() + 3) / 4, 65536);
+    hipLaunchKernelGGL(ipc_null_count_kernel, dim3(g), dim3(256), 0, st, (const IpcPiece*)dvp.p,
+                       (int64_t)vp.size(), (unsigned long long*)dnull.p);
+    GZ_HIP(hipMemcpyAsync(vnull.data(), dnull.p, sizeof(unsigned long long) * vp.size(), hipMemcpyDeviceToHost,
+                          st));
+    GZ_HIP(hipStreamSynchronize(st));
+    ctx_free(ctx, &dvp);
+    ctx_free(ctx, &dnull);
+  }
+  // ---- pass 2: layout of the stream, metadata, tasks
+  plan->part_len.assign(P, 0);
+  plan->tasks.clear();
+  plan->pieces.clear();
+  plan->blob.clear();
+  int64_t pos = 0;
+  auto add_copy = [&](const uint8_t* src_host_blob_or_dev, bool from_blob, int64_t nbytes, int64_t out_bytes) {
+    IpcTask t{};
+    t.kind = IPC_COPY;
+    t.p0 = (int64_t)plan->pieces.size();
+    t.np = 1;
+    t.dst = pos;
+    t.n = nbytes;
+    t.out = out_bytes;
+    IpcPiece pc{};
+    if (from_blob) {
+      pc.src = nullptr;
+      pc.a = (int64_t)plan->blob.size();  // resolved to a device address after the blob upload
+      plan->blob.insert(plan->blob.end(), src_host_blob_or_dev, src_host_blob_or_dev + nbytes);
+      t.pad = 1;
+    } else {
+      pc.src = src_host_blob_or_dev;
+    }
+    plan->pieces.push_back(pc);
+    plan->tasks.push_back(t);
+    pos += out_bytes;
+  };
+  size_t bi = 0;
+  for (int p = 0; p < P; p++) {
+    const int64_t start = pos;
+    bool any = false;
+    while (bi < batches.size() && batches[bi].p == p) {
+      const Batch& bt = batches[bi];
+      if (!any && write_schema && schema_len > 0) add_copy(schema, true, schema_len, schema_len);
+      any = true;
+      // field nodes and buffers (offsets relative to the body start)
+      std::vector<std::pair<int64_t, int64_t>> nodes, bufs;
+      struct Pending {
+        int kind;
+        int c;
+        int64_t off, n, out;
+      };
+      std::vector<Pending> body;
+      int64_t boff = 0;
+      auto add_buf = [&](int kind, int c, int64_t n, int64_t bytes) {
+        bufs.push_back({boff, bytes});
+        body.push_back({kind, c, boff, n, pad8(bytes)});
+        boff += pad8(bytes);
+      };
+      for (int c = 0; c < nout; c++) {
+        const int t = h->out_types[c];
+        if (t == GZ_TYPE_NA) {
+          nodes.push_back({bt.rows, bt.rows});
+          continue;
+        }
+        unsigned long long nulls = 0;
+        for (int64_t k = vp_first[bi * nout + c]; k < vp_first[bi * nout + c + 1]; k++) nulls += vnull[k];
+        nodes.push_back({bt.rows, (int64_t)nulls});
+        if (nulls > 0) add_buf(IPC_BITS, c, bt.rows, (bt.rows + 7) / 8);
+        else bufs.push_back({boff, 0});
+        if (t == GZ_TYPE_BOOL) {
+          add_buf(IPC_BITS, c + (1 << 16), bt.rows, (bt.rows + 7) / 8);
+        } else if (type_is_binary(t)) {
+          add_buf(IPC_OFFS, c, bt.rows, (bt.rows + 1) * 4);
+          // bytes of the batch: from the segments' offsets (read on the host)
+          add_buf(IPC_COPY, c, 0, 0);  // sized below
+        } else {
+          add_buf(IPC_COPY, c, bt.rows * type_width(t), bt.rows * type_width(t));
+        }
+      }
+      // utf8 byte sizes per span: offsets at the span ends (host copies)
+      std::vector<int64_t> span_bytes;  // per (binary column, span)
+      for (auto& pd : body) {
+        if (pd.kind != IPC_COPY || !type_is_binary(h->out_types[pd.c])) continue;
+        int64_t total = 0;
+        for (const Span& sp : bt.spans) {
+          SplitSegment* sg = h->segments[sp.seg];
+          const int32_t* o = (const int32_t*)sg->offsets[pd.c].p + sg->h_row_start[bt.p] + bt.p + sp.row0;
+          int32_t ends[2];
+          GZ_HIP(hipMemcpyAsync(&ends[0], o, 4, hipMemcpyDeviceToHost, st));
+          GZ_HIP(hipMemcpyAsync(&ends[1], o + sp.rows, 4, hipMemcpyDeviceToHost, st));
+          GZ_HIP(hipStreamSynchronize(st));
+          span_bytes.push_back(ends[0]);
+          span_bytes.push_back((int64_t)ends[1] - ends[0]);
+          total += (int64_t)ends[1] - ends[0];
+        }
+        pd.n = total;
+      }
+      // recompute the buffer table and offsets now that utf8 sizes are known
+      bufs.clear();
+      boff = 0;
+      {
+        size_t k = 0;
+        for (int c = 0; c < nout; c++) {
+          const int t = h->out_types[c];
+          if (t == GZ_TYPE_NA) continue;
+          if (nodes[c].second > 0) {
+            bufs.push_back({boff, (bt.rows + 7) / 8});
+            body[k].off = boff;
+            body[k].out = pad8((bt.rows + 7) / 8);
+            boff += body[k].out;
+            k++;
+          } else {
+            bufs.push_back({boff, 0});
+          }
+          const int nb = t == GZ_TYPE_BOOL ? 1 : type_is_binary(t) ? 2 : 1;
+          for (int j = 0; j < nb; j++, k++) {
+            Pending& pd = body[k];
+            const int64_t bytes = pd.kind == IPC_BITS ? (bt.rows + 7) / 8
+                                  : pd.kind == IPC_OFFS ? (bt.rows + 1) * 4
+                                                        : pd.n;
+            bufs.push_back({boff, bytes});
+            pd.off = boff;
+            pd.out = pad8(bytes);
+            boff += pd.out;
+          }
+        }
+      }
+      const std::vector<uint8_t> msg = ipc_record_batch_message(bt.rows, nodes, bufs, boff);
+      add_copy(msg.data(), true, (int64_t)msg.size(), (int64_t)msg.size());
+      // body tasks
+      const int64_t body0 = pos;
+      size_t sb = 0;
+      for (auto& pd : body) {
+        const int c = pd.c & 0xffff;
+        const bool values_bits = (pd.c >> 16) != 0;
+        IpcTask t{};
+        t.kind = pd.kind;
+        t.dst = body0 + pd.off;
+        t.out = pd.out;
+        t.p0 = (int64_t)plan->pieces.size();
+        if (pd.kind == IPC_BITS) {
+          t.n = bt.rows;
+          int64_t dstbit = 0;
+          for (const Span& sp : bt.spans) {
+            SplitSegment* sg = h->segments[sp.seg];
+            const uint8_t* bm = (const uint8_t*)(values_bits ? sg->values[c].p : sg->validity[c].p);
+            plan->pieces.push_back({bm, sg->h_row_start[bt.p] + sp.row0, dstbit, sp.rows, 0});
+            dstbit += sp.rows;
+          }
+        } else if (pd.kind == IPC_OFFS) {
+          t.n = bt.rows;
+          int64_t dstrow = 0, bytes = 0;
+          size_t sbk = sb;
+          for (const Span& sp : bt.spans) {
+            SplitSegment* sg = h->segments[sp.seg];
+            const int32_t* o = (const int32_t*)sg->offsets[c].p + sg->h_row_start[bt.p] + bt.p + sp.row0;
+            plan->pieces.push_back({(const uint8_t*)o, 0, dstrow, sp.rows, bytes});
+            dstrow += sp.rows;
+            bytes += span_bytes[sbk * 2 + 1];
+            sbk++;
+          }
+          t.total = bytes;
+        } else if (type_is_binary(h->out_types[c])) {
+          // utf8 bytes: one COPY task per span
+          int64_t d = t.dst;
+          for (const Span& sp : bt.spans) {
+            SplitSegment* sg = h->segments[sp.seg];
+            int64_t bstart = 0;
+            {
+              int in_col = in_of[c];
+              const int j = in_col >= 0 ? h->base.bin[in_col] : -1;
+              bstart = sg->h_byte_start[(size_t)j * (P + 1) + bt.p];
+            }
+            const int64_t o0 = span_bytes[sb * 2], nb2 = span_bytes[sb * 2 + 1];
+            IpcTask u{};
+            u.kind = IPC_COPY;
+            u.p0 = (int64_t)plan->pieces.size();
+            u.np = 1;
+            u.dst = d;
+            u.n = nb2;
+            u.out = nb2;
+            plan->pieces.push_back({(const uint8_t*)sg->data[c].p + bstart + o0, 0, 0, 0, 0});
+            plan->tasks.push_back(u);
+            d += nb2;
+            sb++;
+          }
+          if (pd.out > pd.n) {  // zero padding of the buffer
+            IpcTask z{};
+            z.kind = IPC_COPY;
+            z.p0 = (int64_t)plan->pieces.size();
+            z.np = 1;
+            z.dst = d;
+            z.n = 0;
+            z.out = pd.out - pd.n;
+            plan->pieces.push_back({nullptr, 0, 0, 0, 0});
+            plan->tasks.push_back(z);
+          }
+          continue;
+        } else {
+          // fixed-width values: one COPY task per span
+          const int w = type_width(h->out_types[c]);
+          int64_t d = t.dst;
+          for (const Span& sp : bt.spans) {
+            SplitSegment* sg = h->segments[sp.seg];
+            IpcTask u{};
+            u.kind = IPC_COPY;
+            u.p0 = (int64_t)plan->pieces.size();
+            u.np = 1;
+            u.dst = d;
+            u.n = sp.rows * w;
+            u.out = sp.rows * w;
+            plan->pieces.push_back(
+                {(const uint8_t*)sg->values[c].p + (sg->h_row_start[bt.p] + sp.row0) * w, 0, 0, 0, 0});
+            plan->tasks.push_back(u);
+            d += sp.rows * w;
+          }
+          if (pd.out > pd.n) {
+            IpcTask z{};
+            z.kind = IPC_COPY;
+            z.p0 = (int64_t)plan->pieces.size();
+            z.np = 1;
+            z.dst = d;
+            z.n = 0;
+            z.out = pd.out - pd.n;
+            plan->pieces.push_back({nullptr, 0, 0, 0, 0});
+            plan->tasks.push_back(z);
+          }
+          continue;
+        }
+        t.np = (int64_t)plan->pieces.size() - t.p0;
+        plan->tasks.push_back(t);
+      }
+      pos = body0 + boff;
+      bi++;
+    }
+    if (any) {
+      static const uint8_t eos[8] = {0xff, 0xff, 0xff, 0xff, 0, 0, 0, 0};
+      add_copy(eos, true, 8, 8);
+    }
+    plan->part_len[p] = pos - start;
+  }
+  plan->total = pos;
+  return GZ_OK;
+}
+
+extern "C" {
+
+int gz_ipc_record_batch_message(int64_t num_rows, int32_t num_nodes, const int64_t* nodes, int32_t num_buffers,
+                                const int64_t* buffers, int64_t body_length, uint8_t* out, int64_t out_capacity,
+                                int64_t* out_length) {
+  if ((num_nodes > 0 && !nodes) || (num_buffers > 0 && !buffers) || !out_length)
+    return fail(GZ_INVALID, "gz_ipc_record_batch_message: null argument");
+  std::vector<std::pair<int64_t, int64_t>> nv, bv;
+  for (int i = 0; i < num_nodes; i++) nv.push_back({nodes[2 * i], nodes[2 * i + 1]});
+  for (int i = 0; i < num_buffers; i++) bv.push_back({buffers[2 * i], buffers[2 * i + 1]});
+  const std::vector<uint8_t> m = ipc_record_batch_message(num_rows, nv, bv, body_length);
+  *out_length = (int64_t)m.size();
+  if (out && out_capacity >= (int64_t)m.size()) memcpy(out, m.data(), m.size());
+  else if (out) return fail(GZ_INVALID, "gz_ipc_record_batch_message: output too small");
+  return GZ_OK;
+}
+
+int gz_split_ipc_prepare(gz_splitter* h, const uint8_t* schema_msg, int64_t schema_len, int32_t write_schema,
+                         int64_t* total_bytes, int64_t* partition_lengths) {
+  if (!h) return fail(GZ_INVALID, "null splitter");
+  if (write_schema && schema_len > 0 && !schema_msg) return fail(GZ_INVALID, "null schema message");
+  GZ_HIP(hipSetDevice(h->ctx->device));
+  if (h->ipc) ipc_plan_free(h->ctx, h->ipc);
+  h->ipc = new gz_ipc_plan();
+  int rc = ipc_prepare(h, schema_msg, schema_len, write_schema != 0, h->ipc);
+  if (rc) {
+    ipc_plan_free(h->ctx, h->ipc);
+    h->ipc = nullptr;
+    return rc;
+  }
+  if (total_bytes) *total_bytes = h->ipc->total;
+  if (partition_lengths) memcpy(partition_lengths, h->ipc->part_len.data(), sizeof(int64_t) * h->P);
+  return GZ_OK;
+}
+
+int gz_split_ipc_write(gz_splitter* h, void* dst, int64_t dst_bytes) {
+  if (!h) return fail(GZ_INVALID, "null splitter");
+  if (!h->ipc) return fail(GZ_INVALID, "gz_split_ipc_prepare must come first");
+  gz_ipc_plan* pl = h->ipc;
+  if (dst_bytes < pl->total) return fail(GZ_INVALID, "destination smaller than the stream");
+  if (pl->total == 0) return GZ_OK;
+  if (!dst) return fail(GZ_INVALID, "null destination");
+  gz_context* ctx = h->ctx;
+  GZ_HIP(hipSetDevice(ctx->device));
+  GZ_TRY(upload(ctx, &pl->d_blob, pl->blob.data(), std::max<size_t>(pl->blob.size(), 1)));
+  // blob pieces point into the uploaded blob
+  for (auto& t : pl->tasks)
+    if (t.pad == 1) {
+      IpcPiece& pc = pl->pieces[t.p0];
+      pc.src = (const uint8_t*)pl->d_blob.p + pc.a;
+    }
+  GZ_TRY(upload(ctx, &pl->d_pieces, pl->pieces.data(), sizeof(IpcPiece) * std::max<size_t>(pl->pieces.size(), 1)));
+  GZ_TRY(upload(ctx, &pl->d_tasks, pl->tasks.data(), sizeof(IpcTask) * std::max<size_t>(pl->tasks.size(), 1)));
+  const unsigned g = (unsigned)std::min<size_t>(pl->tasks.size(), 1 << 20);
+  {
+    KernelTimer kt(ctx, "split_ipc");
+    hipLaunchKernelGGL(ipc_assemble_kernel, dim3(g), dim3(256), 0, ctx->stream, (const IpcTask*)pl->d_tasks.p,
+                       (int64_t)pl->tasks.size(), (const IpcPiece*)pl->d_pieces.p, (uint8_t*)dst);
+  }
+  GZ_HIP(hipGetLastError());
+  GZ_HIP(hipStreamSynchronize(ctx->stream));
   return GZ_OK;
 }
 

@@ -53,3 +53,25 @@ def reference_batch_rows(counts: Sequence[int], buffer_size: int, prefer_spill: 
     if base > 0:
         out.append(base)  # Stop(): CacheRecordBatch of every partition
     return out
+
+
+def record_batch_message(num_rows: int, nodes, buffers, body_length: int) -> bytes:
+    """Encapsulated Arrow IPC message of a record batch (the metadata the device
+    partition writer puts before each body), from the C ABI's host encoder:
+    nodes = [(length, null_count)], buffers = [(offset, length)]."""
+    import ctypes
+
+    import numpy as np
+
+    from . import _lib as L
+
+    nd = np.asarray(nodes, dtype=np.int64).reshape(-1)
+    bf = np.asarray(buffers, dtype=np.int64).reshape(-1)
+    pi64 = ctypes.POINTER(ctypes.c_int64)
+    n = ctypes.c_int64()
+    args = (num_rows, len(nodes), nd.ctypes.data_as(pi64), len(buffers), bf.ctypes.data_as(pi64), body_length)
+    L.check(L.lib().gz_ipc_record_batch_message(*args, None, 0, ctypes.byref(n)))
+    out = np.zeros(n.value, np.uint8)
+    L.check(L.lib().gz_ipc_record_batch_message(*args, out.ctypes.data_as(ctypes.c_void_p), n.value,
+                                                ctypes.byref(n)))
+    return out.tobytes()

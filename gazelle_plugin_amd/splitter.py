@@ -138,6 +138,33 @@ class Splitter:
         segs = [self.segment(s) for s in range(self.num_segments())]
         return concat_segments(segs, self.output_types, self.num_partitions)
 
+    def ipc_stream(self, schema_message: bytes = b"", write_schema: bool = True):
+        """The data file Splitter::Stop writes (partitions' Arrow IPC streams, one
+        after another, uncompressed), assembled on the device and copied to the
+        host: (stream bytes as a numpy uint8 array, partition lengths in bytes).
+        ``schema_message`` is the encapsulated schema message (for example
+        ``pyarrow.Schema.serialize()``), written first in every non-empty
+        partition when ``write_schema``."""
+        import torch
+
+        total = ctypes.c_int64()
+        lengths = np.zeros(L.lib().gz_split_num_partitions(self._h), np.int64)
+        buf = np.frombuffer(schema_message, np.uint8) if schema_message else np.zeros(1, np.uint8)
+        L.check(L.lib().gz_split_ipc_prepare(self._h, buf.ctypes.data_as(ctypes.c_void_p), len(schema_message),
+                                             int(bool(write_schema)), ctypes.byref(total),
+                                             lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        dev = torch.empty(max(1, total.value), dtype=torch.uint8, device="cuda")
+        L.check(L.lib().gz_split_ipc_write(self._h, ctypes.c_void_p(dev.data_ptr()), total.value))
+        torch.cuda.synchronize()
+        return dev[: total.value].cpu().numpy(), lengths
+
+    def write_data_file(self, path: str, schema_message: bytes = b"", write_schema: bool = True) -> np.ndarray:
+        """Splitter::Stop's data file at ``path``; returns the partition lengths."""
+        data, lengths = self.ipc_stream(schema_message, write_schema)
+        with open(path, "wb") as f:
+            f.write(data.tobytes())
+        return lengths
+
     def close(self):
         if self._h:
             L.lib().gz_split_release(self._h)

@@ -174,6 +174,30 @@ int gz_split_segment_batch_counts(gz_splitter* h, int32_t seg, int32_t* num_batc
 int64_t gz_split_compute_pid_time(gz_splitter* h);
 int gz_split_release(gz_splitter* h);
 
+/* Partition writer: replaces the data file Splitter::Stop writes through its
+ * PartitionWriters (splitter.cc:70-185, :395-437) with the default
+ * compression_type UNCOMPRESSED (shuffle/type.h:44).  The stream holds, partition
+ * after partition, each non-empty partition's Arrow IPC stream: the caller's
+ * encapsulated schema message when write_schema (SplitOptions.write_schema,
+ * type.h:47; the bytes the JNI layer receives as schema_arr), the record batches
+ * cut as CacheRecordBatch cuts them (:462-608, buffer policy :889-925), and the
+ * end-of-stream marker; an empty partition writes nothing.  prepare (after
+ * gz_split_stop) lays the stream out and returns its size and the per-partition
+ * lengths (Splitter::PartitionLengths, splitter.h:114); write assembles it into
+ * a device buffer of at least total_bytes.  offheap_per_task > 0 (batch size
+ * from the first batch's row size) is NotImplemented here. */
+int gz_split_ipc_prepare(gz_splitter* h, const uint8_t* schema_msg, int64_t schema_len, int32_t write_schema,
+                         int64_t* total_bytes, int64_t* partition_lengths /* P */);
+int gz_split_ipc_write(gz_splitter* h, void* dst_device, int64_t dst_bytes);
+/* The encapsulated IPC message (continuation token, metadata length, Arrow
+ * Message flatbuffer with a RecordBatch header, MetadataVersion V5, padding) of
+ * a record batch with the given field nodes (length, null_count pairs) and
+ * buffers (offset, length pairs); host only.  *out_length receives the size;
+ * out may be NULL to query it. */
+int gz_ipc_record_batch_message(int64_t num_rows, int32_t num_nodes, const int64_t* nodes, int32_t num_buffers,
+                                const int64_t* buffers, int64_t body_length, uint8_t* out, int64_t out_capacity,
+                                int64_t* out_length);
+
 /* ---- hash aggregate ----------------------------------------------------------
  * Replaces HashAggregateKernel + HashAggregateResultIterator<number>
  * (ext/hash_aggregate_kernel.cc:51-928, kernels_ext.h:285-313).  The Gandiva

@@ -1,0 +1,97 @@
+"""The partition writer (Splitter::Stop's data file) on the GPU: every
+partition's bytes are an Arrow IPC stream that pyarrow reads back, its record
+batches cut as the reference's buffer policy cuts them, and its rows equal the
+oracle's split, value for value."""
+import numpy as np
+import pyarrow as pa
+import pytest
+
+import oracle
+from gazelle_plugin_amd import types as T
+from gazelle_plugin_amd.batch import DeviceBatch
+from gazelle_plugin_amd.partition_writer import reference_batch_rows
+from gazelle_plugin_amd.splitter import SplitOptions, Splitter
+from splitcheck import random_batches
+
+pytestmark = pytest.mark.gpu
+
+ARROW = {T.INT64: pa.int64(), T.INT32: pa.int32(), T.DOUBLE: pa.float64(), T.STRING: pa.string(),
+         T.BOOL: pa.bool_(), T.INT8: pa.int8(), T.UINT16: pa.uint16(), T.NA: pa.null(), T.BINARY: pa.binary(),
+         T.DATE32: pa.date32()}
+
+
+def _bits(bm, start, n):
+    return np.unpackbits(np.frombuffer(bm.tobytes(), np.uint8), bitorder="little")[start:start + n].astype(bool)
+
+
+def _check_partition(arrays, exp, types, p):
+    r0, r1 = int(exp["row_start"][p]), int(exp["row_start"][p + 1])
+    n = r1 - r0
+    for c, t in enumerate(types):
+        a = arrays[c]
+        assert len(a) == n
+        col = exp["columns"][c]
+        if t == T.NA:
+            assert a.null_count == n
+            continue
+        valid = np.ones(n, bool) if col.get("validity") is None else _bits(col["validity"], r0, n)
+        np.testing.assert_array_equal(np.asarray(a.is_valid()), valid, err_msg=f"validity p={p} c={c}")
+        if T.is_binary(t):
+            off = np.asarray(col["offsets"], np.int64)[r0 + p: r1 + p + 1]
+            bs = int(col["byte_start"][p])
+            data = np.asarray(col["data"], np.uint8)
+            got = (a.cast(pa.binary()) if t == T.STRING else a).to_pylist()  # bytes, no utf8 decoding
+            for i in range(n):
+                if valid[i]:
+                    assert got[i] == data[bs + off[i]: bs + off[i + 1]].tobytes(), f"bytes p={p} c={c} row {i}"
+        elif t == T.BOOL:
+            vals = np.asarray(a.fill_null(False), bool)
+            np.testing.assert_array_equal(vals[valid], _bits(col["values"], r0, n)[valid])
+        else:
+            w = T.width(t)
+            got = np.frombuffer(a.buffers()[1], np.uint8)[a.offset * w:(a.offset + n) * w]
+            exp_v = np.asarray(col["values"]).view(np.uint8).reshape(-1)[r0 * w:r1 * w]
+            np.testing.assert_array_equal(got, exp_v, err_msg=f"values p={p} c={c}")
+
+
+@pytest.mark.parametrize("batched", [True, False])
+@pytest.mark.parametrize("buffer_size", [4096, 300])
+def test_data_file_decodes_to_the_split(ctx, batched, buffer_size):
+    types = [T.INT64, T.INT32, T.DOUBLE, T.STRING, T.BOOL, T.INT8, T.UINT16, T.NA, T.BINARY, T.DATE32]
+    batches = random_batches(types, [5000, 0, 1, 4096, 777, 3000], seed=21, null_frac=0.15)
+    P = 11
+    exp = oracle.split(batches, types, P, [0, 3])
+    pids = oracle.split_pids(batches, types, P, [0, 3])
+    sp = Splitter.Make(ctx, "hash", types, P, [0, 3], SplitOptions(buffer_size=buffer_size))
+    dev = [DeviceBatch.from_host(b) for b in batches]
+    if batched:
+        sp.SplitBatches(dev)
+    else:
+        for d in dev:
+            sp.Split(d)
+    sp.Stop()
+    schema = pa.schema([(f"c{i}", ARROW[t]) for i, t in enumerate(types)])
+    data, lengths = sp.ipc_stream(schema.serialize().to_pybytes())
+    sp.close()
+    assert int(lengths.sum()) == len(data)
+    pos = 0
+    row0 = 0
+    for p in range(P):
+        n = int(exp["row_start"][p + 1] - exp["row_start"][p])
+        if n == 0:
+            assert lengths[p] == 0
+            continue
+        part = data[pos:pos + lengths[p]].tobytes()
+        pos += int(lengths[p])
+        reader = pa.ipc.open_stream(pa.py_buffer(part))
+        assert reader.schema.equals(schema)
+        rbs = list(reader)
+        # record batch cuts: the reference's buffer policy over the rows per input batch
+        counts, off = [], 0
+        for b in batches:
+            counts.append(int((pids[off:off + b.num_rows] == p).sum()))
+            off += b.num_rows
+        assert [rb.num_rows for rb in rbs] == reference_batch_rows(counts, buffer_size)
+        table = pa.Table.from_batches(rbs, schema=schema)
+        _check_partition([table.column(c).combine_chunks() for c in range(len(types))], exp, types, p)
+    assert pos == len(data)
