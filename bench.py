@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-writer", action="store_true", help="skip the partition writer measurement")
     ap.add_argument("--tile-rows", type=int, default=0, help="split tile rows (0 = default)")
     ap.add_argument("--sections", default="split,agg,join,serve",
                     help="comma list of split, agg, join, serve (split always runs)")
@@ -190,6 +191,42 @@ def bench_split(args, ctx, stream):
     plain_rows_s = nb * datagen.BATCH_ROWS / (plain_ms / 1e3)
     rows_per_s = N / (ms_per_step / 1e3)
 
+    # partition writer: the data file (per-partition Arrow IPC streams) of one split
+    writer = None
+    if not args.no_writer:
+        import ctypes
+
+        import numpy as np
+        import pyarrow as pa
+
+        from gazelle_plugin_amd import _lib as L
+
+        sp = Splitter.Make(ctx, "hash", datagen.SPLIT_TYPES, P, [0], opts)
+        sp.SplitBatches(tab.batches, arr)
+        sp.Stop()
+        schema = pa.schema([("key", pa.int64()), ("i32", pa.int32()), ("f64", pa.float64()), ("s", pa.string())])
+        sm = np.frombuffer(schema.serialize().to_pybytes(), np.uint8)
+        total = ctypes.c_int64()
+        lengths = np.zeros(P, np.int64)
+        t2 = time.perf_counter()
+        L.check(L.lib().gz_split_ipc_prepare(sp._h, sm.ctypes.data_as(ctypes.c_void_p), len(sm), 1,
+                                             ctypes.byref(total), lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        t3 = time.perf_counter()
+        dst = torch.empty(total.value, dtype=torch.uint8, device="cuda")
+        ctx.set_profiling(True)
+        ctx.reset_kernel_times()
+        for _ in range(3):
+            L.check(L.lib().gz_split_ipc_write(sp._h, ctypes.c_void_p(dst.data_ptr()), total.value))
+        wk = ctx.kernel_time("split_ipc")
+        ctx.set_profiling(False)
+        wms = wk[0] / max(1, wk[1])
+        sp.close()
+        writer = {"stream_bytes": total.value, "prepare_ms_host": (t3 - t2) * 1e3, "assemble_ms": wms,
+                  "assemble_GBps": 2 * total.value / (wms / 1e3) / 1e9,
+                  "note": "uncompressed Arrow IPC per partition (schema, record batches of <= 4096 rows, EOS); "
+                          "GB/s counts bytes read + written"}
+        del dst
+
     traffic = load_traffic("split_scatter_traffic.json")
     roof = {"bound": "hbm", "achieved": algo_bytes / (scat_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
@@ -202,7 +239,7 @@ def bench_split(args, ctx, stream):
            "step_GBps": algo_bytes / (ms_per_step / 1e3) / 1e9,
            "kernels_ms_avg": {k: v[0] / max(1, v[1]) for k, v in kt.items()},
            "plain": {"rows_per_s": plain_rows_s, "batches": nb, "ms": plain_ms},
-           "batched_over_plain": rows_per_s / plain_rows_s, "roofline": roof}
+           "batched_over_plain": rows_per_s / plain_rows_s, "roofline": roof, "partition_writer": writer}
     del tab, arr
     return res
 
@@ -468,6 +505,7 @@ def main():
         "batched_over_plain": split["batched_over_plain"],
         "wall_s": split["wall_s"],
         "roofline": split["roofline"],
+        "partition_writer": split["partition_writer"],
         "cpu_baseline": cpu,
     }
     out.update(extra)

@@ -1394,13 +1394,18 @@ struct IpcTask {
   int64_t total;   // OFFS: the last entry (bytes of the batch)
 };
 
-__device__ __forceinline__ uint32_t ipc_bits32(const uint8_t* src, int64_t bit) {
-  // 32 bits of a bitmap from an arbitrary bit position (5 byte loads)
+__device__ __forceinline__ uint32_t ipc_bits32(const uint8_t* src, int64_t bit, int cnt) {
+  // cnt (1..32) bits of a bitmap from an arbitrary bit position, in the low
+  // bits; only the bytes holding those bits are read (a bitmap may end at the
+  // byte holding its last bit)
   const uint8_t* p = src + (bit >> 3);
+  const int nbytes = ((int)(bit & 7) + cnt + 7) >> 3;  // 1..5
   uint64_t w = 0;
 #pragma unroll
-  for (int k = 0; k < 5; k++) w |= (uint64_t)gld(p + k) << (8 * k);
-  return (uint32_t)(w >> (bit & 7));
+  for (int k = 0; k < 5; k++)
+    if (k < nbytes) w |= (uint64_t)gld(p + k) << (8 * k);
+  w >>= (bit & 7);
+  return cnt < 32 ? (uint32_t)w & ((1u << cnt) - 1) : (uint32_t)w;
 }
 
 __global__ __launch_bounds__(256) void ipc_assemble_kernel(const IpcTask* tasks, int64_t ntasks,
@@ -1410,7 +1415,23 @@ __global__ __launch_bounds__(256) void ipc_assemble_kernel(const IpcTask* tasks,
     uint8_t* dst = out + t.dst;
     if (t.kind == IPC_COPY) {
       const uint8_t* src = pieces[t.p0].src;
-      for (int64_t i = threadIdx.x; i < t.out; i += blockDim.x) gst(dst + i, i < t.n ? gld(src + i) : (uint8_t)0);
+      // widest aligned unit for [0, n), then the tail bytes and the zero padding
+      const uintptr_t al = ((uintptr_t)src | (uintptr_t)dst) & 15;
+      int64_t done = 0;
+      if (t.n > 0 && al == 0) {
+        const int64_t nv = t.n >> 4;
+        for (int64_t i = threadIdx.x; i < nv; i += blockDim.x) gst((u32x4*)dst + i, gld((const u32x4*)src + i));
+        done = nv << 4;
+      } else if (t.n > 0 && (al & 7) == 0) {
+        const int64_t nv = t.n >> 3;
+        for (int64_t i = threadIdx.x; i < nv; i += blockDim.x) gst((uint64_t*)dst + i, gld((const uint64_t*)src + i));
+        done = nv << 3;
+      } else if (t.n > 0 && (al & 3) == 0) {
+        const int64_t nv = t.n >> 2;
+        for (int64_t i = threadIdx.x; i < nv; i += blockDim.x) gst((uint32_t*)dst + i, gld((const uint32_t*)src + i));
+        done = nv << 2;
+      }
+      for (int64_t i = done + threadIdx.x; i < t.out; i += blockDim.x) gst(dst + i, i < t.n ? gld(src + i) : (uint8_t)0);
     } else if (t.kind == IPC_BITS) {
       // output word w holds bits [32w, 32w + 32); pieces are ordered by b
       const IpcPiece* pc = pieces + t.p0;
@@ -1428,7 +1449,7 @@ __global__ __launch_bounds__(256) void ipc_assemble_kernel(const IpcTask* tasks,
             if (s1 <= s0) continue;
             const int cnt = (int)(s1 - s0);
             // a segment without a validity buffer: every row valid
-            uint32_t v = pc[k].src ? ipc_bits32(pc[k].src, pc[k].a + (s0 - pc[k].b)) : 0xffffffffu;
+            uint32_t v = pc[k].src ? ipc_bits32(pc[k].src, pc[k].a + (s0 - pc[k].b), cnt) : 0xffffffffu;
             if (cnt < 32) v &= (1u << cnt) - 1;
             word |= v << (s0 - lo);
           }
@@ -1469,14 +1490,20 @@ __global__ __launch_bounds__(256) void ipc_null_count_kernel(const IpcPiece* pie
     const IpcPiece pc = pieces[i];
     unsigned long long ones = 0;
     for (int64_t o = (int64_t)lane * 32; o < pc.n; o += 32 * kWave) {
-      uint32_t v = ipc_bits32(pc.src, pc.a + o);
-      const int64_t cnt = min<int64_t>(32, pc.n - o);
-      if (cnt < 32) v &= (1u << cnt) - 1;
+      const int cnt = (int)min<int64_t>(32, pc.n - o);
+      const uint32_t v = ipc_bits32(pc.src, pc.a + o, cnt);
       ones += __popc(v);
     }
     ones = wave_reduce_sum(ones);
     if (lane == 0) out[i] = (unsigned long long)pc.n - ones;
   }
+}
+
+// Reads int32 values at a list of device addresses (the utf8 offsets at both
+// ends of every span of a partition writer batch) in one launch.
+__global__ void ipc_gather_i32_kernel(const int32_t* const* addr, int64_t n, int32_t* out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = gld(addr[i]);
 }
 
 // A flatbuffer built back to front (the layout the format's own builders
@@ -1732,6 +1759,35 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
     ctx_free(ctx, &dvp);
     ctx_free(ctx, &dnull);
   }
+  // utf8 offsets at both ends of every (batch, binary column, span), gathered at once
+  std::vector<const int32_t*> oaddr;
+  for (size_t b2 = 0; b2 < batches.size(); b2++) {
+    const Batch& bt = batches[b2];
+    for (int c = 0; c < nout; c++) {
+      if (!type_is_binary(h->out_types[c])) continue;
+      for (const Span& sp : bt.spans) {
+        SplitSegment* sg = h->segments[sp.seg];
+        const int32_t* o = (const int32_t*)sg->offsets[c].p + sg->h_row_start[bt.p] + bt.p + sp.row0;
+        oaddr.push_back(o);
+        oaddr.push_back(o + sp.rows);
+      }
+    }
+  }
+  std::vector<int32_t> oval(std::max<size_t>(oaddr.size(), 1), 0);
+  if (!oaddr.empty()) {
+    DBuf da, dv;
+    GZ_TRY(ctx_alloc(ctx, sizeof(int32_t*) * oaddr.size(), &da));
+    GZ_TRY(ctx_alloc(ctx, sizeof(int32_t) * oaddr.size(), &dv));
+    GZ_HIP(hipMemcpyAsync(da.p, oaddr.data(), sizeof(int32_t*) * oaddr.size(), hipMemcpyHostToDevice, st));
+    const unsigned g = (unsigned)std::min<size_t>((oaddr.size() + 255) / 256, 8192);
+    hipLaunchKernelGGL(ipc_gather_i32_kernel, dim3(g), dim3(256), 0, st, (const int32_t* const*)da.p,
+                       (int64_t)oaddr.size(), (int32_t*)dv.p);
+    GZ_HIP(hipMemcpyAsync(oval.data(), dv.p, sizeof(int32_t) * oaddr.size(), hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    ctx_free(ctx, &da);
+    ctx_free(ctx, &dv);
+  }
+  size_t ov = 0;  // next pair of oval
   // ---- pass 2: layout of the stream, metadata, tasks
   plan->part_len.assign(P, 0);
   plan->tasks.clear();
@@ -1807,16 +1863,10 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
       for (auto& pd : body) {
         if (pd.kind != IPC_COPY || !type_is_binary(h->out_types[pd.c])) continue;
         int64_t total = 0;
-        for (const Span& sp : bt.spans) {
-          SplitSegment* sg = h->segments[sp.seg];
-          const int32_t* o = (const int32_t*)sg->offsets[pd.c].p + sg->h_row_start[bt.p] + bt.p + sp.row0;
-          int32_t ends[2];
-          GZ_HIP(hipMemcpyAsync(&ends[0], o, 4, hipMemcpyDeviceToHost, st));
-          GZ_HIP(hipMemcpyAsync(&ends[1], o + sp.rows, 4, hipMemcpyDeviceToHost, st));
-          GZ_HIP(hipStreamSynchronize(st));
-          span_bytes.push_back(ends[0]);
-          span_bytes.push_back((int64_t)ends[1] - ends[0]);
-          total += (int64_t)ends[1] - ends[0];
+        for (size_t k = 0; k < bt.spans.size(); k++, ov += 2) {
+          span_bytes.push_back(oval[ov]);
+          span_bytes.push_back((int64_t)oval[ov + 1] - oval[ov]);
+          total += (int64_t)oval[ov + 1] - oval[ov];
         }
         pd.n = total;
       }
