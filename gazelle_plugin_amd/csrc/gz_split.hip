@@ -2083,3 +2083,129 @@ int gz_split_ipc_write(gz_splitter* h, void* dst, int64_t dst_bytes) {
 }
 
 }  // extern "C"
+
+// ======================================================= shuffle read (IPC)
+// The reading side of the shuffle for uncompressed streams: what the JVM's
+// SchemaAwareArrowCompressedStreamReader + ShuffleDecompressionJniWrapper
+// (jni_wrapper.cc:1236-1322; decompression is the identity for UNCOMPRESSED)
+// produce, as device batches.  The host parses the message metadata; every
+// buffer of every record batch is addressed inside the device copy of the
+// stream (zero copy: IPC bodies keep buffers 8-byte aligned).
+namespace gz {
+
+// Bounds-checked reads of a flatbuffer (the stream is untrusted input).
+struct FlatView {
+  const uint8_t* b;
+  size_t n;
+  bool ok = true;
+  template <typename T>
+  T get(size_t pos) {
+    if (pos + sizeof(T) > n) {
+      ok = false;
+      return T(0);
+    }
+    T v;
+    memcpy(&v, b + pos, sizeof(T));
+    return v;
+  }
+  // field `id` of the table at `tpos`: its position, or 0 when absent
+  size_t field(size_t tpos, int id) {
+    const int32_t soff = get<int32_t>(tpos);
+    const int64_t vt = (int64_t)tpos - soff;
+    if (vt < 0 || (size_t)vt + 4 > n) {
+      ok = false;
+      return 0;
+    }
+    const uint16_t vsize = get<uint16_t>((size_t)vt);
+    if (4 + 2 * (size_t)id + 2 > vsize) return 0;
+    const uint16_t fo = get<uint16_t>((size_t)vt + 4 + 2 * id);
+    return fo ? tpos + fo : 0;
+  }
+  size_t deref(size_t fpos) { return fpos ? fpos + get<uint32_t>(fpos) : 0; }
+};
+
+}  // namespace gz
+
+extern "C" {
+
+int gz_ipc_read_batches(const uint8_t* stream, int64_t len, int64_t device_base, int32_t num_cols,
+                        const int32_t* col_types, int32_t max_batches, int64_t* batch_rows, int64_t* buf_addrs,
+                        int64_t* buf_sizes, int32_t* num_batches) {
+  if (!stream || len < 0 || !num_batches || (num_cols > 0 && !col_types))
+    return fail(GZ_INVALID, "gz_ipc_read_batches: null argument");
+  int nbufs = 0;
+  for (int c = 0; c < num_cols; c++) {
+    if (!type_supported(col_types[c])) return fail(GZ_NOT_IMPLEMENTED, "column type " + std::to_string(col_types[c]));
+    nbufs += type_num_bufs(col_types[c]);
+  }
+  int64_t pos = 0;
+  int32_t nb = 0;
+  while (pos + 4 <= len) {
+    int32_t w0;
+    memcpy(&w0, stream + pos, 4);
+    int64_t mlen;
+    int64_t mstart;
+    if (w0 == -1) {  // continuation token, then the metadata length
+      if (pos + 8 > len) return fail(GZ_IO_ERROR, "truncated IPC message header");
+      int32_t l;
+      memcpy(&l, stream + pos + 4, 4);
+      mlen = l;
+      mstart = pos + 8;
+    } else {  // the pre-0.15 framing: the length alone
+      mlen = w0;
+      mstart = pos + 4;
+    }
+    if (mlen == 0) break;  // end of stream
+    if (mlen < 0 || mstart + mlen > len) return fail(GZ_IO_ERROR, "IPC message runs past the stream");
+    FlatView fv{stream + mstart, (size_t)mlen};
+    const size_t root = fv.get<uint32_t>(0);
+    const size_t htype_f = fv.field(root, 1), header_f = fv.field(root, 2), blen_f = fv.field(root, 3);
+    const uint8_t htype = htype_f ? fv.get<uint8_t>(htype_f) : 0;
+    const int64_t body_len = blen_f ? fv.get<int64_t>(blen_f) : 0;
+    const int64_t body = mstart + mlen;
+    if (!fv.ok || body_len < 0 || body + body_len > len) return fail(GZ_IO_ERROR, "malformed IPC message");
+    if (htype == 2) return fail(GZ_NOT_IMPLEMENTED, "dictionary batches");
+    if (htype == 3) {  // RecordBatch
+      const size_t rb = fv.deref(header_f);
+      const size_t len_f = fv.field(rb, 0), nodes_f = fv.field(rb, 1), bufs_f = fv.field(rb, 2);
+      if (fv.field(rb, 3)) return fail(GZ_NOT_IMPLEMENTED, "compressed IPC bodies (only UNCOMPRESSED)");
+      const int64_t rows = len_f ? fv.get<int64_t>(len_f) : 0;
+      const size_t nodes = fv.deref(nodes_f), bufs = fv.deref(bufs_f);
+      const uint32_t nn = nodes ? fv.get<uint32_t>(nodes) : 0, nbuf = bufs ? fv.get<uint32_t>(bufs) : 0;
+      if (!fv.ok || (int)nn != num_cols) return fail(GZ_INVALID, "record batch field count does not match the types");
+      int expect = 0;
+      for (int c = 0; c < num_cols; c++)
+        expect += col_types[c] == GZ_TYPE_NA ? 0 : type_is_binary(col_types[c]) ? 3 : 2;
+      if ((int)nbuf != expect) return fail(GZ_INVALID, "record batch buffer count does not match the types");
+      if (nb < max_batches && batch_rows && buf_addrs && buf_sizes) {
+        batch_rows[nb] = rows;
+        int64_t* a = buf_addrs + (int64_t)nb * nbufs;
+        int64_t* s = buf_sizes + (int64_t)nb * nbufs;
+        uint32_t bi = 0;
+        int k = 0;
+        for (int c = 0; c < num_cols; c++) {
+          const int t = col_types[c];
+          if (t == GZ_TYPE_NA) {
+            a[k] = 0, s[k] = 0, k++;
+            a[k] = 0, s[k] = 0, k++;
+            continue;
+          }
+          const int m = type_is_binary(t) ? 3 : 2;
+          for (int j = 0; j < m; j++, bi++, k++) {
+            // vector elements start 4 bytes after its length; each Buffer is 16 bytes
+            const int64_t off = fv.get<int64_t>(bufs + 4 + 16 * bi), blen = fv.get<int64_t>(bufs + 4 + 16 * bi + 8);
+            if (!fv.ok || off < 0 || blen < 0 || off + blen > body_len) return fail(GZ_IO_ERROR, "buffer outside the body");
+            a[k] = blen > 0 ? device_base + body + off : 0;
+            s[k] = blen;
+          }
+        }
+      }
+      nb++;
+    }
+    pos = body + body_len;
+  }
+  *num_batches = nb;
+  return GZ_OK;
+}
+
+}  // extern "C"

@@ -198,6 +198,23 @@ int gz_ipc_record_batch_message(int64_t num_rows, int32_t num_nodes, const int64
                                 const int64_t* buffers, int64_t body_length, uint8_t* out, int64_t out_capacity,
                                 int64_t* out_length);
 
+/* Shuffle read: replaces, for uncompressed streams, the reading side of the
+ * shuffle (SchemaAwareArrowCompressedStreamReader + ShuffleDecompressionJniWrapper
+ * .decompress, jni_wrapper.cc:1236-1322, whose decompression is the identity for
+ * UNCOMPRESSED).  Parses an Arrow IPC stream (host copy in `stream`, the
+ * partition's bytes of a data file) and lays out each record batch in the gz_batch
+ * buffer convention, every address inside the device copy of the same bytes
+ * starting at device_base (zero copy; a zero-length buffer gives address 0).
+ * Per batch: batch_rows[b], then num_bufs addresses and sizes at
+ * buf_addrs/buf_sizes[b * num_bufs ...], where num_bufs counts 3 buffers per
+ * STRING/BINARY column and 2 per other column (NA: two empty buffers).
+ * At most max_batches are written; *num_batches receives the stream's count.
+ * Schema messages are skipped; compressed bodies and dictionaries are
+ * NotImplemented; malformed streams are IOError. */
+int gz_ipc_read_batches(const uint8_t* stream, int64_t len, int64_t device_base, int32_t num_cols,
+                        const int32_t* col_types, int32_t max_batches, int64_t* batch_rows, int64_t* buf_addrs,
+                        int64_t* buf_sizes, int32_t* num_batches);
+
 /* ---- hash aggregate ----------------------------------------------------------
  * Replaces HashAggregateKernel + HashAggregateResultIterator<number>
  * (ext/hash_aggregate_kernel.cc:51-928, kernels_ext.h:285-313).  The Gandiva
