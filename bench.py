@@ -49,6 +49,7 @@ def parse():
                     help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-writer", action="store_true", help="skip the partition writer measurement")
+    ap.add_argument("--no-reader", action="store_true", help="skip the shuffle read measurement")
     ap.add_argument("--tile-rows", type=int, default=0, help="split tile rows (0 = default)")
     ap.add_argument("--sections", default="split,agg,join,serve",
                     help="comma list of split, agg, join, serve (split always runs)")
@@ -134,6 +135,71 @@ def cpu_baseline_agg(target_s: float):
 
 
 # ---------------------------------------------------------------- sections
+def bench_shuffle_read(ctx, dev_file, total, lengths, schema):
+    """Reading the data file back: (1) the uncompressed partition streams parsed
+    into device batches (host metadata only, zero copy); (2) an LZ4_FRAME stream
+    of a sample of the same record batches (compressed by pyarrow on the host)
+    decoded on the device, against pyarrow's threaded CPU decode of it."""
+    import numpy as np
+    import pyarrow as pa
+
+    from gazelle_plugin_amd import datagen
+    from gazelle_plugin_amd.shuffle_reader import ShuffleReader
+
+    host = dev_file[:total].cpu().numpy()
+    rd = ShuffleReader(datagen.SPLIT_TYPES, ctx)
+    t0 = time.perf_counter()
+    parts = rd.read_data_file(host, lengths, device=dev_file)
+    t1 = time.perf_counter()
+    nbatches = sum(len(p) for p in parts)
+    del parts
+    # LZ4 sample: record batches of the first partitions, about 256 MB decoded
+    rbs, size, off = [], 0, 0
+    for n in lengths:
+        n = int(n)
+        if n:
+            rbs += list(pa.ipc.open_stream(pa.py_buffer(host[off:off + n])))
+            size += n
+        off += n
+        if size >= 256 << 20:
+            break
+    sink = pa.BufferOutputStream()
+    with pa.ipc.new_stream(sink, schema, options=pa.ipc.IpcWriteOptions(compression="lz4")) as w:
+        for rb in rbs:
+            w.write_batch(rb)
+    comp = np.frombuffer(sink.getvalue(), np.uint8)
+    dev = torch_upload(comp)
+    rd.read(comp, device=dev)  # warm
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    t2 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        out = rd.read(comp, device=dev)
+    t3 = time.perf_counter()
+    kt = ctx.kernel_time("ipc_lz4_decode")
+    ctx.set_profiling(False)
+    decoded = int(sum(sum(b.sizes[k] for k in range(b.num_bufs)) for b in out))
+    kms = kt[0] / max(1, kt[1])
+    t4 = time.perf_counter()
+    pa.ipc.open_stream(pa.py_buffer(comp)).read_all()
+    t5 = time.perf_counter()
+    return {"partitions_read": int((lengths > 0).sum()), "batches": nbatches,
+            "parse_ms_host": (t1 - t0) * 1e3,
+            "lz4": {"record_batches": len(rbs), "compressed_bytes": int(len(comp)), "decoded_bytes": decoded,
+                    "decode_kernel_ms": kms, "decode_GBps": decoded / (kms / 1e3) / 1e9,
+                    "read_ms_wall": (t3 - t2) / reps * 1e3,
+                    "pyarrow_cpu_read_ms": (t5 - t4) * 1e3,
+                    "note": "GB/s counts decoded bytes; wall includes the host parse, arena allocation and sync"}}
+
+
+def torch_upload(a):
+    import numpy as np
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
 def bench_split(args, ctx, stream):
     import torch
 
@@ -225,6 +291,8 @@ def bench_split(args, ctx, stream):
                   "assemble_GBps": 2 * total.value / (wms / 1e3) / 1e9,
                   "note": "uncompressed Arrow IPC per partition (schema, record batches of <= 4096 rows, EOS); "
                           "GB/s counts bytes read + written"}
+        if not args.no_reader:
+            writer["shuffle_read"] = bench_shuffle_read(ctx, dst, total.value, lengths, schema)
         del dst
 
     traffic = load_traffic("split_scatter_traffic.json")

@@ -198,22 +198,37 @@ int gz_ipc_record_batch_message(int64_t num_rows, int32_t num_nodes, const int64
                                 const int64_t* buffers, int64_t body_length, uint8_t* out, int64_t out_capacity,
                                 int64_t* out_length);
 
-/* Shuffle read: replaces, for uncompressed streams, the reading side of the
- * shuffle (SchemaAwareArrowCompressedStreamReader + ShuffleDecompressionJniWrapper
- * .decompress, jni_wrapper.cc:1236-1322, whose decompression is the identity for
- * UNCOMPRESSED).  Parses an Arrow IPC stream (host copy in `stream`, the
- * partition's bytes of a data file) and lays out each record batch in the gz_batch
- * buffer convention, every address inside the device copy of the same bytes
- * starting at device_base (zero copy; a zero-length buffer gives address 0).
+/* Shuffle read: replaces the reading side of the shuffle
+ * (SchemaAwareArrowCompressedStreamReader + ShuffleDecompressionJniWrapper.make /
+ * .decompress, jni_wrapper.cc:1236-1322, DecompressBuffers jni_common.h:471-560).
+ * Parses an Arrow IPC stream (host copy in `stream`: one partition's bytes of a
+ * data file) and lays out each record batch in the gz_batch buffer convention.
+ * Buffers of uncompressed batches are addressed inside the device copy of the
+ * same bytes, which starts at device_base (zero copy).  Buffers of LZ4_FRAME
+ * compressed batches (the Spark "lz4" codec) are addressed in a decode arena
+ * starting at arena_base, of *arena_bytes bytes, which gz_ipc_decompress fills;
+ * a buffer whose length prefix is -1 is stored raw and stays in the stream.  A
+ * zero-length buffer gives address 0.
  * Per batch: batch_rows[b], then num_bufs addresses and sizes at
  * buf_addrs/buf_sizes[b * num_bufs ...], where num_bufs counts 3 buffers per
  * STRING/BINARY column and 2 per other column (NA: two empty buffers).
  * At most max_batches are written; *num_batches receives the stream's count.
- * Schema messages are skipped; compressed bodies and dictionaries are
- * NotImplemented; malformed streams are IOError. */
-int gz_ipc_read_batches(const uint8_t* stream, int64_t len, int64_t device_base, int32_t num_cols,
-                        const int32_t* col_types, int32_t max_batches, int64_t* batch_rows, int64_t* buf_addrs,
-                        int64_t* buf_sizes, int32_t* num_batches);
+ * Schema messages are skipped; dictionaries, ZSTD and LZ4 frames with a
+ * dictionary id are NotImplemented; malformed streams are IOError.
+ * Host-only: no device work. */
+int gz_ipc_read_batches(const uint8_t* stream, int64_t len, int64_t device_base, int64_t arena_base,
+                        int32_t num_cols, const int32_t* col_types, int32_t max_batches, int64_t* batch_rows,
+                        int64_t* buf_addrs, int64_t* buf_sizes, int32_t* num_batches, int64_t* arena_bytes);
+
+/* Decodes every LZ4_FRAME compressed buffer of the stream (host copy `stream`,
+ * device copy at device_base) into `arena` (device, arena_capacity >= the
+ * *arena_bytes gz_ipc_read_batches reports), on the context's stream, and
+ * waits for it.  Like DecompressBuffer (jni_common.h:471-492): corrupt LZ4 data
+ * is IOError; a buffer that decodes to another length than its prefix is
+ * Invalid ("Failed to fully decompress buffer ...").  Frame and block
+ * checksums, when a frame has them, are skipped, not verified. */
+int gz_ipc_decompress(gz_context* ctx, const uint8_t* stream, int64_t len, int64_t device_base, int32_t num_cols,
+                      const int32_t* col_types, void* arena, int64_t arena_capacity);
 
 /* ---- hash aggregate ----------------------------------------------------------
  * Replaces HashAggregateKernel + HashAggregateResultIterator<number>

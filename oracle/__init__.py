@@ -10,6 +10,7 @@ each one follows:
   agg.c    -- HashAggregateResultIterator + actions  (ext/hash_aggregate_kernel.cc, ext/actions_impl.cc)
   join.c   -- HashRelation / unsafeHashMap + probes  (codegen/common/hash_relation.h,
               third_party/row_wise_memory/hashMap.h, ext/conditioned_probe_kernel.cc)
+  lz4.c    -- LZ4_FRAME decoding of shuffle buffers   (jni/jni_common.h:471-492, lz4 frame format)
 
 Pins (what checks the oracle itself) are listed in oracle/README.md.
 """
@@ -74,6 +75,8 @@ def lib():
         L.gzo_rel_num_keys.restype = ctypes.c_int64
         L.gzo_rel_num_keys.argtypes = [ctypes.c_void_p]
         L.gzo_rel_has_null.argtypes = [ctypes.c_void_p]
+        L.gzo_lz4f_decompress.restype = ctypes.c_int64
+        L.gzo_lz4f_decompress.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64]
         L.gzo_probe.restype = ctypes.c_int64
         L.gzo_probe.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                 ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
@@ -305,3 +308,34 @@ def join(build_keys: Sequence, probe_keys, key_type: int, join_type: int, null_a
         return outs
     finally:
         L.gzo_rel_free(r)
+
+
+# ------------------------------------------------------------------ lz4
+
+def lz4f_decompress(frame, capacity: int) -> np.ndarray:
+    """LZ4 frame(s) -> bytes (at most ``capacity``); ValueError on malformed input."""
+    src = np.frombuffer(bytes(frame), np.uint8) if not isinstance(frame, np.ndarray) else \
+        np.ascontiguousarray(frame, np.uint8)
+    out = np.zeros(max(1, capacity), np.uint8)
+    n = lib().gzo_lz4f_decompress(_ptr(src) if len(src) else None, len(src), _ptr(out), capacity)
+    if n < 0:
+        raise ValueError("malformed LZ4 frame")
+    return out[:n]
+
+
+def ipc_decompress_buffer(buf) -> np.ndarray:
+    """One compressed IPC body buffer (int64 uncompressed length, then an LZ4
+    frame; length -1: stored as is), as DecompressBuffer (jni_common.h:471-492)
+    reads it."""
+    b = np.frombuffer(bytes(buf), np.uint8)
+    if len(b) == 0:
+        return b
+    if len(b) < 8:
+        raise ValueError("compressed buffer shorter than its length prefix")
+    n = int(b[:8].view(np.int64)[0])
+    if n == -1:
+        return b[8:].copy()
+    out = lz4f_decompress(b[8:], n)
+    if len(out) != n:
+        raise ValueError(f"decompressed {len(out)} bytes, expected {n}")
+    return out

@@ -72,7 +72,7 @@ def test_addresses_match_pyarrow(legacy, null_frac):
     schema, rbs = _batches(TYPES, [1000, 1, 0, 4096, 77], seed=3, null_frac=null_frac)
     data = _stream(rbs, schema, use_legacy_format=legacy)
     base = data.ctypes.data
-    rows, addrs, sizes = parse_stream(data, base, TYPES)
+    rows, addrs, sizes, _ = parse_stream(data, base, TYPES)
     read = list(pa.ipc.open_stream(pa.py_buffer(data)))
     assert rows.tolist() == [rb.num_rows for rb in rbs] == [rb.num_rows for rb in read]
     for b, rb in enumerate(read):
@@ -85,7 +85,7 @@ def test_addresses_match_pyarrow(legacy, null_frac):
 def test_offsets_without_base():
     schema, rbs = _batches([T.INT32, T.STRING], [10], seed=4, null_frac=0.5)
     data = _stream(rbs, schema)
-    rows, addrs, sizes = parse_stream(data, 0, [T.INT32, T.STRING])
+    rows, addrs, sizes, _ = parse_stream(data, 0, [T.INT32, T.STRING])
     rb = next(iter(pa.ipc.open_stream(pa.py_buffer(data))))
     vals = rb.column(0).buffers()[1]
     k = 1  # values buffer of column 0
@@ -95,22 +95,45 @@ def test_offsets_without_base():
 
 def test_schema_only_stream_has_no_batches():
     schema, _ = _batches([T.INT64], [], seed=5, null_frac=0)
-    rows, addrs, sizes = parse_stream(_stream([], schema), 0, [T.INT64])
+    rows, addrs, sizes, _ = parse_stream(_stream([], schema), 0, [T.INT64])
     assert len(rows) == 0
 
 
 def test_stream_without_eos_ends_at_its_length():
     schema, rbs = _batches([T.INT64], [5, 6], seed=6, null_frac=0)
     data = _stream(rbs, schema)[:-8]  # drop the end-of-stream marker
-    rows, _, _ = parse_stream(data, 0, [T.INT64])
+    rows, _, _, _ = parse_stream(data, 0, [T.INT64])
     assert rows.tolist() == [5, 6]
 
 
-def test_compressed_stream_is_not_implemented():
+def test_zstd_stream_is_not_implemented():
     schema, rbs = _batches([T.INT64, T.STRING], [100], seed=7, null_frac=0.1)
-    data = _stream(rbs, schema, compression="lz4")
+    data = _stream(rbs, schema, compression="zstd")
     with pytest.raises(L.NotImplementedStatus):
         parse_stream(data, 0, [T.INT64, T.STRING])
+
+
+def test_lz4_stream_lays_out_the_arena():
+    """Compressed buffers go to 64-byte aligned arena slots of their decoded
+    size (pyarrow's decode gives the sizes); empty buffers stay empty."""
+    schema, rbs = _batches(TYPES, [1000, 1, 4096], seed=10, null_frac=0.2)
+    data = _stream(rbs, schema, compression="lz4")
+    rows, addrs, sizes, arena = parse_stream(data, 1 << 40, TYPES, arena_base=1 << 44)
+    read = list(pa.ipc.open_stream(pa.py_buffer(data)))
+    assert rows.tolist() == [rb.num_rows for rb in read]
+    slots = []
+    for b, rb in enumerate(read):
+        for (ea, es), a, s in zip(_expected(rb, TYPES), addrs[b], sizes[b]):
+            if es == 0:
+                assert a == 0 and s == 0
+                continue
+            assert a >= 1 << 44 and (a - (1 << 44)) % 64 == 0
+            assert s >= es  # pyarrow may trim a decoded bitmap or offsets view; the stored length is the prefix
+            slots.append((a - (1 << 44), s))
+    slots.sort()
+    for (a0, s0), (a1, _) in zip(slots, slots[1:]):
+        assert a0 + s0 <= a1
+    assert arena >= slots[-1][0] + slots[-1][1]
 
 
 def test_truncated_stream_is_io_error():
