@@ -143,6 +143,9 @@ _OPTIONAL_SIGS = {
                                    ctypes.POINTER(GzAggOptions), _PP]),
     "gz_agg_process": (ctypes.c_int, [_P, _I64, _PI64, _PI64, _I32]),
     "gz_agg_process_batches": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch)]),
+    "gz_join_agg_make": (ctypes.c_int, [_P, _P, _I32, _PI32, _I32, _I32, _PI32, _PI32, _I32,
+                                        ctypes.POINTER(GzAggAction), ctypes.POINTER(GzAggOptions), _PP]),
+    "gz_join_agg_process_batches": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch)]),
     "gz_agg_num_groups": (ctypes.c_int, [_P, _PI64]),
     "gz_agg_num_result_columns": (ctypes.c_int, [_P]),
     "gz_agg_result_type": (ctypes.c_int, [_P, _I32]),

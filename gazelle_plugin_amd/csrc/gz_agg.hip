@@ -620,6 +620,159 @@ __global__ __launch_bounds__(256) void agg_update_lds_kernel(AggParams p) {
   lds_merge(p, p.inst[cur], s_tab, LS);
 }
 
+// ---------------------------------------------------------------- fused join
+// Join -> aggregate in one pass (whole-stage codegen's conditionedProbeArraysInner
+// feeding hashAggregateArrays, whole_stage_codegen_kernel.cc:48-330): probe rows
+// look up the build relation's hash table, and every joined (probe row, build
+// row) pair updates the aggregate directly; the join output is never written.
+// The aggregate's input columns are join result columns, each read from the
+// probe batch or from the relation's row-indexed store.  The group key is the
+// join key, so a group's first row is the first probe row with that key: the
+// groups come out in the order of the join output, as the reference's.
+struct JoinIn {
+  int side;  // 0 build (relation store), 1 probe batch
+  int idx;   // column on that side
+  int type;
+  const uint8_t* bvalid;  // build side: validity bytes
+  const uint8_t* bvalues; // build side: values (bool: one byte per row)
+};
+
+struct JoinTile {
+  int32_t batch, row_begin, nrows, pad;
+  int64_t row_base;  // global index of the tile's first probe row
+};
+
+struct JoinAggParams {
+  const JoinTile* tiles;
+  int64_t T;
+  const void* const* bufs;  // probe batches: [batch * bstride + col * 3 + k]
+  int bstride;
+  int pkey, key_type;       // probe key column and its type
+  const int64_t* jkeys;
+  const int32_t* jstart;
+  const int32_t* jcount;
+  const int32_t* jrowlist;  // null: unique keys
+  uint64_t jcap;
+  int64_t special_start, special_count;  // the key equal to kEmpty
+  int tiles_per_block;
+  JoinIn in[kAggMaxCols];
+};
+
+struct JoinRow {
+  const JoinAggParams& q;
+  int batch;
+  int64_t r;   // probe row in its batch
+  int64_t br;  // build row
+  __device__ __forceinline__ const void* pb(int col, int k) const {
+    return q.bufs[(int64_t)batch * q.bstride + q.in[col].idx * 3 + k];
+  }
+  __device__ __forceinline__ bool valid(int col) const {
+    const JoinIn& c = q.in[col];
+    if (c.side == 0) return c.bvalid[br] != 0;
+    const uint8_t* vb = (const uint8_t*)pb(col, 0);
+    return !vb || bit_get(vb, r);
+  }
+  __device__ __forceinline__ int64_t i64(int col) const {
+    const JoinIn& c = q.in[col];
+    if (c.side == 0) return c.type == GZ_TYPE_BOOL ? (int64_t)c.bvalues[br] : read_i64(c.bvalues, c.type, br);
+    return read_i64(pb(col, 2), c.type, r);
+  }
+  __device__ __forceinline__ double f64(int col) const {
+    const JoinIn& c = q.in[col];
+    if (c.side == 0) return c.type == GZ_TYPE_BOOL ? (double)c.bvalues[br] : read_f64(c.bvalues, c.type, br);
+    return read_f64(pb(col, 2), c.type, r);
+  }
+  __device__ __forceinline__ uint64_t u64(int col) const {
+    const JoinIn& c = q.in[col];
+    return c.side == 0 ? ((const uint64_t*)c.bvalues)[br] : ((const uint64_t*)pb(col, 2))[r];
+  }
+  __device__ __forceinline__ int64_t raw64(int col) const {
+    const JoinIn& c = q.in[col];
+    return c.side == 0 ? ((const int64_t*)c.bvalues)[br] : ((const int64_t*)pb(col, 2))[r];
+  }
+};
+
+// Persistent workgroups over consecutive probe tiles, an LDS first-level group
+// table each (as the plain aggregate's LDS path: a key that finds no LDS slot
+// within a short probe goes to the HBM table), merged into HBM at the end.
+__global__ __launch_bounds__(256) void join_agg_update_kernel(AggParams p, JoinAggParams q) {
+  extern __shared__ int64_t s_tab[];
+  const int64_t LS = p.lcap + 3;
+  const uint64_t lmask = (uint64_t)p.lcap - 1;
+  const int64_t t0 = (int64_t)blockIdx.x * q.tiles_per_block;
+  const int64_t t1 = min(q.T, t0 + q.tiles_per_block);
+  if (t0 >= t1) return;
+  lds_reset(p, s_tab, LS);
+  __syncthreads();
+  const AggInst I = p.inst[0];
+  const int64_t GS = (int64_t)I.cap + 2;
+  const uint64_t jmask = q.jcap - 1;
+  for (int64_t t = t0; t < t1; t++) {
+    const JoinTile td = q.tiles[t];
+    const uint8_t* kv = (const uint8_t*)q.bufs[(int64_t)td.batch * q.bstride + q.pkey * 3];
+    const void* kd = q.bufs[(int64_t)td.batch * q.bstride + q.pkey * 3 + 2];
+    for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
+      const int64_t r = (int64_t)td.row_begin + lr;
+      if (kv && !bit_get(kv, r)) continue;  // a null key joins nothing
+      const int64_t key = read_i64(kd, q.key_type, r);
+      int64_t st, cnt;
+      if (key == kEmpty) {
+        if (q.special_count == 0) continue;
+        st = q.special_start;
+        cnt = q.jrowlist ? q.special_count : 1;
+      } else {
+        uint64_t s = fmix64((uint64_t)key) & jmask;
+        int64_t m = -1;
+        for (uint64_t i = 0; i < q.jcap; i++) {
+          const int64_t k = gld(q.jkeys + s);
+          if (k == key) {
+            m = (int64_t)s;
+            break;
+          }
+          if (k == kEmpty) break;
+          s = (s + 1) & jmask;
+        }
+        if (m < 0) continue;
+        st = gld(q.jstart + m);
+        cnt = q.jrowlist ? gld(q.jcount + m) : 1;
+      }
+      const int64_t grow = td.row_base + lr;
+      int64_t ls = -1;
+      if (key == kEmpty) {
+        ls = p.lcap + 1;
+      } else {
+        uint64_t s = fmix64((uint64_t)key) & lmask;
+        for (int i = 0; i < 32; i++) {  // short probe: a full LDS table spills to HBM
+          const int64_t k = s_tab[s];
+          if (k == key) {
+            ls = (int64_t)s;
+            break;
+          }
+          if (k == kEmpty) {
+            const unsigned long long prev =
+                atomicCAS((unsigned long long*)&s_tab[s], (unsigned long long)kEmpty, (unsigned long long)key);
+            if ((int64_t)prev == kEmpty || (int64_t)prev == key) {
+              ls = (int64_t)s;
+              break;
+            }
+          }
+          s = (s + 1) & lmask;
+        }
+      }
+      const int64_t g = ls >= 0 ? -1 : table_slot(I, true, key);
+      if (ls < 0 && g < 0) continue;  // HBM table overflow (flagged)
+      for (int64_t j = 0; j < cnt; j++) {
+        const int64_t br = q.jrowlist ? (int64_t)gld(q.jrowlist + st + j) : st;
+        const JoinRow row{q, td.batch, r, br};
+        if (ls >= 0) apply_row(p, s_tab, LS, ls, row, grow);
+        else apply_row(p, I.base, GS, g, row, grow);
+      }
+    }
+  }
+  __syncthreads();
+  lds_merge(p, I, s_tab, LS);
+}
+
 // ---------------------------------------------------------------- partitioned
 // Partitioned placement for medium cardinalities: the rows of an instance are
 // scattered into 2^fan_log2 partitions by the high bits of the key hash (the LDS
@@ -1202,7 +1355,16 @@ struct AggPlan {
   std::vector<ResultSpec> results;
 };
 
+struct JoinAggState {
+  gz_relation* rel = nullptr;
+  std::vector<int> ptypes;  // probe batch column types
+  int pkey = 0;
+  std::vector<int> sides, idx;  // aggregate input column -> join result column
+  int64_t probe_rows = 0;       // global probe row index of the next call's first row
+};
+
 struct gz_agg {
+  JoinAggState* join = nullptr;  // a fused join -> aggregate
   gz_context* ctx = nullptr;
   AggPlan plan;
   gz_agg_options opts;
@@ -1531,6 +1693,7 @@ static int agg_run(gz_agg* h);
 static int agg_process(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of) {
   gz_context* ctx = h->ctx;
   if (h->finished) return fail(GZ_INVALID, "aggregate already finished");
+  if (h->join) return fail(GZ_INVALID, "a join aggregate takes probe batches (gz_join_agg_process_batches)");
   GZ_HIP(hipSetDevice(ctx->device));
   // the first call of a fresh aggregate goes through the planned path (all
   // placements, the partitioned one included); later calls add to its tables
@@ -2159,6 +2322,162 @@ int gz_agg_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types, int
   return agg_set_make(ctx, num_cols, col_types, num_actions, actions, opts, 1, out);
 }
 
+int gz_join_agg_make(gz_relation* rel, gz_context* ctx, int32_t num_probe_cols, const int32_t* probe_types,
+                     int32_t probe_key_col, int32_t num_inputs, const int32_t* input_sides,
+                     const int32_t* input_indices, int32_t num_actions, const gz_agg_action* actions,
+                     const gz_agg_options* opts, gz_agg** out) {
+  if (!rel || !out || !probe_types || !input_sides || !input_indices || !actions)
+    return fail(GZ_INVALID, "gz_join_agg_make: null argument");
+  GZ_TRY(relation_synchronize(rel));
+  RelationView v;
+  GZ_TRY(relation_view(rel, &v));
+  if (!ctx) ctx = v.ctx;
+  if (num_probe_cols <= 0 || num_probe_cols > kAggMaxCols || probe_key_col < 0 || probe_key_col >= num_probe_cols)
+    return fail(GZ_INVALID, "bad probe columns");
+  if (probe_types[probe_key_col] != v.types[v.key_col])
+    return fail(GZ_TYPE_ERROR, "probe and build key types differ");
+  if (num_inputs <= 0 || num_inputs > kAggMaxCols) return fail(GZ_INVALID, "bad input column count");
+  std::vector<int32_t> types(num_inputs);
+  for (int c = 0; c < num_inputs; c++) {
+    const int sd = input_sides[c], ix = input_indices[c];
+    if (sd == 0 && (ix < 0 || ix >= v.ncols)) return fail(GZ_INVALID, "build column out of range");
+    if (sd == 1 && (ix < 0 || ix >= num_probe_cols)) return fail(GZ_INVALID, "probe column out of range");
+    if (sd != 0 && sd != 1) return fail(GZ_INVALID, "input side must be 0 (build) or 1 (probe)");
+    types[c] = sd == 0 ? v.types[ix] : probe_types[ix];
+    if (type_is_binary(types[c])) return fail(GZ_NOT_IMPLEMENTED, "aggregate inputs must be fixed-width");
+  }
+  gz_agg* h = nullptr;
+  GZ_TRY(agg_set_make(ctx, num_inputs, types.data(), num_actions, actions, opts, 1, &h));
+  auto bail = [&](int rc) {
+    gz_agg_release(h);
+    return rc;
+  };
+  const int kc = h->plan.key_col;
+  if (kc < 0) return bail(fail(GZ_NOT_IMPLEMENTED, "a join aggregate groups by the join key"));
+  const bool key_ok = (input_sides[kc] == 1 && input_indices[kc] == probe_key_col) ||
+                      (input_sides[kc] == 0 && input_indices[kc] == v.key_col);
+  if (!key_ok) return bail(fail(GZ_NOT_IMPLEMENTED, "a join aggregate groups by the join key"));
+  if (v.rowlist)
+    for (auto& l : h->plan.lanes)
+      if (l.op == L_FIRSTNN || l.op == L_FIRSTNAN)
+        return bail(fail(GZ_NOT_IMPLEMENTED, "float min/max without NaN check over repeated build keys"));
+  h->join = new JoinAggState();
+  h->join->rel = rel;
+  h->join->ptypes.assign(probe_types, probe_types + num_probe_cols);
+  h->join->pkey = probe_key_col;
+  h->join->sides.assign(input_sides, input_sides + num_inputs);
+  h->join->idx.assign(input_indices, input_indices + num_inputs);
+  *out = h;
+  return GZ_OK;
+}
+
+int gz_join_agg_process_batches(gz_agg* h, int32_t nb, const gz_batch* batches) {
+  if (!h || !h->join) return fail(GZ_INVALID, "not a join aggregate");
+  if (nb < 0 || (nb > 0 && !batches)) return fail(GZ_INVALID, "bad batch list");
+  if (h->finished) return fail(GZ_INVALID, "aggregate already finished");
+  JoinAggState& J = *h->join;
+  gz_context* ctx = h->ctx;
+  GZ_HIP(hipSetDevice(ctx->device));
+  RelationView v;
+  GZ_TRY(relation_view(J.rel, &v));
+  const int nc = (int)J.ptypes.size();
+  constexpr int kTile = 4096;
+  std::vector<JoinTile> tiles;
+  std::vector<const void*> bufs((size_t)std::max(nb, 1) * nc * 3, nullptr);
+  int64_t call_rows = 0;
+  for (int bi = 0; bi < nb; bi++) {
+    const gz_batch& bt = batches[bi];
+    if (bt.num_rows < 0 || bt.num_rows > INT32_MAX) return fail(GZ_INVALID, "bad num_rows");
+    int need = 0;
+    for (int c = 0; c < nc; c++) need += type_num_bufs(J.ptypes[c]);
+    if (bt.num_bufs < need) return fail(GZ_INVALID, "insufficient number of in_buf_addrs");
+    int k = 0;
+    for (int c = 0; c < nc; c++) {
+      const int t = J.ptypes[c];
+      const void** slot = &bufs[((size_t)bi * nc + c) * 3];
+      slot[0] = (bt.buf_addrs[k] && bt.buf_sizes[k] > 0) ? (const void*)bt.buf_addrs[k] : nullptr;
+      if (slot[0] && bt.buf_sizes[k] < (bt.num_rows + 7) / 8) return fail(GZ_INVALID, "validity buffer too small");
+      k++;
+      if (type_is_binary(t)) slot[1] = (const void*)bt.buf_addrs[k++];
+      const int64_t need_b = t == GZ_TYPE_BOOL ? (bt.num_rows + 7) / 8 : bt.num_rows * type_width(t);
+      if (t != GZ_TYPE_NA && !type_is_binary(t) && bt.buf_sizes[k] < need_b)
+        return fail(GZ_INVALID, "values buffer too small");
+      slot[2] = (const void*)bt.buf_addrs[k++];
+      if (t != GZ_TYPE_NA && bt.num_rows > 0 && !slot[2]) return fail(GZ_INVALID, "missing values buffer");
+    }
+    for (int64_t r0 = 0; r0 < bt.num_rows; r0 += kTile) {
+      JoinTile t{};
+      t.batch = bi;
+      t.row_begin = (int32_t)r0;
+      t.nrows = (int32_t)std::min<int64_t>(kTile, bt.num_rows - r0);
+      t.row_base = J.probe_rows + call_rows + r0;
+      tiles.push_back(t);
+    }
+    call_rows += bt.num_rows;
+  }
+  if (call_rows + J.probe_rows >= (1ll << 44)) return fail(GZ_CAPACITY_ERROR, "more than 2^44 probe rows");
+  J.probe_rows += call_rows;
+  h->rows_seen[0] += call_rows;
+  if (tiles.empty()) return GZ_OK;
+  // groups: at most the relation's distinct keys, at most the probe rows so far
+  const double lf = h->opts.load_factor;
+  const int64_t bound = 2 + std::min<int64_t>(v.nkeys, h->g_bound[0] + call_rows);
+  if (!h->tables[0].p) {
+    GZ_TRY(inst_alloc(h, 0, pow2_at_least((uint64_t)(bound / lf) + 16)));
+  } else if ((double)bound > lf * h->inst[0].cap) {
+    GZ_TRY(inst_alloc(h, 0, pow2_at_least((uint64_t)(bound * 2 / lf))));
+  }
+  h->g_bound[0] = bound;
+  if (h->inst_dirty) {
+    GZ_TRY(upload(ctx, &h->d_inst, h->inst.data(), sizeof(AggInst) * h->inst.size()));
+    h->inst_dirty = false;
+  }
+  GZ_TRY(upload(ctx, &h->d_tiles, tiles.data(), sizeof(JoinTile) * tiles.size()));
+  GZ_TRY(upload(ctx, &h->d_bufs, bufs.data(), sizeof(void*) * bufs.size()));
+  AggParams p = base_params(h);
+  p.inst = (const AggInst*)h->d_inst.p;
+  JoinAggParams q;
+  memset(&q, 0, sizeof(q));
+  q.tiles = (const JoinTile*)h->d_tiles.p;
+  q.T = (int64_t)tiles.size();
+  q.bufs = (const void* const*)h->d_bufs.p;
+  q.bstride = nc * 3;
+  q.pkey = J.pkey;
+  q.key_type = J.ptypes[J.pkey];
+  q.jkeys = v.keys;
+  q.jstart = v.start;
+  q.jcount = v.count;
+  q.jrowlist = v.rowlist;
+  q.jcap = v.cap;
+  q.special_start = v.special_start;
+  q.special_count = v.special_count;
+  for (int c = 0; c < h->plan.ncols; c++) {
+    JoinIn& in = q.in[c];
+    in.side = J.sides[c];
+    in.idx = J.idx[c];
+    in.type = h->plan.types[c];
+    if (in.side == 0) {
+      in.bvalid = v.valid[in.idx];
+      in.bvalues = (const uint8_t*)v.values[in.idx];
+    }
+  }
+  int64_t blocks = std::min<int64_t>(q.T, (int64_t)ctx->num_cus * 2);
+  q.tiles_per_block = (int)((q.T + blocks - 1) / blocks);
+  blocks = (q.T + q.tiles_per_block - 1) / q.tiles_per_block;
+  p.lcap = 1024;
+  int lds = lds_bytes(h, p.lcap);
+  while (lds > 96 * 1024 && p.lcap > 64) {
+    p.lcap >>= 1;
+    lds = lds_bytes(h, p.lcap);
+  }
+  {
+    KernelTimer kt(ctx, "join_agg_update");
+    hipLaunchKernelGGL(join_agg_update_kernel, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, p, q);
+  }
+  GZ_HIP(hipGetLastError());
+  return GZ_OK;
+}
+
 int gz_agg_process(gz_agg* h, int64_t num_rows, const int64_t* buf_addrs, const int64_t* buf_sizes,
                    int32_t num_bufs) {
   if (!h) return fail(GZ_INVALID, "null aggregate");
@@ -2229,6 +2548,7 @@ int gz_agg_release(gz_agg* h) {
     ctx_free(ctx, b);
   if (h->L.exec) hipGraphExecDestroy(h->L.exec);
   if (h->L.graph) hipGraphDestroy(h->L.graph);
+  delete h->join;
   delete h;
   return GZ_OK;
 }

@@ -141,6 +141,23 @@ void ctx_resolve_timings(gz_context* ctx, bool wait);
 // buffer: it may be reused as soon as this returns).
 int upload(gz_context* ctx, DBuf* dst, const void* src, size_t bytes);
 
+// Device view of a finished relation (the join's build side) for kernels
+// outside the join: its hash table and its row-indexed column stores.
+struct RelationView {
+  gz_context* ctx;
+  const int64_t* keys;     // [cap] slot keys
+  const int32_t* start;    // [cap] first row (unique keys) or rowlist start
+  const int32_t* count;    // [cap]
+  const int32_t* rowlist;  // rows grouped by key, or null (unique keys)
+  uint64_t cap;
+  int64_t special_start, special_count, nkeys, rows;
+  int key_col, ncols;
+  int types[32];
+  const uint8_t* valid[32];  // validity bytes per row (1 = valid)
+  const void* values[32];    // fixed width: width bytes per row; bool: one byte per row
+};
+int relation_view(gz_relation* r, RelationView* v);
+
 // Probe on another context than the relation's (resident relations).
 void probe_set_context(gz_probe* h, gz_context* ctx);
 // Wait for the relation's build (its context's stream).

@@ -1313,6 +1313,34 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   return GZ_OK;
 }
 
+namespace gz {
+
+int relation_view(gz_relation* r, RelationView* v) {
+  if (!r || !r->finished) return fail(GZ_INVALID, "relation not finished");
+  if (r->ncols > 32) return fail(GZ_NOT_IMPLEMENTED, "relation with more than 32 columns");
+  memset(v, 0, sizeof(*v));
+  v->ctx = r->ctx;
+  v->keys = (const int64_t*)r->keys.p;
+  v->start = (const int32_t*)r->start.p;
+  v->count = (const int32_t*)r->count.p;
+  v->rowlist = r->has_dups ? (const int32_t*)r->rowlist.p : nullptr;
+  v->cap = r->cap;
+  v->special_start = r->special_start;
+  v->special_count = r->special_count;
+  v->nkeys = r->nkeys;
+  v->rows = r->n;
+  v->key_col = r->key_col;
+  v->ncols = r->ncols;
+  for (int c = 0; c < r->ncols; c++) {
+    v->types[c] = r->cols[c].type;
+    v->valid[c] = (const uint8_t*)r->cols[c].valid.p;
+    v->values[c] = (const void*)r->cols[c].values.p;
+  }
+  return GZ_OK;
+}
+
+}  // namespace gz
+
 extern "C" {
 
 int gz_probe_make(gz_relation* r, int32_t num_probe_cols, const int32_t* probe_types, int32_t probe_key_col,

@@ -149,3 +149,55 @@ class ConditionedProbeKernel:
             self.close()
         except Exception:
             pass
+
+
+class JoinAggregateKernel:
+    """The fused stage whole-stage codegen builds from conditionedProbeArraysInner
+    feeding hashAggregateArrays (WholeStageCodeGenKernel,
+    ext/whole_stage_codegen_kernel.cc:48-330): an inner probe of ``relation``
+    whose joined rows update a hash aggregate directly, grouped by the join key.
+    ``inputs``: the aggregate's input columns as join result columns
+    [(side, index)] (side 0 build, 1 probe); ``actions`` index them, as the
+    reference's aggregateExpressions do."""
+
+    def __init__(self, relation: HashRelation, probe_types: Sequence[int], probe_key: int,
+                 inputs: Sequence[Tuple[int, int]], actions, ctx=None, options=None):
+        from .aggregate import AggOptions, HashAggregateKernel, HashAggregateResultIterator, _actions_c
+
+        self.rel = relation
+        self.probe_types = list(probe_types)
+        self.inputs = [tuple(x) for x in inputs]
+        types = [relation.types[i] if s == 0 else self.probe_types[i] for s, i in self.inputs]
+        self.kernel = HashAggregateKernel(ctx or relation.ctx, types, actions, options or AggOptions())
+        it = HashAggregateResultIterator.__new__(HashAggregateResultIterator)
+        it.k = self.kernel
+        it._h = ctypes.c_void_p()
+        it.num_instances = 1
+        it._offset = 0
+        it._finished = False
+        it._results = None
+        opts = self.kernel.options.to_c()
+        acts = _actions_c(self.kernel.actions)
+        L.check(L.lib().gz_join_agg_make(relation.handle, (ctx or relation.ctx).handle, len(self.probe_types),
+                                         L.i32_array(self.probe_types), probe_key, len(self.inputs),
+                                         L.i32_array([s for s, _ in self.inputs]),
+                                         L.i32_array([i for _, i in self.inputs]), len(self.kernel.actions), acts,
+                                         ctypes.byref(opts), ctypes.byref(it._h)))
+        self.iterator = it
+
+    def ProcessBatches(self, batches: Sequence[DeviceBatch], arr=None):
+        from .splitter import make_batch_array
+
+        if arr is None:
+            arr = make_batch_array(batches)
+        L.check(L.lib().gz_join_agg_process_batches(self.iterator._h, len(batches), arr))
+
+    def results(self):
+        """(type, values, valid) per result column, groups in join-output order."""
+        return self.iterator.results()
+
+    def num_groups(self) -> int:
+        return self.iterator.num_groups()
+
+    def close(self):
+        self.iterator.close()

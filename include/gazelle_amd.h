@@ -373,6 +373,27 @@ int gz_probe_result_column(gz_probe* p, int32_t col, void** values, void** valid
                            void** offsets, void** data);
 int gz_probe_release(gz_probe* p);
 
+/* ---- fused join -> aggregate -------------------------------------------------
+ * Replaces whole-stage codegen's inner probe feeding a hash aggregate
+ * (WholeStageCodeGenKernel, ext/whole_stage_codegen_kernel.cc:48-330, with
+ * conditionedProbeArraysInner and hashAggregateArrays children; golden
+ * WSCGTestInnerJoinWithGroupbyAggregate, tests/arrow_compute_test_wscg.cc:4838-5000).
+ * The aggregate reads num_inputs join result columns (input_sides[c] 0 = build
+ * relation column, 1 = probe column; input_indices[c] the column on that side)
+ * and groups by the join key (its GROUPBY input must be the probe key column or
+ * the relation's key column); the joined rows are never materialized.  Returns
+ * a gz_agg handle: gz_agg_num_groups / finish / result_column / release apply,
+ * with groups in the order of the join output (first probe row per key).  ctx
+ * null = the relation's context.  Integer key types; no join condition (the
+ * reference's condition expression is not evaluated). */
+int gz_join_agg_make(gz_relation* rel, gz_context* ctx, int32_t num_probe_cols, const int32_t* probe_types,
+                     int32_t probe_key_col, int32_t num_inputs, const int32_t* input_sides,
+                     const int32_t* input_indices, int32_t num_actions, const gz_agg_action* actions,
+                     const gz_agg_options* opts, gz_agg** out);
+/* Probes the batches and updates the aggregate (ProcessAndCacheOne over a
+ * probe batch of the fused stage). */
+int gz_join_agg_process_batches(gz_agg* h, int32_t num_batches, const gz_batch* batches);
+
 /* ---- serving -----------------------------------------------------------------
  * A fragment server: query fragments (an aggregate, a join followed by an
  * aggregate, a shuffle split) arrive on a timeline and are admitted, first come

@@ -203,6 +203,33 @@ JOIN_FIXTURES = {"tests": [
 ]}
 
 
+# --------------------------------------------------------------------------- whole stage
+# native-sql-engine/cpp/src/tests/arrow_compute_test_wscg.cc:4838-5000,
+# WSCGTestInnerJoinWithGroupbyAggregate: table0 is the build side (key f0),
+# table1 probes it (key f0); the join condition is add(table0_f1, table1_f1) >
+# table0_f2; the aggregate groups by table0_f0 with sum_count(table1_f1) and
+# min(table1_f1).
+WSCG_FIXTURES = {"tests": [
+    {
+        "name": "WSCGTestInnerJoinWithGroupbyAggregate",
+        "build_schema": [["table0_f0", "uint32"], ["table0_f1", "uint32"], ["table0_f2", "uint32"]],
+        "probe_schema": [["table1_f0", "uint32"], ["table1_f1", "uint32"]],
+        "build_key": 0, "probe_key": 0,
+        "condition": "table0_f1 + table1_f1 > table0_f2",
+        "aggregate_inputs": [[0, 0], [1, 1]],  # (side, column): table0_f0, table1_f1
+        "actions": [["action_groupby", [0]], ["action_sum_count", [1]], ["action_min", [1]]],
+        "build": [[[10, 3, 1, 2, 3, 1], [10, 3, 1, 2, 13, 11], [10, 3, 1, 2, 13, 11]],   # :4940-4946
+                  [[6, 12, 5, 8, 6, 10], [6, 12, 5, 8, 16, 110], [6, 12, 5, 8, 16, 110]]],
+        "probe": [[[1, 2, 3, 4, 5, 6], [1, 2, 3, 4, 5, 6]],                            # :4948-4954
+                  [[7, 8, 9, 10, 11, 12], [7, 8, 9, 10, 11, 12]]],
+        "result_schema": [["table0_f0_unique", "uint32"], ["table1_f1_sum", "double"],
+                          ["table1_f1_count", "int64"], ["table1_f1_min", "uint32"]],
+        "expected": [[1, 2, 3, 5, 6, 8, 10, 12], [2, 2, 6, 5, 12, 8, 20, 12],           # :4958-4962
+                     [2, 1, 2, 1, 2, 1, 2, 1], [1, 2, 3, 5, 6, 8, 10, 12]],
+    },
+]}
+
+
 def main():
     with open(os.path.join(HERE, "split_fixtures.json"), "w") as f:
         json.dump(SPLIT_FIXTURES, f, indent=1)
@@ -210,7 +237,9 @@ def main():
         json.dump(AGG_FIXTURES, f, indent=1)
     with open(os.path.join(HERE, "join_fixtures.json"), "w") as f:
         json.dump(JOIN_FIXTURES, f, indent=1)
-    print("wrote split_fixtures.json, agg_fixtures.json, join_fixtures.json")
+    with open(os.path.join(HERE, "wscg_fixtures.json"), "w") as f:
+        json.dump(WSCG_FIXTURES, f, indent=1)
+    print("wrote split_fixtures.json, agg_fixtures.json, join_fixtures.json, wscg_fixtures.json")
 
 
 if __name__ == "__main__":
