@@ -434,9 +434,63 @@ def bench_join(args, ctx, stream):
         out[jt] = {"ms": ms, "min_ms": mn, "out_rows": n, "probe_rows_per_s": probe.num_rows / (ms / 1e3),
                    "GBps": algo / (ms / 1e3) / 1e9, "frac_of_hbm_peak": algo / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
         pk.close()
+    out["join_aggregate"] = bench_join_aggregate(ctx, stream, rel, probe, parr)
     rel.close()
     del build, probe, barr, parr
     return out
+
+
+def bench_join_aggregate(ctx, stream, rel, probe, parr):
+    """Join followed by aggregate (GROUP BY the join key: sum/avg of the build
+    float64, count, min/max/sum of the build int32) over the join workload:
+    the fused stage against the inner probe materializing its output followed
+    by the aggregate over that output."""
+    import ctypes
+
+    import torch
+
+    from gazelle_plugin_amd import _lib as L
+    from gazelle_plugin_amd import types as T
+    from gazelle_plugin_amd.aggregate import HashAggregateKernel
+    from gazelle_plugin_amd.join import ConditionedProbeKernel, JoinAggregateKernel
+    from gazelle_plugin_amd.shuffle_reader import StreamBatch
+
+    inputs = [(1, 0), (0, 2), (0, 1)]
+    res = {}
+
+    def fused():
+        k = JoinAggregateKernel(rel, [T.INT64], 0, inputs, AGG_ACTIONS, ctx=ctx)
+        k.ProcessBatches(probe.batches, parr)
+        res["groups"] = k.num_groups()
+        k.close()
+
+    fused()
+    fused_ms, fused_min = timed(fused, stream, 3)
+    pk = ConditionedProbeKernel.Make(rel, [T.INT64], 0, "inner", False, inputs)
+
+    def unfused():
+        n, _ = pk.ProcessBatches(probe.batches, parr)
+        addrs, sizes = [], []
+        for c, t in enumerate([T.INT64, T.DOUBLE, T.INT32]):
+            vals, valid, offs, data = (ctypes.c_void_p() for _ in range(4))
+            L.check(L.lib().gz_probe_result_column(pk._h, c, ctypes.byref(vals), ctypes.byref(valid),
+                                                   ctypes.byref(offs), ctypes.byref(data)))
+            addrs += [0, vals.value]  # the workload has no nulls: no validity bitmaps
+            sizes += [0, n * T.width(t)]
+        b = StreamBatch(n, [T.INT64, T.DOUBLE, T.INT32], addrs, sizes, None)
+        it = HashAggregateKernel.Make(ctx, [T.INT64, T.DOUBLE, T.INT32], AGG_ACTIONS).MakeResultIterator()
+        it.ProcessBatches([b])
+        res["groups_unfused"] = it.num_groups()
+        it.close()
+        res["rows"] = n
+
+    unfused()
+    unfused_ms, unfused_min = timed(unfused, stream, 3)
+    pk.close()
+    return {"groups": res["groups"], "joined_rows": res["rows"], "fused_ms": fused_ms, "fused_min_ms": fused_min,
+            "unfused_ms": unfused_ms, "unfused_min_ms": unfused_min, "fused_speedup": unfused_ms / fused_ms,
+            "groups_equal": res["groups"] == res["groups_unfused"],
+            "probe_rows_per_s_fused": probe.num_rows / (fused_ms / 1e3)}
 
 
 def bench_serve(args):

@@ -138,9 +138,11 @@ int64_t estimate(const gz_server* s, const gz_fragment& f) {
       return bytes_of(f) + rows / 4 + tiles * (int64_t)s->split_P * 4 * 6 + (64 << 20);
     }
     case GZ_FRAG_JOIN_AGGREGATE: {
+      // materialized join output (the fallback) plus a group table that may
+      // round up to twice the aggregate's bound
       int64_t w = 0;
       for (size_t c = 0; c < s->probe_sides.size(); c++) w += 9;
-      return rows * (w + 8) + agg_bytes(s->join_agg);
+      return rows * (w + 8) + 2 * agg_bytes(s->join_agg);
     }
   }
   return 0;
@@ -193,6 +195,28 @@ int run_fragment(gz_server* s, Slot& sl, int i, int64_t* out_rows) {
     case GZ_FRAG_JOIN_AGGREGATE: {
       if (!s->probe_set || !s->join_agg.set) return fail(GZ_INVALID, "no join plan");
       if (f.relation < 0 || f.relation >= (int)s->relations.size()) return fail(GZ_INDEX_ERROR, "unknown relation");
+      if (s->join_type == GZ_JOIN_INNER) {
+        // the fused stage: the probe updates the aggregate, no join output
+        gz_agg* a = nullptr;
+        int rc = gz_join_agg_make(s->relations[f.relation], sl.ctx, (int32_t)s->probe_types.size(),
+                                  s->probe_types.data(), s->probe_key, (int32_t)s->probe_sides.size(),
+                                  s->probe_sides.data(), s->probe_idx.data(), (int32_t)s->join_agg.actions.size(),
+                                  s->join_agg.actions.data(), nullptr, &a);
+        if (rc == GZ_OK) {
+          rc = gz_join_agg_process_batches(a, f.num_batches, f.batches);
+          if (!rc) rc = gz_agg_finish(a);
+          if (!rc) rc = gz_agg_num_groups(a, out_rows);
+          if (rc) {
+            gz_agg_release(a);
+            return rc;
+          }
+          if (s->keep) s->agg_out[i] = a;
+          else gz_agg_release(a);
+          return GZ_OK;
+        }
+        if (rc != GZ_NOT_IMPLEMENTED) return rc;
+        // plans the fused stage does not cover materialize the join output
+      }
       gz_probe* pr = nullptr;
       GZ_TRY(gz_probe_make(s->relations[f.relation], (int32_t)s->probe_types.size(), s->probe_types.data(),
                            s->probe_key, s->join_type, 0, (int32_t)s->probe_sides.size(), s->probe_sides.data(),

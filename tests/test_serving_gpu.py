@@ -14,6 +14,7 @@ from gazelle_plugin_amd.join import HashRelationKernel
 from gazelle_plugin_amd.serving import AGGREGATE, JOIN_AGGREGATE, SPLIT, FragmentServer, latency_summary
 from gazelle_plugin_amd.splitter import make_batch_array
 from aggcheck import CONFIG0_ACTIONS, assert_results_equal
+from joinaggcheck import expected as joinagg_expected
 from splitcheck import assert_layout_equal
 
 pytestmark = pytest.mark.gpu
@@ -88,6 +89,9 @@ def test_results_alone_equal_results_in_the_mix(ctx):
             if k == AGGREGATE:
                 exp = oracle.aggregate(agg_host[b0:b0 + nb], datagen.AGG_TYPES,
                                        [parse_action(n, j) for n, j in CONFIG0_ACTIONS])
+                assert_results_equal(mixed[i], exp, RTOL)
+            else:  # join then aggregate: the oracle's aggregate over the oracle's join output
+                exp = joinagg_expected(build, probe_host[b0:b0 + nb], 0, 0, T.INT64, JOIN_RESULT, CONFIG0_ACTIONS)
                 assert_results_equal(mixed[i], exp, RTOL)
     summ = latency_summary(stats)
     assert summ["p99_ms"] >= summ["p50_ms"] > 0
