@@ -131,6 +131,7 @@ _SIGS = {
     "gz_split_ipc_write": (ctypes.c_int, [_P, ctypes.c_void_p, _I64]),
     "gz_ipc_record_batch_message": (ctypes.c_int, [_I64, _I32, _PI64, _I32, _PI64, _I64, ctypes.c_void_p, _I64,
                                                    _PI64]),
+    "gz_split_set_compress_type": (ctypes.c_int, [_P, _I32]),
     "gz_ipc_read_batches": (ctypes.c_int, [ctypes.c_void_p, _I64, _I64, _I64, _I32, _PI32, _I32, _PI64, _PI64,
                                            _PI64, _PI32, _PI64]),
     "gz_ipc_decompress": (ctypes.c_int, [_P, ctypes.c_void_p, _I64, _I64, _I32, _PI32, ctypes.c_void_p, _I64]),

@@ -793,6 +793,7 @@ static void ipc_plan_free(gz_context* ctx, gz_ipc_plan* pl);
 struct gz_splitter {
   gz_context* ctx = nullptr;
   gz_ipc_plan* ipc = nullptr;  // partition writer layout (gz_split_ipc_prepare)
+  int codec = GZ_COMPRESSION_UNCOMPRESSED;  // partition writer body compression
   int mode = 0;
   int P = 0;
   int ncols_in = 0;
@@ -1588,14 +1589,22 @@ class FlatBuilder {
 static std::vector<uint8_t> ipc_record_batch_message(int64_t rows,
                                                      const std::vector<std::pair<int64_t, int64_t>>& nodes,
                                                      const std::vector<std::pair<int64_t, int64_t>>& buffers,
-                                                     int64_t body_len) {
+                                                     int64_t body_len, bool lz4 = false) {
   FlatBuilder fb;
+  size_t comp = 0;
+  if (lz4) {
+    fb.start_table();          // BodyCompression
+    fb.field<int8_t>(1, 0);    // BodyCompressionMethod.BUFFER
+    fb.field<int8_t>(0, 0);    // CompressionType.LZ4_FRAME
+    comp = fb.end_table();
+  }
   const size_t vbuf = fb.vector_of_pairs(buffers);
   const size_t vnode = fb.vector_of_pairs(nodes);
   fb.start_table();  // RecordBatch
   fb.field<int64_t>(0, rows);
   fb.field_offset(1, vnode);
   fb.field_offset(2, vbuf);
+  if (lz4) fb.field_offset(3, comp);
   const size_t rb = fb.end_table();
   fb.start_table();  // Message
   fb.field<int64_t>(3, body_len);
@@ -1615,6 +1624,145 @@ static std::vector<uint8_t> ipc_record_batch_message(int64_t rows,
 
 static int64_t pad8(int64_t n) { return (n + 7) / 8 * 8; }
 
+// ---- LZ4 block encoder (partition writer with LZ4_FRAME bodies) ---------------
+// The published LZ4 block format: sequences of a token (literal length : 4,
+// match length - 4 : 4), length extensions (255-runs), literals, a 16-bit
+// offset; a match starts at least 12 bytes before the block end, and the last
+// 5 bytes are literals.  Frames are written with independent blocks (32 KiB,
+// under the 64 KiB maximum the frame declares), so every block compresses on its
+// own.
+struct Lz4CBlock {
+  const uint8_t* src;
+  uint8_t* dst;  // len bytes of room: a block that does not shrink is stored
+  uint32_t len;
+  uint32_t pad;
+};
+
+constexpr int kLz4HashLog = 12;
+constexpr uint32_t kLz4Block = 32768;  // block size: four workgroups' LDS per CU
+constexpr uint32_t kLz4MaxExt = 4096;  // longest match taken at once (longer runs continue in the next sequence)
+
+__device__ __forceinline__ uint32_t lds_rd32(const uint32_t* w, uint32_t p) {
+  return __builtin_amdgcn_alignbyte(w[(p >> 2) + 1], w[p >> 2], p & 3);
+}
+__device__ __forceinline__ uint32_t lds_byte(const uint32_t* w, uint32_t p) { return (w[p >> 2] >> ((p & 3) * 8)) & 0xFFu; }
+
+// One wave per block: the block is staged in LDS; 64 positions at a time
+// every lane hashes its position's 4 bytes, takes the table's earlier position
+// as its match candidate and extends the match; then the wave walks the 64
+// candidates in order and emits a sequence at the first match at or after the
+// end of the previous one (greedy, as the reference encoder's fast mode).
+// csize[b] = compressed bytes, or 0 when the block is stored.
+__global__ __launch_bounds__(64) void lz4_compress_kernel(const Lz4CBlock* __restrict__ blocks, int nblocks,
+                                                          uint32_t* __restrict__ csize) {
+  extern __shared__ uint32_t s_lz[];
+  const int b = blockIdx.x;
+  if (b >= nblocks) return;
+  const int lane = threadIdx.x;
+  const Lz4CBlock bl = blocks[b];
+  const uint32_t len = bl.len;
+  uint32_t* s_in = s_lz;                                 // kLz4Block + 8 bytes
+  uint16_t* s_ht = (uint16_t*)(s_lz + (kLz4Block + 8) / 4);  // 2^kLz4HashLog positions
+  // stage: aligned source dwords realigned into LDS (each dword loaded holds a byte of the block)
+  {
+    const uint64_t a = (uint64_t)bl.src;
+    const uint32_t m = (uint32_t)(a & 3);
+    const uint32_t* sa = (const uint32_t*)(a - m);
+    const uint32_t nw = (len + 3) / 4;
+    for (uint32_t i = lane; i < nw; i += kWave) {
+      const uint32_t w0 = gld(sa + i);
+      const uint32_t w1 = (m && (i + 1) * 4 < len + m) ? gld(sa + i + 1) : 0u;
+      s_in[i] = __builtin_amdgcn_alignbyte(w1, w0, m);
+    }
+    for (uint32_t i = nw + lane; i < nw + 2; i += kWave) s_in[i] = 0;
+    for (int i = lane; i < (1 << kLz4HashLog); i += kWave) s_ht[i] = 0;
+  }
+  __syncthreads();
+  if (len < 13) {  // no room for a match: stored
+    if (lane == 0) csize[b] = 0;
+    return;
+  }
+  const uint32_t mflimit = len - 12;     // last position a match may start at
+  const uint32_t matchlimit = len - 5;   // a match ends at or before this
+  const uint32_t cap = len;              // compressed output must stay below the input size
+  uint8_t* out = bl.dst;
+  uint32_t anchor = 0, cur = 0, op = 0;
+  bool fail = false;
+  // writes a length extension of value v (v >= 0 after the 15 in the nibble) at op
+  auto ext = [&](uint32_t v) {
+    const uint32_t cnt = v / 255 + 1;
+    for (uint32_t k = lane; k < cnt; k += kWave) gst(out + op + k, (uint8_t)(k + 1 < cnt ? 255u : v % 255u));
+    op += cnt;
+  };
+  auto literals = [&](uint32_t from, uint32_t n) {
+    for (uint32_t k = lane; k < n; k += kWave) gst(out + op + k, (uint8_t)lds_byte(s_in, from + k));
+    op += n;
+  };
+  for (uint32_t base = 0; base <= mflimit && !fail; base += kWave) {
+    const uint32_t p = base + lane;
+    const bool active = p <= mflimit;
+    const uint32_t v = active ? lds_rd32(s_in, p) : 0u;
+    const uint32_t h = (v * 2654435761u) >> (32 - kLz4HashLog);
+    const uint32_t cand = active ? s_ht[h] : 0u;
+    __builtin_amdgcn_wave_barrier();  // every lane reads the table before any lane writes it
+    if (active) s_ht[h] = (uint16_t)p;
+    uint32_t mlen = 0;
+    if (active && p >= cur && cand < p && lds_rd32(s_in, cand) == v) {
+      const uint32_t lim = min(matchlimit - p, kLz4MaxExt);
+      uint32_t l = 4;
+      while (l + 4 <= lim) {
+        const uint32_t x = lds_rd32(s_in, cand + l) ^ lds_rd32(s_in, p + l);
+        if (x) {
+          l += (uint32_t)__builtin_ctz(x) >> 3;
+          break;
+        }
+        l += 4;
+      }
+      if (l + 4 > lim)
+        while (l < lim && lds_byte(s_in, cand + l) == lds_byte(s_in, p + l)) l++;
+      mlen = l >= 4 && l <= lim ? l : 0;
+    }
+    for (uint64_t has = __ballot(mlen >= 4); has; has &= has - 1) {
+      const int l = __builtin_ctzll(has);
+      const uint32_t q = base + (uint32_t)l;
+      if (q < cur) continue;
+      const uint32_t ml = (uint32_t)__shfl((int)mlen, l, kWave);
+      const uint32_t off = q - (uint32_t)__shfl((int)cand, l, kWave);
+      const uint32_t lit = q - anchor;
+      const uint32_t need = 1 + (lit >= 15 ? (lit - 15) / 255 + 1 : 0) + lit + 2 +
+                            (ml - 4 >= 15 ? (ml - 4 - 15) / 255 + 1 : 0);
+      if (op + need >= cap) {
+        fail = true;
+        break;
+      }
+      if (lane == 0) gst(out + op, (uint8_t)((min(lit, 15u) << 4) | min(ml - 4, 15u)));
+      op++;
+      if (lit >= 15) ext(lit - 15);
+      literals(anchor, lit);
+      if (lane == 0) {
+        gst(out + op, (uint8_t)(off & 0xFF));
+        gst(out + op + 1, (uint8_t)(off >> 8));
+      }
+      op += 2;
+      if (ml - 4 >= 15) ext(ml - 4 - 15);
+      anchor = cur = q + ml;
+    }
+  }
+  if (!fail) {  // last literals
+    const uint32_t lit = len - anchor;
+    const uint32_t need = 1 + (lit >= 15 ? (lit - 15) / 255 + 1 : 0) + lit;
+    if (op + need >= cap) {
+      fail = true;
+    } else {
+      if (lane == 0) gst(out + op, (uint8_t)(min(lit, 15u) << 4));
+      op++;
+      if (lit >= 15) ext(lit - 15);
+      literals(anchor, lit);
+    }
+  }
+  if (lane == 0) csize[b] = fail ? 0u : op;
+}
+
 }  // namespace gz
 
 struct gz_ipc_plan {
@@ -1624,13 +1772,22 @@ struct gz_ipc_plan {
   std::vector<IpcPiece> pieces;
   std::vector<uint8_t> blob;  // metadata and schema bytes, copied by COPY tasks
   DBuf d_tasks, d_pieces, d_blob;
+  // record batches of the uncompressed layout (what compression rewrites)
+  struct BatchRec {
+    int p;
+    int64_t rows;
+    std::vector<std::pair<int64_t, int64_t>> nodes, bufs;
+    int64_t body_pos;
+  };
+  std::vector<BatchRec> recs;
+  std::vector<uint8_t> schema;  // written first in every non-empty partition, when non-empty
+  DBuf raw, comp, d_cblocks, d_csize;  // LZ4: uncompressed stream, compressed blocks
 };
 
 static void ipc_plan_free(gz_context* ctx, gz_ipc_plan* pl) {
   if (!pl) return;
-  ctx_free(ctx, &pl->d_tasks);
-  ctx_free(ctx, &pl->d_pieces);
-  ctx_free(ctx, &pl->d_blob);
+  for (DBuf* b : {&pl->d_tasks, &pl->d_pieces, &pl->d_blob, &pl->raw, &pl->comp, &pl->d_cblocks, &pl->d_csize})
+    ctx_free(ctx, b);
   delete pl;
 }
 
@@ -1790,6 +1947,9 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
   size_t ov = 0;  // next pair of oval
   // ---- pass 2: layout of the stream, metadata, tasks
   plan->part_len.assign(P, 0);
+  plan->recs.clear();
+  plan->schema.assign(write_schema && schema_len > 0 ? schema : nullptr,
+                      write_schema && schema_len > 0 ? schema + schema_len : nullptr);
   plan->tasks.clear();
   plan->pieces.clear();
   plan->blob.clear();
@@ -1902,6 +2062,7 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
       }
       const std::vector<uint8_t> msg = ipc_record_batch_message(bt.rows, nodes, bufs, boff);
       add_copy(msg.data(), true, (int64_t)msg.size(), (int64_t)msg.size());
+      plan->recs.push_back({p, bt.rows, nodes, bufs, pos});
       // body tasks
       const int64_t body0 = pos;
       size_t sb = 0;
@@ -2018,6 +2179,163 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
   return GZ_OK;
 }
 
+// Uploads a plan's blob, pieces and tasks and assembles its stream into dst.
+static int ipc_assemble(gz_context* ctx, gz_ipc_plan* pl, void* dst) {
+  GZ_TRY(upload(ctx, &pl->d_blob, pl->blob.data(), std::max<size_t>(pl->blob.size(), 1)));
+  // blob pieces point into the uploaded blob
+  for (auto& t : pl->tasks)
+    if (t.pad == 1) {
+      IpcPiece& pc = pl->pieces[t.p0];
+      pc.src = (const uint8_t*)pl->d_blob.p + pc.a;
+    }
+  GZ_TRY(upload(ctx, &pl->d_pieces, pl->pieces.data(), sizeof(IpcPiece) * std::max<size_t>(pl->pieces.size(), 1)));
+  GZ_TRY(upload(ctx, &pl->d_tasks, pl->tasks.data(), sizeof(IpcTask) * std::max<size_t>(pl->tasks.size(), 1)));
+  const unsigned g = (unsigned)std::min<size_t>(pl->tasks.size(), 1 << 20);
+  {
+    KernelTimer kt(ctx, "split_ipc");
+    hipLaunchKernelGGL(ipc_assemble_kernel, dim3(g), dim3(256), 0, ctx->stream, (const IpcTask*)pl->d_tasks.p,
+                       (int64_t)pl->tasks.size(), (const IpcPiece*)pl->d_pieces.p, (uint8_t*)dst);
+  }
+  GZ_HIP(hipGetLastError());
+  return GZ_OK;
+}
+
+// LZ4_FRAME bodies: the uncompressed stream is assembled into scratch, every
+// non-empty body buffer is cut into 32 KiB blocks compressed on the device,
+// and the plan is rewritten for the compressed stream: per buffer the int64
+// uncompressed length, one LZ4 frame (independent blocks, no checksums; a block
+// that does not shrink is stored), buffers 8-byte padded; record batch
+// messages carry BodyCompression{LZ4_FRAME, BUFFER}.
+static int ipc_compress_plan(gz_splitter* h, gz_ipc_plan* plan) {
+  gz_context* ctx = h->ctx;
+  hipStream_t st = ctx->stream;
+  if (plan->total == 0) return GZ_OK;
+  GZ_TRY(ctx_alloc(ctx, (size_t)plan->total + 8, &plan->raw));
+  GZ_TRY(ipc_assemble(ctx, plan, plan->raw.p));
+  const uint8_t* raw = (const uint8_t*)plan->raw.p;
+  constexpr int64_t kBlock = kLz4Block;
+  std::vector<Lz4CBlock> blocks;
+  int64_t coff = 0;
+  for (const auto& r : plan->recs)
+    for (const auto& b : r.bufs)
+      for (int64_t k = 0; k < b.second; k += kBlock) {
+        const int64_t n = std::min(kBlock, b.second - k);
+        blocks.push_back({raw + r.body_pos + b.first + k, (uint8_t*)(intptr_t)coff, (uint32_t)n, 0});
+        coff += (n + 15) / 16 * 16;
+      }
+  std::vector<uint32_t> csize(std::max<size_t>(blocks.size(), 1), 0);
+  if (!blocks.empty()) {
+    GZ_TRY(ctx_alloc(ctx, (size_t)coff, &plan->comp));
+    for (auto& b : blocks) b.dst = (uint8_t*)plan->comp.p + (intptr_t)b.dst;
+    GZ_TRY(upload(ctx, &plan->d_cblocks, blocks.data(), sizeof(Lz4CBlock) * blocks.size()));
+    GZ_TRY(ctx_alloc(ctx, sizeof(uint32_t) * blocks.size(), &plan->d_csize));
+    const size_t lds = kLz4Block + 8 + 2 * (1 << kLz4HashLog);
+    {
+      KernelTimer kt(ctx, "split_ipc_lz4");
+      hipLaunchKernelGGL(lz4_compress_kernel, dim3((unsigned)blocks.size()), dim3(64), lds, st,
+                         (const Lz4CBlock*)plan->d_cblocks.p, (int)blocks.size(), (uint32_t*)plan->d_csize.p);
+    }
+    GZ_HIP(hipGetLastError());
+    GZ_HIP(hipMemcpyAsync(csize.data(), plan->d_csize.p, sizeof(uint32_t) * blocks.size(), hipMemcpyDeviceToHost, st));
+  }
+  GZ_HIP(hipStreamSynchronize(st));
+  // the compressed stream
+  const int P = h->P;
+  plan->tasks.clear();
+  plan->pieces.clear();
+  plan->blob.clear();
+  plan->part_len.assign(P, 0);
+  int64_t pos = 0;
+  auto blob_copy = [&](const std::vector<uint8_t>& bytes, int64_t out) {
+    IpcTask t{};
+    t.kind = IPC_COPY;
+    t.p0 = (int64_t)plan->pieces.size();
+    t.np = 1;
+    t.dst = pos;
+    t.n = (int64_t)bytes.size();
+    t.out = out;
+    t.pad = 1;
+    IpcPiece pc{};
+    pc.a = (int64_t)plan->blob.size();
+    plan->blob.insert(plan->blob.end(), bytes.begin(), bytes.end());
+    plan->pieces.push_back(pc);
+    plan->tasks.push_back(t);
+    pos += out;
+  };
+  auto dev_copy = [&](const uint8_t* src, int64_t n) {
+    IpcTask t{};
+    t.kind = IPC_COPY;
+    t.p0 = (int64_t)plan->pieces.size();
+    t.np = 1;
+    t.dst = pos;
+    t.n = n;
+    t.out = n;
+    IpcPiece pc{};
+    pc.src = src;
+    plan->pieces.push_back(pc);
+    plan->tasks.push_back(t);
+    pos += n;
+  };
+  auto le32 = [](std::vector<uint8_t>& v, uint32_t x) {
+    for (int i = 0; i < 4; i++) v.push_back((uint8_t)(x >> (8 * i)));
+  };
+  // frame header: magic, FLG (version 01, independent blocks), BD (64 KiB maximum
+  // blocks), header checksum = second byte of XXH32(FLG, BD) with seed 0
+  static const uint8_t kFrameHeader[7] = {0x04, 0x22, 0x4D, 0x18, 0x60, 0x40, 0x82};
+  size_t bi = 0, ri = 0;
+  for (int p = 0; p < P; p++) {
+    const int64_t start = pos;
+    bool any = false;
+    for (; ri < plan->recs.size() && plan->recs[ri].p == p; ri++) {
+      const auto& r = plan->recs[ri];
+      if (!any && !plan->schema.empty()) blob_copy(plan->schema, (int64_t)plan->schema.size());
+      any = true;
+      // compressed buffer sizes
+      std::vector<std::pair<int64_t, int64_t>> cb;
+      int64_t boff = 0;
+      size_t bj = bi;
+      for (const auto& b : r.bufs) {
+        if (b.second == 0) {
+          cb.push_back({boff, 0});
+          continue;
+        }
+        int64_t clen = 8 + 7 + 4;
+        for (int64_t k = 0; k < b.second; k += kBlock, bj++) {
+          const int64_t n = std::min(kBlock, b.second - k);
+          clen += 4 + (csize[bj] ? (int64_t)csize[bj] : n);
+        }
+        cb.push_back({boff, clen});
+        boff += pad8(clen);
+      }
+      const std::vector<uint8_t> msg = ipc_record_batch_message(r.rows, r.nodes, cb, boff, true);
+      blob_copy(msg, (int64_t)msg.size());
+      for (size_t q = 0; q < r.bufs.size(); q++) {
+        const auto& b = r.bufs[q];
+        if (b.second == 0) continue;
+        const int64_t buf0 = pos;
+        std::vector<uint8_t> hdr(8);
+        memcpy(hdr.data(), &b.second, 8);  // uncompressed length
+        hdr.insert(hdr.end(), kFrameHeader, kFrameHeader + 7);
+        for (int64_t k = 0; k < b.second; k += kBlock, bi++) {
+          const int64_t n = std::min(kBlock, b.second - k);
+          const bool stored = csize[bi] == 0;
+          le32(hdr, stored ? (uint32_t)n | 0x80000000u : csize[bi]);
+          blob_copy(hdr, (int64_t)hdr.size());
+          hdr.clear();
+          if (stored) dev_copy(raw + r.body_pos + b.first + k, n);
+          else dev_copy(blocks[bi].dst, csize[bi]);
+        }
+        le32(hdr, 0);  // end mark
+        blob_copy(hdr, pad8(pos + 4 - buf0) - (pos - buf0));
+      }
+    }
+    if (any) blob_copy(std::vector<uint8_t>{0xff, 0xff, 0xff, 0xff, 0, 0, 0, 0}, 8);
+    plan->part_len[p] = pos - start;
+  }
+  plan->total = pos;
+  return GZ_OK;
+}
+
 extern "C" {
 
 int gz_ipc_record_batch_message(int64_t num_rows, int32_t num_nodes, const int64_t* nodes, int32_t num_buffers,
@@ -2035,6 +2353,14 @@ int gz_ipc_record_batch_message(int64_t num_rows, int32_t num_nodes, const int64
   return GZ_OK;
 }
 
+int gz_split_set_compress_type(gz_splitter* h, int32_t codec) {
+  if (!h) return fail(GZ_INVALID, "null splitter");
+  if (codec != GZ_COMPRESSION_UNCOMPRESSED && codec != GZ_COMPRESSION_LZ4_FRAME)
+    return fail(GZ_NOT_IMPLEMENTED, "compression type " + std::to_string(codec) + " (UNCOMPRESSED or LZ4_FRAME)");
+  h->codec = codec;
+  return GZ_OK;
+}
+
 int gz_split_ipc_prepare(gz_splitter* h, const uint8_t* schema_msg, int64_t schema_len, int32_t write_schema,
                          int64_t* total_bytes, int64_t* partition_lengths) {
   if (!h) return fail(GZ_INVALID, "null splitter");
@@ -2043,6 +2369,7 @@ int gz_split_ipc_prepare(gz_splitter* h, const uint8_t* schema_msg, int64_t sche
   if (h->ipc) ipc_plan_free(h->ctx, h->ipc);
   h->ipc = new gz_ipc_plan();
   int rc = ipc_prepare(h, schema_msg, schema_len, write_schema != 0, h->ipc);
+  if (!rc && h->codec == GZ_COMPRESSION_LZ4_FRAME) rc = ipc_compress_plan(h, h->ipc);
   if (rc) {
     ipc_plan_free(h->ctx, h->ipc);
     h->ipc = nullptr;
@@ -2062,22 +2389,7 @@ int gz_split_ipc_write(gz_splitter* h, void* dst, int64_t dst_bytes) {
   if (!dst) return fail(GZ_INVALID, "null destination");
   gz_context* ctx = h->ctx;
   GZ_HIP(hipSetDevice(ctx->device));
-  GZ_TRY(upload(ctx, &pl->d_blob, pl->blob.data(), std::max<size_t>(pl->blob.size(), 1)));
-  // blob pieces point into the uploaded blob
-  for (auto& t : pl->tasks)
-    if (t.pad == 1) {
-      IpcPiece& pc = pl->pieces[t.p0];
-      pc.src = (const uint8_t*)pl->d_blob.p + pc.a;
-    }
-  GZ_TRY(upload(ctx, &pl->d_pieces, pl->pieces.data(), sizeof(IpcPiece) * std::max<size_t>(pl->pieces.size(), 1)));
-  GZ_TRY(upload(ctx, &pl->d_tasks, pl->tasks.data(), sizeof(IpcTask) * std::max<size_t>(pl->tasks.size(), 1)));
-  const unsigned g = (unsigned)std::min<size_t>(pl->tasks.size(), 1 << 20);
-  {
-    KernelTimer kt(ctx, "split_ipc");
-    hipLaunchKernelGGL(ipc_assemble_kernel, dim3(g), dim3(256), 0, ctx->stream, (const IpcTask*)pl->d_tasks.p,
-                       (int64_t)pl->tasks.size(), (const IpcPiece*)pl->d_pieces.p, (uint8_t*)dst);
-  }
-  GZ_HIP(hipGetLastError());
+  GZ_TRY(ipc_assemble(ctx, pl, dst));
   GZ_HIP(hipStreamSynchronize(ctx->stream));
   return GZ_OK;
 }

@@ -138,6 +138,14 @@ class Splitter:
         segs = [self.segment(s) for s in range(self.num_segments())]
         return concat_segments(segs, self.output_types, self.num_partitions)
 
+    def SetCompressType(self, codec: str):
+        """``Splitter::SetCompressType`` (splitter.cc:371-385) for the data file:
+        "uncompressed" or "lz4" (LZ4_FRAME, as the reference maps Spark's lz4)."""
+        codes = {"uncompressed": 0, "none": 0, "lz4": 6, "lz4_frame": 6}
+        if codec.lower() not in codes:
+            raise L.NotImplementedStatus(10, f"compression type {codec}")
+        L.check(L.lib().gz_split_set_compress_type(self._h, codes[codec.lower()]))
+
     def ipc_stream(self, schema_message: bytes = b"", write_schema: bool = True):
         """The data file Splitter::Stop writes (partitions' Arrow IPC streams, one
         after another, uncompressed), assembled on the device and copied to the

@@ -174,6 +174,17 @@ int gz_split_segment_batch_counts(gz_splitter* h, int32_t seg, int32_t* num_batc
 int64_t gz_split_compute_pid_time(gz_splitter* h);
 int gz_split_release(gz_splitter* h);
 
+/* Body compression of the partition writer, as arrow::Compression::type values. */
+#define GZ_COMPRESSION_UNCOMPRESSED 0
+#define GZ_COMPRESSION_LZ4_FRAME 6
+/* Replaces Splitter::SetCompressType (splitter.cc:371-385) for the partition
+ * writer: GZ_COMPRESSION_LZ4_FRAME makes gz_split_ipc_prepare compress every
+ * non-empty body buffer on the device (int64 uncompressed length + one LZ4
+ * frame of independent 32 KiB blocks, a block that does not shrink stored) and
+ * mark record batches with BodyCompression{LZ4_FRAME, BUFFER}, as Arrow's IPC
+ * writer does with that codec; other codecs are NotImplemented. */
+int gz_split_set_compress_type(gz_splitter* h, int32_t codec);
+
 /* Partition writer: replaces the data file Splitter::Stop writes through its
  * PartitionWriters (splitter.cc:70-185, :395-437) with the default
  * compression_type UNCOMPRESSED (shuffle/type.h:44).  The stream holds, partition

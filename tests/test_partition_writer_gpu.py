@@ -95,3 +95,127 @@ def test_data_file_decodes_to_the_split(ctx, batched, buffer_size):
         table = pa.Table.from_batches(rbs, schema=schema)
         _check_partition([table.column(c).combine_chunks() for c in range(len(types))], exp, types, p)
     assert pos == len(data)
+
+
+def _compressible_batches(types, sizes, seed):
+    """Random batches whose values repeat (small ranges, a small vocabulary),
+    so LZ4 blocks really compress; nulls in every nullable column."""
+    from gazelle_plugin_amd.batch import HostBatch, HostColumn, bitmap_from_bools
+
+    rng = np.random.default_rng(seed)
+    words = [b"alpha", b"beta", b"gamma", b"delta-" * 3, b""]
+    out = []
+    for n in sizes:
+        cols = []
+        for t in types:
+            valid = rng.random(n) > 0.1
+            vb = bitmap_from_bools(valid)
+            if t == T.STRING or t == T.BINARY:
+                vals = [words[i] for i in rng.integers(0, len(words), n)]
+                off = np.zeros(n + 1, np.int32)
+                off[1:] = np.cumsum([len(v) for v in vals])
+                cols.append(HostColumn(t, np.frombuffer(b"".join(vals), np.uint8).copy(), vb, off))
+            elif t == T.BOOL:
+                cols.append(HostColumn(t, bitmap_from_bools(rng.random(n) < 0.5), vb))
+            elif t == T.NA:
+                cols.append(HostColumn(t, None))
+            elif t == T.DOUBLE:
+                cols.append(HostColumn(t, rng.integers(0, 8, n).astype(np.float64) / 4, vb))
+            else:
+                cols.append(HostColumn(t, rng.integers(0, 20, n).astype(T.numpy_dtype(t)), vb))
+        out.append(HostBatch(n, cols))
+    return out
+
+
+@pytest.mark.parametrize("compressible", [True, False])
+def test_lz4_data_file_decodes_to_the_split(ctx, compressible):
+    """With LZ4_FRAME bodies (SetCompressType("lz4")) every partition still
+    decodes with pyarrow to the oracle's rows, batch cuts unchanged; the device
+    reader decodes the same file back to the same rows."""
+    from gazelle_plugin_amd.shuffle_reader import ShuffleReader
+
+    types = [T.INT64, T.INT32, T.DOUBLE, T.STRING, T.BOOL, T.INT8, T.NA, T.BINARY]
+    sizes = [70000, 0, 1, 4096, 3000]
+    batches = (_compressible_batches(types, sizes, 23) if compressible
+               else random_batches(types, sizes, seed=23, null_frac=0.15))
+    P = 5
+    exp = oracle.split(batches, types, P, [0])
+    pids = oracle.split_pids(batches, types, P, [0])
+    schema = pa.schema([(f"c{i}", ARROW[t]) for i, t in enumerate(types)])
+    files = {}
+    for codec in ("uncompressed", "lz4"):
+        sp = Splitter.Make(ctx, "hash", types, P, [0], SplitOptions(buffer_size=20000))
+        sp.SetCompressType(codec)
+        sp.SplitBatches([DeviceBatch.from_host(b) for b in batches])
+        sp.Stop()
+        files[codec] = sp.ipc_stream(schema.serialize().to_pybytes())
+        sp.close()
+    data, lengths = files["lz4"]
+    assert int(lengths.sum()) == len(data)
+    if compressible:
+        assert len(data) < 0.7 * len(files["uncompressed"][0])
+    pos = 0
+    reader = ShuffleReader(types, ctx)
+    for p in range(P):
+        n = int(exp["row_start"][p + 1] - exp["row_start"][p])
+        part = data[pos:pos + lengths[p]]
+        pos += int(lengths[p])
+        if n == 0:
+            assert lengths[p] == 0
+            continue
+        rd = pa.ipc.open_stream(pa.py_buffer(part.tobytes()))
+        assert rd.schema.equals(schema)
+        rbs = list(rd)
+        counts, off = [], 0
+        for b in batches:
+            counts.append(int((pids[off:off + b.num_rows] == p).sum()))
+            off += b.num_rows
+        assert [rb.num_rows for rb in rbs] == reference_batch_rows(counts, 20000)
+        table = pa.Table.from_batches(rbs, schema=schema)
+        _check_partition([table.column(c).combine_chunks() for c in range(len(types))], exp, types, p)
+        # the device reader: the decoded batches split again reproduce the partition
+        dev = reader.read(part)
+        assert [b.num_rows for b in dev] == [rb.num_rows for rb in rbs]
+        sp = Splitter.Make(ctx, "single", types, 1)
+        sp.SplitBatches(dev)
+        sp.Stop()
+        got = sp.read_partitions()
+        sp.close()
+        for c, t in enumerate(types):
+            if t == T.NA:
+                continue
+            g = got["columns"][c]
+            e = exp["columns"][c]
+            r0, r1 = int(exp["row_start"][p]), int(exp["row_start"][p + 1])
+            gv = np.ones(n, bool) if g["validity"] is None else _bits(g["validity"], 0, n)
+            ev = np.ones(n, bool) if e.get("validity") is None else _bits(e["validity"], r0, n)
+            np.testing.assert_array_equal(gv, ev, err_msg=f"validity p={p} c={c}")
+    assert pos == len(data)
+
+
+def test_lz4_encoder_block_format_cases(ctx):
+    """The device encoder on the block format's cases (incompressible data, runs
+    of periods 1..70000 with matches longer than one sequence takes, long
+    literal runs, inputs below the 13-byte minimum, several 64 KiB blocks):
+    each case is the one value of a single-partition binary column, so its
+    bytes are one body buffer; pyarrow's LZ4 decoder must give them back."""
+    from test_oracle_lz4 import _cases
+
+    cases = [d for _, d in _cases()]
+    types = [T.BINARY]
+    schema = pa.schema([("v", pa.binary())])
+    from gazelle_plugin_amd.batch import HostBatch, HostColumn
+
+    for d in cases:
+        off = np.array([0, len(d)], np.int32)
+        hb = HostBatch(1, [HostColumn(T.BINARY, np.frombuffer(d, np.uint8).copy(), None, off)])
+        sp = Splitter.Make(ctx, "single", types, 1)
+        sp.SetCompressType("lz4")
+        sp.Split(DeviceBatch.from_host(hb))
+        sp.Stop()
+        data, lengths = sp.ipc_stream(schema.serialize().to_pybytes())
+        sp.close()
+        rbs = list(pa.ipc.open_stream(pa.py_buffer(data.tobytes())))
+        assert len(rbs) == 1 and rbs[0].column(0).to_pylist() == [d]
+        if len(d) >= 1 << 20:  # a megabyte of zeros: 16 blocks that really compress
+            assert len(data) < len(d) // 50
