@@ -692,6 +692,56 @@ struct JoinRow {
   }
 };
 
+// A joined row's used input columns loaded once into registers (integers
+// widened as read_i64, floats as double bits, 8-byte types raw), so the lane
+// updates read no memory: every input load of the pair is issued before the
+// first update.
+struct RegRow {
+  const AggParams& p;
+  int64_t w0, w1, w2, w3, w4, w5, w6, w7;
+  uint32_t nulls;  // bit k: used column k is null
+  __device__ __forceinline__ int64_t word(int col) const {
+    // a select chain (a switch becomes a lookup table in scratch memory)
+    const int u = p.uslot[col];
+    const int64_t a = u & 1 ? w1 : w0, b = u & 1 ? w3 : w2, c = u & 1 ? w5 : w4, d = u & 1 ? w7 : w6;
+    const int64_t e = u & 2 ? b : a, f = u & 2 ? d : c;
+    return u & 4 ? f : e;
+  }
+  __device__ __forceinline__ bool valid(int col) const { return !((nulls >> p.uslot[col]) & 1); }
+  __device__ __forceinline__ int64_t i64(int col) const { return word(col); }
+  __device__ __forceinline__ double f64(int col) const {
+    const int64_t x = word(col);
+    const int t = p.col_type[col];
+    if (t == GZ_TYPE_DOUBLE || t == GZ_TYPE_FLOAT) return __longlong_as_double(x);
+    if (t == GZ_TYPE_UINT64) return (double)(uint64_t)x;
+    return (double)x;
+  }
+  __device__ __forceinline__ uint64_t u64(int col) const { return (uint64_t)word(col); }
+  __device__ __forceinline__ int64_t raw64(int col) const { return word(col); }
+};
+
+// Word of a join input column for RegRow; *ok = not null.
+__device__ __forceinline__ int64_t join_word(const JoinAggParams& q, int col, int batch, int64_t r, int64_t br,
+                                             bool* ok) {
+  const JoinIn& c = q.in[col];
+  const void* v;
+  int64_t row;
+  if (c.side == 0) {
+    *ok = c.bvalid[br] != 0;
+    v = c.bvalues;
+    row = br;
+    if (c.type == GZ_TYPE_BOOL) return (int64_t)c.bvalues[br];
+  } else {
+    const uint8_t* vb = (const uint8_t*)q.bufs[(int64_t)batch * q.bstride + c.idx * 3];
+    *ok = !vb || bit_get(vb, r);
+    v = q.bufs[(int64_t)batch * q.bstride + c.idx * 3 + 2];
+    row = r;
+  }
+  if (c.type == GZ_TYPE_DOUBLE) return ((const int64_t*)v)[row];
+  if (c.type == GZ_TYPE_FLOAT) return __double_as_longlong((double)((const float*)v)[row]);
+  return read_i64(v, c.type, row);
+}
+
 // Persistent workgroups over consecutive probe tiles, an LDS first-level group
 // table each (as the plain aggregate's LDS path: a key that finds no LDS slot
 // within a short probe goes to the HBM table), merged into HBM at the end.
@@ -742,7 +792,7 @@ __global__ __launch_bounds__(256) void join_agg_update_kernel(AggParams p, JoinA
         ls = p.lcap + 1;
       } else {
         uint64_t s = fmix64((uint64_t)key) & lmask;
-        for (int i = 0; i < 32; i++) {  // short probe: a full LDS table spills to HBM
+        for (int i = 0; i < 8; i++) {  // short probe: a full LDS table spills to HBM
           const int64_t k = s_tab[s];
           if (k == key) {
             ls = (int64_t)s;
@@ -763,9 +813,23 @@ __global__ __launch_bounds__(256) void join_agg_update_kernel(AggParams p, JoinA
       if (ls < 0 && g < 0) continue;  // HBM table overflow (flagged)
       for (int64_t j = 0; j < cnt; j++) {
         const int64_t br = q.jrowlist ? (int64_t)gld(q.jrowlist + st + j) : st;
-        const JoinRow row{q, td.batch, r, br};
-        if (ls >= 0) apply_row(p, s_tab, LS, ls, row, grow);
-        else apply_row(p, I.base, GS, g, row, grow);
+        if (p.nu >= 0) {
+          int64_t w[kMaxUsed];
+          uint32_t nulls = 0;
+#pragma unroll
+          for (int k = 0; k < kMaxUsed; k++) {
+            bool ok = true;
+            w[k] = k < p.nu ? join_word(q, p.ucol[k], td.batch, r, br, &ok) : 0;
+            if (!ok) nulls |= 1u << k;
+          }
+          const RegRow row{p, w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], nulls};
+          if (ls >= 0) apply_row(p, s_tab, LS, ls, row, grow);
+          else apply_row(p, I.base, GS, g, row, grow);
+        } else {
+          const JoinRow row{q, td.batch, r, br};
+          if (ls >= 0) apply_row(p, s_tab, LS, ls, row, grow);
+          else apply_row(p, I.base, GS, g, row, grow);
+        }
       }
     }
   }
@@ -2461,12 +2525,14 @@ int gz_join_agg_process_batches(gz_agg* h, int32_t nb, const gz_batch* batches) 
       in.bvalues = (const uint8_t*)v.values[in.idx];
     }
   }
-  int64_t blocks = std::min<int64_t>(q.T, (int64_t)ctx->num_cus * 2);
+  // four workgroups per CU: the LDS table holds the head of a skewed key
+  // distribution (512 keys), the rest spills to HBM
+  int64_t blocks = std::min<int64_t>(q.T, (int64_t)ctx->num_cus * 4);
   q.tiles_per_block = (int)((q.T + blocks - 1) / blocks);
   blocks = (q.T + q.tiles_per_block - 1) / q.tiles_per_block;
-  p.lcap = 1024;
+  p.lcap = 512;
   int lds = lds_bytes(h, p.lcap);
-  while (lds > 96 * 1024 && p.lcap > 64) {
+  while (lds > 40 * 1024 && p.lcap > 64) {
     p.lcap >>= 1;
     lds = lds_bytes(h, p.lcap);
   }
