@@ -1726,8 +1726,9 @@ __global__ __launch_bounds__(64) void lz4_compress_kernel(const Lz4CBlock* __res
       const int l = __builtin_ctzll(has);
       const uint32_t q = base + (uint32_t)l;
       if (q < cur) continue;
-      const uint32_t ml = (uint32_t)__shfl((int)mlen, l, kWave);
-      const uint32_t off = q - (uint32_t)__shfl((int)cand, l, kWave);
+      // lane l's match, read into scalar registers (v_readlane, no LDS round trip)
+      const uint32_t ml = (uint32_t)__builtin_amdgcn_readlane((int)mlen, l);
+      const uint32_t off = q - (uint32_t)__builtin_amdgcn_readlane((int)cand, l);
       const uint32_t lit = q - anchor;
       const uint32_t need = 1 + (lit >= 15 ? (lit - 15) / 255 + 1 : 0) + lit + 2 +
                             (ml - 4 >= 15 ? (ml - 4 - 15) / 255 + 1 : 0);
@@ -2757,123 +2758,120 @@ __device__ __forceinline__ int64_t wave_min_i64(int64_t v) {
 // pending match is always among them, and matches reaching far back (the usual
 // case) all go in the first round.  Short runs are copied by their lane, long
 // ones by the whole wave.
-// A 1 KiB window of a block's input held across the wave's registers (16
-// bytes per lane), read byte by byte with v_readlane: the serial sequence parse
-// then costs no memory latency per byte.  Lane chunks past the frame's end mark
-// are not loaded; an aligned 16-byte chunk holding a valid byte never crosses a
-// page.
-struct InWindow {
-  const uint8_t* blk;
-  uint32_t blen;
-  uint64_t wp;  // window start (16-byte aligned address)
-  u32x4 w;
-  int lane;
-  __device__ void load(uint32_t pos) {
-    wp = (uint64_t)(blk + pos) & ~15ull;
-    const uint64_t c = wp + 16ull * lane, lim = (uint64_t)(blk + blen) + 4;
-    w = c < lim ? gld((const u32x4*)c) : u32x4{0, 0, 0, 0};
-  }
-  __device__ bool has(uint32_t pos, uint32_t k) const {
-    const uint64_t a = (uint64_t)(blk + pos);
-    return a >= wp && a + k <= wp + 1024;
-  }
-  __device__ uint32_t byte(uint32_t pos) const {
-    const uint32_t q = (uint32_t)((uint64_t)(blk + pos) - wp);
-    const uint32_t sel = (q >> 2) & 3;
-    const uint32_t v = sel == 0 ? w.x : sel == 1 ? w.y : sel == 2 ? w.z : w.w;
-    const uint32_t x = __builtin_amdgcn_readlane(v, (int)(q >> 4));
-    return (x >> ((q & 3) * 8)) & 0xFFu;
-  }
-  __device__ uint32_t get(uint32_t pos) {
-    if (!has(pos, 1)) load(pos);
-    return byte(pos);
-  }
-};
+// v_writelane_b32: value (wave-uniform) into lane l of old, the other lanes kept.
+__device__ int gz_writelane(int value, int l, int old) __asm("llvm.amdgcn.writelane.i32");
+
+__device__ __forceinline__ int64_t readlane64(int64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 
 __device__ int64_t lz4_block(const uint8_t* blk, uint32_t blen, uint8_t* dst, int64_t op, int64_t fbase,
                              int64_t cap, int lane) {
   constexpr uint32_t kLong = 32;
-  InWindow win{blk, blen, 0, u32x4{0, 0, 0, 0}, lane};
-  win.load(0);
+  typedef const __attribute__((address_space(4))) u32x4* kptr_u32x4;
   uint32_t ip = 0;  // input offset within the block (blocks are at most 4 MiB)
   while (ip < blen) {
+    // The serial parse stays in scalar registers: 16 input bytes per scalar
+    // load, 32-bit bounds (output room and match reach refreshed per window),
+    // each sequence's fields written into lane n with v_writelane.
     int n = 0;
-    const uint8_t* my_lit = nullptr;
-    uint32_t my_lit_len = 0, my_off = 0, my_mlen = 0;
-    int64_t my_mdst = 0;
+    uint32_t my_lsrc = 0, my_lit_len = 0, my_off = 0, my_mlen = 0, my_dlo = 0, my_dhi = 0;
+    const int64_t op0 = op;
+    uint32_t room = cap - op > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)(cap - op);
+    uint32_t back = op - fbase > 65536 ? 65536u : (uint32_t)(op - fbase);
+    uint32_t rel = 0;  // output produced in this window
     while (n < kWave && ip < blen) {
-      uint32_t lit, off = 0, mlen = 0, lsrc;
-      // Fast path: token, literals and offset within 16 bytes, no length
-      // extensions.
-      if (!win.has(ip, 16)) win.load(ip);
-      const uint32_t token = win.byte(ip);
-      lit = token >> 4;
-      const uint32_t ml = token & 15;
-      if (lit < 15 && ml < 15 && lit <= 13) {
-        lsrc = ip + 1;
-        if (lsrc + lit >= blen) {  // the last sequence carries literals only
-          if (lsrc + lit > blen) return -1;
-          ip = blen;
-        } else {
-          off = win.byte(lsrc + lit) | (win.byte(lsrc + lit + 1) << 8);
-          mlen = ml + 4;
-          ip = lsrc + lit + 2;
-          if (ip > blen) return -1;
+      uint32_t lit = 0, off = 0, mlen = 0, lsrc = 0;
+      bool fast = false;
+      if (ip + 13 <= blen + 4) {
+        // 16 bytes from the dword holding ip: inside the frame, whose 4-byte
+        // end mark follows the last block
+        const uint64_t a = (uint64_t)(blk + ip);
+        const u32x4 w = *(kptr_u32x4)(a & ~3ull);
+        const uint32_t m = (uint32_t)(a & 3);
+        auto byte = [&](uint32_t q) {
+          const uint32_t x = q < 8 ? (q < 4 ? w.x : w.y) : (q < 12 ? w.z : w.w);
+          return (x >> ((q & 3) * 8)) & 0xFFu;
+        };
+        const uint32_t token = byte(m);
+        const uint32_t l = token >> 4, ml = token & 15;
+        if (l < 15 && ml < 15 && m + 3 + l <= 16) {
+          lit = l;
+          lsrc = ip + 1;
+          if (lsrc + lit >= blen) {  // the last sequence carries literals only
+            if (lsrc + lit > blen) return -1;
+            ip = blen;
+          } else {
+            off = byte(m + 1 + lit) | (byte(m + 2 + lit) << 8);
+            mlen = ml + 4;
+            ip = lsrc + lit + 2;
+            if (ip > blen) return -1;
+          }
+          fast = true;
         }
-      } else {  // byte by byte: length extensions
-        ip++;
-        uint64_t l = lit;
+      }
+      if (!fast) {  // byte by byte: length extensions, the end of the block
+        const uint32_t token = ubyte(blk + ip++);
+        uint32_t l = token >> 4;
         if (l == 15) {
           uint32_t b;
           do {
             if (ip >= blen) return -1;
-            b = win.get(ip++);
+            b = ubyte(blk + ip++);
             l += b;
-          } while (b == 255 && l < (1ull << 32));
+          } while (b == 255 && l <= blen);
         }
         if (l > blen - ip) return -1;
-        lit = (uint32_t)l;
+        lit = l;
         lsrc = ip;
         ip += lit;
         if (ip < blen) {
           if (blen - ip < 2) return -1;
-          off = win.get(ip);
-          off |= win.get(ip + 1) << 8;
+          off = ubyte(blk + ip) | (ubyte(blk + ip + 1) << 8);
           ip += 2;
-          uint64_t m = ml;
+          uint32_t m = token & 15;
           if (m == 15) {
             uint32_t b;
             do {
               if (ip >= blen) return -1;
-              b = win.get(ip++);
+              b = ubyte(blk + ip++);
               m += b;
-            } while (b == 255 && m < (1ull << 32));
+            } while (b == 255 && m <= 0x7FFFFFFFu);
           }
-          if (m + 4 >= (1ull << 32)) return -1;
-          mlen = (uint32_t)m + 4;
+          if (m > 0x7FFFFFFFu) return -1;
+          mlen = m + 4;
         }
       }
-      if ((int64_t)lit > cap - op) return -1;
-      op += lit;
-      if (mlen && (off == 0 || (int64_t)off > op - fbase || (int64_t)mlen > cap - op)) return -1;
-      if (lane == n) {
-        my_lit = blk + lsrc;
-        my_lit_len = lit;
-        my_off = off;
-        my_mlen = mlen;
-        my_mdst = op;
-      }
-      op += mlen;
+      if (lit > room) return -1;
+      room -= lit;
+      rel += lit;
+      back = back + lit > 65536u ? 65536u : back + lit;
+      if (mlen && (off == 0 || off > back || mlen > room)) return -1;
+      const int64_t mdst = op0 + rel;
+      my_lsrc = (uint32_t)gz_writelane((int)lsrc, n, (int)my_lsrc);
+      my_lit_len = (uint32_t)gz_writelane((int)lit, n, (int)my_lit_len);
+      my_off = (uint32_t)gz_writelane((int)off, n, (int)my_off);
+      my_mlen = (uint32_t)gz_writelane((int)mlen, n, (int)my_mlen);
+      my_dlo = (uint32_t)gz_writelane((int)(uint32_t)mdst, n, (int)my_dlo);
+      my_dhi = (uint32_t)gz_writelane((int)(uint32_t)((uint64_t)mdst >> 32), n, (int)my_dhi);
+      room -= mlen;
+      rel += mlen;
+      back = back + mlen > 65536u ? 65536u : back + mlen;
       n++;
     }
+    op = op0 + rel;
+    const uint8_t* my_lit = blk + my_lsrc;
+    const int64_t my_mdst = (int64_t)(((uint64_t)my_dhi << 32) | my_dlo);
     const bool has = lane < n;
     // (2) literals
     if (has && my_lit_len <= kLong) lane_copy(dst + my_mdst - my_lit_len, my_lit, my_lit_len);
     for (uint64_t lg = __ballot(has && my_lit_len > kLong); lg; lg &= lg - 1) {
       const int l = __builtin_ctzll(lg);
-      const uint8_t* s = (const uint8_t*)__shfl((long long)my_lit, l, kWave);
-      const uint32_t len = __shfl(my_lit_len, l, kWave);
-      const int64_t d = __shfl((long long)my_mdst, l, kWave) - len;
+      const uint8_t* s = (const uint8_t*)readlane64((int64_t)my_lit, l);
+      const uint32_t len = (uint32_t)__builtin_amdgcn_readlane((int)my_lit_len, l);
+      const int64_t d = readlane64(my_mdst, l) - len;
       wave_copy(dst + d, s, len, lane);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -2892,8 +2890,9 @@ __device__ int64_t lz4_block(const uint8_t* blk, uint32_t blen, uint8_t* dst, in
       }
       for (uint64_t lg = __ballot(ready && my_mlen > kLong); lg; lg &= lg - 1) {
         const int l = __builtin_ctzll(lg);
-        const int64_t d = __shfl((long long)my_mdst, l, kWave);
-        const uint32_t off = __shfl(my_off, l, kWave), len = __shfl(my_mlen, l, kWave);
+        const int64_t d = readlane64(my_mdst, l);
+        const uint32_t off = (uint32_t)__builtin_amdgcn_readlane((int)my_off, l),
+                       len = (uint32_t)__builtin_amdgcn_readlane((int)my_mlen, l);
         if (off >= len)
           wave_copy(dst + d, dst + d - off, len, lane);
         else
