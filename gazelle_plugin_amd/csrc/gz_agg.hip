@@ -83,6 +83,7 @@ struct AggParams {
   const AggInst* inst;
   int tiles_per_block;
   int lcap;
+  int lprobe;                 // LDS table slots a key tries before going to the HBM table
   // partitioned placement: the used input columns of a record
   int nu;                     // used columns (lane inputs other than the key)
   int ucol[kMaxUsed];         // column of used slot k
@@ -540,6 +541,7 @@ __global__ __launch_bounds__(256) void agg_update_kernel(AggParams p) {
   }
 }
 
+
 // LDS path: persistent workgroups over static chunks of consecutive tiles; an
 // LDS table of lcap slots (+2 special) per workgroup, merged into HBM whenever
 // the instance changes and at the end.
@@ -597,7 +599,7 @@ __global__ __launch_bounds__(256) void agg_update_lds_kernel(AggParams p) {
       else if (key == kEmpty) ls = p.lcap + 1;
       else {
         uint64_t s = fmix64((uint64_t)key) & lmask;
-        for (int i = 0; i < 32; i++) {  // short probe: a full LDS table spills to HBM
+        for (int i = 0; i < p.lprobe; i++) {  // short probe: a full LDS table spills to HBM
           int64_t k = s_tab[s];
           if (k == key) { ls = (int64_t)s; break; }
           if (k == kEmpty) {
@@ -792,7 +794,7 @@ __global__ __launch_bounds__(256) void join_agg_update_kernel(AggParams p, JoinA
         ls = p.lcap + 1;
       } else {
         uint64_t s = fmix64((uint64_t)key) & lmask;
-        for (int i = 0; i < 8; i++) {  // short probe: a full LDS table spills to HBM
+        for (int i = 0; i < p.lprobe; i++) {  // short probe: a full LDS table spills to HBM
           const int64_t k = s_tab[s];
           if (k == key) {
             ls = (int64_t)s;
@@ -1450,8 +1452,8 @@ struct gz_agg {
   struct Launch {
     bool ready = false;
     AggParams p_hbm, p_lds;
-    unsigned grid_hbm = 0, blocks_lds = 0;
-    int lds = 0;
+    unsigned grid_hbm = 0, blocks_hbm = 0, blocks_lds = 0;
+    int lds = 0, lds_hbm = 0;
     int64_t total_slots = 0;
     DBuf d_hbm, d_lds, d_slot_prefix, d_bufs;
     std::vector<int64_t> rows;
@@ -1602,6 +1604,7 @@ static AggParams base_params(gz_agg* h) {
   for (int l = 0; l < p.nl; l++) p.lanes[l] = h->plan.lanes[l];
   for (int c = 0; c < h->plan.ncols; c++) p.col_type[c] = h->plan.types[c];
   p.lcap = 1024;
+  p.lprobe = 32;
   // used input columns of a partition record
   for (int c = 0; c < kAggMaxCols; c++) p.uslot[c] = -1;
   p.nu = 0;
@@ -1622,6 +1625,26 @@ static AggParams base_params(gz_agg* h) {
 // LDS table: lcap slots, the null-key and kEmpty-key slots, and a trash slot.
 static int lds_bytes(const gz_agg* h, int lcap) {
   return (int)((lcap + 3) * (2 + h->plan.lanes.size()) * 8);
+}
+
+// HBM-placed instances (more groups than an LDS table holds) still get a small
+// LDS first level when the launch has tiles enough (eight per CU) to pay for
+// the tables' reset and merge: persistent workgroups (four per CU) with a
+// 512-key table and a 4-slot probe, so the head of a skewed key distribution
+// aggregates in LDS and only its tail goes to the HBM table; with uniform keys
+// the table fills at once and costs a few LDS reads per row.
+static void hbm_first_level(const gz_agg* h, AggParams* q, unsigned* blocks_out, int* lds_out) {
+  int64_t blocks = std::min<int64_t>(q->T, (int64_t)h->ctx->num_cus * 4);
+  q->tiles_per_block = (int)((q->T + blocks - 1) / blocks);
+  *blocks_out = (unsigned)((q->T + q->tiles_per_block - 1) / q->tiles_per_block);
+  q->lcap = 512;
+  q->lprobe = 4;
+  int lds = lds_bytes(h, q->lcap);
+  while (lds > 40 * 1024 && q->lcap > 64) {
+    q->lcap >>= 1;
+    lds = lds_bytes(h, q->lcap);
+  }
+  *lds_out = lds;
 }
 
 // (Re)allocate an instance's table with capacity cap (copying existing groups).
@@ -1866,9 +1889,17 @@ static int agg_process(gz_agg* h, int32_t nb, const gz_batch* batches, const int
       GZ_TRY(upload(ctx, &h->d_tiles, hbm_tiles.data(), sizeof(AggTile) * hbm_tiles.size()));
     p.tiles = (const AggTile*)h->d_tiles.p;
     p.T = (int64_t)hbm_tiles.size();
-    int64_t grid = (p.T + 7) / 8 * 8;
     KernelTimer kt(ctx, "agg_update");
-    hipLaunchKernelGGL(agg_update_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
+    if (p.T >= 8ll * ctx->num_cus) {
+      AggParams ph = p;
+      unsigned blocks = 0;
+      int lds = 0;
+      hbm_first_level(h, &ph, &blocks, &lds);
+      hipLaunchKernelGGL(agg_update_lds_kernel, dim3(blocks), dim3(256), lds, ctx->stream, ph);
+    } else {  // a small call: too few tiles per workgroup to pay for an LDS table's reset and merge
+      const int64_t grid = (p.T + 7) / 8 * 8;
+      hipLaunchKernelGGL(agg_update_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
+    }
   }
   if (!lds_tiles.empty()) {
     DBuf* dt = hbm_tiles.empty() ? &h->d_tiles : nullptr;
@@ -2180,12 +2211,14 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   h->L.p_hbm = p;
   h->L.p_lds = p;
   h->L.grid_hbm = 0;
+  h->L.blocks_hbm = 0;
   h->L.blocks_lds = 0;
   if (!hbm_tiles.empty()) {
     GZ_TRY(upload(ctx, &h->L.d_hbm, hbm_tiles.data(), sizeof(AggTile) * hbm_tiles.size()));
     h->L.p_hbm.tiles = (const AggTile*)h->L.d_hbm.p;
     h->L.p_hbm.T = (int64_t)hbm_tiles.size();
-    h->L.grid_hbm = (unsigned)((h->L.p_hbm.T + 7) / 8 * 8);
+    if (h->L.p_hbm.T >= 8ll * ctx->num_cus) hbm_first_level(h, &h->L.p_hbm, &h->L.blocks_hbm, &h->L.lds_hbm);
+    else h->L.grid_hbm = (unsigned)((h->L.p_hbm.T + 7) / 8 * 8);
   }
   if (!lds_tiles.empty()) {
     GZ_TRY(upload(ctx, &h->L.d_lds, lds_tiles.data(), sizeof(AggTile) * lds_tiles.size()));
@@ -2217,6 +2250,8 @@ static int agg_run_kernels(gz_agg* h) {
   unsigned gi = (unsigned)std::min<int64_t>((h->L.total_slots + 255) / 256, 16384);
   hipLaunchKernelGGL(agg_init_all_kernel, dim3(gi), dim3(256), 0, st, (const AggInst*)h->d_inst.p, h->ninst,
                      (const int64_t*)h->L.d_slot_prefix.p, h->L.total_slots, nl, (const int*)h->d_lane_ops.p);
+  if (h->L.blocks_hbm)
+    hipLaunchKernelGGL(agg_update_lds_kernel, dim3(h->L.blocks_hbm), dim3(256), h->L.lds_hbm, st, h->L.p_hbm);
   if (h->L.grid_hbm)
     hipLaunchKernelGGL(agg_update_kernel, dim3(h->L.grid_hbm), dim3(256), 0, st, h->L.p_hbm);
   if (h->L.blocks_lds)
@@ -2531,6 +2566,7 @@ int gz_join_agg_process_batches(gz_agg* h, int32_t nb, const gz_batch* batches) 
   q.tiles_per_block = (int)((q.T + blocks - 1) / blocks);
   blocks = (q.T + q.tiles_per_block - 1) / q.tiles_per_block;
   p.lcap = 512;
+  p.lprobe = 8;
   int lds = lds_bytes(h, p.lcap);
   while (lds > 40 * 1024 && p.lcap > 64) {
     p.lcap >>= 1;
