@@ -232,15 +232,6 @@ int lz4_walk(const uint8_t* stream, int64_t src, int64_t n, int64_t dev_base, st
   return GZ_OK;
 }
 
-typedef const __attribute__((address_space(4))) uint32_t* kptr_u32;
-
-// Byte at a wave-uniform address, through the scalar cache (an aligned dword
-// load; the dword never leaves the allocation holding the byte).
-__device__ __forceinline__ uint32_t ubyte(const uint8_t* p) {
-  const uint64_t a = (uint64_t)p;
-  const uint32_t w = *(kptr_u32)(a & ~3ull);
-  return (w >> ((a & 3) * 8)) & 0xFFu;
-}
 
 // dst[k] = src[k] for k < n by one wave (the ranges never overlap where this is
 // used).  Short copies go byte per lane.  Longer ones copy a byte head up to a
@@ -380,83 +371,89 @@ __device__ __forceinline__ int64_t readlane64(int64_t v, int l) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+constexpr uint32_t kLz4Cand = 512;  // input positions decoded as candidate sequence starts per window
+constexpr uint32_t kLz4Bad = 0xFFFFFFFFu;
+
+// LDS of one wave for the parse: the window's input bytes (plus a lookahead)
+// and, for every position of the window, the sequence that would start there.
+struct Lz4Lds {
+  uint8_t in[kLz4Cand + 64];
+  u32x4 seq[kLz4Cand];  // next start (kLz4Bad: malformed), literal length, match length, header | offset << 16
+};
+
+// Sequence header at block position p, read from the staged window where it
+// holds the bytes (from HBM past it): the position of the next sequence, the
+// literal run (length, header bytes before it), the match (length, offset).
+// Any position may be tried; only the chain from the block's start is used.
+__device__ __forceinline__ u32x4 lz4_candidate(const uint8_t* blk, uint32_t blen, const Lz4Lds& L, uint32_t wbase,
+                                               uint32_t p) {
+  auto byte = [&](uint32_t x) -> uint32_t {
+    const uint32_t o = x - wbase;
+    return o < kLz4Cand + 64 ? (uint32_t)L.in[o] : (uint32_t)gld(blk + x);
+  };
+  const u32x4 bad{kLz4Bad, 0u, 0u, 0u};
+  if (p >= blen) return bad;
+  const uint32_t t = byte(p);
+  uint32_t lit = t >> 4, q = p + 1;
+  if (lit == 15) {
+    uint32_t b;
+    do {
+      if (q >= blen) return bad;
+      b = byte(q++);
+      lit += b;
+    } while (b == 255);
+  }
+  if (lit > blen - q) return bad;
+  const uint32_t hdr = q - p;
+  uint32_t e = q + lit;
+  if (e == blen) return u32x4{blen, lit, 0u, hdr};  // the last sequence: literals only
+  if (blen - e < 2) return bad;
+  const uint32_t off = byte(e) | (byte(e + 1) << 8);
+  e += 2;
+  uint32_t ml = t & 15;
+  if (ml == 15) {
+    uint32_t b;
+    do {
+      if (e >= blen) return bad;
+      b = byte(e++);
+      ml += b;
+    } while (b == 255 && ml <= 0x7FFFFFFFu);
+    if (ml > 0x7FFFFFFFu) return bad;
+  }
+  if (hdr > 0xFFFFu) return bad;
+  return u32x4{e, lit, ml + 4, hdr | (off << 16)};
+}
+
 __device__ int64_t lz4_block(const uint8_t* blk, uint32_t blen, uint8_t* dst, int64_t op, int64_t fbase,
-                             int64_t cap, int lane) {
+                             int64_t cap, int lane, Lz4Lds& L) {
   constexpr uint32_t kLong = 32;
-  typedef const __attribute__((address_space(4))) u32x4* kptr_u32x4;
+  // The sequence chain is serial, but a sequence's header can be decoded from
+  // any position: every lane decodes the would-be sequences at its positions
+  // of a 512-byte window (staged in LDS), then the wave walks the real chain
+  // through them, one LDS read per sequence, placing each sequence's fields in
+  // lane n with v_writelane.
+  uint32_t wbase = kLz4Bad;
+  auto stage = [&](uint32_t base) {
+    wbase = base;
+    for (uint32_t k = lane; k < kLz4Cand + 64; k += kWave)
+      L.in[k] = base + k < blen ? gld(blk + base + k) : (uint8_t)0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (uint32_t k = lane; k < kLz4Cand; k += kWave) L.seq[k] = lz4_candidate(blk, blen, L, base, base + k);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  };
   uint32_t ip = 0;  // input offset within the block (blocks are at most 4 MiB)
   while (ip < blen) {
-    // The serial parse stays in scalar registers: 16 input bytes per scalar
-    // load, 32-bit bounds (output room and match reach refreshed per window),
-    // each sequence's fields written into lane n with v_writelane.
     int n = 0;
     uint32_t my_lsrc = 0, my_lit_len = 0, my_off = 0, my_mlen = 0, my_dlo = 0, my_dhi = 0;
     const int64_t op0 = op;
     uint32_t room = cap - op > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)(cap - op);
     uint32_t back = op - fbase > 65536 ? 65536u : (uint32_t)(op - fbase);
-    uint32_t rel = 0;  // output produced in this window
+    uint32_t rel = 0;  // output produced by this window's sequences
     while (n < kWave && ip < blen) {
-      uint32_t lit = 0, off = 0, mlen = 0, lsrc = 0;
-      bool fast = false;
-      if (ip + 13 <= blen + 4) {
-        // 16 bytes from the dword holding ip: inside the frame, whose 4-byte
-        // end mark follows the last block
-        const uint64_t a = (uint64_t)(blk + ip);
-        const u32x4 w = *(kptr_u32x4)(a & ~3ull);
-        const uint32_t m = (uint32_t)(a & 3);
-        auto byte = [&](uint32_t q) {
-          const uint32_t x = q < 8 ? (q < 4 ? w.x : w.y) : (q < 12 ? w.z : w.w);
-          return (x >> ((q & 3) * 8)) & 0xFFu;
-        };
-        const uint32_t token = byte(m);
-        const uint32_t l = token >> 4, ml = token & 15;
-        if (l < 15 && ml < 15 && m + 3 + l <= 16) {
-          lit = l;
-          lsrc = ip + 1;
-          if (lsrc + lit >= blen) {  // the last sequence carries literals only
-            if (lsrc + lit > blen) return -1;
-            ip = blen;
-          } else {
-            off = byte(m + 1 + lit) | (byte(m + 2 + lit) << 8);
-            mlen = ml + 4;
-            ip = lsrc + lit + 2;
-            if (ip > blen) return -1;
-          }
-          fast = true;
-        }
-      }
-      if (!fast) {  // byte by byte: length extensions, the end of the block
-        const uint32_t token = ubyte(blk + ip++);
-        uint32_t l = token >> 4;
-        if (l == 15) {
-          uint32_t b;
-          do {
-            if (ip >= blen) return -1;
-            b = ubyte(blk + ip++);
-            l += b;
-          } while (b == 255 && l <= blen);
-        }
-        if (l > blen - ip) return -1;
-        lit = l;
-        lsrc = ip;
-        ip += lit;
-        if (ip < blen) {
-          if (blen - ip < 2) return -1;
-          off = ubyte(blk + ip) | (ubyte(blk + ip + 1) << 8);
-          ip += 2;
-          uint32_t m = token & 15;
-          if (m == 15) {
-            uint32_t b;
-            do {
-              if (ip >= blen) return -1;
-              b = ubyte(blk + ip++);
-              m += b;
-            } while (b == 255 && m <= 0x7FFFFFFFu);
-          }
-          if (m > 0x7FFFFFFFu) return -1;
-          mlen = m + 4;
-        }
-      }
+      if (wbase == kLz4Bad || ip - wbase >= kLz4Cand) stage(ip);
+      const u32x4 f = L.seq[ip - wbase];
+      if (f.x == kLz4Bad) return -1;
+      const uint32_t lit = f.y, mlen = f.z, lsrc = ip + (f.w & 0xFFFFu), off = f.w >> 16;
       if (lit > room) return -1;
       room -= lit;
       rel += lit;
@@ -472,6 +469,7 @@ __device__ int64_t lz4_block(const uint8_t* blk, uint32_t blen, uint8_t* dst, in
       room -= mlen;
       rel += mlen;
       back = back + mlen > 65536u ? 65536u : back + mlen;
+      ip = f.x;
       n++;
     }
     op = op0 + rel;
@@ -522,6 +520,7 @@ __device__ int64_t lz4_block(const uint8_t* blk, uint32_t blen, uint8_t* dst, in
 __global__ __launch_bounds__(256) void lz4_decode_kernel(const Lz4Task* __restrict__ tasks, int ntasks,
                                                          const Lz4Block* __restrict__ blocks,
                                                          int64_t* __restrict__ produced) {
+  __shared__ Lz4Lds s_lz4[4];
   const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (t >= ntasks) return;
@@ -539,7 +538,7 @@ __global__ __launch_bounds__(256) void lz4_decode_kernel(const Lz4Task* __restri
       op += bl.len;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     } else {
-      op = lz4_block(bl.src, bl.len, tk.dst, op, fbase, tk.cap, lane);
+      op = lz4_block(bl.src, bl.len, tk.dst, op, fbase, tk.cap, lane, s_lz4[threadIdx.x >> 6]);
       if (op < 0) break;
     }
   }
