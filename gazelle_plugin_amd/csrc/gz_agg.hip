@@ -1430,6 +1430,7 @@ struct JoinAggState {
 };
 
 struct gz_agg {
+  MultiKey* mk = nullptr;        // grouped by several key columns
   JoinAggState* join = nullptr;  // a fused join -> aggregate
   gz_context* ctx = nullptr;
   AggPlan plan;
@@ -1706,6 +1707,21 @@ static int agg_set_make(gz_context* ctx, int32_t ncols, const int32_t* types, in
                         gz_agg** out) {
   if (!ctx || !out || (ncols > 0 && !types) || !acts) return fail(GZ_INVALID, "gz_agg_make: null argument");
   if (ninst <= 0 || ninst > 65535) return fail(GZ_INVALID, "instance count must be in [1, 65535]");
+  int nkeys = 0;
+  for (int i = 0; i < nact; i++) nkeys += acts[i].kind == GZ_ACTION_GROUPBY;
+  if (nkeys > 1) {
+    if (ninst > 1) return fail(GZ_NOT_IMPLEMENTED, "several group-by keys in an aggregate set");
+    if (nact > kMaxActions) return fail(GZ_INVALID, "bad action count");
+    auto* h = new gz_agg();
+    h->ctx = ctx;
+    int rc = mk_make(ctx, ncols, types, nact, acts, opts, &h->mk);
+    if (rc) {
+      delete h;
+      return rc;
+    }
+    *out = h;
+    return GZ_OK;
+  }
   auto* h = new gz_agg();
   h->ctx = ctx;
   int rc = build_plan(h->plan, ncols, types, nact, acts);
@@ -2586,17 +2602,20 @@ int gz_agg_process(gz_agg* h, int64_t num_rows, const int64_t* buf_addrs, const 
   if (num_rows == 0) return GZ_OK;  // ProcessAndCacheOne returns early on empty input
   if (!buf_addrs || !buf_sizes) return fail(GZ_INVALID, "buf_addrs can't be null");
   gz_batch b{num_rows, buf_addrs, buf_sizes, num_bufs};
+  if (h->mk) return mk_process(h->mk, 1, &b);
   return agg_process(h, 1, &b, nullptr);
 }
 
 int gz_agg_process_batches(gz_agg* h, int32_t num_batches, const gz_batch* batches) {
   if (!h) return fail(GZ_INVALID, "null aggregate");
   if (num_batches < 0 || (num_batches > 0 && !batches)) return fail(GZ_INVALID, "bad batch list");
+  if (h->mk) return mk_process(h->mk, num_batches, batches);
   return agg_process(h, num_batches, batches, nullptr);
 }
 
 int gz_agg_num_groups(gz_agg* h, int64_t* out) {
   if (!h || !out) return fail(GZ_INVALID, "null argument");
+  if (h->mk) return mk_num_groups(h->mk, out);
   if (h->finished) {
     *out = h->ngroups[0];
     return GZ_OK;
@@ -2614,20 +2633,26 @@ int gz_agg_num_groups(gz_agg* h, int64_t* out) {
   return GZ_OK;
 }
 
-int gz_agg_num_result_columns(gz_agg* h) { return h ? (int)h->plan.results.size() : 0; }
+int gz_agg_num_result_columns(gz_agg* h) {
+  if (h && h->mk) return mk_num_result_columns(h->mk);
+  return h ? (int)h->plan.results.size() : 0;
+}
 
 int gz_agg_result_type(gz_agg* h, int32_t col) {
+  if (h && h->mk) return mk_result_type(h->mk, col);
   if (!h || col < 0 || col >= (int)h->plan.results.size()) return -1;
   return h->plan.results[col].rtype;
 }
 
 int gz_agg_finish(gz_agg* h) {
   if (!h) return fail(GZ_INVALID, "null aggregate");
+  if (h->mk) return mk_finish(h->mk);
   return agg_finish(h);
 }
 
 int gz_agg_result_column(gz_agg* h, int32_t col, void** values, void** valid) {
   if (!h) return fail(GZ_INVALID, "null aggregate");
+  if (h->mk) return mk_result_column(h->mk, col, values, valid);
   if (!h->finished) return fail(GZ_INVALID, "call gz_agg_finish first");
   if (col < 0 || col >= (int)h->plan.results.size()) return fail(GZ_INDEX_ERROR, "result column out of range");
   int w = std::max(1, type_width(h->plan.results[col].rtype));
@@ -2638,6 +2663,11 @@ int gz_agg_result_column(gz_agg* h, int32_t col, void** values, void** valid) {
 
 int gz_agg_release(gz_agg* h) {
   if (!h) return GZ_OK;
+  if (h->mk) {
+    mk_release(h->mk);
+    delete h;
+    return GZ_OK;
+  }
   gz_context* ctx = h->ctx;
   for (auto& b : h->tables) ctx_free(ctx, &b);
   for (auto& b : h->counters) ctx_free(ctx, &b);
@@ -2664,12 +2694,14 @@ int gz_agg_set_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types,
 
 int gz_agg_set_process(gz_agg* h, int32_t num_batches, const gz_batch* batches, const int32_t* instance_of) {
   if (!h) return fail(GZ_INVALID, "null aggregate set");
+  if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   if (num_batches < 0 || (num_batches > 0 && (!batches || !instance_of))) return fail(GZ_INVALID, "bad batch list");
   return agg_process(h, num_batches, batches, instance_of);
 }
 
 int gz_agg_set_num_groups(gz_agg* h, int32_t inst, int64_t* out) {
   if (!h || !out) return fail(GZ_INVALID, "null argument");
+  if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   if (!h->finished) return fail(GZ_INVALID, "call gz_agg_finish first");
   if (inst < 0 || inst >= h->ninst) return fail(GZ_INDEX_ERROR, "instance out of range");
   *out = h->ngroups[inst];
@@ -2678,6 +2710,7 @@ int gz_agg_set_num_groups(gz_agg* h, int32_t inst, int64_t* out) {
 
 int gz_agg_set_result_column(gz_agg* h, int32_t inst, int32_t col, void** values, void** valid) {
   if (!h) return fail(GZ_INVALID, "null aggregate set");
+  if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   if (!h->finished) return fail(GZ_INVALID, "call gz_agg_finish first");
   if (inst < 0 || inst >= h->ninst) return fail(GZ_INDEX_ERROR, "instance out of range");
   if (col < 0 || col >= (int)h->plan.results.size()) return fail(GZ_INDEX_ERROR, "result column out of range");
@@ -2689,34 +2722,40 @@ int gz_agg_set_result_column(gz_agg* h, int32_t inst, int32_t col, void** values
 
 int gz_agg_set_placement(gz_agg* h, int32_t inst, int32_t* out) {
   if (!h || !out || inst < 0 || inst >= h->ninst) return fail(GZ_INVALID, "bad argument");
+  if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   *out = h->placement[inst];
   return GZ_OK;
 }
 
 int gz_agg_set_prepare(gz_agg* h, int32_t num_batches, const gz_batch* batches, const int32_t* instance_of) {
   if (!h) return fail(GZ_INVALID, "null aggregate set");
+  if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   if (num_batches < 0 || (num_batches > 0 && (!batches || !instance_of))) return fail(GZ_INVALID, "bad batch list");
   return agg_prepare(h, num_batches, batches, instance_of);
 }
 
 int gz_agg_set_run(gz_agg* h) {
   if (!h) return fail(GZ_INVALID, "null aggregate set");
+  if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   return agg_run(h);
 }
 
 int gz_agg_set_capture(gz_agg* h) {
   if (!h) return fail(GZ_INVALID, "null aggregate set");
+  if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   return agg_capture(h);
 }
 
 int gz_agg_set_launch(gz_agg* h) {
   if (!h) return fail(GZ_INVALID, "null aggregate set");
+  if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   return agg_launch(h);
 }
 
 int gz_agg_set_autotune(gz_agg* h, int32_t num_batches, const gz_batch* batches, const int32_t* instance_of,
                         int32_t* chosen) {
   if (!h) return fail(GZ_INVALID, "null aggregate set");
+  if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   if (num_batches < 0 || (num_batches > 0 && (!batches || !instance_of))) return fail(GZ_INVALID, "bad batch list");
   return agg_autotune(h, num_batches, batches, instance_of, chosen);
 }

@@ -158,6 +158,20 @@ struct RelationView {
 };
 int relation_view(gz_relation* r, RelationView* v);
 
+// Hash aggregate grouped by several key columns (gz_keys.hip): a device key
+// dictionary maps every row's key tuple to a dense code, and a single-key
+// aggregate groups by the code.
+struct MultiKey;
+int mk_make(gz_context* ctx, int32_t ncols, const int32_t* types, int32_t nact, const gz_agg_action* acts,
+            const gz_agg_options* opts, MultiKey** out);
+int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches);
+int mk_num_groups(MultiKey* m, int64_t* out);
+int mk_num_result_columns(MultiKey* m);
+int mk_result_type(MultiKey* m, int col);
+int mk_finish(MultiKey* m);
+int mk_result_column(MultiKey* m, int col, void** values, void** valid);
+void mk_release(MultiKey* m);
+
 // Probe on another context than the relation's (resident relations).
 void probe_set_context(gz_probe* h, gz_context* ctx);
 // Wait for the relation's build (its context's stream).

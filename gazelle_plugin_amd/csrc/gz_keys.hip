@@ -1,0 +1,519 @@
+// Group-by over several key columns (the hash aggregate's multi-key form).
+//
+// The reference builds an UnsafeRow of the key columns for every row (a null
+// bitset, then the raw bytes of each non-null key) and looks the bytes up in a
+// LargeStringHashMap that hands out ids in first-seen order
+// (ext/hash_aggregate_kernel.cc:970-1000, third_party/row_wise_memory/
+// unsafe_row.h:45-149, precompile/unsafe_array.h:58-108).  Two rows share a
+// group iff the same keys are null and the other keys have identical bits.
+//
+// Here a device key dictionary turns every row's key tuple into a dense int64
+// code with that equality, and the single-key aggregate groups by the code:
+//   key_find_kernel    one thread per row: load the key tuple (raw bits per
+//                      key, null mask), hash it, probe a 64-bit slot table
+//                      (fingerprint | reference); an empty slot is claimed by
+//                      CAS with the row's own reference, a fingerprint match is
+//                      compared against the referenced tuple (an earlier code's
+//                      stored tuple, or the input row that claimed the slot in
+//                      this launch).  No thread ever waits for another.
+//   key_assign_kernel  the row that claimed a slot takes the next code, stores
+//                      its tuple and turns the slot into that code
+//   key_code_kernel    every row reads its slot's code
+// The aggregate's own first-row order then gives the reference's group order,
+// and the finish gathers each key column from the stored tuples by code.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "gz_internal.h"
+
+namespace gz {
+
+constexpr int kMaxKeys = 8;
+constexpr uint32_t kPermanent = 0x80000000u;  // slot reference is a code
+
+struct KeyBatchPtrs {
+  const uint8_t* valid[kMaxKeys];
+  const uint8_t* vals[kMaxKeys];
+};
+
+struct KeyParams {
+  const KeyBatchPtrs* kb;  // per batch of the call
+  const int64_t* prefix;   // [nb + 1] first call row of each batch
+  int nb;
+  int K;
+  int width[kMaxKeys];  // bytes per value: 1, 2, 4 or 8
+  int64_t rows;         // rows of the call
+  unsigned long long* table;
+  uint64_t mask;  // table capacity - 1
+  unsigned long long* kvals;  // [code * K + j] raw key bits (0 under a null)
+  uint32_t* kmask;            // [code] null mask
+  unsigned long long* counter;
+  int64_t* rowslot;  // per call row: its slot, then its code
+};
+
+__device__ __forceinline__ int batch_of(const int64_t* prefix, int nb, int64_t g) {
+  int lo = 0, hi = nb;  // prefix[lo] <= g < prefix[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Key tuple of call row g: raw bits per key (zero-extended), null mask.
+__device__ __forceinline__ uint32_t load_key(const KeyParams& p, int64_t g, uint64_t* raw) {
+  const int b = batch_of(p.prefix, p.nb, g);
+  const int64_t r = g - p.prefix[b];
+  const KeyBatchPtrs& B = p.kb[b];
+  uint32_t m = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxKeys; j++) {
+    raw[j] = 0;
+    if (j < p.K) {
+      const uint8_t* vb = B.valid[j];
+      const bool ok = !vb || ((vb[r >> 3] >> (r & 7)) & 1);
+      if (ok) {
+        const uint8_t* v = B.vals[j];
+        switch (p.width[j]) {
+          case 1: raw[j] = v[r]; break;
+          case 2: raw[j] = ((const uint16_t*)v)[r]; break;
+          case 4: raw[j] = ((const uint32_t*)v)[r]; break;
+          default: raw[j] = ((const uint64_t*)v)[r]; break;
+        }
+      } else {
+        m |= 1u << j;
+      }
+    }
+  }
+  return m;
+}
+
+__device__ __forceinline__ uint64_t key_hash(const uint64_t* raw, uint32_t m, int K) {
+  uint64_t h = 0x9e3779b97f4a7c15ull ^ m;
+#pragma unroll
+  for (int j = 0; j < kMaxKeys; j++) {
+    if (j < K) {
+      h = (h ^ raw[j]) * 0xff51afd7ed558ccdull;
+      h ^= h >> 32;
+    }
+  }
+  h ^= h >> 33;
+  h *= 0xc4ceb9fe1a85ec53ull;
+  h ^= h >> 33;
+  return h;
+}
+
+__device__ __forceinline__ bool key_equal(const KeyParams& p, uint32_t ref, const uint64_t* raw, uint32_t m) {
+  uint64_t o[kMaxKeys];
+  uint32_t om;
+  if (ref & kPermanent) {
+    const uint64_t c = ref & ~kPermanent;
+    om = p.kmask[c];
+#pragma unroll
+    for (int j = 0; j < kMaxKeys; j++) o[j] = j < p.K ? p.kvals[c * p.K + j] : 0;
+  } else {
+    om = load_key(p, (int64_t)ref, o);
+  }
+  bool eq = om == m;
+#pragma unroll
+  for (int j = 0; j < kMaxKeys; j++)
+    if (j < p.K) eq = eq && o[j] == raw[j];
+  return eq;
+}
+
+__global__ __launch_bounds__(256) void key_find_kernel(KeyParams p) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= p.rows) return;
+  uint64_t raw[kMaxKeys];
+  const uint32_t m = load_key(p, g, raw);
+  const uint64_t h = key_hash(raw, m, p.K);
+  const uint64_t fp = (h >> 32) | 1u;  // never zero: an occupied slot is never 0
+  const unsigned long long mine = (fp << 32) | (uint64_t)g;
+  uint64_t s = h & p.mask;
+  // the table is at most half full, so the walk ends
+  while (true) {
+    unsigned long long w = __hip_atomic_load(&p.table[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (w == 0) {
+      const unsigned long long old = atomicCAS(&p.table[s], 0ull, mine);
+      if (old == 0) break;
+      w = old;
+    }
+    if ((w >> 32) == fp && key_equal(p, (uint32_t)w, raw, m)) break;
+    s = (s + 1) & p.mask;
+  }
+  p.rowslot[g] = (int64_t)s;
+}
+
+__global__ __launch_bounds__(256) void key_assign_kernel(KeyParams p) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= p.rows) return;
+  const int64_t s = p.rowslot[g];
+  const unsigned long long w = p.table[s];
+  // only the row that claimed the slot holds its own reference (a code has
+  // the top bit set, call rows are below 2^31)
+  if ((uint32_t)w != (uint32_t)g) return;
+  uint64_t raw[kMaxKeys];
+  const uint32_t m = load_key(p, g, raw);
+  const unsigned long long c = atomicAdd(p.counter, 1ull);
+#pragma unroll
+  for (int j = 0; j < kMaxKeys; j++)
+    if (j < p.K) p.kvals[c * p.K + j] = raw[j];
+  p.kmask[c] = m;
+  p.table[s] = (w & 0xffffffff00000000ull) | kPermanent | (uint32_t)c;
+}
+
+__global__ __launch_bounds__(256) void key_code_kernel(const unsigned long long* table, int64_t* rowslot, int64_t rows) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= rows) return;
+  rowslot[g] = (int64_t)(table[rowslot[g]] & 0x7fffffffull);
+}
+
+// Re-inserts codes [0, n) into a larger empty table.
+__global__ __launch_bounds__(256) void key_rehash_kernel(unsigned long long* table, uint64_t mask,
+                                                         const unsigned long long* kvals, const uint32_t* kmask,
+                                                         int K, int64_t n) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  uint64_t raw[kMaxKeys];
+#pragma unroll
+  for (int j = 0; j < kMaxKeys; j++) raw[j] = j < K ? kvals[c * K + j] : 0;
+  const uint64_t h = key_hash(raw, kmask[c], K);
+  const unsigned long long word = (((h >> 32) | 1u) << 32) | kPermanent | (uint32_t)c;
+  uint64_t s = h & mask;
+  while (atomicCAS(&table[s], 0ull, word) != 0ull) s = (s + 1) & mask;
+}
+
+struct KeyGatherParams {
+  const int64_t* codes;  // per group
+  int64_t n;
+  int K;
+  int width[kMaxKeys];
+  const unsigned long long* kvals;
+  const uint32_t* kmask;
+  void* out[kMaxKeys];
+  uint8_t* valid[kMaxKeys];
+};
+
+__global__ __launch_bounds__(256) void key_gather_kernel(KeyGatherParams f) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = f.codes[i];
+    const uint32_t m = f.kmask[c];
+    for (int j = 0; j < f.K; j++) {
+      const uint64_t v = f.kvals[c * f.K + j];
+      switch (f.width[j]) {
+        case 1: ((uint8_t*)f.out[j])[i] = (uint8_t)v; break;
+        case 2: ((uint16_t*)f.out[j])[i] = (uint16_t)v; break;
+        case 4: ((uint32_t*)f.out[j])[i] = (uint32_t)v; break;
+        default: ((uint64_t*)f.out[j])[i] = v; break;
+      }
+      f.valid[j][i] = ((m >> j) & 1) ? 0 : 1;
+    }
+  }
+}
+
+__global__ void fill_u64_kernel_keys(unsigned long long* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// ------------------------------------------------------------------ host side
+
+struct MultiKey {
+  gz_context* ctx = nullptr;
+  gz_agg* inner = nullptr;  // single-key aggregate over (columns..., code)
+  int ncols = 0;
+  std::vector<int> types;
+  int K = 0;
+  std::vector<int> key_cols, key_types;
+  std::vector<int> res_map;  // outer result -> inner result (>= 0) or key -(j + 1)
+  int code_res = 0;          // inner result holding the code
+  // dictionary
+  DBuf table, kvals, kmask, counter;
+  uint64_t tcap = 0;
+  int64_t kcap = 0, ncodes = 0;
+  // per call
+  DBuf rowslot, d_kb, d_prefix;
+  // finish
+  bool finished = false;
+  int64_t ngroups = 0;
+  std::vector<DBuf> out_vals, out_valid;
+};
+
+static bool key_type_ok(int t) {
+  switch (t) {
+    case GZ_TYPE_INT8: case GZ_TYPE_UINT8: case GZ_TYPE_INT16: case GZ_TYPE_UINT16: case GZ_TYPE_INT32:
+    case GZ_TYPE_UINT32: case GZ_TYPE_INT64: case GZ_TYPE_UINT64: case GZ_TYPE_FLOAT: case GZ_TYPE_DOUBLE:
+    case GZ_TYPE_DATE32: case GZ_TYPE_DATE64: case GZ_TYPE_TIMESTAMP:
+      return true;
+    default:
+      return false;
+  }
+}
+
+int mk_make(gz_context* ctx, int32_t ncols, const int32_t* types, int32_t nact, const gz_agg_action* acts,
+            const gz_agg_options* opts, MultiKey** out) {
+  auto* m = new MultiKey();
+  m->ctx = ctx;
+  m->ncols = ncols;
+  m->types.assign(types, types + ncols);
+  std::vector<gz_agg_action> inner;
+  int inner_pos = 0;
+  auto bad = [&](int code, const std::string& msg) {
+    delete m;
+    return fail(code, msg);
+  };
+  for (int i = 0; i < nact; i++) {
+    const gz_agg_action& a = acts[i];
+    if (a.kind == GZ_ACTION_GROUPBY) {
+      if (a.input0 < 0 || a.input0 >= ncols) return bad(GZ_INVALID, "groupby input out of range");
+      const int t = types[a.input0];
+      if (!key_type_ok(t))
+        return bad(GZ_NOT_IMPLEMENTED, "group-by key type " + std::to_string(t) +
+                                           " in a multi-column key (fixed-width numeric keys only)");
+      if (m->K == kMaxKeys) return bad(GZ_NOT_IMPLEMENTED, "more than 8 group-by keys");
+      m->res_map.push_back(-(m->K + 1));
+      if (m->K == 0) {
+        inner.push_back({GZ_ACTION_GROUPBY, ncols, -1, 0});
+        m->code_res = inner_pos++;
+      }
+      m->key_cols.push_back(a.input0);
+      m->key_types.push_back(t);
+      m->K++;
+    } else {
+      inner.push_back(a);
+      const int nres = a.kind == GZ_ACTION_SUM_COUNT ? 2 : 1;
+      for (int r = 0; r < nres; r++) m->res_map.push_back(inner_pos++);
+    }
+  }
+  std::vector<int32_t> itypes(types, types + ncols);
+  itypes.push_back(GZ_TYPE_INT64);
+  int rc = gz_agg_make(ctx, ncols + 1, itypes.data(), (int32_t)inner.size(), inner.data(), opts, &m->inner);
+  if (rc) {
+    delete m;
+    return rc;
+  }
+  rc = ctx_alloc(ctx, sizeof(unsigned long long), &m->counter);
+  if (!rc) rc = hipMemsetAsync(m->counter.p, 0, sizeof(unsigned long long), ctx->stream) == hipSuccess
+                    ? GZ_OK
+                    : fail(GZ_HIP_ERROR, "clearing the key counter");
+  if (rc) {
+    gz_agg_release(m->inner);
+    ctx_free(ctx, &m->counter);
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return GZ_OK;
+}
+
+static int key_width(int t) { return type_width(t); }
+
+// Grows the code store to hold `need` codes and the table to at least twice
+// that, re-inserting the codes already assigned.
+static int mk_reserve(MultiKey* m, int64_t need) {
+  gz_context* ctx = m->ctx;
+  if (need > m->kcap) {
+    const int64_t cap = std::max<int64_t>({need, 2 * m->kcap, 1024});
+    DBuf nv, nm;
+    GZ_TRY(ctx_alloc(ctx, sizeof(unsigned long long) * (size_t)cap * m->K, &nv));
+    GZ_TRY(ctx_alloc(ctx, sizeof(uint32_t) * (size_t)cap, &nm));
+    if (m->ncodes > 0) {
+      GZ_HIP(hipMemcpyAsync(nv.p, m->kvals.p, sizeof(unsigned long long) * (size_t)m->ncodes * m->K,
+                            hipMemcpyDeviceToDevice, ctx->stream));
+      GZ_HIP(hipMemcpyAsync(nm.p, m->kmask.p, sizeof(uint32_t) * (size_t)m->ncodes, hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    }
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+    ctx_free(ctx, &m->kvals);
+    ctx_free(ctx, &m->kmask);
+    m->kvals = nv;
+    m->kmask = nm;
+    m->kcap = cap;
+  }
+  uint64_t tcap = 1024;
+  while (tcap < 2 * (uint64_t)need) tcap <<= 1;
+  if (tcap > m->tcap) {
+    DBuf nt;
+    GZ_TRY(ctx_alloc(ctx, sizeof(unsigned long long) * tcap, &nt));
+    const unsigned fill_blocks = (unsigned)std::min<uint64_t>((tcap + 255) / 256, 65536);
+    hipLaunchKernelGGL(fill_u64_kernel_keys, dim3(fill_blocks), dim3(256), 0, ctx->stream,
+                       (unsigned long long*)nt.p, (int64_t)tcap);
+    if (m->ncodes > 0) {
+      KernelTimer kt(ctx, "key_rehash");
+      hipLaunchKernelGGL(key_rehash_kernel, dim3((unsigned)((m->ncodes + 255) / 256)), dim3(256), 0, ctx->stream,
+                         (unsigned long long*)nt.p, tcap - 1, (const unsigned long long*)m->kvals.p,
+                         (const uint32_t*)m->kmask.p, m->K, m->ncodes);
+    }
+    GZ_HIP(hipGetLastError());
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+    ctx_free(ctx, &m->table);
+    m->table = nt;
+    m->tcap = tcap;
+  }
+  return GZ_OK;
+}
+
+int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
+  gz_context* ctx = m->ctx;
+  if (m->finished) return fail(GZ_INVALID, "aggregate already finished");
+  GZ_HIP(hipSetDevice(ctx->device));
+  int need_bufs = 0;
+  std::vector<int> col_buf(m->ncols);
+  for (int c = 0; c < m->ncols; c++) {
+    col_buf[c] = need_bufs;
+    need_bufs += type_num_bufs(m->types[c]);
+  }
+  std::vector<KeyBatchPtrs> kb(std::max(nb, 1));
+  std::vector<int64_t> prefix(nb + 1, 0);
+  for (int b = 0; b < nb; b++) {
+    const gz_batch& bt = batches[b];
+    if (bt.num_rows < 0 || bt.num_rows > INT32_MAX) return fail(GZ_INVALID, "bad num_rows");
+    if (bt.num_rows > 0 && (!bt.buf_addrs || !bt.buf_sizes)) return fail(GZ_INVALID, "buf_addrs can't be null");
+    if (bt.num_rows > 0 && bt.num_bufs < need_bufs) return fail(GZ_INVALID, "insufficient number of in_buf_addrs");
+    memset(&kb[b], 0, sizeof(KeyBatchPtrs));
+    if (bt.num_rows > 0) {
+      for (int j = 0; j < m->K; j++) {
+        const int k = col_buf[m->key_cols[j]];
+        const bool has_valid = bt.buf_addrs[k] != 0 && bt.buf_sizes[k] > 0;
+        if (has_valid && bt.buf_sizes[k] < (bt.num_rows + 7) / 8) return fail(GZ_INVALID, "validity buffer too small");
+        if (!bt.buf_addrs[k + 1] || bt.buf_sizes[k + 1] < bt.num_rows * key_width(m->key_types[j]))
+          return fail(GZ_INVALID, "values buffer too small");
+        kb[b].valid[j] = has_valid ? (const uint8_t*)bt.buf_addrs[k] : nullptr;
+        kb[b].vals[j] = (const uint8_t*)bt.buf_addrs[k + 1];
+      }
+    }
+    prefix[b + 1] = prefix[b] + bt.num_rows;
+  }
+  const int64_t rows = prefix[nb];
+  if (rows == 0) return GZ_OK;  // ProcessAndCacheOne returns early on empty input
+  if (rows > INT32_MAX) return fail(GZ_NOT_IMPLEMENTED, "2^31 or more rows in one multi-key call");
+  if (m->ncodes + rows > INT32_MAX) return fail(GZ_CAPACITY_ERROR, "2^31 or more groups");
+  GZ_TRY(mk_reserve(m, m->ncodes + rows));
+  GZ_TRY(upload(ctx, &m->d_kb, kb.data(), sizeof(KeyBatchPtrs) * kb.size()));
+  GZ_TRY(upload(ctx, &m->d_prefix, prefix.data(), sizeof(int64_t) * prefix.size()));
+  GZ_TRY(ctx_ensure(ctx, &m->rowslot, sizeof(int64_t) * (size_t)rows));
+  KeyParams p;
+  memset(&p, 0, sizeof(p));
+  p.kb = (const KeyBatchPtrs*)m->d_kb.p;
+  p.prefix = (const int64_t*)m->d_prefix.p;
+  p.nb = nb;
+  p.K = m->K;
+  for (int j = 0; j < m->K; j++) p.width[j] = key_width(m->key_types[j]);
+  p.rows = rows;
+  p.table = (unsigned long long*)m->table.p;
+  p.mask = m->tcap - 1;
+  p.kvals = (unsigned long long*)m->kvals.p;
+  p.kmask = (uint32_t*)m->kmask.p;
+  p.counter = (unsigned long long*)m->counter.p;
+  p.rowslot = (int64_t*)m->rowslot.p;
+  const unsigned grid = (unsigned)((rows + 255) / 256);
+  {
+    KernelTimer kt(ctx, "key_find");
+    hipLaunchKernelGGL(key_find_kernel, dim3(grid), dim3(256), 0, ctx->stream, p);
+  }
+  {
+    KernelTimer kt(ctx, "key_assign");
+    hipLaunchKernelGGL(key_assign_kernel, dim3(grid), dim3(256), 0, ctx->stream, p);
+    hipLaunchKernelGGL(key_code_kernel, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const unsigned long long*)m->table.p, (int64_t*)m->rowslot.p, rows);
+  }
+  GZ_HIP(hipGetLastError());
+  unsigned long long nc = 0;
+  GZ_HIP(hipMemcpyAsync(&nc, m->counter.p, sizeof(nc), hipMemcpyDeviceToHost, ctx->stream));
+  GZ_HIP(hipStreamSynchronize(ctx->stream));
+  m->ncodes = (int64_t)nc;
+  // the single-key aggregate over the batches with the code column appended
+  std::vector<int64_t> addrs((size_t)nb * (need_bufs + 2)), sizes(addrs.size());
+  std::vector<gz_batch> inner(nb);
+  for (int b = 0; b < nb; b++) {
+    const gz_batch& bt = batches[b];
+    int64_t* a = &addrs[(size_t)b * (need_bufs + 2)];
+    int64_t* s = &sizes[(size_t)b * (need_bufs + 2)];
+    for (int k = 0; k < need_bufs; k++) {
+      a[k] = bt.num_rows > 0 ? bt.buf_addrs[k] : 0;
+      s[k] = bt.num_rows > 0 ? bt.buf_sizes[k] : 0;
+    }
+    a[need_bufs] = 0;
+    s[need_bufs] = 0;
+    a[need_bufs + 1] = (int64_t)((int64_t*)m->rowslot.p + prefix[b]);
+    s[need_bufs + 1] = bt.num_rows * (int64_t)sizeof(int64_t);
+    inner[b] = gz_batch{bt.num_rows, a, s, need_bufs + 2};
+  }
+  return gz_agg_process_batches(m->inner, nb, inner.data());
+}
+
+int mk_num_result_columns(MultiKey* m) { return (int)m->res_map.size(); }
+
+int mk_result_type(MultiKey* m, int col) {
+  if (col < 0 || col >= (int)m->res_map.size()) return -1;
+  const int r = m->res_map[col];
+  return r < 0 ? m->key_types[-r - 1] : gz_agg_result_type(m->inner, r);
+}
+
+int mk_finish(MultiKey* m) {
+  if (m->finished) return GZ_OK;
+  gz_context* ctx = m->ctx;
+  GZ_HIP(hipSetDevice(ctx->device));
+  GZ_TRY(gz_agg_finish(m->inner));
+  GZ_TRY(gz_agg_num_groups(m->inner, &m->ngroups));
+  void* codes = nullptr;
+  GZ_TRY(gz_agg_result_column(m->inner, m->code_res, &codes, nullptr));
+  for (auto& b : m->out_vals) ctx_free(ctx, &b);
+  for (auto& b : m->out_valid) ctx_free(ctx, &b);
+  m->out_vals.assign(m->K, DBuf());
+  m->out_valid.assign(m->K, DBuf());
+  KeyGatherParams f;
+  memset(&f, 0, sizeof(f));
+  f.codes = (const int64_t*)codes;
+  f.n = m->ngroups;
+  f.K = m->K;
+  f.kvals = (const unsigned long long*)m->kvals.p;
+  f.kmask = (const uint32_t*)m->kmask.p;
+  for (int j = 0; j < m->K; j++) {
+    f.width[j] = key_width(m->key_types[j]);
+    GZ_TRY(ctx_alloc(ctx, (size_t)std::max<int64_t>(m->ngroups, 1) * f.width[j], &m->out_vals[j]));
+    GZ_TRY(ctx_alloc(ctx, (size_t)std::max<int64_t>(m->ngroups, 1), &m->out_valid[j]));
+    f.out[j] = m->out_vals[j].p;
+    f.valid[j] = (uint8_t*)m->out_valid[j].p;
+  }
+  if (m->ngroups > 0) {
+    KernelTimer kt(ctx, "key_gather");
+    const unsigned gx = (unsigned)std::min<int64_t>((m->ngroups + 255) / 256, 8192);
+    hipLaunchKernelGGL(key_gather_kernel, dim3(gx), dim3(256), 0, ctx->stream, f);
+  }
+  GZ_HIP(hipGetLastError());
+  GZ_HIP(hipStreamSynchronize(ctx->stream));
+  m->finished = true;
+  return GZ_OK;
+}
+
+int mk_num_groups(MultiKey* m, int64_t* out) {
+  GZ_TRY(mk_finish(m));
+  *out = m->ngroups;
+  return GZ_OK;
+}
+
+int mk_result_column(MultiKey* m, int col, void** values, void** valid) {
+  if (!m->finished) return fail(GZ_INVALID, "call gz_agg_finish first");
+  if (col < 0 || col >= (int)m->res_map.size()) return fail(GZ_INDEX_ERROR, "result column out of range");
+  const int r = m->res_map[col];
+  if (r >= 0) return gz_agg_result_column(m->inner, r, values, valid);
+  if (values) *values = m->out_vals[-r - 1].p;
+  if (valid) *valid = m->out_valid[-r - 1].p;
+  return GZ_OK;
+}
+
+void mk_release(MultiKey* m) {
+  if (!m) return;
+  gz_context* ctx = m->ctx;
+  gz_agg_release(m->inner);
+  for (auto& b : m->out_vals) ctx_free(ctx, &b);
+  for (auto& b : m->out_valid) ctx_free(ctx, &b);
+  for (DBuf* b : {&m->table, &m->kvals, &m->kmask, &m->counter, &m->rowslot, &m->d_kb, &m->d_prefix})
+    ctx_free(ctx, b);
+  delete m;
+}
+
+}  // namespace gz

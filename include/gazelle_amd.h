@@ -275,8 +275,12 @@ typedef struct {
 void gz_agg_options_defaults(gz_agg_options* opts);
 
 /* Replaces HashAggregateKernel::Make + MakeResultIterator (hash_aggregate_kernel.cc:1238-1260,
- * :147-224).  A plan has at most one GZ_ACTION_GROUPBY (single numeric key) or
- * none (global aggregate). */
+ * :147-224).  A plan has one GZ_ACTION_GROUPBY (an integer key), none (the
+ * global aggregate), or up to 8 of them: then rows group by all those columns
+ * together with the reference's UnsafeRow key equality (the same keys null,
+ * identical bits in the others: hash_aggregate_kernel.cc:970-1000), each key
+ * fixed-width numeric (integers, dates, timestamps, float, double); the
+ * gz_agg_set_* forms take one key only. */
 int gz_agg_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types,
                 int32_t num_actions, const gz_agg_action* actions,
                 const gz_agg_options* opts, gz_agg** out);

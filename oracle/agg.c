@@ -81,6 +81,17 @@ struct gzo_agg {
   int32_t null_id;      /* -1 until the null key is seen */
   int64_t ngroups, gcap;
   int global_seen;
+  /* several group-by keys: the UnsafeRow key bytes of every group, in a
+   * byte-string first-seen map (LargeStringHashMap) */
+  int nkeys;
+  int32_t kcols[GZO_MAX_KEYS];
+  uint8_t* arena;
+  int64_t arena_len, arena_cap;
+  int64_t* soff;
+  int32_t* slen;
+  int32_t* sid;
+  uint64_t* shash;
+  int64_t scap, ssize;
 };
 
 static uint64_t mix64(uint64_t x) {
@@ -132,6 +143,74 @@ static int32_t get_or_insert(gzo_agg* a, int64_t k) {
   return (int32_t)a->ngroups++;
 }
 
+static uint64_t bytes_hash(const uint8_t* p, int32_t n) {
+  uint64_t h = 0xcbf29ce484222325ULL; /* FNV-1a, then a final mix */
+  for (int32_t i = 0; i < n; i++) h = (h ^ p[i]) * 0x100000001b3ULL;
+  return mix64(h);
+}
+
+static void bytes_map_grow(gzo_agg* a) {
+  int64_t nc = a->scap ? a->scap * 2 : 1024;
+  int64_t* no = (int64_t*)malloc(sizeof(int64_t) * nc);
+  int32_t* nl = (int32_t*)malloc(sizeof(int32_t) * nc);
+  int32_t* ni = (int32_t*)malloc(sizeof(int32_t) * nc);
+  uint64_t* nh = (uint64_t*)malloc(sizeof(uint64_t) * nc);
+  for (int64_t i = 0; i < nc; i++) ni[i] = -1;
+  for (int64_t i = 0; i < a->scap; i++) {
+    if (a->sid[i] < 0) continue;
+    uint64_t h = a->shash[i] & (uint64_t)(nc - 1);
+    while (ni[h] >= 0) h = (h + 1) & (uint64_t)(nc - 1);
+    no[h] = a->soff[i]; nl[h] = a->slen[i]; ni[h] = a->sid[i]; nh[h] = a->shash[i];
+  }
+  free(a->soff); free(a->slen); free(a->sid); free(a->shash);
+  a->soff = no; a->slen = nl; a->sid = ni; a->shash = nh; a->scap = nc;
+}
+
+/* LargeStringHashMap::GetOrInsert over the key's bytes: the id of an equal
+ * byte string, else a new id (= size) */
+static int32_t bytes_get_or_insert(gzo_agg* a, const uint8_t* p, int32_t n) {
+  if ((a->ssize + 1) * 2 > a->scap) bytes_map_grow(a);
+  uint64_t hv = bytes_hash(p, n);
+  uint64_t h = hv & (uint64_t)(a->scap - 1);
+  while (a->sid[h] >= 0) {
+    if (a->shash[h] == hv && a->slen[h] == n && memcmp(a->arena + a->soff[h], p, (size_t)n) == 0)
+      return a->sid[h];
+    h = (h + 1) & (uint64_t)(a->scap - 1);
+  }
+  if (a->arena_len + n > a->arena_cap) {
+    a->arena_cap = (a->arena_cap + n) * 2;
+    a->arena = (uint8_t*)realloc(a->arena, (size_t)a->arena_cap);
+  }
+  memcpy(a->arena + a->arena_len, p, (size_t)n);
+  a->soff[h] = a->arena_len; a->slen[h] = n; a->sid[h] = (int32_t)a->ngroups; a->shash[h] = hv;
+  a->arena_len += n;
+  a->ssize++;
+  return (int32_t)a->ngroups++;
+}
+
+/* The key of row r as an UnsafeRow (unsafe_row.h:45-149, unsafe_array.h:58-108):
+ * a null bitset and an "empty" bitset of nkeys/8+1 bytes each, then, for every
+ * non-null key in order, its value's sizeof(T) bytes; a null key sets its bit
+ * and appends nothing.  Returns the row's byte length. */
+static int32_t unsafe_row_key(const gzo_agg* a, const gzo_batch* b, int64_t r, uint8_t* row) {
+  int32_t bits = a->nkeys / 8 + 1;
+  memset(row, 0, (size_t)(2 * bits));
+  int32_t cur = 2 * bits;
+  for (int k = 0; k < a->nkeys; k++) {
+    int c = a->kcols[k];
+    const uint8_t* vb = (const uint8_t*)b->bufs[3 * c];
+    const uint8_t* v = (const uint8_t*)b->bufs[3 * c + 2];
+    if (vb && !bit_get(vb, r)) {
+      row[k >> 3] |= (uint8_t)(1u << (k & 7));
+      continue;
+    }
+    int w = width_of(a->types[c]);
+    memcpy(row + cur, v + r * w, (size_t)w);
+    cur += w;
+  }
+  return cur;
+}
+
 gzo_agg* gzo_agg_make(int ncols, const int32_t* types, int nactions, const int32_t* kinds,
                       const int32_t* in0, const int32_t* in1, const int32_t* nan_check) {
   gzo_agg* a = (gzo_agg*)calloc(1, sizeof(gzo_agg));
@@ -148,8 +227,13 @@ gzo_agg* gzo_agg_make(int ncols, const int32_t* types, int nactions, const int32
     a->acts[i].in1 = in1[i];
     a->acts[i].nan_check = nan_check[i];
     a->acts[i].in_type = in0[i] >= 0 ? types[in0[i]] : T_INT64;
-    if (kinds[i] == A_GROUPBY) a->key_col = in0[i];
+    if (kinds[i] == A_GROUPBY) {
+      a->key_col = in0[i];
+      if (a->nkeys < GZO_MAX_KEYS) a->kcols[a->nkeys] = in0[i];
+      a->nkeys++;
+    }
   }
+  if (a->nkeys > GZO_MAX_KEYS) { gzo_agg_free(a); return NULL; }
   return a;
 }
 
@@ -245,6 +329,14 @@ int gzo_agg_process(gzo_agg* a, const gzo_batch* b) {
     for (int64_t r = 0; r < n; r++) idx[r] = 0;
     a->ngroups = 1;
     a->global_seen = 1;
+  } else if (a->nkeys > 1) {
+    /* several keys: HashAggregateResultIterator<StringType> with an UnsafeRow
+     * key (hash_aggregate_kernel.cc:970-1000) */
+    uint8_t row[2 * (GZO_MAX_KEYS / 8 + 1) + 8 * GZO_MAX_KEYS];
+    for (int64_t r = 0; r < n; r++) {
+      int32_t len = unsafe_row_key(a, b, r, row);
+      idx[r] = bytes_get_or_insert(a, row, len);
+    }
   } else {
     int32_t kt = a->types[a->key_col];
     const uint8_t* kv = (const uint8_t*)b->bufs[3 * a->key_col];
@@ -356,14 +448,14 @@ void gzo_agg_result(gzo_agg* a, int col, void* values, uint8_t* valid) {
     }
     valid[g] = ok;
     uint8_t* dst = (uint8_t*)values + g * w;
-    if (rt == T_DOUBLE) {
+    if (x->kind == A_GROUPBY) {
+      memcpy(dst, &iv, (size_t)w); /* the key's bits (a float key's as stored) */
+    } else if (rt == T_DOUBLE) {
       double d = isf ? dv : (double)iv;
       memcpy(dst, &d, 8);
     } else if (rt == T_FLOAT) {
-      if (x->kind == A_GROUPBY) memcpy(dst, &iv, 4);
-      else { float f = (float)dv; memcpy(dst, &f, 4); }
-    } else if (x->kind == A_GROUPBY && is_float(rt)) {
-      memcpy(dst, &iv, (size_t)w);
+      float f = (float)dv;
+      memcpy(dst, &f, 4);
     } else {
       memcpy(dst, &iv, (size_t)w); /* little-endian truncation to the result width */
     }
@@ -378,5 +470,6 @@ void gzo_agg_free(gzo_agg* a) {
     free(x->valid); free(x->nullf); free(x->key);
   }
   free(a->acts); free(a->types); free(a->mkeys); free(a->mids); free(a->mused);
+  free(a->arena); free(a->soff); free(a->slen); free(a->sid); free(a->shash);
   free(a);
 }

@@ -70,7 +70,10 @@ void gzo_splitter_free(gzo_splitter* s);
 
 /* ---- hash aggregate ---- */
 typedef struct gzo_agg gzo_agg;
-/* kinds: as GZ_ACTION_* of include/gazelle_amd.h */
+/* kinds: as GZ_ACTION_* of include/gazelle_amd.h.  Several GROUPBY actions
+ * group by all their columns together (an UnsafeRow key, at most
+ * GZO_MAX_KEYS fixed-width columns); NULL if there are more. */
+#define GZO_MAX_KEYS 8
 gzo_agg* gzo_agg_make(int ncols, const int32_t* types, int nactions, const int32_t* kinds,
                       const int32_t* in0, const int32_t* in1, const int32_t* nan_check);
 int gzo_agg_process(gzo_agg* a, const gzo_batch* b);

@@ -41,3 +41,51 @@ def python_groupby(batches, key_col, cols):
             for j, (v, ok) in enumerate(vals):
                 g[j].append(v[r].item() if ok[r] else None)
     return groups
+
+
+def multikey_batches(sizes, seed=0, null_frac=0.15, domains=(5, 7)):
+    """Batches of (int32 k0, int64 k1, double v, double k3, int16 v4): three key
+    columns over small domains (k3 holds 0.0, -0.0 and two NaN payloads among
+    its values) with nulls, so every null pattern and collision shows up."""
+    from gazelle_plugin_amd.batch import HostBatch, HostColumn, bitmap_from_bools
+    rng = np.random.default_rng(seed)
+    k3_pool = np.array([0, 0x8000000000000000, 0x3FF8000000000000, 0x7FF8000000000000, 0x7FF8000000000001],
+                       np.uint64).view(np.float64)  # 0.0, -0.0, 1.5, two NaN payloads
+    out = []
+    for n in sizes:
+        def vb():
+            return bitmap_from_bools(rng.random(n) >= null_frac) if n else None
+        k0 = rng.integers(-2, domains[0] - 2, n).astype(np.int32)
+        k1 = (rng.integers(0, domains[1], n) * 1000003).astype(np.int64)
+        k3 = k3_pool[rng.integers(0, len(k3_pool), n)]
+        v = rng.normal(size=n)
+        v4 = rng.integers(-300, 300, n).astype(np.int16)
+        out.append(HostBatch(n, [HostColumn(T.INT32, k0, vb()), HostColumn(T.INT64, k1, vb()),
+                                 HostColumn(T.DOUBLE, v, vb()), HostColumn(T.DOUBLE, k3, vb()),
+                                 HostColumn(T.INT16, v4, vb())]))
+    return out
+
+
+def python_groupby_keys(batches, key_cols, cols):
+    """Brute force over several key columns: {key tuple: [row values per col]} in
+    first-seen order, where a key tuple holds None for a null key and the raw
+    bits of the value otherwise (so -0.0 and 0.0 differ, NaN payloads differ)."""
+    from gazelle_plugin_amd.batch import bools_from_bitmap
+
+    groups = {}
+    for b in batches:
+        keys = []
+        for c in key_cols:
+            col = b.columns[c]
+            raw = col.values[: b.num_rows].view(np.dtype(f"u{col.values.dtype.itemsize}"))
+            keys.append((raw, bools_from_bitmap(col.validity, b.num_rows)))
+        vals = []
+        for c in cols:
+            col = b.columns[c]
+            vals.append((col.values, bools_from_bitmap(col.validity, b.num_rows)))
+        for r in range(b.num_rows):
+            k = tuple(int(raw[r]) if ok[r] else None for raw, ok in keys)
+            g = groups.setdefault(k, [[] for _ in cols])
+            for j, (v, ok) in enumerate(vals):
+                g[j].append(v[r].item() if ok[r] else None)
+    return groups

@@ -104,3 +104,64 @@ def test_min_max_without_nan_check_follow_first_value():
                             parse_action("action_max", [1])])
     assert math.isnan(res[1][1][0]) and math.isnan(res[2][1][0])
     assert res[1][1][1] == 2.0 and res[2][1][1] == 5.0
+
+
+MK_TYPES = [T.INT32, T.INT64, T.DOUBLE, T.DOUBLE, T.INT16]
+MK_ACTIONS = [("action_groupby", [0]), ("action_sum", [2]), ("action_groupby", [1]),
+              ("action_countLiteral_1", []), ("action_groupby", [3]), ("action_min", [4]),
+              ("action_max", [4]), ("action_sum_count", [2]), ("action_count", [4])]
+
+
+def test_multi_key_brute_force():
+    """Several group-by keys: groups, their first-seen order, key values and
+    aggregates against a brute force over key tuples (None for a null key, raw
+    bits otherwise)."""
+    from aggcheck import multikey_batches, python_groupby_keys
+    batches = multikey_batches([700, 0, 1, 513], seed=3)
+    res = oracle.aggregate(batches, MK_TYPES, [parse_action(n, i) for n, i in MK_ACTIONS])
+    groups = python_groupby_keys(batches, [0, 1, 3], [2, 4])
+    keys = list(groups)
+    assert len(res) == 10 and len(res[0][1]) == len(keys)
+    # key columns: result 0, 2, 4 (raw bits compared, nulls as None)
+    for j, col in enumerate((0, 2, 4)):
+        t, v, ok = res[col]
+        bits = v.view(np.dtype(f"u{v.dtype.itemsize}"))
+        assert [int(bits[i]) if ok[i] else None for i in range(len(v))] == [k[j] for k in keys]
+    for i, k in enumerate(keys):
+        vals, v4 = groups[k]
+        nn = [x for x in vals if x is not None]
+        nn4 = [x for x in v4 if x is not None]
+        assert _same(_column(res, 1)[i], math.fsum(nn) if nn else None, 1e-12)
+        assert _column(res, 3)[i] == len(vals)
+        assert _column(res, 5)[i] == (min(nn4) if nn4 else None)
+        assert _column(res, 6)[i] == (max(nn4) if nn4 else None)
+        assert _same(_column(res, 7)[i], math.fsum(nn) if nn else None, 1e-12)
+        assert _column(res, 8)[i] == (len(nn) if nn else None)
+        assert _column(res, 9)[i] == len(nn4)
+
+
+def test_multi_key_unsafe_row_equality_cases():
+    """The UnsafeRow key compares the null bitset and the raw bytes of the
+    non-null keys: the value under a null is ignored, (null, 5) and (5, null)
+    differ although their appended bytes agree, -0.0 differs from 0.0, and NaNs
+    differ by payload."""
+    from gazelle_plugin_amd.batch import HostBatch, HostColumn, bitmap_from_bools
+    k0 = np.array([5, 9, 5, 5, 7, 7, 7, 7, 7], np.int64)
+    k0_ok = np.array([0, 0, 1, 1, 1, 1, 1, 1, 1], bool)
+    k1 = np.array([5, 5, 3, 3, 0, 0, 0, 0, 0], np.int64)
+    k1_ok = np.array([1, 1, 0, 0, 1, 1, 1, 1, 1], bool)
+    d = np.array([0, 0, 0, 0, 0, 0x8000000000000000, 0, 0x7FF8000000000000, 0x7FF8000000000001],
+                 np.uint64).view(np.float64)
+    b = HostBatch(9, [HostColumn(T.INT64, k0, bitmap_from_bools(k0_ok)),
+                      HostColumn(T.INT64, k1, bitmap_from_bools(k1_ok)),
+                      HostColumn(T.DOUBLE, d, None)])
+    acts = [("action_groupby", [0]), ("action_groupby", [1]), ("action_groupby", [2]),
+            ("action_countLiteral_1", [])]
+    res = oracle.aggregate([b], [T.INT64, T.INT64, T.DOUBLE], [parse_action(n, i) for n, i in acts])
+    # rows 0,1: (null,5,0.0) twice (value under the null differs); rows 2,3:
+    # (5,null,0.0) twice; row 4 and 6: (7,0,0.0); 5: -0.0; 7, 8: two NaNs
+    assert _column(res, 3) == [2, 2, 2, 1, 1, 1]
+    assert _column(res, 0) == [None, 5, 7, 7, 7, 7]
+    assert _column(res, 1) == [5, None, 0, 0, 0, 0]
+    bits = res[2][1].view(np.uint64).tolist()
+    assert bits == [0, 0, 0, 0x8000000000000000, 0x7FF8000000000000, 0x7FF8000000000001]
