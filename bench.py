@@ -37,6 +37,10 @@ AGG_ACTIONS = [("action_groupby", [0]), ("action_sum", [1]), ("action_avg", [1])
                ("action_min", [2]), ("action_max", [2]), ("action_sum", [2])]
 
 
+MK_ACTIONS = [("action_groupby", [0]), ("action_groupby", [2]), ("action_sum", [1]), ("action_avg", [1]),
+              ("action_countLiteral_1", [])]
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,6 +380,32 @@ def bench_agg(args, ctx, stream):
                           "plain_ms": pm, "plain_rows_per_s": tab.num_rows / (pm / 1e3),
                           "batched_over_plain": pm / bm}
     out["plain_query"]["frac_of_hbm_peak"] = out["plain_query"]["batched_GBps"] / HBM_PEAK_GBS
+    # the same rows grouped by two columns (key, qty): the key dictionary, then
+    # the single-key aggregate over its codes
+    mk = HashAggregateKernel.Make(ctx, datagen.AGG_TYPES, MK_ACTIONS)
+    groups_box = []
+
+    def multikey():
+        it = mk.MakeResultIterator()
+        it.ProcessBatches(tab.batches, arr)
+        it.finish()
+        groups_box[:] = [it.num_groups()]
+        it.close()
+
+    multikey()
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    mm, mmin = timed(multikey, stream, 5)
+    kern = {}
+    for name in ("key_find", "key_assign"):
+        t, n = ctx.kernel_time(name)
+        kern[name] = t / max(n, 1)
+    ctx.set_profiling(False)
+    out["multi_key_query"] = {"rows": tab.num_rows, "keys": "(int64 key, int32 qty)", "groups": groups_box[0],
+                              "ms": mm, "min_ms": mmin, "rows_per_s": tab.num_rows / (mm / 1e3),
+                              "kernels_ms_avg": kern,
+                              "note": "GROUP BY two columns with UnsafeRow key equality: key_find (probe/claim) + "
+                                      "key_assign (codes) + the single-key aggregate over the codes"}
     del tab, arr
     # batched set: 512 independent GROUP BYs
     tab, rows, groups = datagen.batched_agg_device(512)

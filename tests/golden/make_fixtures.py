@@ -144,6 +144,39 @@ AGG_FIXTURES = {"tests": [
         },
         "tolerance": 1e-5,
     },
+    {   # GroupByTwoAggregateTest, :911-1040: GROUP BY (f0 int64, f5 utf8).  Its
+        # strings are distinct non-empty 2-byte values, so the UnsafeRow byte key
+        # of (f0, f5) is equal exactly when (f0, code(f5)) is: f5 is given here as
+        # the int32 code BJ 0, SH 1, HZ 2, BH 3, NY 4, TK 5, PH 6.  The
+        # stddev_samp action is left out.
+        "name": "GroupByTwoAggregateTest",
+        "schema": [["f0", "int64"], ["f1", "uint32"], ["f2", "double"], ["f3", "double"], ["f4", "double"],
+                   ["f5_code", "int32"]],
+        "actions": [["action_groupby", [0]], ["action_groupby", [5]], ["action_sum_count", [1]],
+                    ["action_min", [1]], ["action_max", [1]], ["action_avgByCount", [2, 0]]],
+        "batches": [
+            [[1, 2, 3, 4, 5, None, 4, 1, 2, 2, 1, 1, 1, 4, 4, 3, 5, 5, 5, 5],
+             [1, 2, 3, 4, 5, 5, 4, 1, 2, 2, 1, 1, 1, 4, 4, 3, 5, 5, 5, 5],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [0, 1, 2, 3, 4, 1, 3, 0, 1, 1, 0, 0, 0, 3, 3, 2, 4, 4, 4, 4]],
+            [[6, 7, 8, 9, 10, 10, 9, 6, 7, 7, 6, 6, 6, 9, 9, 8, 10, 10, 10, 10],
+             [6, 7, 8, 9, 10, 10, 9, 6, 7, 7, 6, 6, 6, 9, 9, 8, 10, 10, 10, 10],
+             [7, 8, 4, 5, 6, 1, 34, 54, 65, 66, 78, 12, 32, 24, 32, 45, 12, 24, 35, 46],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [0, 1, 5, 1, 6, 6, 1, 0, 1, 1, 0, 0, 0, 1, 1, 5, 6, 6, 6, 6]],
+        ],
+        "expected": [[1, 2, 3, 4, 5, None, 6, 7, 8, 9, 10],                                 # :1020-1031
+                     [0, 1, 2, 3, 4, 1, 0, 1, 5, 1, 6],
+                     [5, 6, 6, 16, 25, 5, 30, 21, 16, 36, 60],
+                     [5, 3, 2, 4, 5, 1, 5, 3, 2, 4, 6],
+                     [1, 2, 3, 4, 5, 5, 6, 7, 8, 9, 10],
+                     [1, 2, 3, 4, 5, 5, 6, 7, 8, 9, 10],
+                     [16.4, 6.5, 5, 5.875, 5.48, None, 6.1, 6.61905, 3.0625, 2.63889, 2.06667]],
+        "tolerance": 1e-5,
+    },
 ]}
 
 
