@@ -152,6 +152,7 @@ _OPTIONAL_SIGS = {
     "gz_agg_result_type": (ctypes.c_int, [_P, _I32]),
     "gz_agg_finish": (ctypes.c_int, [_P]),
     "gz_agg_result_column": (ctypes.c_int, [_P, _I32, _PP, _PP]),
+    "gz_agg_result_binary": (ctypes.c_int, [_P, _I32, _PP, _PP, _PP]),
     "gz_agg_release": (ctypes.c_int, [_P]),
     "gz_agg_set_make": (ctypes.c_int, [_P, _I32, _PI32, _I32, ctypes.POINTER(GzAggAction),
                                        ctypes.POINTER(GzAggOptions), _I32, _PP]),

@@ -195,6 +195,17 @@ class HashAggregateResultIterator:
         out = []
         for c in range(ncol):
             t = L.lib().gz_agg_result_type(self._h, c)
+            if t == T.STRING:  # a utf8 key: bytes objects per group (see valid)
+                offp, datap, validp = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+                L.check(L.lib().gz_agg_result_binary(self._h, c, ctypes.byref(offp), ctypes.byref(datap),
+                                                     ctypes.byref(validp)))
+                offs = to_host(offp.value, (g + 1) * 4, np.dtype(np.int32))
+                nbytes = int(offs[-1])
+                data = to_host(datap.value, nbytes, np.dtype(np.uint8)) if nbytes else np.zeros(0, np.uint8)
+                ok = to_host(validp.value, g, np.uint8).astype(bool) if g else np.zeros(0, bool)
+                v = np.array([data[offs[i]:offs[i + 1]].tobytes() for i in range(g)] + [None], object)[:g]
+                out.append((t, v, ok))
+                continue
             vals, valid = ctypes.c_void_p(), ctypes.c_void_p()
             if self.num_instances == 1:
                 L.check(L.lib().gz_agg_result_column(self._h, c, ctypes.byref(vals), ctypes.byref(valid)))

@@ -1708,9 +1708,14 @@ static int agg_set_make(gz_context* ctx, int32_t ncols, const int32_t* types, in
   if (!ctx || !out || (ncols > 0 && !types) || !acts) return fail(GZ_INVALID, "gz_agg_make: null argument");
   if (ninst <= 0 || ninst > 65535) return fail(GZ_INVALID, "instance count must be in [1, 65535]");
   int nkeys = 0;
-  for (int i = 0; i < nact; i++) nkeys += acts[i].kind == GZ_ACTION_GROUPBY;
-  if (nkeys > 1) {
-    if (ninst > 1) return fail(GZ_NOT_IMPLEMENTED, "several group-by keys in an aggregate set");
+  bool str_key = false;
+  for (int i = 0; i < nact; i++) {
+    if (acts[i].kind != GZ_ACTION_GROUPBY) continue;
+    nkeys++;
+    if (acts[i].input0 >= 0 && acts[i].input0 < ncols && types[acts[i].input0] == GZ_TYPE_STRING) str_key = true;
+  }
+  if (nkeys > 1 || str_key) {
+    if (ninst > 1) return fail(GZ_NOT_IMPLEMENTED, "multi-column or utf8 group-by keys in an aggregate set");
     if (nact > kMaxActions) return fail(GZ_INVALID, "bad action count");
     auto* h = new gz_agg();
     h->ctx = ctx;
@@ -2659,6 +2664,12 @@ int gz_agg_result_column(gz_agg* h, int32_t col, void** values, void** valid) {
   if (values) *values = (char*)h->res_values[col].p + h->group_off[0] * w;
   if (valid) *valid = (char*)h->res_valid[col].p + h->group_off[0];
   return GZ_OK;
+}
+
+int gz_agg_result_binary(gz_agg* h, int32_t col, void** offsets, void** data, void** valid) {
+  if (!h) return fail(GZ_INVALID, "null aggregate");
+  if (!h->mk) return fail(GZ_TYPE_ERROR, "result column is not utf8");
+  return mk_result_binary(h->mk, col, offsets, data, valid);
 }
 
 int gz_agg_release(gz_agg* h) {

@@ -158,7 +158,7 @@ struct RelationView {
 };
 int relation_view(gz_relation* r, RelationView* v);
 
-// Hash aggregate grouped by several key columns (gz_keys.hip): a device key
+// Hash aggregate grouped by several key columns or a utf8 key (gz_keys.hip): a device key
 // dictionary maps every row's key tuple to a dense code, and a single-key
 // aggregate groups by the code.
 struct MultiKey;
@@ -170,6 +170,7 @@ int mk_num_result_columns(MultiKey* m);
 int mk_result_type(MultiKey* m, int col);
 int mk_finish(MultiKey* m);
 int mk_result_column(MultiKey* m, int col, void** values, void** valid);
+int mk_result_binary(MultiKey* m, int col, void** offsets, void** data, void** valid);
 void mk_release(MultiKey* m);
 
 // Probe on another context than the relation's (resident relations).

@@ -21,8 +21,15 @@
 //   key_code_kernel    every row reads its slot's code
 // The aggregate's own first-row order then gives the reference's group order,
 // and the finish gathers each key column from the stored tuples by code.
+//
+// A utf8 key (alone, or the last of several keys) enters the tuple as a 64-bit
+// hash of its bytes; equal hashes are confirmed on the bytes, and every code
+// keeps its string in a device byte heap.  Alone it groups like the reference's
+// LargeStringHashMap (the null key its own group); last in an UnsafeRow its
+// bytes end the row, so tuple equality is the row's byte equality there too.
 #include <algorithm>
 #include <cstring>
+#include <rocprim/rocprim.hpp>
 #include <vector>
 
 #include "gz_internal.h"
@@ -34,7 +41,8 @@ constexpr uint32_t kPermanent = 0x80000000u;  // slot reference is a code
 
 struct KeyBatchPtrs {
   const uint8_t* valid[kMaxKeys];
-  const uint8_t* vals[kMaxKeys];
+  const uint8_t* vals[kMaxKeys];  // the utf8 key: its data buffer
+  const int32_t* soffs;           // the utf8 key's offsets
 };
 
 struct KeyParams {
@@ -43,14 +51,41 @@ struct KeyParams {
   int nb;
   int K;
   int width[kMaxKeys];  // bytes per value: 1, 2, 4 or 8
+  int sj;               // index of the utf8 key (the last one), -1 if none
   int64_t rows;         // rows of the call
   unsigned long long* table;
   uint64_t mask;  // table capacity - 1
   unsigned long long* kvals;  // [code * K + j] raw key bits (0 under a null)
   uint32_t* kmask;            // [code] null mask
-  unsigned long long* counter;
+  unsigned long long* counter;  // [0] codes, [1] heap bytes used
   int64_t* rowslot;  // per call row: its slot, then its code
+  uint8_t* heap;     // strings of the codes (utf8 key)
+  int64_t* koff;     // [code] heap offset of its string
+  int32_t* klen;     // [code] its length
 };
+
+// A utf8 key's bytes as seen by a comparison.
+struct StrRef {
+  const uint8_t* p;
+  int32_t n;
+};
+
+__device__ __forceinline__ uint64_t bytes_hash64(const uint8_t* p, int32_t n) {
+  uint64_t h = 0x27d4eb2f165667c5ull ^ (uint64_t)n;
+  uint64_t w = 0;
+  int k = 0;
+  for (int32_t i = 0; i < n; i++) {
+    w |= (uint64_t)p[i] << (8 * k);
+    if (++k == 8) {
+      h = (h ^ w) * 0x9e3779b97f4a7c15ull;
+      h ^= h >> 29;
+      w = 0;
+      k = 0;
+    }
+  }
+  h = (h ^ w ^ ((uint64_t)k << 56)) * 0xbf58476d1ce4e5b9ull;
+  return h ^ (h >> 31);
+}
 
 __device__ __forceinline__ int batch_of(const int64_t* prefix, int nb, int64_t g) {
   int lo = 0, hi = nb;  // prefix[lo] <= g < prefix[hi]
@@ -62,19 +97,27 @@ __device__ __forceinline__ int batch_of(const int64_t* prefix, int nb, int64_t g
   return lo;
 }
 
-// Key tuple of call row g: raw bits per key (zero-extended), null mask.
-__device__ __forceinline__ uint32_t load_key(const KeyParams& p, int64_t g, uint64_t* raw) {
+// Key tuple of call row g: raw bits per key (zero-extended; the utf8 key's
+// hash), null mask, and the utf8 key's bytes.
+__device__ __forceinline__ uint32_t load_key(const KeyParams& p, int64_t g, uint64_t* raw, StrRef* str) {
   const int b = batch_of(p.prefix, p.nb, g);
   const int64_t r = g - p.prefix[b];
   const KeyBatchPtrs& B = p.kb[b];
   uint32_t m = 0;
+  str->p = nullptr;
+  str->n = 0;
 #pragma unroll
   for (int j = 0; j < kMaxKeys; j++) {
     raw[j] = 0;
     if (j < p.K) {
       const uint8_t* vb = B.valid[j];
       const bool ok = !vb || ((vb[r >> 3] >> (r & 7)) & 1);
-      if (ok) {
+      if (ok && j == p.sj) {
+        const int32_t o0 = B.soffs[r];
+        str->p = B.vals[j] + o0;
+        str->n = B.soffs[r + 1] - o0;
+        raw[j] = bytes_hash64(str->p, str->n);
+      } else if (ok) {
         const uint8_t* v = B.vals[j];
         switch (p.width[j]) {
           case 1: raw[j] = v[r]; break;
@@ -105,21 +148,29 @@ __device__ __forceinline__ uint64_t key_hash(const uint64_t* raw, uint32_t m, in
   return h;
 }
 
-__device__ __forceinline__ bool key_equal(const KeyParams& p, uint32_t ref, const uint64_t* raw, uint32_t m) {
+__device__ __forceinline__ bool key_equal(const KeyParams& p, uint32_t ref, const uint64_t* raw, uint32_t m,
+                                          const StrRef& str) {
   uint64_t o[kMaxKeys];
   uint32_t om;
+  StrRef os;
   if (ref & kPermanent) {
     const uint64_t c = ref & ~kPermanent;
     om = p.kmask[c];
 #pragma unroll
     for (int j = 0; j < kMaxKeys; j++) o[j] = j < p.K ? p.kvals[c * p.K + j] : 0;
+    os.p = p.sj >= 0 ? p.heap + p.koff[c] : nullptr;
+    os.n = p.sj >= 0 ? p.klen[c] : 0;
   } else {
-    om = load_key(p, (int64_t)ref, o);
+    om = load_key(p, (int64_t)ref, o, &os);
   }
   bool eq = om == m;
 #pragma unroll
   for (int j = 0; j < kMaxKeys; j++)
     if (j < p.K) eq = eq && o[j] == raw[j];
+  if (eq && p.sj >= 0) {
+    eq = os.n == str.n;
+    for (int32_t i = 0; eq && i < str.n; i++) eq = os.p[i] == str.p[i];
+  }
   return eq;
 }
 
@@ -127,7 +178,8 @@ __global__ __launch_bounds__(256) void key_find_kernel(KeyParams p) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= p.rows) return;
   uint64_t raw[kMaxKeys];
-  const uint32_t m = load_key(p, g, raw);
+  StrRef str;
+  const uint32_t m = load_key(p, g, raw, &str);
   const uint64_t h = key_hash(raw, m, p.K);
   const uint64_t fp = (h >> 32) | 1u;  // never zero: an occupied slot is never 0
   const unsigned long long mine = (fp << 32) | (uint64_t)g;
@@ -140,7 +192,7 @@ __global__ __launch_bounds__(256) void key_find_kernel(KeyParams p) {
       if (old == 0) break;
       w = old;
     }
-    if ((w >> 32) == fp && key_equal(p, (uint32_t)w, raw, m)) break;
+    if ((w >> 32) == fp && key_equal(p, (uint32_t)w, raw, m, str)) break;
     s = (s + 1) & p.mask;
   }
   p.rowslot[g] = (int64_t)s;
@@ -155,12 +207,19 @@ __global__ __launch_bounds__(256) void key_assign_kernel(KeyParams p) {
   // the top bit set, call rows are below 2^31)
   if ((uint32_t)w != (uint32_t)g) return;
   uint64_t raw[kMaxKeys];
-  const uint32_t m = load_key(p, g, raw);
+  StrRef str;
+  const uint32_t m = load_key(p, g, raw, &str);
   const unsigned long long c = atomicAdd(p.counter, 1ull);
 #pragma unroll
   for (int j = 0; j < kMaxKeys; j++)
     if (j < p.K) p.kvals[c * p.K + j] = raw[j];
   p.kmask[c] = m;
+  if (p.sj >= 0) {
+    const unsigned long long at = str.n > 0 ? atomicAdd(p.counter + 1, (unsigned long long)str.n) : 0ull;
+    for (int32_t i = 0; i < str.n; i++) p.heap[at + i] = str.p[i];
+    p.koff[c] = (int64_t)at;
+    p.klen[c] = str.n;
+  }
   p.table[s] = (w & 0xffffffff00000000ull) | kPermanent | (uint32_t)c;
 }
 
@@ -189,6 +248,7 @@ struct KeyGatherParams {
   const int64_t* codes;  // per group
   int64_t n;
   int K;
+  int sj;  // the utf8 key (validity only here), -1 if none
   int width[kMaxKeys];
   const unsigned long long* kvals;
   const uint32_t* kmask;
@@ -201,6 +261,8 @@ __global__ __launch_bounds__(256) void key_gather_kernel(KeyGatherParams f) {
     const int64_t c = f.codes[i];
     const uint32_t m = f.kmask[c];
     for (int j = 0; j < f.K; j++) {
+      f.valid[j][i] = ((m >> j) & 1) ? 0 : 1;
+      if (j == f.sj) continue;
       const uint64_t v = f.kvals[c * f.K + j];
       switch (f.width[j]) {
         case 1: ((uint8_t*)f.out[j])[i] = (uint8_t)v; break;
@@ -208,9 +270,26 @@ __global__ __launch_bounds__(256) void key_gather_kernel(KeyGatherParams f) {
         case 4: ((uint32_t*)f.out[j])[i] = (uint32_t)v; break;
         default: ((uint64_t*)f.out[j])[i] = v; break;
       }
-      f.valid[j][i] = ((m >> j) & 1) ? 0 : 1;
     }
   }
+}
+
+// utf8 key result: lengths per group (n + 1 entries, the last 0), scanned into
+// offsets, then each group's bytes copied from the heap.
+__global__ __launch_bounds__(256) void key_str_len_kernel(const int64_t* codes, int64_t n, const int32_t* klen,
+                                                          int32_t* lens) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) lens[i] = klen[codes[i]];
+  else if (i == n) lens[n] = 0;
+}
+
+__global__ __launch_bounds__(256) void key_str_copy_kernel(const int64_t* codes, int64_t n, const int64_t* koff,
+                                                           const int32_t* offs, const uint8_t* heap, uint8_t* data) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint8_t* src = heap + koff[codes[i]];
+  const int32_t o = offs[i], len = offs[i + 1] - o;
+  for (int32_t b = 0; b < len; b++) data[o + b] = src[b];
 }
 
 __global__ void fill_u64_kernel_keys(unsigned long long* p, int64_t n) {
@@ -229,16 +308,20 @@ struct MultiKey {
   std::vector<int> key_cols, key_types;
   std::vector<int> res_map;  // outer result -> inner result (>= 0) or key -(j + 1)
   int code_res = 0;          // inner result holding the code
+  int sj = -1;               // the utf8 key (last), -1 if none
   // dictionary
   DBuf table, kvals, kmask, counter;
   uint64_t tcap = 0;
   int64_t kcap = 0, ncodes = 0;
+  DBuf heap, koff, klen;  // utf8 key: strings of the codes
+  int64_t heap_cap = 0, heap_used = 0;
   // per call
   DBuf rowslot, d_kb, d_prefix;
   // finish
   bool finished = false;
   int64_t ngroups = 0;
-  std::vector<DBuf> out_vals, out_valid;
+  std::vector<DBuf> out_vals, out_valid;  // utf8 key: out_vals = offsets
+  DBuf str_data, str_tmp;
 };
 
 static bool key_type_ok(int t) {
@@ -269,9 +352,11 @@ int mk_make(gz_context* ctx, int32_t ncols, const int32_t* types, int32_t nact, 
     if (a.kind == GZ_ACTION_GROUPBY) {
       if (a.input0 < 0 || a.input0 >= ncols) return bad(GZ_INVALID, "groupby input out of range");
       const int t = types[a.input0];
-      if (!key_type_ok(t))
+      if (m->sj >= 0) return bad(GZ_NOT_IMPLEMENTED, "a group-by key after a utf8 key");
+      if (t == GZ_TYPE_STRING) m->sj = m->K;
+      else if (!key_type_ok(t))
         return bad(GZ_NOT_IMPLEMENTED, "group-by key type " + std::to_string(t) +
-                                           " in a multi-column key (fixed-width numeric keys only)");
+                                           " (fixed-width numeric keys and one utf8 key, last)");
       if (m->K == kMaxKeys) return bad(GZ_NOT_IMPLEMENTED, "more than 8 group-by keys");
       m->res_map.push_back(-(m->K + 1));
       if (m->K == 0) {
@@ -294,8 +379,8 @@ int mk_make(gz_context* ctx, int32_t ncols, const int32_t* types, int32_t nact, 
     delete m;
     return rc;
   }
-  rc = ctx_alloc(ctx, sizeof(unsigned long long), &m->counter);
-  if (!rc) rc = hipMemsetAsync(m->counter.p, 0, sizeof(unsigned long long), ctx->stream) == hipSuccess
+  rc = ctx_alloc(ctx, 2 * sizeof(unsigned long long), &m->counter);
+  if (!rc) rc = hipMemsetAsync(m->counter.p, 0, 2 * sizeof(unsigned long long), ctx->stream) == hipSuccess
                     ? GZ_OK
                     : fail(GZ_HIP_ERROR, "clearing the key counter");
   if (rc) {
@@ -308,29 +393,39 @@ int mk_make(gz_context* ctx, int32_t ncols, const int32_t* types, int32_t nact, 
   return GZ_OK;
 }
 
-static int key_width(int t) { return type_width(t); }
+static int key_width(int t) { return t == GZ_TYPE_STRING ? 0 : type_width(t); }
+
+// Grows a device buffer to `bytes`, keeping its first `keep` bytes.
+static int grow_keep(gz_context* ctx, DBuf* b, size_t bytes, size_t keep) {
+  DBuf nb;
+  GZ_TRY(ctx_alloc(ctx, bytes, &nb));
+  if (keep > 0) GZ_HIP(hipMemcpyAsync(nb.p, b->p, keep, hipMemcpyDeviceToDevice, ctx->stream));
+  GZ_HIP(hipStreamSynchronize(ctx->stream));
+  ctx_free(ctx, b);
+  *b = nb;
+  return GZ_OK;
+}
 
 // Grows the code store to hold `need` codes and the table to at least twice
 // that, re-inserting the codes already assigned.
-static int mk_reserve(MultiKey* m, int64_t need) {
+static int mk_reserve(MultiKey* m, int64_t need, int64_t heap_need) {
   gz_context* ctx = m->ctx;
   if (need > m->kcap) {
     const int64_t cap = std::max<int64_t>({need, 2 * m->kcap, 1024});
-    DBuf nv, nm;
-    GZ_TRY(ctx_alloc(ctx, sizeof(unsigned long long) * (size_t)cap * m->K, &nv));
-    GZ_TRY(ctx_alloc(ctx, sizeof(uint32_t) * (size_t)cap, &nm));
-    if (m->ncodes > 0) {
-      GZ_HIP(hipMemcpyAsync(nv.p, m->kvals.p, sizeof(unsigned long long) * (size_t)m->ncodes * m->K,
-                            hipMemcpyDeviceToDevice, ctx->stream));
-      GZ_HIP(hipMemcpyAsync(nm.p, m->kmask.p, sizeof(uint32_t) * (size_t)m->ncodes, hipMemcpyDeviceToDevice,
-                            ctx->stream));
+    const size_t n = (size_t)m->ncodes;
+    GZ_TRY(grow_keep(ctx, &m->kvals, sizeof(unsigned long long) * (size_t)cap * m->K,
+                     sizeof(unsigned long long) * n * m->K));
+    GZ_TRY(grow_keep(ctx, &m->kmask, sizeof(uint32_t) * (size_t)cap, sizeof(uint32_t) * n));
+    if (m->sj >= 0) {
+      GZ_TRY(grow_keep(ctx, &m->koff, sizeof(int64_t) * (size_t)cap, sizeof(int64_t) * n));
+      GZ_TRY(grow_keep(ctx, &m->klen, sizeof(int32_t) * (size_t)cap, sizeof(int32_t) * n));
     }
-    GZ_HIP(hipStreamSynchronize(ctx->stream));
-    ctx_free(ctx, &m->kvals);
-    ctx_free(ctx, &m->kmask);
-    m->kvals = nv;
-    m->kmask = nm;
     m->kcap = cap;
+  }
+  if (m->sj >= 0 && (heap_need > m->heap_cap || !m->heap.p)) {
+    const int64_t cap = std::max<int64_t>({heap_need, 2 * m->heap_cap, 4096});
+    GZ_TRY(grow_keep(ctx, &m->heap, (size_t)cap, (size_t)m->heap_used));
+    m->heap_cap = cap;
   }
   uint64_t tcap = 1024;
   while (tcap < 2 * (uint64_t)need) tcap <<= 1;
@@ -367,6 +462,7 @@ int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
   }
   std::vector<KeyBatchPtrs> kb(std::max(nb, 1));
   std::vector<int64_t> prefix(nb + 1, 0);
+  int64_t heap_need = m->heap_used;
   for (int b = 0; b < nb; b++) {
     const gz_batch& bt = batches[b];
     if (bt.num_rows < 0 || bt.num_rows > INT32_MAX) return fail(GZ_INVALID, "bad num_rows");
@@ -378,9 +474,18 @@ int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
         const int k = col_buf[m->key_cols[j]];
         const bool has_valid = bt.buf_addrs[k] != 0 && bt.buf_sizes[k] > 0;
         if (has_valid && bt.buf_sizes[k] < (bt.num_rows + 7) / 8) return fail(GZ_INVALID, "validity buffer too small");
+        kb[b].valid[j] = has_valid ? (const uint8_t*)bt.buf_addrs[k] : nullptr;
+        if (j == m->sj) {
+          if (!bt.buf_addrs[k + 1] || bt.buf_sizes[k + 1] < (bt.num_rows + 1) * 4)
+            return fail(GZ_INVALID, "offsets buffer too small");
+          if (bt.buf_sizes[k + 2] > 0 && !bt.buf_addrs[k + 2]) return fail(GZ_INVALID, "missing values buffer");
+          kb[b].soffs = (const int32_t*)bt.buf_addrs[k + 1];
+          kb[b].vals[j] = (const uint8_t*)bt.buf_addrs[k + 2];
+          heap_need += std::max<int64_t>(bt.buf_sizes[k + 2], 0);
+          continue;
+        }
         if (!bt.buf_addrs[k + 1] || bt.buf_sizes[k + 1] < bt.num_rows * key_width(m->key_types[j]))
           return fail(GZ_INVALID, "values buffer too small");
-        kb[b].valid[j] = has_valid ? (const uint8_t*)bt.buf_addrs[k] : nullptr;
         kb[b].vals[j] = (const uint8_t*)bt.buf_addrs[k + 1];
       }
     }
@@ -390,7 +495,7 @@ int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
   if (rows == 0) return GZ_OK;  // ProcessAndCacheOne returns early on empty input
   if (rows > INT32_MAX) return fail(GZ_NOT_IMPLEMENTED, "2^31 or more rows in one multi-key call");
   if (m->ncodes + rows > INT32_MAX) return fail(GZ_CAPACITY_ERROR, "2^31 or more groups");
-  GZ_TRY(mk_reserve(m, m->ncodes + rows));
+  GZ_TRY(mk_reserve(m, m->ncodes + rows, heap_need));
   GZ_TRY(upload(ctx, &m->d_kb, kb.data(), sizeof(KeyBatchPtrs) * kb.size()));
   GZ_TRY(upload(ctx, &m->d_prefix, prefix.data(), sizeof(int64_t) * prefix.size()));
   GZ_TRY(ctx_ensure(ctx, &m->rowslot, sizeof(int64_t) * (size_t)rows));
@@ -408,6 +513,10 @@ int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
   p.kmask = (uint32_t*)m->kmask.p;
   p.counter = (unsigned long long*)m->counter.p;
   p.rowslot = (int64_t*)m->rowslot.p;
+  p.sj = m->sj;
+  p.heap = (uint8_t*)m->heap.p;
+  p.koff = (int64_t*)m->koff.p;
+  p.klen = (int32_t*)m->klen.p;
   const unsigned grid = (unsigned)((rows + 255) / 256);
   {
     KernelTimer kt(ctx, "key_find");
@@ -420,10 +529,11 @@ int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
                        (const unsigned long long*)m->table.p, (int64_t*)m->rowslot.p, rows);
   }
   GZ_HIP(hipGetLastError());
-  unsigned long long nc = 0;
-  GZ_HIP(hipMemcpyAsync(&nc, m->counter.p, sizeof(nc), hipMemcpyDeviceToHost, ctx->stream));
+  unsigned long long nc[2] = {0, 0};
+  GZ_HIP(hipMemcpyAsync(nc, m->counter.p, sizeof(nc), hipMemcpyDeviceToHost, ctx->stream));
   GZ_HIP(hipStreamSynchronize(ctx->stream));
-  m->ncodes = (int64_t)nc;
+  m->ncodes = (int64_t)nc[0];
+  m->heap_used = (int64_t)nc[1];
   // the single-key aggregate over the batches with the code column appended
   std::vector<int64_t> addrs((size_t)nb * (need_bufs + 2)), sizes(addrs.size());
   std::vector<gz_batch> inner(nb);
@@ -471,9 +581,13 @@ int mk_finish(MultiKey* m) {
   f.K = m->K;
   f.kvals = (const unsigned long long*)m->kvals.p;
   f.kmask = (const uint32_t*)m->kmask.p;
+  f.sj = m->sj;
   for (int j = 0; j < m->K; j++) {
     f.width[j] = key_width(m->key_types[j]);
-    GZ_TRY(ctx_alloc(ctx, (size_t)std::max<int64_t>(m->ngroups, 1) * f.width[j], &m->out_vals[j]));
+    // the utf8 key's values are its int32 offsets (groups + 1)
+    const size_t vbytes = j == m->sj ? sizeof(int32_t) * (size_t)(m->ngroups + 1)
+                                     : (size_t)std::max<int64_t>(m->ngroups, 1) * f.width[j];
+    GZ_TRY(ctx_alloc(ctx, vbytes, &m->out_vals[j]));
     GZ_TRY(ctx_alloc(ctx, (size_t)std::max<int64_t>(m->ngroups, 1), &m->out_valid[j]));
     f.out[j] = m->out_vals[j].p;
     f.valid[j] = (uint8_t*)m->out_valid[j].p;
@@ -482,6 +596,30 @@ int mk_finish(MultiKey* m) {
     KernelTimer kt(ctx, "key_gather");
     const unsigned gx = (unsigned)std::min<int64_t>((m->ngroups + 255) / 256, 8192);
     hipLaunchKernelGGL(key_gather_kernel, dim3(gx), dim3(256), 0, ctx->stream, f);
+  }
+  if (m->sj >= 0) {
+    if (m->heap_used > INT32_MAX) return fail(GZ_CAPACITY_ERROR, "utf8 keys of 2^31 bytes or more");
+    ctx_free(ctx, &m->str_data);
+    GZ_TRY(ctx_alloc(ctx, (size_t)std::max<int64_t>(m->heap_used, 1), &m->str_data));
+    DBuf lens;
+    GZ_TRY(ctx_alloc(ctx, sizeof(int32_t) * (size_t)(m->ngroups + 1), &lens));
+    int32_t* offs = (int32_t*)m->out_vals[m->sj].p;
+    const unsigned g1 = (unsigned)((m->ngroups + 1 + 255) / 256);
+    hipLaunchKernelGGL(key_str_len_kernel, dim3(g1), dim3(256), 0, ctx->stream, (const int64_t*)codes, m->ngroups,
+                       (const int32_t*)m->klen.p, (int32_t*)lens.p);
+    size_t tmp = 0;
+    GZ_HIP(rocprim::exclusive_scan(nullptr, tmp, (const int32_t*)lens.p, offs, 0, (size_t)(m->ngroups + 1),
+                                   rocprim::plus<int32_t>(), ctx->stream));
+    GZ_TRY(ctx_ensure(ctx, &m->str_tmp, std::max<size_t>(tmp, 1)));
+    GZ_HIP(rocprim::exclusive_scan(m->str_tmp.p, tmp, (const int32_t*)lens.p, offs, 0, (size_t)(m->ngroups + 1),
+                                   rocprim::plus<int32_t>(), ctx->stream));
+    if (m->ngroups > 0)
+      hipLaunchKernelGGL(key_str_copy_kernel, dim3((unsigned)((m->ngroups + 255) / 256)), dim3(256), 0, ctx->stream,
+                         (const int64_t*)codes, m->ngroups, (const int64_t*)m->koff.p, (const int32_t*)offs,
+                         (const uint8_t*)m->heap.p, (uint8_t*)m->str_data.p);
+    GZ_HIP(hipGetLastError());
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+    ctx_free(ctx, &lens);
   }
   GZ_HIP(hipGetLastError());
   GZ_HIP(hipStreamSynchronize(ctx->stream));
@@ -505,13 +643,25 @@ int mk_result_column(MultiKey* m, int col, void** values, void** valid) {
   return GZ_OK;
 }
 
+int mk_result_binary(MultiKey* m, int col, void** offsets, void** data, void** valid) {
+  if (!m->finished) return fail(GZ_INVALID, "call gz_agg_finish first");
+  if (col < 0 || col >= (int)m->res_map.size()) return fail(GZ_INDEX_ERROR, "result column out of range");
+  const int r = m->res_map[col];
+  if (r >= 0 || -r - 1 != m->sj) return fail(GZ_TYPE_ERROR, "result column is not utf8");
+  if (offsets) *offsets = m->out_vals[m->sj].p;
+  if (data) *data = m->str_data.p;
+  if (valid) *valid = m->out_valid[m->sj].p;
+  return GZ_OK;
+}
+
 void mk_release(MultiKey* m) {
   if (!m) return;
   gz_context* ctx = m->ctx;
   gz_agg_release(m->inner);
   for (auto& b : m->out_vals) ctx_free(ctx, &b);
   for (auto& b : m->out_valid) ctx_free(ctx, &b);
-  for (DBuf* b : {&m->table, &m->kvals, &m->kmask, &m->counter, &m->rowslot, &m->d_kb, &m->d_prefix})
+  for (DBuf* b : {&m->table, &m->kvals, &m->kmask, &m->counter, &m->rowslot, &m->d_kb, &m->d_prefix, &m->heap,
+                  &m->koff, &m->klen, &m->str_data, &m->str_tmp})
     ctx_free(ctx, b);
   delete m;
 }

@@ -275,12 +275,13 @@ typedef struct {
 void gz_agg_options_defaults(gz_agg_options* opts);
 
 /* Replaces HashAggregateKernel::Make + MakeResultIterator (hash_aggregate_kernel.cc:1238-1260,
- * :147-224).  A plan has one GZ_ACTION_GROUPBY (an integer key), none (the
- * global aggregate), or up to 8 of them: then rows group by all those columns
- * together with the reference's UnsafeRow key equality (the same keys null,
- * identical bits in the others: hash_aggregate_kernel.cc:970-1000), each key
- * fixed-width numeric (integers, dates, timestamps, float, double); the
- * gz_agg_set_* forms take one key only. */
+ * :147-224).  A plan has one GZ_ACTION_GROUPBY (an integer or utf8 key), none
+ * (the global aggregate), or up to 8 of them: then rows group by all those
+ * columns together with the reference's UnsafeRow key equality (the same keys
+ * null, identical bytes in the others: hash_aggregate_kernel.cc:970-1000), each
+ * key fixed-width numeric (integers, dates, timestamps, float, double) except
+ * that the last may be utf8.  A utf8 key groups by its bytes, the null key its
+ * own group (:1001-1020).  The gz_agg_set_* forms take one integer key only. */
 int gz_agg_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types,
                 int32_t num_actions, const gz_agg_action* actions,
                 const gz_agg_options* opts, gz_agg** out);
@@ -299,6 +300,11 @@ int gz_agg_result_type(gz_agg* h, int32_t col);
  * the result type) and validity as one byte per group (1 = valid). */
 int gz_agg_finish(gz_agg* h);
 int gz_agg_result_column(gz_agg* h, int32_t col, void** values, void** valid_bytes);
+/* A utf8 result column (a utf8 group-by key): int32 offsets (num_groups + 1),
+ * the bytes, validity one byte per group; gz_agg_result_column gives the
+ * offsets as its values.  Replaces the StringType UniqueAction's Finish
+ * (actions_impl.cc:132-290). */
+int gz_agg_result_binary(gz_agg* h, int32_t col, void** offsets, void** data, void** valid_bytes);
 int gz_agg_release(gz_agg* h);
 
 /* Aggregate sets: many independent GROUP BYs (instances) sharing one plan,

@@ -67,6 +67,10 @@ def lib():
         L.gzo_agg_num_results.argtypes = [ctypes.c_void_p]
         L.gzo_agg_result_type.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.gzo_agg_result.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.gzo_agg_result_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.gzo_agg_result_bytes.restype = ctypes.c_int64
+        L.gzo_agg_result_binary.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p]
         L.gzo_agg_free.argtypes = [ctypes.c_void_p]
         L.gzo_rel_make.restype = ctypes.c_void_p
         L.gzo_rel_make.argtypes = [ctypes.c_int32, ctypes.c_int32]
@@ -268,6 +272,15 @@ def aggregate(batches, types: Sequence[int], actions: Sequence[Sequence[int]]) -
         out = []
         for c in range(L.gzo_agg_num_results(h)):
             t = L.gzo_agg_result_type(h, c)
+            if t in (13, 14):  # a utf8 key: bytes objects (None-free; see valid)
+                nbytes = L.gzo_agg_result_bytes(h, c)
+                offs = np.zeros(g + 1, np.int32)
+                data = np.zeros(max(1, nbytes), np.uint8)
+                valid = np.zeros(max(1, g), np.uint8)
+                L.gzo_agg_result_binary(h, c, _ptr(offs), _ptr(data), _ptr(valid))
+                vals = np.array([data[offs[i]:offs[i + 1]].tobytes() for i in range(g)] + [None], object)[:g]
+                out.append((t, vals, valid[:g].astype(bool)))
+                continue
             vals = np.zeros(max(1, g) * _WIDTH.get(t, 8), np.uint8)
             valid = np.zeros(max(1, g), np.uint8)
             L.gzo_agg_result(h, c, _ptr(vals), _ptr(valid))

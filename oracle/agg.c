@@ -15,7 +15,7 @@
 
 #include "gz_oracle.h"
 
-enum { T_NA = 0, T_BOOL = 1, T_UINT8 = 2, T_INT8 = 3, T_UINT16 = 4, T_INT16 = 5,
+enum { T_NA = 0, T_BOOL = 1, T_STRING = 13, T_BINARY = 14, T_UINT8 = 2, T_INT8 = 3, T_UINT16 = 4, T_INT16 = 5,
        T_UINT32 = 6, T_INT32 = 7, T_UINT64 = 8, T_INT64 = 9, T_FLOAT = 11, T_DOUBLE = 12,
        T_DATE32 = 16, T_DATE64 = 17, T_TIMESTAMP = 18 };
 enum { A_GROUPBY = 0, A_COUNT = 1, A_COUNT_LITERAL = 2, A_SUM = 3, A_AVG = 4, A_MIN = 5,
@@ -65,6 +65,10 @@ typedef struct {
   uint8_t* valid;  /* cache_validity_ */
   uint8_t* nullf;  /* UniqueAction null_flag_ */
   int64_t* key;    /* UniqueAction cache_ (bit pattern) */
+  int64_t* soff;   /* UniqueAction cache_ of a utf8 key: bytes in sarena */
+  int32_t* slen;
+  uint8_t* sarena;
+  int64_t sused, scap_bytes;
 } action_t;
 
 struct gzo_agg {
@@ -92,7 +96,11 @@ struct gzo_agg {
   int32_t* sid;
   uint64_t* shash;
   int64_t scap, ssize;
+  uint8_t* rowbuf;  /* the current row's UnsafeRow bytes */
+  int64_t rowcap;
 };
+
+static int is_binary(int32_t t) { return t == T_STRING || t == T_BINARY; }
 
 static uint64_t mix64(uint64_t x) {
   x ^= x >> 30; x *= 0xbf58476d1ce4e5b9ULL;
@@ -125,6 +133,7 @@ static void grow_groups(gzo_agg* a, int64_t need) {
 #define GROW(f, T) do { x->f = (T*)realloc(x->f, sizeof(T) * nc); memset(x->f + a->gcap, 0, sizeof(T) * (nc - a->gcap)); } while (0)
     GROW(dsum, double); GROW(isum, int64_t); GROW(cnt, int64_t); GROW(imm, int64_t);
     GROW(fmm, double); GROW(valid, uint8_t); GROW(nullf, uint8_t); GROW(key, int64_t);
+    GROW(soff, int64_t); GROW(slen, int32_t);
 #undef GROW
   }
   a->gcap = nc;
@@ -190,10 +199,27 @@ static int32_t bytes_get_or_insert(gzo_agg* a, const uint8_t* p, int32_t n) {
 
 /* The key of row r as an UnsafeRow (unsafe_row.h:45-149, unsafe_array.h:58-108):
  * a null bitset and an "empty" bitset of nkeys/8+1 bytes each, then, for every
- * non-null key in order, its value's sizeof(T) bytes; a null key sets its bit
- * and appends nothing.  Returns the row's byte length. */
-static int32_t unsafe_row_key(const gzo_agg* a, const gzo_batch* b, int64_t r, uint8_t* row) {
+ * non-null key in order, its value's sizeof(T) bytes, or a string's bytes (no
+ * length; an empty string sets its "empty" bit and appends nothing); a null key
+ * sets its bit and appends nothing.  Returns the row's byte length; the bytes
+ * are in a->rowbuf. */
+static int32_t unsafe_row_key(gzo_agg* a, const gzo_batch* b, int64_t r) {
   int32_t bits = a->nkeys / 8 + 1;
+  int64_t need = 2 * bits;
+  for (int k = 0; k < a->nkeys; k++) {
+    int c = a->kcols[k];
+    if (is_binary(a->types[c])) {
+      const int32_t* off = (const int32_t*)b->bufs[3 * c + 1];
+      need += off[r + 1] - off[r];
+    } else {
+      need += width_of(a->types[c]);
+    }
+  }
+  if (need > a->rowcap) {
+    a->rowcap = need * 2;
+    a->rowbuf = (uint8_t*)realloc(a->rowbuf, (size_t)a->rowcap);
+  }
+  uint8_t* row = a->rowbuf;
   memset(row, 0, (size_t)(2 * bits));
   int32_t cur = 2 * bits;
   for (int k = 0; k < a->nkeys; k++) {
@@ -202,6 +228,14 @@ static int32_t unsafe_row_key(const gzo_agg* a, const gzo_batch* b, int64_t r, u
     const uint8_t* v = (const uint8_t*)b->bufs[3 * c + 2];
     if (vb && !bit_get(vb, r)) {
       row[k >> 3] |= (uint8_t)(1u << (k & 7));
+      continue;
+    }
+    if (is_binary(a->types[c])) {
+      const int32_t* off = (const int32_t*)b->bufs[3 * c + 1];
+      int32_t len = off[r + 1] - off[r];
+      if (len == 0) row[bits + (k >> 3)] |= (uint8_t)(1u << (k & 7));
+      memcpy(row + cur, v + off[r], (size_t)len);
+      cur += len;
       continue;
     }
     int w = width_of(a->types[c]);
@@ -248,7 +282,18 @@ static void update(action_t* x, const gzo_batch* b, int64_t r, int32_t g) {
       if (!x->valid[g]) {
         x->valid[g] = 1;
         if (is_null) x->nullf[g] = 1;
-        else if (t == T_DOUBLE) memcpy(&x->key[g], (const double*)v + r, 8);
+        else if (is_binary(t)) {
+          const int32_t* off = (const int32_t*)b->bufs[3 * x->in0 + 1];
+          int32_t len = off[r + 1] - off[r];
+          if (x->sused + len > x->scap_bytes) {
+            x->scap_bytes = (x->scap_bytes + len) * 2 + 64;
+            x->sarena = (uint8_t*)realloc(x->sarena, (size_t)x->scap_bytes);
+          }
+          memcpy(x->sarena + x->sused, (const uint8_t*)v + off[r], (size_t)len);
+          x->soff[g] = x->sused;
+          x->slen[g] = len;
+          x->sused += len;
+        } else if (t == T_DOUBLE) memcpy(&x->key[g], (const double*)v + r, 8);
         else if (t == T_FLOAT) { float f = ((const float*)v)[r]; memcpy(&x->key[g], &f, 4); }
         else x->key[g] = get_i64(v, t, r);
       }
@@ -332,10 +377,23 @@ int gzo_agg_process(gzo_agg* a, const gzo_batch* b) {
   } else if (a->nkeys > 1) {
     /* several keys: HashAggregateResultIterator<StringType> with an UnsafeRow
      * key (hash_aggregate_kernel.cc:970-1000) */
-    uint8_t row[2 * (GZO_MAX_KEYS / 8 + 1) + 8 * GZO_MAX_KEYS];
     for (int64_t r = 0; r < n; r++) {
-      int32_t len = unsafe_row_key(a, b, r, row);
-      idx[r] = bytes_get_or_insert(a, row, len);
+      int32_t len = unsafe_row_key(a, b, r);
+      idx[r] = bytes_get_or_insert(a, a->rowbuf, len);
+    }
+  } else if (is_binary(a->types[a->key_col])) {
+    /* one utf8 key: LargeStringHashMap over the string's bytes, the null key
+     * its own group (hash_aggregate_kernel.cc:1001-1020) */
+    const uint8_t* kv = (const uint8_t*)b->bufs[3 * a->key_col];
+    const int32_t* off = (const int32_t*)b->bufs[3 * a->key_col + 1];
+    const uint8_t* data = (const uint8_t*)b->bufs[3 * a->key_col + 2];
+    for (int64_t r = 0; r < n; r++) {
+      if (kv && !bit_get(kv, r)) {
+        if (a->null_id < 0) a->null_id = (int32_t)a->ngroups++;
+        idx[r] = a->null_id;
+      } else {
+        idx[r] = bytes_get_or_insert(a, data + off[r], off[r + 1] - off[r]);
+      }
     }
   } else {
     int32_t kt = a->types[a->key_col];
@@ -463,13 +521,40 @@ void gzo_agg_result(gzo_agg* a, int col, void* values, uint8_t* valid) {
   }
 }
 
+int64_t gzo_agg_result_bytes(gzo_agg* a, int col) {
+  int i, s;
+  locate(a, col, &i, &s);
+  if (i < 0 || a->acts[i].kind != A_GROUPBY || !is_binary(a->acts[i].in_type)) return -1;
+  int64_t n = 0;
+  for (int64_t g = 0; g < a->ngroups; g++)
+    if (a->acts[i].valid[g] && !a->acts[i].nullf[g]) n += a->acts[i].slen[g];
+  return n;
+}
+
+void gzo_agg_result_binary(gzo_agg* a, int col, int32_t* offsets, uint8_t* data, uint8_t* valid) {
+  int i, s;
+  locate(a, col, &i, &s);
+  action_t* x = &a->acts[i];
+  int32_t pos = 0;
+  offsets[0] = 0;
+  for (int64_t g = 0; g < a->ngroups; g++) {
+    uint8_t ok = x->valid[g] && !x->nullf[g];
+    valid[g] = ok;
+    if (ok) {
+      memcpy(data + pos, x->sarena + x->soff[g], (size_t)x->slen[g]);
+      pos += x->slen[g];
+    }
+    offsets[g + 1] = pos;
+  }
+}
+
 void gzo_agg_free(gzo_agg* a) {
   for (int i = 0; i < a->nactions; i++) {
     action_t* x = &a->acts[i];
     free(x->dsum); free(x->isum); free(x->cnt); free(x->imm); free(x->fmm);
-    free(x->valid); free(x->nullf); free(x->key);
+    free(x->valid); free(x->nullf); free(x->key); free(x->soff); free(x->slen); free(x->sarena);
   }
   free(a->acts); free(a->types); free(a->mkeys); free(a->mids); free(a->mused);
-  free(a->arena); free(a->soff); free(a->slen); free(a->sid); free(a->shash);
+  free(a->arena); free(a->soff); free(a->slen); free(a->sid); free(a->shash); free(a->rowbuf);
   free(a);
 }

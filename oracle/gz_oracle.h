@@ -71,8 +71,9 @@ void gzo_splitter_free(gzo_splitter* s);
 /* ---- hash aggregate ---- */
 typedef struct gzo_agg gzo_agg;
 /* kinds: as GZ_ACTION_* of include/gazelle_amd.h.  Several GROUPBY actions
- * group by all their columns together (an UnsafeRow key, at most
- * GZO_MAX_KEYS fixed-width columns); NULL if there are more. */
+ * group by all their columns together (an UnsafeRow key of at most
+ * GZO_MAX_KEYS columns); NULL if there are more.  A utf8 key column groups by
+ * its bytes. */
 #define GZO_MAX_KEYS 8
 gzo_agg* gzo_agg_make(int ncols, const int32_t* types, int nactions, const int32_t* kinds,
                       const int32_t* in0, const int32_t* in1, const int32_t* nan_check);
@@ -82,6 +83,10 @@ int gzo_agg_num_results(gzo_agg* a);
 int gzo_agg_result_type(gzo_agg* a, int col);
 /* values: element width of the result type; valid: one byte per group */
 void gzo_agg_result(gzo_agg* a, int col, void* values, uint8_t* valid);
+/* A utf8 key's result column: its byte count (-1 if the column is not one),
+ * then offsets (num_groups + 1), bytes and validity. */
+int64_t gzo_agg_result_bytes(gzo_agg* a, int col);
+void gzo_agg_result_binary(gzo_agg* a, int col, int32_t* offsets, uint8_t* data, uint8_t* valid);
 void gzo_agg_free(gzo_agg* a);
 
 /* ---- hash join ---- */

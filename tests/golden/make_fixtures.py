@@ -177,6 +177,38 @@ AGG_FIXTURES = {"tests": [
                      [16.4, 6.5, 5, 5.875, 5.48, None, 6.1, 6.61905, 3.0625, 2.63889, 2.06667]],
         "tolerance": 1e-5,
     },
+    {   # GroupByTwoAggregateTest, :911-1040, with its utf8 key column as written
+        # (the stddev_samp action left out)
+        "name": "GroupByTwoAggregateTestUtf8",
+        "schema": [["f0", "int64"], ["f1", "uint32"], ["f2", "double"], ["f3", "double"], ["f4", "double"],
+                   ["f5", "utf8"]],
+        "actions": [["action_groupby", [0]], ["action_groupby", [5]], ["action_sum_count", [1]],
+                    ["action_min", [1]], ["action_max", [1]], ["action_avgByCount", [2, 0]]],
+        "batches": [
+            [[1, 2, 3, 4, 5, None, 4, 1, 2, 2, 1, 1, 1, 4, 4, 3, 5, 5, 5, 5],
+             [1, 2, 3, 4, 5, 5, 4, 1, 2, 2, 1, 1, 1, 4, 4, 3, 5, 5, 5, 5],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             ["BJ", "SH", "HZ", "BH", "NY", "SH", "BH", "BJ", "SH", "SH", "BJ", "BJ", "BJ", "BH", "BH", "HZ",
+              "NY", "NY", "NY", "NY"]],
+            [[6, 7, 8, 9, 10, 10, 9, 6, 7, 7, 6, 6, 6, 9, 9, 8, 10, 10, 10, 10],
+             [6, 7, 8, 9, 10, 10, 9, 6, 7, 7, 6, 6, 6, 9, 9, 8, 10, 10, 10, 10],
+             [7, 8, 4, 5, 6, 1, 34, 54, 65, 66, 78, 12, 32, 24, 32, 45, 12, 24, 35, 46],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             ["BJ", "SH", "TK", "SH", "PH", "PH", "SH", "BJ", "SH", "SH", "BJ", "BJ", "BJ", "SH", "SH", "TK",
+              "PH", "PH", "PH", "PH"]],
+        ],
+        "expected": [[1, 2, 3, 4, 5, None, 6, 7, 8, 9, 10],                                 # :1020-1031
+                     ["BJ", "SH", "HZ", "BH", "NY", "SH", "BJ", "SH", "TK", "SH", "PH"],
+                     [5, 6, 6, 16, 25, 5, 30, 21, 16, 36, 60],
+                     [5, 3, 2, 4, 5, 1, 5, 3, 2, 4, 6],
+                     [1, 2, 3, 4, 5, 5, 6, 7, 8, 9, 10],
+                     [1, 2, 3, 4, 5, 5, 6, 7, 8, 9, 10],
+                     [16.4, 6.5, 5, 5.875, 5.48, None, 6.1, 6.61905, 3.0625, 2.63889, 2.06667]],
+        "tolerance": 1e-5,
+    },
 ]}
 
 

@@ -122,16 +122,17 @@ def test_multi_key_types_and_equality_cases(ctx):
 
 
 def test_multi_key_rejects_what_it_does_not_group(ctx):
-    """A utf8 column in a multi-column key and the set/planned forms of a
-    multi-key plan are NotImplemented (the reference groups utf8 keys; this
-    build does not)."""
+    """A utf8 key before another key (its bytes would not end the UnsafeRow)
+    and the set/planned forms of a multi-key plan are NotImplemented."""
     h = ctypes.c_void_p()
     acts = (L.GzAggAction * 2)()
     acts[0].kind, acts[0].input0, acts[0].input1 = 0, 0, -1
     acts[1].kind, acts[1].input0, acts[1].input1 = 0, 1, -1
-    types = L.i32_array([T.INT64, T.STRING])
+    types = L.i32_array([T.STRING, T.INT64])
     rc = L.lib().gz_agg_make(ctx.handle, 2, types, 2, acts, None, ctypes.byref(h))
     assert rc == 10, rc  # GZ_NOT_IMPLEMENTED
+    types = L.i32_array([T.STRING, T.STRING])
+    assert L.lib().gz_agg_make(ctx.handle, 2, types, 2, acts, None, ctypes.byref(h)) == 10
     types = L.i32_array([T.INT64, T.INT32])
     assert L.lib().gz_agg_make(ctx.handle, 2, types, 2, acts, None, ctypes.byref(h)) == 0
     try:
@@ -139,3 +140,54 @@ def test_multi_key_rejects_what_it_does_not_group(ctx):
         assert L.lib().gz_agg_set_placement(h, 0, ctypes.byref(out)) == 10
     finally:
         L.lib().gz_agg_release(h)
+
+
+def _utf8_batches(sizes, seed, pool_size=300, null_frac=0.1):
+    """(int32 k, utf8 s, double v) batches; s from a pool of random byte strings
+    of length 0-40 plus a few of 100-300 bytes (the empty string included)."""
+    rng = np.random.default_rng(seed)
+    pool = [b""] + [rng.bytes(int(rng.integers(1, 41))) for _ in range(pool_size - 6)]
+    pool += [rng.bytes(int(rng.integers(100, 301))) for _ in range(5)]
+    out = []
+    for n in sizes:
+        pick = rng.integers(0, len(pool), n)
+        enc = [pool[i] for i in pick]
+        off = np.zeros(n + 1, np.int32)
+        off[1:] = np.cumsum([len(e) for e in enc])
+        data = np.frombuffer(b"".join(enc), np.uint8).copy() if off[-1] else np.zeros(0, np.uint8)
+        sv = bitmap_from_bools(rng.random(n) >= null_frac)
+        k = rng.integers(0, 5, n).astype(np.int32)
+        v = rng.normal(size=n)
+        out.append(HostBatch(n, [HostColumn(T.INT32, k, bitmap_from_bools(rng.random(n) >= null_frac)),
+                                 HostColumn(T.STRING, data, sv, off), HostColumn(T.DOUBLE, v)]))
+    return out
+
+
+UTF8_TYPES = [T.INT32, T.STRING, T.DOUBLE]
+
+
+@pytest.mark.parametrize("mode", ["batched", "one", "calls"])
+def test_utf8_key_alone(ctx, mode):
+    """One utf8 key (the reference's LargeStringHashMap route): groups by the
+    bytes, the empty string and the null key are groups of their own."""
+    batches = _utf8_batches([3000, 0, 1, 2500, 4096], seed=21)
+    acts = [("action_sum", [2]), ("action_groupby", [1]), ("action_countLiteral_1", []), ("action_min", [0]),
+            ("action_count", [2])]
+    got = run_gpu(ctx, batches, UTF8_TYPES, acts, mode)
+    exp = expected(batches, UTF8_TYPES, acts)
+    assert len(exp[1][1]) > 250
+    assert_results_equal(got, exp, RTOL)
+
+
+@pytest.mark.parametrize("mode", ["batched", "one"])
+def test_int_then_utf8_keys(ctx, mode):
+    """(int32, utf8) keys: the utf8 key last in the UnsafeRow, ~1500 groups,
+    strings up to 300 bytes, the byte heap growing across calls."""
+    batches = _utf8_batches([4096] * 9 + [77], seed=22)
+    acts = [("action_groupby", [0]), ("action_groupby", [1]), ("action_sum", [2]), ("action_countLiteral_1", []),
+            ("action_max", [2])]
+    got = run_gpu(ctx, batches, UTF8_TYPES, acts, mode)
+    exp = expected(batches, UTF8_TYPES, acts)
+    assert len(exp[0][1]) > 1000
+    _assert_keys_bits(got, exp, [0])
+    assert_results_equal(got, exp, RTOL)

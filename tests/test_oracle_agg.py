@@ -18,6 +18,8 @@ def _same(a, b, tol):
         return a is None
     if isinstance(b, str) and b == "NaN":
         return a is not None and math.isnan(a)
+    if isinstance(b, str):
+        return a == b
     if a is None or (isinstance(a, float) and math.isnan(a)):
         return False
     return abs(a - b) <= tol * max(1.0, abs(b))
@@ -25,10 +27,12 @@ def _same(a, b, tol):
 
 def _column(res, i):
     t, v, ok = res[i]
+    if t == T.STRING:
+        return [v[j].decode() if ok[j] else None for j in range(len(v))]
     return [v[j].item() if ok[j] else None for j in range(len(v))]
 
 
-@pytest.mark.parametrize("case", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("case", [0, 1, 2, 3, 4, 5])
 def test_reference_aggregate_fixtures(case):
     fx = F.load("agg_fixtures.json")["tests"][case]
     schema = fx["schema"]
@@ -165,3 +169,61 @@ def test_multi_key_unsafe_row_equality_cases():
     assert _column(res, 1) == [5, None, 0, 0, 0, 0]
     bits = res[2][1].view(np.uint64).tolist()
     assert bits == [0, 0, 0, 0x8000000000000000, 0x7FF8000000000000, 0x7FF8000000000001]
+
+
+def test_utf8_key_brute_force():
+    """One utf8 key: groups by the bytes in first-seen order, the empty string
+    and the null key each a group of their own (brute force over Python bytes);
+    and (int32, utf8) keys against a brute force over (int or None, bytes or
+    None) tuples."""
+    from gazelle_plugin_amd.batch import HostBatch, HostColumn, bitmap_from_bools, bools_from_bitmap
+    rng = np.random.default_rng(4)
+    pool = [b"", b"a", b"ab", b"b", b"\x00", b"\x00\x00", b"abc" * 40]
+    batches = []
+    for n in (500, 0, 300):
+        enc = [pool[i] for i in rng.integers(0, len(pool), n)]
+        off = np.zeros(n + 1, np.int32)
+        off[1:] = np.cumsum([len(e) for e in enc])
+        data = np.frombuffer(b"".join(enc), np.uint8).copy() if off[-1] else np.zeros(0, np.uint8)
+        batches.append(HostBatch(n, [HostColumn(T.INT32, rng.integers(0, 3, n).astype(np.int32),
+                                                bitmap_from_bools(rng.random(n) > 0.2)),
+                                     HostColumn(T.STRING, data, bitmap_from_bools(rng.random(n) > 0.2), off)]))
+    rows = []
+    for b in batches:
+        kv = bools_from_bitmap(b.columns[0].validity, b.num_rows)
+        sv = bools_from_bitmap(b.columns[1].validity, b.num_rows)
+        o, d = b.columns[1].offsets, b.columns[1].values
+        for r in range(b.num_rows):
+            rows.append((int(b.columns[0].values[r]) if kv[r] else None,
+                         d[o[r]:o[r + 1]].tobytes() if sv[r] else None))
+    for keys, acts in (([1], [("action_groupby", [1]), ("action_countLiteral_1", [])]),
+                       ([0, 1], [("action_groupby", [0]), ("action_groupby", [1]), ("action_countLiteral_1", [])])):
+        groups = {}
+        for row in rows:
+            k = tuple(row[c] for c in keys)
+            groups[k] = groups.get(k, 0) + 1
+        res = oracle.aggregate(batches, [T.INT32, T.STRING], [parse_action(n, i) for n, i in acts])
+        cols = [[res[j][1][i] if res[j][2][i] else None for i in range(len(res[-1][1]))] for j in range(len(keys))]
+        got = list(zip(*cols))
+        got = [tuple(x if not isinstance(x, np.integer) else int(x) for x in k) for k in got]
+        assert got == list(groups), keys
+        assert res[-1][1].tolist() == list(groups.values())
+
+
+def test_unsafe_row_string_bytes_carry_no_length():
+    """The reference appends a utf8 key's bytes to the UnsafeRow without their
+    length (unsafe_row.h:126-137), so ("ab", "c") and ("a", "bc") are one group
+    there; the oracle keeps that (the device rejects a utf8 key before another
+    key, where this matters)."""
+    from gazelle_plugin_amd.batch import HostBatch, HostColumn
+
+    def col(strs):
+        enc = [s.encode() for s in strs]
+        off = np.zeros(len(enc) + 1, np.int32)
+        off[1:] = np.cumsum([len(e) for e in enc])
+        return HostColumn(T.STRING, np.frombuffer(b"".join(enc), np.uint8).copy(), None, off)
+
+    b = HostBatch(3, [col(["ab", "a", "a"]), col(["c", "bc", "c"])])
+    acts = [("action_groupby", [0]), ("action_groupby", [1]), ("action_countLiteral_1", [])]
+    res = oracle.aggregate([b], [T.STRING, T.STRING], [parse_action(n, i) for n, i in acts])
+    assert res[2][1].tolist() == [2, 1]
