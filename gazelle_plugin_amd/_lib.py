@@ -174,6 +174,7 @@ _OPTIONAL_SIGS = {
     "gz_server_release_results": (ctypes.c_int, [_P]),
     "gz_server_destroy": (ctypes.c_int, [_P]),
     "gz_relation_make": (ctypes.c_int, [_P, _I32, _PI32, _I32, _I32, _PP]),
+    "gz_relation_make_keys": (ctypes.c_int, [_P, _I32, _PI32, _I32, _PI32, _I32, _PP]),
     "gz_relation_append": (ctypes.c_int, [_P, _I64, _PI64, _PI64, _I32]),
     "gz_relation_append_batches": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch)]),
     "gz_relation_finish": (ctypes.c_int, [_P]),
@@ -181,6 +182,7 @@ _OPTIONAL_SIGS = {
     "gz_relation_has_null_key": (ctypes.c_int, [_P, _PI32]),
     "gz_relation_release": (ctypes.c_int, [_P]),
     "gz_probe_make": (ctypes.c_int, [_P, _I32, _PI32, _I32, _I32, _I32, _I32, _PI32, _PI32, _PP]),
+    "gz_probe_make_keys": (ctypes.c_int, [_P, _I32, _PI32, _I32, _PI32, _I32, _I32, _I32, _PI32, _PI32, _PP]),
     "gz_probe_process": (ctypes.c_int, [_P, _I64, _PI64, _PI64, _I32, _PI64]),
     "gz_probe_process_batches": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch), _PI64, _PI64]),
     "gz_probe_result_column": (ctypes.c_int, [_P, _I32, _PP, _PP, _PP, _PP]),

@@ -161,6 +161,25 @@ int relation_view(gz_relation* r, RelationView* v);
 // Hash aggregate grouped by several key columns or a utf8 key (gz_keys.hip): a device key
 // dictionary maps every row's key tuple to a dense code, and a single-key
 // aggregate groups by the code.
+struct KeyDict;
+// Per-call scratch of a dictionary's user (its codes: one int64 per row).
+struct KeyScratch {
+  DBuf rowslot, d_kb, d_prefix;
+};
+int kd_make(gz_context* ctx, int32_t K, const int32_t* key_types, KeyDict** out);
+void kd_release(KeyDict* d);
+void kd_scratch_free(gz_context* ctx, KeyScratch* sc);
+// Codes of the batches' key tuples (key_cols of a ncols/types schema), on
+// ctx's stream: mode 0 inserts every tuple (a null key is part of the tuple),
+// 1 inserts tuples without nulls (a null key: code -1), 2 only looks tuples up
+// (a miss or a null key: code -2).  *codes (device, one per row of the
+// batches in order) stays valid until the scratch's next use.
+int kd_encode(KeyDict* d, gz_context* ctx, KeyScratch* sc, int32_t ncols, const int32_t* types,
+              const int32_t* key_cols, int32_t nb, const gz_batch* batches, int mode, const int64_t** codes,
+              int64_t* rows_out);
+// The batches' buffer lists with the int64 code column appended.
+void with_code_column(int32_t ncols, const int32_t* types, int32_t nb, const gz_batch* batches, const int64_t* codes,
+                      std::vector<int64_t>& addrs, std::vector<int64_t>& sizes, std::vector<gz_batch>& out);
 struct MultiKey;
 int mk_make(gz_context* ctx, int32_t ncols, const int32_t* types, int32_t nact, const gz_agg_action* acts,
             const gz_agg_options* opts, MultiKey** out);

@@ -768,6 +768,12 @@ struct gz_relation {
   uint64_t cap = 0;
   bool finished = false, has_null = false, has_dups = false;
   int64_t nkeys = 0, special_start = 0, special_count = 0;
+  // several key columns (or a utf8 key): rows are keyed by a dictionary code
+  // appended as the last column
+  KeyDict* kd = nullptr;
+  std::vector<int> kcols, otypes;
+  int ocols = 0;
+  KeyScratch ksc;
 };
 
 static int rel_grow(gz_relation* r, int64_t need) {
@@ -847,6 +853,7 @@ static int rel_append_one(gz_relation* r, const gz_batch& bt) {
         cs.data = nd;
         cs.data_cap = ncap;
       }
+      if (nb > 0 && !a.values) return fail(GZ_INVALID, "missing values buffer");
       if (nb > 0)
         GZ_HIP(hipMemcpyAsync((uint8_t*)cs.data.p + cs.data_bytes, (const uint8_t*)a.values + ends[0], (size_t)nb,
                               hipMemcpyDeviceToDevice, ctx->stream));
@@ -861,7 +868,8 @@ static int rel_append_one(gz_relation* r, const gz_batch& bt) {
     }
     k++;
     a.dvalid = (uint8_t*)cs.valid.p;
-    if (cs.type != GZ_TYPE_NA && !a.values) return fail(GZ_INVALID, "missing values buffer");
+    // (utf8 bytes may be absent when every string is empty)
+    if (cs.type != GZ_TYPE_NA && !type_is_binary(cs.type) && !a.values) return fail(GZ_INVALID, "missing values buffer");
   }
   unsigned blocks = (unsigned)std::min<int64_t>((bt.num_rows + 255) / 256, 2048);
   {
@@ -903,16 +911,32 @@ int gz_relation_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types
   return GZ_OK;
 }
 
+int gz_relation_append_batches(gz_relation* r, int32_t num_batches, const gz_batch* batches);
+
 int gz_relation_append(gz_relation* r, int64_t num_rows, const int64_t* buf_addrs,
                        const int64_t* buf_sizes, int32_t num_bufs) {
   if (!r) return fail(GZ_INVALID, "null relation");
   if (!buf_addrs || !buf_sizes) return fail(GZ_INVALID, "buf_addrs can't be null");
   gz_batch b{num_rows, buf_addrs, buf_sizes, num_bufs};
+  if (r->kd) return gz_relation_append_batches(r, 1, &b);
   return rel_append_one(r, b);
 }
 
 int gz_relation_append_batches(gz_relation* r, int32_t num_batches, const gz_batch* batches) {
   if (!r) return fail(GZ_INVALID, "null relation");
+  std::vector<int64_t> kaddrs, ksizes;
+  std::vector<gz_batch> kbatches;
+  if (r->kd) {
+    // AppendKeyColumn with UnsafeRow keys (hash_relation.h:176-198): a row with
+    // a null key is never inserted, so it gets a code no probe row carries
+    if (r->finished) return fail(GZ_INVALID, "relation already finished");
+    const int64_t* codes = nullptr;
+    int64_t rows = 0;
+    GZ_TRY(kd_encode(r->kd, r->ctx, &r->ksc, r->ocols, r->otypes.data(), r->kcols.data(), num_batches, batches, 1,
+                     &codes, &rows));
+    with_code_column(r->ocols, r->otypes.data(), num_batches, batches, codes, kaddrs, ksizes, kbatches);
+    batches = kbatches.data();
+  }
   // reserve once so the column store grows a single time
   int64_t total = r->n;
   for (int i = 0; i < num_batches; i++) total += batches[i].num_rows;
@@ -1027,6 +1051,8 @@ int gz_relation_has_null_key(gz_relation* r, int32_t* out) {
 int gz_relation_release(gz_relation* r) {
   if (!r) return GZ_OK;
   gz_context* ctx = r->ctx;
+  kd_release(r->kd);
+  kd_scratch_free(ctx, &r->ksc);
   for (auto& c : r->cols)
     for (DBuf* b : {&c.valid, &c.values, &c.offsets, &c.data}) ctx_free(ctx, b);
   for (DBuf* b : {&r->keys, &r->start, &r->count, &r->rowlist, &r->row_slot, &r->flags, &r->sort_a,
@@ -1058,7 +1084,23 @@ struct gz_probe {
   };
   std::vector<Out> outs;
   int64_t out_rows = 0;
+  // several key columns: probe rows are looked up in the relation's dictionary
+  KeyDict* kd = nullptr;
+  std::vector<int> kcols, otypes;
+  int ocols = 0;
+  KeyScratch ksc;
 };
+
+// Probe batches of a multi-key probe with their code column appended.
+static int probe_codes(gz_probe* h, int32_t nb, const gz_batch* batches, std::vector<int64_t>& addrs,
+                       std::vector<int64_t>& sizes, std::vector<gz_batch>& out) {
+  const int64_t* codes = nullptr;
+  int64_t rows = 0;
+  GZ_TRY(kd_encode(h->kd, h->ctx, &h->ksc, h->ocols, h->otypes.data(), h->kcols.data(), nb, batches, 2, &codes,
+                   &rows));
+  with_code_column(h->ocols, h->otypes.data(), nb, batches, codes, addrs, sizes, out);
+  return GZ_OK;
+}
 
 static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* out_rows,
                      int64_t* out_batch_rows) {
@@ -1091,7 +1133,8 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
       if (t != GZ_TYPE_NA && !type_is_binary(t) && bt.buf_sizes[k] < need_b)
         return fail(GZ_INVALID, "values buffer too small");
       slot[2] = (const void*)bt.buf_addrs[k++];
-      if (h->ptypes[c] != GZ_TYPE_NA && bt.num_rows > 0 && !slot[2])
+      // utf8 bytes may be absent when every string is empty
+      if (h->ptypes[c] != GZ_TYPE_NA && bt.num_rows > 0 && !slot[2] && !(type_is_binary(t) && bt.buf_sizes[k - 1] <= 0))
         return fail(GZ_INVALID, "missing values buffer");
     }
     for (int64_t r0 = 0; r0 < bt.num_rows; r0 += TILE) {
@@ -1318,6 +1361,7 @@ namespace gz {
 int relation_view(gz_relation* r, RelationView* v) {
   if (!r || !r->finished) return fail(GZ_INVALID, "relation not finished");
   if (r->ncols > 32) return fail(GZ_NOT_IMPLEMENTED, "relation with more than 32 columns");
+  if (r->kd) return fail(GZ_NOT_IMPLEMENTED, "the fused stage joins on one key column");
   memset(v, 0, sizeof(*v));
   v->ctx = r->ctx;
   v->keys = (const int64_t*)r->keys.p;
@@ -1389,6 +1433,12 @@ int gz_probe_process(gz_probe* h, int64_t num_rows, const int64_t* buf_addrs, co
   if (!h) return fail(GZ_INVALID, "null probe");
   if (!buf_addrs || !buf_sizes) return fail(GZ_INVALID, "buf_addrs can't be null");
   gz_batch b{num_rows, buf_addrs, buf_sizes, num_bufs};
+  if (h->kd) {
+    std::vector<int64_t> a, s;
+    std::vector<gz_batch> kb;
+    GZ_TRY(probe_codes(h, 1, &b, a, s, kb));
+    return probe_run(h, 1, kb.data(), out_rows, nullptr);
+  }
   return probe_run(h, 1, &b, out_rows, nullptr);
 }
 
@@ -1396,6 +1446,12 @@ int gz_probe_process_batches(gz_probe* h, int32_t num_batches, const gz_batch* b
                              int64_t* out_batch_rows) {
   if (!h) return fail(GZ_INVALID, "null probe");
   if (num_batches < 0 || (num_batches > 0 && !batches)) return fail(GZ_INVALID, "bad batch list");
+  if (h->kd) {
+    std::vector<int64_t> a, s;
+    std::vector<gz_batch> kb;
+    GZ_TRY(probe_codes(h, num_batches, batches, a, s, kb));
+    return probe_run(h, num_batches, kb.data(), out_rows, out_batch_rows);
+  }
   return probe_run(h, num_batches, batches, out_rows, out_batch_rows);
 }
 
@@ -1414,12 +1470,77 @@ int gz_probe_result_column(gz_probe* h, int32_t col, void** values, void** valid
 int gz_probe_release(gz_probe* h) {
   if (!h) return GZ_OK;
   gz_context* ctx = h->ctx;
+  kd_scratch_free(ctx, &h->ksc);
   for (auto& o : h->outs)
     for (DBuf* b : {&o.values, &o.valid, &o.offsets, &o.data, &o.lens}) ctx_free(ctx, b);
   for (DBuf* b : {&h->d_tiles, &h->d_bufs, &h->match, &h->tile_count, &h->tile_off, &h->scan_tmp,
                   &h->pair_probe, &h->pair_build, &h->payload})
     ctx_free(ctx, b);
   delete h;
+  return GZ_OK;
+}
+
+int gz_relation_make_keys(gz_context* ctx, int32_t num_cols, const int32_t* col_types, int32_t num_keys,
+                          const int32_t* key_cols, int32_t builder_type, gz_relation** out) {
+  if (!ctx || !out || (num_cols > 0 && !col_types) || (num_keys > 0 && !key_cols))
+    return fail(GZ_INVALID, "gz_relation_make_keys: null argument");
+  if (num_keys < 1) return fail(GZ_INVALID, "no key column");
+  for (int j = 0; j < num_keys; j++)
+    if (key_cols[j] < 0 || key_cols[j] >= num_cols) return fail(GZ_INVALID, "key column out of range");
+  if (num_keys == 1 && col_types[key_cols[0]] != GZ_TYPE_STRING)
+    return gz_relation_make(ctx, num_cols, col_types, key_cols[0], builder_type, out);
+  if (num_cols + 1 > kJoinMaxCols) return fail(GZ_NOT_IMPLEMENTED, "more than 31 build columns with several keys");
+  std::vector<int32_t> kt, ext(col_types, col_types + num_cols);
+  for (int j = 0; j < num_keys; j++) kt.push_back(col_types[key_cols[j]]);
+  ext.push_back(GZ_TYPE_INT64);
+  KeyDict* kd = nullptr;
+  GZ_TRY(kd_make(ctx, num_keys, kt.data(), &kd));
+  gz_relation* r = nullptr;
+  int rc = gz_relation_make(ctx, num_cols + 1, ext.data(), num_cols, builder_type, &r);
+  if (rc) {
+    kd_release(kd);
+    return rc;
+  }
+  r->kd = kd;
+  r->kcols.assign(key_cols, key_cols + num_keys);
+  r->otypes.assign(col_types, col_types + num_cols);
+  r->ocols = num_cols;
+  *out = r;
+  return GZ_OK;
+}
+
+int gz_probe_make_keys(gz_relation* r, int32_t num_probe_cols, const int32_t* probe_types, int32_t num_keys,
+                       const int32_t* probe_key_cols, int32_t join_type, int32_t null_aware_anti,
+                       int32_t num_result_cols, const int32_t* result_sides, const int32_t* result_indices,
+                       gz_probe** out) {
+  if (!r || !out || (num_probe_cols > 0 && !probe_types) || (num_keys > 0 && !probe_key_cols))
+    return fail(GZ_INVALID, "gz_probe_make_keys: null argument");
+  if (!r->kd) {
+    if (num_keys != 1) return fail(GZ_INVALID, "the relation was built with one key column");
+    return gz_probe_make(r, num_probe_cols, probe_types, probe_key_cols[0], join_type, null_aware_anti,
+                         num_result_cols, result_sides, result_indices, out);
+  }
+  if (num_keys != (int)r->kcols.size()) return fail(GZ_INVALID, "probe and build key counts differ");
+  for (int j = 0; j < num_keys; j++) {
+    if (probe_key_cols[j] < 0 || probe_key_cols[j] >= num_probe_cols) return fail(GZ_INVALID, "probe key out of range");
+    if (probe_types[probe_key_cols[j]] != r->otypes[r->kcols[j]])
+      return fail(GZ_TYPE_ERROR, "probe and build key types differ");
+  }
+  if (null_aware_anti) return fail(GZ_NOT_IMPLEMENTED, "null-aware anti join with several keys");
+  for (int c = 0; c < num_result_cols; c++)
+    if (result_sides && result_indices && result_sides[c] == 0 && result_indices[c] >= r->ocols)
+      return fail(GZ_INVALID, "result column out of range");
+  if (num_probe_cols + 1 > kJoinMaxCols) return fail(GZ_NOT_IMPLEMENTED, "more than 31 probe columns with several keys");
+  std::vector<int32_t> ext(probe_types, probe_types + num_probe_cols);
+  ext.push_back(GZ_TYPE_INT64);
+  gz_probe* h = nullptr;
+  GZ_TRY(gz_probe_make(r, num_probe_cols + 1, ext.data(), num_probe_cols, join_type, 0, num_result_cols,
+                       result_sides, result_indices, &h));
+  h->kd = r->kd;
+  h->kcols.assign(probe_key_cols, probe_key_cols + num_keys);
+  h->otypes.assign(probe_types, probe_types + num_probe_cols);
+  h->ocols = num_probe_cols;
+  *out = h;
   return GZ_OK;
 }
 

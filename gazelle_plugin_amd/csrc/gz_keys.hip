@@ -52,6 +52,7 @@ struct KeyParams {
   int K;
   int width[kMaxKeys];  // bytes per value: 1, 2, 4 or 8
   int sj;               // index of the utf8 key (the last one), -1 if none
+  int mode;             // 0 insert every tuple, 1 insert (a null key: code -1), 2 look up (miss or null: -2)
   int64_t rows;         // rows of the call
   unsigned long long* table;
   uint64_t mask;  // table capacity - 1
@@ -180,6 +181,10 @@ __global__ __launch_bounds__(256) void key_find_kernel(KeyParams p) {
   uint64_t raw[kMaxKeys];
   StrRef str;
   const uint32_t m = load_key(p, g, raw, &str);
+  if (p.mode != 0 && m != 0) {  // join keys: a row with a null key never matches
+    p.rowslot[g] = p.mode == 1 ? -1 : -2;
+    return;
+  }
   const uint64_t h = key_hash(raw, m, p.K);
   const uint64_t fp = (h >> 32) | 1u;  // never zero: an occupied slot is never 0
   const unsigned long long mine = (fp << 32) | (uint64_t)g;
@@ -187,6 +192,10 @@ __global__ __launch_bounds__(256) void key_find_kernel(KeyParams p) {
   // the table is at most half full, so the walk ends
   while (true) {
     unsigned long long w = __hip_atomic_load(&p.table[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (w == 0 && p.mode == 2) {  // look-up: an empty slot ends the walk
+      p.rowslot[g] = -2;
+      return;
+    }
     if (w == 0) {
       const unsigned long long old = atomicCAS(&p.table[s], 0ull, mine);
       if (old == 0) break;
@@ -202,6 +211,7 @@ __global__ __launch_bounds__(256) void key_assign_kernel(KeyParams p) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= p.rows) return;
   const int64_t s = p.rowslot[g];
+  if (s < 0) return;
   const unsigned long long w = p.table[s];
   // only the row that claimed the slot holds its own reference (a code has
   // the top bit set, call rows are below 2^31)
@@ -226,7 +236,8 @@ __global__ __launch_bounds__(256) void key_assign_kernel(KeyParams p) {
 __global__ __launch_bounds__(256) void key_code_kernel(const unsigned long long* table, int64_t* rowslot, int64_t rows) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= rows) return;
-  rowslot[g] = (int64_t)(table[rowslot[g]] & 0x7fffffffull);
+  const int64_t s = rowslot[g];
+  if (s >= 0) rowslot[g] = (int64_t)(table[s] & 0x7fffffffull);
 }
 
 // Re-inserts codes [0, n) into a larger empty table.
@@ -299,24 +310,29 @@ __global__ void fill_u64_kernel_keys(unsigned long long* p, int64_t n) {
 
 // ------------------------------------------------------------------ host side
 
+// The key dictionary: slot table, per-code key tuples, utf8 key heap.
+struct KeyDict {
+  gz_context* ctx = nullptr;
+  int K = 0;
+  std::vector<int> key_types;
+  int sj = -1;  // the utf8 key (last), -1 if none
+  DBuf table, kvals, kmask, counter;  // counter: [0] codes, [1] heap bytes used
+  uint64_t tcap = 0;
+  int64_t kcap = 0, ncodes = 0;
+  DBuf heap, koff, klen;
+  int64_t heap_cap = 0, heap_used = 0;
+};
+
 struct MultiKey {
   gz_context* ctx = nullptr;
   gz_agg* inner = nullptr;  // single-key aggregate over (columns..., code)
   int ncols = 0;
   std::vector<int> types;
-  int K = 0;
-  std::vector<int> key_cols, key_types;
+  std::vector<int> key_cols;
   std::vector<int> res_map;  // outer result -> inner result (>= 0) or key -(j + 1)
   int code_res = 0;          // inner result holding the code
-  int sj = -1;               // the utf8 key (last), -1 if none
-  // dictionary
-  DBuf table, kvals, kmask, counter;
-  uint64_t tcap = 0;
-  int64_t kcap = 0, ncodes = 0;
-  DBuf heap, koff, klen;  // utf8 key: strings of the codes
-  int64_t heap_cap = 0, heap_used = 0;
-  // per call
-  DBuf rowslot, d_kb, d_prefix;
+  KeyDict* kd = nullptr;
+  KeyScratch sc;
   // finish
   bool finished = false;
   int64_t ngroups = 0;
@@ -335,6 +351,47 @@ static bool key_type_ok(int t) {
   }
 }
 
+int kd_make(gz_context* ctx, int32_t K, const int32_t* key_types, KeyDict** out) {
+  if (K < 1 || K > kMaxKeys) return fail(GZ_NOT_IMPLEMENTED, "1 to 8 key columns");
+  auto* d = new KeyDict();
+  d->ctx = ctx;
+  d->K = K;
+  for (int j = 0; j < K; j++) {
+    const int t = key_types[j];
+    if (d->sj >= 0) {
+      delete d;
+      return fail(GZ_NOT_IMPLEMENTED, "a key column after a utf8 key");
+    }
+    if (t == GZ_TYPE_STRING) d->sj = j;
+    else if (!key_type_ok(t)) {
+      delete d;
+      return fail(GZ_NOT_IMPLEMENTED, "key type " + std::to_string(t) +
+                                          " (fixed-width numeric keys and one utf8 key, last)");
+    }
+    d->key_types.push_back(t);
+  }
+  int rc = ctx_alloc(ctx, 2 * sizeof(unsigned long long), &d->counter);
+  if (!rc && hipMemsetAsync(d->counter.p, 0, 2 * sizeof(unsigned long long), ctx->stream) != hipSuccess)
+    rc = fail(GZ_HIP_ERROR, "clearing the key counter");
+  if (rc) {
+    ctx_free(ctx, &d->counter);
+    delete d;
+    return rc;
+  }
+  *out = d;
+  return GZ_OK;
+}
+
+void kd_release(KeyDict* d) {
+  if (!d) return;
+  for (DBuf* b : {&d->table, &d->kvals, &d->kmask, &d->counter, &d->heap, &d->koff, &d->klen}) ctx_free(d->ctx, b);
+  delete d;
+}
+
+void kd_scratch_free(gz_context* ctx, KeyScratch* sc) {
+  for (DBuf* b : {&sc->rowslot, &sc->d_kb, &sc->d_prefix}) ctx_free(ctx, b);
+}
+
 int mk_make(gz_context* ctx, int32_t ncols, const int32_t* types, int32_t nact, const gz_agg_action* acts,
             const gz_agg_options* opts, MultiKey** out) {
   auto* m = new MultiKey();
@@ -351,41 +408,31 @@ int mk_make(gz_context* ctx, int32_t ncols, const int32_t* types, int32_t nact, 
     const gz_agg_action& a = acts[i];
     if (a.kind == GZ_ACTION_GROUPBY) {
       if (a.input0 < 0 || a.input0 >= ncols) return bad(GZ_INVALID, "groupby input out of range");
-      const int t = types[a.input0];
-      if (m->sj >= 0) return bad(GZ_NOT_IMPLEMENTED, "a group-by key after a utf8 key");
-      if (t == GZ_TYPE_STRING) m->sj = m->K;
-      else if (!key_type_ok(t))
-        return bad(GZ_NOT_IMPLEMENTED, "group-by key type " + std::to_string(t) +
-                                           " (fixed-width numeric keys and one utf8 key, last)");
-      if (m->K == kMaxKeys) return bad(GZ_NOT_IMPLEMENTED, "more than 8 group-by keys");
-      m->res_map.push_back(-(m->K + 1));
-      if (m->K == 0) {
+      if ((int)m->key_cols.size() == kMaxKeys) return bad(GZ_NOT_IMPLEMENTED, "more than 8 group-by keys");
+      m->res_map.push_back(-((int)m->key_cols.size() + 1));
+      if (m->key_cols.empty()) {
         inner.push_back({GZ_ACTION_GROUPBY, ncols, -1, 0});
         m->code_res = inner_pos++;
       }
       m->key_cols.push_back(a.input0);
-      m->key_types.push_back(t);
-      m->K++;
     } else {
       inner.push_back(a);
       const int nres = a.kind == GZ_ACTION_SUM_COUNT ? 2 : 1;
       for (int r = 0; r < nres; r++) m->res_map.push_back(inner_pos++);
     }
   }
-  std::vector<int32_t> itypes(types, types + ncols);
-  itypes.push_back(GZ_TYPE_INT64);
-  int rc = gz_agg_make(ctx, ncols + 1, itypes.data(), (int32_t)inner.size(), inner.data(), opts, &m->inner);
+  std::vector<int32_t> ktypes;
+  for (int c : m->key_cols) ktypes.push_back(types[c]);
+  int rc = kd_make(ctx, (int32_t)ktypes.size(), ktypes.data(), &m->kd);
   if (rc) {
     delete m;
     return rc;
   }
-  rc = ctx_alloc(ctx, 2 * sizeof(unsigned long long), &m->counter);
-  if (!rc) rc = hipMemsetAsync(m->counter.p, 0, 2 * sizeof(unsigned long long), ctx->stream) == hipSuccess
-                    ? GZ_OK
-                    : fail(GZ_HIP_ERROR, "clearing the key counter");
+  std::vector<int32_t> itypes(types, types + ncols);
+  itypes.push_back(GZ_TYPE_INT64);
+  rc = gz_agg_make(ctx, ncols + 1, itypes.data(), (int32_t)inner.size(), inner.data(), opts, &m->inner);
   if (rc) {
-    gz_agg_release(m->inner);
-    ctx_free(ctx, &m->counter);
+    kd_release(m->kd);
     delete m;
     return rc;
   }
@@ -406,63 +453,78 @@ static int grow_keep(gz_context* ctx, DBuf* b, size_t bytes, size_t keep) {
   return GZ_OK;
 }
 
-// Grows the code store to hold `need` codes and the table to at least twice
-// that, re-inserting the codes already assigned.
-static int mk_reserve(MultiKey* m, int64_t need, int64_t heap_need) {
-  gz_context* ctx = m->ctx;
-  if (need > m->kcap) {
-    const int64_t cap = std::max<int64_t>({need, 2 * m->kcap, 1024});
-    const size_t n = (size_t)m->ncodes;
-    GZ_TRY(grow_keep(ctx, &m->kvals, sizeof(unsigned long long) * (size_t)cap * m->K,
-                     sizeof(unsigned long long) * n * m->K));
-    GZ_TRY(grow_keep(ctx, &m->kmask, sizeof(uint32_t) * (size_t)cap, sizeof(uint32_t) * n));
-    if (m->sj >= 0) {
-      GZ_TRY(grow_keep(ctx, &m->koff, sizeof(int64_t) * (size_t)cap, sizeof(int64_t) * n));
-      GZ_TRY(grow_keep(ctx, &m->klen, sizeof(int32_t) * (size_t)cap, sizeof(int32_t) * n));
+// Grows the code store to hold `need` codes, the utf8 heap to `heap_need`
+// bytes, and the table to at least twice `need` slots, re-inserting the codes
+// already assigned.
+static int kd_reserve(KeyDict* d, int64_t need, int64_t heap_need) {
+  gz_context* ctx = d->ctx;
+  if (need > d->kcap) {
+    const int64_t cap = std::max<int64_t>({need, 2 * d->kcap, 1024});
+    const size_t n = (size_t)d->ncodes;
+    GZ_TRY(grow_keep(ctx, &d->kvals, sizeof(unsigned long long) * (size_t)cap * d->K,
+                     sizeof(unsigned long long) * n * d->K));
+    GZ_TRY(grow_keep(ctx, &d->kmask, sizeof(uint32_t) * (size_t)cap, sizeof(uint32_t) * n));
+    if (d->sj >= 0) {
+      GZ_TRY(grow_keep(ctx, &d->koff, sizeof(int64_t) * (size_t)cap, sizeof(int64_t) * n));
+      GZ_TRY(grow_keep(ctx, &d->klen, sizeof(int32_t) * (size_t)cap, sizeof(int32_t) * n));
     }
-    m->kcap = cap;
+    d->kcap = cap;
   }
-  if (m->sj >= 0 && (heap_need > m->heap_cap || !m->heap.p)) {
-    const int64_t cap = std::max<int64_t>({heap_need, 2 * m->heap_cap, 4096});
-    GZ_TRY(grow_keep(ctx, &m->heap, (size_t)cap, (size_t)m->heap_used));
-    m->heap_cap = cap;
+  if (d->sj >= 0 && (heap_need > d->heap_cap || !d->heap.p)) {
+    const int64_t cap = std::max<int64_t>({heap_need, 2 * d->heap_cap, 4096});
+    GZ_TRY(grow_keep(ctx, &d->heap, (size_t)cap, (size_t)d->heap_used));
+    d->heap_cap = cap;
   }
   uint64_t tcap = 1024;
   while (tcap < 2 * (uint64_t)need) tcap <<= 1;
-  if (tcap > m->tcap) {
+  if (tcap > d->tcap) {
     DBuf nt;
     GZ_TRY(ctx_alloc(ctx, sizeof(unsigned long long) * tcap, &nt));
     const unsigned fill_blocks = (unsigned)std::min<uint64_t>((tcap + 255) / 256, 65536);
     hipLaunchKernelGGL(fill_u64_kernel_keys, dim3(fill_blocks), dim3(256), 0, ctx->stream,
                        (unsigned long long*)nt.p, (int64_t)tcap);
-    if (m->ncodes > 0) {
+    if (d->ncodes > 0) {
       KernelTimer kt(ctx, "key_rehash");
-      hipLaunchKernelGGL(key_rehash_kernel, dim3((unsigned)((m->ncodes + 255) / 256)), dim3(256), 0, ctx->stream,
-                         (unsigned long long*)nt.p, tcap - 1, (const unsigned long long*)m->kvals.p,
-                         (const uint32_t*)m->kmask.p, m->K, m->ncodes);
+      hipLaunchKernelGGL(key_rehash_kernel, dim3((unsigned)((d->ncodes + 255) / 256)), dim3(256), 0, ctx->stream,
+                         (unsigned long long*)nt.p, tcap - 1, (const unsigned long long*)d->kvals.p,
+                         (const uint32_t*)d->kmask.p, d->K, d->ncodes);
     }
     GZ_HIP(hipGetLastError());
     GZ_HIP(hipStreamSynchronize(ctx->stream));
-    ctx_free(ctx, &m->table);
-    m->table = nt;
-    m->tcap = tcap;
+    ctx_free(ctx, &d->table);
+    d->table = nt;
+    d->tcap = tcap;
   }
   return GZ_OK;
 }
 
-int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
-  gz_context* ctx = m->ctx;
-  if (m->finished) return fail(GZ_INVALID, "aggregate already finished");
-  GZ_HIP(hipSetDevice(ctx->device));
+int kd_encode(KeyDict* d, gz_context* ctx, KeyScratch* sc, int32_t ncols, const int32_t* types,
+              const int32_t* key_cols, int32_t nb, const gz_batch* batches, int mode, const int64_t** codes,
+              int64_t* rows_out) {
+  *rows_out = 0;
+  if (mode == 2 && !d->table.p) {
+    // nothing was inserted: every probe row misses
+    int64_t rows = 0;
+    for (int b = 0; b < nb; b++) rows += std::max<int64_t>(batches[b].num_rows, 0);
+    if (rows == 0) return GZ_OK;
+    GZ_TRY(ctx_ensure(ctx, &sc->rowslot, sizeof(int64_t) * (size_t)rows));
+    std::vector<int64_t> miss((size_t)rows, -2);
+    GZ_HIP(hipMemcpyAsync(sc->rowslot.p, miss.data(), sizeof(int64_t) * (size_t)rows, hipMemcpyHostToDevice,
+                          ctx->stream));
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+    *codes = (const int64_t*)sc->rowslot.p;
+    *rows_out = rows;
+    return GZ_OK;
+  }
   int need_bufs = 0;
-  std::vector<int> col_buf(m->ncols);
-  for (int c = 0; c < m->ncols; c++) {
+  std::vector<int> col_buf(ncols);
+  for (int c = 0; c < ncols; c++) {
     col_buf[c] = need_bufs;
-    need_bufs += type_num_bufs(m->types[c]);
+    need_bufs += type_num_bufs(types[c]);
   }
   std::vector<KeyBatchPtrs> kb(std::max(nb, 1));
   std::vector<int64_t> prefix(nb + 1, 0);
-  int64_t heap_need = m->heap_used;
+  int64_t heap_need = d->heap_used;
   for (int b = 0; b < nb; b++) {
     const gz_batch& bt = batches[b];
     if (bt.num_rows < 0 || bt.num_rows > INT32_MAX) return fail(GZ_INVALID, "bad num_rows");
@@ -470,12 +532,12 @@ int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
     if (bt.num_rows > 0 && bt.num_bufs < need_bufs) return fail(GZ_INVALID, "insufficient number of in_buf_addrs");
     memset(&kb[b], 0, sizeof(KeyBatchPtrs));
     if (bt.num_rows > 0) {
-      for (int j = 0; j < m->K; j++) {
-        const int k = col_buf[m->key_cols[j]];
+      for (int j = 0; j < d->K; j++) {
+        const int k = col_buf[key_cols[j]];
         const bool has_valid = bt.buf_addrs[k] != 0 && bt.buf_sizes[k] > 0;
         if (has_valid && bt.buf_sizes[k] < (bt.num_rows + 7) / 8) return fail(GZ_INVALID, "validity buffer too small");
         kb[b].valid[j] = has_valid ? (const uint8_t*)bt.buf_addrs[k] : nullptr;
-        if (j == m->sj) {
+        if (j == d->sj) {
           if (!bt.buf_addrs[k + 1] || bt.buf_sizes[k + 1] < (bt.num_rows + 1) * 4)
             return fail(GZ_INVALID, "offsets buffer too small");
           if (bt.buf_sizes[k + 2] > 0 && !bt.buf_addrs[k + 2]) return fail(GZ_INVALID, "missing values buffer");
@@ -484,7 +546,7 @@ int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
           heap_need += std::max<int64_t>(bt.buf_sizes[k + 2], 0);
           continue;
         }
-        if (!bt.buf_addrs[k + 1] || bt.buf_sizes[k + 1] < bt.num_rows * key_width(m->key_types[j]))
+        if (!bt.buf_addrs[k + 1] || bt.buf_sizes[k + 1] < bt.num_rows * key_width(d->key_types[j]))
           return fail(GZ_INVALID, "values buffer too small");
         kb[b].vals[j] = (const uint8_t*)bt.buf_addrs[k + 1];
       }
@@ -492,31 +554,34 @@ int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
     prefix[b + 1] = prefix[b] + bt.num_rows;
   }
   const int64_t rows = prefix[nb];
-  if (rows == 0) return GZ_OK;  // ProcessAndCacheOne returns early on empty input
+  if (rows == 0) return GZ_OK;
   if (rows > INT32_MAX) return fail(GZ_NOT_IMPLEMENTED, "2^31 or more rows in one multi-key call");
-  if (m->ncodes + rows > INT32_MAX) return fail(GZ_CAPACITY_ERROR, "2^31 or more groups");
-  GZ_TRY(mk_reserve(m, m->ncodes + rows, heap_need));
-  GZ_TRY(upload(ctx, &m->d_kb, kb.data(), sizeof(KeyBatchPtrs) * kb.size()));
-  GZ_TRY(upload(ctx, &m->d_prefix, prefix.data(), sizeof(int64_t) * prefix.size()));
-  GZ_TRY(ctx_ensure(ctx, &m->rowslot, sizeof(int64_t) * (size_t)rows));
+  if (mode != 2) {
+    if (d->ncodes + rows > INT32_MAX) return fail(GZ_CAPACITY_ERROR, "2^31 or more distinct keys");
+    GZ_TRY(kd_reserve(d, d->ncodes + rows, heap_need));
+  }
+  GZ_TRY(upload(ctx, &sc->d_kb, kb.data(), sizeof(KeyBatchPtrs) * kb.size()));
+  GZ_TRY(upload(ctx, &sc->d_prefix, prefix.data(), sizeof(int64_t) * prefix.size()));
+  GZ_TRY(ctx_ensure(ctx, &sc->rowslot, sizeof(int64_t) * (size_t)rows));
   KeyParams p;
   memset(&p, 0, sizeof(p));
-  p.kb = (const KeyBatchPtrs*)m->d_kb.p;
-  p.prefix = (const int64_t*)m->d_prefix.p;
+  p.kb = (const KeyBatchPtrs*)sc->d_kb.p;
+  p.prefix = (const int64_t*)sc->d_prefix.p;
   p.nb = nb;
-  p.K = m->K;
-  for (int j = 0; j < m->K; j++) p.width[j] = key_width(m->key_types[j]);
+  p.K = d->K;
+  for (int j = 0; j < d->K; j++) p.width[j] = key_width(d->key_types[j]);
+  p.sj = d->sj;
+  p.mode = mode;
   p.rows = rows;
-  p.table = (unsigned long long*)m->table.p;
-  p.mask = m->tcap - 1;
-  p.kvals = (unsigned long long*)m->kvals.p;
-  p.kmask = (uint32_t*)m->kmask.p;
-  p.counter = (unsigned long long*)m->counter.p;
-  p.rowslot = (int64_t*)m->rowslot.p;
-  p.sj = m->sj;
-  p.heap = (uint8_t*)m->heap.p;
-  p.koff = (int64_t*)m->koff.p;
-  p.klen = (int32_t*)m->klen.p;
+  p.table = (unsigned long long*)d->table.p;
+  p.mask = d->tcap - 1;
+  p.kvals = (unsigned long long*)d->kvals.p;
+  p.kmask = (uint32_t*)d->kmask.p;
+  p.counter = (unsigned long long*)d->counter.p;
+  p.rowslot = (int64_t*)sc->rowslot.p;
+  p.heap = (uint8_t*)d->heap.p;
+  p.koff = (int64_t*)d->koff.p;
+  p.klen = (int32_t*)d->klen.p;
   const unsigned grid = (unsigned)((rows + 255) / 256);
   {
     KernelTimer kt(ctx, "key_find");
@@ -524,19 +589,32 @@ int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
   }
   {
     KernelTimer kt(ctx, "key_assign");
-    hipLaunchKernelGGL(key_assign_kernel, dim3(grid), dim3(256), 0, ctx->stream, p);
+    if (mode != 2) hipLaunchKernelGGL(key_assign_kernel, dim3(grid), dim3(256), 0, ctx->stream, p);
     hipLaunchKernelGGL(key_code_kernel, dim3(grid), dim3(256), 0, ctx->stream,
-                       (const unsigned long long*)m->table.p, (int64_t*)m->rowslot.p, rows);
+                       (const unsigned long long*)d->table.p, (int64_t*)sc->rowslot.p, rows);
   }
   GZ_HIP(hipGetLastError());
-  unsigned long long nc[2] = {0, 0};
-  GZ_HIP(hipMemcpyAsync(nc, m->counter.p, sizeof(nc), hipMemcpyDeviceToHost, ctx->stream));
-  GZ_HIP(hipStreamSynchronize(ctx->stream));
-  m->ncodes = (int64_t)nc[0];
-  m->heap_used = (int64_t)nc[1];
-  // the single-key aggregate over the batches with the code column appended
-  std::vector<int64_t> addrs((size_t)nb * (need_bufs + 2)), sizes(addrs.size());
-  std::vector<gz_batch> inner(nb);
+  if (mode != 2) {
+    unsigned long long nc[2] = {0, 0};
+    GZ_HIP(hipMemcpyAsync(nc, d->counter.p, sizeof(nc), hipMemcpyDeviceToHost, ctx->stream));
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+    d->ncodes = (int64_t)nc[0];
+    d->heap_used = (int64_t)nc[1];
+  }
+  *codes = (const int64_t*)sc->rowslot.p;
+  *rows_out = rows;
+  return GZ_OK;
+}
+
+// Each batch's buffer list with an int64 column of codes (no validity) appended.
+void with_code_column(int32_t ncols, const int32_t* types, int32_t nb, const gz_batch* batches, const int64_t* codes,
+                      std::vector<int64_t>& addrs, std::vector<int64_t>& sizes, std::vector<gz_batch>& out) {
+  int need_bufs = 0;
+  for (int c = 0; c < ncols; c++) need_bufs += type_num_bufs(types[c]);
+  addrs.assign((size_t)nb * (need_bufs + 2), 0);
+  sizes.assign(addrs.size(), 0);
+  out.resize(nb);
+  int64_t off = 0;
   for (int b = 0; b < nb; b++) {
     const gz_batch& bt = batches[b];
     int64_t* a = &addrs[(size_t)b * (need_bufs + 2)];
@@ -545,12 +623,25 @@ int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
       a[k] = bt.num_rows > 0 ? bt.buf_addrs[k] : 0;
       s[k] = bt.num_rows > 0 ? bt.buf_sizes[k] : 0;
     }
-    a[need_bufs] = 0;
-    s[need_bufs] = 0;
-    a[need_bufs + 1] = (int64_t)((int64_t*)m->rowslot.p + prefix[b]);
+    a[need_bufs + 1] = bt.num_rows > 0 ? (int64_t)(codes + off) : 0;
     s[need_bufs + 1] = bt.num_rows * (int64_t)sizeof(int64_t);
-    inner[b] = gz_batch{bt.num_rows, a, s, need_bufs + 2};
+    out[b] = gz_batch{bt.num_rows, a, s, need_bufs + 2};
+    off += std::max<int64_t>(bt.num_rows, 0);
   }
+}
+
+int mk_process(MultiKey* m, int32_t nb, const gz_batch* batches) {
+  gz_context* ctx = m->ctx;
+  if (m->finished) return fail(GZ_INVALID, "aggregate already finished");
+  GZ_HIP(hipSetDevice(ctx->device));
+  const int64_t* codes = nullptr;
+  int64_t rows = 0;
+  GZ_TRY(kd_encode(m->kd, ctx, &m->sc, m->ncols, m->types.data(), m->key_cols.data(), nb, batches, 0, &codes,
+                   &rows));
+  if (rows == 0) return GZ_OK;  // ProcessAndCacheOne returns early on empty input
+  std::vector<int64_t> addrs, sizes;
+  std::vector<gz_batch> inner;
+  with_code_column(m->ncols, m->types.data(), nb, batches, codes, addrs, sizes, inner);
   return gz_agg_process_batches(m->inner, nb, inner.data());
 }
 
@@ -559,10 +650,11 @@ int mk_num_result_columns(MultiKey* m) { return (int)m->res_map.size(); }
 int mk_result_type(MultiKey* m, int col) {
   if (col < 0 || col >= (int)m->res_map.size()) return -1;
   const int r = m->res_map[col];
-  return r < 0 ? m->key_types[-r - 1] : gz_agg_result_type(m->inner, r);
+  return r < 0 ? m->kd->key_types[-r - 1] : gz_agg_result_type(m->inner, r);
 }
 
 int mk_finish(MultiKey* m) {
+  KeyDict* d = m->kd;
   if (m->finished) return GZ_OK;
   gz_context* ctx = m->ctx;
   GZ_HIP(hipSetDevice(ctx->device));
@@ -572,20 +664,20 @@ int mk_finish(MultiKey* m) {
   GZ_TRY(gz_agg_result_column(m->inner, m->code_res, &codes, nullptr));
   for (auto& b : m->out_vals) ctx_free(ctx, &b);
   for (auto& b : m->out_valid) ctx_free(ctx, &b);
-  m->out_vals.assign(m->K, DBuf());
-  m->out_valid.assign(m->K, DBuf());
+  m->out_vals.assign(d->K, DBuf());
+  m->out_valid.assign(d->K, DBuf());
   KeyGatherParams f;
   memset(&f, 0, sizeof(f));
   f.codes = (const int64_t*)codes;
   f.n = m->ngroups;
-  f.K = m->K;
-  f.kvals = (const unsigned long long*)m->kvals.p;
-  f.kmask = (const uint32_t*)m->kmask.p;
-  f.sj = m->sj;
-  for (int j = 0; j < m->K; j++) {
-    f.width[j] = key_width(m->key_types[j]);
+  f.K = d->K;
+  f.kvals = (const unsigned long long*)d->kvals.p;
+  f.kmask = (const uint32_t*)d->kmask.p;
+  f.sj = d->sj;
+  for (int j = 0; j < d->K; j++) {
+    f.width[j] = key_width(d->key_types[j]);
     // the utf8 key's values are its int32 offsets (groups + 1)
-    const size_t vbytes = j == m->sj ? sizeof(int32_t) * (size_t)(m->ngroups + 1)
+    const size_t vbytes = j == d->sj ? sizeof(int32_t) * (size_t)(m->ngroups + 1)
                                      : (size_t)std::max<int64_t>(m->ngroups, 1) * f.width[j];
     GZ_TRY(ctx_alloc(ctx, vbytes, &m->out_vals[j]));
     GZ_TRY(ctx_alloc(ctx, (size_t)std::max<int64_t>(m->ngroups, 1), &m->out_valid[j]));
@@ -597,16 +689,16 @@ int mk_finish(MultiKey* m) {
     const unsigned gx = (unsigned)std::min<int64_t>((m->ngroups + 255) / 256, 8192);
     hipLaunchKernelGGL(key_gather_kernel, dim3(gx), dim3(256), 0, ctx->stream, f);
   }
-  if (m->sj >= 0) {
-    if (m->heap_used > INT32_MAX) return fail(GZ_CAPACITY_ERROR, "utf8 keys of 2^31 bytes or more");
+  if (d->sj >= 0) {
+    if (d->heap_used > INT32_MAX) return fail(GZ_CAPACITY_ERROR, "utf8 keys of 2^31 bytes or more");
     ctx_free(ctx, &m->str_data);
-    GZ_TRY(ctx_alloc(ctx, (size_t)std::max<int64_t>(m->heap_used, 1), &m->str_data));
+    GZ_TRY(ctx_alloc(ctx, (size_t)std::max<int64_t>(d->heap_used, 1), &m->str_data));
     DBuf lens;
     GZ_TRY(ctx_alloc(ctx, sizeof(int32_t) * (size_t)(m->ngroups + 1), &lens));
-    int32_t* offs = (int32_t*)m->out_vals[m->sj].p;
+    int32_t* offs = (int32_t*)m->out_vals[d->sj].p;
     const unsigned g1 = (unsigned)((m->ngroups + 1 + 255) / 256);
     hipLaunchKernelGGL(key_str_len_kernel, dim3(g1), dim3(256), 0, ctx->stream, (const int64_t*)codes, m->ngroups,
-                       (const int32_t*)m->klen.p, (int32_t*)lens.p);
+                       (const int32_t*)d->klen.p, (int32_t*)lens.p);
     size_t tmp = 0;
     GZ_HIP(rocprim::exclusive_scan(nullptr, tmp, (const int32_t*)lens.p, offs, 0, (size_t)(m->ngroups + 1),
                                    rocprim::plus<int32_t>(), ctx->stream));
@@ -615,8 +707,8 @@ int mk_finish(MultiKey* m) {
                                    rocprim::plus<int32_t>(), ctx->stream));
     if (m->ngroups > 0)
       hipLaunchKernelGGL(key_str_copy_kernel, dim3((unsigned)((m->ngroups + 255) / 256)), dim3(256), 0, ctx->stream,
-                         (const int64_t*)codes, m->ngroups, (const int64_t*)m->koff.p, (const int32_t*)offs,
-                         (const uint8_t*)m->heap.p, (uint8_t*)m->str_data.p);
+                         (const int64_t*)codes, m->ngroups, (const int64_t*)d->koff.p, (const int32_t*)offs,
+                         (const uint8_t*)d->heap.p, (uint8_t*)m->str_data.p);
     GZ_HIP(hipGetLastError());
     GZ_HIP(hipStreamSynchronize(ctx->stream));
     ctx_free(ctx, &lens);
@@ -647,10 +739,10 @@ int mk_result_binary(MultiKey* m, int col, void** offsets, void** data, void** v
   if (!m->finished) return fail(GZ_INVALID, "call gz_agg_finish first");
   if (col < 0 || col >= (int)m->res_map.size()) return fail(GZ_INDEX_ERROR, "result column out of range");
   const int r = m->res_map[col];
-  if (r >= 0 || -r - 1 != m->sj) return fail(GZ_TYPE_ERROR, "result column is not utf8");
-  if (offsets) *offsets = m->out_vals[m->sj].p;
+  if (r >= 0 || -r - 1 != m->kd->sj) return fail(GZ_TYPE_ERROR, "result column is not utf8");
+  if (offsets) *offsets = m->out_vals[m->kd->sj].p;
   if (data) *data = m->str_data.p;
-  if (valid) *valid = m->out_valid[m->sj].p;
+  if (valid) *valid = m->out_valid[m->kd->sj].p;
   return GZ_OK;
 }
 
@@ -658,11 +750,11 @@ void mk_release(MultiKey* m) {
   if (!m) return;
   gz_context* ctx = m->ctx;
   gz_agg_release(m->inner);
+  kd_release(m->kd);
+  kd_scratch_free(ctx, &m->sc);
   for (auto& b : m->out_vals) ctx_free(ctx, &b);
   for (auto& b : m->out_valid) ctx_free(ctx, &b);
-  for (DBuf* b : {&m->table, &m->kvals, &m->kmask, &m->counter, &m->rowslot, &m->d_kb, &m->d_prefix, &m->heap,
-                  &m->koff, &m->klen, &m->str_data, &m->str_tmp})
-    ctx_free(ctx, b);
+  for (DBuf* b : {&m->str_data, &m->str_tmp}) ctx_free(ctx, b);
   delete m;
 }
 

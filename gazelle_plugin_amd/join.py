@@ -32,13 +32,18 @@ _NP = {T.UINT8: np.uint8, T.INT8: np.int8, T.UINT16: np.uint16, T.INT16: np.int1
 class HashRelation:
     """The build side: a resident table in HBM (an opaque handle)."""
 
-    def __init__(self, ctx: Context, types: Sequence[int], key_col: int, builder_type: int = 1):
+    def __init__(self, ctx: Context, types: Sequence[int], key_col, builder_type: int = 1):
+        """key_col: a column index, or a list of them (several key columns)."""
         self.ctx = ctx
         self.types = list(types)
         self.key_col = key_col
         self._h = ctypes.c_void_p()
-        L.check(L.lib().gz_relation_make(ctx.handle, len(types), L.i32_array(types), key_col, builder_type,
-                                         ctypes.byref(self._h)))
+        if isinstance(key_col, (list, tuple)):  # several key columns (UnsafeRow keys)
+            L.check(L.lib().gz_relation_make_keys(ctx.handle, len(types), L.i32_array(types), len(key_col),
+                                                  L.i32_array(list(key_col)), builder_type, ctypes.byref(self._h)))
+        else:
+            L.check(L.lib().gz_relation_make(ctx.handle, len(types), L.i32_array(types), key_col, builder_type,
+                                             ctypes.byref(self._h)))
         self.finished = False
 
     def Evaluate(self, batch: DeviceBatch):
@@ -94,9 +99,15 @@ class ConditionedProbeKernel:
         self._h = ctypes.c_void_p()
         sides = [s for s, _ in self.result]
         idx = [i for _, i in self.result]
-        L.check(L.lib().gz_probe_make(relation.handle, len(probe_types), L.i32_array(probe_types), probe_key,
-                                      JOIN_TYPES[join_type], int(null_aware), len(self.result),
-                                      L.i32_array(sides), L.i32_array(idx), ctypes.byref(self._h)))
+        if isinstance(probe_key, (list, tuple)):
+            L.check(L.lib().gz_probe_make_keys(relation.handle, len(probe_types), L.i32_array(probe_types),
+                                               len(probe_key), L.i32_array(list(probe_key)), JOIN_TYPES[join_type],
+                                               int(null_aware), len(self.result), L.i32_array(sides),
+                                               L.i32_array(idx), ctypes.byref(self._h)))
+        else:
+            L.check(L.lib().gz_probe_make(relation.handle, len(probe_types), L.i32_array(probe_types), probe_key,
+                                          JOIN_TYPES[join_type], int(null_aware), len(self.result),
+                                          L.i32_array(sides), L.i32_array(idx), ctypes.byref(self._h)))
 
     @classmethod
     def Make(cls, relation, probe_types, probe_key, join_type="inner", null_aware=False, result=()):

@@ -392,6 +392,23 @@ int gz_probe_process_batches(gz_probe* p, int32_t num_batches, const gz_batch* b
                              int64_t* out_rows, int64_t* out_batch_rows);
 int gz_probe_result_column(gz_probe* p, int32_t col, void** values, void** valid_bytes,
                            void** offsets, void** data);
+
+/* Several join key columns (HashRelationKernel / ConditionedProbe with
+ * UnsafeRow keys: hash_relation_kernel.cc:264-277, hash_relation.h:176-198,
+ * conditioned_probe_kernel.cc:609-640), or one utf8 key.  A build row with a
+ * null key is never inserted; a probe row matches build rows whose keys have
+ * identical bytes (fixed-width numeric keys, and a utf8 key only as the last
+ * one); the probe's key types must equal the build's.  Build and probe take
+ * batches, results and release as the one-key forms; one fixed-width key
+ * falls back to gz_relation_make / gz_probe_make.  Null-aware anti joins with
+ * several keys, and the fused join -> aggregate over such a relation, are
+ * NotImplemented. */
+int gz_relation_make_keys(gz_context* ctx, int32_t num_cols, const int32_t* col_types, int32_t num_keys,
+                          const int32_t* key_cols, int32_t builder_type, gz_relation** out);
+int gz_probe_make_keys(gz_relation* r, int32_t num_probe_cols, const int32_t* probe_types, int32_t num_keys,
+                       const int32_t* probe_key_cols, int32_t join_type, int32_t null_aware_anti,
+                       int32_t num_result_cols, const int32_t* result_sides, const int32_t* result_indices,
+                       gz_probe** out);
 int gz_probe_release(gz_probe* p);
 
 /* ---- fused join -> aggregate -------------------------------------------------

@@ -323,6 +323,52 @@ def join(build_keys: Sequence, probe_keys, key_type: int, join_type: int, null_a
         L.gzo_rel_free(r)
 
 
+def _key_tuples(batch, cols) -> list:
+    """Per row of a HostBatch: None if any of the key columns is null, else the
+    tuple of the keys' raw bits (a utf8 key's bytes)."""
+    n = batch.num_rows
+    parts = []
+    for c in cols:
+        col = batch.columns[c]
+        ok = (np.ones(n, bool) if col.validity is None else
+              np.unpackbits(np.asarray(col.validity, np.uint8), bitorder="little")[:n].astype(bool))
+        if col.type_id in (13, 14):
+            vals = [bytes(col.values[col.offsets[r]:col.offsets[r + 1]]) for r in range(n)]
+        else:
+            raw = col.values[:n].view(np.dtype(f"u{col.values.dtype.itemsize}"))
+            vals = [int(v) for v in raw]
+        parts.append((vals, ok))
+    return [None if not all(ok[r] for _, ok in parts) else tuple(v[r] for v, _ in parts) for r in range(n)]
+
+
+def join_multi(build, build_key_cols, probe, probe_key_cols, join_type: int, builder_type: int = 1):
+    """Hash join on several key columns.  The reference keys such relations by
+    UnsafeRow bytes: a build row with a null key is never inserted
+    (hash_relation.h:176-198), and a probe row matches the build rows whose key
+    bytes are equal (conditioned_probe_kernel.cc:609-640), so a probe row with a
+    null key matches nothing.  Restated as the one-key join over first-seen ids
+    of the build side's key tuples: a build row with a null key gets a null id,
+    and so does a probe row with a null key or a tuple the build side lacks.
+    Same return as join()."""
+    def bitmap(ok):
+        bits = np.ones((len(ok) + 63) // 64 * 64, np.uint8)
+        bits[:len(ok)] = ok
+        return np.packbits(bits, bitorder="little")
+
+    ids = {}
+    bk, pk = [], []
+    for b in build:
+        t = _key_tuples(b, build_key_cols)
+        vals = np.array([ids.setdefault(k, len(ids)) if k is not None else 0 for k in t], np.int64)
+        bk.append((vals, bitmap(np.array([k is not None for k in t], np.uint8)), b.num_rows))
+    for b in probe:
+        t = _key_tuples(b, probe_key_cols)
+        got = [ids.get(k) if k is not None else None for k in t]
+        vals = np.array([g if g is not None else 0 for g in got], np.int64)
+        pk.append((vals, bitmap(np.array([g is not None for g in got], np.uint8)), b.num_rows))
+    return join(bk, pk, 9, join_type, False, builder_type)
+
+
 # ------------------------------------------------------------------ lz4
 
 def lz4f_decompress(frame, capacity: int) -> np.ndarray:

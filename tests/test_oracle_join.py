@@ -108,3 +108,54 @@ def test_inner_join_matches_pyarrow():
     j = l.join(r, "k", join_type="inner")
     exp = sorted(zip(j["pi"].to_pylist(), j["bi"].to_pylist()))
     assert got == exp
+
+
+def multikey_join_batches(seed, nbuild=(300, 0, 257), nprobe=(400, 1, 333), null_frac=0.1):
+    """Build (int32 k0, int64 k1, double v) and probe (int64 k1, int32 w, int32 k0)
+    batches over small key domains with nulls and repeated build keys."""
+    from gazelle_plugin_amd.batch import HostBatch, HostColumn, bitmap_from_bools
+    rng = np.random.default_rng(seed)
+
+    def vb(n):
+        return bitmap_from_bools(rng.random(n) >= null_frac)
+
+    build = [HostBatch(n, [HostColumn(T.INT32, rng.integers(0, 6, n).astype(np.int32), vb(n)),
+                           HostColumn(T.INT64, rng.integers(0, 9, n).astype(np.int64) * 7919, vb(n)),
+                           HostColumn(T.DOUBLE, rng.normal(size=n), vb(n))]) for n in nbuild]
+    probe = [HostBatch(n, [HostColumn(T.INT64, rng.integers(0, 10, n).astype(np.int64) * 7919, vb(n)),
+                           HostColumn(T.INT32, rng.integers(-9, 9, n).astype(np.int32)),
+                           HostColumn(T.INT32, rng.integers(0, 7, n).astype(np.int32), vb(n))]) for n in nprobe]
+    return build, probe
+
+
+@pytest.mark.parametrize("join_type", ["inner", "semi", "anti"])
+@pytest.mark.parametrize("builder_type", [1, 3])
+def test_multi_key_join_brute_force(join_type, builder_type):
+    """Several key columns: the oracle against a nested loop over key tuples
+    (a null key in a build row or a probe row never matches), matches in probe
+    order and build insertion order (the first build row per key with
+    builder_type 3)."""
+    build, probe = multikey_join_batches(1)
+    bt = [k for b in build for k in oracle._key_tuples(b, [0, 1])]
+    outs = oracle.join_multi(build, [0, 1], probe, [2, 0], JT[join_type], builder_type)
+    for pb, out in zip(probe, outs):
+        pt = oracle._key_tuples(pb, [2, 0])
+        exp_p, exp_b = [], []
+        for r, k in enumerate(pt):
+            seen = set()
+            hits = []
+            for g, bk in enumerate(bt):
+                if k is not None and bk == k:
+                    if builder_type == 3 and bk in seen:
+                        continue
+                    seen.add(bk)
+                    hits.append(g)
+            if join_type == "inner":
+                exp_p += [r] * len(hits)
+                exp_b += hits
+            elif (join_type == "semi") == bool(hits):
+                exp_p.append(r)
+        if join_type == "inner":
+            assert out[0].tolist() == exp_p and out[1].tolist() == exp_b
+        else:
+            assert out.tolist() == exp_p
