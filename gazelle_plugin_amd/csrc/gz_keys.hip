@@ -189,9 +189,10 @@ __global__ __launch_bounds__(256) void key_find_kernel(KeyParams p) {
   const uint64_t fp = (h >> 32) | 1u;  // never zero: an occupied slot is never 0
   const unsigned long long mine = (fp << 32) | (uint64_t)g;
   uint64_t s = h & p.mask;
+  unsigned long long w;
   // the table is at most half full, so the walk ends
   while (true) {
-    unsigned long long w = __hip_atomic_load(&p.table[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    w = __hip_atomic_load(&p.table[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (w == 0 && p.mode == 2) {  // look-up: an empty slot ends the walk
       p.rowslot[g] = -2;
       return;
@@ -204,7 +205,8 @@ __global__ __launch_bounds__(256) void key_find_kernel(KeyParams p) {
     if ((w >> 32) == fp && key_equal(p, (uint32_t)w, raw, m, str)) break;
     s = (s + 1) & p.mask;
   }
-  p.rowslot[g] = (int64_t)s;
+  // a look-up only meets finished codes: the code itself, no code pass
+  p.rowslot[g] = p.mode == 2 ? (int64_t)(w & 0x7fffffffull) : (int64_t)s;
 }
 
 __global__ __launch_bounds__(256) void key_assign_kernel(KeyParams p) {
@@ -589,9 +591,11 @@ int kd_encode(KeyDict* d, gz_context* ctx, KeyScratch* sc, int32_t ncols, const 
   }
   {
     KernelTimer kt(ctx, "key_assign");
-    if (mode != 2) hipLaunchKernelGGL(key_assign_kernel, dim3(grid), dim3(256), 0, ctx->stream, p);
-    hipLaunchKernelGGL(key_code_kernel, dim3(grid), dim3(256), 0, ctx->stream,
-                       (const unsigned long long*)d->table.p, (int64_t*)sc->rowslot.p, rows);
+    if (mode != 2) {
+      hipLaunchKernelGGL(key_assign_kernel, dim3(grid), dim3(256), 0, ctx->stream, p);
+      hipLaunchKernelGGL(key_code_kernel, dim3(grid), dim3(256), 0, ctx->stream,
+                         (const unsigned long long*)d->table.p, (int64_t*)sc->rowslot.p, rows);
+    }
   }
   GZ_HIP(hipGetLastError());
   if (mode != 2) {
