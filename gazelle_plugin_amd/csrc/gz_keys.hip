@@ -38,6 +38,7 @@ namespace gz {
 
 constexpr int kMaxKeys = 8;
 constexpr uint32_t kPermanent = 0x80000000u;  // slot reference is a code
+constexpr int64_t kClaimed = 1ll << 62;        // per-row slot: this row claimed it
 
 struct KeyBatchPtrs {
   const uint8_t* valid[kMaxKeys];
@@ -199,7 +200,10 @@ __global__ __launch_bounds__(256) void key_find_kernel(KeyParams p) {
     }
     if (w == 0) {
       const unsigned long long old = atomicCAS(&p.table[s], 0ull, mine);
-      if (old == 0) break;
+      if (old == 0) {
+        p.rowslot[g] = (int64_t)s | kClaimed;
+        return;
+      }
       w = old;
     }
     if ((w >> 32) == fp && key_equal(p, (uint32_t)w, raw, m, str)) break;
@@ -210,24 +214,49 @@ __global__ __launch_bounds__(256) void key_find_kernel(KeyParams p) {
 }
 
 __global__ __launch_bounds__(256) void key_assign_kernel(KeyParams p) {
+  // codes and heap bytes are reserved once per workgroup (LDS counts, one
+  // device atomic each), not once per claiming row on a single word
+  __shared__ unsigned int s_n;
+  __shared__ unsigned long long s_bytes, s_code0, s_heap0;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= p.rows) return;
-  const int64_t s = p.rowslot[g];
-  if (s < 0) return;
-  const unsigned long long w = p.table[s];
-  // only the row that claimed the slot holds its own reference (a code has
-  // the top bit set, call rows are below 2^31)
-  if ((uint32_t)w != (uint32_t)g) return;
+  bool claimer = false;
+  int64_t s = 0;
+  if (g < p.rows) {
+    const int64_t rs = p.rowslot[g];
+    // only the rows that claimed a slot (marked by the walk) take codes
+    claimer = rs >= 0 && (rs & kClaimed);
+    s = rs & ~kClaimed;
+  }
+  if (threadIdx.x == 0) {
+    s_n = 0;
+    s_bytes = 0;
+  }
+  __syncthreads();
   uint64_t raw[kMaxKeys];
-  StrRef str;
-  const uint32_t m = load_key(p, g, raw, &str);
-  const unsigned long long c = atomicAdd(p.counter, 1ull);
+  StrRef str{nullptr, 0};
+  uint32_t m = 0;
+  unsigned my = 0;
+  unsigned long long my_off = 0;
+  if (claimer) {
+    m = load_key(p, g, raw, &str);
+    my = atomicAdd(&s_n, 1u);
+    if (p.sj >= 0 && str.n > 0) my_off = atomicAdd(&s_bytes, (unsigned long long)str.n);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_code0 = s_n ? atomicAdd(p.counter, (unsigned long long)s_n) : 0ull;
+    s_heap0 = s_bytes ? atomicAdd(p.counter + 1, s_bytes) : 0ull;
+  }
+  __syncthreads();
+  if (!claimer) return;
+  const unsigned long long w = p.table[s];
+  const unsigned long long c = s_code0 + my;
 #pragma unroll
   for (int j = 0; j < kMaxKeys; j++)
     if (j < p.K) p.kvals[c * p.K + j] = raw[j];
   p.kmask[c] = m;
   if (p.sj >= 0) {
-    const unsigned long long at = str.n > 0 ? atomicAdd(p.counter + 1, (unsigned long long)str.n) : 0ull;
+    const unsigned long long at = str.n > 0 ? s_heap0 + my_off : 0ull;
     for (int32_t i = 0; i < str.n; i++) p.heap[at + i] = str.p[i];
     p.koff[c] = (int64_t)at;
     p.klen[c] = str.n;
@@ -239,7 +268,7 @@ __global__ __launch_bounds__(256) void key_code_kernel(const unsigned long long*
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= rows) return;
   const int64_t s = rowslot[g];
-  if (s >= 0) rowslot[g] = (int64_t)(table[s] & 0x7fffffffull);
+  if (s >= 0) rowslot[g] = (int64_t)(table[s & ~kClaimed] & 0x7fffffffull);
 }
 
 // Re-inserts codes [0, n) into a larger empty table.
