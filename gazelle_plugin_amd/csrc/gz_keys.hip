@@ -46,10 +46,16 @@ struct KeyBatchPtrs {
   const int32_t* soffs;           // the utf8 key's offsets
 };
 
+// 256 rows of one batch: the work of one workgroup.  A row's reference in the
+// slot table is tile * 256 + lane.
+struct KeyTile {
+  int32_t batch, row0, nrows, pad;
+  int64_t g0;  // call row of row0
+};
+
 struct KeyParams {
   const KeyBatchPtrs* kb;  // per batch of the call
-  const int64_t* prefix;   // [nb + 1] first call row of each batch
-  int nb;
+  const KeyTile* tiles;
   int K;
   int width[kMaxKeys];  // bytes per value: 1, 2, 4 or 8
   int sj;               // index of the utf8 key (the last one), -1 if none
@@ -89,21 +95,9 @@ __device__ __forceinline__ uint64_t bytes_hash64(const uint8_t* p, int32_t n) {
   return h ^ (h >> 31);
 }
 
-__device__ __forceinline__ int batch_of(const int64_t* prefix, int nb, int64_t g) {
-  int lo = 0, hi = nb;  // prefix[lo] <= g < prefix[hi]
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (prefix[mid] <= g) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
 // Key tuple of call row g: raw bits per key (zero-extended; the utf8 key's
 // hash), null mask, and the utf8 key's bytes.
-__device__ __forceinline__ uint32_t load_key(const KeyParams& p, int64_t g, uint64_t* raw, StrRef* str) {
-  const int b = batch_of(p.prefix, p.nb, g);
-  const int64_t r = g - p.prefix[b];
+__device__ __forceinline__ uint32_t load_key(const KeyParams& p, int b, int64_t r, uint64_t* raw, StrRef* str) {
   const KeyBatchPtrs& B = p.kb[b];
   uint32_t m = 0;
   str->p = nullptr;
@@ -163,7 +157,8 @@ __device__ __forceinline__ bool key_equal(const KeyParams& p, uint32_t ref, cons
     os.p = p.sj >= 0 ? p.heap + p.koff[c] : nullptr;
     os.n = p.sj >= 0 ? p.klen[c] : 0;
   } else {
-    om = load_key(p, (int64_t)ref, o, &os);
+    const KeyTile T = p.tiles[ref >> 8];
+    om = load_key(p, T.batch, (int64_t)T.row0 + (ref & 255), o, &os);
   }
   bool eq = om == m;
 #pragma unroll
@@ -177,18 +172,20 @@ __device__ __forceinline__ bool key_equal(const KeyParams& p, uint32_t ref, cons
 }
 
 __global__ __launch_bounds__(256) void key_find_kernel(KeyParams p) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= p.rows) return;
+  const KeyTile T = p.tiles[blockIdx.x];
+  if ((int)threadIdx.x >= T.nrows) return;
+  const int64_t g = T.g0 + threadIdx.x;
+  const uint32_t ref = blockIdx.x * 256u + threadIdx.x;
   uint64_t raw[kMaxKeys];
   StrRef str;
-  const uint32_t m = load_key(p, g, raw, &str);
+  const uint32_t m = load_key(p, T.batch, (int64_t)T.row0 + threadIdx.x, raw, &str);
   if (p.mode != 0 && m != 0) {  // join keys: a row with a null key never matches
     p.rowslot[g] = p.mode == 1 ? -1 : -2;
     return;
   }
   const uint64_t h = key_hash(raw, m, p.K);
   const uint64_t fp = (h >> 32) | 1u;  // never zero: an occupied slot is never 0
-  const unsigned long long mine = (fp << 32) | (uint64_t)g;
+  const unsigned long long mine = (fp << 32) | (uint64_t)ref;
   uint64_t s = h & p.mask;
   unsigned long long w;
   // the table is at most half full, so the walk ends
@@ -218,10 +215,11 @@ __global__ __launch_bounds__(256) void key_assign_kernel(KeyParams p) {
   // device atomic each), not once per claiming row on a single word
   __shared__ unsigned int s_n;
   __shared__ unsigned long long s_bytes, s_code0, s_heap0;
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const KeyTile T = p.tiles[blockIdx.x];
+  const int64_t g = T.g0 + threadIdx.x;
   bool claimer = false;
   int64_t s = 0;
-  if (g < p.rows) {
+  if ((int)threadIdx.x < T.nrows) {
     const int64_t rs = p.rowslot[g];
     // only the rows that claimed a slot (marked by the walk) take codes
     claimer = rs >= 0 && (rs & kClaimed);
@@ -238,7 +236,7 @@ __global__ __launch_bounds__(256) void key_assign_kernel(KeyParams p) {
   unsigned my = 0;
   unsigned long long my_off = 0;
   if (claimer) {
-    m = load_key(p, g, raw, &str);
+    m = load_key(p, T.batch, (int64_t)T.row0 + threadIdx.x, raw, &str);
     my = atomicAdd(&s_n, 1u);
     if (p.sj >= 0 && str.n > 0) my_off = atomicAdd(&s_bytes, (unsigned long long)str.n);
   }
@@ -592,13 +590,20 @@ int kd_encode(KeyDict* d, gz_context* ctx, KeyScratch* sc, int32_t ncols, const 
     GZ_TRY(kd_reserve(d, d->ncodes + rows, heap_need));
   }
   GZ_TRY(upload(ctx, &sc->d_kb, kb.data(), sizeof(KeyBatchPtrs) * kb.size()));
-  GZ_TRY(upload(ctx, &sc->d_prefix, prefix.data(), sizeof(int64_t) * prefix.size()));
+  std::vector<KeyTile> tiles;
+  tiles.reserve((size_t)(rows / 256 + nb + 1));
+  for (int b = 0; b < nb; b++)
+    for (int64_t r0 = 0; r0 < batches[b].num_rows; r0 += 256)
+      tiles.push_back(KeyTile{b, (int32_t)r0, (int32_t)std::min<int64_t>(256, batches[b].num_rows - r0), 0,
+                              prefix[b] + r0});
+  if (tiles.size() >= (1u << 23)) return fail(GZ_NOT_IMPLEMENTED, "2^23 or more row tiles in one multi-key call");
+  GZ_TRY(upload(ctx, &sc->d_prefix, tiles.data(), sizeof(KeyTile) * tiles.size()));
+  const unsigned ntiles = (unsigned)tiles.size();
   GZ_TRY(ctx_ensure(ctx, &sc->rowslot, sizeof(int64_t) * (size_t)rows));
   KeyParams p;
   memset(&p, 0, sizeof(p));
   p.kb = (const KeyBatchPtrs*)sc->d_kb.p;
-  p.prefix = (const int64_t*)sc->d_prefix.p;
-  p.nb = nb;
+  p.tiles = (const KeyTile*)sc->d_prefix.p;
   p.K = d->K;
   for (int j = 0; j < d->K; j++) p.width[j] = key_width(d->key_types[j]);
   p.sj = d->sj;
@@ -616,12 +621,12 @@ int kd_encode(KeyDict* d, gz_context* ctx, KeyScratch* sc, int32_t ncols, const 
   const unsigned grid = (unsigned)((rows + 255) / 256);
   {
     KernelTimer kt(ctx, "key_find");
-    hipLaunchKernelGGL(key_find_kernel, dim3(grid), dim3(256), 0, ctx->stream, p);
+    hipLaunchKernelGGL(key_find_kernel, dim3(ntiles), dim3(256), 0, ctx->stream, p);
   }
   {
     KernelTimer kt(ctx, "key_assign");
     if (mode != 2) {
-      hipLaunchKernelGGL(key_assign_kernel, dim3(grid), dim3(256), 0, ctx->stream, p);
+      hipLaunchKernelGGL(key_assign_kernel, dim3(ntiles), dim3(256), 0, ctx->stream, p);
       hipLaunchKernelGGL(key_code_kernel, dim3(grid), dim3(256), 0, ctx->stream,
                          (const unsigned long long*)d->table.p, (int64_t*)sc->rowslot.p, rows);
     }
