@@ -125,26 +125,31 @@ class HashAggregateResultIterator:
 
     # -- input
     def ProcessAndCacheOne(self, batch: DeviceBatch):
+        self.k.ctx.hold(batch)
         L.check(L.lib().gz_agg_process(self._h, batch.num_rows, batch.addrs, batch.sizes, batch.num_bufs))
 
     def ProcessBatches(self, batches: Sequence[DeviceBatch], arr=None):
+        self.k.ctx.hold(list(batches))
         if arr is None:
             arr = make_batch_array(batches)
         L.check(L.lib().gz_agg_process_batches(self._h, len(batches), arr))
 
     def ProcessInstances(self, batches: Sequence[DeviceBatch], instance_of: Sequence[int], arr=None):
+        self.k.ctx.hold(list(batches))
         if arr is None:
             arr = make_batch_array(batches)
         L.check(L.lib().gz_agg_set_process(self._h, len(batches), arr, L.i32_array(instance_of)))
 
     # -- planned form: prepare once, then run / capture + launch (HIP graph)
     def Prepare(self, batches: Sequence[DeviceBatch], instance_of: Sequence[int], arr=None):
+        self._plan_batches = list(batches)  # a plan reads its batches at every run / launch
         self._plan_arr = arr if arr is not None else make_batch_array(batches)  # must outlive the plan
         self._plan_inst = L.i32_array(instance_of)
         L.check(L.lib().gz_agg_set_prepare(self._h, len(batches), self._plan_arr, self._plan_inst))
         self._finished = False
 
     def Autotune(self, batches: Sequence[DeviceBatch], instance_of: Sequence[int], arr=None) -> int:
+        self._plan_batches = list(batches)
         self._plan_arr = arr if arr is not None else make_batch_array(batches)
         self._plan_inst = L.i32_array(instance_of)
         chosen = ctypes.c_int32()

@@ -14,6 +14,11 @@ class Context:
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, stream: Optional[int] = None):
         self._h = ctypes.c_void_p()
+        # Inputs are read by work queued on the context's stream after a call
+        # returns (stream-ordered borrowing): the device batches passed in are
+        # held here until the stream is synchronized, so their memory cannot be
+        # reused under a running kernel.
+        self._held = []
         L.check(L.lib().gz_context_create(device, hbm_budget, ctypes.byref(self._h)))
         if stream is not None:
             self.set_stream(stream)
@@ -25,8 +30,13 @@ class Context:
     def set_stream(self, stream: Optional[int]):
         L.check(L.lib().gz_context_set_stream(self._h, ctypes.c_void_p(stream or 0)))
 
+    def hold(self, *objs):
+        """Keep input batches alive until the next synchronize()."""
+        self._held.extend(objs)
+
     def synchronize(self):
         L.check(L.lib().gz_context_synchronize(self._h))
+        self._held.clear()
 
     def bytes_in_use(self) -> int:
         return L.lib().gz_context_bytes_in_use(self._h)
@@ -50,6 +60,7 @@ class Context:
         if self._h:
             L.lib().gz_context_destroy(self._h)
             self._h = ctypes.c_void_p()
+        self._held = []
 
     def __enter__(self):
         return self

@@ -45,9 +45,21 @@ constexpr int kHllRegs = 1 << kHllBits;
 constexpr int kMaxUsed = 8;        // used input columns of a partition record
 constexpr int kMaxFanout = 256;    // partitions per instance on the partitioned path
 
+// Floating min/max keep one input value per group, as the reference's
+// accumulators do (MinAction / MaxAction, ext/actions_impl.cc:905-936,
+// :1504-1539): L_MINF / L_MAXF order the non-NaN values with -0.0 and 0.0 as one
+// key; L_FIRSTZ (row * 2 + sign of the first zero) gives a zero result the sign
+// of the first zero seen; the NaN a result keeps is named by a row lane
+// (L_FIRSTNAN, L_LASTNAN, L_GLASTNAN) and its bits are copied into L_NANBITS by
+// a fix-up pass over the rows (agg_fix_kernel), since no 64-bit atomic can
+// carry both a row index and a NaN payload.
 enum LaneOp : int {
   L_CNT = 0, L_NN, L_NN2, L_SUMI, L_SUMF, L_SUMCNT, L_MINI, L_MAXI, L_MINU, L_MAXU,
-  L_MINF, L_MAXF, L_HASNAN, L_FIRSTNN, L_FIRSTNAN
+  L_MINF, L_MAXF, L_HASNAN, L_FIRSTNN, L_FIRSTNAN,
+  L_FIRSTZ,    // min over zero rows of row * 2 + sign bit
+  L_LASTNAN,   // max over NaN rows of the row (max with NaN check: every NaN overwrites)
+  L_GLASTNAN,  // no group-by key, max with NaN check: the first NaN of the last batch holding one
+  L_NANBITS    // bits of the NaN named by lane col2 (-1: the group's first row), written by the fix-up
 };
 
 struct Lane {
@@ -89,6 +101,7 @@ struct AggParams {
   int ucol[kMaxUsed];         // column of used slot k
   int uslot[kAggMaxCols];     // used slot of a column
   int64_t rstride;            // records: word w of record q at recs[w * rstride + q]
+  int* fix_flag;              // set once a NaN reaches a lane whose bits a result keeps (null: no such lane)
 };
 
 __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
@@ -125,10 +138,39 @@ __device__ __forceinline__ double read_f64(const void* v, int t, int64_t r) {
   if (t == GZ_TYPE_UINT64) return (double)((const uint64_t*)v)[r];
   return (double)read_i64(v, t, r);
 }
-// Order-preserving map of a non-NaN double to uint64.
+// Order-preserving map of a non-NaN double to uint64; -0.0 maps as 0.0 (the
+// reference's `<` / `>` never prefer one zero to the other).
 __device__ __forceinline__ uint64_t ordered(double d) {
   uint64_t b = (uint64_t)__double_as_longlong(d);
+  if ((b << 1) == 0) b = 0;
   return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+constexpr int64_t kCanonNaN = 0x7ff8000000000000ll;
+__device__ __forceinline__ bool word_is_nan(int64_t w) { return (w & 0x7fffffffffffffffll) > 0x7ff0000000000000ll; }
+// A float group key as the table holds it (SparseHashMap<double>,
+// sparse_hash_map.h:86-160; phmap Hash<double>, phmap_utils.h:251-258): the
+// double's bits with -0.0 as 0.0 and every NaN as one NaN.
+__device__ __forceinline__ int64_t norm_float_key(int64_t w) {
+  if ((w << 1) == 0) return 0;
+  return word_is_nan(w) ? kCanonNaN : w;
+}
+__device__ __forceinline__ void note_nan(const AggParams& p) {
+  if (p.fix_flag && *(volatile int*)p.fix_flag == 0) *(volatile int*)p.fix_flag = 1;
+}
+// Value bits as stored (a float's 32 bits zero-extended): the bits a min / max
+// result or a float group key comes out with.
+__device__ __forceinline__ int64_t raw_bits(const void* v, int t, int64_t r) {
+  if (t == GZ_TYPE_FLOAT) return (int64_t)((const uint32_t*)v)[r];
+  return ((const int64_t*)v)[r];
+}
+__device__ __forceinline__ bool raw_is_nan(int64_t raw, int t) {
+  if (t == GZ_TYPE_FLOAT) return (raw & 0x7fffffff) > 0x7f800000;
+  return word_is_nan(raw);
+}
+// Ordinal of a NaN row for L_GLASTNAN: the latest batch first, then the
+// earliest row in it (batch start < 2^39 rows, rows per batch < 2^24).
+__device__ __forceinline__ int64_t glast_ord(int64_t grow, int64_t r) {
+  return ((grow - r) << 24) | (int64_t)(0xffffff - r);
 }
 
 __device__ __forceinline__ int64_t lane_init(int op) {
@@ -136,7 +178,8 @@ __device__ __forceinline__ int64_t lane_init(int op) {
     case L_MINI: return INT64_MAX;
     case L_MAXI: return INT64_MIN;
     case L_MINU: case L_MINF: return -1;  // all ones
-    case L_FIRSTNN: case L_FIRSTNAN: return INT64_MAX;
+    case L_FIRSTNN: case L_FIRSTNAN: case L_FIRSTZ: return INT64_MAX;
+    case L_LASTNAN: case L_GLASTNAN: return -1;
     default: return 0;
   }
 }
@@ -148,7 +191,15 @@ __device__ __forceinline__ bool row_key(const AggParams& p, int batch, int64_t r
     return true;
   }
   if (!is_valid(p, batch, p.key_col, r)) return false;
-  *key = read_i64(abuf(p, batch, p.key_col, 2), p.key_type, r);
+  const void* v = abuf(p, batch, p.key_col, 2);
+  if (p.key_type == GZ_TYPE_DOUBLE || p.key_type == GZ_TYPE_FLOAT) {
+    const int64_t w = p.key_type == GZ_TYPE_DOUBLE ? ((const int64_t*)v)[r]
+                                                   : __double_as_longlong((double)((const float*)v)[r]);
+    if (word_is_nan(w)) note_nan(p);
+    *key = norm_float_key(w);
+    return true;
+  }
+  *key = read_i64(v, p.key_type, r);
   return true;
 }
 
@@ -226,7 +277,15 @@ __device__ __forceinline__ uint32_t load_keys(const AggParams& p, int batch, int
   const void* const* bt = p.bufs + (int64_t)batch * p.bstride + p.key_col * 3;
   valid_bytes<RPT, S>((const uint8_t*)bt[0], r0, lr0, n, vx);
   load_words<RPT, S>(bt[2], p.key_type, r0, lr0, n, key);
-  return valid_mask<RPT, S>(vx, r0, lr0, n);
+  const uint32_t vm = valid_mask<RPT, S>(vx, r0, lr0, n);
+  if (p.key_type == GZ_TYPE_DOUBLE || p.key_type == GZ_TYPE_FLOAT) {  // load_words gives a float's double bits
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+      if (((vm >> k) & 1) && word_is_nan(key[k])) note_nan(p);
+      key[k] = norm_float_key(key[k]);
+    }
+  }
+  return vm;
 }
 
 // Slot of a key in an HBM table (insert if new); -1 when the table is full.
@@ -251,6 +310,21 @@ __device__ __forceinline__ int64_t table_slot(const AggInst& I, bool valid, int6
     s = (s + 1) & mask;
   }
   atomicOr(I.overflow, 1);
+  return -1;
+}
+
+// Slot of a key already in an HBM table; -1 when absent.
+__device__ __forceinline__ int64_t table_find(const AggInst& I, bool valid, int64_t key) {
+  if (!valid) return (int64_t)I.cap;
+  if (key == kEmpty) return (int64_t)I.cap + 1;
+  const uint64_t mask = I.cap - 1;
+  uint64_t s = fmix64((uint64_t)key) & mask;
+  for (uint64_t i = 0; i < I.cap; i++) {
+    const int64_t k = I.base[s];
+    if (k == key) return (int64_t)s;
+    if (k == kEmpty) return -1;
+    s = (s + 1) & mask;
+  }
   return -1;
 }
 
@@ -283,6 +357,7 @@ struct ArrowRow {
   }
   __device__ __forceinline__ uint64_t u64(int col) const { return ((const uint64_t*)abuf(p, batch, col, 2))[r]; }
   __device__ __forceinline__ int64_t raw64(int col) const { return ((const int64_t*)abuf(p, batch, col, 2))[r]; }
+  __device__ __forceinline__ int64_t rib() const { return r; }  // row in its batch
 };
 
 // Records are stored word-major (word w of record q at rec[w * rstride]); the
@@ -307,11 +382,13 @@ struct RecRow {
   }
   __device__ __forceinline__ uint64_t u64(int col) const { return (uint64_t)word(col); }
   __device__ __forceinline__ int64_t raw64(int col) const { return word(col); }
+  __device__ __forceinline__ int64_t rib() const { return 0; }  // records carry no batch row (no L_GLASTNAN here)
 };
 
 // Apply one row to one lane's state at ptr.
 template <class Row>
-__device__ __forceinline__ void apply_lane(const Lane ln, int64_t* ptr, const Row& row, int64_t grow) {
+__device__ __forceinline__ void apply_lane(const AggParams& p, const Lane ln, int64_t* ptr, const Row& row,
+                                           int64_t grow) {
   switch (ln.op) {
     case L_CNT: atomicAdd((unsigned long long*)ptr, 1ull); break;
     case L_NN:
@@ -363,9 +440,33 @@ __device__ __forceinline__ void apply_lane(const Lane ln, int64_t* ptr, const Ro
     case L_FIRSTNAN: {
       if (!row.valid(ln.col)) break;
       double d = row.f64(ln.col);
-      if (d != d && grow < *ptr) atomicMin((long long*)ptr, (long long)grow);
+      if (d != d) {
+        note_nan(p);
+        if (grow < *ptr) atomicMin((long long*)ptr, (long long)grow);
+      }
       break;
     }
+    case L_FIRSTZ: {
+      if (!row.valid(ln.col)) break;
+      const int64_t w = __double_as_longlong(row.f64(ln.col));
+      if ((w << 1) == 0) {
+        const int64_t z = grow * 2 + (int64_t)((uint64_t)w >> 63);
+        if (z < *ptr) atomicMin((long long*)ptr, (long long)z);
+      }
+      break;
+    }
+    case L_LASTNAN:
+    case L_GLASTNAN: {
+      if (!row.valid(ln.col)) break;
+      double d = row.f64(ln.col);
+      if (d != d) {
+        note_nan(p);
+        const int64_t o = ln.op == L_LASTNAN ? grow : glast_ord(grow, row.rib());
+        if (o > *ptr) atomicMax((long long*)ptr, (long long)o);
+      }
+      break;
+    }
+    case L_NANBITS: break;  // written by the fix-up pass
   }
 }
 
@@ -376,7 +477,7 @@ __device__ __forceinline__ void apply_row(const AggParams& p, int64_t* base, int
                                           const Row& row, int64_t grow) {
   int64_t* fr = base + S + slot;
   if (grow < *fr) atomicMin((long long*)fr, (long long)grow);
-  for (int l = 0; l < p.nl; l++) apply_lane(p.lanes[l], base + (2 + l) * S + slot, row, grow);
+  for (int l = 0; l < p.nl; l++) apply_lane(p, p.lanes[l], base + (2 + l) * S + slot, row, grow);
 }
 
 // Merge one LDS slot's lanes into an HBM slot.  The atomics return nothing
@@ -402,9 +503,13 @@ __device__ __forceinline__ void merge_slot(const AggParams& p, int64_t* g, int64
       case L_MINU: case L_MINF: if (v != -1) atomicMin((unsigned long long*)ptr, (unsigned long long)v); break;
       case L_MAXU: case L_MAXF: if (v != 0) atomicMax((unsigned long long*)ptr, (unsigned long long)v); break;
       case L_HASNAN: if (v) atomicOr((unsigned long long*)ptr, 1ull); break;
-      case L_FIRSTNN: case L_FIRSTNAN:
+      case L_FIRSTNN: case L_FIRSTNAN: case L_FIRSTZ:
         if (v != INT64_MAX) atomicMin((long long*)ptr, (long long)v);
         break;
+      case L_LASTNAN: case L_GLASTNAN:
+        if (v != -1) atomicMax((long long*)ptr, (long long)v);
+        break;
+      case L_NANBITS: break;
     }
   }
 }
@@ -469,6 +574,19 @@ __device__ __forceinline__ void lds_apply_records(const AggParams& p, int64_t* t
       case L_HASNAN: GZ_EACH(const double d = f64(k); atomicOr(up, ok(u, k) && d != d ? 1ull : 0ull)) break;
       case L_FIRSTNN: GZ_EACH(atomicMin(sp, ok(u, k) ? (long long)grow[k] : INT64_MAX)) break;
       case L_FIRSTNAN: GZ_EACH(const double d = f64(k); atomicMin(sp, ok(u, k) && d != d ? (long long)grow[k] : INT64_MAX)) break;
+      case L_FIRSTZ:
+        GZ_EACH(const int64_t w = __double_as_longlong(f64(k));
+                atomicMin(sp, ok(u, k) && (w << 1) == 0 ? (long long)(grow[k] * 2 + (int64_t)((uint64_t)w >> 63))
+                                                        : INT64_MAX))
+        break;
+      case L_LASTNAN: GZ_EACH(const double d = f64(k); atomicMax(sp, ok(u, k) && d != d ? (long long)grow[k] : -1ll)) break;
+      default: break;  // L_GLASTNAN (no group-by key: never partitioned), L_NANBITS (fix-up pass)
+    }
+    if (ln.op == L_FIRSTNAN || ln.op == L_LASTNAN) {
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < RPT; k++) any = any || (ok(u, k) && dv[k] != dv[k]);
+      if (any) note_nan(p);
     }
 #undef GZ_EACH
   }
@@ -622,6 +740,92 @@ __global__ __launch_bounds__(256) void agg_update_lds_kernel(AggParams p) {
   lds_merge(p, p.inst[cur], s_tab, LS);
 }
 
+// ---------------------------------------------------------------- fix-up
+// The bits of the NaN a result keeps: after a call's update kernels, every
+// valid NaN row of a column with an L_NANBITS lane looks up its group and, if
+// its ordinal is the one the group's row lane kept, stores its bits.  Each
+// group's row lane names one row, so one store per lane and group lands.  The
+// pass returns at once unless a NaN reached such a lane (the flag is sticky).
+__global__ __launch_bounds__(256) void agg_fix_kernel(AggParams p) {
+  if (*(volatile int*)p.fix_flag == 0) return;
+  const int64_t t = blockIdx.x;
+  if (t >= p.T) return;
+  const AggTile td = p.tiles[t];
+  const AggInst I = p.inst[td.inst];
+  const int64_t S = (int64_t)I.cap + 2;
+  for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
+    const int64_t r = (int64_t)td.row_begin + lr, grow = td.row_base + lr;
+    int64_t key;
+    const bool kvalid = row_key(p, td.batch, r, &key);
+    int64_t slot = -2;  // looked up on first need
+    for (int l = 0; l < p.nl; l++) {
+      const Lane ln = p.lanes[l];
+      if (ln.op != L_NANBITS || !is_valid(p, td.batch, ln.col, r)) continue;
+      const int ty = p.col_type[ln.col];
+      const int64_t raw = raw_bits(abuf(p, td.batch, ln.col, 2), ty, r);
+      if (!raw_is_nan(raw, ty)) continue;
+      if (slot == -2) slot = table_find(I, kvalid, key);
+      if (slot < 0) break;
+      int64_t want = grow, have;
+      if (ln.col2 < 0) {
+        have = I.base[S + slot];
+      } else {
+        if (p.lanes[ln.col2].op == L_GLASTNAN) want = glast_ord(grow, r);
+        have = I.base[(2 + ln.col2) * S + slot];
+      }
+      if (have == want) I.base[(2 + l) * S + slot] = raw;
+    }
+  }
+}
+
+// A batch of an aggregate without a group-by key.
+struct GBatch {
+  int32_t batch, inst;
+  int64_t nrows, base;  // rows; instance-global index of its first row
+};
+
+// Integer min/max without a group-by key merge arrow MinMax per batch
+// (MinAction::GetMinResult / MaxAction::GetMaxResult, ext/actions_impl.cc:995-1011,
+// :1605-1621): a batch whose input is all null yields a null scalar of value 0,
+// which is merged once an earlier batch has set the state (a non-null row before
+// the batch's first row).  One workgroup per batch.
+__global__ __launch_bounds__(256) void agg_allnull_kernel(AggParams p, const GBatch* gb, int nb) {
+  __shared__ int s_any;
+  const int b = blockIdx.x;
+  if (b >= nb) return;
+  const GBatch g = gb[b];
+  const AggInst I = p.inst[g.inst];
+  const int64_t S = (int64_t)I.cap + 2;
+  for (int l = 0; l < p.nl; l++) {
+    const Lane ln = p.lanes[l];
+    if (ln.op != L_MINI && ln.op != L_MAXI && ln.op != L_MINU) continue;  // unsigned max with 0 changes nothing
+    const uint8_t* vb = (const uint8_t*)abuf(p, g.batch, ln.col, 0);
+    if (!vb || g.nrows == 0) continue;  // no bitmap: every row valid
+    __syncthreads();
+    if (threadIdx.x == 0) s_any = 0;
+    __syncthreads();
+    const int64_t nbytes = (g.nrows + 7) / 8;
+    for (int64_t i = threadIdx.x; i < nbytes; i += blockDim.x) {
+      int m = vb[i];
+      if (i == nbytes - 1 && (g.nrows & 7)) m &= (1 << (g.nrows & 7)) - 1;
+      if (m) s_any = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && !s_any) {
+      int fl = -1;
+      for (int k = 0; k < p.nl; k++)
+        if (p.lanes[k].op == L_FIRSTNN && p.lanes[k].col == ln.col) fl = k;
+      const int64_t slot = table_find(I, true, 0);
+      if (fl >= 0 && slot >= 0 && I.base[(2 + fl) * S + slot] < g.base) {
+        int64_t* ptr = I.base + (2 + l) * S + slot;
+        if (ln.op == L_MINI) atomicMin((long long*)ptr, 0ll);
+        else if (ln.op == L_MAXI) atomicMax((long long*)ptr, 0ll);
+        else atomicMin((unsigned long long*)ptr, 0ull);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- fused join
 // Join -> aggregate in one pass (whole-stage codegen's conditionedProbeArraysInner
 // feeding hashAggregateArrays, whole_stage_codegen_kernel.cc:48-330): probe rows
@@ -692,6 +896,7 @@ struct JoinRow {
     const JoinIn& c = q.in[col];
     return c.side == 0 ? ((const int64_t*)c.bvalues)[br] : ((const int64_t*)pb(col, 2))[r];
   }
+  __device__ __forceinline__ int64_t rib() const { return 0; }  // a join aggregate always has a key
 };
 
 // A joined row's used input columns loaded once into registers (integers
@@ -720,6 +925,7 @@ struct RegRow {
   }
   __device__ __forceinline__ uint64_t u64(int col) const { return (uint64_t)word(col); }
   __device__ __forceinline__ int64_t raw64(int col) const { return word(col); }
+  __device__ __forceinline__ int64_t rib() const { return 0; }  // a join aggregate always has a key
 };
 
 // Word of a join input column for RegRow; *ok = not null.
@@ -837,6 +1043,72 @@ __global__ __launch_bounds__(256) void join_agg_update_kernel(AggParams p, JoinA
   }
   __syncthreads();
   lds_merge(p, I, s_tab, LS);
+}
+
+// Fix-up pass of the fused join -> aggregate (agg_fix_kernel's rule over joined
+// pairs): one workgroup per probe tile walks the relation again, and a NaN pair
+// whose probe row is the one its group's row lane kept stores its bits.  Pairs
+// of one probe row share the ordinal; their values are equal whenever that
+// matters (a build-side float min / max over repeated build keys is refused).
+__global__ __launch_bounds__(256) void join_agg_fix_kernel(AggParams p, JoinAggParams q) {
+  if (*(volatile int*)p.fix_flag == 0) return;
+  const int64_t t = blockIdx.x;
+  if (t >= q.T) return;
+  const JoinTile td = q.tiles[t];
+  const AggInst I = p.inst[0];
+  const int64_t GS = (int64_t)I.cap + 2;
+  const uint64_t jmask = q.jcap - 1;
+  const uint8_t* kv = (const uint8_t*)q.bufs[(int64_t)td.batch * q.bstride + q.pkey * 3];
+  const void* kd = q.bufs[(int64_t)td.batch * q.bstride + q.pkey * 3 + 2];
+  for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
+    const int64_t r = (int64_t)td.row_begin + lr;
+    if (kv && !bit_get(kv, r)) continue;
+    const int64_t key = read_i64(kd, q.key_type, r);
+    int64_t st, cnt;
+    if (key == kEmpty) {
+      if (q.special_count == 0) continue;
+      st = q.special_start;
+      cnt = q.jrowlist ? q.special_count : 1;
+    } else {
+      uint64_t s = fmix64((uint64_t)key) & jmask;
+      int64_t m = -1;
+      for (uint64_t i = 0; i < q.jcap; i++) {
+        const int64_t k = q.jkeys[s];
+        if (k == key) {
+          m = (int64_t)s;
+          break;
+        }
+        if (k == kEmpty) break;
+        s = (s + 1) & jmask;
+      }
+      if (m < 0) continue;
+      st = q.jstart[m];
+      cnt = q.jrowlist ? q.jcount[m] : 1;
+    }
+    const int64_t g = table_find(I, true, key);
+    if (g < 0) continue;
+    const int64_t grow = td.row_base + lr;
+    for (int64_t j = 0; j < cnt; j++) {
+      const int64_t br = q.jrowlist ? (int64_t)q.jrowlist[st + j] : st;
+      for (int l = 0; l < p.nl; l++) {
+        const Lane ln = p.lanes[l];
+        if (ln.op != L_NANBITS) continue;
+        const JoinIn& c = q.in[ln.col];
+        int64_t raw;
+        if (c.side == 0) {
+          if (!c.bvalid[br]) continue;
+          raw = raw_bits(c.bvalues, c.type, br);
+        } else {
+          const uint8_t* vb = (const uint8_t*)q.bufs[(int64_t)td.batch * q.bstride + c.idx * 3];
+          if (vb && !bit_get(vb, r)) continue;
+          raw = raw_bits(q.bufs[(int64_t)td.batch * q.bstride + c.idx * 3 + 2], c.type, r);
+        }
+        if (!raw_is_nan(raw, c.type)) continue;
+        const int64_t have = ln.col2 < 0 ? I.base[GS + g] : I.base[(2 + ln.col2) * GS + g];
+        if (have == grow) I.base[(2 + l) * GS + g] = raw;
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------- partitioned
@@ -1297,9 +1569,17 @@ struct ResultSpec {
   int kind;      // action kind
   int sub;       // 0 / 1 (sum_count yields two columns)
   int rtype;     // result type
-  int lane_a, lane_b, lane_c, lane_d;  // lanes used
+  int lane_a, lane_b, lane_c, lane_d, lane_e, lane_f;  // lanes used
   int nan_check;
   int global;    // no group-by key
+  // floating min / max: lane_a ordered value, lane_b non-null count, lane_c
+  // first zero (L_FIRSTZ), lane_d the NaN row lane, lane_e its bits
+  // (L_NANBITS), lane_f first non-null row; fmode says when the result is the
+  // kept NaN: 0 when no non-NaN value was seen (min with NaN check, and every
+  // min without a group-by key), 1 when any NaN was seen (max with NaN check),
+  // 2 when the first non-null value was NaN (plain compares).
+  // A float group key: lane_c first zero, lane_e the first row's NaN bits.
+  int fmode;
 };
 
 constexpr int kMaxResults = 40;
@@ -1347,10 +1627,25 @@ __global__ void agg_finish_kernel(FinishParams f) {
       int64_t iv = 0;
       double dv = 0;
       bool isf = false;
+      bool rawset = false;  // raw: the value's bits as stored
+      int64_t raw = 0;
       switch (rs.kind) {
         case GZ_ACTION_GROUPBY:
           ok = slot != (int64_t)I.cap;  // the null-key group
           iv = b[slot];
+          if (rs.rtype == GZ_TYPE_DOUBLE || rs.rtype == GZ_TYPE_FLOAT) {
+            // the group's first row's bits: the table holds -0.0 as 0.0 and
+            // one NaN for all
+            rawset = true;
+            if (iv == 0) {
+              const bool neg = (lane(rs.lane_c) & 1) != 0;
+              raw = rs.rtype == GZ_TYPE_DOUBLE ? (neg ? INT64_MIN : 0) : (neg ? 0x80000000ll : 0);
+            } else if (iv == kCanonNaN) {
+              raw = lane(rs.lane_e);
+            } else {
+              raw = rs.rtype == GZ_TYPE_DOUBLE ? iv : (int64_t)(uint32_t)__float_as_uint((float)__longlong_as_double(iv));
+            }
+          }
           break;
         case GZ_ACTION_COUNT:
         case GZ_ACTION_COUNT_LITERAL: iv = lane(rs.lane_a); break;
@@ -1382,19 +1677,21 @@ __global__ void agg_finish_kernel(FinishParams f) {
         case GZ_ACTION_MAX: {
           ok = lane(rs.lane_b) != 0;
           if (rs.rtype == GZ_TYPE_DOUBLE || rs.rtype == GZ_TYPE_FLOAT) {
-            isf = true;
-            uint64_t u = (uint64_t)lane(rs.lane_a);
-            bool has_value = rs.kind == GZ_ACTION_MIN ? u != ~0ull : u != 0ull;
+            const uint64_t u = (uint64_t)lane(rs.lane_a);
+            const bool has_value = rs.kind == GZ_ACTION_MIN ? u != ~0ull : u != 0ull;
             bool nan;
-            if (rs.nan_check) {
-              bool has_nan = lane(rs.lane_c) != 0;
-              nan = rs.kind == GZ_ACTION_MIN ? (!has_value && has_nan) : has_nan;
+            if (rs.fmode == 0) nan = !has_value;
+            else if (rs.fmode == 1) nan = lane(rs.lane_d) != -1;
+            else nan = lane(rs.lane_f) == lane(rs.lane_d);
+            rawset = true;
+            if (nan) {
+              raw = lane(rs.lane_e);
             } else {
-              // plain comparisons from the first value: NaN iff the first
-              // non-null value was NaN
-              nan = lane(rs.lane_d) != INT64_MAX && lane(rs.lane_d) == lane(rs.lane_c);
+              double d = has_value ? decode_ordered(u) : 0.0;
+              if (d == 0.0) d = (lane(rs.lane_c) & 1) ? -0.0 : 0.0;  // the first zero's sign
+              raw = rs.rtype == GZ_TYPE_DOUBLE ? __double_as_longlong(d)
+                                               : (int64_t)(uint32_t)__float_as_uint((float)d);
             }
-            dv = nan ? __longlong_as_double(0x7ff8000000000000ll) : (has_value ? decode_ordered(u) : 0.0);
           } else {
             iv = lane(rs.lane_a);
           }
@@ -1402,7 +1699,12 @@ __global__ void agg_finish_kernel(FinishParams f) {
         }
       }
       f.out_valid[c][i] = ok ? 1 : 0;
-      store_typed(f.out_values[c], i, rs.rtype, iv, dv, isf);
+      if (rawset) {
+        if (rs.rtype == GZ_TYPE_DOUBLE) ((int64_t*)f.out_values[c])[i] = raw;
+        else ((uint32_t*)f.out_values[c])[i] = (uint32_t)raw;
+      } else {
+        store_typed(f.out_values[c], i, rs.rtype, iv, dv, isf);
+      }
     }
   }
 }
@@ -1456,11 +1758,18 @@ struct gz_agg {
     unsigned grid_hbm = 0, blocks_hbm = 0, blocks_lds = 0;
     int lds = 0, lds_hbm = 0;
     int64_t total_slots = 0;
-    DBuf d_hbm, d_lds, d_slot_prefix, d_bufs;
+    DBuf d_hbm, d_lds, d_slot_prefix, d_bufs, d_fix, d_gb;
+    AggParams p_fix;
+    int64_t T_fix = 0;
+    int n_gb = 0;
     std::vector<int64_t> rows;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } L;
+  // value-keeping float min / max and float keys: the NaN flag of the fix-up
+  // pass; integer min / max without a group-by key: the all-null batch rule
+  DBuf d_fix, d_fix_tiles, d_gbatch;
+  bool need_fix = false, need_allnull = false, has_glast = false;
   int lds_threshold = 256;  // groups up to which an instance gets the LDS first level
   int part_groups = 512;    // groups per partition on the partitioned placement (0 = off)
   // partitioned placement buffers of the plan
@@ -1504,18 +1813,23 @@ static int build_plan(AggPlan& P, int ncols, const int32_t* types, int nact, con
     r.kind = a.kind;
     r.nan_check = a.nan_check;
     r.global = global;
-    r.lane_a = r.lane_b = r.lane_c = r.lane_d = 0;
+    r.lane_a = r.lane_b = r.lane_c = r.lane_d = r.lane_e = r.lane_f = 0;
+    r.fmode = 0;
     int t = col_ok(a.input0) ? types[a.input0] : -1;
     switch (a.kind) {
       case GZ_ACTION_GROUPBY:
         if (P.key_col >= 0) return fail(GZ_NOT_IMPLEMENTED, "multiple group-by keys");
         if (!col_ok(a.input0)) return fail(GZ_INVALID, "groupby input out of range");
-        if (!is_int_t(t))
+        if (!is_int_t(t) && !is_float_t(t))
           return fail(GZ_NOT_IMPLEMENTED, "group-by key type " + std::to_string(t) +
-                                              " (the HIP aggregate groups integer keys)");
+                                              " (the HIP aggregate groups integer and floating keys)");
         P.key_col = a.input0;
         P.key_type = t;
         r.rtype = t;
+        if (is_float_t(t)) {
+          r.lane_c = add_lane(P, L_FIRSTZ, a.input0, -1);
+          r.lane_e = add_lane(P, L_NANBITS, a.input0, -1);
+        }
         P.results.push_back(r);
         break;
       case GZ_ACTION_COUNT:
@@ -1564,16 +1878,27 @@ static int build_plan(AggPlan& P, int ncols, const int32_t* types, int nact, con
         r.rtype = t;
         r.lane_b = add_lane(P, L_NN, a.input0, -1);
         if (is_float_t(t)) {
+          // which NaN a result keeps (see ResultSpec::fmode)
           r.lane_a = add_lane(P, mn ? L_MINF : L_MAXF, a.input0, -1);
-          if (a.nan_check) r.lane_c = add_lane(P, L_HASNAN, a.input0, -1);
-          else {
-            r.lane_c = add_lane(P, L_FIRSTNAN, a.input0, -1);
-            r.lane_d = add_lane(P, L_FIRSTNN, a.input0, -1);
+          r.lane_c = add_lane(P, L_FIRSTZ, a.input0, -1);
+          if (mn && (a.nan_check || global)) {
+            r.fmode = 0;
+            r.lane_d = add_lane(P, L_FIRSTNAN, a.input0, -1);
+          } else if (!mn && a.nan_check) {
+            r.fmode = 1;
+            r.lane_d = add_lane(P, global ? L_GLASTNAN : L_LASTNAN, a.input0, -1);
+          } else {
+            r.fmode = 2;
+            r.lane_d = add_lane(P, L_FIRSTNAN, a.input0, -1);
+            r.lane_f = add_lane(P, L_FIRSTNN, a.input0, -1);
           }
+          r.lane_e = r.lane_d < 0 ? -1 : add_lane(P, L_NANBITS, a.input0, r.lane_d);
         } else if (t == GZ_TYPE_UINT64) {
           r.lane_a = add_lane(P, mn ? L_MINU : L_MAXU, a.input0, -1);
+          if (global) r.lane_f = add_lane(P, L_FIRSTNN, a.input0, -1);  // for the all-null batch rule
         } else if (is_int_t(t)) {
           r.lane_a = add_lane(P, mn ? L_MINI : L_MAXI, a.input0, -1);
+          if (global) r.lane_f = add_lane(P, L_FIRSTNN, a.input0, -1);
         } else {
           return fail(GZ_NOT_IMPLEMENTED, "min/max input type");
         }
@@ -1583,7 +1908,7 @@ static int build_plan(AggPlan& P, int ncols, const int32_t* types, int nact, con
       default:
         return fail(GZ_NOT_IMPLEMENTED, "action kind " + std::to_string(a.kind) + " is not implemented");
     }
-    if (r.lane_a < 0 || r.lane_b < 0 || r.lane_c < 0 || r.lane_d < 0)
+    if (r.lane_a < 0 || r.lane_b < 0 || r.lane_c < 0 || r.lane_d < 0 || r.lane_e < 0 || r.lane_f < 0)
       return fail(GZ_NOT_IMPLEMENTED, "too many aggregate states");
   }
   if ((int)P.results.size() > kMaxResults) return fail(GZ_NOT_IMPLEMENTED, "too many result columns");
@@ -1606,10 +1931,12 @@ static AggParams base_params(gz_agg* h) {
   for (int c = 0; c < h->plan.ncols; c++) p.col_type[c] = h->plan.types[c];
   p.lcap = 1024;
   p.lprobe = 32;
+  p.fix_flag = h->need_fix ? (int*)h->d_fix.p : nullptr;
   // used input columns of a partition record
   for (int c = 0; c < kAggMaxCols; c++) p.uslot[c] = -1;
   p.nu = 0;
   for (int l = 0; l < p.nl; l++) {
+    if (p.lanes[l].op == L_NANBITS) continue;  // read by the fix-up pass from the input; col2 names a lane
     for (int c : {p.lanes[l].col, p.lanes[l].col2}) {
       if (c < 0 || p.uslot[c] >= 0) continue;
       if (p.nu == kMaxUsed) {
@@ -1734,6 +2061,19 @@ static int agg_set_make(gz_context* ctx, int32_t ncols, const int32_t* types, in
     delete h;
     return rc;
   }
+  for (auto& l : h->plan.lanes) {
+    if (l.op == L_NANBITS) h->need_fix = true;
+    if (l.op == L_GLASTNAN) h->has_glast = true;
+    if (h->plan.key_col < 0 && (l.op == L_MINI || l.op == L_MAXI || l.op == L_MINU)) h->need_allnull = true;
+  }
+  if (h->need_fix) {
+    rc = ctx_alloc(ctx, 256, &h->d_fix);
+    if (rc) {
+      delete h;
+      return rc;
+    }
+    hipLaunchKernelGGL(fill_u32_kernel_agg, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)h->d_fix.p, (int64_t)1);
+  }
   gz_agg_options_defaults(&h->opts);
   if (opts) h->opts = *opts;
   if (h->opts.load_factor <= 0 || h->opts.load_factor > 0.9) h->opts.load_factor = 0.5;
@@ -1751,7 +2091,7 @@ static int agg_set_make(gz_context* ctx, int32_t ncols, const int32_t* types, in
 // Tile table + buffer table of a call.
 static int agg_stage(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of,
                      std::vector<AggTile>& tiles, std::vector<int64_t>& rows_per_inst, int tile_rows,
-                     DBuf* bufs_dst = nullptr) {
+                     DBuf* bufs_dst = nullptr, std::vector<GBatch>* gbatches = nullptr) {
   const int nc = h->plan.ncols;
   std::vector<const void*> bufs((size_t)nb * nc * 3, nullptr);
   rows_per_inst.assign(h->ninst, 0);
@@ -1761,6 +2101,8 @@ static int agg_stage(gz_agg* h, int32_t nb, const gz_batch* batches, const int32
     int ii = inst_of ? inst_of[bi] : 0;
     if (ii < 0 || ii >= h->ninst) return fail(GZ_INDEX_ERROR, "instance out of range");
     if (bt.num_rows < 0 || bt.num_rows > INT32_MAX) return fail(GZ_INVALID, "bad num_rows");
+    if (h->has_glast && (bt.num_rows >= (1 << 24) || base[ii] + bt.num_rows >= (1ll << 39)))
+      return fail(GZ_NOT_IMPLEMENTED, "max with NaN check and no group-by key: batches of 2^24 rows or more");
     int need = 0;
     for (int c = 0; c < nc; c++) need += type_num_bufs(h->plan.types[c]);
     if (bt.num_bufs < need) return fail(GZ_INVALID, "insufficient number of in_buf_addrs");
@@ -1788,6 +2130,7 @@ static int agg_stage(gz_agg* h, int32_t nb, const gz_batch* batches, const int32
       td.row_base = base[ii] + r0;
       tiles.push_back(td);
     }
+    if (gbatches && bt.num_rows > 0) gbatches->push_back(GBatch{bi, ii, bt.num_rows, base[ii]});
     base[ii] += bt.num_rows;
     rows_per_inst[ii] += bt.num_rows;
   }
@@ -1819,7 +2162,8 @@ static int agg_process(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   const int tile_rows = h->opts.tile_rows > 0 ? h->opts.tile_rows : (call_rows < (1 << 20) ? 512 : 4096);
   std::vector<AggTile> tiles;
   std::vector<int64_t> rows;
-  GZ_TRY(agg_stage(h, nb, batches, inst_of, tiles, rows, tile_rows));
+  std::vector<GBatch> gb;
+  GZ_TRY(agg_stage(h, nb, batches, inst_of, tiles, rows, tile_rows, nullptr, &gb));
   if (tiles.empty()) return GZ_OK;
   GZ_TRY(upload(ctx, &h->d_tiles, tiles.data(), sizeof(AggTile) * tiles.size()));
   AggParams p = base_params(h);
@@ -1944,6 +2288,18 @@ static int agg_process(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     KernelTimer kt(ctx, "agg_update_lds");
     hipLaunchKernelGGL(agg_update_lds_kernel, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, p);
     if (second.p) ctx_free(ctx, &second);
+  }
+  if (h->need_fix) {  // NaN bits of value-keeping results (returns at once without a NaN)
+    GZ_TRY(upload(ctx, &h->d_fix_tiles, tiles.data(), sizeof(AggTile) * tiles.size()));
+    AggParams pf = p;
+    pf.tiles = (const AggTile*)h->d_fix_tiles.p;
+    pf.T = (int64_t)tiles.size();
+    hipLaunchKernelGGL(agg_fix_kernel, dim3((unsigned)pf.T), dim3(256), 0, ctx->stream, pf);
+  }
+  if (h->need_allnull && !gb.empty()) {
+    GZ_TRY(upload(ctx, &h->d_gbatch, gb.data(), sizeof(GBatch) * gb.size()));
+    hipLaunchKernelGGL(agg_allnull_kernel, dim3((unsigned)gb.size()), dim3(256), 0, ctx->stream, p,
+                       (const GBatch*)h->d_gbatch.p, (int)gb.size());
   }
   GZ_HIP(hipGetLastError());
   for (int i = 0; i < h->ninst; i++) h->rows_seen[i] += rows[i];
@@ -2092,7 +2448,8 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   std::vector<int64_t> rows;
   // a prepared run aggregates its batches from scratch: instance rows start at 0
   std::fill(h->rows_seen.begin(), h->rows_seen.end(), 0);
-  GZ_TRY(agg_stage(h, nb, batches, inst_of, tiles, rows, tile_rows, &h->L.d_bufs));
+  std::vector<GBatch> gb;
+  GZ_TRY(agg_stage(h, nb, batches, inst_of, tiles, rows, tile_rows, &h->L.d_bufs, &gb));
   AggParams p = base_params(h);
   p.bufs = (const void* const*)h->L.d_bufs.p;  // the plan's own batch table
   p.bstride = h->plan.ncols * 3;
@@ -2257,6 +2614,18 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     }
     h->L.lds = lds;
   }
+  h->L.T_fix = 0;
+  h->L.n_gb = 0;
+  if (h->need_fix && !tiles.empty()) {
+    GZ_TRY(upload(ctx, &h->L.d_fix, tiles.data(), sizeof(AggTile) * tiles.size()));
+    h->L.p_fix = p;
+    h->L.p_fix.tiles = (const AggTile*)h->L.d_fix.p;
+    h->L.p_fix.T = h->L.T_fix = (int64_t)tiles.size();
+  }
+  if (h->need_allnull && !gb.empty()) {
+    GZ_TRY(upload(ctx, &h->L.d_gb, gb.data(), sizeof(GBatch) * gb.size()));
+    h->L.n_gb = (int)gb.size();
+  }
   h->L.rows = rows;
   h->L.ready = true;
   GZ_HIP(hipGetLastError());
@@ -2303,6 +2672,11 @@ static int agg_run_kernels(gz_agg* h) {
                        (const int64_t*)h->recs.p, (const int64_t*)h->pstart.p, (const int*)h->pinst.p, h->B,
                        h->part_chunk);
   }
+  if (h->L.T_fix)
+    hipLaunchKernelGGL(agg_fix_kernel, dim3((unsigned)h->L.T_fix), dim3(256), 0, st, h->L.p_fix);
+  if (h->L.n_gb)
+    hipLaunchKernelGGL(agg_allnull_kernel, dim3((unsigned)h->L.n_gb), dim3(256), 0, st, h->L.p_hbm,
+                       (const GBatch*)h->L.d_gb.p, h->L.n_gb);
   GZ_HIP(hipGetLastError());
   return GZ_OK;
 }
@@ -2474,13 +2848,18 @@ int gz_join_agg_make(gz_relation* rel, gz_context* ctx, int32_t num_probe_cols, 
   };
   const int kc = h->plan.key_col;
   if (kc < 0) return bail(fail(GZ_NOT_IMPLEMENTED, "a join aggregate groups by the join key"));
+  if (is_float_t(h->plan.key_type)) return bail(fail(GZ_NOT_IMPLEMENTED, "a join aggregate over a float join key"));
   const bool key_ok = (input_sides[kc] == 1 && input_indices[kc] == probe_key_col) ||
                       (input_sides[kc] == 0 && input_indices[kc] == v.key_col);
   if (!key_ok) return bail(fail(GZ_NOT_IMPLEMENTED, "a join aggregate groups by the join key"));
+  // the pairs of one probe row share its ordinal, so a value-keeping float
+  // min / max must not see different values of one probe row: refuse a build
+  // column when build keys repeat
   if (v.rowlist)
     for (auto& l : h->plan.lanes)
-      if (l.op == L_FIRSTNN || l.op == L_FIRSTNAN)
-        return bail(fail(GZ_NOT_IMPLEMENTED, "float min/max without NaN check over repeated build keys"));
+      if ((l.op == L_FIRSTNN || l.op == L_FIRSTNAN || l.op == L_FIRSTZ || l.op == L_LASTNAN ||
+           l.op == L_NANBITS) && l.col >= 0 && input_sides[l.col] == 0 && l.col != kc)
+        return bail(fail(GZ_NOT_IMPLEMENTED, "float min/max of a build column over repeated build keys"));
   h->join = new JoinAggState();
   h->join->rel = rel;
   h->join->ptypes.assign(probe_types, probe_types + num_probe_cols);
@@ -2597,6 +2976,8 @@ int gz_join_agg_process_batches(gz_agg* h, int32_t nb, const gz_batch* batches) 
     KernelTimer kt(ctx, "join_agg_update");
     hipLaunchKernelGGL(join_agg_update_kernel, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, p, q);
   }
+  if (p.fix_flag)
+    hipLaunchKernelGGL(join_agg_fix_kernel, dim3((unsigned)q.T), dim3(256), 0, ctx->stream, p, q);
   GZ_HIP(hipGetLastError());
   return GZ_OK;
 }
@@ -2687,7 +3068,8 @@ int gz_agg_release(gz_agg* h) {
   for (DBuf* b : {&h->d_inst, &h->d_lane_ops, &h->d_tiles, &h->d_bufs, &h->d_regs, &h->d_est, &h->skeys,
                   &h->svals, &h->tkeys, &h->tvals, &h->cursor, &h->ccpre, &h->counts, &h->sort_tmp, &h->L.d_hbm,
                   &h->L.d_lds, &h->L.d_slot_prefix, &h->L.d_bufs, &h->d_part, &h->pcount, &h->pstart, &h->pinst,
-                  &h->recs, &h->tcount, &h->tprefix, &h->tcsum, &h->ttot, &h->it0, &h->it1})
+                  &h->recs, &h->tcount, &h->tprefix, &h->tcsum, &h->ttot, &h->it0, &h->it1, &h->d_fix,
+                  &h->d_fix_tiles, &h->d_gbatch, &h->L.d_fix, &h->L.d_gb})
     ctx_free(ctx, b);
   if (h->L.exec) hipGraphExecDestroy(h->L.exec);
   if (h->L.graph) hipGraphDestroy(h->L.graph);

@@ -47,9 +47,11 @@ class HashRelation:
         self.finished = False
 
     def Evaluate(self, batch: DeviceBatch):
+        self.ctx.hold(batch)
         L.check(L.lib().gz_relation_append(self._h, batch.num_rows, batch.addrs, batch.sizes, batch.num_bufs))
 
     def EvaluateBatches(self, batches: Sequence[DeviceBatch]):
+        self.ctx.hold(list(batches))
         L.check(L.lib().gz_relation_append_batches(self._h, len(batches), make_batch_array(batches)))
 
     def Finish(self):
@@ -114,12 +116,14 @@ class ConditionedProbeKernel:
         return cls(relation, probe_types, probe_key, join_type, null_aware, result)
 
     def Process(self, batch: DeviceBatch) -> int:
+        self.rel.ctx.hold(batch)
         n = ctypes.c_int64()
         L.check(L.lib().gz_probe_process(self._h, batch.num_rows, batch.addrs, batch.sizes, batch.num_bufs,
                                          ctypes.byref(n)))
         return n.value
 
     def ProcessBatches(self, batches: Sequence[DeviceBatch], arr=None) -> Tuple[int, np.ndarray]:
+        self.rel.ctx.hold(list(batches))
         if arr is None:
             arr = make_batch_array(batches)
         n = ctypes.c_int64()
@@ -199,6 +203,7 @@ class JoinAggregateKernel:
     def ProcessBatches(self, batches: Sequence[DeviceBatch], arr=None):
         from .splitter import make_batch_array
 
+        self.kernel.ctx.hold(list(batches))
         if arr is None:
             arr = make_batch_array(batches)
         L.check(L.lib().gz_join_agg_process_batches(self.iterator._h, len(batches), arr))

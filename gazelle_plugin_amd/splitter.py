@@ -67,11 +67,13 @@ class Splitter:
     # -- splitting
     def Split(self, batch: DeviceBatch):
         """One batch, one launch sequence (the plain per-batch path)."""
+        self.ctx.hold(batch)
         L.check(L.lib().gz_split_batch(self._h, batch.num_rows, batch.addrs, batch.sizes, batch.num_bufs))
 
     def SplitBatches(self, batches: Sequence[DeviceBatch], arr=None):
         """Batched form: all batches in one launch sequence.  ``arr`` may be a
         prebuilt ``make_batch_array(batches)``."""
+        self.ctx.hold(list(batches))
         if arr is None:
             arr = make_batch_array(batches)
         L.check(L.lib().gz_split_batches(self._h, len(batches), arr))

@@ -15,6 +15,15 @@
  * A null-typed column contributes two (empty) buffers.  Buffers carry no
  * offset: row 0 is bit 0 / element 0.
  *
+ * Input buffers are borrowed in stream order: a call validates its arguments
+ * and queues device work on the context's stream, and that work reads the
+ * batch after the call returns.  The caller keeps every input buffer allocated
+ * and unchanged until the stream has passed the call: after
+ * gz_context_synchronize, or in work that the caller orders after it on the
+ * same stream (gz_context_set_stream).  The reference's Splitter copies a batch into pool
+ * memory during Split (splitter.cc:336-356); here the copy into the partition
+ * segments is itself the queued work.
+ *
  * Thread safety: a context may be shared by threads; a Splitter, aggregate,
  * hash relation or probe handle is used by one thread at a time (one Spark task
  * per handle, as in the reference).
@@ -129,8 +138,8 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
 
 /* Replaces Splitter::Split(const RecordBatch&) (splitter.cc:387-393) /
  * ShuffleSplitterJniWrapper.split (jni_wrapper.cc:1156-1195): one batch, one
- * launch sequence (the plain per-batch path).  The buffers are borrowed for the
- * duration of the call only. */
+ * launch sequence (the plain per-batch path).  The buffers are borrowed in
+ * stream order (see the top of this file). */
 int gz_split_batch(gz_splitter* h, int64_t num_rows, const int64_t* buf_addrs,
                    const int64_t* buf_sizes, int32_t num_bufs);
 

@@ -251,9 +251,40 @@ class CpuSplitter:
 
 # ------------------------------------------------------------------ aggregate
 
+class ResultColumn(tuple):
+    """(type, values, valid) of one result column; ``approx`` is True for a
+    floating sum or average (its summation order is free, so a check allows a
+    relative tolerance) and False for every other column, whose bits are the
+    reference's (min / max keep one input value, NaN payload and zero sign
+    included; group keys echo an input value)."""
+
+    approx = False
+
+
+_A_SUM, _A_AVG, _A_SUM_COUNT, _A_AVG_BY_COUNT = 3, 4, 7, 8
+
+
+def _approx_columns(actions) -> List[bool]:
+    out = []
+    for a in actions:
+        k = a[0]
+        if k == _A_SUM_COUNT:
+            out += [True, False]
+        else:
+            out.append(k in (_A_SUM, _A_AVG, _A_AVG_BY_COUNT))
+    return out
+
+
+def _result(t, vals, valid, approx: bool) -> ResultColumn:
+    r = ResultColumn((t, vals, valid))
+    r.approx = approx
+    return r
+
+
 def aggregate(batches, types: Sequence[int], actions: Sequence[Sequence[int]]) -> List:
     """actions: (kind, in0, in1, nan_check) tuples.  Returns [(type, values, valid)]
-    per result column, groups in first-seen order."""
+    per result column (``ResultColumn``), groups in first-seen order."""
+    approx = _approx_columns(actions)
     L = lib()
     L.gzo_agg_make.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -279,7 +310,7 @@ def aggregate(batches, types: Sequence[int], actions: Sequence[Sequence[int]]) -
                 valid = np.zeros(max(1, g), np.uint8)
                 L.gzo_agg_result_binary(h, c, _ptr(offs), _ptr(data), _ptr(valid))
                 vals = np.array([data[offs[i]:offs[i + 1]].tobytes() for i in range(g)] + [None], object)[:g]
-                out.append((t, vals, valid[:g].astype(bool)))
+                out.append(_result(t, vals, valid[:g].astype(bool), False))
                 continue
             vals = np.zeros(max(1, g) * _WIDTH.get(t, 8), np.uint8)
             valid = np.zeros(max(1, g), np.uint8)
@@ -287,7 +318,8 @@ def aggregate(batches, types: Sequence[int], actions: Sequence[Sequence[int]]) -
             dt = {2: np.uint8, 3: np.int8, 4: np.uint16, 5: np.int16, 6: np.uint32, 7: np.int32,
                   8: np.uint64, 9: np.int64, 11: np.float32, 12: np.float64, 16: np.int32,
                   17: np.int64, 18: np.int64}[t]
-            out.append((t, vals[: g * _WIDTH.get(t, 8)].view(dt).copy(), valid[:g].astype(bool)))
+            out.append(_result(t, vals[: g * _WIDTH.get(t, 8)].view(dt).copy(), valid[:g].astype(bool),
+                               approx[c] and t in (11, 12)))
         return out
     finally:
         L.gzo_agg_free(h)

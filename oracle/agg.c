@@ -53,6 +53,12 @@ static double get_f64(const void* v, int32_t t, int64_t r) {
   return (double)get_i64(v, t, r);
 }
 
+/* a float or double value's bits as stored (a float's 32 bits zero-extended) */
+static int64_t raw_bits(const void* v, int32_t t, int64_t r) {
+  if (t == T_FLOAT) return (int64_t)((const uint32_t*)v)[r];
+  return ((const int64_t*)v)[r];
+}
+
 typedef struct {
   int32_t kind, in0, in1, nan_check;
   int32_t in_type;
@@ -61,7 +67,10 @@ typedef struct {
   int64_t* isum;   /* integer sum, count, countLiteral */
   int64_t* cnt;    /* avg / sum_count / avgByCount counts */
   int64_t* imm;    /* integer min/max */
-  double* fmm;     /* floating min/max */
+  double* fmm;     /* floating min/max (value, for comparisons) */
+  int64_t* fraw;   /* floating min/max: the cached value's bits as stored (a
+                    * float's 32 bits zero-extended), so NaN payloads and the
+                    * sign of a zero come out as the reference keeps them */
   uint8_t* valid;  /* cache_validity_ */
   uint8_t* nullf;  /* UniqueAction null_flag_ */
   int64_t* key;    /* UniqueAction cache_ (bit pattern) */
@@ -132,7 +141,7 @@ static void grow_groups(gzo_agg* a, int64_t need) {
     action_t* x = &a->acts[i];
 #define GROW(f, T) do { x->f = (T*)realloc(x->f, sizeof(T) * nc); memset(x->f + a->gcap, 0, sizeof(T) * (nc - a->gcap)); } while (0)
     GROW(dsum, double); GROW(isum, int64_t); GROW(cnt, int64_t); GROW(imm, int64_t);
-    GROW(fmm, double); GROW(valid, uint8_t); GROW(nullf, uint8_t); GROW(key, int64_t);
+    GROW(fmm, double); GROW(fraw, int64_t); GROW(valid, uint8_t); GROW(nullf, uint8_t); GROW(key, int64_t);
     GROW(soff, int64_t); GROW(slen, int32_t);
 #undef GROW
   }
@@ -336,21 +345,22 @@ static void update(action_t* x, const gzo_batch* b, int64_t r, int32_t g) {
       if (is_null) break;
       if (is_float(t)) {
         double d = get_f64(v, t, r);
+        int64_t raw = raw_bits(v, t, r);
         if (x->kind == A_MIN) {
           /* MinAction (:905-936): LessNaNSafe when NaN_check */
-          if (!x->valid[g]) { x->fmm[g] = d; x->valid[g] = 1; }
+          if (!x->valid[g]) { x->fmm[g] = d; x->fraw[g] = raw; x->valid[g] = 1; }
           int less;
           if (x->nan_check) {
             int ln = isnan(d), rn = isnan(x->fmm[g]);
             less = (ln && rn) ? 0 : ln ? 0 : rn ? 1 : d < x->fmm[g];
           } else less = d < x->fmm[g];
-          if (less) x->fmm[g] = d;
+          if (less) { x->fmm[g] = d; x->fraw[g] = raw; }
         } else {
-          /* MaxAction (:1504-1539): a NaN is sticky when NaN_check */
-          if (x->nan_check && isnan(d)) { x->fmm[g] = d; x->valid[g] = 1; }
+          /* MaxAction (:1504-1539): every NaN overwrites when NaN_check */
+          if (x->nan_check && isnan(d)) { x->fmm[g] = d; x->fraw[g] = raw; x->valid[g] = 1; }
           else {
-            if (!x->valid[g]) { x->fmm[g] = d; x->valid[g] = 1; }
-            if (d > x->fmm[g]) x->fmm[g] = d;
+            if (!x->valid[g]) { x->fmm[g] = d; x->fraw[g] = raw; x->valid[g] = 1; }
+            if (d > x->fmm[g]) { x->fmm[g] = d; x->fraw[g] = raw; }
           }
         }
       } else if (t == T_UINT64) {
@@ -366,15 +376,74 @@ static void update(action_t* x, const gzo_batch* b, int64_t r, int32_t g) {
   }
 }
 
+/* Min/max of a batch with no group-by key: the action's Evaluate(in)
+ * (hash_aggregate_kernel.cc:818-839), which differs from the grouped lambdas.
+ * Floats (MinAction::GetMinResult, actions_impl.cc:976-993; MaxAction::
+ * GetMaxResult, :1581-1603): min always compares with LessNaNSafe, whatever
+ * NaN_check says; max with NaN_check takes a batch's first NaN and stops
+ * reading that batch.  Integers (:995-1011, :1605-1621): arrow MinMax of the
+ * batch, merged with `>` / `<`; an all-null batch yields a null scalar whose
+ * value is 0, which is merged all the same once an earlier batch has set the
+ * state. */
+static void global_min_max(action_t* x, const gzo_batch* b) {
+  const uint8_t* vb = (const uint8_t*)b->bufs[3 * x->in0];
+  const void* v = b->bufs[3 * x->in0 + 2];
+  const int32_t t = x->in_type;
+  const int mn = x->kind == A_MIN;
+  const int64_t n = b->num_rows;
+  if (is_float(t)) {
+    for (int64_t r = 0; r < n; r++) {
+      if (vb && !bit_get(vb, r)) continue;
+      double d = get_f64(v, t, r);
+      int64_t raw = raw_bits(v, t, r);
+      if (!mn && x->nan_check && isnan(d)) { x->fmm[0] = d; x->fraw[0] = raw; x->valid[0] = 1; break; }
+      if (!x->valid[0]) { x->fmm[0] = d; x->fraw[0] = raw; x->valid[0] = 1; }
+      int better;
+      if (mn) {
+        int ln = isnan(d), rn = isnan(x->fmm[0]);
+        better = (ln && rn) ? 0 : ln ? 0 : rn ? 1 : d < x->fmm[0];
+      } else better = d > x->fmm[0];
+      if (better) { x->fmm[0] = d; x->fraw[0] = raw; }
+    }
+    return;
+  }
+  int64_t nn = 0, bm = 0;
+  for (int64_t r = 0; r < n; r++) {
+    if (vb && !bit_get(vb, r)) continue;
+    int64_t i = get_i64(v, t, r);
+    if (t == T_UINT64) {
+      if (nn == 0 || (mn ? (uint64_t)i < (uint64_t)bm : (uint64_t)i > (uint64_t)bm)) bm = i;
+    } else if (nn == 0 || (mn ? i < bm : i > bm)) bm = i;
+    nn++;
+  }
+  if (nn > 0 && !x->valid[0]) {
+    x->valid[0] = 1;
+    x->imm[0] = bm;
+    return;
+  }
+  const int64_t val = nn > 0 ? bm : 0; /* the null scalar's value */
+  int take;
+  if (t == T_UINT64) take = mn ? (uint64_t)x->imm[0] > (uint64_t)val : (uint64_t)x->imm[0] < (uint64_t)val;
+  else take = mn ? x->imm[0] > val : x->imm[0] < val;
+  if (take) x->imm[0] = val;
+}
+
 int gzo_agg_process(gzo_agg* a, const gzo_batch* b) {
   int64_t n = b->num_rows;
   if (n == 0) return 0; /* ProcessAndCacheOne returns early on empty input (:808) */
-  int32_t* idx = (int32_t*)malloc(sizeof(int32_t) * n);
   if (a->key_col < 0) {
-    for (int64_t r = 0; r < n; r++) idx[r] = 0;
     a->ngroups = 1;
     a->global_seen = 1;
-  } else if (a->nkeys > 1) {
+    grow_groups(a, 1);
+    for (int i = 0; i < a->nactions; i++) {
+      action_t* x = &a->acts[i];
+      if (x->kind == A_MIN || x->kind == A_MAX) global_min_max(x, b);
+      else for (int64_t r = 0; r < n; r++) update(x, b, r, 0);
+    }
+    return 0;
+  }
+  int32_t* idx = (int32_t*)malloc(sizeof(int32_t) * n);
+  if (a->nkeys > 1) {
     /* several keys: HashAggregateResultIterator<StringType> with an UnsafeRow
      * key (hash_aggregate_kernel.cc:970-1000) */
     for (int64_t r = 0; r < n; r++) {
@@ -405,14 +474,20 @@ int gzo_agg_process(gzo_agg* a, const gzo_batch* b) {
         idx[r] = a->null_id;
       } else {
         int64_t key;
+        /* SparseHashMap<float/double> (sparse_hash_map.h:86-160): every NaN is
+         * one group (nan_index_); -0.0 and 0.0 hash alike (phmap Hash<double>,
+         * phmap_utils.h:251-258) and compare equal, so they are one group too.
+         * The group's key comes out as its first row's value (UniqueAction). */
         if (kt == T_DOUBLE) {
           double d = ((const double*)k)[r];
           if (isnan(d)) d = NAN;
+          if (d == 0.0) d = 0.0;
           memcpy(&key, &d, 8);
         } else if (kt == T_FLOAT) {
           float f = ((const float*)k)[r];
           int32_t fb;
           if (isnan(f)) f = NAN;
+          if (f == 0.0f) f = 0.0f;
           memcpy(&fb, &f, 4);
           key = fb;
         } else key = get_i64(k, kt, r);
@@ -506,7 +581,9 @@ void gzo_agg_result(gzo_agg* a, int col, void* values, uint8_t* valid) {
     }
     valid[g] = ok;
     uint8_t* dst = (uint8_t*)values + g * w;
-    if (x->kind == A_GROUPBY) {
+    if ((x->kind == A_MIN || x->kind == A_MAX) && is_float(x->in_type)) {
+      memcpy(dst, &x->fraw[g], (size_t)w); /* the kept value's bits */
+    } else if (x->kind == A_GROUPBY) {
       memcpy(dst, &iv, (size_t)w); /* the key's bits (a float key's as stored) */
     } else if (rt == T_DOUBLE) {
       double d = isf ? dv : (double)iv;
@@ -551,7 +628,7 @@ void gzo_agg_result_binary(gzo_agg* a, int col, int32_t* offsets, uint8_t* data,
 void gzo_agg_free(gzo_agg* a) {
   for (int i = 0; i < a->nactions; i++) {
     action_t* x = &a->acts[i];
-    free(x->dsum); free(x->isum); free(x->cnt); free(x->imm); free(x->fmm);
+    free(x->dsum); free(x->isum); free(x->cnt); free(x->imm); free(x->fmm); free(x->fraw);
     free(x->valid); free(x->nullf); free(x->key); free(x->soff); free(x->slen); free(x->sarena);
   }
   free(a->acts); free(a->types); free(a->mkeys); free(a->mids); free(a->mused);
