@@ -135,6 +135,7 @@ _SIGS = {
     "gz_ipc_read_batches": (ctypes.c_int, [ctypes.c_void_p, _I64, _I64, _I64, _I32, _PI32, _I32, _PI64, _PI64,
                                            _PI64, _PI32, _PI64]),
     "gz_ipc_decompress": (ctypes.c_int, [_P, ctypes.c_void_p, _I64, _I64, _I32, _PI32, ctypes.c_void_p, _I64]),
+    "gz_ipc_decompress_noalloc": (ctypes.c_int, [_P, ctypes.c_void_p, _I64, _I64, _I32, _PI32, ctypes.c_void_p, _I64]),
 }
 
 # symbols added by later operators are bound when present

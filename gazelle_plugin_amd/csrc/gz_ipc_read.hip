@@ -12,6 +12,8 @@
 // frame; those buffers are decoded on the device into a caller-provided arena.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -549,6 +551,76 @@ __global__ __launch_bounds__(256) void lz4_decode_kernel(const Lz4Task* __restri
 
 using namespace gz;
 
+// The decode jobs of a stream: one task per LZ4_FRAME buffer (largest first: a
+// wave decodes one buffer serially, so the tail is the longest buffer started
+// last) and the frame's blocks.
+struct Lz4Jobs {
+  std::vector<Lz4Task> tasks;
+  std::vector<Lz4Block> blocks;
+};
+
+static int lz4_jobs(gz_context* ctx, const uint8_t* stream, int64_t len, int64_t device_base, int32_t num_cols,
+                    const int32_t* col_types, void* arena, int64_t arena_capacity, Lz4Jobs* J) {
+  if (!ctx || !stream || len < 0 || !device_base || num_cols < 0 || (num_cols > 0 && !col_types))
+    return fail(GZ_INVALID, "gz_ipc_decompress: null argument");
+  ParsedStream ps;
+  GZ_TRY(ipc_parse(stream, len, num_cols, col_types, &ps));
+  if (ps.arena == 0) return GZ_OK;
+  if (!arena || arena_capacity < ps.arena) return fail(GZ_INVALID, "decompression arena smaller than the stream needs");
+  std::vector<Lz4Task> tasks;
+  for (const ParsedBuf& pb : ps.bufs) {
+    if (pb.where != kInArena) continue;
+    const int32_t b0 = (int32_t)J->blocks.size();
+    GZ_TRY(lz4_walk(stream, pb.src, pb.src_len, device_base, &J->blocks));
+    tasks.push_back({(uint8_t*)arena + pb.off, pb.size, b0, (int32_t)J->blocks.size() - b0});
+  }
+  std::vector<int32_t> order(tasks.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int32_t)i;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return tasks[a].cap > tasks[b].cap; });
+  J->tasks.resize(tasks.size());
+  for (size_t i = 0; i < order.size(); i++) J->tasks[i] = tasks[order[i]];
+  return GZ_OK;
+}
+
+// DecompressBuffer's statuses (jni_common.h:471-492) from the decoded lengths.
+static int lz4_check(const Lz4Jobs& J, const std::vector<int64_t>& prod) {
+  for (size_t i = 0; i < prod.size(); i++) {
+    if (prod[i] < 0) return fail(GZ_IO_ERROR, "LZ4 decompress failed: corrupt block data");
+    if (prod[i] != J.tasks[i].cap)
+      return fail(GZ_INVALID, "Failed to fully decompress buffer, expected " + std::to_string(J.tasks[i].cap) +
+                                  " bytes but decompressed " + std::to_string(prod[i]));
+  }
+  return GZ_OK;
+}
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ---- allocation-free decode (gz_ipc_decompress_noalloc)
+// Its task table, block table and decoded lengths live in this file-scope
+// device array instead of memory allocated per call.  Each GPU holds its own
+// copy until the process ends; in host code `scratch` names a host-side
+// stand-in, so the device address is taken once per device with
+// hipGetSymbolAddress (that device current) and only that address is used.
+constexpr int64_t kNoAllocScratchBytes = 8ll << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static uint8_t* g_scratch_ptr[kMaxScratchDevices];
+
+static int scratch_address(int device, uint8_t** out) {
+  if (device < 0 || device >= kMaxScratchDevices) return fail(GZ_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    uint8_t* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(GZ_HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return GZ_OK;
+}
+
 extern "C" {
 
 int gz_ipc_read_batches(const uint8_t* stream, int64_t len, int64_t device_base, int64_t arena_base, int32_t num_cols,
@@ -578,44 +650,26 @@ int gz_ipc_read_batches(const uint8_t* stream, int64_t len, int64_t device_base,
 
 int gz_ipc_decompress(gz_context* ctx, const uint8_t* stream, int64_t len, int64_t device_base, int32_t num_cols,
                       const int32_t* col_types, void* arena, int64_t arena_capacity) {
-  if (!ctx || !stream || len < 0 || !device_base || num_cols < 0 || (num_cols > 0 && !col_types))
-    return fail(GZ_INVALID, "gz_ipc_decompress: null argument");
-  ParsedStream ps;
-  GZ_TRY(ipc_parse(stream, len, num_cols, col_types, &ps));
-  if (ps.arena == 0) return GZ_OK;
-  if (!arena || arena_capacity < ps.arena) return fail(GZ_INVALID, "decompression arena smaller than the stream needs");
-  std::vector<Lz4Block> blocks;
-  std::vector<Lz4Task> tasks;
-  for (const ParsedBuf& pb : ps.bufs) {
-    if (pb.where != kInArena) continue;
-    const int32_t b0 = (int32_t)blocks.size();
-    GZ_TRY(lz4_walk(stream, pb.src, pb.src_len, device_base, &blocks));
-    tasks.push_back({(uint8_t*)arena + pb.off, pb.size, b0, (int32_t)blocks.size() - b0});
-  }
-  // largest buffers first: a wave decodes one buffer serially, so the tail is
-  // the longest buffer started last
-  std::vector<int32_t> order(tasks.size());
-  for (size_t i = 0; i < order.size(); i++) order[i] = (int32_t)i;
-  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return tasks[a].cap > tasks[b].cap; });
-  std::vector<Lz4Task> sorted(tasks.size());
-  for (size_t i = 0; i < order.size(); i++) sorted[i] = tasks[order[i]];
+  Lz4Jobs J;
+  GZ_TRY(lz4_jobs(ctx, stream, len, device_base, num_cols, col_types, arena, arena_capacity, &J));
+  if (J.tasks.empty()) return GZ_OK;
   GZ_HIP(hipSetDevice(ctx->device));
   DBuf d_tasks, d_blocks, d_prod;
   int rc = GZ_OK;
-  if ((rc = upload(ctx, &d_tasks, sorted.data(), sizeof(Lz4Task) * sorted.size())) ||
-      (rc = upload(ctx, &d_blocks, blocks.data(), sizeof(Lz4Block) * std::max<size_t>(blocks.size(), 1))) ||
-      (rc = ctx_alloc(ctx, sizeof(int64_t) * sorted.size(), &d_prod))) {
+  if ((rc = upload(ctx, &d_tasks, J.tasks.data(), sizeof(Lz4Task) * J.tasks.size())) ||
+      (rc = upload(ctx, &d_blocks, J.blocks.data(), sizeof(Lz4Block) * std::max<size_t>(J.blocks.size(), 1))) ||
+      (rc = ctx_alloc(ctx, sizeof(int64_t) * J.tasks.size(), &d_prod))) {
     ctx_free(ctx, &d_tasks);
     ctx_free(ctx, &d_blocks);
     return rc;
   }
-  const int nt = (int)sorted.size();
+  const int nt = (int)J.tasks.size();
   {
     KernelTimer kt(ctx, "ipc_lz4_decode");
     hipLaunchKernelGGL(lz4_decode_kernel, dim3((nt + 3) / 4), dim3(256), 0, ctx->stream,
                        (const Lz4Task*)d_tasks.p, nt, (const Lz4Block*)d_blocks.p, (int64_t*)d_prod.p);
   }
-  std::vector<int64_t> prod(sorted.size());
+  std::vector<int64_t> prod(J.tasks.size());
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(prod.data(), d_prod.p, sizeof(int64_t) * prod.size(), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -623,13 +677,44 @@ int gz_ipc_decompress(gz_context* ctx, const uint8_t* stream, int64_t len, int64
   ctx_free(ctx, &d_blocks);
   ctx_free(ctx, &d_prod);
   if (e != hipSuccess) return fail(GZ_HIP_ERROR, std::string("gz_ipc_decompress: ") + hipGetErrorString(e));
-  for (size_t i = 0; i < prod.size(); i++) {
-    if (prod[i] < 0) return fail(GZ_IO_ERROR, "LZ4 decompress failed: corrupt block data");
-    if (prod[i] != sorted[i].cap)
-      return fail(GZ_INVALID, "Failed to fully decompress buffer, expected " + std::to_string(sorted[i].cap) +
-                                  " bytes but decompressed " + std::to_string(prod[i]));
+  return lz4_check(J, prod);
+}
+
+int gz_ipc_decompress_noalloc(gz_context* ctx, const uint8_t* stream, int64_t len, int64_t device_base,
+                              int32_t num_cols, const int32_t* col_types, void* arena, int64_t arena_capacity) {
+  Lz4Jobs J;
+  GZ_TRY(lz4_jobs(ctx, stream, len, device_base, num_cols, col_types, arena, arena_capacity, &J));
+  if (J.tasks.empty()) return GZ_OK;
+  const size_t tb = align_up(sizeof(Lz4Task) * J.tasks.size(), 256);
+  const size_t bb = align_up(sizeof(Lz4Block) * J.blocks.size(), 256);
+  const size_t pb = sizeof(int64_t) * J.tasks.size();
+  if (tb + bb + pb > (size_t)kNoAllocScratchBytes)
+    return fail(GZ_CAPACITY_ERROR, "gz_ipc_decompress_noalloc: " + std::to_string(J.tasks.size()) + " buffers and " +
+                                       std::to_string(J.blocks.size()) + " LZ4 blocks need " +
+                                       std::to_string(tb + bb + pb) + " bytes of scratch, more than its " +
+                                       std::to_string(kNoAllocScratchBytes));
+  GZ_HIP(hipSetDevice(ctx->device));
+  uint8_t* sp = nullptr;
+  GZ_TRY(scratch_address(ctx->device, &sp));
+  Lz4Task* d_tasks = (Lz4Task*)sp;
+  Lz4Block* d_blocks = (Lz4Block*)(sp + tb);
+  int64_t* d_prod = (int64_t*)(sp + tb + bb);  // every task's word is written by the kernel
+  GZ_HIP(hipMemcpyAsync(d_tasks, J.tasks.data(), sizeof(Lz4Task) * J.tasks.size(), hipMemcpyHostToDevice, ctx->stream));
+  if (!J.blocks.empty())
+    GZ_HIP(hipMemcpyAsync(d_blocks, J.blocks.data(), sizeof(Lz4Block) * J.blocks.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+  const int nt = (int)J.tasks.size();
+  {
+    KernelTimer kt(ctx, "ipc_lz4_decode");
+    hipLaunchKernelGGL(lz4_decode_kernel, dim3((nt + 3) / 4), dim3(256), 0, ctx->stream, (const Lz4Task*)d_tasks, nt,
+                       (const Lz4Block*)d_blocks, d_prod);
   }
-  return GZ_OK;
+  std::vector<int64_t> prod(J.tasks.size());
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(prod.data(), d_prod, pb, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail(GZ_HIP_ERROR, std::string("gz_ipc_decompress_noalloc: ") + hipGetErrorString(e));
+  return lz4_check(J, prod);
 }
 
 }  // extern "C"

@@ -250,6 +250,19 @@ int gz_ipc_read_batches(const uint8_t* stream, int64_t len, int64_t device_base,
 int gz_ipc_decompress(gz_context* ctx, const uint8_t* stream, int64_t len, int64_t device_base, int32_t num_cols,
                       const int32_t* col_types, void* arena, int64_t arena_capacity);
 
+/* gz_ipc_decompress without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The caller passes the same inputs and the output arena; the decode's task
+ * table, block table and decoded lengths live in a library-owned 8 MiB device
+ * array per GPU (allocated with the library, kept until the process ends), so
+ * a stream may hold at most as many LZ4 buffers and blocks as fit in it
+ * (32 B per compressed buffer + 16 B per LZ4 block, each table 256-byte
+ * aligned): more is CapacityError.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other threads, other streams) race. */
+int gz_ipc_decompress_noalloc(gz_context* ctx, const uint8_t* stream, int64_t len, int64_t device_base,
+                              int32_t num_cols, const int32_t* col_types, void* arena, int64_t arena_capacity);
+
 /* ---- hash aggregate ----------------------------------------------------------
  * Replaces HashAggregateKernel + HashAggregateResultIterator<number>
  * (ext/hash_aggregate_kernel.cc:51-928, kernels_ext.h:285-313).  The Gandiva
