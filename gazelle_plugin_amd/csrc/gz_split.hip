@@ -102,8 +102,11 @@ __device__ __forceinline__ void load_tile_ptrs(const SplitParams& p, int batch, 
 }
 
 // Spark hash of key column c at row r, chained through h (Murmur3Hash semantics:
-// a null value leaves the running hash unchanged).
-__device__ __forceinline__ uint32_t hash_key(const void* const* bp, int c, int t, int64_t r, uint32_t h) {
+// a null value leaves the running hash unchanged).  A float / double key hashes
+// its bits (hashInt(floatToIntBits) / hashLong(doubleToLongBits)); -0.0 and NaN,
+// whose Spark hash differs between versions and is not fixed by anything the
+// reference holds, set bit 4 of *err, and the split then fails at Stop.
+__device__ __forceinline__ uint32_t hash_key(const void* const* bp, int c, int t, int64_t r, uint32_t h, int* err) {
   const uint8_t* vb = (const uint8_t*)bp[c * 3 + 0];
   if (vb && !bit_get(vb, r)) return h;
   const void* v = bp[c * 3 + 2];
@@ -115,17 +118,13 @@ __device__ __forceinline__ uint32_t hash_key(const void* const* bp, int c, int t
     case GZ_TYPE_INT8: return spark_hash_int((uint32_t)(int32_t)gld((const int8_t*)v + r), h);
     case GZ_TYPE_INT16: return spark_hash_int((uint32_t)(int32_t)gld((const int16_t*)v + r), h);
     case GZ_TYPE_FLOAT: {
-      float f = gld((const float*)v + r);
-      uint32_t bits;
-      if (f == 0.0f) f = 0.0f;
-      if (f != f) bits = 0x7fc00000u; else bits = __float_as_uint(f);
+      const uint32_t bits = (uint32_t)gld((const int32_t*)v + r);
+      if (bits == 0x80000000u || (bits & 0x7fffffffu) > 0x7f800000u) atomicOr(err, 4);
       return spark_hash_int(bits, h);
     }
     case GZ_TYPE_DOUBLE: {
-      double d = gld((const double*)v + r);
-      uint64_t bits;
-      if (d == 0.0) d = 0.0;
-      if (d != d) bits = 0x7ff8000000000000ull; else bits = (uint64_t)__double_as_longlong(d);
+      const uint64_t bits = (uint64_t)gld((const int64_t*)v + r);
+      if (bits == 0x8000000000000000ull || (bits & 0x7fffffffffffffffull) > 0x7ff0000000000000ull) atomicOr(err, 4);
       return spark_hash_long(bits, h);
     }
     case GZ_TYPE_STRING:
@@ -153,7 +152,7 @@ __device__ __forceinline__ int compute_pid(const SplitParams& p, const void* con
   int64_t r = (int64_t)td.row_begin + lr;
   if (p.mode == 0) {
     uint32_t h = p.seed;
-    for (int k = 0; k < p.nkeys; k++) h = hash_key(bp, p.key_col[k], p.key_type[k], r, h);
+    for (int k = 0; k < p.nkeys; k++) h = hash_key(bp, p.key_col[k], p.key_type[k], r, h, p.err);
     return pmod_hash(h, p.fm);
   }
   if (p.mode == 1) return (int)fastmod_u32((uint32_t)(td.rr + lr), p.fm);
@@ -1237,6 +1236,9 @@ static int resolve_segment(gz_splitter* h, SplitSegment* s) {
   GZ_HIP(hipStreamSynchronize(ctx->stream));
   if (err & 1) return fail(GZ_INVALID, "Partition id is equal or greater than num_partitions");
   if (err & 2) return fail(GZ_CAPACITY_ERROR, "a partition's utf8 data exceeds 2 GiB");
+  if (err & 4)
+    return fail(GZ_NOT_IMPLEMENTED,
+                "a float or double hash partitioning key holds -0.0 or NaN (its Spark hash is version-dependent)");
   s->resolved = true;
   return GZ_OK;
 }

@@ -101,19 +101,22 @@ static int key_hash(const gzo_batch* b, int c, int32_t t, int64_t r, int32_t h, 
     case T_INT16: return gzo_hash_int(((const int16_t*)v)[r], h);
     case T_INT32: case T_DATE32: return gzo_hash_int(((const int32_t*)v)[r], h);
     case T_INT64: return gzo_hash_long(((const int64_t*)v)[r], h);
+    /* Spark hashes a float as hashInt(floatToIntBits(f)) and a double as
+     * hashLong(doubleToLongBits(d)) (hash32_spark / hash64_spark).  -0.0 and NaN
+     * are refused (status 10): whether they hash as 0.0 / one canonical NaN
+     * depends on the Spark version and on the Gandiva fork, which is not here. */
     case T_FLOAT: {
-      float f = ((const float*)v)[r];
-      int32_t bits;
-      if (f == 0.0f) f = 0.0f;
-      if (f != f) bits = 0x7fc00000; else memcpy(&bits, &f, 4);
-      return gzo_hash_int(bits, h);
+      uint32_t bits = ((const uint32_t*)v)[r];
+      if (bits == 0x80000000u || (bits & 0x7fffffffu) > 0x7f800000u) { *ok = 0; return h; }
+      return gzo_hash_int((int32_t)bits, h);
     }
     case T_DOUBLE: {
-      double d = ((const double*)v)[r];
-      int64_t bits;
-      if (d == 0.0) d = 0.0;
-      if (d != d) bits = 0x7ff8000000000000LL; else memcpy(&bits, &d, 8);
-      return gzo_hash_long(bits, h);
+      uint64_t bits = ((const uint64_t*)v)[r];
+      if (bits == 0x8000000000000000ULL || (bits & 0x7fffffffffffffffULL) > 0x7ff0000000000000ULL) {
+        *ok = 0;
+        return h;
+      }
+      return gzo_hash_long((int64_t)bits, h);
     }
     case T_STRING: case T_BINARY: {
       const int32_t* off = (const int32_t*)BUF(b, c, 1);

@@ -25,6 +25,23 @@ def random_column(rng, t: int, n: int, null_frac: float = 0.0) -> HostColumn:
     return HostColumn(t, raw.view(T.numpy_dtype(t)) if t != T.DECIMAL128 else raw, vb)
 
 
+def ordinary_float_keys(batches, cols):
+    """Replace -0.0 and NaN bit patterns in float / double key columns (whose
+    Spark hash the split refuses) by other values, in place."""
+    for b in batches:
+        for c in cols:
+            col = b.columns[c]
+            if col.type_id == T.DOUBLE:
+                u, sign, exp = col.values.view(np.uint64), np.uint64(1 << 63), np.uint64(0x7FF0000000000000)
+            elif col.type_id == T.FLOAT:
+                u, sign, exp = col.values.view(np.uint32), np.uint32(1 << 31), np.uint32(0x7F800000)
+            else:
+                continue
+            u[(u & exp) == exp] ^= exp & (exp >> u.dtype.type(1))  # drop the exponent's top bit: finite
+            u[u == sign] = 0
+    return batches
+
+
 def random_batches(types, sizes, seed=0, null_frac=0.1):
     rng = np.random.default_rng(seed)
     return [HostBatch(n, [random_column(rng, t, n, null_frac if t != T.NA else 0) for t in types])

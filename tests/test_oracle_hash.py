@@ -84,3 +84,90 @@ def test_null_key_keeps_seed():
     pids = oracle.split_pids([b], [T.INT64], 200, [0])
     assert pids[0] == 42 % 200
     assert pids[1] == mm3(np.int64(7).tobytes(), 42) % 200
+
+
+# ---- Spark hashUnsafeBytes (utf8 / binary keys, hashbuf_spark)
+#
+# Spark's Murmur3_x86_32.hashUnsafeBytes mixes the whole 4-byte words as
+# Murmur3 does, then every tail byte on its own as one more block holding the
+# byte sign-extended to an int (Platform.getByte), and finishes with
+# fmix(h, length).  Murmur3 over the same words with each tail byte expanded to
+# that 4-byte block differs only in the length fmix xors in; fmix's mixing part
+# is invertible, so the expected value follows from scikit-learn's independent
+# Murmur3 over the expanded bytes: undo its finalizer, swap the length, redo it.
+
+M32 = 0xFFFFFFFF
+
+
+def _fmix_core(h):
+    h ^= h >> 16
+    h = (h * 0x85EBCA6B) & M32
+    h ^= h >> 13
+    h = (h * 0xC2B2AE35) & M32
+    return h ^ (h >> 16)
+
+
+def _fmix_core_inverse(h):
+    h ^= h >> 16
+    h = (h * pow(0xC2B2AE35, -1, 1 << 32)) & M32
+    h ^= (h >> 13) ^ (h >> 26)
+    h = (h * pow(0x85EBCA6B, -1, 1 << 32)) & M32
+    return h ^ (h >> 16)
+
+
+def spark_hash_unsafe_bytes(b: bytes, seed: int) -> int:
+    q = len(b) - len(b) % 4
+    expanded = b[:q] + b"".join(int(np.int8(np.uint8(x))).to_bytes(4, "little", signed=True) for x in b[q:])
+    h = _fmix_core_inverse(mm3(expanded, seed & M32) & M32) ^ len(expanded)  # Murmur3's state before fmix
+    return int(np.int32(np.uint32(_fmix_core(h ^ len(b)))))
+
+
+def test_fmix_inverse_is_an_inverse():
+    rng = np.random.default_rng(11)
+    for x in [0, 1, M32, 0x80000000] + [int(v) for v in rng.integers(0, 1 << 32, 2000, dtype=np.uint64)]:
+        assert _fmix_core(_fmix_core_inverse(x)) == x and _fmix_core_inverse(_fmix_core(x)) == x
+
+
+@pytest.mark.parametrize("seed", [42, 0, -7])
+def test_hash_bytes_tail_is_spark_hash_unsafe_bytes(seed):
+    """Every length 0..40 (every tail length), bytes >= 0x80 in the tail (the
+    sign extension), and chained seeds as a multi-key hash uses them."""
+    rng = np.random.default_rng(12 + (seed & 0xFF))
+    for n in range(41):
+        for _ in range(25):
+            b = bytes(rng.integers(0, 256, size=n, dtype=np.uint8))
+            assert oracle.hash_bytes(b, seed) == spark_hash_unsafe_bytes(b, seed), (n, b.hex())
+    for b in (b"\x80", b"\xff\xff\xff", b"abcd\x80", b"a", b"ab", b"abc", "é".encode()):
+        assert oracle.hash_bytes(b, seed) == spark_hash_unsafe_bytes(b, seed)
+
+
+def test_tail_rule_differs_from_standard_murmur3():
+    """The pin is not vacuous: standard Murmur3 (zero-extended, one tail block)
+    disagrees with Spark's rule on tails, and so would a zero-extension."""
+    b = b"abcde\xf0"
+    assert spark_hash_unsafe_bytes(b, 42) != mm3(b, 42)
+    zero_ext = b[:4] + b"".join(x.to_bytes(4, "little") for x in b[4:])
+    h = _fmix_core_inverse(mm3(zero_ext, 42) & M32) ^ len(zero_ext)
+    assert spark_hash_unsafe_bytes(b, 42) != int(np.int32(np.uint32(_fmix_core(h ^ len(b)))))
+
+
+# ---- float / double keys (hash32_spark / hash64_spark on floatToIntBits / doubleToLongBits)
+
+def test_float_and_double_keys_hash_their_bits():
+    """An ordinary float or double key hashes as Murmur3 over its little-endian
+    bits; -0.0 and NaN (version-dependent in Spark) are refused."""
+    from gazelle_plugin_amd import types as T
+    from gazelle_plugin_amd.batch import HostBatch, HostColumn
+
+    rng = np.random.default_rng(13)
+    d = np.concatenate([rng.standard_normal(500) * 1e6, [0.0, 1.0, -1.0, np.inf, -np.inf, 5e-324]])
+    b = HostBatch(len(d), [HostColumn(T.DOUBLE, d)])
+    assert list(oracle.split_pids([b], [T.DOUBLE], 200, [0])) == [mm3(x.tobytes(), 42) % 200 for x in d]
+    f = d.astype(np.float32)
+    b = HostBatch(len(f), [HostColumn(T.FLOAT, f)])
+    assert list(oracle.split_pids([b], [T.FLOAT], 200, [0])) == [mm3(x.tobytes(), 42) % 200 for x in f]
+    for bad in (-0.0, np.nan):
+        for t, dt in ((T.DOUBLE, np.float64), (T.FLOAT, np.float32)):
+            b = HostBatch(2, [HostColumn(t, np.array([1.0, bad], dt))])
+            with pytest.raises(ValueError):
+                oracle.split_pids([b], [t], 200, [0])

@@ -9,7 +9,7 @@ from gazelle_plugin_amd.batch import DeviceBatch, HostBatch, HostColumn
 from gazelle_plugin_amd.splitter import SplitOptions, Splitter
 from gazelle_plugin_amd._lib import GazelleError, Invalid
 from golden import fixtures as F
-from splitcheck import assert_layout_equal, random_batches
+from splitcheck import ordinary_float_keys, assert_layout_equal, random_batches
 
 pytestmark = pytest.mark.gpu
 
@@ -58,7 +58,7 @@ def test_split_partition_counts(ctx, P):
 @pytest.mark.parametrize("key_type", [T.INT32, T.INT16, T.INT8, T.BOOL, T.DATE32, T.DOUBLE, T.FLOAT, T.STRING])
 def test_split_key_types_match_oracle(ctx, key_type):
     types = [key_type, T.INT64]
-    batches = random_batches(types, [3000, 123], seed=key_type, null_frac=0.1)
+    batches = ordinary_float_keys(random_batches(types, [3000, 123], seed=key_type, null_frac=0.1), [0])
     exp = oracle.split(batches, types, 200, [0])
     got = run_gpu(ctx, batches, types, 200, [0])
     assert_layout_equal(got, exp, types)
@@ -201,3 +201,25 @@ def test_batches_sharing_one_utf8_buffer(ctx):
     got = sp.read_partitions()
     sp.close()
     assert_layout_equal(got, exp, types)
+
+
+@pytest.mark.parametrize("key_type", [T.DOUBLE, T.FLOAT])
+@pytest.mark.parametrize("bits", ["neg_zero", "nan"])
+def test_split_refuses_unpinned_float_keys(ctx, key_type, bits):
+    """-0.0 and NaN hash differently across Spark versions: Stop reports
+    NotImplemented, as the oracle refuses them."""
+    from gazelle_plugin_amd import _lib as L
+
+    types = [key_type, T.INT64]
+    batches = ordinary_float_keys(random_batches(types, [500], seed=3, null_frac=0.0), [0])
+    vals = batches[0].columns[0].values
+    u = vals.view(np.uint64 if key_type == T.DOUBLE else np.uint32)
+    u[123] = (1 << 63 if key_type == T.DOUBLE else 1 << 31) if bits == "neg_zero" else \
+        (0x7FF8000000000001 if key_type == T.DOUBLE else 0x7FC00001)
+    with pytest.raises(ValueError):
+        oracle.split(batches, types, 200, [0])
+    sp = Splitter.Make(ctx, "hash", types, 200, [0])
+    sp.SplitBatches([DeviceBatch.from_host(b) for b in batches])
+    with pytest.raises(L.NotImplementedStatus):
+        sp.Stop()
+    sp.close()
