@@ -151,54 +151,61 @@ __global__ void join_table_init_kernel(int64_t* keys, int32_t* start, int32_t* c
 //        [3] rows with key kJEmpty, [4] first such row
 __global__ void join_build_kernel(const uint8_t* kvalid, const uint8_t* kvals, int ktype, int kwidth,
                                   int64_t n, int64_t* keys, int32_t* start, int32_t* count,
-                                  uint64_t cap, int32_t* row_slot, int skip_dup,
+                                  uint64_t cap, int32_t* row_slot, int skip_dup, int codes,
                                   unsigned long long* flags) {
   const uint64_t mask = cap - 1;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += (int64_t)gridDim.x * blockDim.x) {
     row_slot[r] = -1;
+    bool fresh = false;  // this row inserted a new key
     if (!kvalid[r]) {
       flags[0] = 1;
-      continue;
-    }
-    bool nan;
-    int64_t key;
-    if (ktype == GZ_TYPE_BOOL) {
-      key = kvals[r];
-      nan = false;
     } else {
-      key = join_key(kvals, ktype, r, &nan);
-    }
-    if (nan) continue;  // a NaN build key can never be matched
-    if (key == kJEmpty) {
-      unsigned long long c = atomicAdd(&flags[3], 1ull);
-      atomicMin((long long*)&flags[4], (long long)r);
-      if (c >= 1) flags[1] = 1;
-      row_slot[r] = -2;
-      continue;
-    }
-    uint64_t s = jmix64((uint64_t)key) & mask;
-    for (uint64_t i = 0; i < cap; i++) {
-      int64_t k = keys[s];
-      if (k == kJEmpty) {
-        unsigned long long prev = atomicCAS((unsigned long long*)&keys[s], (unsigned long long)kJEmpty,
-                                            (unsigned long long)key);
-        if ((int64_t)prev == kJEmpty) {
-          atomicAdd(&flags[2], 1ull);
-          k = key;
-        } else {
-          k = (int64_t)prev;
+      bool nan;
+      int64_t key;
+      if (ktype == GZ_TYPE_BOOL) {
+        key = kvals[r];
+        nan = false;
+      } else {
+        key = join_key(kvals, ktype, r, &nan);
+      }
+      // a NaN build key can never be matched; a multi-column key with a null
+      // part (code -1) is never inserted (hash_relation.h:186-187)
+      if (nan || (codes && key == -1)) {
+      } else if (key == kJEmpty) {
+        unsigned long long c = atomicAdd(&flags[3], 1ull);
+        atomicMin((long long*)&flags[4], (long long)r);
+        if (c >= 1) flags[1] = 1;
+        row_slot[r] = -2;
+      } else {
+        uint64_t s = jmix64((uint64_t)key) & mask;
+        for (uint64_t i = 0; i < cap; i++) {
+          int64_t k = keys[s];
+          if (k == kJEmpty) {
+            unsigned long long prev = atomicCAS((unsigned long long*)&keys[s], (unsigned long long)kJEmpty,
+                                                (unsigned long long)key);
+            if ((int64_t)prev == kJEmpty) {
+              fresh = true;
+              k = key;
+            } else {
+              k = (int64_t)prev;
+            }
+          }
+          if (k == key) {
+            int c = atomicAdd(&count[s], 1);
+            if (c >= 1) flags[1] = 1;
+            atomicMin(&start[s], (int32_t)r);
+            row_slot[r] = (int32_t)s;
+            break;
+          }
+          s = (s + 1) & mask;
         }
       }
-      if (k == key) {
-        int c = atomicAdd(&count[s], 1);
-        if (c >= 1) flags[1] = 1;
-        atomicMin(&start[s], (int32_t)r);
-        row_slot[r] = (int32_t)s;
-        break;
-      }
-      s = (s + 1) & mask;
     }
+    // distinct keys: one atomic per wave, by its lowest active lane
+    const unsigned long long m = __ballot(fresh);
+    const unsigned long long act = __ballot(1);
+    if (m && (int)__lane_id() == __ffsll((long long)act) - 1) atomicAdd(&flags[2], (unsigned long long)__popcll(m));
   }
   (void)kwidth;
   (void)skip_dup;
@@ -976,7 +983,7 @@ int gz_relation_finish(gz_relation* r) {
     hipLaunchKernelGGL(join_build_kernel, dim3(gb), dim3(256), 0, st, (const uint8_t*)kc.valid.p,
                        (const uint8_t*)kc.values.p, kc.type, kc.width, n, (int64_t*)r->keys.p,
                        (int32_t*)r->start.p, (int32_t*)r->count.p, cap, (int32_t*)r->row_slot.p,
-                       r->builder_type == 3 ? 1 : 0, (unsigned long long*)r->flags.p);
+                       r->builder_type == 3 ? 1 : 0, r->kd ? 1 : 0, (unsigned long long*)r->flags.p);
   }
   GZ_HIP(hipGetLastError());
   GZ_HIP(hipMemcpyAsync(hflags, r->flags.p, sizeof(hflags), hipMemcpyDeviceToHost, st));
@@ -1527,9 +1534,11 @@ int gz_probe_make_keys(gz_relation* r, int32_t num_probe_cols, const int32_t* pr
       return fail(GZ_TYPE_ERROR, "probe and build key types differ");
   }
   if (null_aware_anti) return fail(GZ_NOT_IMPLEMENTED, "null-aware anti join with several keys");
-  for (int c = 0; c < num_result_cols; c++)
-    if (result_sides && result_indices && result_sides[c] == 0 && result_indices[c] >= r->ocols)
-      return fail(GZ_INVALID, "result column out of range");
+  for (int c = 0; c < num_result_cols; c++) {
+    if (!result_sides || !result_indices) break;
+    const int lim = result_sides[c] == 0 ? r->ocols : num_probe_cols;  // the appended code column is internal
+    if (result_indices[c] < 0 || result_indices[c] >= lim) return fail(GZ_INVALID, "result column out of range");
+  }
   if (num_probe_cols + 1 > kJoinMaxCols) return fail(GZ_NOT_IMPLEMENTED, "more than 31 probe columns with several keys");
   std::vector<int32_t> ext(probe_types, probe_types + num_probe_cols);
   ext.push_back(GZ_TYPE_INT64);

@@ -554,6 +554,23 @@ int kd_encode(KeyDict* d, gz_context* ctx, KeyScratch* sc, int32_t ncols, const 
   std::vector<KeyBatchPtrs> kb(std::max(nb, 1));
   std::vector<int64_t> prefix(nb + 1, 0);
   int64_t heap_need = d->heap_used;
+  // a utf8 key's bytes per batch: both ends of its offsets, read once for all
+  // batches, bound the copies into the heap and must fit the declared data size
+  std::vector<int32_t> ends;
+  if (d->sj >= 0 && mode != 2) {
+    ends.assign((size_t)nb * 2, 0);
+    const int k = col_buf[key_cols[d->sj]];
+    for (int b = 0; b < nb; b++) {
+      const gz_batch& bt = batches[b];
+      if (bt.num_rows <= 0 || !bt.buf_addrs || !bt.buf_sizes || bt.num_bufs <= k + 1 || !bt.buf_addrs[k + 1] ||
+          bt.buf_sizes[k + 1] < (bt.num_rows + 1) * 4)
+        continue;  // rejected below
+      const int32_t* off = (const int32_t*)bt.buf_addrs[k + 1];
+      GZ_HIP(hipMemcpyAsync(&ends[2 * b], off, 4, hipMemcpyDeviceToHost, ctx->stream));
+      GZ_HIP(hipMemcpyAsync(&ends[2 * b + 1], off + bt.num_rows, 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+  }
   for (int b = 0; b < nb; b++) {
     const gz_batch& bt = batches[b];
     if (bt.num_rows < 0 || bt.num_rows > INT32_MAX) return fail(GZ_INVALID, "bad num_rows");
@@ -572,7 +589,14 @@ int kd_encode(KeyDict* d, gz_context* ctx, KeyScratch* sc, int32_t ncols, const 
           if (bt.buf_sizes[k + 2] > 0 && !bt.buf_addrs[k + 2]) return fail(GZ_INVALID, "missing values buffer");
           kb[b].soffs = (const int32_t*)bt.buf_addrs[k + 1];
           kb[b].vals[j] = (const uint8_t*)bt.buf_addrs[k + 2];
-          heap_need += std::max<int64_t>(bt.buf_sizes[k + 2], 0);
+          if (!ends.empty()) {
+            const int64_t lo = ends[2 * b], hi = ends[2 * b + 1];
+            if (lo < 0 || hi < lo || hi > bt.buf_sizes[k + 2])
+              return fail(GZ_INVALID, "utf8 key offsets run past the declared data buffer (" + std::to_string(lo) +
+                                          ".." + std::to_string(hi) + " of " + std::to_string(bt.buf_sizes[k + 2]) +
+                                          " bytes)");
+            heap_need += hi - lo;
+          }
           continue;
         }
         if (!bt.buf_addrs[k + 1] || bt.buf_sizes[k + 1] < bt.num_rows * key_width(d->key_types[j]))

@@ -109,3 +109,61 @@ def test_multi_key_join_rejects(ctx):
                                     idx, ctypes.byref(h))
     assert rc == 10, rc  # GZ_NOT_IMPLEMENTED
     rel.close()
+
+
+def test_multi_key_probe_rejects_internal_code_column(ctx):
+    """A probe-side result index equal to the probe column count would name the
+    appended code column: Invalid, as is a negative index."""
+    from gazelle_plugin_amd.join import HashRelationKernel
+    from gazelle_plugin_amd.batch import DeviceBatch
+    build, _ = multikey_join_batches(10)
+    rel = HashRelationKernel.Make(ctx, BTYPES, [0, 1])
+    rel.EvaluateBatches([DeviceBatch.from_host(b) for b in build])
+    rel.Finish()
+    h = ctypes.c_void_p()
+    for bad in (3, -1):
+        rc = L.lib().gz_probe_make_keys(rel.handle, 3, L.i32_array(PTYPES), 2, L.i32_array([2, 0]), 0, 0, 1,
+                                        L.i32_array([1]), L.i32_array([bad]), ctypes.byref(h))
+        assert rc == 4, (bad, rc)  # GZ_INVALID
+    rel.close()
+
+
+@pytest.mark.parametrize("builder_type", [1, 3])
+def test_multi_key_relation_counts_non_null_tuples(ctx, builder_type):
+    """Rows whose key has a null part are never inserted (hash_relation.h:186-187):
+    the relation's key count is the number of distinct all-non-null tuples, and
+    such rows do not set the null-key flag."""
+    from gazelle_plugin_amd.batch import DeviceBatch, bools_from_bitmap
+    from gazelle_plugin_amd.join import HashRelationKernel
+    build, _ = multikey_join_batches(11)
+    rel = HashRelationKernel.Make(ctx, BTYPES, [0, 1], builder_type)
+    rel.EvaluateBatches([DeviceBatch.from_host(b) for b in build])
+    rel.Finish()
+    tuples = set()
+    for b in build:
+        cols = [(b.columns[c].values.view(np.dtype(f"u{b.columns[c].values.dtype.itemsize}")),
+                 bools_from_bitmap(b.columns[c].validity, b.num_rows)) for c in (0, 1)]
+        for r in range(b.num_rows):
+            if all(ok[r] for _, ok in cols):
+                tuples.add(tuple(int(v[r]) for v, _ in cols))
+    assert rel.num_keys() == len(tuples)
+    assert not rel.has_null_key()
+    rel.close()
+
+
+def test_utf8_key_offsets_past_the_data_buffer_are_invalid(ctx):
+    """A batch whose utf8 key offsets end past its declared data size is
+    rejected before any copy into the key dictionary's byte heap."""
+    from gazelle_plugin_amd.aggregate import HashAggregateKernel
+    from gazelle_plugin_amd.batch import DeviceBatch
+    n = 100
+    off = np.arange(n + 1, dtype=np.int32) * 5
+    data = np.full(int(off[-1]), ord("x"), np.uint8)
+    hb = HostBatch(n, [HostColumn(T.STRING, data, None, off), HostColumn(T.INT64, np.arange(n, dtype=np.int64))])
+    db = DeviceBatch.from_host(hb)
+    db.sizes[2] = 100  # declared data bytes: fewer than the 500 the offsets span
+    it = HashAggregateKernel.Make(ctx, [T.STRING, T.INT64], [("action_groupby", [0]), ("action_sum", [1])]) \
+        .MakeResultIterator()
+    with pytest.raises(L.Invalid):
+        it.ProcessAndCacheOne(db)
+    it.close()
