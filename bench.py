@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--sections", default="split,agg,join,serve",
                     help="comma list of split, agg, join, serve (split always runs)")
     ap.add_argument("--join-probe-rows", type=int, default=256 << 20)
-    ap.add_argument("--serve-fragments", type=int, default=1000)
+    ap.add_argument("--serve-fragments", type=int, default=10000)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run a device copy of this many MiB (PMC calibration)")
@@ -95,47 +95,162 @@ def timed(fn, stream, reps):
 
 
 # ---------------------------------------------------------------- CPU baseline
-def cpu_baseline_split(target_s: float):
-    """The oracle's per-batch split loop (splitter.cc DoSplit restated) on one
-    host core, over a bounded sample of the same schema."""
+# The oracle (the reference's algorithms restated in C) timed on this host: one
+# core (one Spark task, as the reference runs) and every core of this process's
+# CPU share (one independent task per core over disjoint batch slices, as an
+# executor runs its tasks).  Each task is a C loop over its batches, so Python
+# holds no lock while they run.
+
+def cpu_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _run_tasks(tasks):
+    """Run one callable per thread, started together; wall seconds until all end."""
+    import threading
+
+    gate = threading.Barrier(len(tasks) + 1)
+
+    def wrap(fn):
+        gate.wait()
+        fn()
+
+    ths = [threading.Thread(target=wrap, args=(t,)) for t in tasks]
+    for t in ths:
+        t.start()
+    gate.wait()
+    t0 = time.perf_counter()
+    for t in ths:
+        t.join()
+    return time.perf_counter() - t0
+
+
+def _timed_loop(body, seconds):
+    """Repeat body() (returns rows) until `seconds` pass; (rows, seconds)."""
+    rows, t0 = 0, time.perf_counter()
+    while True:
+        rows += body()
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            return rows, dt
+
+
+def _baseline(one, many, threads, unit, sample):
+    rows1, s1 = one
+    rows_n, s_n = many
+    base = {"unit": unit, "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+    return dict(base, value=rows_n / s_n, cores=threads,
+                sample=f"{sample}; all cores: {threads} tasks, {rows_n} rows in {s_n:.1f} s",
+                one_core=dict(base, value=rows1 / s1, cores=1, sample=f"{sample}; one task, {rows1} rows in {s1:.1f} s"))
+
+
+def cpu_baseline_split(seconds: float):
+    """The oracle's per-batch split loop (splitter.cc DoSplit restated) over
+    batches of the split workload's schema, 64 batches (262k rows) per pass."""
     import oracle
     from gazelle_plugin_amd import datagen
 
-    rows = 256 * 4096
-    batches = datagen.split_host(rows)
+    T = cpu_threads()
+    per = 64
+    batches = datagen.split_host(T * per * datagen.BATCH_ROWS)
     cb = oracle._CBatches(batches)
-    total_rows, total_t, passes = 0, 0.0, 0
-    while total_t < target_s or passes == 0:
-        sp = oracle.CpuSplitter(datagen.SPLIT_TYPES, 200, [0])
-        t0 = time.perf_counter()
-        for i in range(len(batches)):
-            sp.split(cb, i)
-        total_t += time.perf_counter() - t0
-        sp.close()
-        total_rows += rows
-        passes += 1
-    return {"value": total_rows / total_t, "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": f"{total_rows} rows ({passes} passes over 256 batches x 4096 rows) of the split "
-                      f"schema, oracle per-batch split loop, {total_t:.1f} s on {os.cpu_count()} visible cores "
-                      f"(1 used)"}
+
+    def task(i0, secs, out):
+        def one_pass():
+            sp = oracle.CpuSplitter(datagen.SPLIT_TYPES, 200, [0])
+            sp.split_many(cb, i0, per)
+            sp.close()
+            return per * datagen.BATCH_ROWS
+        out.append(_timed_loop(one_pass, secs))
+
+    one = []
+    task(0, seconds / 2, one)
+    res = [[] for _ in range(T)]
+    wall = _run_tasks([lambda i=i: task(i * per, seconds / 2, res[i]) for i in range(T)])
+    return _baseline(one[0], (sum(r[0][0] for r in res), wall), T, "rows/s",
+                     "split schema (int64, int32, float64 2% nulls, utf8 8-32), 200 partitions, passes of 64 "
+                     "batches x 4096 rows, oracle per-batch split loop")
 
 
-def cpu_baseline_agg(target_s: float):
+def cpu_baseline_agg(seconds: float):
+    """The oracle's hash aggregate (phmap-style first-seen map + the reference's
+    accumulators) over the plain query's schema, 10k groups, 1M-row slices."""
     import oracle
     from gazelle_plugin_amd import datagen
     from gazelle_plugin_amd.aggregate import parse_action
 
-    batches = datagen.agg_host(1 << 20, num_keys=10000)
+    T = cpu_threads()
+    per = 256
+    batches = datagen.agg_host(T * per * datagen.BATCH_ROWS, num_keys=10000)
+    cb = oracle._CBatches(batches)
     acts = [parse_action(n, i) for n, i in AGG_ACTIONS]
-    total_rows, total_t = 0, 0.0
-    while total_t < target_s or total_rows == 0:
-        t0 = time.perf_counter()
-        oracle.aggregate(batches, datagen.AGG_TYPES, acts)
-        total_t += time.perf_counter() - t0
-        total_rows += 1 << 20
-    return {"value": total_rows / total_t, "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": f"{total_rows} rows of the plain-query schema (10k groups), oracle hash aggregate, "
-                      f"{total_t:.1f} s"}
+
+    def task(i0, secs, out):
+        def one_pass():
+            a = oracle.CpuAggregate(datagen.AGG_TYPES, acts)
+            a.process_many(cb, i0, per)
+            a.close()
+            return per * datagen.BATCH_ROWS
+        out.append(_timed_loop(one_pass, secs))
+
+    one = []
+    task(0, seconds / 2, one)
+    res = [[] for _ in range(T)]
+    wall = _run_tasks([lambda i=i: task(i * per, seconds / 2, res[i]) for i in range(T)])
+    return _baseline(one[0], (sum(r[0][0] for r in res), wall), T, "rows/s",
+                     "plain query (GROUP BY int64 key, 10k groups: sum/avg price, count, min/max/sum qty), "
+                     "passes of 256 batches x 4096 rows, oracle hash aggregate")
+
+
+def cpu_baseline_join(build_keys, probe_keys, seconds: float):
+    """The oracle's hash relation (unsafeHashMap-style table, triangular
+    probing) built once from the join workload's 4M build keys, then inner
+    probes of 4M-row slices of its probe keys (Zipf 1.1, 5% absent)."""
+    import numpy as np
+
+    import oracle
+
+    T = cpu_threads()
+    rel = oracle.CpuRelation()
+    t0 = time.perf_counter()
+    for a in range(0, len(build_keys), 4096):
+        rel.append(build_keys[a:a + 4096])
+    rel.finish()
+    build_s = time.perf_counter() - t0
+    per = min(4 << 20, len(probe_keys) // T)
+
+    def task(i, secs, out):
+        keys = probe_keys[i * per:(i + 1) * per]
+        op = np.zeros(per, np.int64)
+        ob = np.zeros(per, np.int64)
+        out.append(_timed_loop(lambda: (rel.probe(keys, 0, op, ob), per)[1], secs))
+
+    one = []
+    task(0, seconds / 2, one)
+    res = [[] for _ in range(T)]
+    wall = _run_tasks([lambda i=i: task(i, seconds / 2, res[i]) for i in range(T)])
+    rel.close()
+    out = _baseline(one[0], (sum(r[0][0] for r in res), wall), T, "probe rows/s",
+                    f"inner probe of {per}-row slices of the join workload's probe keys against the oracle "
+                    f"relation of its {len(build_keys)} build keys")
+    out["build_s_one_core"] = build_s
+    out["build_rows_per_s_one_core"] = len(build_keys) / build_s
+    return out
 
 
 # ---------------------------------------------------------------- sections
@@ -495,6 +610,12 @@ def bench_join(args, ctx, stream):
     out["join_aggregate"] = bench_join_aggregate(ctx, stream, rel, probe, parr)
     out["two_key"] = bench_join_two_keys(ctx, stream, rel, build, probe)
     rel.close()
+    if not args.no_cpu:
+        bk = build.tensors[0].cpu().numpy()
+        pk = probe.tensors[0][:64 << 20].cpu().numpy()
+        torch.cuda.synchronize()
+        out["cpu_baseline"] = cpu_baseline_join(bk, pk, min(6.0, args.cpu_seconds))
+        del bk, pk
     del build, probe, barr, parr
     return out
 
@@ -731,7 +852,7 @@ def main():
         torch.cuda.synchronize()
     cpu = None if args.no_cpu else cpu_baseline_split(args.cpu_seconds)
     if not args.no_cpu and "aggregate" in extra and "error" not in extra["aggregate"]:
-        extra["aggregate"]["cpu_baseline"] = cpu_baseline_agg(min(5.0, args.cpu_seconds))
+        extra["aggregate"]["cpu_baseline"] = cpu_baseline_agg(min(6.0, args.cpu_seconds))
     out = {
         "metric": "rows/s + HBM GB/s (% peak), batched vs plain GPU path; p50/p99 fragment latency",
         "value": split["rows_per_s"],

@@ -2,6 +2,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <iterator>
+
 #include "gz_internal.h"
 
 namespace gz {
@@ -46,22 +48,30 @@ int DevicePool::alloc(size_t bytes, void** out) {
   if (budget_ > 0 && in_use_ + (int64_t)r > budget_)
     return fail(GZ_OUT_OF_MEMORY, "HBM budget exceeded: " + std::to_string(in_use_) + " + " +
                                       std::to_string(r) + " > " + std::to_string(budget_));
-  auto it = free_.find(r);
+  // best fit among cached blocks of at most 1.25x the request (sizes vary
+  // between calls, so exact-size reuse alone would keep growing the cache)
+  auto it = free_.lower_bound(r);
   void* p = nullptr;
-  if (it != free_.end()) {
+  if (it != free_.end() && it->first <= r + r / 4) {
     p = it->second;
+    r = it->first;
     free_.erase(it);
     cached_ -= r;
   } else {
     hipError_t e = hipMalloc(&p, r);
     if (e != hipSuccess) {
-      // release cached blocks and retry once
+      // release cached blocks and retry once.  The failed call left its error
+      // as the thread's last error, which a later hipGetLastError (after a
+      // kernel launch) would report: clear it, the retry decides.
+      (void)hipGetLastError();
       for (auto& kv : free_) hipFree(kv.second);
       free_.clear();
       cached_ = 0;
       e = hipMalloc(&p, r);
-      if (e != hipSuccess)
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
         return fail(GZ_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+      }
     }
   }
   live_[p] = r;
@@ -81,6 +91,16 @@ void DevicePool::free(void* p) {
   in_use_ -= r;
   free_.emplace(r, p);
   cached_ += r;
+  if (cached_ > kCacheLimit) {
+    // the largest blocks go back to HIP first, down to half the limit;
+    // hipFree waits for the device, so work still reading a block has finished
+    while (cached_ > kCacheLimit / 2 && !free_.empty()) {
+      auto last = std::prev(free_.end());
+      hipFree(last->second);
+      cached_ -= (int64_t)last->first;
+      free_.erase(last);
+    }
+  }
 }
 
 int ctx_alloc(gz_context* ctx, size_t bytes, DBuf* out) {

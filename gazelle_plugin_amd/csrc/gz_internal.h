@@ -53,7 +53,9 @@ inline int type_num_bufs(int t) { return type_is_binary(t) ? 3 : 2; }
 // ---- device memory ----------------------------------------------------------------
 // A caching pool: blocks are rounded (power of two up to 64 MiB, 2 MiB multiples
 // above) and kept on free lists; the budget counts live blocks only, and cached
-// blocks are returned to HIP before an allocation would exceed it.
+// blocks are returned to HIP before an allocation would exceed it, when a
+// hipMalloc fails, and when the cache outgrows kCacheLimit (so that pools of
+// several contexts on one device do not hold its memory between them).
 class DevicePool {
  public:
   DevicePool(int device, int64_t budget) : device_(device), budget_(budget) {}
@@ -67,6 +69,7 @@ class DevicePool {
 
  private:
   static size_t round(size_t bytes);
+  static constexpr int64_t kCacheLimit = 16ll << 30;
   int device_;
   int64_t budget_;
   int64_t in_use_ = 0, peak_ = 0, cached_ = 0;

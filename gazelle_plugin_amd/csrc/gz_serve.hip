@@ -100,6 +100,8 @@ struct gz_server {
   Clock::time_point t0;
   std::vector<gz_agg*> agg_out;
   std::vector<gz_splitter*> split_out;
+  int first_failed = -1;    // fragment of the first failure and its message
+  std::string first_msg;
 };
 
 namespace {
@@ -274,7 +276,12 @@ void worker(gz_server* s, int k) {
       hipError_t e = hipStreamSynchronize(sl.ctx->stream);
       if (e != hipSuccess) rc = fail(GZ_HIP_ERROR, hipGetErrorString(e));
     }
+    std::string msg = rc == GZ_OK ? std::string() : std::string(gz_last_error());
     lk.lock();
+    if (rc != GZ_OK && s->first_failed < 0) {
+      s->first_failed = i;
+      s->first_msg = msg;
+    }
     gz_fragment_stats& st = s->stats[i];
     st.done_s = since(s);
     st.status = rc;
@@ -373,6 +380,8 @@ int gz_server_run(gz_server* s, int32_t n, const gz_fragment* frags, int32_t kee
   }
   s->in_use = 0;
   s->done = 0;
+  s->first_failed = -1;
+  s->first_msg.clear();
   s->t0 = Clock::now();
   int next = 0;  // next fragment to admit (first come, first served)
   while (s->done < n) {
@@ -405,8 +414,13 @@ int gz_server_run(gz_server* s, int32_t n, const gz_fragment* frags, int32_t kee
   int first_error = GZ_OK;
   for (int i = 0; i < n; i++)
     if (stats[i].status != GZ_OK && first_error == GZ_OK) first_error = stats[i].status;
+  const std::string msg = s->first_failed < 0 ? std::string()
+                                              : "fragment " + std::to_string(s->first_failed) + " (kind " +
+                                                    std::to_string(frags[s->first_failed].kind) + ", " +
+                                                    std::to_string(rows_of(frags[s->first_failed])) +
+                                                    " rows) failed first: " + s->first_msg;
   lk.unlock();
-  if (first_error != GZ_OK) return fail(first_error, "a fragment failed (see its status)");
+  if (first_error != GZ_OK) return fail(first_error, msg.empty() ? "a fragment failed (see its status)" : msg);
   return GZ_OK;
 }
 

@@ -240,6 +240,12 @@ class CpuSplitter:
     def split(self, cbatches: "_CBatches", i: int):
         lib().gzo_splitter_split(self.h, ctypes.byref(cbatches.arr[i]))
 
+    def split_many(self, cbatches: "_CBatches", i0: int, nb: int):
+        """Batches i0 .. i0 + nb in one C loop (the GIL is released meanwhile)."""
+        L = lib()
+        L.gzo_splitter_split_many.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.gzo_splitter_split_many(self.h, ctypes.byref(cbatches.arr, i0 * ctypes.sizeof(_Batch)), nb)
+
     def rows(self) -> np.ndarray:
         return np.array([lib().gzo_splitter_rows(self.h, p) for p in range(self.P)], np.int64)
 
@@ -250,6 +256,58 @@ class CpuSplitter:
 
 
 # ------------------------------------------------------------------ aggregate
+
+class CpuAggregate:
+    """The reference hash aggregate over whole batch slices (for timing the CPU
+    baseline): actions are (kind, in0, in1, nan_check) tuples."""
+
+    def __init__(self, types: Sequence[int], actions: Sequence[Sequence[int]]):
+        L = lib()
+        L.gzo_agg_make.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.gzo_agg_process_many.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        na = len(actions)
+        self._keep = [(ctypes.c_int32 * na)(*[a[j] for a in actions]) for j in range(4)]
+        self._t = (ctypes.c_int32 * len(types))(*types)
+        self.h = L.gzo_agg_make(len(types), self._t, na, *self._keep)
+
+    def process_many(self, cbatches: "_CBatches", i0: int, nb: int):
+        rc = lib().gzo_agg_process_many(self.h, ctypes.byref(cbatches.arr, i0 * ctypes.sizeof(_Batch)), nb)
+        if rc:
+            raise ValueError(f"oracle aggregate failed with status {rc}")
+
+    def num_groups(self) -> int:
+        return lib().gzo_agg_num_groups(self.h)
+
+    def close(self):
+        if self.h:
+            lib().gzo_agg_free(self.h)
+            self.h = None
+
+
+class CpuRelation:
+    """The reference hash relation over one int64 key column, built once and
+    probed from any number of threads (the probe only reads it)."""
+
+    def __init__(self, key_type: int = 9, builder_type: int = 1):
+        self.h = lib().gzo_rel_make(key_type, builder_type)
+        self._keep = []
+
+    def append(self, keys: np.ndarray):
+        keys = np.ascontiguousarray(keys)
+        self._keep.append(keys)
+        lib().gzo_rel_append(self.h, _ptr(keys), None, len(keys))
+
+    def finish(self):
+        lib().gzo_rel_finish(self.h)
+
+    def probe(self, keys: np.ndarray, join_type: int, out_probe: np.ndarray, out_build: np.ndarray) -> int:
+        return lib().gzo_probe(self.h, join_type, 0, _ptr(keys), None, len(keys), _ptr(out_probe), _ptr(out_build))
+
+    def close(self):
+        if self.h:
+            lib().gzo_rel_free(self.h)
+            self.h = None
 
 class ResultColumn(tuple):
     """(type, values, valid) of one result column; ``approx`` is True for a

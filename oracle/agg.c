@@ -635,3 +635,11 @@ void gzo_agg_free(gzo_agg* a) {
   free(a->arena); free(a->soff); free(a->slen); free(a->sid); free(a->shash); free(a->rowbuf);
   free(a);
 }
+
+int gzo_agg_process_many(gzo_agg* a, const gzo_batch* batches, int nb) {
+  for (int i = 0; i < nb; i++) {
+    int rc = gzo_agg_process(a, &batches[i]);
+    if (rc) return rc;
+  }
+  return 0;
+}

@@ -65,6 +65,9 @@ typedef struct gzo_splitter gzo_splitter;
 gzo_splitter* gzo_splitter_make(int ncols, const int32_t* types, int P, const int32_t* keys,
                                 int nkeys, int32_t seed);
 void gzo_splitter_split(gzo_splitter* s, const gzo_batch* b);
+/* gzo_splitter_split over batches[0..nb) in order (a task's batch loop, timed
+ * without a caller between batches) */
+void gzo_splitter_split_many(gzo_splitter* s, const gzo_batch* batches, int nb);
 int64_t gzo_splitter_rows(gzo_splitter* s, int p);
 void gzo_splitter_free(gzo_splitter* s);
 
@@ -78,6 +81,8 @@ typedef struct gzo_agg gzo_agg;
 gzo_agg* gzo_agg_make(int ncols, const int32_t* types, int nactions, const int32_t* kinds,
                       const int32_t* in0, const int32_t* in1, const int32_t* nan_check);
 int gzo_agg_process(gzo_agg* a, const gzo_batch* b);
+/* gzo_agg_process over batches[0..nb) in order */
+int gzo_agg_process_many(gzo_agg* a, const gzo_batch* batches, int nb);
 int64_t gzo_agg_num_groups(gzo_agg* a);
 int gzo_agg_num_results(gzo_agg* a);
 int gzo_agg_result_type(gzo_agg* a, int col);

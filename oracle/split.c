@@ -421,3 +421,7 @@ void gzo_splitter_free(gzo_splitter* s) {
   free(s->reducer_offsets);
   free(s);
 }
+
+void gzo_splitter_split_many(gzo_splitter* s, const gzo_batch* batches, int nb) {
+  for (int i = 0; i < nb; i++) gzo_splitter_split(s, &batches[i]);
+}
