@@ -49,7 +49,14 @@ class GzBatch(ctypes.Structure):
 class GzSplitOptions(ctypes.Structure):
     _fields_ = [("buffer_size", ctypes.c_int32), ("prefer_spill", ctypes.c_int32),
                 ("offheap_per_task", ctypes.c_int64), ("tile_rows", ctypes.c_int32),
-                ("hash_seed", ctypes.c_int32), ("append_row_ids", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("hash_seed", ctypes.c_int32), ("append_row_ids", ctypes.c_int32),
+                ("batch_compress_threshold", ctypes.c_int32)]
+
+
+class GzSplitMetrics(ctypes.Structure):
+    _fields_ = [("total_compute_pid_time", ctypes.c_int64), ("total_write_time", ctypes.c_int64),
+                ("total_spill_time", ctypes.c_int64), ("total_compress_time", ctypes.c_int64),
+                ("total_bytes_written", ctypes.c_int64), ("total_bytes_spilled", ctypes.c_int64)]
 
 
 class GzAggAction(ctypes.Structure):
@@ -126,6 +133,9 @@ _SIGS = {
     "gz_split_segment_column": (ctypes.c_int, [_P, _I32, _I32, _PP, _PP, _PP, _PP, _PI64, _PI64]),
     "gz_split_segment_batch_counts": (ctypes.c_int, [_P, _I32, _PI32, _PI32]),
     "gz_split_compute_pid_time": (_I64, [_P]),
+    "gz_split_raw_partition_lengths": (ctypes.c_int, [_P, _PI64]),
+    "gz_split_batch_rows": (ctypes.c_int, [_P, _PI64]),
+    "gz_split_get_metrics": (ctypes.c_int, [_P, ctypes.POINTER(GzSplitMetrics)]),
     "gz_split_release": (ctypes.c_int, [_P]),
     "gz_split_ipc_prepare": (ctypes.c_int, [_P, ctypes.c_void_p, _I64, _I32, _PI64, _PI64]),
     "gz_split_ipc_write": (ctypes.c_int, [_P, ctypes.c_void_p, _I64]),

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -807,7 +808,38 @@ struct gz_splitter {
   DBuf d_tiles, d_bufs;
   bool stopped = false;
   int64_t compute_pid_ns = 0;
+  // metrics: events around every partition id pass (summed at Stop), host
+  // wall time of the data file's layout and assembly, and of its compression
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pid_events, pid_free;
+  int64_t write_ns = 0, compress_ns = 0;
 };
+
+// Elapsed time of the recorded partition id passes, added to compute_pid_ns;
+// their events are kept for reuse.
+static void drain_pid_events(gz_splitter* h) {
+  for (auto& ev : h->pid_events) {
+    float ms = 0;
+    if (hipEventSynchronize(ev.second) == hipSuccess && hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess)
+      h->compute_pid_ns += (int64_t)((double)ms * 1e6);
+    h->pid_free.push_back(ev);
+  }
+  h->pid_events.clear();
+  (void)hipGetLastError();
+}
+
+// A pair of events around this call's partition id pass (at most 256 pending).
+static std::pair<hipEvent_t, hipEvent_t> pid_event_pair(gz_splitter* h) {
+  if (h->pid_events.size() >= 256) drain_pid_events(h);
+  std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+  if (!h->pid_free.empty()) {
+    ev = h->pid_free.back();
+    h->pid_free.pop_back();
+  } else if (hipEventCreate(&ev.first) != hipSuccess || hipEventCreate(&ev.second) != hipSuccess) {
+    (void)hipGetLastError();
+    return {nullptr, nullptr};
+  }
+  return ev;
+}
 
 static int split_release_segment(gz_context* ctx, SplitSegment* s) {
   for (DBuf* b : {&s->counts, &s->prefix, &s->csum, &s->tot, &s->row_start, &s->byte_start, &s->err,
@@ -835,9 +867,9 @@ void gz_split_options_defaults(gz_split_options* o) {
   o->prefer_spill = 1;
   o->offheap_per_task = 0;
   o->tile_rows = 0;
+  o->batch_compress_threshold = 256;
   o->hash_seed = 42;
   o->append_row_ids = 0;
-  o->reserved = 0;
 }
 
 int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
@@ -1142,11 +1174,17 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   for (int c = 0; c < nc; c++) add_pre(c);
   if (T > 0) {
     KernelTimer kt(ctx, "split_count");
+    const std::pair<hipEvent_t, hipEvent_t> ev = pid_event_pair(h);
+    if (ev.first) {
+      hipEventRecord(ev.first, st);
+      h->pid_events.push_back(ev);
+    }
     size_t lds = sizeof(uint32_t) * R;
     if (keyk)
       hipLaunchKernelGGL((split_count_kernel<256, 1>), dim3((unsigned)grid), dim3(256), lds, st, prm);
     else
       hipLaunchKernelGGL((split_count_kernel<256, 0>), dim3((unsigned)grid), dim3(256), lds, st, prm);
+    if (ev.first) hipEventRecord(ev.second, st);
   }
   {
     KernelTimer kt(ctx, "split_scan");
@@ -1265,11 +1303,7 @@ int gz_split_batches(gz_splitter* h, int32_t num_batches, const gz_batch* batche
 int gz_split_stop(gz_splitter* h) {
   if (!h) return fail(GZ_INVALID, "null splitter");
   for (auto* s : h->segments) GZ_TRY(resolve_segment(h, s));
-  if (h->ctx->profiling) {
-    double ms = 0;
-    gz_context_kernel_time(h->ctx, "split_count", &ms, nullptr);
-    h->compute_pid_ns = (int64_t)(ms * 1e6);
-  }
+  drain_pid_events(h);
   h->stopped = true;
   return GZ_OK;
 }
@@ -1350,6 +1384,11 @@ int gz_split_release(gz_splitter* h) {
   hipSetDevice(h->ctx->device);
   // No synchronization: the pool hands a freed block only to work enqueued
   // later on the context's stream, which runs after everything queued here.
+  for (auto* list : {&h->pid_events, &h->pid_free})
+    for (auto& ev : *list) {
+      hipEventDestroy(ev.first);
+      hipEventDestroy(ev.second);
+    }
   for (auto* s : h->segments) split_release_segment(h->ctx, s);
   if (h->ipc) ipc_plan_free(h->ctx, h->ipc);
   ctx_free(h->ctx, &h->d_tiles);
@@ -1785,6 +1824,8 @@ struct gz_ipc_plan {
   std::vector<BatchRec> recs;
   std::vector<uint8_t> schema;  // written first in every non-empty partition, when non-empty
   DBuf raw, comp, d_cblocks, d_csize;  // LZ4: uncompressed stream, compressed blocks
+  std::vector<int64_t> raw_len;        // RawPartitionLengths: in-memory bytes of the cached batches
+  int64_t batch_rows = 0;              // rows of a partition buffer (CalculateSplitBatchSize)
 };
 
 static void ipc_plan_free(gz_context* ctx, gz_ipc_plan* pl) {
@@ -1794,29 +1835,86 @@ static void ipc_plan_free(gz_context* ctx, gz_ipc_plan* pl) {
   delete pl;
 }
 
-// Rows of each record batch of one partition, from its rows per input batch in
-// order (the reference's buffer policy; partition_writer.py restates it).
-static std::vector<int64_t> ipc_batch_rows(const std::vector<int32_t>& counts, int64_t bs, bool prefer_spill) {
-  std::vector<int64_t> out;
-  int64_t size = 0, base = 0;
-  for (int32_t cnt : counts) {
+// The reference's partition buffers for one partition over its input batches
+// (Splitter::DoSplit, splitter.cc:876-938): a buffer holds max(batch_rows, rows
+// of this input batch) rows; when the next input batch's rows do not fit, the
+// filled buffer is cached as one record batch (CacheRecordBatch, :462-608) and
+// is reused (prefer_spill and the new size fits) or reset and allocated again.
+// Every cached batch also records which columns carry a validity buffer in
+// memory (for RawPartitionLengths): a buffer allocated while the column's
+// sticky input_has_null_ flag is set gets one (AllocatePartitionBuffers,
+// :610-713), an input batch with nulls in the column gives one to every
+// buffer it adds rows to (SplitValidityBuffer, :1179-1207), and a reused
+// buffer keeps it.  The sticky flag of array slot k is set by an input batch
+// with nulls in schema column array_idx[k] only when that column index is below
+// the fixed-width column count (:878-886: the index compared is the schema's,
+// as written there).
+struct CutBatch {
+  int64_t rows;
+  int64_t last_batch;          // global index of the last input batch it holds
+  std::vector<uint8_t> valid;  // per output column: a validity buffer in memory
+};
+
+static std::vector<CutBatch> ipc_cut(const std::vector<int32_t>& counts, int64_t bs, bool prefer_spill,
+                                     const std::vector<uint8_t>& sticky,   // [batch][slot], after the batch's update
+                                     const std::vector<uint8_t>& nulls,    // [batch][out col]
+                                     const std::vector<int>& slot_col, int nout) {
+  std::vector<CutBatch> out;
+  const int ns = (int)slot_col.size();
+  int64_t size = 0, base = 0, last = -1;
+  std::vector<uint8_t> v(nout, 0);
+  auto allocate = [&](int64_t b) {
+    for (int k = 0; k < ns; k++) v[slot_col[k]] = sticky[(size_t)b * ns + k];
+  };
+  for (size_t b = 0; b < counts.size(); b++) {
+    const int32_t cnt = counts[b];
     if (cnt <= 0) continue;
     const int64_t new_size = std::max<int64_t>(bs, cnt);
     if (size == 0) {
       size = new_size;
+      allocate((int64_t)b);
     } else if (base + cnt > size) {
-      if (base > 0) out.push_back(base);
+      if (base > 0) out.push_back({base, last, v});
       base = 0;
-      if (prefer_spill) {
-        if (new_size > size) size = new_size;
-      } else {
+      if (!prefer_spill || new_size > size) {  // reset, allocate again
         size = new_size;
+        allocate((int64_t)b);
       }
     }
+    for (int k = 0; k < ns; k++)
+      if (nulls[b * nout + slot_col[k]]) v[slot_col[k]] = 1;
     base += cnt;
+    last = (int64_t)b;
   }
-  if (base > 0) out.push_back(base);
+  if (base > 0) out.push_back({base, last, v});
   return out;
+}
+
+// Rows per (input batch, partition), nulls per (input batch, output column) and
+// utf8 bytes per (input batch, binary column) of a segment, from its tile
+// counter rows.
+static int segment_batch_stats(gz_splitter* h, SplitSegment* s, std::vector<int32_t>& counts,
+                               std::vector<uint8_t>& nulls, std::vector<int64_t>& bin_bytes) {
+  const int P = h->P, nbin = h->base.nbin, nout = (int)h->out_types.size();
+  std::vector<uint32_t> tc((size_t)s->R * std::max<int64_t>(s->T, 1));
+  if (s->T > 0) {
+    GZ_HIP(hipMemcpyAsync(tc.data(), s->counts.p, sizeof(uint32_t) * s->R * s->T, hipMemcpyDeviceToHost,
+                          h->ctx->stream));
+    GZ_HIP(hipStreamSynchronize(h->ctx->stream));
+  }
+  counts.assign((size_t)s->nbatches * P, 0);
+  nulls.assign((size_t)s->nbatches * std::max(nout, 1), 0);
+  bin_bytes.assign((size_t)s->nbatches * std::max(nbin, 1), 0);
+  for (int64_t t = 0; t < s->T; t++) {
+    const int b = s->tile_batch[t];
+    const uint32_t* row = &tc[(size_t)t * s->R];
+    for (int p = 0; p < P; p++) counts[(size_t)b * P + p] += (int32_t)row[p];
+    for (int j = 0; j < nbin; j++)
+      for (int p = 0; p < P; p++) bin_bytes[(size_t)b * nbin + j] += row[(size_t)P * (1 + j) + p];
+    for (int oc = 0; oc < nout; oc++)
+      if (row[(size_t)P * (1 + nbin) + oc]) nulls[(size_t)b * nout + oc] = 1;
+  }
+  return GZ_OK;
 }
 
 static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len, bool write_schema,
@@ -1825,18 +1923,73 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
   hipStream_t st = ctx->stream;
   const int P = h->P;
   const int nout = (int)h->out_types.size();
-  if (h->opts.offheap_per_task > 0)
-    return fail(GZ_NOT_IMPLEMENTED, "partition writer: offheap_per_task sizing is not supported");
-  const int64_t bs = h->opts.buffer_size > 0 ? h->opts.buffer_size : 4096;
+  const int nbin = h->base.nbin;
   const int nseg = (int)h->segments.size();
   for (auto* s : h->segments) GZ_TRY(resolve_segment(h, s));
-  // rows per input batch per partition, segment by segment (counts[s][b * P + p])
+  // rows per input batch per partition, segment by segment (counts[s][b * P + p]),
+  // and per input batch (all segments in order) its rows, nulls per output
+  // column and utf8 bytes per binary column
   std::vector<std::vector<int32_t>> cnt(nseg);
+  std::vector<uint8_t> gnull;
+  std::vector<int64_t> gbin, grows;
   for (int s = 0; s < nseg; s++) {
-    cnt[s].assign((size_t)h->segments[s]->nbatches * P, 0);
-    int32_t nb = 0;
-    GZ_TRY(gz_split_segment_batch_counts(h, s, &nb, cnt[s].data()));
+    std::vector<uint8_t> nl;
+    std::vector<int64_t> bb;
+    GZ_TRY(segment_batch_stats(h, h->segments[s], cnt[s], nl, bb));
+    const int nb = h->segments[s]->nbatches;
+    for (int b = 0; b < nb; b++) {
+      int64_t r = 0;
+      for (int p = 0; p < P; p++) r += cnt[s][(size_t)b * P + p];
+      grows.push_back(r);
+      gnull.insert(gnull.end(), nl.begin() + (size_t)b * std::max(nout, 1),
+                   nl.begin() + (size_t)b * std::max(nout, 1) + nout);
+      for (int j = 0; j < nbin; j++) gbin.push_back(bb[(size_t)b * nbin + j]);
+    }
   }
+  const int64_t NB = (int64_t)grows.size();
+  // the reference's array slots: fixed-width columns in schema order, then the
+  // binary ones (null-typed columns and the row-id column have none)
+  std::vector<int> slot_col;
+  for (int oc = 0; oc < nout; oc++)
+    if (oc != h->base.rowid_oc && h->out_types[oc] != GZ_TYPE_NA && !type_is_binary(h->out_types[oc]))
+      slot_col.push_back(oc);
+  const int fixed_cnt = (int)slot_col.size();
+  for (int oc = 0; oc < nout; oc++)
+    if (oc != h->base.rowid_oc && type_is_binary(h->out_types[oc])) slot_col.push_back(oc);
+  const int ns = (int)slot_col.size();
+  std::vector<uint8_t> sticky((size_t)std::max<int64_t>(NB, 1) * std::max(ns, 1), 0);
+  {
+    std::vector<uint8_t> run(ns, 0);
+    for (int64_t b = 0; b < NB; b++)
+      for (int k = 0; k < ns; k++) {
+        if (slot_col[k] < fixed_cnt && gnull[(size_t)b * nout + slot_col[k]]) run[k] = 1;
+        sticky[(size_t)b * ns + k] = run[k];
+      }
+  }
+  // rows of a partition buffer: CalculateSplitBatchSize (splitter.cc:800-853),
+  // the row size from the first input batch (binary columns: ceil(bytes / rows))
+  int64_t bs = h->opts.buffer_size > 0 ? h->opts.buffer_size : 4096;
+  if (h->opts.offheap_per_task > 0) {
+    int64_t b0 = 0;
+    while (b0 < NB && grows[b0] == 0) b0++;
+    if (b0 < NB) {
+      int64_t spr = 0;
+      for (int j = 0; j < nbin; j++) {
+        const int64_t bytes = gbin[(size_t)b0 * nbin + j];
+        if (bytes == 0)
+          return fail(GZ_NOT_IMPLEMENTED, "offheap_per_task sizing with a binary column empty in the first batch");
+        spr += (bytes + grows[b0] - 1) / grows[b0];
+      }
+      for (int k = 0; k < ns; k++)
+        if (slot_col[k] < fixed_cnt) {
+          const int t = h->out_types[slot_col[k]];
+          spr += type_is_binary(t) || t == GZ_TYPE_BOOL ? 0 : type_width(t);  // arrow::bit_width(t) >> 3
+        }
+      const int64_t pre = spr > 0 ? (h->opts.offheap_per_task / spr / P) >> 2 : bs;
+      bs = std::min(pre, bs);
+    }
+  }
+  plan->batch_rows = bs;
   // input column of each output column (range mode drops the pid column)
   std::vector<int> in_of(nout, -1);
   for (int c = 0; c < h->ncols_in; c++)
@@ -1851,6 +2004,7 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
     int p;
     int64_t rows;
     std::vector<Span> spans;
+    std::vector<uint8_t> valid;  // per output column: a validity buffer in the reference's memory
   };
   std::vector<Batch> batches;
   for (int p = 0; p < P; p++) {
@@ -1864,13 +2018,15 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
       }
       seg_rows.push_back({s, r});
     }
-    std::vector<int64_t> rows = ipc_batch_rows(seq, bs, h->opts.prefer_spill != 0);
+    const std::vector<CutBatch> cuts = ipc_cut(seq, bs, h->opts.prefer_spill != 0, sticky, gnull, slot_col, nout);
     size_t si = 0;
     int64_t used = 0;  // rows of segment seg_rows[si] already taken
-    for (int64_t n : rows) {
+    for (const CutBatch& cb : cuts) {
+      const int64_t n = cb.rows;
       Batch bt;
       bt.p = p;
       bt.rows = n;
+      bt.valid = cb.valid;
       int64_t left = n;
       while (left > 0) {
         while (si < seg_rows.size() && used >= seg_rows[si].second) {
@@ -1950,6 +2106,7 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
   size_t ov = 0;  // next pair of oval
   // ---- pass 2: layout of the stream, metadata, tasks
   plan->part_len.assign(P, 0);
+  plan->raw_len.assign(P, 0);
   plan->recs.clear();
   plan->schema.assign(write_schema && schema_len > 0 ? schema : nullptr,
                       write_schema && schema_len > 0 ? schema + schema_len : nullptr);
@@ -2032,6 +2189,24 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
           total += (int64_t)oval[ov + 1] - oval[ov];
         }
         pd.n = total;
+      }
+      // the cached batch's in-memory bytes (batch_nbytes, splitter.cc:438-453)
+      {
+        const int64_t n = bt.rows, bits = (n + 7) / 8;
+        for (int oc : slot_col) {
+          const int t = h->out_types[oc];
+          plan->raw_len[p] += bt.valid[oc] ? bits : 0;
+          if (t == GZ_TYPE_BOOL) {
+            plan->raw_len[p] += bits;
+          } else if (type_is_binary(t)) {
+            int64_t bytes = 0;
+            for (auto& pd : body)
+              if (pd.kind == IPC_COPY && pd.c == oc) bytes = pd.n;
+            plan->raw_len[p] += (n + 1) * 4 + bytes;
+          } else {
+            plan->raw_len[p] += n * type_width(t);
+          }
+        }
       }
       // recompute the buffer table and offsets now that utf8 sizes are known
       bufs.clear();
@@ -2217,10 +2392,15 @@ static int ipc_compress_plan(gz_splitter* h, gz_ipc_plan* plan) {
   GZ_TRY(ipc_assemble(ctx, plan, plan->raw.p));
   const uint8_t* raw = (const uint8_t*)plan->raw.p;
   constexpr int64_t kBlock = kLz4Block;
+  // record batches of at most batch_compress_threshold rows are written
+  // uncompressed (splitter.cc:586-590: the tiny-batch options' codec is
+  // UNCOMPRESSED)
+  const int64_t tiny = h->opts.batch_compress_threshold;
   std::vector<Lz4CBlock> blocks;
   int64_t coff = 0;
   for (const auto& r : plan->recs)
-    for (const auto& b : r.bufs)
+    if (r.rows > tiny)
+      for (const auto& b : r.bufs)
       for (int64_t k = 0; k < b.second; k += kBlock) {
         const int64_t n = std::min(kBlock, b.second - k);
         blocks.push_back({raw + r.body_pos + b.first + k, (uint8_t*)(intptr_t)coff, (uint32_t)n, 0});
@@ -2293,6 +2473,14 @@ static int ipc_compress_plan(gz_splitter* h, gz_ipc_plan* plan) {
       const auto& r = plan->recs[ri];
       if (!any && !plan->schema.empty()) blob_copy(plan->schema, (int64_t)plan->schema.size());
       any = true;
+      if (r.rows <= tiny) {  // the uncompressed record batch: message and body as laid out
+        int64_t body = 0;
+        for (const auto& b : r.bufs) body = std::max<int64_t>(body, b.first + pad8(b.second));
+        const std::vector<uint8_t> m = ipc_record_batch_message(r.rows, r.nodes, r.bufs, body);
+        blob_copy(m, (int64_t)m.size());
+        if (body) dev_copy(raw + r.body_pos, body);
+        continue;
+      }
       // compressed buffer sizes
       std::vector<std::pair<int64_t, int64_t>> cb;
       int64_t boff = 0;
@@ -2371,8 +2559,13 @@ int gz_split_ipc_prepare(gz_splitter* h, const uint8_t* schema_msg, int64_t sche
   GZ_HIP(hipSetDevice(h->ctx->device));
   if (h->ipc) ipc_plan_free(h->ctx, h->ipc);
   h->ipc = new gz_ipc_plan();
+  const auto t0 = std::chrono::steady_clock::now();
   int rc = ipc_prepare(h, schema_msg, schema_len, write_schema != 0, h->ipc);
+  const auto t1 = std::chrono::steady_clock::now();
   if (!rc && h->codec == GZ_COMPRESSION_LZ4_FRAME) rc = ipc_compress_plan(h, h->ipc);
+  const auto t2 = std::chrono::steady_clock::now();
+  h->write_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+  h->compress_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(t2 - t1).count();
   if (rc) {
     ipc_plan_free(h->ctx, h->ipc);
     h->ipc = nullptr;
@@ -2392,9 +2585,39 @@ int gz_split_ipc_write(gz_splitter* h, void* dst, int64_t dst_bytes) {
   if (!dst) return fail(GZ_INVALID, "null destination");
   gz_context* ctx = h->ctx;
   GZ_HIP(hipSetDevice(ctx->device));
+  const auto t0 = std::chrono::steady_clock::now();
   GZ_TRY(ipc_assemble(ctx, pl, dst));
   GZ_HIP(hipStreamSynchronize(ctx->stream));
+  h->write_ns +=
+      std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
   return GZ_OK;
 }
+
+int gz_split_get_metrics(gz_splitter* h, gz_split_metrics* out) {
+  if (!h || !out) return fail(GZ_INVALID, "null argument");
+  drain_pid_events(h);
+  out->total_compute_pid_time = h->compute_pid_ns;
+  out->total_write_time = h->write_ns;
+  out->total_spill_time = 0;
+  out->total_compress_time = h->compress_ns;
+  out->total_bytes_written = h->ipc ? h->ipc->total : 0;
+  out->total_bytes_spilled = 0;
+  return GZ_OK;
+}
+
+int gz_split_raw_partition_lengths(gz_splitter* h, int64_t* out) {
+  if (!h || !out) return fail(GZ_INVALID, "null argument");
+  if (!h->ipc) return fail(GZ_INVALID, "gz_split_ipc_prepare must come first");
+  memcpy(out, h->ipc->raw_len.data(), sizeof(int64_t) * h->P);
+  return GZ_OK;
+}
+
+int gz_split_batch_rows(gz_splitter* h, int64_t* out) {
+  if (!h || !out) return fail(GZ_INVALID, "null argument");
+  if (!h->ipc) return fail(GZ_INVALID, "gz_split_ipc_prepare must come first");
+  *out = h->ipc->batch_rows;
+  return GZ_OK;
+}
+
 
 }  // extern "C"

@@ -29,6 +29,7 @@ class SplitOptions:
     tile_rows: int = 0
     hash_seed: int = 42
     append_row_ids: bool = False  # last output column: int32 row index within the split call
+    batch_compress_threshold: int = 256  # LZ4: record batches of at most this many rows stay uncompressed
 
     def to_c(self) -> L.GzSplitOptions:
         o = L.GzSplitOptions()
@@ -38,6 +39,7 @@ class SplitOptions:
         o.tile_rows = self.tile_rows
         o.hash_seed = self.hash_seed
         o.append_row_ids = int(self.append_row_ids)
+        o.batch_compress_threshold = self.batch_compress_threshold
         return o
 
 
@@ -90,6 +92,26 @@ class Splitter:
 
     def TotalComputePidTime(self) -> int:
         return L.lib().gz_split_compute_pid_time(self._h)
+
+    def RawPartitionLengths(self) -> np.ndarray:
+        """In-memory bytes of each partition's record batches (after the data file
+        is laid out: ipc_stream / gz_split_ipc_prepare)."""
+        out = np.zeros(self.num_partitions, np.int64)
+        L.check(L.lib().gz_split_raw_partition_lengths(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return out
+
+    def BatchRows(self) -> int:
+        """Rows of a partition buffer (CalculateSplitBatchSize) used for the cuts."""
+        n = ctypes.c_int64()
+        L.check(L.lib().gz_split_batch_rows(self._h, ctypes.byref(n)))
+        return n.value
+
+    def Metrics(self) -> Dict[str, int]:
+        """TotalComputePidTime / WriteTime / SpillTime / CompressTime (ns) and
+        TotalBytesWritten / BytesSpilled, as the JNI stop's SplitResult has them."""
+        m = L.GzSplitMetrics()
+        L.check(L.lib().gz_split_get_metrics(self._h, ctypes.byref(m)))
+        return {f: getattr(m, f) for f, _ in L.GzSplitMetrics._fields_}
 
     def num_segments(self) -> int:
         return L.lib().gz_split_num_segments(self._h)

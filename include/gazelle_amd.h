@@ -116,11 +116,16 @@ typedef struct {
   int32_t buffer_size;        /* SplitOptions.buffer_size (type.h:41), default 4096 */
   int32_t prefer_spill;       /* SplitOptions.prefer_spill (type.h:45), default 1 */
   int64_t offheap_per_task;   /* SplitOptions.offheap_per_task (type.h:40) */
-  int32_t tile_rows;          /* rows per workgroup tile: 0 = autotuned */
+  int32_t tile_rows;          /* rows per workgroup tile: 4096, 2048 or 1024; 0 = the
+                                 default, 2048 (1024-row tiles are too small for LDS
+                                 staging above 1024 partitions and become 2048) */
   int32_t hash_seed;          /* 42, the seed Spark uses (splitter.cc:1405) */
   int32_t append_row_ids;     /* 1 = add a last output column, int32, holding each
                                  row's index among the rows of its split call */
-  int32_t reserved;
+  int32_t batch_compress_threshold; /* SplitOptions.batch_compress_threshold
+                                 (type.h:43), default 256: with LZ4_FRAME, record
+                                 batches of at most this many rows are written
+                                 uncompressed (splitter.cc:586-590) */
 } gz_split_options;
 
 void gz_split_options_defaults(gz_split_options* opts);
@@ -204,11 +209,34 @@ int gz_split_set_compress_type(gz_splitter* h, int32_t codec);
  * end-of-stream marker; an empty partition writes nothing.  prepare (after
  * gz_split_stop) lays the stream out and returns its size and the per-partition
  * lengths (Splitter::PartitionLengths, splitter.h:114); write assembles it into
- * a device buffer of at least total_bytes.  offheap_per_task > 0 (batch size
- * from the first batch's row size) is NotImplemented here. */
+ * a device buffer of at least total_bytes.  With offheap_per_task > 0 the
+ * buffer rows come from CalculateSplitBatchSize (gz_split_batch_rows). */
 int gz_split_ipc_prepare(gz_splitter* h, const uint8_t* schema_msg, int64_t schema_len, int32_t write_schema,
                          int64_t* total_bytes, int64_t* partition_lengths /* P */);
 int gz_split_ipc_write(gz_splitter* h, void* dst_device, int64_t dst_bytes);
+
+/* Replaces Splitter::RawPartitionLengths (splitter.h:120-122), after
+ * gz_split_ipc_prepare: per partition the in-memory bytes (batch_nbytes,
+ * splitter.cc:438-453) of the record batches the reference caches, including the
+ * validity buffers its partition buffers hold (allocated while a column's
+ * input_has_null_ flag is set, or when an input batch with nulls adds rows). */
+int gz_split_raw_partition_lengths(gz_splitter* h, int64_t* out /* P */);
+/* Rows of a partition buffer used for the record batch cuts: buffer_size, or with
+ * offheap_per_task > 0 CalculateSplitBatchSize (splitter.cc:800-853) from the first
+ * input batch's row size (a binary column empty in that batch: NotImplemented). */
+int gz_split_batch_rows(gz_splitter* h, int64_t* out);
+
+/* The metrics of ShuffleSplitterJniWrapper.stop's SplitResult (jni_wrapper.cc:
+ * 1197-1225; Splitter::TotalComputePidTime ... TotalBytesSpilled, splitter.h:105-118). */
+typedef struct {
+  int64_t total_compute_pid_time; /* ns of the partition id passes on the device */
+  int64_t total_write_time;       /* ns laying out and assembling the data file */
+  int64_t total_spill_time;       /* 0: the device split keeps partitions in HBM */
+  int64_t total_compress_time;    /* ns compressing body buffers (LZ4_FRAME) */
+  int64_t total_bytes_written;    /* bytes of the data file (after gz_split_ipc_prepare) */
+  int64_t total_bytes_spilled;    /* 0 */
+} gz_split_metrics;
+int gz_split_get_metrics(gz_splitter* h, gz_split_metrics* out);
 /* The encapsulated IPC message (continuation token, metadata length, Arrow
  * Message flatbuffer with a RecordBatch header, MetadataVersion V5, padding) of
  * a record batch with the given field nodes (length, null_count pairs) and

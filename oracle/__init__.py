@@ -488,3 +488,124 @@ def ipc_decompress_buffer(buf) -> np.ndarray:
     if len(out) != n:
         raise ValueError(f"decompressed {len(out)} bytes, expected {n}")
     return out
+
+
+# ------------------------------------------------------------------ partition buffers
+
+_BITWIDTH_BYTES = {1: 0, 2: 1, 3: 1, 4: 2, 5: 2, 6: 4, 7: 4, 8: 8, 9: 8, 11: 4, 12: 8, 16: 4, 17: 8, 18: 8, 23: 16}
+
+
+def split_buffer_policy(batches, types: Sequence[int], pids_per_batch, P: int, buffer_size: int = 4096,
+                        offheap_per_task: int = 0, prefer_spill: bool = True) -> Dict:
+    """The reference's partition buffers over a task's input batches, batch by
+    batch as Splitter::DoSplit runs them (cpp/src/shuffle/splitter.cc:855-938):
+    CalculateSplitBatchSize (:800-853), AllocatePartitionBuffers' validity rule
+    (:610-713), SplitValidityBuffer's lazy validity (:1179-1207),
+    CacheRecordBatch's cuts and batch_nbytes (:438-608), Stop (:395-437).
+
+    Returns {"batch_rows": the last CalculateSplitBatchSize value, "cuts": rows
+    of each cached record batch per partition, "raw": RawPartitionLengths}."""
+    ncol = len(types)
+    fixed = [c for c in range(ncol) if types[c] not in (_T_NA, _T_STRING, _T_BINARY)]
+    binary = [c for c in range(ncol) if types[c] in (_T_STRING, _T_BINARY)]
+    array_idx = fixed + binary  # slot -> schema column
+    fixed_cnt = len(fixed)
+    ns = len(array_idx)
+    input_has_null = [False] * ns
+    emp = [0] * len(binary)
+    size = [0] * P
+    base = [0] * P
+    valid = [[False] * ns for _ in range(P)]
+    bytes_in = [[0] * len(binary) for _ in range(P)]  # utf8 bytes in the partition buffer
+    cuts = [[] for _ in range(P)]
+    raw = [0] * P
+    last_bs = buffer_size
+
+    def null_count(b, c):
+        vb = b.columns[c].validity
+        if vb is None:
+            return 0
+        bits = np.unpackbits(np.asarray(vb, np.uint8), bitorder="little")[: b.num_rows]
+        return int(b.num_rows - bits.sum())
+
+    def calc(b):
+        nonlocal last_bs
+        n = b.num_rows
+        for j, c in enumerate(binary):
+            if emp[j] == 0:
+                off = b.columns[c].offsets
+                length = int(off[n]) - int(off[0])
+                emp[j] = length // n if length % n == 0 else length // n + 1
+        spr = sum(emp)
+        for c in array_idx:
+            if c < fixed_cnt:  # the schema index, as the reference compares it
+                spr += _BITWIDTH_BYTES.get(types[c], 0)
+        pre = (offheap_per_task // spr // P) >> 2 if offheap_per_task > 0 and spr > 0 else buffer_size
+        last_bs = min(pre, buffer_size)
+        return last_bs
+
+    def allocate(p, new):
+        size[p] = new
+        valid[p] = [input_has_null[k] for k in range(ns)]
+
+    def cache(p, reset):
+        n = base[p]
+        if n > 0:
+            total = 0
+            for k, c in enumerate(array_idx):
+                t = types[c]
+                if valid[p][k]:
+                    total += (n + 7) // 8
+                if t in (_T_STRING, _T_BINARY):
+                    total += (n + 1) * 4 + bytes_in[p][binary.index(c)]
+                elif t == _T_BOOL:
+                    total += (n + 7) // 8
+                else:
+                    total += n * _WIDTH[t]
+            raw[p] += total
+            cuts[p].append(n)
+        base[p] = 0
+        bytes_in[p] = [0] * len(binary)
+        if reset:
+            valid[p] = [False] * ns
+
+    for b, pids in zip(batches, pids_per_batch):
+        if b.num_rows == 0:
+            continue
+        pids = np.asarray(pids)
+        cnt = np.bincount(pids, minlength=P)
+        nulls = {c: null_count(b, c) for c in array_idx}
+        for k, c in enumerate(array_idx):
+            if c < fixed_cnt and not input_has_null[k] and nulls[c] != 0:
+                input_has_null[k] = True
+        for p in range(P):
+            if cnt[p] == 0:
+                continue
+            if size[p] == 0:
+                allocate(p, max(calc(b), int(cnt[p])))
+            elif base[p] + cnt[p] > size[p]:
+                new = max(calc(b), int(cnt[p]))
+                if prefer_spill:
+                    if new > size[p]:
+                        cache(p, True)
+                        allocate(p, new)
+                    else:
+                        cache(p, False)
+                else:
+                    cache(p, True)
+                    allocate(p, new)
+        for k, c in enumerate(array_idx):
+            if nulls[c] > 0:
+                for p in np.nonzero(cnt)[0]:
+                    valid[p][k] = True
+        for j, c in enumerate(binary):
+            off = np.asarray(b.columns[c].offsets, np.int64)
+            lens = off[1:b.num_rows + 1] - off[:b.num_rows]
+            per = np.bincount(pids, weights=lens, minlength=P)
+            for p in range(P):
+                bytes_in[p][j] += int(per[p])
+        for p in range(P):
+            base[p] += int(cnt[p])
+    for p in range(P):
+        cache(p, True)
+    return {"batch_rows": last_bs, "cuts": cuts, "raw": raw}

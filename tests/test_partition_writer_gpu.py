@@ -209,7 +209,9 @@ def test_lz4_encoder_block_format_cases(ctx):
     for d in cases:
         off = np.array([0, len(d)], np.int32)
         hb = HostBatch(1, [HostColumn(T.BINARY, np.frombuffer(d, np.uint8).copy(), None, off)])
-        sp = Splitter.Make(ctx, "single", types, 1)
+        # one-row batches: threshold 0 so the encoder runs (the default leaves
+        # batches of at most 256 rows uncompressed)
+        sp = Splitter.Make(ctx, "single", types, 1, [], SplitOptions(batch_compress_threshold=0))
         sp.SetCompressType("lz4")
         sp.Split(DeviceBatch.from_host(hb))
         sp.Stop()
