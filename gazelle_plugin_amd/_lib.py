@@ -145,6 +145,10 @@ _SIGS = {
     "gz_ipc_read_batches": (ctypes.c_int, [ctypes.c_void_p, _I64, _I64, _I64, _I32, _PI32, _I32, _PI64, _PI64,
                                            _PI64, _PI32, _PI64]),
     "gz_ipc_decompress": (ctypes.c_int, [_P, ctypes.c_void_p, _I64, _I64, _I32, _PI32, ctypes.c_void_p, _I64]),
+    "gz_agg_result_bitmap": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_void_p)]),
+    "gz_agg_set_result_bitmap": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(ctypes.c_void_p)]),
+    "gz_probe_result_bitmap": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_void_p)]),
+    "gz_validity_bitmap": (ctypes.c_int, [_P, ctypes.c_void_p, _I64, ctypes.c_void_p]),
     "gz_ipc_decompress_noalloc": (ctypes.c_int, [_P, ctypes.c_void_p, _I64, _I64, _I32, _PI32, ctypes.c_void_p, _I64]),
 }
 

@@ -1749,6 +1749,8 @@ struct gz_agg {
   std::vector<int64_t> ngroups, group_off;
   DBuf skeys, svals, tkeys, tvals, cursor, ccpre, counts, sort_tmp;
   std::vector<DBuf> res_values, res_valid;
+  std::vector<DBuf> res_bits;  // Arrow validity bitmaps per (instance, column), packed on request
+  std::vector<uint8_t> bits_ready;
   int64_t total_groups = 0;
   bool inst_dirty = true;
   // planned form (gz_agg_set_prepare / run / capture / launch)
@@ -2317,6 +2319,7 @@ static int agg_process(gz_agg* h, int32_t nb, const gz_batch* batches, const int
 
 static int agg_finish(gz_agg* h) {
   if (h->finished) return GZ_OK;
+  std::fill(h->bits_ready.begin(), h->bits_ready.end(), 0);
   gz_context* ctx = h->ctx;
   GZ_HIP(hipSetDevice(ctx->device));
   if (h->L.ready) {
@@ -3056,6 +3059,7 @@ int gz_agg_result_binary(gz_agg* h, int32_t col, void** offsets, void** data, vo
 int gz_agg_release(gz_agg* h) {
   if (!h) return GZ_OK;
   if (h->mk) {
+    for (auto& b : h->res_bits) ctx_free(h->ctx, &b);
     mk_release(h->mk);
     delete h;
     return GZ_OK;
@@ -3071,6 +3075,7 @@ int gz_agg_release(gz_agg* h) {
                   &h->recs, &h->tcount, &h->tprefix, &h->tcsum, &h->ttot, &h->it0, &h->it1, &h->d_fix,
                   &h->d_fix_tiles, &h->d_gbatch, &h->L.d_fix, &h->L.d_gb})
     ctx_free(ctx, b);
+  for (auto& b : h->res_bits) ctx_free(ctx, &b);
   if (h->L.exec) hipGraphExecDestroy(h->L.exec);
   if (h->L.graph) hipGraphDestroy(h->L.graph);
   delete h->join;
@@ -3112,6 +3117,37 @@ int gz_agg_set_result_column(gz_agg* h, int32_t inst, int32_t col, void** values
   if (valid) *valid = (char*)h->res_valid[col].p + h->group_off[inst];
   return GZ_OK;
 }
+
+int gz_agg_set_result_bitmap(gz_agg* h, int32_t inst, int32_t col, void** bitmap) {
+  if (!h || !bitmap) return fail(GZ_INVALID, "null argument");
+  void* valid = nullptr;
+  int64_t n = 0;
+  int nres = 0;
+  if (h->mk) {
+    if (inst != 0) return fail(GZ_INDEX_ERROR, "instance out of range");
+    GZ_TRY(mk_result_column(h->mk, col, nullptr, &valid));
+    GZ_TRY(mk_num_groups(h->mk, &n));
+    nres = gz_agg_num_result_columns(h);
+  } else {
+    GZ_TRY(gz_agg_set_result_column(h, inst, col, nullptr, &valid));
+    n = h->ngroups[inst];
+    nres = (int)h->plan.results.size();
+  }
+  const size_t k = (size_t)inst * nres + col;
+  if (h->res_bits.size() < (size_t)h->ninst * nres) {
+    h->res_bits.resize((size_t)h->ninst * nres);
+    h->bits_ready.resize((size_t)h->ninst * nres, 0);
+  }
+  if (!h->bits_ready[k]) {
+    GZ_TRY(ctx_ensure(h->ctx, &h->res_bits[k], (size_t)std::max<int64_t>((n + 63) / 64 * 8, 8)));
+    GZ_TRY(pack_validity(h->ctx, (const uint8_t*)valid, n, (uint8_t*)h->res_bits[k].p));
+    h->bits_ready[k] = 1;
+  }
+  *bitmap = h->res_bits[k].p;
+  return GZ_OK;
+}
+
+int gz_agg_result_bitmap(gz_agg* h, int32_t col, void** bitmap) { return gz_agg_set_result_bitmap(h, 0, col, bitmap); }
 
 int gz_agg_set_placement(gz_agg* h, int32_t inst, int32_t* out) {
   if (!h || !out || inst < 0 || inst >= h->ninst) return fail(GZ_INVALID, "bad argument");

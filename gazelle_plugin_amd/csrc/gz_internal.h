@@ -160,6 +160,9 @@ struct RelationView {
   const void* values[32];    // fixed width: width bytes per row; bool: one byte per row
 };
 int relation_view(gz_relation* r, RelationView* v);
+// Validity bytes (1 = valid) to an Arrow bitmap of (n + 63) / 64 * 8 bytes (1 bits
+// past row n), on the context's stream (gz_serve.hip).
+int pack_validity(gz_context* ctx, const uint8_t* valid, int64_t n, uint8_t* bits);
 
 // Hash aggregate grouped by several key columns or a utf8 key (gz_keys.hip): a device key
 // dictionary maps every row's key tuple to a dense code, and a single-key

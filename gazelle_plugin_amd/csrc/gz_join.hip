@@ -1087,7 +1087,7 @@ struct gz_probe {
   std::vector<int> pay_off;  // per output column: byte offset in the record, -1 not in it
   struct Out {
     int type = 0, width = 0;
-    DBuf values, valid, offsets, data, lens;
+    DBuf values, valid, offsets, data, lens, bits;  // bits: Arrow bitmap, packed on request
   };
   std::vector<Out> outs;
   int64_t out_rows = 0;
@@ -1474,12 +1474,22 @@ int gz_probe_result_column(gz_probe* h, int32_t col, void** values, void** valid
   return GZ_OK;
 }
 
+int gz_probe_result_bitmap(gz_probe* h, int32_t col, void** bitmap) {
+  if (!h || !bitmap) return fail(GZ_INVALID, "null argument");
+  if (col < 0 || col >= (int)h->outs.size()) return fail(GZ_INDEX_ERROR, "result column out of range");
+  auto& o = h->outs[col];
+  GZ_TRY(ctx_ensure(h->ctx, &o.bits, (size_t)std::max<int64_t>((h->out_rows + 63) / 64 * 8, 8)));
+  GZ_TRY(pack_validity(h->ctx, (const uint8_t*)o.valid.p, h->out_rows, (uint8_t*)o.bits.p));
+  *bitmap = o.bits.p;
+  return GZ_OK;
+}
+
 int gz_probe_release(gz_probe* h) {
   if (!h) return GZ_OK;
   gz_context* ctx = h->ctx;
   kd_scratch_free(ctx, &h->ksc);
   for (auto& o : h->outs)
-    for (DBuf* b : {&o.values, &o.valid, &o.offsets, &o.data, &o.lens}) ctx_free(ctx, b);
+    for (DBuf* b : {&o.values, &o.valid, &o.offsets, &o.data, &o.lens, &o.bits}) ctx_free(ctx, b);
   for (DBuf* b : {&h->d_tiles, &h->d_bufs, &h->match, &h->tile_count, &h->tile_off, &h->scan_tmp,
                   &h->pair_probe, &h->pair_build, &h->payload})
     ctx_free(ctx, b);

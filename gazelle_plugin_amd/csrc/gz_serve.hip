@@ -31,17 +31,28 @@ namespace gz {
 constexpr int kMaxJoinAggCols = 16;  // join result columns an aggregate may follow
 
 // Validity bytes (1 = valid) of a join output column to an Arrow bitmap, one
-// thread per output byte (8 rows).
-__global__ void pack_valid_kernel(const uint8_t* valid, int64_t n, uint8_t* bits) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n + 7) / 8;
+// thread per output byte (8 rows); bytes from (n + 7) / 8 up to nbytes are 0xff.
+__global__ void pack_valid_kernel(const uint8_t* valid, int64_t n, uint8_t* bits, int64_t nbytes) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbytes;
        i += (int64_t)gridDim.x * blockDim.x) {
     uint32_t b = 0;
     for (int k = 0; k < 8; k++) {
       int64_t r = i * 8 + k;
-      if (r < n && valid[r]) b |= 1u << k;
+      if (r >= n || valid[r]) b |= 1u << k;
     }
     bits[i] = (uint8_t)b;
   }
+}
+
+// Validity bytes -> an Arrow bitmap (LSB first) padded with 1 bits to a multiple
+// of 64 rows (Arrow's 8-byte buffer padding), on the context's stream.
+int pack_validity(gz_context* ctx, const uint8_t* valid, int64_t n, uint8_t* bits) {
+  if (n <= 0) return GZ_OK;
+  const int64_t nb = (n + 63) / 64 * 8;
+  const unsigned g = (unsigned)std::min<int64_t>((nb + 255) / 256, 4096);
+  hipLaunchKernelGGL(pack_valid_kernel, dim3(g), dim3(256), 0, ctx->stream, valid, n, bits, nb);
+  GZ_HIP(hipGetLastError());
+  return GZ_OK;
 }
 
 }  // namespace gz
@@ -238,7 +249,7 @@ int run_fragment(gz_server* s, Slot& sl, int i, int64_t* out_rows) {
         if (n > 0) {
           unsigned g = (unsigned)std::min<int64_t>(((n + 7) / 8 + 255) / 256, 4096);
           hipLaunchKernelGGL(pack_valid_kernel, dim3(g), dim3(256), 0, sl.ctx->stream, (const uint8_t*)valid, n,
-                             (uint8_t*)sl.bits[c].p);
+                             (uint8_t*)sl.bits[c].p, (n + 7) / 8);
         }
         addrs.push_back((int64_t)(uintptr_t)sl.bits[c].p);
         sizes.push_back((n + 7) / 8);
@@ -451,6 +462,12 @@ int gz_server_release_results(gz_server* s) {
       p = nullptr;
     }
   return GZ_OK;
+}
+
+int gz_validity_bitmap(gz_context* ctx, const void* valid_bytes, int64_t n, void* bitmap) {
+  if (!ctx || n < 0 || (n > 0 && (!valid_bytes || !bitmap))) return fail(GZ_INVALID, "gz_validity_bitmap: bad argument");
+  GZ_HIP(hipSetDevice(ctx->device));
+  return pack_validity(ctx, (const uint8_t*)valid_bytes, n, (uint8_t*)bitmap);
 }
 
 int gz_server_destroy(gz_server* s) {

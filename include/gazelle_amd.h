@@ -355,6 +355,11 @@ int gz_agg_result_column(gz_agg* h, int32_t col, void** values, void** valid_byt
  * offsets as its values.  Replaces the StringType UniqueAction's Finish
  * (actions_impl.cc:132-290). */
 int gz_agg_result_binary(gz_agg* h, int32_t col, void** offsets, void** data, void** valid_bytes);
+/* The result column's validity as an Arrow bitmap (LSB first, 1 = valid, padded
+ * with 1 bits to a multiple of 64 rows), as ResultIterator::Next's RecordBatch
+ * holds it (result_iterator.h:37-82): a device buffer owned by the handle, packed
+ * on the context's stream, valid until the next finish or release. */
+int gz_agg_result_bitmap(gz_agg* h, int32_t col, void** bitmap);
 int gz_agg_release(gz_agg* h);
 
 /* Aggregate sets: many independent GROUP BYs (instances) sharing one plan,
@@ -371,6 +376,8 @@ int gz_agg_set_result_column(gz_agg* h, int32_t inst, int32_t col, void** values
                              void** valid_bytes);
 /* Placement the autotuner chose for an instance: 1 = LDS first level, 2 = HBM. */
 int gz_agg_set_placement(gz_agg* h, int32_t inst, int32_t* out);
+/* gz_agg_result_bitmap of one instance. */
+int gz_agg_set_result_bitmap(gz_agg* h, int32_t inst, int32_t col, void** bitmap);
 
 /* Planned, replayable form of a set.  gz_agg_set_prepare sizes every
  * instance's table for the given batches (cardinality estimate, allocation,
@@ -459,6 +466,9 @@ int gz_probe_make_keys(gz_relation* r, int32_t num_probe_cols, const int32_t* pr
                        const int32_t* probe_key_cols, int32_t join_type, int32_t null_aware_anti,
                        int32_t num_result_cols, const int32_t* result_sides, const int32_t* result_indices,
                        gz_probe** out);
+/* The last call's output column validity as an Arrow bitmap (as
+ * gz_agg_result_bitmap), owned by the handle, valid until the next call. */
+int gz_probe_result_bitmap(gz_probe* p, int32_t col, void** bitmap);
 int gz_probe_release(gz_probe* p);
 
 /* ---- fused join -> aggregate -------------------------------------------------
@@ -558,6 +568,13 @@ int gz_server_fragment_agg(gz_server* s, int32_t i, gz_agg** out);
 int gz_server_fragment_split(gz_server* s, int32_t i, gz_splitter** out);
 int gz_server_release_results(gz_server* s);
 int gz_server_destroy(gz_server* s);
+
+/* Validity bytes (1 = valid) of n rows to an Arrow bitmap in a caller-provided
+ * device buffer of (n + 63) / 64 * 8 bytes (1 bits past row n), on the context's
+ * stream: for result columns taken as bytes (gz_agg_result_column,
+ * gz_probe_result_column).  No counterpart in the reference, whose builders
+ * produce bitmaps. */
+int gz_validity_bitmap(gz_context* ctx, const void* valid_bytes, int64_t n, void* bitmap);
 
 #ifdef __cplusplus
 }
