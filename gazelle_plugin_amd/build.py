@@ -1,4 +1,6 @@
-"""Builds libgazelle_amd.so (HIP, gfx950) in-tree with hipcc."""
+"""Builds libgazelle_amd.so (HIP, gfx950) in-tree with hipcc, and over it
+libgazelle_amd_arrow.so, the C++ operator interfaces on Arrow C++ (g++; the
+Arrow C++ headers and library come with pyarrow)."""
 import os
 import subprocess
 import sys
@@ -17,6 +19,42 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Wno-unused-result", "-Wno-unused-value"]
 
 
+ARROW_LIB = os.path.join(HERE, "libgazelle_amd_arrow.so")
+ARROW_SOURCES = [os.path.join(HERE, "cpp", "gz_arrow.cc")]
+
+
+def pyarrow_dir() -> str:
+    import pyarrow
+
+    return os.path.dirname(pyarrow.__file__)
+
+
+def _run(cmd, verbose, what):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l)
+    if res.returncode != 0:
+        raise RuntimeError(what + " failed:\n" + err)
+    if verbose and err.strip():
+        print(err, file=sys.stderr)
+
+
+def build_arrow_layer(verbose: bool = False) -> str:
+    """libgazelle_amd_arrow.so: host C++ over libgazelle_amd.so and libarrow
+    (Arrow C++ 25 needs C++20)."""
+    pa = pyarrow_dir()
+    cmd = (["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-Wall", "-D__HIP_PLATFORM_AMD__",
+            "-I", os.path.join(HERE, "..", "include"), "-I", os.path.join(pa, "include"), "-I", "/opt/rocm/include"]
+           + ARROW_SOURCES
+           + ["-o", ARROW_LIB + ".tmp", "-L", HERE, "-lgazelle_amd", "-L", pa, "-l:libarrow.so.2500",
+              "-L", "/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + pa,
+              "-Wl,-rpath,/opt/rocm/lib"])
+    _run(cmd, verbose, "g++ (Arrow layer)")
+    os.replace(ARROW_LIB + ".tmp", ARROW_LIB)
+    return ARROW_LIB
+
+
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     cmd = [HIPCC] + FLAGS + ["-I", os.path.join(HERE, "..", "include")] + srcs + ["-o", LIB + ".tmp"]
@@ -29,6 +67,7 @@ def build(verbose: bool = False) -> str:
     if verbose and err.strip():
         print(err, file=sys.stderr)
     os.replace(LIB + ".tmp", LIB)
+    build_arrow_layer(verbose)
     return LIB
 
 

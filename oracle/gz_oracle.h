@@ -27,7 +27,7 @@ typedef struct {
  * third_party/murmurhash/murmurhash32.h:74-108 with a caller seed). */
 int32_t gzo_hash_int(int32_t v, int32_t seed);
 int32_t gzo_hash_long(int64_t v, int32_t seed);
-/* Spark hashUnsafeBytes (legacy per-byte tail).  Parity unpinned. */
+/* Spark hashUnsafeBytes (legacy per-byte tail). */
 int32_t gzo_hash_bytes(const uint8_t* p, int32_t len, int32_t seed);
 /* Join-internal hash: murmurhash32::hash32 over the double bits of a numeric
  * key, seed 0 (murmurhash32.h:251-265). */

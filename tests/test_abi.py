@@ -43,3 +43,30 @@ def test_error_path_without_device():
     rc = lib.gz_split_make(None, b"hash", 0, None, 2, None, 0, None, None)
     assert rc == 4  # Invalid
     assert b"null" in lib.gz_last_error()
+
+
+ARROW_LIB = os.path.join(ROOT, "gazelle_plugin_amd", "libgazelle_amd_arrow.so")
+ARROW_HEADER = os.path.join(ROOT, "include", "gazelle_amd_arrow.h")
+
+
+def test_arrow_layer_loads_and_defines_the_declared_classes():
+    """libgazelle_amd_arrow.so loads (libarrow, libgazelle_amd and the HIP
+    runtime resolve) and defines the factories and iterator methods its header
+    declares."""
+    if not os.path.exists(ARROW_LIB):
+        pytest.skip("libgazelle_amd_arrow.so not built")
+    import subprocess
+
+    ctypes.CDLL(ARROW_LIB)
+    syms = subprocess.run(["nm", "-D", "-C", "--defined-only", ARROW_LIB], capture_output=True, text=True,
+                          check=True).stdout
+    for name in ("gazelle_amd::Splitter::Make(", "gazelle_amd::Splitter::Split(", "gazelle_amd::Splitter::Stop()",
+                 "gazelle_amd::Splitter::CompressedSize(", "gazelle_amd::Splitter::SetCompressType(",
+                 "gazelle_amd::Splitter::SpillFixedSize(", "gazelle_amd::Splitter::RawPartitionBytes()",
+                 "gazelle_amd::HashAggregateKernel::Make(", "gazelle_amd::HashAggregateKernel::MakeResultIterator(",
+                 "gazelle_amd::HashRelationKernel::Make(", "gazelle_amd::HashRelationKernel::Evaluate(",
+                 "gazelle_amd::HashRelationKernel::MakeResultIterator(", "gazelle_amd::ConditionedProbeKernel::Make(",
+                 "gazelle_amd::ConditionedProbeKernel::MakeResultIterator(", "gazelle_amd::Context::Make(",
+                 "gazelle_amd::DeviceBatch::Make(", "gazelle_amd::HashRelation::NumKeys()", "gazelle_amd::FromGz("):
+        assert name in syms, name
+        assert name.split("(")[0].split("::")[-1] in open(ARROW_HEADER).read()
