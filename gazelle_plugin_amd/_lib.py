@@ -190,6 +190,7 @@ _OPTIONAL_SIGS = {
     "gz_server_destroy": (ctypes.c_int, [_P]),
     "gz_relation_make": (ctypes.c_int, [_P, _I32, _PI32, _I32, _I32, _PP]),
     "gz_relation_make_keys": (ctypes.c_int, [_P, _I32, _PI32, _I32, _PI32, _I32, _PP]),
+    "gz_relation_normalize_keys": (ctypes.c_int, [_P, _I32, _PI32]),
     "gz_relation_append": (ctypes.c_int, [_P, _I64, _PI64, _PI64, _I32]),
     "gz_relation_append_batches": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch)]),
     "gz_relation_finish": (ctypes.c_int, [_P]),

@@ -616,7 +616,7 @@ class RelationIterator : public ResultIterator<HashRelation> {
 
 arrow::Status HashRelationKernel::Make(std::shared_ptr<Context> ctx, const arrow::FieldVector& input_field_list,
                                        const std::vector<int>& key_indices, int builder_type,
-                                       std::shared_ptr<KernalBase>* out) {
+                                       std::shared_ptr<KernalBase>* out, const std::vector<bool>& normalize_keys) {
   if (!ctx) {
     ARROW_ASSIGN_OR_RAISE(ctx, Context::Make(0));
   }
@@ -628,6 +628,12 @@ arrow::Status HashRelationKernel::Make(std::shared_ptr<Context> ctx, const arrow
   GZ_RETURN(gz_relation_make_keys(ctx->handle(), static_cast<int32_t>(types.size()), types.data(),
                                   static_cast<int32_t>(keys.size()), keys.data(), builder_type, &k->r_),
             "HashRelationKernel::Make");
+  if (std::find(normalize_keys.begin(), normalize_keys.end(), true) != normalize_keys.end()) {
+    std::vector<int32_t> flags(key_indices.size(), 0);
+    for (size_t j = 0; j < flags.size() && j < normalize_keys.size(); j++) flags[j] = normalize_keys[j] ? 1 : 0;
+    GZ_RETURN(gz_relation_normalize_keys(k->r_, static_cast<int32_t>(flags.size()), flags.data()),
+              "HashRelationKernel::Make");
+  }
   k->kernel_name_ = "HashRelationKernel";
   *out = k;
   return arrow::Status::OK();

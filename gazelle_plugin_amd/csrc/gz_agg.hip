@@ -1866,10 +1866,19 @@ static int build_plan(AggPlan& P, int ncols, const int32_t* types, int nact, con
         }
         break;
       case GZ_ACTION_AVG_BY_COUNT:
-        if (!col_ok(a.input0) || !col_ok(a.input1) || types[a.input1] != GZ_TYPE_INT64)
-          return fail(GZ_INVALID, "avgByCount needs (sum, int64 count) inputs");
+        if (!col_ok(a.input0) || !col_ok(a.input1)) return fail(GZ_INVALID, "avgByCount needs (sum, count) inputs");
         r.lane_a = add_lane(P, L_SUMF, a.input0, -1);
-        r.lane_b = add_lane(P, L_SUMCNT, a.input0, a.input1);
+        if (global) {
+          // AvgByCountAction::Evaluate (actions_impl.cc:3818-3838): arrow Sum of
+          // each input, each skipping its own nulls; any integer count type
+          if (!is_int_t(types[a.input1])) return fail(GZ_INVALID, "avgByCount needs an integer count input");
+          r.lane_b = add_lane(P, L_SUMI, a.input1, -1);
+        } else {
+          // the grouped lambda (:3785-3795) adds the count of rows whose sum
+          // input is non-null, read as int64 (undefined for narrower columns)
+          if (types[a.input1] != GZ_TYPE_INT64) return fail(GZ_INVALID, "avgByCount needs (sum, int64 count) inputs");
+          r.lane_b = add_lane(P, L_SUMCNT, a.input0, a.input1);
+        }
         r.rtype = GZ_TYPE_DOUBLE;
         P.results.push_back(r);
         break;

@@ -80,6 +80,20 @@ __device__ __forceinline__ int bit_get(const uint8_t* bm, int64_t i) {
   return (gld(bm + (i >> 3)) >> (i & 7)) & 1;
 }
 
+// ---- join key normalisation ---------------------------------------------------
+// Spark's "normalize" of a floating join key, normalize_nan_zero
+// (precompile/gandiva.h:214-222): every NaN becomes one NaN and |x| < 1e-7
+// becomes 0.0.  On raw key bits of width 4 (float) or 8 (double).
+__device__ __forceinline__ double norm_nan_zero(double d) {
+  if (d != d) return __longlong_as_double(0x7ff8000000000000ll);
+  return fabs(d) < 0.0000001 ? 0.0 : d;
+}
+__device__ __forceinline__ uint64_t norm_key_bits(uint64_t raw, int width) {
+  if (width == 8) return (uint64_t)__double_as_longlong(norm_nan_zero(__longlong_as_double((long long)raw)));
+  const float f = (float)norm_nan_zero((double)__uint_as_float((uint32_t)raw));
+  return (uint64_t)__float_as_uint(f);
+}
+
 // ---- wave helpers -------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() {

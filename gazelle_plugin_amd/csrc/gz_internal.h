@@ -183,9 +183,15 @@ void kd_scratch_free(gz_context* ctx, KeyScratch* sc);
 int kd_encode(KeyDict* d, gz_context* ctx, KeyScratch* sc, int32_t ncols, const int32_t* types,
               const int32_t* key_cols, int32_t nb, const gz_batch* batches, int mode, const int64_t** codes,
               int64_t* rows_out);
-// The batches' buffer lists with the int64 code column appended.
+// Normalise the float/double keys whose bit is set in mask (Spark's join-key
+// "normalize") before they are hashed and compared; before the first insert.
+int kd_set_normalize(KeyDict* d, uint32_t mask);
+// The batches' buffer lists with the int64 code column appended; valid_col >= 0
+// gives the code column that column's validity (a one-key dictionary: a null
+// key is then a null code).
 void with_code_column(int32_t ncols, const int32_t* types, int32_t nb, const gz_batch* batches, const int64_t* codes,
-                      std::vector<int64_t>& addrs, std::vector<int64_t>& sizes, std::vector<gz_batch>& out);
+                      std::vector<int64_t>& addrs, std::vector<int64_t>& sizes, std::vector<gz_batch>& out,
+                      int valid_col = -1);
 struct MultiKey;
 int mk_make(gz_context* ctx, int32_t ncols, const int32_t* types, int32_t nact, const gz_agg_action* acts,
             const gz_agg_options* opts, MultiKey** out);

@@ -50,9 +50,11 @@ __device__ __forceinline__ uint64_t jmix64(uint64_t k) {
 }
 
 // Join key of a typed value, widened to 64 bits: integers sign/zero extended,
-// booleans 0/1, floats as the bits of the double value.  *nan marks a NaN key,
-// which never equals anything (the reference compares typed keys with ==).
-__device__ __forceinline__ int64_t join_key(const void* v, int t, int64_t r, bool* nan) {
+// booleans 0/1, floats as the bits of the double value (after Spark's
+// normalize when norm is set).  *nan marks a NaN key, which never equals
+// anything: the reference compares one typed key with == after its hash
+// (hash_relation.h:368-383), normalised or not.
+__device__ __forceinline__ int64_t join_key(const void* v, int t, int64_t r, bool* nan, int norm = 0) {
   *nan = false;
   switch (t) {
     case GZ_TYPE_BOOL: return bit_get((const uint8_t*)v, r);
@@ -65,11 +67,13 @@ __device__ __forceinline__ int64_t join_key(const void* v, int t, int64_t r, boo
     case GZ_TYPE_FLOAT: {
       double d = (double)((const float*)v)[r];
       *nan = d != d;
+      if (norm) d = (double)(float)norm_nan_zero(d);
       return __double_as_longlong(d);
     }
     case GZ_TYPE_DOUBLE: {
       double d = ((const double*)v)[r];
       *nan = d != d;
+      if (norm) d = norm_nan_zero(d);
       return __double_as_longlong(d);
     }
     default: return ((const int64_t*)v)[r];
@@ -151,7 +155,7 @@ __global__ void join_table_init_kernel(int64_t* keys, int32_t* start, int32_t* c
 //        [3] rows with key kJEmpty, [4] first such row
 __global__ void join_build_kernel(const uint8_t* kvalid, const uint8_t* kvals, int ktype, int kwidth,
                                   int64_t n, int64_t* keys, int32_t* start, int32_t* count,
-                                  uint64_t cap, int32_t* row_slot, int skip_dup, int codes,
+                                  uint64_t cap, int32_t* row_slot, int skip_dup, int codes, int norm,
                                   unsigned long long* flags) {
   const uint64_t mask = cap - 1;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
@@ -167,7 +171,7 @@ __global__ void join_build_kernel(const uint8_t* kvalid, const uint8_t* kvals, i
         key = kvals[r];
         nan = false;
       } else {
-        key = join_key(kvals, ktype, r, &nan);
+        key = join_key(kvals, ktype, r, &nan, norm);
       }
       // a NaN build key can never be matched; a multi-column key with a null
       // part (code -1) is never inserted (hash_relation.h:186-187)
@@ -297,6 +301,7 @@ struct ProbeParams {
   int bstride;
   int key_col, key_type;
   int join_type, null_aware;
+  int normalize;            // Spark's normalize on a float/double key
   int build_has_null;
   int64_t build_keys;       // keys in the table (null keys excluded)
   JTable J;
@@ -381,7 +386,7 @@ __device__ __forceinline__ uint32_t probe_keys(const ProbeParams& p, const PTile
 #pragma unroll
       for (int k = 0; k < RPT; k++) {
         bool nan = false;
-        key[k] = lr0 + k * S < n ? join_key(kd, p.key_type, rb + lr0 + k * S, &nan) : 0;
+        key[k] = lr0 + k * S < n ? join_key(kd, p.key_type, rb + lr0 + k * S, &nan, p.normalize) : 0;
         if (nan) *nanm |= 1u << k;
       }
       break;
@@ -768,6 +773,7 @@ struct gz_relation {
   int ncols = 0;
   std::vector<int> types;
   int key_col = 0, builder_type = 1;
+  bool normalize = false;  // Spark's normalize on the (one) float/double key
   std::vector<ColStore> cols;
   int64_t n = 0, cap_rows = 0;
   // table
@@ -941,7 +947,10 @@ int gz_relation_append_batches(gz_relation* r, int32_t num_batches, const gz_bat
     int64_t rows = 0;
     GZ_TRY(kd_encode(r->kd, r->ctx, &r->ksc, r->ocols, r->otypes.data(), r->kcols.data(), num_batches, batches, 1,
                      &codes, &rows));
-    with_code_column(r->ocols, r->otypes.data(), num_batches, batches, codes, kaddrs, ksizes, kbatches);
+    // one utf8 key: a null key is a null code, recorded as the relation's null
+    // key (InsertNull, hash_relation.h:244-270)
+    with_code_column(r->ocols, r->otypes.data(), num_batches, batches, codes, kaddrs, ksizes, kbatches,
+                     r->kcols.size() == 1 ? r->kcols[0] : -1);
     batches = kbatches.data();
   }
   // reserve once so the column store grows a single time
@@ -983,7 +992,8 @@ int gz_relation_finish(gz_relation* r) {
     hipLaunchKernelGGL(join_build_kernel, dim3(gb), dim3(256), 0, st, (const uint8_t*)kc.valid.p,
                        (const uint8_t*)kc.values.p, kc.type, kc.width, n, (int64_t*)r->keys.p,
                        (int32_t*)r->start.p, (int32_t*)r->count.p, cap, (int32_t*)r->row_slot.p,
-                       r->builder_type == 3 ? 1 : 0, r->kd ? 1 : 0, (unsigned long long*)r->flags.p);
+                       r->builder_type == 3 ? 1 : 0, r->kd ? 1 : 0, r->normalize ? 1 : 0,
+                       (unsigned long long*)r->flags.p);
   }
   GZ_HIP(hipGetLastError());
   GZ_HIP(hipMemcpyAsync(hflags, r->flags.p, sizeof(hflags), hipMemcpyDeviceToHost, st));
@@ -1105,7 +1115,8 @@ static int probe_codes(gz_probe* h, int32_t nb, const gz_batch* batches, std::ve
   int64_t rows = 0;
   GZ_TRY(kd_encode(h->kd, h->ctx, &h->ksc, h->ocols, h->otypes.data(), h->kcols.data(), nb, batches, 2, &codes,
                    &rows));
-  with_code_column(h->ocols, h->otypes.data(), nb, batches, codes, addrs, sizes, out);
+  with_code_column(h->ocols, h->otypes.data(), nb, batches, codes, addrs, sizes, out,
+                   h->kcols.size() == 1 ? h->kcols[0] : -1);
   return GZ_OK;
 }
 
@@ -1162,6 +1173,7 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   p.key_type = h->ptypes[h->key_col];
   p.join_type = h->join_type;
   p.null_aware = h->null_aware;
+  p.normalize = r->normalize ? 1 : 0;
   p.build_has_null = r->has_null;
   p.build_keys = r->nkeys;
   p.tile_rows = TILE;
@@ -1526,6 +1538,24 @@ int gz_relation_make_keys(gz_context* ctx, int32_t num_cols, const int32_t* col_
   return GZ_OK;
 }
 
+int gz_relation_normalize_keys(gz_relation* r, int32_t num_keys, const int32_t* normalize) {
+  if (!r || (num_keys > 0 && !normalize)) return fail(GZ_INVALID, "gz_relation_normalize_keys: null argument");
+  if (r->n > 0 || r->finished) return fail(GZ_INVALID, "key normalisation is set before the first append");
+  const int K = r->kd ? (int)r->kcols.size() : 1;
+  if (num_keys != K) return fail(GZ_INVALID, "the relation has " + std::to_string(K) + " key columns");
+  uint32_t mask = 0;
+  for (int j = 0; j < K; j++) {
+    if (!normalize[j]) continue;
+    const int t = r->kd ? r->otypes[r->kcols[j]] : r->types[r->key_col];
+    if (t != GZ_TYPE_FLOAT && t != GZ_TYPE_DOUBLE)
+      return fail(GZ_TYPE_ERROR, "only float and double keys are normalised (key " + std::to_string(j) + ")");
+    mask |= 1u << j;
+  }
+  if (r->kd) return kd_set_normalize(r->kd, mask);
+  r->normalize = mask != 0;
+  return GZ_OK;
+}
+
 int gz_probe_make_keys(gz_relation* r, int32_t num_probe_cols, const int32_t* probe_types, int32_t num_keys,
                        const int32_t* probe_key_cols, int32_t join_type, int32_t null_aware_anti,
                        int32_t num_result_cols, const int32_t* result_sides, const int32_t* result_indices,
@@ -1543,7 +1573,12 @@ int gz_probe_make_keys(gz_relation* r, int32_t num_probe_cols, const int32_t* pr
     if (probe_types[probe_key_cols[j]] != r->otypes[r->kcols[j]])
       return fail(GZ_TYPE_ERROR, "probe and build key types differ");
   }
-  if (null_aware_anti) return fail(GZ_NOT_IMPLEMENTED, "null-aware anti join with several keys");
+  // Several keys: the reference's UnsafeRow anti probe (conditioned_probe_kernel.cc:957-982)
+  // never reads is_null_aware_anti_join_: a probe row is kept unless its key
+  // row exists, and a key row with a null part never does (the build skips
+  // such rows).  So the null-aware anti join over several keys is the plain
+  // one; one utf8 key keeps the one-key null-aware rules (:1016-1031).
+  const int na = num_keys == 1 ? null_aware_anti : 0;
   for (int c = 0; c < num_result_cols; c++) {
     if (!result_sides || !result_indices) break;
     const int lim = result_sides[c] == 0 ? r->ocols : num_probe_cols;  // the appended code column is internal
@@ -1553,7 +1588,7 @@ int gz_probe_make_keys(gz_relation* r, int32_t num_probe_cols, const int32_t* pr
   std::vector<int32_t> ext(probe_types, probe_types + num_probe_cols);
   ext.push_back(GZ_TYPE_INT64);
   gz_probe* h = nullptr;
-  GZ_TRY(gz_probe_make(r, num_probe_cols + 1, ext.data(), num_probe_cols, join_type, 0, num_result_cols,
+  GZ_TRY(gz_probe_make(r, num_probe_cols + 1, ext.data(), num_probe_cols, join_type, na, num_result_cols,
                        result_sides, result_indices, &h));
   h->kd = r->kd;
   h->kcols.assign(probe_key_cols, probe_key_cols + num_keys);

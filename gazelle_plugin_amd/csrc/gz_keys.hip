@@ -32,6 +32,7 @@
 #include <rocprim/rocprim.hpp>
 #include <vector>
 
+#include "gz_device.h"
 #include "gz_internal.h"
 
 namespace gz {
@@ -59,6 +60,7 @@ struct KeyParams {
   int K;
   int width[kMaxKeys];  // bytes per value: 1, 2, 4 or 8
   int sj;               // index of the utf8 key (the last one), -1 if none
+  uint32_t norm;        // float/double keys normalised before hashing (bit j: key j)
   int mode;             // 0 insert every tuple, 1 insert (a null key: code -1), 2 look up (miss or null: -2)
   int64_t rows;         // rows of the call
   unsigned long long* table;
@@ -121,6 +123,7 @@ __device__ __forceinline__ uint32_t load_key(const KeyParams& p, int b, int64_t 
           case 4: raw[j] = ((const uint32_t*)v)[r]; break;
           default: raw[j] = ((const uint64_t*)v)[r]; break;
         }
+        if ((p.norm >> j) & 1) raw[j] = norm_key_bits(raw[j], p.width[j]);
       } else {
         m |= 1u << j;
       }
@@ -345,6 +348,7 @@ struct KeyDict {
   int K = 0;
   std::vector<int> key_types;
   int sj = -1;  // the utf8 key (last), -1 if none
+  uint32_t norm = 0;  // normalised float/double keys
   DBuf table, kvals, kmask, counter;  // counter: [0] codes, [1] heap bytes used
   uint64_t tcap = 0;
   int64_t kcap = 0, ncodes = 0;
@@ -408,6 +412,17 @@ int kd_make(gz_context* ctx, int32_t K, const int32_t* key_types, KeyDict** out)
     return rc;
   }
   *out = d;
+  return GZ_OK;
+}
+
+int kd_set_normalize(KeyDict* d, uint32_t mask) {
+  if (d->ncodes > 0) return fail(GZ_INVALID, "key normalisation is set before the first insert");
+  for (int j = 0; j < 32; j++) {
+    if (!((mask >> j) & 1)) continue;
+    if (j >= d->K || (d->key_types[j] != GZ_TYPE_FLOAT && d->key_types[j] != GZ_TYPE_DOUBLE))
+      return fail(GZ_TYPE_ERROR, "only float and double keys are normalised (key " + std::to_string(j) + ")");
+  }
+  d->norm = mask;
   return GZ_OK;
 }
 
@@ -631,6 +646,7 @@ int kd_encode(KeyDict* d, gz_context* ctx, KeyScratch* sc, int32_t ncols, const 
   p.K = d->K;
   for (int j = 0; j < d->K; j++) p.width[j] = key_width(d->key_types[j]);
   p.sj = d->sj;
+  p.norm = d->norm;
   p.mode = mode;
   p.rows = rows;
   p.table = (unsigned long long*)d->table.p;
@@ -670,9 +686,13 @@ int kd_encode(KeyDict* d, gz_context* ctx, KeyScratch* sc, int32_t ncols, const 
 
 // Each batch's buffer list with an int64 column of codes (no validity) appended.
 void with_code_column(int32_t ncols, const int32_t* types, int32_t nb, const gz_batch* batches, const int64_t* codes,
-                      std::vector<int64_t>& addrs, std::vector<int64_t>& sizes, std::vector<gz_batch>& out) {
-  int need_bufs = 0;
-  for (int c = 0; c < ncols; c++) need_bufs += type_num_bufs(types[c]);
+                      std::vector<int64_t>& addrs, std::vector<int64_t>& sizes, std::vector<gz_batch>& out,
+                      int valid_col) {
+  int need_bufs = 0, vbuf = -1;
+  for (int c = 0; c < ncols; c++) {
+    if (c == valid_col) vbuf = need_bufs;
+    need_bufs += type_num_bufs(types[c]);
+  }
   addrs.assign((size_t)nb * (need_bufs + 2), 0);
   sizes.assign(addrs.size(), 0);
   out.resize(nb);
@@ -684,6 +704,10 @@ void with_code_column(int32_t ncols, const int32_t* types, int32_t nb, const gz_
     for (int k = 0; k < need_bufs; k++) {
       a[k] = bt.num_rows > 0 ? bt.buf_addrs[k] : 0;
       s[k] = bt.num_rows > 0 ? bt.buf_sizes[k] : 0;
+    }
+    if (vbuf >= 0 && bt.num_rows > 0) {
+      a[need_bufs] = bt.buf_addrs[vbuf];
+      s[need_bufs] = bt.buf_sizes[vbuf];
     }
     a[need_bufs + 1] = bt.num_rows > 0 ? (int64_t)(codes + off) : 0;
     s[need_bufs + 1] = bt.num_rows * (int64_t)sizeof(int64_t);

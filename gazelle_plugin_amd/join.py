@@ -32,8 +32,10 @@ _NP = {T.UINT8: np.uint8, T.INT8: np.int8, T.UINT16: np.uint16, T.INT16: np.int1
 class HashRelation:
     """The build side: a resident table in HBM (an opaque handle)."""
 
-    def __init__(self, ctx: Context, types: Sequence[int], key_col, builder_type: int = 1):
-        """key_col: a column index, or a list of them (several key columns)."""
+    def __init__(self, ctx: Context, types: Sequence[int], key_col, builder_type: int = 1, normalize=None):
+        """key_col: a column index, or a list of them (several key columns).
+        normalize: per key, apply Spark's "normalize" to a float/double key
+        (gz_relation_normalize_keys); a bool for one key."""
         self.ctx = ctx
         self.types = list(types)
         self.key_col = key_col
@@ -44,6 +46,10 @@ class HashRelation:
         else:
             L.check(L.lib().gz_relation_make(ctx.handle, len(types), L.i32_array(types), key_col, builder_type,
                                              ctypes.byref(self._h)))
+        if normalize is not None:
+            flags = [int(bool(x)) for x in (normalize if isinstance(normalize, (list, tuple)) else [normalize])]
+            if any(flags):
+                L.check(L.lib().gz_relation_normalize_keys(self._h, len(flags), L.i32_array(flags)))
         self.finished = False
 
     def Evaluate(self, batch: DeviceBatch):
@@ -87,8 +93,8 @@ class HashRelation:
 
 class HashRelationKernel:
     @staticmethod
-    def Make(ctx: Context, types: Sequence[int], key_col: int, builder_type: int = 1) -> HashRelation:
-        return HashRelation(ctx, types, key_col, builder_type)
+    def Make(ctx: Context, types: Sequence[int], key_col, builder_type: int = 1, normalize=None) -> HashRelation:
+        return HashRelation(ctx, types, key_col, builder_type, normalize)
 
 
 class ConditionedProbeKernel:

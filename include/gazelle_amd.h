@@ -457,15 +457,29 @@ int gz_probe_result_column(gz_probe* p, int32_t col, void** values, void** valid
  * identical bytes (fixed-width numeric keys, and a utf8 key only as the last
  * one); the probe's key types must equal the build's.  Build and probe take
  * batches, results and release as the one-key forms; one fixed-width key
- * falls back to gz_relation_make / gz_probe_make.  Null-aware anti joins with
- * several keys, and the fused join -> aggregate over such a relation, are
- * NotImplemented. */
+ * falls back to gz_relation_make / gz_probe_make.  A null-aware anti join with
+ * several keys is the plain anti join, as in the reference, whose UnsafeRow anti
+ * probe never reads the null-aware flag (conditioned_probe_kernel.cc:957-982);
+ * one utf8 key keeps the one-key null-aware rules (a null build key is the
+ * relation's null key, a null probe key is null: :1016-1031).  The fused
+ * join -> aggregate over such a relation is NotImplemented. */
 int gz_relation_make_keys(gz_context* ctx, int32_t num_cols, const int32_t* col_types, int32_t num_keys,
                           const int32_t* key_cols, int32_t builder_type, gz_relation** out);
 int gz_probe_make_keys(gz_relation* r, int32_t num_probe_cols, const int32_t* probe_types, int32_t num_keys,
                        const int32_t* probe_key_cols, int32_t join_type, int32_t null_aware_anti,
                        int32_t num_result_cols, const int32_t* result_sides, const int32_t* result_indices,
                        gz_probe** out);
+/* Spark's join-key "normalize" (NormalizeFloatingNumbers; the reference's
+ * normalize_nan_zero, precompile/gandiva.h:214-222, applied to the key
+ * expressions, tests/arrow_compute_test_join_wocg.cc:1349-1462): for each key
+ * with normalize[j] != 0 (float or double only), every NaN becomes one NaN and
+ * |x| < 1e-7 becomes 0.0 before the key is hashed and compared, on the build
+ * and the probe side.  Output columns keep the original values.  With several
+ * keys NaN then matches NaN (key rows compare by bytes); one key still compares
+ * with == after normalisation, so a NaN key never matches (hash_relation.h:
+ * 368-383).  num_keys is the relation's key count; called before the first
+ * append. */
+int gz_relation_normalize_keys(gz_relation* r, int32_t num_keys, const int32_t* normalize);
 /* The last call's output column validity as an Arrow bitmap (as
  * gz_agg_result_bitmap), owned by the handle, valid until the next call. */
 int gz_probe_result_bitmap(gz_probe* p, int32_t col, void** bitmap);

@@ -234,9 +234,11 @@ class HashRelationKernel : public KernalBase, public std::enable_shared_from_thi
   // HashRelationKernel::Make (kernels_ext.h:535-558): the build side's columns,
   // its key columns (one, or several: the UnsafeRow-keyed relation) and the
   // builder type (1 = unsafeHashMap, 3 = skipping duplicate keys for semi joins).
+  // normalize_keys[j] marks a float/double key wrapped in Spark's "normalize"
+  // (gz_relation_normalize_keys); it applies to the probe keys as well.
   static arrow::Status Make(std::shared_ptr<Context> ctx, const arrow::FieldVector& input_field_list,
                             const std::vector<int>& key_indices, int builder_type,
-                            std::shared_ptr<KernalBase>* out);
+                            std::shared_ptr<KernalBase>* out, const std::vector<bool>& normalize_keys = {});
   ~HashRelationKernel() override;
   // Evaluate (hash_relation_kernel.cc:170-198): one build batch's columns.
   arrow::Status Evaluate(ArrayList& in) override;

@@ -385,12 +385,28 @@ def aggregate(batches, types: Sequence[int], actions: Sequence[Sequence[int]]) -
 
 # ------------------------------------------------------------------ join
 
+def normalize_nan_zero(values: np.ndarray) -> np.ndarray:
+    """Spark's join-key "normalize" on a float/double array, as the reference's
+    normalize_nan_zero (precompile/gandiva.h:214-222): NaN -> NaN, |x| < 1e-7
+    -> 0.0, computed in double and stored back in the array's type."""
+    d = np.asarray(values).astype(np.float64)
+    with np.errstate(invalid="ignore"):
+        out = np.where(np.isnan(d), np.nan, np.where(np.abs(d) < 0.0000001, 0.0, d))
+    return out.astype(np.asarray(values).dtype)
+
+
 def join(build_keys: Sequence, probe_keys, key_type: int, join_type: int, null_aware: bool = False,
-         builder_type: int = 1):
+         builder_type: int = 1, normalize: bool = False):
     """build_keys: list of (values ndarray, validity bitmap or None) per build batch.
     probe_keys: list of (values, validity, n) per probe batch.
     Returns per probe batch (probe_rows, build_rows) for inner, probe_rows otherwise.
-    Build rows are global indices (batch order, then row order)."""
+    Build rows are global indices (batch order, then row order).  normalize: the
+    float/double key values go through normalize_nan_zero on both sides first
+    (the key expressions of the reference's join, which then compares the typed
+    keys as usual: a NaN key still matches nothing)."""
+    if normalize:
+        build_keys = [(normalize_nan_zero(v), valid, n) for v, valid, n in build_keys]
+        probe_keys = [(normalize_nan_zero(v), valid, n) for v, valid, n in probe_keys]
     L = lib()
     r = L.gzo_rel_make(key_type, builder_type)
     try:
@@ -413,33 +429,42 @@ def join(build_keys: Sequence, probe_keys, key_type: int, join_type: int, null_a
         L.gzo_rel_free(r)
 
 
-def _key_tuples(batch, cols) -> list:
+def _key_tuples(batch, cols, normalize=None) -> list:
     """Per row of a HostBatch: None if any of the key columns is null, else the
-    tuple of the keys' raw bits (a utf8 key's bytes)."""
+    tuple of the keys' raw bits (a utf8 key's bytes); normalize[j] puts key j
+    through normalize_nan_zero first."""
     n = batch.num_rows
     parts = []
-    for c in cols:
+    for j, c in enumerate(cols):
         col = batch.columns[c]
         ok = (np.ones(n, bool) if col.validity is None else
               np.unpackbits(np.asarray(col.validity, np.uint8), bitorder="little")[:n].astype(bool))
         if col.type_id in (13, 14):
             vals = [bytes(col.values[col.offsets[r]:col.offsets[r + 1]]) for r in range(n)]
         else:
-            raw = col.values[:n].view(np.dtype(f"u{col.values.dtype.itemsize}"))
+            v = col.values[:n]
+            if normalize is not None and normalize[j]:
+                v = normalize_nan_zero(v)
+            raw = v.view(np.dtype(f"u{col.values.dtype.itemsize}"))
             vals = [int(v) for v in raw]
         parts.append((vals, ok))
     return [None if not all(ok[r] for _, ok in parts) else tuple(v[r] for v, _ in parts) for r in range(n)]
 
 
-def join_multi(build, build_key_cols, probe, probe_key_cols, join_type: int, builder_type: int = 1):
-    """Hash join on several key columns.  The reference keys such relations by
-    UnsafeRow bytes: a build row with a null key is never inserted
-    (hash_relation.h:176-198), and a probe row matches the build rows whose key
-    bytes are equal (conditioned_probe_kernel.cc:609-640), so a probe row with a
-    null key matches nothing.  Restated as the one-key join over first-seen ids
-    of the build side's key tuples: a build row with a null key gets a null id,
-    and so does a probe row with a null key or a tuple the build side lacks.
-    Same return as join()."""
+def join_multi(build, build_key_cols, probe, probe_key_cols, join_type: int, builder_type: int = 1,
+               null_aware: bool = False, normalize=None):
+    """Hash join on several key columns (or one utf8 key).  The reference keys
+    such relations by UnsafeRow bytes: a build row with a null key is never
+    inserted (hash_relation.h:176-198), and a probe row matches the build rows
+    whose key bytes are equal (conditioned_probe_kernel.cc:609-640), so a probe
+    row with a null key matches nothing.  Restated as the one-key join over
+    first-seen ids of the build side's key tuples: a build row with a null key
+    gets a null id, so does a probe row with a null key, and a tuple the build
+    side lacks gets id -1.  The UnsafeRow anti probe never reads the null-aware
+    flag (:957-982), so it applies to one utf8 key only, with the one-key rules
+    (a null build key is the relation's null key, InsertNull, hash_relation.h:
+    244-270).  normalize[j]: key j goes through normalize_nan_zero on both
+    sides.  Same return as join()."""
     def bitmap(ok):
         bits = np.ones((len(ok) + 63) // 64 * 64, np.uint8)
         bits[:len(ok)] = ok
@@ -448,15 +473,15 @@ def join_multi(build, build_key_cols, probe, probe_key_cols, join_type: int, bui
     ids = {}
     bk, pk = [], []
     for b in build:
-        t = _key_tuples(b, build_key_cols)
+        t = _key_tuples(b, build_key_cols, normalize)
         vals = np.array([ids.setdefault(k, len(ids)) if k is not None else 0 for k in t], np.int64)
         bk.append((vals, bitmap(np.array([k is not None for k in t], np.uint8)), b.num_rows))
     for b in probe:
-        t = _key_tuples(b, probe_key_cols)
-        got = [ids.get(k) if k is not None else None for k in t]
+        t = _key_tuples(b, probe_key_cols, normalize)
+        got = [ids.get(k, -1) if k is not None else None for k in t]
         vals = np.array([g if g is not None else 0 for g in got], np.int64)
         pk.append((vals, bitmap(np.array([g is not None for g in got], np.uint8)), b.num_rows))
-    return join(bk, pk, 9, join_type, False, builder_type)
+    return join(bk, pk, 9, join_type, null_aware and len(build_key_cols) == 1, builder_type)
 
 
 # ------------------------------------------------------------------ lz4

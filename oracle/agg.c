@@ -277,6 +277,14 @@ gzo_agg* gzo_agg_make(int ncols, const int32_t* types, int nactions, const int32
     }
   }
   if (a->nkeys > GZO_MAX_KEYS) { gzo_agg_free(a); return NULL; }
+  for (int i = 0; i < nactions; i++) {
+    /* avgByCount's count column: int64 when grouped (the lambda reads int64),
+     * any integer type for the global aggregate (arrow Sum) */
+    if (kinds[i] != A_AVG_BY_COUNT) continue;
+    const int32_t ct = in1[i] >= 0 && in1[i] < ncols ? types[in1[i]] : -1;
+    const int ok = a->key_col >= 0 ? ct == T_INT64 : (ct >= 2 && ct <= 9);
+    if (!ok) { gzo_agg_free(a); return NULL; }
+  }
   return a;
 }
 
@@ -332,7 +340,7 @@ static void update(action_t* x, const gzo_batch* b, int64_t r, int32_t g) {
         x->cnt[g] += 1;
       }
       break;
-    case A_AVG_BY_COUNT: { /* AvgByCountAction (:3785-3795) */
+    case A_AVG_BY_COUNT: { /* AvgByCountAction (:3785-3795): an int64 count column */
       if (!is_null) {
         const void* cv = b->bufs[3 * x->in1 + 2];
         x->dsum[g] += get_f64(v, t, r);
@@ -428,6 +436,21 @@ static void global_min_max(action_t* x, const gzo_batch* b) {
   if (take) x->imm[0] = val;
 }
 
+/* The global aggregate's AvgByCountAction::Evaluate (actions_impl.cc:3818-3838):
+ * arrow Sum of the sum column and of the (integer) count column, each over its
+ * own non-null rows. */
+static void global_avg_by_count(gzo_agg* a, action_t* x, const gzo_batch* b) {
+  const uint8_t* sb = (const uint8_t*)b->bufs[3 * x->in0];
+  const void* sv = b->bufs[3 * x->in0 + 2];
+  const uint8_t* cb = (const uint8_t*)b->bufs[3 * x->in1];
+  const void* cv = b->bufs[3 * x->in1 + 2];
+  const int32_t ct = a->types[x->in1];
+  for (int64_t r = 0; r < b->num_rows; r++) {
+    if (!sb || bit_get(sb, r)) x->dsum[0] += get_f64(sv, x->in_type, r);
+    if (!cb || bit_get(cb, r)) x->cnt[0] += get_i64(cv, ct, r);
+  }
+}
+
 int gzo_agg_process(gzo_agg* a, const gzo_batch* b) {
   int64_t n = b->num_rows;
   if (n == 0) return 0; /* ProcessAndCacheOne returns early on empty input (:808) */
@@ -438,6 +461,7 @@ int gzo_agg_process(gzo_agg* a, const gzo_batch* b) {
     for (int i = 0; i < a->nactions; i++) {
       action_t* x = &a->acts[i];
       if (x->kind == A_MIN || x->kind == A_MAX) global_min_max(x, b);
+      else if (x->kind == A_AVG_BY_COUNT) global_avg_by_count(a, x, b);
       else for (int64_t r = 0; r < n; r++) update(x, b, r, 0);
     }
     return 0;

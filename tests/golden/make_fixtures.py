@@ -209,6 +209,69 @@ AGG_FIXTURES = {"tests": [
                      [16.4, 6.5, 5, 5.875, 5.48, None, 6.1, 6.61905, 3.0625, 2.63889, 2.06667]],
         "tolerance": 1e-5,
     },
+    {   # AggregateTest, :30-221 (global aggregate): the in-scope actions.  Left
+        # out: action_stddev_samp_final, action_min/max over utf8 and
+        # action_first_partial/final, which are not on the built path.
+        "name": "AggregateTest",
+        "schema": [["f0", "int32"], ["f1", "int32"], ["f2", "double"], ["f3", "double"], ["f4", "double"],
+                   ["f5", "utf8"], ["f6", "utf8"], ["f7", "bool"]],
+        "actions": [["action_sum", [0]], ["action_count", [0]], ["action_sum_count", [0]],
+                    ["action_avgByCount", [2, 1]], ["action_min", [0]], ["action_max", [0]],
+                    ["action_countLiteral_1", []]],
+        "batches": [
+            [[1, 2, 3, 4, 5, None, 4, 1, 2, 2, 1, 1, 1, 4, 4, 3, 5, 5, 5, 5],           # :160-178
+             [1, 2, 3, 4, 5, None, 4, 1, 2, 2, 1, 1, 1, 4, 4, 3, 5, 5, 5, 5],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             ["BJK", "SH", "SZ", "NY", "WH", "WH", "AU", "BJ", "SH", "DL", "cD", "CD", "Bj", "LA", "HZ", "LA",
+              "WH", "NY", "WH", "WH"],
+             ["ZZ", "SH", "SZ", "NY", "WH", "WH", "AU", "BJ", "SH", "DL", "cD", "CD", "Bj", "LA", "HZ", "LA",
+              "WH", "NY", "WH", "WH"],
+             [True, False, True, False, False, True, False, True, False, False, True, False, True, False,
+              False, True, False, True, False, False]],
+            [[6, 7, 8, 9, 10, 10, 9, 6, 7, 7, 6, 6, 6, 9, 9, 8, 10, 10, 10, 10],          # :180-199
+             [6, 7, 8, 9, 10, 10, 9, 6, 7, 7, 6, 6, 6, 9, 9, 8, 10, 10, 10, 10],
+             [7, 8, 4, 5, 6, 1, 34, 54, 65, 66, 78, 12, 32, 24, 32, 45, 12, 24, 35, 46],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             [2, 4, 5, 7, 8, 2, 45, 32, 23, 12, 14, 16, 18, 19, 23, 25, 57, 59, 12, 1],
+             ["BJ", "SHL", "SZ", "NY", "WH", "WH", "au", "BJ", "SH", "DL", "cD", "CD", "Bj", "LA", "HZ", "LA",
+              "WH", "NY", "WHZ", "wH"],
+             ["BJ", "SHL", "SZ", "NY", "WH", "WH", "au", "BJ", "SH", "DL", "cD", "CD", "Bj", "LA", "HZ", "LA",
+              "WH", "NY", "WHZ", "wH"],
+             [True, False, True, False, False, True, False, True, False, False, True, False, True, False,
+              False, True, False, True, False, False]],
+        ],
+        # :203-207: sum, count, sum_count (sum, count), avgByCount, min, max, count_all
+        "expected": [[221], [39], [221], [39], [4.40724], [1], [10], [40]],
+        "tolerance": 1e-5,
+    },
+    {   # GroupByCountAll, :755-832: GROUP BY a utf8 key, countLiteral_1
+        "name": "GroupByCountAll",
+        "schema": [["f0", "utf8"]],
+        "actions": [["action_groupby", [0]], ["action_countLiteral_1", []]],
+        "batches": [
+            [["BJ", "SH", "SZ", "HZ", "WH", "WH", "HZ", "BJ", "SH", "SH", "BJ", "BJ", "BJ", "HZ", "HZ", "SZ",
+              "WH", "WH", "WH", "WH"]],
+            [["CD", "DL", "NY", "LA", "AU", "AU", "LA", "CD", "DL", "DL", "CD", "CD", "CD", "LA", "LA", "NY",
+              "AU", "AU", "AU", "AU"]],
+            [["BJ", "SH", "SZ", "NY", "WH", "WH", "AU", "BJ", "SH", "DL", "CD", "CD", "BJ", "LA", "HZ", "LA",
+              "WH", "NY", "WH", "WH"]],
+        ],
+        "expected": [["BJ", "SH", "SZ", "HZ", "WH", "CD", "DL", "NY", "LA", "AU"],         # :823-825
+                     [8, 5, 3, 5, 11, 7, 4, 4, 6, 7]],
+    },
+    {   # GroupByCountOnMutipleCols, :833-910: GROUP BY a utf8 key, count over
+        # two utf8 columns (rows where both are non-null)
+        "name": "GroupByCountOnMutipleCols",
+        "schema": [["f0", "utf8"], ["f1", "utf8"], ["f2", "utf8"]],
+        "actions": [["action_groupby", [0]], ["action_count", [1, 2]]],
+        "batches": [
+            [["a", "a", "a", "x", "x"], ["b", "b", "b", "y", "q"], [None, "c", "d", "z", None]],
+            [["b", "a", "b", "a", "x"], ["b", "b", "b", None, "q"], ["c", None, "d", "z", None]],
+        ],
+        "expected": [["a", "x", "b"], [2, 1, 2]],                                          # :901
+    },
 ]}
 
 
@@ -256,6 +319,57 @@ JOIN_FIXTURES = {"tests": [
         "build": _BUILD, "probe": _PROBE_STR,
         "expected": [[[1, 3, 5, 6], ["BJ", "TY", "SH", "HZ"]], [[8, 10, 12], ["NY", "IT", "TL"]]],
     },
+    {   # JoinWOCGTestSemiJoinType2, :1934-2054 (uint32 key)
+        "name": "SemiJoinType2", "join_type": "semi", "null_aware": False,
+        "build_schema": [["table0_f0", "uint32"], ["table0_f1", "uint32"], ["table0_f2", "uint32"]],
+        "probe_schema": [["table1_f0", "uint32"], ["table1_f1", "utf8"]],
+        "build_key": 0, "probe_key": 0, "result": [[1, 0], [1, 1]],
+        "build": _BUILD, "probe": _PROBE_STR,
+        "expected": [[[1, 3, 5, 6], ["BJ", "TY", "SH", "HZ"]], [[8, 10, 12], ["NY", "IT", "TL"]]],
+    },
+    {   # JoinWOCGTestAntiJoinMulKeys, :1820-1932: conditionedProbeArraysAnti_true
+        # (null-aware) over two int32 keys (UnsafeRow keys, hash config 1)
+        "name": "AntiJoinMulKeys", "join_type": "anti", "null_aware": True,
+        "build_schema": [["table0_f0", "int32"], ["table0_f1", "int32"], ["table0_f2", "int32"]],
+        "probe_schema": [["table1_f0", "int32"], ["table1_f1", "int32"], ["table1_f2", "int32"]],
+        "build_key": [0, 1], "probe_key": [0, 1], "result": [[1, 0], [1, 1]],
+        "build": [[[1, None, None, 1, 1], [1, None, 3, None, 2], [2, 2, 2, 2, None]]],
+        "probe": [[[1, 2, None, 1, None, None, 1, None], [1, 1, 1, 3, None, 3, None, 2], [2, 3, 6, 1, 6, 2, 9, 2]]],
+        "expected": [[[2, None, 1, None, None, 1, None], [1, 1, 3, None, 3, None, 2]]],
+    },
+    {   # JoinWOCGTestInnerJoinNaN, :1349-1462: two double keys, each wrapped in
+        # "normalize" (normalize_nan_zero), UnsafeRow keys; result: both build
+        # columns, then both probe columns
+        "name": "InnerJoinNaN", "join_type": "inner", "null_aware": False,
+        "build_schema": [["table0_f0", "double"], ["table0_f1", "double"]],
+        "probe_schema": [["table1_f0", "double"], ["table1_f1", "double"]],
+        "build_key": [0, 1], "probe_key": [0, 1], "normalize": [True, True],
+        "result": [[0, 0], [0, 1], [1, 0], [1, 1]],
+        "build": [[[NAN, 0.0, -0.0], [NAN, 0.0, -0.0]]],
+        "probe": [[[NAN, 0.0, -0.0], [NAN, 0.0, -0.0]]],
+        "expected": [[[NAN, 0.0, -0.0, 0.0, -0.0], [NAN, 0.0, -0.0, 0.0, -0.0],
+                      [NAN, 0.0, 0.0, -0.0, -0.0], [NAN, 0.0, 0.0, -0.0, -0.0]]],
+    },
+    {   # JoinWOCGTestStringInnerJoin, :95-225: a utf8 key; the probe key is the
+        # projection upper(table1_f0), given here as the probe's third column
+        # (the value the reference's key projection computes before probing)
+        "name": "StringInnerJoin", "join_type": "inner", "null_aware": False,
+        "build_schema": [["table0_f0", "utf8"], ["table0_f1", "utf8"], ["table0_f2", "uint32"]],
+        "probe_schema": [["table1_f0", "utf8"], ["table1_f1", "uint32"], ["upper_table1_f0", "utf8"]],
+        "build_key": 0, "probe_key": 2, "result": [[0, 0], [0, 1], [0, 2], [1, 0], [1, 1]],
+        "build": [[["BJ", "SH", "HZ", "BH", "NY", "SH"], ["A", "A", "C", "D", "C", "D"], [10, 3, 1, 2, 13, 11]],
+                  [["TK", "SH", "PH", "NJ", "NB", "SZ"], ["F", "F", "A", "B", "D", "C"], [6, 12, 5, 8, 16, 110]]],
+        "probe": [[["sh", "sz", "bj", None, "ny", "hz"], [1, 2, 3, 4, 5, 6],
+                   ["SH", "SZ", "BJ", None, "NY", "HZ"]],
+                  [["ph", None, "jh", "kk", "nj", "sz"], [7, 8, 9, 10, None, 12],
+                   ["PH", None, "JH", "KK", "NJ", "SZ"]]],
+        "expected": [[["SH", "SH", "SH", "SZ", "BJ", "NY", "HZ"], ["A", "D", "F", "C", "A", "C", "C"],
+                      [3, 11, 12, 110, 10, 13, 1], ["sh", "sh", "sh", "sz", "bj", "ny", "hz"],
+                      [1, 1, 1, 2, 3, 5, 6]],
+                     [["PH", "NJ", "SZ"], ["A", "B", "C"], [5, 8, 110], ["ph", "nj", "sz"], [7, None, 12]]],
+    },
+    # JoinWOCGTestProjectKeyInnerJoin (:41-94) only builds the probe's code
+    # generator: it holds no input rows and no expected result to transcribe.
     {   # JoinWOCGTestInnerJoinType2WithUInt16 (repeated build keys), :2405-2532
         "name": "InnerJoinType2WithUInt16", "join_type": "inner", "null_aware": False,
         "build_schema": [["table0_f0", "uint16"], ["table0_f1", "uint32"], ["table0_f2", "uint32"]],

@@ -51,7 +51,7 @@ def test_cardinalities(ctx, num_keys):
     assert_results_equal(got, expected(batches, datagen.AGG_TYPES, CONFIG0_ACTIONS), RTOL)
 
 
-@pytest.mark.parametrize("case", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("case", range(len(F.load("agg_fixtures.json")["tests"])))
 def test_reference_fixtures_on_gpu(ctx, case):
     fx = F.load("agg_fixtures.json")["tests"][case]
     schema = fx["schema"]

@@ -11,7 +11,7 @@ from gazelle_plugin_amd import types as T
 from gazelle_plugin_amd.batch import DeviceBatch, HostBatch, HostColumn, bitmap_from_bools, bools_from_bitmap
 from gazelle_plugin_amd.join import ConditionedProbeKernel, HashRelationKernel
 from golden import fixtures as F
-from test_oracle_join import JT, key_of, take
+from test_oracle_join import JOIN_FIXTURES, JT, canon, fixture_keys, key_of, oracle_fixture_outputs, take
 
 pytestmark = pytest.mark.gpu
 
@@ -53,8 +53,8 @@ def check_output(got, exp_cols):
 
 
 def run_case(ctx, build, probe, btypes, ptypes, bkey, pkey, join_type, null_aware, result, batched,
-             builder_type=1):
-    rel = HashRelationKernel.Make(ctx, btypes, bkey, builder_type)
+             builder_type=1, normalize=None):
+    rel = HashRelationKernel.Make(ctx, btypes, bkey, builder_type, normalize)
     if batched:
         rel.EvaluateBatches([DeviceBatch.from_host(b) for b in build])
     else:
@@ -111,22 +111,57 @@ def expected_cols(build, probe, bkey, pkey, ktype, join_type, null_aware, result
     return res
 
 
-@pytest.mark.parametrize("case", range(5))
+@pytest.mark.parametrize("case", range(len(JOIN_FIXTURES)))
 @pytest.mark.parametrize("batched", [True, False])
 def test_reference_join_fixtures_on_gpu(ctx, case, batched):
-    fx = F.load("join_fixtures.json")["tests"][case]
+    """Every join golden, bit for bit (floats by their bits, NaN as NaN), and
+    equal to the oracle's result on the same input."""
+    fx = JOIN_FIXTURES[case]
     build = [F.batch(fx["build_schema"], b) for b in fx["build"]]
     probe = [F.batch(fx["probe_schema"], b) for b in fx["probe"]]
     btypes = [F.type_id(t) for _, t in fx["build_schema"]]
     ptypes = [F.type_id(t) for _, t in fx["probe_schema"]]
-    outs = run_case(ctx, build, probe, btypes, ptypes, fx["build_key"], fx["probe_key"], fx["join_type"],
-                    fx["null_aware"], fx["result"], batched)
-    for got, exp in zip(outs, fx["expected"]):
+    bk, pk = fixture_keys(fx)
+    outs = run_case(ctx, build, probe, btypes, ptypes, bk, pk, fx["join_type"], fx["null_aware"], fx["result"],
+                    batched, normalize=fx.get("normalize"))
+    assert len(outs) == len(fx["expected"])
+    for got, exp, ora in zip(outs, fx["expected"], oracle_fixture_outputs(fx, build, probe)):
         vals = []
         for t, v, ok in got:
-            vals.append([x.decode() if isinstance(x, bytes) else (bool(x) if t == T.BOOL else x.item())
-                         for x in v])
-        assert vals == exp, (fx["name"], vals, exp)
+            vals.append([None if not o else (x.decode() if isinstance(x, bytes) else
+                                             (bool(x) if t == T.BOOL else x.item()))
+                         for x, o in zip(v, ok)])
+        assert canon(vals) == canon(exp), (fx["name"], vals, exp)
+        assert canon(vals) == canon(ora), (fx["name"], vals, ora)
+
+
+@pytest.mark.parametrize("keys", ["one", "two"])
+def test_normalized_float_keys_match_the_oracle(ctx, keys):
+    """Spark's normalize on double keys: NaN, -0.0, tiny values and ordinary
+    values, duplicates on the build side; one key (NaN never matches) and two
+    keys (NaN matches NaN by bytes)."""
+    rng = np.random.default_rng(17)
+    special = np.array([np.nan, 0.0, -0.0, 1e-8, -5e-8, 1.0, -1.0, 2.5])
+    def batch(n):
+        k0 = special[rng.integers(0, len(special), n)]
+        k1 = special[rng.integers(0, 3, n)]
+        v = rng.random(n) > 0.1
+        return HostBatch(n, [HostColumn(T.DOUBLE, k0.copy(), bitmap_from_bools(v)), HostColumn(T.DOUBLE, k1.copy()),
+                             HostColumn(T.INT32, np.arange(n, dtype=np.int32))])
+    build, probe = [batch(300), batch(77)], [batch(500), batch(1000)]
+    types = [T.DOUBLE, T.DOUBLE, T.INT32]
+    bk = 0 if keys == "one" else [0, 1]
+    norm = [True] if keys == "one" else [True, True]
+    for join_type in ("inner", "semi", "anti"):
+        fx = {"build_key": bk, "probe_key": bk, "join_type": join_type, "null_aware": False, "normalize": norm,
+              "build_schema": [["k0", "double"], ["k1", "double"], ["r", "int32"]],
+              "result": [[1, 2], [0, 2]] if join_type == "inner" else [[1, 2]]}
+        got = run_case(ctx, build, probe, types, types, bk, bk, join_type, False, fx["result"], True,
+                       normalize=norm)
+        exp = oracle_fixture_outputs(fx, build, probe)
+        for g, e in zip(got, exp):
+            vals = [[x.item() if o else None for x, o in zip(v, ok)] for _, v, ok in g]
+            assert vals == e, (keys, join_type)
 
 
 def _workload(rng, nbuild, nprobe, dup_keys, key_type=T.INT64, nulls=0.02):

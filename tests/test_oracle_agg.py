@@ -32,7 +32,7 @@ def _column(res, i):
     return [v[j].item() if ok[j] else None for j in range(len(v))]
 
 
-@pytest.mark.parametrize("case", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("case", range(len(F.load("agg_fixtures.json")["tests"])))
 def test_reference_aggregate_fixtures(case):
     fx = F.load("agg_fixtures.json")["tests"][case]
     schema = fx["schema"]

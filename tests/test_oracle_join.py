@@ -1,5 +1,8 @@
 """Pins of the oracle's hash join: the reference's own join tests
 (arrow_compute_test_join_wocg.cc), a brute-force nested loop, and pyarrow's join."""
+import math
+import struct
+
 import numpy as np
 import pytest
 
@@ -32,19 +35,79 @@ def take(batch, c, rows):
     return out
 
 
-@pytest.mark.parametrize("case", range(5))
+JOIN_FIXTURES = F.load("join_fixtures.json")["tests"]
+
+
+def fixture_keys(fx):
+    """(build keys, probe keys) as lists when the fixture keys by several
+    columns or one utf8 column (the key-dictionary path), else single ints."""
+    bk, pk = fx["build_key"], fx["probe_key"]
+    if isinstance(bk, list):
+        return bk, pk
+    if F.type_id(fx["build_schema"][bk][1]) == T.STRING:
+        return [bk], [pk]
+    return bk, pk
+
+
+def canon(cols):
+    """Column values with floats by their bits (NaN as "NaN"), for exact comparison."""
+    def one(v):
+        if isinstance(v, float):
+            return "NaN" if math.isnan(v) else struct.pack("<d", v).hex()
+        return v
+    return [[one(v) for v in col] for col in cols]
+
+
+def oracle_fixture_outputs(fx, build, probe):
+    """The oracle's result columns per probe batch for a join fixture."""
+    bk, pk = fixture_keys(fx)
+    jt = JT[fx["join_type"]]
+    norm = fx.get("normalize")
+    if isinstance(bk, list):
+        outs = oracle.join_multi(build, bk, probe, pk, jt, 1, fx["null_aware"], norm)
+    else:
+        kt = F.type_id(fx["build_schema"][bk][1])
+        outs = oracle.join([key_of(b, bk) for b in build], [key_of(b, pk) for b in probe], kt, jt,
+                           fx["null_aware"], normalize=bool(norm and norm[0]))
+    bflat = [(bi, r) for bi, b in enumerate(build) for r in range(b.num_rows)]
+    res = []
+    for pb, out in zip(probe, outs):
+        prow, brow = out if jt == 0 else (out, None)
+        cols = []
+        for s, i in fx["result"]:
+            if s == 1:
+                cols.append(take(pb, i, list(prow)))
+            else:
+                cols.append([take(build[bflat[g][0]], i, [bflat[g][1]])[0] for g in brow])
+        res.append(cols)
+    return res
+
+
+@pytest.mark.parametrize("case", range(len(JOIN_FIXTURES)))
 def test_reference_join_fixtures(case):
-    fx = F.load("join_fixtures.json")["tests"][case]
+    fx = JOIN_FIXTURES[case]
     build = [F.batch(fx["build_schema"], b) for b in fx["build"]]
     probe = [F.batch(fx["probe_schema"], b) for b in fx["probe"]]
-    kt = F.type_id(fx["build_schema"][fx["build_key"]][1])
-    outs = oracle.join([key_of(b, fx["build_key"]) for b in build],
-                       [key_of(b, fx["probe_key"]) for b in probe], kt, JT[fx["join_type"]],
-                       fx["null_aware"])
-    for pb, out, exp in zip(probe, outs, fx["expected"]):
-        rows = out[0] if fx["join_type"] == "inner" else out
-        got = [take(pb, i, list(rows)) for s, i in fx["result"]]
-        assert got == exp, (fx["name"], got, exp)
+    outs = oracle_fixture_outputs(fx, build, probe)
+    assert len(outs) == len(fx["expected"])
+    for got, exp in zip(outs, fx["expected"]):
+        assert canon(got) == canon(exp), (fx["name"], got, exp)
+
+
+def test_normalized_single_double_key():
+    """One normalised double key: normalize_nan_zero maps -0.0 and |x| < 1e-7 to
+    0.0 and the typed == still never matches NaN (hash_relation.h:368-383), so
+    the inner pairs follow by hand; without normalisation -0.0, 1e-8 and 0.0 are
+    three keys (their hashes differ) and only equal bits match."""
+    nan = float("nan")
+    bk = np.array([nan, 0.0, -0.0, 1e-8, 2.0])
+    pk = np.array([nan, -0.0, 5e-8, 2.0, 3.0])
+    (prow, brow), = oracle.join([(bk, None, 5)], [(pk, None, 5)], T.DOUBLE, 0, normalize=True)
+    assert list(zip(prow.tolist(), brow.tolist())) == [(1, 1), (1, 2), (1, 3), (2, 1), (2, 2), (2, 3), (3, 4)]
+    (prow, brow), = oracle.join([(bk, None, 5)], [(pk, None, 5)], T.DOUBLE, 0)
+    assert list(zip(prow.tolist(), brow.tolist())) == [(1, 2), (3, 4)]
+    anti, = oracle.join([(bk, None, 5)], [(pk, None, 5)], T.DOUBLE, 2, normalize=True)
+    assert anti.tolist() == [0, 4]
 
 
 def brute(build_keys, probe_keys, join_type, null_aware=False):
