@@ -237,7 +237,8 @@ class HashAggregateResultIterator:
 
     def close(self):
         if self._h:
-            L.lib().gz_agg_release(self._h)
+            if self.k.ctx.alive:
+                L.lib().gz_agg_release(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

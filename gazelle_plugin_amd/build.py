@@ -9,7 +9,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgazelle_amd.so")
 
-SOURCES = ["gz_context.cpp", "gz_split.hip", "gz_ipc_read.hip", "gz_agg.hip", "gz_keys.hip", "gz_join.hip", "gz_serve.hip"]
+SOURCES = ["gz_context.cpp", "gz_split.hip", "gz_ipc_read.hip", "gz_agg.hip", "gz_keys.hip", "gz_join.hip", "gz_serve.hip",
+           "gz_stage.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled: hipcc otherwise forms v_pk_* from

@@ -27,6 +27,12 @@ class Context:
     def handle(self):
         return self._h
 
+    @property
+    def alive(self) -> bool:
+        """False once destroyed: the handles made on it were freed with it
+        and must not be released again."""
+        return bool(self._h)
+
     def set_stream(self, stream: Optional[int]):
         L.check(L.lib().gz_context_set_stream(self._h, ctypes.c_void_p(stream or 0)))
 

@@ -1735,6 +1735,8 @@ struct gz_agg {
   MultiKey* mk = nullptr;        // grouped by several key columns
   JoinAggState* join = nullptr;  // a fused join -> aggregate
   gz_context* ctx = nullptr;
+  std::vector<int32_t> in_types;  // input column types
+  BatchStage stage;               // per-batch calls staged for the batched path (gz_agg_process)
   AggPlan plan;
   gz_agg_options opts;
   int ninst = 1;
@@ -2057,6 +2059,7 @@ static int agg_set_make(gz_context* ctx, int32_t ncols, const int32_t* types, in
     if (nact > kMaxActions) return fail(GZ_INVALID, "bad action count");
     auto* h = new gz_agg();
     h->ctx = ctx;
+    h->in_types.assign(types, types + ncols);
     int rc = mk_make(ctx, ncols, types, nact, acts, opts, &h->mk);
     if (rc) {
       delete h;
@@ -2067,6 +2070,7 @@ static int agg_set_make(gz_context* ctx, int32_t ncols, const int32_t* types, in
   }
   auto* h = new gz_agg();
   h->ctx = ctx;
+  h->in_types.assign(types, types + ncols);
   int rc = build_plan(h->plan, ncols, types, nact, acts);
   if (rc) {
     delete h;
@@ -2994,12 +2998,36 @@ int gz_join_agg_process_batches(gz_agg* h, int32_t nb, const gz_batch* batches) 
   return GZ_OK;
 }
 
+// Runs the staged per-batch calls through the batched path.
+static int agg_flush(gz_agg* h) {
+  if (h->stage.empty()) return GZ_OK;
+  std::vector<gz_batch> v;
+  stage_view(h->stage, &v);
+  const int rc = h->mk ? mk_process(h->mk, (int32_t)v.size(), v.data()) : agg_process(h, (int32_t)v.size(), v.data(), nullptr);
+  stage_clear(&h->stage);  // the arena is reused by later copies, queued after this run's kernels
+  return rc;
+}
+
 int gz_agg_process(gz_agg* h, int64_t num_rows, const int64_t* buf_addrs, const int64_t* buf_sizes,
                    int32_t num_bufs) {
   if (!h) return fail(GZ_INVALID, "null aggregate");
   if (num_rows == 0) return GZ_OK;  // ProcessAndCacheOne returns early on empty input
   if (!buf_addrs || !buf_sizes) return fail(GZ_INVALID, "buf_addrs can't be null");
+  if (h->join) return fail(GZ_INVALID, "a join aggregate takes probe batches (gz_join_agg_process_batches)");
   gz_batch b{num_rows, buf_addrs, buf_sizes, num_bufs};
+  if (h->ninst == 1 && !h->finished && !(h->mk && mk_finished(h->mk))) {
+    // staged for the batched path: results are observed only at finish
+    GZ_HIP(hipSetDevice(h->ctx->device));
+    GZ_TRY(stage_check((int32_t)h->in_types.size(), h->in_types.data(), b));
+    bool staged = false;
+    GZ_TRY(stage_add(h->ctx, &h->stage, b, &staged));
+    if (!staged && !h->stage.empty()) {  // the arena is full: run what it holds, then stage again
+      GZ_TRY(agg_flush(h));
+      GZ_TRY(stage_add(h->ctx, &h->stage, b, &staged));
+    }
+    if (staged) return GZ_OK;
+  }
+  GZ_TRY(agg_flush(h));
   if (h->mk) return mk_process(h->mk, 1, &b);
   return agg_process(h, 1, &b, nullptr);
 }
@@ -3007,12 +3035,14 @@ int gz_agg_process(gz_agg* h, int64_t num_rows, const int64_t* buf_addrs, const 
 int gz_agg_process_batches(gz_agg* h, int32_t num_batches, const gz_batch* batches) {
   if (!h) return fail(GZ_INVALID, "null aggregate");
   if (num_batches < 0 || (num_batches > 0 && !batches)) return fail(GZ_INVALID, "bad batch list");
+  GZ_TRY(agg_flush(h));
   if (h->mk) return mk_process(h->mk, num_batches, batches);
   return agg_process(h, num_batches, batches, nullptr);
 }
 
 int gz_agg_num_groups(gz_agg* h, int64_t* out) {
   if (!h || !out) return fail(GZ_INVALID, "null argument");
+  GZ_TRY(agg_flush(h));
   if (h->mk) return mk_num_groups(h->mk, out);
   if (h->finished) {
     *out = h->ngroups[0];
@@ -3044,6 +3074,7 @@ int gz_agg_result_type(gz_agg* h, int32_t col) {
 
 int gz_agg_finish(gz_agg* h) {
   if (!h) return fail(GZ_INVALID, "null aggregate");
+  GZ_TRY(agg_flush(h));
   if (h->mk) return mk_finish(h->mk);
   return agg_finish(h);
 }
@@ -3067,6 +3098,7 @@ int gz_agg_result_binary(gz_agg* h, int32_t col, void** offsets, void** data, vo
 
 int gz_agg_release(gz_agg* h) {
   if (!h) return GZ_OK;
+  stage_release(h->ctx, &h->stage);
   if (h->mk) {
     for (auto& b : h->res_bits) ctx_free(h->ctx, &b);
     mk_release(h->mk);
@@ -3103,6 +3135,7 @@ int gz_agg_set_process(gz_agg* h, int32_t num_batches, const gz_batch* batches, 
   if (!h) return fail(GZ_INVALID, "null aggregate set");
   if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   if (num_batches < 0 || (num_batches > 0 && (!batches || !instance_of))) return fail(GZ_INVALID, "bad batch list");
+  GZ_TRY(agg_flush(h));
   return agg_process(h, num_batches, batches, instance_of);
 }
 
@@ -3169,6 +3202,7 @@ int gz_agg_set_prepare(gz_agg* h, int32_t num_batches, const gz_batch* batches, 
   if (!h) return fail(GZ_INVALID, "null aggregate set");
   if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   if (num_batches < 0 || (num_batches > 0 && (!batches || !instance_of))) return fail(GZ_INVALID, "bad batch list");
+  GZ_TRY(agg_flush(h));
   return agg_prepare(h, num_batches, batches, instance_of);
 }
 
@@ -3195,6 +3229,7 @@ int gz_agg_set_autotune(gz_agg* h, int32_t num_batches, const gz_batch* batches,
   if (!h) return fail(GZ_INVALID, "null aggregate set");
   if (h->mk) return fail(GZ_NOT_IMPLEMENTED, "planned and set forms group by one key column");
   if (num_batches < 0 || (num_batches > 0 && (!batches || !instance_of))) return fail(GZ_INVALID, "bad batch list");
+  GZ_TRY(agg_flush(h));
   return agg_autotune(h, num_batches, batches, instance_of, chosen);
 }
 

@@ -160,6 +160,28 @@ struct RelationView {
   const void* values[32];    // fixed width: width bytes per row; bool: one byte per row
 };
 int relation_view(gz_relation* r, RelationView* v);
+// Per-batch staging (gz_stage.hip): a per-batch call copies its borrowed buffers
+// into the handle's device arena (one gather-copy kernel on the context's
+// stream) and the handle runs its batched path over the staged batches later.
+struct BatchStage {
+  DBuf arena;
+  int64_t cap = 0, used = 0;
+  std::vector<int64_t> addrs, sizes;  // staged buffer lists, batch after batch
+  std::vector<size_t> first;          // index of each batch's first buffer
+  std::vector<int64_t> rows;
+  std::vector<int32_t> nbufs;
+  bool empty() const { return rows.empty(); }
+};
+// The buffer-size checks of the batched paths for one batch of a schema.
+int stage_check(int32_t ncols, const int32_t* types, const gz_batch& b);
+// Stages b when it fits (*staged); a batch above a quarter of the arena, or no
+// arena under the HBM budget, is left to the caller to run directly.
+int stage_add(gz_context* ctx, BatchStage* s, const gz_batch& b, bool* staged);
+// gz_batch views of the staged batches (valid until the next stage_add / clear).
+void stage_view(const BatchStage& s, std::vector<gz_batch>* out);
+void stage_clear(BatchStage* s);
+void stage_release(gz_context* ctx, BatchStage* s);
+
 // Validity bytes (1 = valid) to an Arrow bitmap of (n + 63) / 64 * 8 bytes (1 bits
 // past row n), on the context's stream (gz_serve.hip).
 int pack_validity(gz_context* ctx, const uint8_t* valid, int64_t n, uint8_t* bits);
@@ -200,6 +222,7 @@ int mk_num_groups(MultiKey* m, int64_t* out);
 int mk_num_result_columns(MultiKey* m);
 int mk_result_type(MultiKey* m, int col);
 int mk_finish(MultiKey* m);
+bool mk_finished(const MultiKey* m);
 int mk_result_column(MultiKey* m, int col, void** values, void** valid);
 int mk_result_binary(MultiKey* m, int col, void** offsets, void** data, void** valid);
 void mk_release(MultiKey* m);

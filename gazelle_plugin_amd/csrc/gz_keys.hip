@@ -739,6 +739,8 @@ int mk_result_type(MultiKey* m, int col) {
   return r < 0 ? m->kd->key_types[-r - 1] : gz_agg_result_type(m->inner, r);
 }
 
+bool mk_finished(const MultiKey* m) { return m->finished; }
+
 int mk_finish(MultiKey* m) {
   KeyDict* d = m->kd;
   if (m->finished) return GZ_OK;

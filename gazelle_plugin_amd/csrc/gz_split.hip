@@ -812,6 +812,8 @@ struct gz_splitter {
   // wall time of the data file's layout and assembly, and of its compression
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pid_events, pid_free;
   int64_t write_ns = 0, compress_ns = 0;
+  // per-batch calls staged for the batched path (gz_split_batch)
+  BatchStage stage;
 };
 
 // Elapsed time of the recorded partition id passes, added to compute_pid_ns;
@@ -1255,6 +1257,16 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   return GZ_OK;
 }
 
+// Runs the staged per-batch calls as one segment.
+static int split_flush(gz_splitter* h) {
+  if (h->stage.empty()) return GZ_OK;
+  std::vector<gz_batch> v;
+  stage_view(h->stage, &v);
+  const int rc = split_run(h, (int32_t)v.size(), v.data());
+  stage_clear(&h->stage);  // the arena is reused by later copies, queued after this segment's kernels
+  return rc;
+}
+
 static int resolve_segment(gz_splitter* h, SplitSegment* s) {
   if (s->resolved) return GZ_OK;
   gz_context* ctx = h->ctx;
@@ -1287,7 +1299,23 @@ int gz_split_batch(gz_splitter* h, int64_t num_rows, const int64_t* buf_addrs,
                    const int64_t* buf_sizes, int32_t num_bufs) {
   if (!h) return fail(GZ_INVALID, "null splitter");
   if (!buf_addrs || !buf_sizes) return fail(GZ_INVALID, "Native split: buf_addrs can't be null");
+  if (h->stopped) return fail(GZ_INVALID, "splitter already stopped");
   gz_batch b{num_rows, buf_addrs, buf_sizes, num_bufs};
+  // The batch is staged for the batched path, except where a call's own
+  // result is observable: the range splitter reports a bad partition id from
+  // the call that holds it, and row ids count the rows of one call.
+  if (h->mode != 2 && !h->opts.append_row_ids) {
+    GZ_HIP(hipSetDevice(h->ctx->device));
+    GZ_TRY(stage_check(h->ncols_in, h->types_in.data(), b));
+    bool staged = false;
+    GZ_TRY(stage_add(h->ctx, &h->stage, b, &staged));
+    if (!staged && !h->stage.empty()) {  // the arena is full: run what it holds, then stage again
+      GZ_TRY(split_flush(h));
+      GZ_TRY(stage_add(h->ctx, &h->stage, b, &staged));
+    }
+    if (staged) return GZ_OK;
+  }
+  GZ_TRY(split_flush(h));
   return split_run(h, 1, &b);
 }
 
@@ -1297,11 +1325,14 @@ int gz_split_batches(gz_splitter* h, int32_t num_batches, const gz_batch* batche
   for (int i = 0; i < num_batches; i++)
     if (!batches[i].buf_addrs || !batches[i].buf_sizes)
       return fail(GZ_INVALID, "Native split: buf_addrs can't be null");
+  if (h->stopped) return fail(GZ_INVALID, "splitter already stopped");
+  GZ_TRY(split_flush(h));
   return split_run(h, num_batches, batches);
 }
 
 int gz_split_stop(gz_splitter* h) {
   if (!h) return fail(GZ_INVALID, "null splitter");
+  if (!h->stopped) GZ_TRY(split_flush(h));
   for (auto* s : h->segments) GZ_TRY(resolve_segment(h, s));
   drain_pid_events(h);
   h->stopped = true;
@@ -1310,10 +1341,15 @@ int gz_split_stop(gz_splitter* h) {
 
 int gz_split_num_partitions(gz_splitter* h) { return h ? h->P : 0; }
 int gz_split_num_output_columns(gz_splitter* h) { return h ? (int)h->out_types.size() : 0; }
-int gz_split_num_segments(gz_splitter* h) { return h ? (int)h->segments.size() : 0; }
+int gz_split_num_segments(gz_splitter* h) {
+  if (!h) return 0;
+  if (!h->stopped && split_flush(h) != GZ_OK) return -1;
+  return (int)h->segments.size();
+}
 
 int gz_split_partition_rows(gz_splitter* h, int64_t* out) {
   if (!h || !out) return fail(GZ_INVALID, "null argument");
+  if (!h->stopped) GZ_TRY(split_flush(h));
   for (int p = 0; p < h->P; p++) out[p] = 0;
   for (auto* s : h->segments) {
     GZ_TRY(resolve_segment(h, s));
@@ -1390,6 +1426,7 @@ int gz_split_release(gz_splitter* h) {
       hipEventDestroy(ev.second);
     }
   for (auto* s : h->segments) split_release_segment(h->ctx, s);
+  stage_release(h->ctx, &h->stage);
   if (h->ipc) ipc_plan_free(h->ctx, h->ipc);
   ctx_free(h->ctx, &h->d_tiles);
   ctx_free(h->ctx, &h->d_bufs);
@@ -1924,6 +1961,7 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
   const int P = h->P;
   const int nout = (int)h->out_types.size();
   const int nbin = h->base.nbin;
+  if (!h->stopped) GZ_TRY(split_flush(h));
   const int nseg = (int)h->segments.size();
   for (auto* s : h->segments) GZ_TRY(resolve_segment(h, s));
   // rows per input batch per partition, segment by segment (counts[s][b * P + p]),

@@ -81,7 +81,8 @@ class HashRelation:
 
     def close(self):
         if self._h:
-            L.lib().gz_relation_release(self._h)
+            if self.ctx.alive:
+                L.lib().gz_relation_release(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):
@@ -162,7 +163,8 @@ class ConditionedProbeKernel:
 
     def close(self):
         if self._h:
-            L.lib().gz_probe_release(self._h)
+            if self.rel.ctx.alive:
+                L.lib().gz_probe_release(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

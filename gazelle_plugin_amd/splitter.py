@@ -199,7 +199,8 @@ class Splitter:
 
     def close(self):
         if self._h:
-            L.lib().gz_split_release(self._h)
+            if self.ctx.alive:
+                L.lib().gz_split_release(self._h)
             self._h = None
 
     def __del__(self):

@@ -82,6 +82,8 @@ typedef struct gz_context gz_context;
 /* Replaces: the per-task arrow::MemoryPool handed to the kernels
  * (jni_wrapper.cc:1081-1085).  hbm_budget_bytes <= 0 means "no cap". */
 int gz_context_create(int device, int64_t hbm_budget_bytes, gz_context** out);
+/* Frees the context and all device memory of its pool: every handle made on it
+ * (splitter, aggregate, relation, probe) is released first or not at all. */
 int gz_context_destroy(gz_context* ctx);
 /* Launch on a caller-owned HIP stream (hipStream_t) instead of the context's. */
 int gz_context_set_stream(gz_context* ctx, void* hip_stream);
@@ -142,9 +144,15 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
                   const gz_split_options* opts, gz_splitter** out);
 
 /* Replaces Splitter::Split(const RecordBatch&) (splitter.cc:387-393) /
- * ShuffleSplitterJniWrapper.split (jni_wrapper.cc:1156-1195): one batch, one
- * launch sequence (the plain per-batch path).  The buffers are borrowed in
- * stream order (see the top of this file). */
+ * ShuffleSplitterJniWrapper.split (jni_wrapper.cc:1156-1195): one batch (the
+ * plain per-batch path).  The call checks the buffers and copies them, with one
+ * kernel on the context's stream, into a device arena of the splitter (up to
+ * 128 MiB, about 32 batches of the first one's size); the staged batches are
+ * split together, as by gz_split_batches, when the arena fills or at the next
+ * gz_split_batches / gz_split_stop.  A batch above a quarter of the arena, the
+ * "range" splitter (its bad partition ids are the call's error) and
+ * append_row_ids run at once.  The buffers are borrowed in stream order (see
+ * the top of this file): the copy is the queued work that reads them. */
 int gz_split_batch(gz_splitter* h, int64_t num_rows, const int64_t* buf_addrs,
                    const int64_t* buf_sizes, int32_t num_bufs);
 
@@ -162,8 +170,9 @@ int gz_split_num_partitions(gz_splitter* h);
 /* Output columns (the "range" pid column excluded) and their types. */
 int gz_split_num_output_columns(gz_splitter* h);
 
-/* The output is a list of segments (one per gz_split_batch/gz_split_batches
- * call).  Inside a segment, the rows of partition p are contiguous, in input
+/* The output is a list of segments (one per gz_split_batches call or run of
+ * staged gz_split_batch calls, or per gz_split_batch that ran at once).
+ * Inside a segment, the rows of partition p are contiguous, in input
  * order, at row positions [row_start[p], row_start[p+1]).  Per-partition
  * concatenation of the segments in order is the reference's partition stream.
  *   values:   fixed-width values at global row positions (bool: bitmap)

@@ -4,8 +4,9 @@
 1k-1M and cardinalities log-uniform over 10-10^7 (capped at the rows), plus two
 instances of 1M rows with 600k and 150k groups, so that every placement runs
 inside one set: the LDS first level, the partitioned placement and the HBM
-table (more than 256 partitions of 512 groups).  The set is prepared,
-autotuned, captured as a HIP graph and replayed; every instance equals the
+table (more than 256 partitions of 512 groups).  The set runs on the default
+plan (every placement), then autotuned, captured as a HIP graph and replayed
+(the tuner may pick any placements); every instance equals the
 oracle's aggregate of its batch (integers and min/max bit for bit, FP64 sums
 and averages within 1e-12)."""
 import numpy as np
@@ -43,13 +44,22 @@ def test_batched_workload_shape_matches_oracle(ctx):
     host = [instance_batch(int(r), int(g), 1000 + i) for i, (r, g) in enumerate(zip(rows, groups))]
     dev = [DeviceBatch.from_host(b) for b in host]
     k = HashAggregateKernel.Make(ctx, datagen.AGG_TYPES, CONFIG0_ACTIONS)
-    it = HashAggregateResultIterator(k, num_instances=len(host))
-    it.Autotune(dev, list(range(len(host))))
-    placements = {it.placement(i) for i in range(len(host))}
-    assert {1, 2} <= placements, placements  # the LDS first level and the HBM table (600k groups)
-    it.Capture()
     acts = [parse_action(n, i) for n, i in CONFIG0_ACTIONS]
     exp = [oracle.aggregate([b], datagen.AGG_TYPES, acts) for b in host]
+    # the default plan places instances by their estimated groups: the LDS first
+    # level, the partitioned placement and the HBM table all run in one set
+    it = HashAggregateResultIterator(k, num_instances=len(host))
+    it.Prepare(dev, list(range(len(host))))
+    placements = {it.placement(i) for i in range(len(host))}
+    assert placements == {1, 2, 3}, placements
+    it.Run()
+    for i in range(len(host)):
+        assert_results_equal(it.results(i), exp[i], RTOL)
+    it.close()
+    # the autotuned plan (whichever placements won the timing), captured and replayed
+    it = HashAggregateResultIterator(k, num_instances=len(host))
+    it.Autotune(dev, list(range(len(host))))
+    it.Capture()
     for rep in range(2):
         it.Launch()
         ctx.synchronize()

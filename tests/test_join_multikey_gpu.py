@@ -188,9 +188,12 @@ def test_multi_key_relation_counts_non_null_tuples(ctx, builder_type):
     rel.close()
 
 
-def test_utf8_key_offsets_past_the_data_buffer_are_invalid(ctx):
+@pytest.mark.parametrize("per_batch", [True, False])
+def test_utf8_key_offsets_past_the_data_buffer_are_invalid(ctx, per_batch):
     """A batch whose utf8 key offsets end past its declared data size is
-    rejected before any copy into the key dictionary's byte heap."""
+    rejected before any copy into the key dictionary's byte heap: by the
+    batched call, or, for a staged per-batch call, when the staged batches run
+    (at the latest at finish)."""
     from gazelle_plugin_amd.aggregate import HashAggregateKernel
     from gazelle_plugin_amd.batch import DeviceBatch
     n = 100
@@ -201,6 +204,12 @@ def test_utf8_key_offsets_past_the_data_buffer_are_invalid(ctx):
     db.sizes[2] = 100  # declared data bytes: fewer than the 500 the offsets span
     it = HashAggregateKernel.Make(ctx, [T.STRING, T.INT64], [("action_groupby", [0]), ("action_sum", [1])]) \
         .MakeResultIterator()
-    with pytest.raises(L.Invalid):
-        it.ProcessAndCacheOne(db)
-    it.close()
+    try:
+        with pytest.raises(L.Invalid):
+            if per_batch:
+                it.ProcessAndCacheOne(db)
+                it.finish()
+            else:
+                it.ProcessBatches([db])
+    finally:
+        it.close()
