@@ -135,6 +135,45 @@ __global__ void join_append_kernel(AppendParams p) {
   }
 }
 
+// Many batches of fixed-width columns in one launch: one workgroup per tile of
+// a batch (the tile table), the batch's buffers from a device table.
+struct AppendTile {
+  int32_t batch, row_begin, nrows, pad;
+  int64_t row0;  // store row of the tile's first row
+};
+
+struct AppendManyParams {
+  int ncols;
+  int type[kJoinMaxCols], width[kJoinMaxCols];
+  uint8_t* dvalid[kJoinMaxCols];
+  uint8_t* dvalues[kJoinMaxCols];
+  const AppendTile* tiles;
+  const void* const* bufs;  // [batch * ncols * 2 + col * 2 + {0 validity, 1 values}]
+};
+
+__global__ __launch_bounds__(256) void join_append_many_kernel(AppendManyParams p) {
+  const AppendTile T = p.tiles[blockIdx.x];
+  const void* const* b = p.bufs + (int64_t)T.batch * p.ncols * 2;
+  for (int i = threadIdx.x; i < T.nrows; i += blockDim.x) {
+    const int64_t r = T.row_begin + i, d = T.row0 + i;
+    for (int c = 0; c < p.ncols; c++) {
+      const int t = p.type[c];
+      if (t == GZ_TYPE_NA) continue;
+      const uint8_t* vb = (const uint8_t*)b[c * 2];
+      const uint8_t* v = (const uint8_t*)b[c * 2 + 1];
+      p.dvalid[c][d] = vb ? (uint8_t)bit_get(vb, r) : 1;
+      const int w = p.width[c];
+      if (t == GZ_TYPE_BOOL) p.dvalues[c][d] = (uint8_t)bit_get(v, r);
+      else if (w == 8) ((uint64_t*)p.dvalues[c])[d] = ((const uint64_t*)v)[r];
+      else if (w == 4) ((uint32_t*)p.dvalues[c])[d] = ((const uint32_t*)v)[r];
+      else if (w == 2) ((uint16_t*)p.dvalues[c])[d] = ((const uint16_t*)v)[r];
+      else if (w == 1) p.dvalues[c][d] = v[r];
+      else
+        for (int k = 0; k < w; k++) p.dvalues[c][d * w + k] = v[r * w + k];
+    }
+  }
+}
+
 __global__ void join_copy_bytes_kernel(uint8_t* dst, const uint8_t* src, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -196,9 +235,18 @@ __global__ void join_build_kernel(const uint8_t* kvalid, const uint8_t* kvals, i
             }
           }
           if (k == key) {
-            int c = atomicAdd(&count[s], 1);
-            if (c >= 1) flags[1] = 1;
-            atomicMin(&start[s], (int32_t)r);
+            // the row that inserted the key owns the slot: its count and start
+            // are plain stores; a later row of the key marks repeated keys,
+            // whose counts and starts the finish recomputes (skip-duplicate
+            // building keeps the smallest row)
+            if (fresh) {
+              count[s] = 1;
+              if (skip_dup) atomicMin(&start[s], (int32_t)r);
+              else start[s] = (int32_t)r;
+            } else {
+              flags[1] = 1;
+              if (skip_dup) atomicMin(&start[s], (int32_t)r);
+            }
             row_slot[r] = (int32_t)s;
             break;
           }
@@ -212,7 +260,21 @@ __global__ void join_build_kernel(const uint8_t* kvalid, const uint8_t* kvals, i
     if (m && (int)__lane_id() == __ffsll((long long)act) - 1) atomicAdd(&flags[2], (unsigned long long)__popcll(m));
   }
   (void)kwidth;
-  (void)skip_dup;
+}
+
+// Rows per slot of a table with repeated keys (the build counts only inserts).
+__global__ void join_recount_kernel(const int32_t* row_slot, int64_t n, int32_t* count) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t s = row_slot[r];
+    if (s >= 0) atomicAdd(&count[s], 1);
+  }
+}
+
+__global__ void join_zero_i32_kernel(int32_t* p, uint64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
 }
 
 // With skip-duplicate building (semi joins) every key keeps one row.
@@ -402,6 +464,45 @@ __device__ __forceinline__ uint32_t probe_keys(const ProbeParams& p, const PTile
   return vm;
 }
 
+// Match codes (see ProbeParams::match) of the tile rows lr0 + k * S: keys and
+// home slots loaded together, collisions walk on.
+template <int RPT, int S>
+__device__ __forceinline__ void probe_match(const ProbeParams& p, const PTile& td, int lr0, int32_t (&m)[RPT]) {
+  const uint64_t mask = p.J.cap - 1;
+  int64_t key[RPT];
+  uint32_t nanm;
+  const uint32_t vm = probe_keys<RPT, S>(p, td, lr0, key, &nanm);
+  uint64_t s[RPT];
+  int64_t sk[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    s[k] = jmix64((uint64_t)key[k]) & mask;
+    const bool look = ((vm >> k) & 1) && !((nanm >> k) & 1) && key[k] != kJEmpty;
+    sk[k] = look ? gld(p.J.keys + s[k]) : kJEmpty;
+  }
+#pragma unroll
+  for (int k = 0; k < RPT; k++) {
+    m[k] = -1;
+    if (lr0 + k * S >= td.nrows) continue;
+    if (!((vm >> k) & 1)) m[k] = -3;
+    else if ((nanm >> k) & 1) m[k] = -1;
+    else if (key[k] == kJEmpty) m[k] = p.J.special_count > 0 ? -2 : -1;
+    else {
+      int64_t v = sk[k];
+      uint64_t q = s[k];
+      for (uint64_t i = 0; i < p.J.cap; i++) {
+        if (v == key[k]) {
+          m[k] = (int32_t)q;
+          break;
+        }
+        if (v == kJEmpty) break;
+        q = (q + 1) & mask;
+        v = gld(p.J.keys + q);
+      }
+    }
+  }
+}
+
 // Per probe row: the match code and the number of output rows; per tile the
 // total.  Rows are lane-consecutive (coalesced key loads and match stores), the
 // home slots of all RPT rows are loaded together and collisions walk on.
@@ -412,46 +513,18 @@ __global__ __launch_bounds__(BLOCK) void join_probe_count_kernel(ProbeParams p) 
   if (t < 0) return;
   const PTile td = p.tiles[t];
   const int tid = threadIdx.x;
-  const uint64_t mask = p.J.cap - 1;
   uint64_t sum = 0;
   for (int base = 0; base < td.nrows; base += BLOCK * RPT) {
-    int64_t key[RPT];
-    uint32_t nanm;
-    const uint32_t vm = probe_keys<RPT, BLOCK>(p, td, base + tid, key, &nanm);
-    uint64_t s[RPT];
-    int64_t sk[RPT];
-#pragma unroll
-    for (int k = 0; k < RPT; k++) {
-      s[k] = jmix64((uint64_t)key[k]) & mask;
-      const bool look = ((vm >> k) & 1) && !((nanm >> k) & 1) && key[k] != kJEmpty;
-      sk[k] = look ? gld(p.J.keys + s[k]) : kJEmpty;
-    }
+    int32_t m[RPT];
+    probe_match<RPT, BLOCK>(p, td, base + tid, m);
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
       const int lr = base + k * BLOCK + tid;
       if (lr >= td.nrows) continue;
-      int32_t m;
-      if (!((vm >> k) & 1)) m = -3;
-      else if ((nanm >> k) & 1) m = -1;
-      else if (key[k] == kJEmpty) m = p.J.special_count > 0 ? -2 : -1;
-      else {
-        m = -1;
-        int64_t v = sk[k];
-        uint64_t q = s[k];
-        for (uint64_t i = 0; i < p.J.cap; i++) {
-          if (v == key[k]) {
-            m = (int32_t)q;
-            break;
-          }
-          if (v == kJEmpty) break;
-          q = (q + 1) & mask;
-          v = gld(p.J.keys + q);
-        }
-      }
-      gst(p.match + t * p.tile_rows + lr, m);
+      gst(p.match + t * p.tile_rows + lr, m[k]);
       // repeated keys: the slot's row count (inner joins only read it)
-      const int32_t cs = m >= 0 && p.J.rowlist && p.join_type == GZ_JOIN_INNER ? gld(p.J.count + m) : 1;
-      sum += (uint64_t)out_count(p, m, cs);
+      const int32_t cs = m[k] >= 0 && p.J.rowlist && p.join_type == GZ_JOIN_INNER ? gld(p.J.count + m[k]) : 1;
+      sum += (uint64_t)out_count(p, m[k], cs);
     }
   }
   uint64_t total;
@@ -787,6 +860,7 @@ struct gz_relation {
   std::vector<int> kcols, otypes;
   int ocols = 0;
   KeyScratch ksc;
+  DBuf d_atiles, d_abufs;  // tile and buffer tables of a batched append
 };
 
 static int rel_grow(gz_relation* r, int64_t need) {
@@ -894,6 +968,65 @@ static int rel_append_one(gz_relation* r, const gz_batch& bt) {
   return GZ_OK;
 }
 
+// Many batches of a relation without utf8 columns: checked on the host, copied
+// by one launch over a tile table.
+static int rel_append_many(gz_relation* r, int32_t nb, const gz_batch* batches) {
+  gz_context* ctx = r->ctx;
+  if (r->finished) return fail(GZ_INVALID, "relation already finished");
+  const int nc = r->ncols;
+  int need = 0;
+  for (int c = 0; c < nc; c++) need += type_num_bufs(r->types[c]);
+  std::vector<const void*> bufs((size_t)std::max(nb, 1) * nc * 2, nullptr);
+  std::vector<AppendTile> tiles;
+  int64_t row0 = r->n;
+  for (int b = 0; b < nb; b++) {
+    const gz_batch& bt = batches[b];
+    if (bt.num_rows < 0) return fail(GZ_INVALID, "bad num_rows");
+    if (bt.num_rows == 0) continue;
+    if (!bt.buf_addrs || !bt.buf_sizes) return fail(GZ_INVALID, "buf_addrs can't be null");
+    if (row0 + bt.num_rows > INT32_MAX) return fail(GZ_CAPACITY_ERROR, "build side exceeds 2^31 rows");
+    if (bt.num_bufs < need) return fail(GZ_INVALID, "insufficient number of in_buf_addrs");
+    for (int c = 0, k = 0; c < nc; c++, k += 2) {
+      const int t = r->types[c];
+      const void* vb = (bt.buf_addrs[k] && bt.buf_sizes[k] > 0) ? (const void*)bt.buf_addrs[k] : nullptr;
+      if (vb && bt.buf_sizes[k] < (bt.num_rows + 7) / 8) return fail(GZ_INVALID, "validity buffer too small");
+      if (t != GZ_TYPE_NA) {
+        const int64_t need_b = t == GZ_TYPE_BOOL ? (bt.num_rows + 7) / 8 : bt.num_rows * type_width(t);
+        if (!bt.buf_addrs[k + 1]) return fail(GZ_INVALID, "missing values buffer");
+        if (bt.buf_sizes[k + 1] < need_b) return fail(GZ_INVALID, "values buffer too small");
+      }
+      bufs[((size_t)b * nc + c) * 2] = vb;
+      bufs[((size_t)b * nc + c) * 2 + 1] = (const void*)bt.buf_addrs[k + 1];
+    }
+    for (int64_t r0 = 0; r0 < bt.num_rows; r0 += 1024)
+      tiles.push_back(AppendTile{b, (int32_t)r0, (int32_t)std::min<int64_t>(1024, bt.num_rows - r0), 0, row0 + r0});
+    row0 += bt.num_rows;
+  }
+  if (tiles.empty()) return GZ_OK;
+  GZ_TRY(rel_grow(r, row0));
+  AppendManyParams ap;
+  memset(&ap, 0, sizeof(ap));
+  ap.ncols = nc;
+  for (int c = 0; c < nc; c++) {
+    ColStore& cs = r->cols[c];
+    ap.type[c] = cs.type;
+    ap.width[c] = cs.width;
+    ap.dvalid[c] = (uint8_t*)cs.valid.p;
+    ap.dvalues[c] = (uint8_t*)cs.values.p;
+  }
+  GZ_TRY(upload(ctx, &r->d_atiles, tiles.data(), sizeof(AppendTile) * tiles.size()));
+  GZ_TRY(upload(ctx, &r->d_abufs, bufs.data(), sizeof(void*) * bufs.size()));
+  ap.tiles = (const AppendTile*)r->d_atiles.p;
+  ap.bufs = (const void* const*)r->d_abufs.p;
+  {
+    KernelTimer kt(ctx, "join_append");
+    hipLaunchKernelGGL(join_append_many_kernel, dim3((unsigned)tiles.size()), dim3(256), 0, ctx->stream, ap);
+  }
+  GZ_HIP(hipGetLastError());
+  r->n = row0;
+  return GZ_OK;
+}
+
 extern "C" {
 
 int gz_relation_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types, int32_t key_col,
@@ -953,6 +1086,10 @@ int gz_relation_append_batches(gz_relation* r, int32_t num_batches, const gz_bat
                      r->kcols.size() == 1 ? r->kcols[0] : -1);
     batches = kbatches.data();
   }
+  bool fixed = num_batches > 1;
+  for (int c = 0; c < r->ncols && fixed; c++)
+    if (type_is_binary(r->types[c])) fixed = false;
+  if (fixed) return rel_append_many(r, num_batches, batches);
   // reserve once so the column store grows a single time
   int64_t total = r->n;
   for (int i = 0; i < num_batches; i++) total += batches[i].num_rows;
@@ -1011,6 +1148,13 @@ int gz_relation_finish(gz_relation* r) {
   }
   r->has_dups = dups;
   if (dups) {
+    // the build counted inserts only: count every row of its slot
+    hipLaunchKernelGGL(join_zero_i32_kernel, dim3(gc), dim3(256), 0, st, (int32_t*)r->count.p, cap);
+    {
+      unsigned gr = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+      hipLaunchKernelGGL(join_recount_kernel, dim3(gr), dim3(256), 0, st, (const int32_t*)r->row_slot.p, n,
+                         (int32_t*)r->count.p);
+    }
     // stable (slot, row) order of all keyed rows, then each slot's start in it
     GZ_TRY(ctx_alloc(ctx, sizeof(unsigned long long) * n, &r->sort_a));
     GZ_TRY(ctx_alloc(ctx, sizeof(unsigned long long) * n, &r->sort_b));
@@ -1070,6 +1214,8 @@ int gz_relation_release(gz_relation* r) {
   gz_context* ctx = r->ctx;
   kd_release(r->kd);
   kd_scratch_free(ctx, &r->ksc);
+  ctx_free(ctx, &r->d_atiles);
+  ctx_free(ctx, &r->d_abufs);
   for (auto& c : r->cols)
     for (DBuf* b : {&c.valid, &c.values, &c.offsets, &c.data}) ctx_free(ctx, b);
   for (DBuf* b : {&r->keys, &r->start, &r->count, &r->rowlist, &r->row_slot, &r->flags, &r->sort_a,
