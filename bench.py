@@ -580,7 +580,7 @@ def bench_join(args, ctx, stream):
 
     def do_build():
         r = HashRelationKernel.Make(ctx, [T.INT64, T.INT32, T.DOUBLE], 0)
-        r.EvaluateBatches(build.batches)
+        r.EvaluateBatches(build.batches, barr)
         r.Finish()
         return r
 

@@ -367,9 +367,10 @@ struct ProbeParams {
   int build_has_null;
   int64_t build_keys;       // keys in the table (null keys excluded)
   JTable J;
-  // per probe row (tile-major): the matched slot, -1 no match, -2 the kJEmpty
-  // key, -3 a null key
+  // per probe row (tile-major): the matched slot (with match_rows: the matched
+  // build row), -1 no match, -2 the kJEmpty key, -3 a null key
   int32_t* match;
+  int match_rows;  // inner joins over unique keys: the count pass resolves rows
   uint32_t* tile_count;
   const uint64_t* tile_off;
   int ncols;
@@ -474,11 +475,13 @@ __device__ __forceinline__ void probe_match(const ProbeParams& p, const PTile& t
   const uint32_t vm = probe_keys<RPT, S>(p, td, lr0, key, &nanm);
   uint64_t s[RPT];
   int64_t sk[RPT];
+  int32_t st[RPT];  // the home slot's first row, loaded with its key (match_rows)
 #pragma unroll
   for (int k = 0; k < RPT; k++) {
     s[k] = jmix64((uint64_t)key[k]) & mask;
     const bool look = ((vm >> k) & 1) && !((nanm >> k) & 1) && key[k] != kJEmpty;
     sk[k] = look ? gld(p.J.keys + s[k]) : kJEmpty;
+    st[k] = look && p.match_rows ? gld(p.J.start + s[k]) : 0;
   }
 #pragma unroll
   for (int k = 0; k < RPT; k++) {
@@ -492,7 +495,7 @@ __device__ __forceinline__ void probe_match(const ProbeParams& p, const PTile& t
       uint64_t q = s[k];
       for (uint64_t i = 0; i < p.J.cap; i++) {
         if (v == key[k]) {
-          m[k] = (int32_t)q;
+          m[k] = !p.match_rows ? (int32_t)q : q == s[k] ? st[k] : gld(p.J.start + q);
           break;
         }
         if (v == kJEmpty) break;
@@ -602,7 +605,8 @@ __global__ __launch_bounds__(BLOCK) void join_probe_write01_kernel(ProbeParams p
     for (int g = 0; g < G; g++) {
       const int k = k0 + g;
       b[g] = -1;
-      if (inner && (pre[k] >> 31)) b[g] = m[k] >= 0 ? gld(p.J.start + m[k]) : (int32_t)p.J.special_start;
+      if (inner && (pre[k] >> 31))
+        b[g] = m[k] < 0 ? (int32_t)p.J.special_start : p.match_rows ? m[k] : gld(p.J.start + m[k]);
     }
 #pragma unroll
     for (int g = 0; g < G; g++) pl[g] = p.payload && b[g] >= 0 ? gld(p.payload + b[g]) : u32x4{0u, 0u, 0u, 0u};
@@ -1330,6 +1334,7 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   p.J.cap = r->cap;
   p.J.special_start = r->special_start;
   p.J.special_count = r->special_count;
+  p.match_rows = p.join_type == GZ_JOIN_INNER && !p.J.rowlist ? 1 : 0;
   std::vector<uint32_t> tc(std::max<int64_t>(T, 1), 0);
   uint64_t total = 0;
   if (T > 0) {

@@ -56,9 +56,12 @@ class HashRelation:
         self.ctx.hold(batch)
         L.check(L.lib().gz_relation_append(self._h, batch.num_rows, batch.addrs, batch.sizes, batch.num_bufs))
 
-    def EvaluateBatches(self, batches: Sequence[DeviceBatch]):
+    def EvaluateBatches(self, batches: Sequence[DeviceBatch], arr=None):
+        """arr: a prebuilt gz_batch array of the batches (make_batch_array)."""
         self.ctx.hold(list(batches))
-        L.check(L.lib().gz_relation_append_batches(self._h, len(batches), make_batch_array(batches)))
+        if arr is None:
+            arr = make_batch_array(batches)
+        L.check(L.lib().gz_relation_append_batches(self._h, len(batches), arr))
 
     def Finish(self):
         L.check(L.lib().gz_relation_finish(self._h))
