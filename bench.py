@@ -611,6 +611,20 @@ def bench_join(args, ctx, stream):
         algo = probe.num_rows * (8 + 12) + n * per_out
         out[jt] = {"ms": ms, "min_ms": mn, "out_rows": n, "probe_rows_per_s": probe.num_rows / (ms / 1e3),
                    "GBps": algo / (ms / 1e3) / 1e9, "frac_of_hbm_peak": algo / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+        if jt == "inner":
+            # the plain per-batch path (one Process call per 4096-row batch, as
+            # ConditionedProbe's Process is called), host wall time
+            nbp = min(2048, len(probe.batches))
+            for b in probe.batches[:16]:
+                pk.Process(b)
+            torch.cuda.synchronize()
+            tp = time.perf_counter()
+            for b in probe.batches[:nbp]:
+                pk.Process(b)
+            torch.cuda.synchronize()
+            pms = (time.perf_counter() - tp) * 1e3
+            out[jt]["plain"] = {"batches": nbp, "ms": pms,
+                                "probe_rows_per_s": nbp * probe.batches[0].num_rows / (pms / 1e3)}
         pk.close()
     out["join_aggregate"] = bench_join_aggregate(ctx, stream, rel, probe, parr)
     out["two_key"] = bench_join_two_keys(ctx, stream, rel, build, probe)

@@ -37,7 +37,12 @@ constexpr int kJoinMaxCols = 32;
 #ifndef GZ_WRITE01_BLOCK
 #define GZ_WRITE01_BLOCK 1024
 #endif
-constexpr int kWrite01Block = GZ_WRITE01_BLOCK;  // threads of the at-most-one-row write pass (4096-row tiles)
+constexpr int kWrite01Block = GZ_WRITE01_BLOCK;
+// one-batch calls: 256-row blocks of one wave, at most one wave of blocks
+constexpr int kSingleTile = 256;
+constexpr int kSingleMaxBlocks = 64;
+constexpr int kProbeInlineBufs = 48;  // buffers of up to 16 probe columns
+constexpr int kLookSpins = 1 << 20;   // polls of an earlier block's count before giving up  // threads of the at-most-one-row write pass (4096-row tiles)
 constexpr int64_t kJEmpty = INT64_MIN;
 
 __device__ __forceinline__ uint64_t jmix64(uint64_t k) {
@@ -382,10 +387,21 @@ struct ProbeParams {
   // build columns gathered from the row-major payload (null: per-column gathers)
   const u32x4* payload;
   int pay_off[kJoinMaxCols];  // byte offset of output column c in a payload record
+  // a call of one small batch (join_probe_write01_kernel with SINGLE = 1): the
+  // batch and its buffers ride in the arguments (bufs null), each block posts
+  // its emitted-row count tagged with the call's epoch, and the row total goes
+  // straight to host memory
+  PTile tile0;
+  uint64_t* look;       // [kSingleMaxBlocks]: epoch << 32 | rows of that block
+  uint32_t epoch;
+  uint32_t* host_out;   // [0] output rows, [1] set when a wait for an earlier block ran out
+  const void* ibufs[kProbeInlineBufs];
 };
+static_assert(sizeof(ProbeParams) <= 4096, "kernel arguments");
 
 __device__ __forceinline__ const void* pbuf(const ProbeParams& p, int batch, int col, int k) {
-  return p.bufs[(int64_t)batch * p.bstride + col * 3 + k];
+  const int64_t i = (int64_t)batch * p.bstride + col * 3 + k;
+  return p.bufs ? p.bufs[i] : p.ibufs[i];
 }
 
 // Output rows of a probe row given its match code (see ProbeParams::match) and
@@ -554,25 +570,45 @@ __device__ __forceinline__ void store_value(const OutCol& oc, uint64_t o, uint64
 // consecutive: wave ballots give each emitting row its place, so the stores of
 // a wave land on consecutive output rows.  Build columns come from the payload
 // (one 16-byte load per emitted row) when it holds them.
-template <int TILE, int BLOCK>
+// SINGLE = 1: a call of one batch of at most kSingleMaxBlocks tiles probes and
+// writes in this one launch: block b takes tile b, computes its match codes,
+// posts its emitted-row count and sums the counts of the blocks before it (the
+// lanes of one wave poll one earlier block each) for its output offset.  Blocks
+// are dispatched in index order, so each wait is on a block already running;
+// the polls are bounded all the same and a wait that runs out is reported.
+template <int TILE, int BLOCK, int SINGLE>
 __global__ __launch_bounds__(BLOCK) void join_probe_write01_kernel(ProbeParams p) {
   constexpr int RPT = TILE / BLOCK;
   constexpr int NW = BLOCK / kWave;
   constexpr int G = 4;  // rows per thread whose gathers are in flight together
   static_assert(RPT % G == 0, "groups of G rounds");
   __shared__ uint32_t s_wt[RPT][NW];
-  int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
-  if (t < 0) return;
-  const PTile td = p.tiles[t];
+  __shared__ uint32_t s_base;
+  int64_t t;
+  PTile td;
+  if (SINGLE) {
+    t = blockIdx.x;
+    td = p.tile0;
+    td.row_begin = (int32_t)(t * TILE);
+    td.nrows = min(TILE, p.tile0.nrows - td.row_begin);
+  } else {
+    t = xcd_tile(blockIdx.x, gridDim.x, p.T);
+    if (t < 0) return;
+    td = p.tiles[t];
+  }
   const int tid = threadIdx.x, w = tid / kWave, lane = __lane_id();
   const int n = td.nrows;
   const int64_t rb = td.row_begin;
   int32_t m[RPT];
   uint32_t pre[RPT];  // bit 31: the row emits; low bits: emitting rows before it in its wave and round
+  if (SINGLE) {
+    probe_match<RPT, BLOCK>(p, td, tid, m);
+  } else {
 #pragma unroll
-  for (int k = 0; k < RPT; k++) {
-    const int lr = k * BLOCK + tid;
-    m[k] = lr < n ? gld(p.match + t * p.tile_rows + lr) : -1;
+    for (int k = 0; k < RPT; k++) {
+      const int lr = k * BLOCK + tid;
+      m[k] = lr < n ? gld(p.match + t * p.tile_rows + lr) : -1;
+    }
   }
 #pragma unroll
   for (int k = 0; k < RPT; k++) {
@@ -583,7 +619,45 @@ __global__ __launch_bounds__(BLOCK) void join_probe_write01_kernel(ProbeParams p
     if (lane == 0) s_wt[k][w] = (uint32_t)__popcll(b);
   }
   __syncthreads();
-  const uint64_t obase = p.tile_off[t];
+  uint64_t obase;
+  if (SINGLE) {
+    if (w == 0) {
+      uint32_t rows = 0;
+      for (int k = 0; k < RPT; k++)
+        for (int x = 0; x < NW; x++) rows += s_wt[k][x];
+      const int id = (int)t;
+      if (lane == 0)
+        __hip_atomic_store(p.look + id, (uint64_t)p.epoch << 32 | rows, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t before = 0;
+      bool late = false;
+      if (lane < id) {
+        uint64_t v;
+        int spins = 0;
+        while (((v = __hip_atomic_load(p.look + lane, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >> 32) !=
+               p.epoch) {
+          if (++spins > kLookSpins) {
+            late = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        before = late ? 0u : (uint32_t)v;
+      }
+#pragma unroll
+      for (int o = kWave / 2; o > 0; o >>= 1) before += __shfl_xor(before, o);
+      const bool any_late = __ballot(late) != 0;
+      if (lane == 0) {
+        s_base = before;
+        if (any_late) gst(p.host_out + 1, 1u);
+        if (id == (int)gridDim.x - 1) gst(p.host_out, before + rows);
+        __threadfence_system();
+      }
+    }
+    __syncthreads();
+    obase = s_base;
+  } else {
+    obase = p.tile_off[t];
+  }
   uint32_t acc = 0;  // rows emitted by earlier rounds
 #pragma unroll
   for (int k = 0; k < RPT; k++) {
@@ -1240,6 +1314,12 @@ struct gz_probe {
   std::vector<int> sides, idx;
   int tile_rows = 4096;
   DBuf d_tiles, d_bufs, match, tile_count, tile_off, scan_tmp, pair_probe, pair_build;
+  // one-batch calls: per-block counts tagged with the call's epoch, and the
+  // pinned host words the kernel writes the row total to
+  DBuf look;
+  uint32_t epoch = 0;
+  uint32_t* host_out = nullptr;
+  uint32_t* host_out_dev = nullptr;
   // row-major payload of the fixed-width build output columns (built at the
   // first probe call when they fit a 16-byte record)
   DBuf payload;
@@ -1337,7 +1417,42 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   p.match_rows = p.join_type == GZ_JOIN_INNER && !p.J.rowlist ? 1 : 0;
   std::vector<uint32_t> tc(std::max<int64_t>(T, 1), 0);
   uint64_t total = 0;
-  if (T > 0) {
+  // a call of one small batch whose probe rows emit at most one row each,
+  // without utf8 outputs: one launch probes and writes (outputs sized for the
+  // batch), with no uploads and no copy back
+  bool any_bin_out = false;
+  for (auto& o : h->outs) any_bin_out |= type_is_binary(o.type);
+  const int64_t rows0 = nb == 1 ? batches[0].num_rows : 0;
+  const int64_t sblocks = (rows0 + kSingleTile - 1) / kSingleTile;
+  const bool single = nb == 1 && rows0 > 0 && sblocks <= kSingleMaxBlocks && nc * 3 <= kProbeInlineBufs &&
+                      (p.join_type != GZ_JOIN_INNER || !p.J.rowlist) && !any_bin_out;
+  if (single) {
+    if (!h->look.p) {
+      GZ_TRY(ctx_ensure(ctx, &h->look, sizeof(uint64_t) * kSingleMaxBlocks));
+      hipLaunchKernelGGL(join_zero_i32_kernel, dim3(1), dim3(256), 0, st, (int32_t*)h->look.p,
+                         (uint64_t)(2 * kSingleMaxBlocks));
+      h->epoch = 0;
+    }
+    if (!h->host_out) {
+      GZ_HIP(hipHostMalloc((void**)&h->host_out, 2 * sizeof(uint32_t), hipHostMallocDefault));
+      GZ_HIP(hipHostGetDevicePointer((void**)&h->host_out_dev, h->host_out, 0));
+    }
+    if (++h->epoch == 0) {  // a fresh epoch never matches a count a block posted before
+      hipLaunchKernelGGL(join_zero_i32_kernel, dim3(1), dim3(256), 0, st, (int32_t*)h->look.p,
+                         (uint64_t)(2 * kSingleMaxBlocks));
+      h->epoch = 1;
+    }
+    p.tile0 = PTile{0, 0, (int32_t)rows0, 0};
+    p.look = (uint64_t*)h->look.p;
+    p.epoch = h->epoch;
+    p.host_out = h->host_out_dev;
+    for (int i = 0; i < nc * 3; i++) p.ibufs[i] = bufs[i];
+    p.bufs = nullptr;
+    p.tiles = nullptr;
+    ((volatile uint32_t*)h->host_out)[0] = 0xffffffffu;
+    ((volatile uint32_t*)h->host_out)[1] = 0;
+    total = (uint64_t)rows0;  // output rows the columns are sized for
+  } else if (T > 0) {
     GZ_TRY(upload(ctx, &h->d_tiles, tiles.data(), sizeof(PTile) * T));
     GZ_TRY(upload(ctx, &h->d_bufs, bufs.data(), sizeof(void*) * bufs.size()));
     GZ_TRY(ctx_ensure(ctx, &h->match, sizeof(int32_t) * T * TILE));
@@ -1445,15 +1560,28 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   for (int c = (int)h->pay_off.size(); c < nres; c++) p.pay_off[c] = -1;
   p.payload = use_payload ? (const u32x4*)h->payload.p : nullptr;
   const bool at_most_one = p.join_type != GZ_JOIN_INNER || !p.J.rowlist;
-  if (T > 0 && total > 0) {
+  if (single) {
+    {
+      KernelTimer kt(ctx, "join_probe_write");
+      hipLaunchKernelGGL((join_probe_write01_kernel<kSingleTile, kWave, 1>), dim3((unsigned)sblocks), dim3(kWave), 0,
+                         st, p);
+    }
+    GZ_HIP(hipGetLastError());
+    GZ_HIP(hipStreamSynchronize(st));
+    const uint32_t rows = ((volatile uint32_t*)h->host_out)[0];
+    if (((volatile uint32_t*)h->host_out)[1] || rows > (uint64_t)rows0)
+      return fail(GZ_HIP_ERROR, "probe: a block's wait for the blocks before it ran out");
+    total = rows;
+    tc[0] = rows;
+  } else if (T > 0 && total > 0) {
     int64_t grid = (T + 7) / 8 * 8;
     KernelTimer kt(ctx, "join_probe_write");
     if (at_most_one) {
       if (TILE == 4096)
-        hipLaunchKernelGGL((join_probe_write01_kernel<4096, kWrite01Block>), dim3((unsigned)grid),
+        hipLaunchKernelGGL((join_probe_write01_kernel<4096, kWrite01Block, 0>), dim3((unsigned)grid),
                            dim3(kWrite01Block), 0, st, p);
       else
-        hipLaunchKernelGGL((join_probe_write01_kernel<1024, 256>), dim3((unsigned)grid), dim3(256), 0, st, p);
+        hipLaunchKernelGGL((join_probe_write01_kernel<1024, 256, 0>), dim3((unsigned)grid), dim3(256), 0, st, p);
     } else if (TILE == 4096) {
       hipLaunchKernelGGL((join_probe_write_kernel<4096, 256>), dim3((unsigned)grid), dim3(256), 0, st, p);
     } else {
@@ -1654,8 +1782,12 @@ int gz_probe_release(gz_probe* h) {
   for (auto& o : h->outs)
     for (DBuf* b : {&o.values, &o.valid, &o.offsets, &o.data, &o.lens, &o.bits}) ctx_free(ctx, b);
   for (DBuf* b : {&h->d_tiles, &h->d_bufs, &h->match, &h->tile_count, &h->tile_off, &h->scan_tmp,
-                  &h->pair_probe, &h->pair_build, &h->payload})
+                  &h->pair_probe, &h->pair_build, &h->payload, &h->look})
     ctx_free(ctx, b);
+  if (h->host_out) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipHostFree(h->host_out);
+  }
   delete h;
   return GZ_OK;
 }

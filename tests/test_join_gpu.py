@@ -248,3 +248,37 @@ def test_empty_build_side(ctx):
         got = run_case(ctx, build, probe, btypes, ptypes, 0, 0, jt, na, [[1, 0]], True)
         n = sum(len(g[0][1]) for g in got)
         assert n == (sum(b.num_rows for b in probe) if expect_all else 0)
+
+
+@pytest.mark.parametrize("join_type,dup_keys,null_aware", [("inner", False, False), ("semi", False, False),
+                                                           ("anti", False, False), ("anti", False, True),
+                                                           ("semi", True, False), ("inner", True, False)])
+def test_one_batch_calls_match_the_oracle(ctx, join_type, dup_keys, null_aware):
+    """Per-batch calls with fixed-width outputs (a call of one batch of up to
+    16384 rows runs as one launch of 256-row blocks that sum the emitted rows of
+    the blocks before them): batch sizes around the block edges and past the
+    largest such batch, checked row for row against the oracle."""
+    rng = np.random.default_rng(7 + dup_keys)
+    nbuild = 6000
+    bk = rng.integers(0, nbuild // 3 + 1, size=nbuild) if dup_keys else rng.permutation(nbuild * 4)[:nbuild]
+    build = []
+    for b0 in range(0, nbuild, 2500):
+        n = min(nbuild, b0 + 2500) - b0
+        build.append(HostBatch(n, [HostColumn(T.INT64, bk[b0:b0 + n].astype(np.int64),
+                                              bitmap_from_bools(rng.random(n) >= 0.01)),
+                                   HostColumn(T.INT32, rng.integers(-1000, 1000, size=n).astype(np.int32),
+                                              bitmap_from_bools(rng.random(n) >= 0.1)),
+                                   HostColumn(T.DOUBLE, rng.standard_normal(n))]))
+    probe = []
+    for n in (1, 63, 64, 255, 256, 257, 1000, 4096, 16383, 16384, 16385, 0, 20000):
+        pk = np.where(rng.random(n) < 0.8, bk[rng.integers(0, nbuild, size=n)], rng.integers(nbuild * 4, nbuild * 8, size=n))
+        probe.append(HostBatch(n, [HostColumn(T.INT64, pk.astype(np.int64), bitmap_from_bools(rng.random(n) >= 0.02)),
+                                   HostColumn(T.INT32, rng.integers(0, 1 << 30, size=n).astype(np.int32),
+                                              bitmap_from_bools(rng.random(n) >= 0.05))]))
+    btypes, ptypes = [T.INT64, T.INT32, T.DOUBLE], [T.INT64, T.INT32]
+    result = [[1, 0], [0, 1], [0, 2], [1, 1]] if join_type == "inner" else [[1, 1], [1, 0]]
+    got = run_case(ctx, build, probe, btypes, ptypes, 0, 0, join_type, null_aware, result, False)
+    exp = expected_cols(build, probe, 0, 0, T.INT64, join_type, null_aware, result)
+    assert len(got) == len(exp)
+    for g, e in zip(got, exp):
+        check_output(g, e)
