@@ -38,7 +38,7 @@ constexpr int kJoinMaxCols = 32;
 #define GZ_WRITE01_BLOCK 1024
 #endif
 constexpr int kWrite01Block = GZ_WRITE01_BLOCK;
-// one-batch calls: 256-row blocks of one wave, at most one wave of blocks
+// one-batch calls: 256-row workgroups, one row per thread, at most one wave of workgroups
 constexpr int kSingleTile = 256;
 constexpr int kSingleMaxBlocks = 64;
 constexpr int kProbeInlineBufs = 48;  // buffers of up to 16 probe columns
@@ -580,7 +580,7 @@ template <int TILE, int BLOCK, int SINGLE>
 __global__ __launch_bounds__(BLOCK) void join_probe_write01_kernel(ProbeParams p) {
   constexpr int RPT = TILE / BLOCK;
   constexpr int NW = BLOCK / kWave;
-  constexpr int G = 4;  // rows per thread whose gathers are in flight together
+  constexpr int G = RPT < 4 ? RPT : 4;  // rows per thread whose gathers are in flight together
   static_assert(RPT % G == 0, "groups of G rounds");
   __shared__ uint32_t s_wt[RPT][NW];
   __shared__ uint32_t s_base;
@@ -1563,8 +1563,8 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   if (single) {
     {
       KernelTimer kt(ctx, "join_probe_write");
-      hipLaunchKernelGGL((join_probe_write01_kernel<kSingleTile, kWave, 1>), dim3((unsigned)sblocks), dim3(kWave), 0,
-                         st, p);
+      hipLaunchKernelGGL((join_probe_write01_kernel<kSingleTile, kSingleTile, 1>), dim3((unsigned)sblocks),
+                         dim3(kSingleTile), 0, st, p);
     }
     GZ_HIP(hipGetLastError());
     GZ_HIP(hipStreamSynchronize(st));

@@ -20,7 +20,8 @@ namespace {
 
 constexpr int kStageAlign = 256;
 constexpr int kCopyJobs = 96;                   // buffers per copy launch
-constexpr int64_t kArenaMin = 4ll << 20;        // first arena: 32 batches, at least 4 MiB
+constexpr int64_t kArenaBatches = 256;          // the first arena holds this many batches like the first
+constexpr int64_t kArenaMin = 4ll << 20;        // at least 4 MiB
 constexpr int64_t kArenaMax = 128ll << 20;      // largest arena
 
 struct CopyJob {
@@ -94,8 +95,8 @@ int stage_add(gz_context* ctx, BatchStage* s, const gz_batch& b, bool* staged) {
   *staged = false;
   const int64_t need = stage_bytes(b);
   if (s->cap == 0 || (s->empty() && need * 4 > s->cap && s->cap < kArenaMax)) {
-    // sized for about 32 batches like this one (an empty arena grows for a larger batch)
-    const int64_t cap = std::min(kArenaMax, std::max(kArenaMin, align_up(need * 32)));
+    // sized for kArenaBatches batches like this one (an empty arena grows for a larger batch)
+    const int64_t cap = std::min(kArenaMax, std::max(kArenaMin, align_up(need * kArenaBatches)));
     if (need * 4 > cap) return GZ_OK;  // a large batch: the caller runs it directly
     ctx_free(ctx, &s->arena);  // stream-ordered: a later owner of the block runs after the queued copies
     s->cap = 0;
