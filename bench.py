@@ -625,6 +625,7 @@ def bench_join(args, ctx, stream):
             pms = (time.perf_counter() - tp) * 1e3
             out[jt]["plain"] = {"batches": nbp, "ms": pms,
                                 "probe_rows_per_s": nbp * probe.batches[0].num_rows / (pms / 1e3)}
+            out[jt]["plain_threads"] = bench_probe_threads(rel, probe, result, nbp)
         pk.close()
     out["join_aggregate"] = bench_join_aggregate(ctx, stream, rel, probe, parr)
     out["two_key"] = bench_join_two_keys(ctx, stream, rel, build, probe)
@@ -636,6 +637,49 @@ def bench_join(args, ctx, stream):
         out["cpu_baseline"] = cpu_baseline_join(bk, pk, min(6.0, args.cpu_seconds))
         del bk, pk
     del build, probe, barr, parr
+    return out
+
+
+def bench_probe_threads(rel, probe, result, nbp, counts=(4, 8)):
+    """Task threads probing the shared relation per batch, each probe on its own
+    context (stream), as an executor's task threads call Process: the same nbp
+    batches split over the threads, host wall time."""
+    import threading
+
+    import torch
+
+    from gazelle_plugin_amd import types as T
+    from gazelle_plugin_amd.context import Context
+    from gazelle_plugin_amd.join import ConditionedProbeKernel
+
+    out = {}
+    for nt in counts:
+        ctxs = [Context(0) for _ in range(nt)]
+        pks = [ConditionedProbeKernel.Make(rel, [T.INT64], 0, "inner", False, result, ctx=c) for c in ctxs]
+        for t in range(nt):
+            for b in probe.batches[t:64:nt]:
+                pks[t].Process(b)
+            ctxs[t].synchronize()
+
+        def task(t):
+            for b in probe.batches[t:nbp:nt]:
+                pks[t].Process(b)
+
+        threads = [threading.Thread(target=task, args=(t,)) for t in range(nt)]
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        for c in ctxs:
+            c.synchronize()
+        pms = (time.perf_counter() - tp) * 1e3
+        out[str(nt)] = {"batches": nbp, "ms": pms, "probe_rows_per_s": nbp * probe.batches[0].num_rows / (pms / 1e3)}
+        for pk in pks:
+            pk.close()
+        for c in ctxs:
+            c.close()
     return out
 
 

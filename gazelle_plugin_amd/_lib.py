@@ -202,6 +202,7 @@ _OPTIONAL_SIGS = {
     "gz_probe_process": (ctypes.c_int, [_P, _I64, _PI64, _PI64, _I32, _PI64]),
     "gz_probe_process_batches": (ctypes.c_int, [_P, _I32, ctypes.POINTER(GzBatch), _PI64, _PI64]),
     "gz_probe_result_column": (ctypes.c_int, [_P, _I32, _PP, _PP, _PP, _PP]),
+    "gz_probe_set_context": (ctypes.c_int, [_P, _P]),
     "gz_probe_release": (ctypes.c_int, [_P]),
 }
 

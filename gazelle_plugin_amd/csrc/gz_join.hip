@@ -1320,6 +1320,7 @@ struct gz_probe {
   uint32_t epoch = 0;
   uint32_t* host_out = nullptr;
   uint32_t* host_out_dev = nullptr;
+  bool used = false;  // a process call ran (its buffers came from ctx)
   // row-major payload of the fixed-width build output columns (built at the
   // first probe call when they fit a 16-byte record)
   DBuf payload;
@@ -1341,6 +1342,7 @@ struct gz_probe {
 // Probe batches of a multi-key probe with their code column appended.
 static int probe_codes(gz_probe* h, int32_t nb, const gz_batch* batches, std::vector<int64_t>& addrs,
                        std::vector<int64_t>& sizes, std::vector<gz_batch>& out) {
+  h->used = true;
   const int64_t* codes = nullptr;
   int64_t rows = 0;
   GZ_TRY(kd_encode(h->kd, h->ctx, &h->ksc, h->ocols, h->otypes.data(), h->kcols.data(), nb, batches, 2, &codes,
@@ -1356,6 +1358,7 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   gz_context* ctx = h->ctx;
   hipStream_t st = ctx->stream;
   GZ_HIP(hipSetDevice(ctx->device));
+  h->used = true;
   const int TILE = h->tile_rows;
   const int nc = h->nprobe;
   std::vector<PTile> tiles;
@@ -1751,6 +1754,16 @@ int gz_probe_process_batches(gz_probe* h, int32_t num_batches, const gz_batch* b
     return probe_run(h, num_batches, kb.data(), out_rows, out_batch_rows);
   }
   return probe_run(h, num_batches, batches, out_rows, out_batch_rows);
+}
+
+int gz_probe_set_context(gz_probe* h, gz_context* ctx) {
+  if (!h || !ctx) return fail(GZ_INVALID, "null argument");
+  if (h->used) return fail(GZ_INVALID, "a probe's context is set before its first process call");
+  if (ctx->device != h->rel->ctx->device) return fail(GZ_INVALID, "the probe's context is on another device");
+  GZ_HIP(hipSetDevice(ctx->device));
+  GZ_HIP(hipStreamSynchronize(h->rel->ctx->stream));
+  h->ctx = ctx;
+  return GZ_OK;
 }
 
 int gz_probe_result_column(gz_probe* h, int32_t col, void** values, void** valid_bytes, void** offsets,

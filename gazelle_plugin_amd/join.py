@@ -103,8 +103,9 @@ class HashRelationKernel:
 
 class ConditionedProbeKernel:
     def __init__(self, relation: HashRelation, probe_types: Sequence[int], probe_key: int, join_type: str,
-                 null_aware: bool, result: Sequence[Tuple[int, int]]):
+                 null_aware: bool, result: Sequence[Tuple[int, int]], ctx: Optional[Context] = None):
         self.rel = relation
+        self.ctx = ctx or relation.ctx
         self.probe_types = list(probe_types)
         self.result = [tuple(r) for r in result]
         self.result_types = [relation.types[i] if s == 0 else self.probe_types[i] for s, i in self.result]
@@ -120,20 +121,24 @@ class ConditionedProbeKernel:
             L.check(L.lib().gz_probe_make(relation.handle, len(probe_types), L.i32_array(probe_types), probe_key,
                                           JOIN_TYPES[join_type], int(null_aware), len(self.result),
                                           L.i32_array(sides), L.i32_array(idx), ctypes.byref(self._h)))
+        if self.ctx is not relation.ctx:
+            L.check(L.lib().gz_probe_set_context(self._h, self.ctx.handle))
 
     @classmethod
-    def Make(cls, relation, probe_types, probe_key, join_type="inner", null_aware=False, result=()):
-        return cls(relation, probe_types, probe_key, join_type, null_aware, result)
+    def Make(cls, relation, probe_types, probe_key, join_type="inner", null_aware=False, result=(), ctx=None):
+        """ctx: the probe's own context (stream and memory), for task threads that
+        probe one shared relation at once; the relation's by default."""
+        return cls(relation, probe_types, probe_key, join_type, null_aware, result, ctx)
 
     def Process(self, batch: DeviceBatch) -> int:
-        self.rel.ctx.hold(batch)
+        self.ctx.hold(batch)
         n = ctypes.c_int64()
         L.check(L.lib().gz_probe_process(self._h, batch.num_rows, batch.addrs, batch.sizes, batch.num_bufs,
                                          ctypes.byref(n)))
         return n.value
 
     def ProcessBatches(self, batches: Sequence[DeviceBatch], arr=None) -> Tuple[int, np.ndarray]:
-        self.rel.ctx.hold(list(batches))
+        self.ctx.hold(list(batches))
         if arr is None:
             arr = make_batch_array(batches)
         n = ctypes.c_int64()
@@ -146,7 +151,7 @@ class ConditionedProbeKernel:
         """Host copy of the last output: per column (type, values, valid[, offsets, data]).
         The kernels run on the context's non-blocking stream, so the stream is
         drained before the null-stream copies read their output."""
-        self.rel.ctx.synchronize()
+        self.ctx.synchronize()
         out = []
         for c, t in enumerate(self.result_types):
             vals, valid, offs, data = (ctypes.c_void_p() for _ in range(4))
@@ -166,7 +171,7 @@ class ConditionedProbeKernel:
 
     def close(self):
         if self._h:
-            if self.rel.ctx.alive:
+            if self.ctx.alive and self.rel.ctx.alive:
                 L.lib().gz_probe_release(self._h)
             self._h = ctypes.c_void_p()
 

@@ -458,6 +458,14 @@ int gz_probe_process_batches(gz_probe* p, int32_t num_batches, const gz_batch* b
                              int64_t* out_rows, int64_t* out_batch_rows);
 int gz_probe_result_column(gz_probe* p, int32_t col, void** values, void** valid_bytes,
                            void** offsets, void** data);
+/* The probe's stream and device memory: the relation's context by default.
+ * SetDependencies hands one HashRelation to the probe of every task that joins
+ * against it (a broadcast relation, conditioned_probe_kernel.cc:380-465); task
+ * threads probing it at once give each probe its own context, so their calls
+ * run on separate streams.  Call before the probe's first process call; it
+ * waits for the relation's stream, so the finished relation is visible to the
+ * new one.  The context must be on the relation's device and outlive the probe. */
+int gz_probe_set_context(gz_probe* p, gz_context* ctx);
 
 /* Several join key columns (HashRelationKernel / ConditionedProbe with
  * UnsafeRow keys: hash_relation_kernel.cc:264-277, hash_relation.h:176-198,

@@ -282,3 +282,67 @@ def test_one_batch_calls_match_the_oracle(ctx, join_type, dup_keys, null_aware):
     assert len(got) == len(exp)
     for g, e in zip(got, exp):
         check_output(g, e)
+
+
+def test_task_threads_probe_one_relation_on_their_own_contexts(ctx):
+    """A relation shared by several task threads (a broadcast relation): each
+    thread's probe runs on its own context, per batch, at the same time as the
+    others; every thread's output equals the oracle's for its batches."""
+    import threading
+
+    from gazelle_plugin_amd.context import Context
+
+    rng = np.random.default_rng(11)
+    build, probe = _workload(rng, 8000, 4 * 8 * 4096, False)
+    probe = [HostBatch(b.num_rows, [b.columns[0]]) for b in probe]  # the key only: fixed-width outputs
+    btypes = [T.INT64, T.INT32, T.DOUBLE]
+    rel = HashRelationKernel.Make(ctx, btypes, 0)
+    rel.EvaluateBatches([DeviceBatch.from_host(b) for b in build])
+    rel.Finish()
+    result = [[1, 0], [0, 1], [0, 2]]
+    exp = expected_cols(build, probe, 0, 0, T.INT64, "inner", False, result)
+    nthreads = 4
+    ctxs = [Context(0) for _ in range(nthreads)]
+    pks = [ConditionedProbeKernel.Make(rel, [T.INT64], 0, "inner", False, result, ctx=c) for c in ctxs]
+    dev = [DeviceBatch.from_host(b) for b in probe]
+    got = [None] * len(probe)
+    errors = []
+
+    def task(t):
+        try:
+            for i in range(t, len(probe), nthreads):
+                n = pks[t].Process(dev[i])
+                got[i] = pks[t].result_columns(n)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=task, args=(t,)) for t in range(nthreads)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    for g, e in zip(got, exp):
+        check_output(g, e)
+    for pk in pks:
+        pk.close()
+    for c in ctxs:
+        c.close()
+    rel.close()
+
+
+def test_probe_context_is_set_before_the_first_call(ctx):
+    from gazelle_plugin_amd import _lib as L
+    from gazelle_plugin_amd.context import Context
+
+    b = HostBatch(3, [HostColumn(T.INT64, np.array([1, 2, 3], np.int64))])
+    rel = HashRelationKernel.Make(ctx, [T.INT64], 0)
+    rel.Evaluate(DeviceBatch.from_host(b))
+    rel.Finish()
+    pk = ConditionedProbeKernel.Make(rel, [T.INT64], 0, "semi", False, [[1, 0]])
+    assert pk.Process(DeviceBatch.from_host(b)) == 3
+    other = Context(0)
+    assert L.lib().gz_probe_set_context(pk._h, other.handle) == 4  # Invalid
+    pk.close()
+    other.close()
+    rel.close()
