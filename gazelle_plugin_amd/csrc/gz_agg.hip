@@ -1557,12 +1557,26 @@ __global__ __launch_bounds__(256) void agg_collect_kernel(const AggInst* inst, i
 }
 
 // Groups per instance from the chunk scan.
-__global__ void agg_inst_counts_kernel(const unsigned long long* cc, const unsigned long long* ccpre, int ninst,
-                                       int64_t X, unsigned long long* counts) {
+// Occupied slots per instance, and at counts[ninst] the number of instances
+// whose table overflowed.
+__global__ void agg_inst_counts_kernel(const AggInst* inst, const unsigned long long* cc,
+                                       const unsigned long long* ccpre, int ninst, int64_t X,
+                                       unsigned long long* counts) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ninst) return;
   const int64_t last = (int64_t)i * X + X - 1;
   counts[i] = ccpre[last] + cc[last] - ccpre[(int64_t)i * X];
+  if (inst[i].overflow && *inst[i].overflow) atomicAdd(counts + ninst, 1ull);
+}
+
+// Keys inserted and overflow flag of every instance, gathered for one copy:
+// out[2 * i] = keys inserted, out[2 * i + 1] = overflow.
+__global__ void agg_read_counters_kernel(const AggInst* inst, int ninst, unsigned int* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst) return;
+  const AggInst I = inst[i];  // an instance without a table yet reads as empty
+  out[2 * i] = I.counter ? *I.counter : 0u;
+  out[2 * i + 1] = I.overflow ? (unsigned)*I.overflow : 0u;
 }
 
 struct ResultSpec {
@@ -1745,7 +1759,7 @@ struct gz_agg {
   std::vector<int64_t> rows_seen;  // per instance
   std::vector<int64_t> g_bound;    // upper bound of groups per instance
   std::vector<int> placement;      // 1 = LDS first level, 2 = HBM only
-  DBuf d_inst, d_lane_ops, d_tiles, d_bufs, d_regs, d_est;
+  DBuf d_inst, d_lane_ops, d_tiles, d_bufs, d_regs, d_est, d_blob, d_cnt;
   // finish
   bool finished = false;
   std::vector<int64_t> ngroups, group_off;
@@ -2028,16 +2042,25 @@ static int inst_alloc(gz_agg* h, int i, uint64_t cap) {
   return GZ_OK;
 }
 
+// Groups bound (keys inserted + the two special slots) and overflow of every
+// instance: one gather kernel, one copy, one synchronization.
 static int read_counts(gz_agg* h, std::vector<int64_t>& groups, int* overflow) {
+  gz_context* ctx = h->ctx;
   *overflow = 0;
   groups.assign(h->ninst, 0);
-  std::vector<unsigned int> buf(h->ninst * 4);
-  for (int i = 0; i < h->ninst; i++)
-    GZ_HIP(hipMemcpyAsync(&buf[i * 4], h->counters[i].p, 8, hipMemcpyDeviceToHost, h->ctx->stream));
-  GZ_HIP(hipStreamSynchronize(h->ctx->stream));
+  if (h->inst_dirty) {
+    GZ_TRY(upload(ctx, &h->d_inst, h->inst.data(), sizeof(AggInst) * h->inst.size()));
+    h->inst_dirty = false;
+  }
+  GZ_TRY(ctx_ensure(ctx, &h->d_cnt, sizeof(unsigned) * 2 * h->ninst));
+  hipLaunchKernelGGL(agg_read_counters_kernel, dim3((unsigned)((h->ninst + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (const AggInst*)h->d_inst.p, h->ninst, (unsigned*)h->d_cnt.p);
+  std::vector<unsigned int> buf(2 * h->ninst);
+  GZ_HIP(hipMemcpyAsync(buf.data(), h->d_cnt.p, sizeof(unsigned) * 2 * h->ninst, hipMemcpyDeviceToHost, ctx->stream));
+  GZ_HIP(hipStreamSynchronize(ctx->stream));
   for (int i = 0; i < h->ninst; i++) {
-    groups[i] = buf[i * 4] + 2;  // + the two special slots
-    if (buf[i * 4 + 2]) *overflow = 1;
+    groups[i] = buf[2 * i] + 2;
+    if (buf[2 * i + 1]) *overflow = 1;
   }
   return GZ_OK;
 }
@@ -2098,7 +2121,7 @@ static int agg_set_make(gz_context* ctx, int32_t ncols, const int32_t* types, in
   h->counters.resize(ninst);
   h->rows_seen.assign(ninst, 0);
   h->g_bound.assign(ninst, 2);
-  h->placement.assign(ninst, h->opts.placement == 1 ? 1 : 2);
+  h->placement.assign(ninst, h->opts.placement >= 1 && h->opts.placement <= 3 ? h->opts.placement : 2);
   *out = h;
   return GZ_OK;
 }
@@ -2106,7 +2129,7 @@ static int agg_set_make(gz_context* ctx, int32_t ncols, const int32_t* types, in
 // Tile table + buffer table of a call.
 static int agg_stage(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of,
                      std::vector<AggTile>& tiles, std::vector<int64_t>& rows_per_inst, int tile_rows,
-                     DBuf* bufs_dst = nullptr, std::vector<GBatch>* gbatches = nullptr) {
+                     DBuf* bufs_dst = nullptr, std::vector<GBatch>* gbatches = nullptr, UploadBatch* ub = nullptr) {
   const int nc = h->plan.ncols;
   std::vector<const void*> bufs((size_t)nb * nc * 3, nullptr);
   rows_per_inst.assign(h->ninst, 0);
@@ -2149,8 +2172,10 @@ static int agg_stage(gz_agg* h, int32_t nb, const gz_batch* batches, const int32
     base[ii] += bt.num_rows;
     rows_per_inst[ii] += bt.num_rows;
   }
-  GZ_TRY(upload(h->ctx, bufs_dst ? bufs_dst : &h->d_bufs, bufs.data(), sizeof(void*) * std::max<size_t>(bufs.size(), 1)));
-  return GZ_OK;
+  DBuf* dst = bufs_dst ? bufs_dst : &h->d_bufs;
+  const size_t nbytes = sizeof(void*) * std::max<size_t>(bufs.size(), 1);
+  if (ub) return ub->add(h->ctx, dst, bufs.data(), nbytes);
+  return upload(h->ctx, dst, bufs.data(), nbytes);
 }
 
 static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of);
@@ -2335,14 +2360,6 @@ static int agg_finish(gz_agg* h) {
   std::fill(h->bits_ready.begin(), h->bits_ready.end(), 0);
   gz_context* ctx = h->ctx;
   GZ_HIP(hipSetDevice(ctx->device));
-  if (h->L.ready) {
-    // a planned run sized its tables from estimates: verify none overflowed
-    std::vector<int64_t> g;
-    int ovf = 0;
-    GZ_TRY(read_counts(h, g, &ovf));
-    if (ovf) return fail(GZ_CAPACITY_ERROR, "aggregate hash table overflowed (cardinality estimate); "
-                                            "prepare again with expected_groups");
-  }
   // instances never fed get an empty table so the collect kernel can run
   for (int i = 0; i < h->ninst; i++)
     if (!h->tables[i].p) GZ_TRY(inst_alloc(h, i, 16));
@@ -2365,7 +2382,7 @@ static int agg_finish(gz_agg* h) {
   const int64_t NC = X * h->ninst;
   GZ_TRY(ctx_ensure(ctx, &h->cursor, sizeof(uint64_t) * NC));  // chunk counts
   GZ_TRY(ctx_ensure(ctx, &h->ccpre, sizeof(uint64_t) * NC));   // their exclusive scan
-  GZ_TRY(ctx_ensure(ctx, &h->counts, sizeof(uint64_t) * h->ninst));
+  GZ_TRY(ctx_ensure(ctx, &h->counts, sizeof(uint64_t) * (h->ninst + 1)));
   {
     KernelTimer kt(ctx, "agg_collect");
     hipLaunchKernelGGL(agg_occupancy_kernel, dim3((unsigned)X, (unsigned)h->ninst), dim3(256), 0, ctx->stream,
@@ -2381,13 +2398,20 @@ static int agg_finish(gz_agg* h) {
     hipLaunchKernelGGL(agg_collect_kernel, dim3((unsigned)X, (unsigned)h->ninst), dim3(256), 0, ctx->stream,
                        (const AggInst*)h->d_inst.p, X, (const unsigned long long*)h->ccpre.p,
                        (unsigned long long*)h->tkeys.p, (unsigned long long*)h->tvals.p);
+    hipLaunchKernelGGL(fill_u32_kernel_agg, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)h->counts.p + 2 * h->ninst,
+                       (int64_t)2);
     hipLaunchKernelGGL(agg_inst_counts_kernel, dim3((unsigned)((h->ninst + 255) / 256)), dim3(256), 0, ctx->stream,
-                       (const unsigned long long*)h->cursor.p, (const unsigned long long*)h->ccpre.p, h->ninst, X,
-                       (unsigned long long*)h->counts.p);
+                       (const AggInst*)h->d_inst.p, (const unsigned long long*)h->cursor.p,
+                       (const unsigned long long*)h->ccpre.p, h->ninst, X, (unsigned long long*)h->counts.p);
   }
   std::vector<unsigned long long> cnt(h->ninst + 1, 0);
-  GZ_HIP(hipMemcpyAsync(cnt.data(), h->counts.p, sizeof(uint64_t) * h->ninst, hipMemcpyDeviceToHost, ctx->stream));
+  GZ_HIP(hipMemcpyAsync(cnt.data(), h->counts.p, sizeof(uint64_t) * (h->ninst + 1), hipMemcpyDeviceToHost,
+                        ctx->stream));
   GZ_HIP(hipStreamSynchronize(ctx->stream));
+  // tables sized from an estimate or a hint may have overflowed: rows were lost
+  if (cnt[h->ninst]) return fail(GZ_CAPACITY_ERROR, "aggregate hash table overflowed (cardinality estimate); "
+                                                    "prepare again with expected_groups");
+  cnt[h->ninst] = 0;
   for (int i = 0; i < h->ninst; i++) cnt[h->ninst] += cnt[i];
   const int64_t n = (int64_t)cnt[h->ninst];
   h->ngroups.assign(h->ninst, 0);
@@ -2454,6 +2478,25 @@ static void drop_graph(gz_agg* h) {
   h->L.graph = nullptr;
 }
 
+// A plan's table for instance i: no initialization and no rehash (every run of
+// the plan resets its tables first).
+static int plan_table_alloc(gz_agg* h, int i, uint64_t cap) {
+  gz_context* ctx = h->ctx;
+  const int nl = (int)h->plan.lanes.size();
+  ctx_free(ctx, &h->tables[i]);
+  ctx_free(ctx, &h->counters[i]);
+  GZ_TRY(ctx_alloc(ctx, sizeof(int64_t) * (cap + 2) * (2 + nl), &h->tables[i]));
+  GZ_TRY(ctx_alloc(ctx, 16, &h->counters[i]));
+  AggInst ni{};
+  ni.base = (int64_t*)h->tables[i].p;
+  ni.cap = cap;
+  ni.counter = (unsigned int*)h->counters[i].p;
+  ni.overflow = (int*)((char*)h->counters[i].p + 8);
+  h->inst[i] = ni;
+  h->inst_dirty = true;
+  return GZ_OK;
+}
+
 static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of) {
   gz_context* ctx = h->ctx;
   GZ_HIP(hipSetDevice(ctx->device));
@@ -2465,7 +2508,10 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   // a prepared run aggregates its batches from scratch: instance rows start at 0
   std::fill(h->rows_seen.begin(), h->rows_seen.end(), 0);
   std::vector<GBatch> gb;
-  GZ_TRY(agg_stage(h, nb, batches, inst_of, tiles, rows, tile_rows, &h->L.d_bufs, &gb));
+  // the plan's host arrays go up in two transfers: the batch table (and the
+  // tiles the estimate reads) before the estimate, everything else after it
+  UploadBatch ub;
+  GZ_TRY(agg_stage(h, nb, batches, inst_of, tiles, rows, tile_rows, &h->L.d_bufs, &gb, &ub));
   AggParams p = base_params(h);
   p.bufs = (const void* const*)h->L.d_bufs.p;  // the plan's own batch table
   p.bstride = h->plan.ncols * 3;
@@ -2476,7 +2522,8 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   for (int i = 0; i < h->ninst; i++)
     if (rows[i] > 4096 && h->opts.expected_groups <= 0) need_est = true;
   if (need_est && !tiles.empty()) {
-    GZ_TRY(upload(ctx, &h->d_tiles, tiles.data(), sizeof(AggTile) * tiles.size()));
+    GZ_TRY(ub.add(ctx, &h->d_tiles, tiles.data(), sizeof(AggTile) * tiles.size()));
+    GZ_TRY(ub.flush(ctx, &h->d_blob));
     p.tiles = (const AggTile*)h->d_tiles.p;
     p.T = (int64_t)tiles.size();
     GZ_TRY(ctx_ensure(ctx, &h->d_regs, sizeof(unsigned) * kHllRegs * h->ninst));
@@ -2503,20 +2550,14 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     else if (rows[i] > 4096) g = std::min<double>(est[i] * 1.3 + 64, (double)rows[i]);
     else g = (double)rows[i];
     uint64_t cap = h->plan.key_col < 0 ? 16 : pow2_at_least((uint64_t)(g / lf) + 16);
-    if (!h->tables[i].p || h->inst[i].cap < cap || h->inst[i].cap > 4 * cap) {
-      if (h->tables[i].p) {
-        ctx_free(ctx, &h->tables[i]);
-        ctx_free(ctx, &h->counters[i]);
-      }
-      GZ_TRY(inst_alloc(h, i, cap));
-    }
+    if (!h->tables[i].p || h->inst[i].cap < cap || h->inst[i].cap > 4 * cap) GZ_TRY(plan_table_alloc(h, i, cap));
     h->g_bound[i] = 2 + (int64_t)g;
     const bool part_ok = p.nu >= 0 && h->plan.key_col >= 0 && h->part_groups > 0 &&
                          g <= (double)kMaxFanout * h->part_groups;
     if (h->opts.placement == 0)
       h->placement[i] = (g <= (double)h->lds_threshold || h->plan.key_col < 0) ? 1 : part_ok ? 3 : 2;
-    else if (h->opts.placement == 3 && !part_ok)
-      h->placement[i] = 2;
+    else
+      h->placement[i] = h->opts.placement == 3 && !part_ok ? 2 : h->opts.placement;
     h->inst[i].fan_log2 = 0;
     h->inst[i].bucket_base = 0;
     if (h->placement[i] == 3) {
@@ -2536,13 +2577,13 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     B += 1 << h->inst[i].fan_log2;
   }
   h->B = B;
-  GZ_TRY(upload(ctx, &h->d_inst, h->inst.data(), sizeof(AggInst) * h->inst.size()));
+  GZ_TRY(ub.add(ctx, &h->d_inst, h->inst.data(), sizeof(AggInst) * h->inst.size()));
   h->inst_dirty = false;
-  GZ_TRY(upload(ctx, &h->L.d_slot_prefix, prefix.data(), sizeof(int64_t) * prefix.size()));
+  GZ_TRY(ub.add(ctx, &h->L.d_slot_prefix, prefix.data(), sizeof(int64_t) * prefix.size()));
   h->L.total_slots = prefix[h->ninst];
   std::vector<int> ops(std::max<size_t>(h->plan.lanes.size(), 1));
   for (size_t l = 0; l < h->plan.lanes.size(); l++) ops[l] = h->plan.lanes[l].op;
-  GZ_TRY(upload(ctx, &h->d_lane_ops, ops.data(), sizeof(int) * ops.size()));
+  GZ_TRY(ub.add(ctx, &h->d_lane_ops, ops.data(), sizeof(int) * ops.size()));
   p.inst = (const AggInst*)h->d_inst.p;
   // launch plan
   std::vector<AggTile> lds_tiles, hbm_tiles, part_tiles;
@@ -2573,15 +2614,15 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
       if (h->placement[i] == 3) h->R = std::max(h->R, 1 << h->inst[i].fan_log2);
     const int64_t TP = (int64_t)part_tiles.size();
     const int64_t C = (TP + kPartScanChunk - 1) / kPartScanChunk;
-    GZ_TRY(upload(ctx, &h->it0, t0.data(), sizeof(int) * t0.size()));
-    GZ_TRY(upload(ctx, &h->it1, t1.data(), sizeof(int) * t1.size()));
+    GZ_TRY(ub.add(ctx, &h->it0, t0.data(), sizeof(int) * t0.size()));
+    GZ_TRY(ub.add(ctx, &h->it1, t1.data(), sizeof(int) * t1.size()));
     GZ_TRY(ctx_ensure(ctx, &h->tcount, sizeof(uint32_t) * TP * h->R));
     GZ_TRY(ctx_ensure(ctx, &h->tprefix, sizeof(uint32_t) * TP * h->R));
     GZ_TRY(ctx_ensure(ctx, &h->tcsum, sizeof(uint64_t) * C * h->R));
     GZ_TRY(ctx_ensure(ctx, &h->ttot, sizeof(uint64_t) * h->R));
     for (auto& t : part_tiles) h->nrec += t.nrows;
-    GZ_TRY(upload(ctx, &h->d_part, part_tiles.data(), sizeof(AggTile) * part_tiles.size()));
-    GZ_TRY(upload(ctx, &h->pinst, pinst.data(), sizeof(int) * pinst.size()));
+    GZ_TRY(ub.add(ctx, &h->d_part, part_tiles.data(), sizeof(AggTile) * part_tiles.size()));
+    GZ_TRY(ub.add(ctx, &h->pinst, pinst.data(), sizeof(int) * pinst.size()));
     GZ_TRY(ctx_ensure(ctx, &h->pcount, sizeof(unsigned) * B));
     GZ_TRY(ctx_ensure(ctx, &h->pstart, sizeof(int64_t) * (B + 1)));
     GZ_TRY(ctx_ensure(ctx, &h->recs, sizeof(int64_t) * (2 + p.nu) * h->nrec));
@@ -2608,14 +2649,14 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   h->L.blocks_hbm = 0;
   h->L.blocks_lds = 0;
   if (!hbm_tiles.empty()) {
-    GZ_TRY(upload(ctx, &h->L.d_hbm, hbm_tiles.data(), sizeof(AggTile) * hbm_tiles.size()));
+    GZ_TRY(ub.add(ctx, &h->L.d_hbm, hbm_tiles.data(), sizeof(AggTile) * hbm_tiles.size()));
     h->L.p_hbm.tiles = (const AggTile*)h->L.d_hbm.p;
     h->L.p_hbm.T = (int64_t)hbm_tiles.size();
     if (h->L.p_hbm.T >= 8ll * ctx->num_cus) hbm_first_level(h, &h->L.p_hbm, &h->L.blocks_hbm, &h->L.lds_hbm);
     else h->L.grid_hbm = (unsigned)((h->L.p_hbm.T + 7) / 8 * 8);
   }
   if (!lds_tiles.empty()) {
-    GZ_TRY(upload(ctx, &h->L.d_lds, lds_tiles.data(), sizeof(AggTile) * lds_tiles.size()));
+    GZ_TRY(ub.add(ctx, &h->L.d_lds, lds_tiles.data(), sizeof(AggTile) * lds_tiles.size()));
     AggParams& q = h->L.p_lds;
     q.tiles = (const AggTile*)h->L.d_lds.p;
     q.T = (int64_t)lds_tiles.size();
@@ -2633,15 +2674,16 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   h->L.T_fix = 0;
   h->L.n_gb = 0;
   if (h->need_fix && !tiles.empty()) {
-    GZ_TRY(upload(ctx, &h->L.d_fix, tiles.data(), sizeof(AggTile) * tiles.size()));
+    GZ_TRY(ub.add(ctx, &h->L.d_fix, tiles.data(), sizeof(AggTile) * tiles.size()));
     h->L.p_fix = p;
     h->L.p_fix.tiles = (const AggTile*)h->L.d_fix.p;
     h->L.p_fix.T = h->L.T_fix = (int64_t)tiles.size();
   }
   if (h->need_allnull && !gb.empty()) {
-    GZ_TRY(upload(ctx, &h->L.d_gb, gb.data(), sizeof(GBatch) * gb.size()));
+    GZ_TRY(ub.add(ctx, &h->L.d_gb, gb.data(), sizeof(GBatch) * gb.size()));
     h->L.n_gb = (int)gb.size();
   }
+  GZ_TRY(ub.flush(ctx, &h->d_blob));
   h->L.rows = rows;
   h->L.ready = true;
   GZ_HIP(hipGetLastError());
@@ -3114,7 +3156,7 @@ int gz_agg_release(gz_agg* h) {
                   &h->svals, &h->tkeys, &h->tvals, &h->cursor, &h->ccpre, &h->counts, &h->sort_tmp, &h->L.d_hbm,
                   &h->L.d_lds, &h->L.d_slot_prefix, &h->L.d_bufs, &h->d_part, &h->pcount, &h->pstart, &h->pinst,
                   &h->recs, &h->tcount, &h->tprefix, &h->tcsum, &h->ttot, &h->it0, &h->it1, &h->d_fix,
-                  &h->d_fix_tiles, &h->d_gbatch, &h->L.d_fix, &h->L.d_gb})
+                  &h->d_fix_tiles, &h->d_gbatch, &h->L.d_fix, &h->L.d_gb, &h->d_blob, &h->d_cnt})
     ctx_free(ctx, b);
   for (auto& b : h->res_bits) ctx_free(ctx, &b);
   if (h->L.exec) hipGraphExecDestroy(h->L.exec);

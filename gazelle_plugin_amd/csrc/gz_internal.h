@@ -144,6 +144,25 @@ void ctx_resolve_timings(gz_context* ctx, bool wait);
 // buffer: it may be reused as soon as this returns).
 int upload(gz_context* ctx, DBuf* dst, const void* src, size_t bytes);
 
+// Several uploads as one: the host arrays are packed into one blob, copied to
+// the device in one transfer and scattered to their buffers by one kernel, all
+// on the context's stream.  add() sizes the destination at once (its address is
+// final when add returns) and copies the source; the data lands at flush().
+// A destination is added at most once per batch.
+class UploadBatch {
+ public:
+  int add(gz_context* ctx, DBuf* dst, const void* src, size_t bytes);
+  int flush(gz_context* ctx, DBuf* blob);  // blob: device scratch the batch is staged in
+
+ private:
+  struct Item {
+    uint8_t* dst;
+    size_t off, bytes;
+  };
+  std::vector<uint8_t> host_;
+  std::vector<Item> items_;
+};
+
 // Device view of a finished relation (the join's build side) for kernels
 // outside the join: its hash table and its row-indexed column stores.
 struct RelationView {

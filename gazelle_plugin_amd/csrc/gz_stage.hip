@@ -54,7 +54,53 @@ __global__ __launch_bounds__(256) void stage_copy_kernel(CopyList L) {
 
 int64_t align_up(int64_t v) { return (v + kStageAlign - 1) / kStageAlign * kStageAlign; }
 
+// Segment s of an upload blob: bytes [off, off + n) of the blob go to dst.
+struct BlobSeg {
+  uint8_t* dst;
+  int64_t off, n;
+};
+
+// One workgroup per segment (the segment table sits at blob + table), 16 bytes
+// per thread and step: segment offsets and destinations are 16-byte aligned
+// (kStageAlign and the pool's allocations), and sizes are rounded up to 16
+// within the destination's capacity and the blob.
+__global__ __launch_bounds__(256) void blob_scatter_kernel(const uint8_t* blob, int64_t table, int nseg) {
+  if ((int)blockIdx.x >= nseg) return;
+  const BlobSeg S = ((const BlobSeg*)(blob + table))[blockIdx.x];
+  const uint4* src = (const uint4*)(blob + S.off);
+  uint4* dst = (uint4*)S.dst;
+  for (int64_t i = threadIdx.x; i < S.n >> 4; i += blockDim.x) dst[i] = src[i];
+}
+
 }  // namespace
+
+int UploadBatch::add(gz_context* ctx, DBuf* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return GZ_OK;
+  // destinations hold whole 16-byte words, so the copy may round up
+  GZ_TRY(ctx_ensure(ctx, dst, (bytes + 15) / 16 * 16));
+  const size_t off = (size_t)align_up((int64_t)host_.size());
+  host_.resize(off + bytes);
+  memcpy(host_.data() + off, src, bytes);
+  items_.push_back(Item{(uint8_t*)dst->p, off, bytes});
+  return GZ_OK;
+}
+
+int UploadBatch::flush(gz_context* ctx, DBuf* blob) {
+  if (items_.empty()) return GZ_OK;
+  // blob: [data | segment table]; the last segment's round-up stays inside the blob
+  const size_t table = (size_t)align_up((int64_t)host_.size() + 16);
+  host_.resize(table + sizeof(BlobSeg) * items_.size());
+  BlobSeg* segs = (BlobSeg*)(host_.data() + table);
+  for (size_t k = 0; k < items_.size(); k++)
+    segs[k] = BlobSeg{items_[k].dst, (int64_t)items_[k].off, (int64_t)((items_[k].bytes + 15) / 16 * 16)};
+  GZ_TRY(upload(ctx, blob, host_.data(), host_.size()));
+  hipLaunchKernelGGL(blob_scatter_kernel, dim3((unsigned)items_.size()), dim3(256), 0, ctx->stream,
+                     (const uint8_t*)blob->p, (int64_t)table, (int)items_.size());
+  GZ_HIP(hipGetLastError());
+  host_.clear();
+  items_.clear();
+  return GZ_OK;
+}
 
 int stage_check(int32_t ncols, const int32_t* types, const gz_batch& b) {
   if (b.num_rows < 0 || b.num_rows > INT32_MAX) return fail(GZ_INVALID, "bad num_rows");

@@ -204,3 +204,19 @@ def test_high_cardinality_requested_partitioned(ctx, batched):
     batches = datagen.agg_host(330000, num_keys=300000, batch_rows=8192)
     got = run_gpu(ctx, batches, datagen.AGG_TYPES, CONFIG0_ACTIONS, batched, placement=3)
     assert_results_equal(got, expected(batches, datagen.AGG_TYPES, CONFIG0_ACTIONS), RTOL)
+
+
+@pytest.mark.parametrize("placement", [0, 2, 3])
+def test_table_overflow_is_reported(ctx, placement):
+    """A cardinality hint (500) far below the true 20000 groups overflows the
+    HBM table sized from it: the call or the finish reports CapacityError."""
+    from gazelle_plugin_amd._lib import GazelleError
+
+    bl = datagen.agg_host(60000, num_keys=20000, seed=25)
+    k = HashAggregateKernel.Make(ctx, datagen.AGG_TYPES, CONFIG0_ACTIONS,
+                                 AggOptions(placement=placement, expected_groups=500))
+    it = k.MakeResultIterator()
+    with pytest.raises(GazelleError, match="overflow"):
+        it.ProcessBatches([DeviceBatch.from_host(b) for b in bl])
+        it.finish()
+    it.close()
