@@ -1309,6 +1309,9 @@ __global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, cons
   }
 }
 
+#ifndef GZ_PART_GROUPS
+#define GZ_PART_GROUPS 512
+#endif
 #ifndef GZ_UPD_RPT
 #define GZ_UPD_RPT 4
 #endif
@@ -1569,6 +1572,36 @@ __global__ void agg_inst_counts_kernel(const AggInst* inst, const unsigned long 
   if (inst[i].overflow && *inst[i].overflow) atomicAdd(counts + ninst, 1ull);
 }
 
+// First-seen order for few groups: the sort keys are distinct, so a key's
+// place is the number of smaller keys.  64 keys per workgroup, each of its four
+// waves counting over a quarter of all keys (read at one address per step by
+// the whole wave).
+constexpr int64_t kRankSortMax = 16384;
+
+__global__ __launch_bounds__(256) void agg_rank_sort_kernel(const unsigned long long* __restrict__ keys,
+                                                            const unsigned long long* __restrict__ vals, int64_t n,
+                                                            unsigned long long* skeys, unsigned long long* svals) {
+  __shared__ unsigned s_c[4][64];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 64 + l;
+  const unsigned long long mine = i < n ? keys[i] : ~0ull;
+  const int64_t q0 = n * w / 4, q1 = n * (w + 1) / 4;
+  unsigned c = 0;
+  int64_t j = q0;
+  for (; j + 4 <= q1; j += 4) {
+    const unsigned long long a = keys[j], b = keys[j + 1], d = keys[j + 2], e = keys[j + 3];
+    c += (a < mine) + (b < mine) + (d < mine) + (e < mine);
+  }
+  for (; j < q1; j++) c += keys[j] < mine;
+  s_c[w][l] = c;
+  __syncthreads();
+  if (w == 0 && i < n) {
+    const unsigned r = s_c[0][l] + s_c[1][l] + s_c[2][l] + s_c[3][l];
+    skeys[r] = mine;
+    svals[r] = vals[i];
+  }
+}
+
 // Keys inserted and overflow flag of every instance, gathered for one copy:
 // out[2 * i] = keys inserted, out[2 * i + 1] = overflow.
 __global__ void agg_read_counters_kernel(const AggInst* inst, int ninst, unsigned int* out) {
@@ -1789,7 +1822,7 @@ struct gz_agg {
   DBuf d_fix, d_fix_tiles, d_gbatch;
   bool need_fix = false, need_allnull = false, has_glast = false;
   int lds_threshold = 256;  // groups up to which an instance gets the LDS first level
-  int part_groups = 512;    // groups per partition on the partitioned placement (0 = off)
+  int part_groups = GZ_PART_GROUPS;  // groups per partition on the partitioned placement (0 = off)
   // partitioned placement buffers of the plan
   DBuf d_part, pcount, pstart, pinst, recs, tcount, tprefix, tcsum, ttot, it0, it1;
   int R = 0;                // largest fan-out (columns of the tile count matrix)
@@ -2424,7 +2457,12 @@ static int agg_finish(gz_agg* h) {
   }
   h->total_groups = n;
   // sort by (instance, first row): first-seen order inside each instance
-  if (n > 0) {
+  if (n > 0 && n <= kRankSortMax) {
+    KernelTimer kt(ctx, "agg_sort");
+    hipLaunchKernelGGL(agg_rank_sort_kernel, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, ctx->stream,
+                       (const unsigned long long*)h->tkeys.p, (const unsigned long long*)h->tvals.p, n,
+                       (unsigned long long*)h->skeys.p, (unsigned long long*)h->svals.p);
+  } else if (n > 0) {
     int end_bit = 44;
     while ((1ll << (end_bit - 44)) < h->ninst) end_bit++;
     size_t tmp = 0;
@@ -2539,8 +2577,22 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     hipLaunchKernelGGL(agg_hll_estimate_kernel, dim3((unsigned)h->ninst), dim3(256), 0, ctx->stream,
                        (const unsigned int*)h->d_regs.p, (double*)h->d_est.p);
     GZ_HIP(hipMemcpyAsync(est.data(), h->d_est.p, sizeof(double) * h->ninst, hipMemcpyDeviceToHost, ctx->stream));
-    GZ_HIP(hipStreamSynchronize(ctx->stream));
   }
+  // while the estimate runs: every instance's tiles cut to kPartTile rows, the
+  // tiles of one instance contiguous (the partition scan relies on it)
+  std::vector<AggTile> split;
+  split.reserve(tiles.size() * ((tile_rows + kPartTile - 1) / kPartTile));
+  for (auto& t : tiles)
+    for (int32_t r0 = 0; r0 < t.nrows; r0 += kPartTile) {
+      AggTile u = t;
+      u.row_begin = t.row_begin + r0;
+      u.nrows = std::min<int32_t>(kPartTile, t.nrows - r0);
+      u.row_base = t.row_base + r0;
+      split.push_back(u);
+    }
+  if (!std::is_sorted(split.begin(), split.end(), [](const AggTile& a, const AggTile& b) { return a.inst < b.inst; }))
+    std::stable_sort(split.begin(), split.end(), [](const AggTile& a, const AggTile& b) { return a.inst < b.inst; });
+  if (need_est && !tiles.empty()) GZ_HIP(hipStreamSynchronize(ctx->stream));
   // tables (reused when large enough: the run resets them) and placement
   std::vector<int64_t> prefix(h->ninst + 1, 0);
   for (int i = 0; i < h->ninst; i++) {
@@ -2587,25 +2639,21 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   p.inst = (const AggInst*)h->d_inst.p;
   // launch plan
   std::vector<AggTile> lds_tiles, hbm_tiles, part_tiles;
+  bool all_part = true;
   for (auto& t : tiles) {
-    if (h->placement[t.inst] != 3) {
-      (h->placement[t.inst] == 1 ? lds_tiles : hbm_tiles).push_back(t);
-      continue;
-    }
-    for (int32_t r0 = 0; r0 < t.nrows; r0 += kPartTile) {  // partition tiles are kPartTile rows
-      AggTile u = t;
-      u.row_begin = t.row_begin + r0;
-      u.nrows = std::min<int32_t>(kPartTile, t.nrows - r0);
-      u.row_base = t.row_base + r0;
-      part_tiles.push_back(u);
-    }
+    if (h->placement[t.inst] == 3) continue;
+    all_part = false;
+    (h->placement[t.inst] == 1 ? lds_tiles : hbm_tiles).push_back(t);
+  }
+  if (all_part) {
+    part_tiles.swap(split);
+  } else {
+    for (auto& t : split)
+      if (h->placement[t.inst] == 3) part_tiles.push_back(t);
   }
   h->grid_part = 0;
   h->nrec = 0;
   if (!part_tiles.empty()) {
-    // tiles of one instance contiguous (the partition scan relies on it)
-    std::stable_sort(part_tiles.begin(), part_tiles.end(),
-                     [](const AggTile& a, const AggTile& b) { return a.inst < b.inst; });
     std::vector<int> t0(h->ninst, 0), t1(h->ninst, 0);
     for (int64_t t = (int64_t)part_tiles.size() - 1; t >= 0; t--) t0[part_tiles[t].inst] = (int)t;
     for (int64_t t = 0; t < (int64_t)part_tiles.size(); t++) t1[part_tiles[t].inst] = (int)t + 1;

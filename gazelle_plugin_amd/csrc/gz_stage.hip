@@ -60,16 +60,17 @@ struct BlobSeg {
   int64_t off, n;
 };
 
-// One workgroup per segment (the segment table sits at blob + table), 16 bytes
-// per thread and step: segment offsets and destinations are 16-byte aligned
-// (kStageAlign and the pool's allocations), and sizes are rounded up to 16
-// within the destination's capacity and the blob.
+// blockIdx.y: the segment (the segment table sits at blob + table); the x
+// blocks stride over its 16-byte words: segment offsets and destinations are
+// 16-byte aligned (kStageAlign and the pool's allocations), and sizes are
+// rounded up to 16 within the destination's capacity and the blob.
 __global__ __launch_bounds__(256) void blob_scatter_kernel(const uint8_t* blob, int64_t table, int nseg) {
-  if ((int)blockIdx.x >= nseg) return;
-  const BlobSeg S = ((const BlobSeg*)(blob + table))[blockIdx.x];
+  if ((int)blockIdx.y >= nseg) return;
+  const BlobSeg S = ((const BlobSeg*)(blob + table))[blockIdx.y];
   const uint4* src = (const uint4*)(blob + S.off);
   uint4* dst = (uint4*)S.dst;
-  for (int64_t i = threadIdx.x; i < S.n >> 4; i += blockDim.x) dst[i] = src[i];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.n >> 4; i += stride) dst[i] = src[i];
 }
 
 }  // namespace
@@ -94,7 +95,10 @@ int UploadBatch::flush(gz_context* ctx, DBuf* blob) {
   for (size_t k = 0; k < items_.size(); k++)
     segs[k] = BlobSeg{items_[k].dst, (int64_t)items_[k].off, (int64_t)((items_[k].bytes + 15) / 16 * 16)};
   GZ_TRY(upload(ctx, blob, host_.data(), host_.size()));
-  hipLaunchKernelGGL(blob_scatter_kernel, dim3((unsigned)items_.size()), dim3(256), 0, ctx->stream,
+  size_t most = 0;
+  for (auto& it : items_) most = std::max(most, it.bytes);
+  const unsigned gx = (unsigned)std::min<size_t>(64, (most + 4095) / 4096);  // 4 KiB per workgroup and step
+  hipLaunchKernelGGL(blob_scatter_kernel, dim3(gx, (unsigned)items_.size()), dim3(256), 0, ctx->stream,
                      (const uint8_t*)blob->p, (int64_t)table, (int)items_.size());
   GZ_HIP(hipGetLastError());
   host_.clear();
