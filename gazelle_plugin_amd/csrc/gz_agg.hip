@@ -67,13 +67,23 @@ struct Lane {
 };
 
 struct AggInst {
-  int64_t* base;          // keys [S], first_row [S], lanes [nl][S] with S = cap + 2
+  int64_t* base;          // keys [S], first_row [S], lanes [S][nl] with S = cap + 2 (a slot's lanes contiguous)
   uint64_t cap;           // power of two; slot cap = null key, cap + 1 = key equal to kEmpty
   unsigned int* counter;  // keys inserted
   int* overflow;
   int32_t fan_log2;       // partitioned placement: log2 of the partition count (0 = off)
   int32_t bucket_base;    // first global partition of the instance
+  int32_t nl;             // lanes per slot
 };
+
+// The HBM table's first-row word and lanes of slot s: keys and first rows are
+// arrays of their own (probes and occupancy scans read them coalesced); a
+// slot's lanes are one contiguous run, so a group's state is one or two cache
+// lines for its atomics and for the gathers of the finish.
+__device__ __forceinline__ int64_t* tab_first(const AggInst& I, int64_t s) { return I.base + (int64_t)I.cap + 2 + s; }
+__device__ __forceinline__ int64_t* tab_lanes(const AggInst& I, int64_t s) {
+  return I.base + 2 * ((int64_t)I.cap + 2) + s * I.nl;
+}
 
 struct AggTile {
   int32_t batch;
@@ -470,25 +480,38 @@ __device__ __forceinline__ void apply_lane(const AggParams& p, const Lane ln, in
   }
 }
 
-// Apply one row to the lanes of `slot` in a table laid out as base + lane * S.
-// Works on HBM and on LDS (the atomics are generic-address).
+// Apply one row to a slot's state: its first-row word and its lanes, lane l
+// at lanes[l * lstride].  Works on HBM and on LDS (the atomics are
+// generic-address).
 template <class Row>
-__device__ __forceinline__ void apply_row(const AggParams& p, int64_t* base, int64_t S, int64_t slot,
+__device__ __forceinline__ void apply_row(const AggParams& p, int64_t* fr, int64_t* lanes, int64_t lstride,
                                           const Row& row, int64_t grow) {
-  int64_t* fr = base + S + slot;
   if (grow < *fr) atomicMin((long long*)fr, (long long)grow);
-  for (int l = 0; l < p.nl; l++) apply_lane(p, p.lanes[l], base + (2 + l) * S + slot, row, grow);
+  for (int l = 0; l < p.nl; l++) apply_lane(p, p.lanes[l], lanes + l * lstride, row, grow);
+}
+// ... to slot `slot` of an HBM table
+template <class Row>
+__device__ __forceinline__ void apply_row_hbm(const AggParams& p, const AggInst& I, int64_t slot, const Row& row,
+                                              int64_t grow) {
+  apply_row(p, tab_first(I, slot), tab_lanes(I, slot), 1, row, grow);
+}
+// ... to slot `slot` of an LDS table laid out as word w of slot s at tab[w * LS + s]
+template <class Row>
+__device__ __forceinline__ void apply_row_lds(const AggParams& p, int64_t* tab, int64_t LS, int64_t slot,
+                                              const Row& row, int64_t grow) {
+  apply_row(p, tab + LS + slot, tab + 2 * LS + slot, LS, row, grow);
 }
 
 // Merge one LDS slot's lanes into an HBM slot.  The atomics return nothing
 // and nothing waits on them: no HBM round trip per lane.
-__device__ __forceinline__ void merge_slot(const AggParams& p, int64_t* g, int64_t GS, int64_t gslot,
-                                           const int64_t* l, int64_t LS, int64_t lslot) {
+__device__ __forceinline__ void merge_slot(const AggParams& p, const AggInst& I, int64_t gslot, const int64_t* l,
+                                           int64_t LS, int64_t lslot) {
   const int64_t fr = l[LS + lslot];
-  if (fr != INT64_MAX) atomicMin((long long*)(g + GS + gslot), (long long)fr);
+  if (fr != INT64_MAX) atomicMin((long long*)tab_first(I, gslot), (long long)fr);
+  int64_t* glanes = tab_lanes(I, gslot);
   for (int k = 0; k < p.nl; k++) {
     int64_t v = l[(2 + k) * LS + lslot];
-    int64_t* ptr = g + (2 + k) * GS + gslot;
+    int64_t* ptr = glanes + k;
     switch (p.lanes[k].op) {
       case L_CNT: case L_NN: case L_NN2: case L_SUMI: case L_SUMCNT:
         if (v) atomicAdd((unsigned long long*)ptr, (unsigned long long)v);
@@ -608,8 +631,9 @@ __global__ void agg_init_kernel(const AggInst* inst, int ninst, int nl, const in
   for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < S;
        s += (int64_t)gridDim.x * blockDim.x) {
     I.base[s] = kEmpty;
-    I.base[S + s] = INT64_MAX;
-    for (int l = 0; l < nl; l++) I.base[(2 + l) * S + s] = lane_init(lane_ops[l]);
+    *tab_first(I, s) = INT64_MAX;
+    int64_t* ln = tab_lanes(I, s);
+    for (int l = 0; l < nl; l++) ln[l] = lane_init(lane_ops[l]);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *I.counter = 0;
@@ -630,11 +654,11 @@ __global__ void agg_init_all_kernel(const AggInst* inst, int ninst, const int64_
       if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
     }
     const AggInst I = inst[lo];
-    const int64_t S = (int64_t)I.cap + 2;
     const int64_t s = g - prefix[lo];
     I.base[s] = kEmpty;
-    I.base[S + s] = INT64_MAX;
-    for (int l = 0; l < nl; l++) I.base[(2 + l) * S + s] = lane_init(lane_ops[l]);
+    *tab_first(I, s) = INT64_MAX;
+    int64_t* ln = tab_lanes(I, s);
+    for (int l = 0; l < nl; l++) ln[l] = lane_init(lane_ops[l]);
     if (s == 0) {
       *I.counter = 0;
       *I.overflow = 0;
@@ -648,14 +672,13 @@ __global__ __launch_bounds__(256) void agg_update_kernel(AggParams p) {
   if (t < 0) return;
   const AggTile td = p.tiles[t];
   const AggInst I = p.inst[td.inst];
-  const int64_t S = (int64_t)I.cap + 2;
   for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
     int64_t r = (int64_t)td.row_begin + lr;
     int64_t key;
     bool valid = row_key(p, td.batch, r, &key);
     int64_t slot = table_slot(I, valid, key);
     if (slot < 0) continue;
-    apply_row(p, I.base, S, slot, ArrowRow{p, td.batch, r}, td.row_base + lr);
+    apply_row_hbm(p, I, slot, ArrowRow{p, td.batch, r}, td.row_base + lr);
   }
 }
 
@@ -664,7 +687,6 @@ __global__ __launch_bounds__(256) void agg_update_kernel(AggParams p) {
 // LDS table of lcap slots (+2 special) per workgroup, merged into HBM whenever
 // the instance changes and at the end.
 __device__ void lds_merge(const AggParams& p, const AggInst& I, int64_t* l, int64_t LS) {
-  const int64_t GS = (int64_t)I.cap + 2;
   for (int64_t s = threadIdx.x; s < (int64_t)p.lcap + 2; s += blockDim.x) {
     bool special = s >= p.lcap;
     int64_t key = l[s];
@@ -674,7 +696,7 @@ __device__ void lds_merge(const AggParams& p, const AggInst& I, int64_t* l, int6
     if (special) g = (int64_t)I.cap + (s - p.lcap);
     else g = table_slot(I, true, key);
     if (g < 0) continue;
-    merge_slot(p, I.base, GS, g, l, LS, s);
+    merge_slot(p, I, g, l, LS, s);
   }
 }
 
@@ -729,10 +751,10 @@ __global__ __launch_bounds__(256) void agg_update_lds_kernel(AggParams p) {
         }
       }
       if (ls >= 0) {
-        apply_row(p, s_tab, LS, ls, ArrowRow{p, td.batch, r}, grow);
+        apply_row_lds(p, s_tab, LS, ls, ArrowRow{p, td.batch, r}, grow);
       } else {
         int64_t g = table_slot(I, valid, key);
-        if (g >= 0) apply_row(p, I.base, (int64_t)I.cap + 2, g, ArrowRow{p, td.batch, r}, grow);
+        if (g >= 0) apply_row_hbm(p, I, g, ArrowRow{p, td.batch, r}, grow);
       }
     }
   }
@@ -752,7 +774,6 @@ __global__ __launch_bounds__(256) void agg_fix_kernel(AggParams p) {
   if (t >= p.T) return;
   const AggTile td = p.tiles[t];
   const AggInst I = p.inst[td.inst];
-  const int64_t S = (int64_t)I.cap + 2;
   for (int lr = threadIdx.x; lr < td.nrows; lr += blockDim.x) {
     const int64_t r = (int64_t)td.row_begin + lr, grow = td.row_base + lr;
     int64_t key;
@@ -768,12 +789,12 @@ __global__ __launch_bounds__(256) void agg_fix_kernel(AggParams p) {
       if (slot < 0) break;
       int64_t want = grow, have;
       if (ln.col2 < 0) {
-        have = I.base[S + slot];
+        have = *tab_first(I, slot);
       } else {
         if (p.lanes[ln.col2].op == L_GLASTNAN) want = glast_ord(grow, r);
-        have = I.base[(2 + ln.col2) * S + slot];
+        have = tab_lanes(I, slot)[ln.col2];
       }
-      if (have == want) I.base[(2 + l) * S + slot] = raw;
+      if (have == want) tab_lanes(I, slot)[l] = raw;
     }
   }
 }
@@ -795,7 +816,6 @@ __global__ __launch_bounds__(256) void agg_allnull_kernel(AggParams p, const GBa
   if (b >= nb) return;
   const GBatch g = gb[b];
   const AggInst I = p.inst[g.inst];
-  const int64_t S = (int64_t)I.cap + 2;
   for (int l = 0; l < p.nl; l++) {
     const Lane ln = p.lanes[l];
     if (ln.op != L_MINI && ln.op != L_MAXI && ln.op != L_MINU) continue;  // unsigned max with 0 changes nothing
@@ -816,8 +836,8 @@ __global__ __launch_bounds__(256) void agg_allnull_kernel(AggParams p, const GBa
       for (int k = 0; k < p.nl; k++)
         if (p.lanes[k].op == L_FIRSTNN && p.lanes[k].col == ln.col) fl = k;
       const int64_t slot = table_find(I, true, 0);
-      if (fl >= 0 && slot >= 0 && I.base[(2 + fl) * S + slot] < g.base) {
-        int64_t* ptr = I.base + (2 + l) * S + slot;
+      if (fl >= 0 && slot >= 0 && tab_lanes(I, slot)[fl] < g.base) {
+        int64_t* ptr = tab_lanes(I, slot) + l;
         if (ln.op == L_MINI) atomicMin((long long*)ptr, 0ll);
         else if (ln.op == L_MAXI) atomicMax((long long*)ptr, 0ll);
         else atomicMin((unsigned long long*)ptr, 0ull);
@@ -963,7 +983,6 @@ __global__ __launch_bounds__(256) void join_agg_update_kernel(AggParams p, JoinA
   lds_reset(p, s_tab, LS);
   __syncthreads();
   const AggInst I = p.inst[0];
-  const int64_t GS = (int64_t)I.cap + 2;
   const uint64_t jmask = q.jcap - 1;
   for (int64_t t = t0; t < t1; t++) {
     const JoinTile td = q.tiles[t];
@@ -1031,12 +1050,12 @@ __global__ __launch_bounds__(256) void join_agg_update_kernel(AggParams p, JoinA
             if (!ok) nulls |= 1u << k;
           }
           const RegRow row{p, w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], nulls};
-          if (ls >= 0) apply_row(p, s_tab, LS, ls, row, grow);
-          else apply_row(p, I.base, GS, g, row, grow);
+          if (ls >= 0) apply_row_lds(p, s_tab, LS, ls, row, grow);
+          else apply_row_hbm(p, I, g, row, grow);
         } else {
           const JoinRow row{q, td.batch, r, br};
-          if (ls >= 0) apply_row(p, s_tab, LS, ls, row, grow);
-          else apply_row(p, I.base, GS, g, row, grow);
+          if (ls >= 0) apply_row_lds(p, s_tab, LS, ls, row, grow);
+          else apply_row_hbm(p, I, g, row, grow);
         }
       }
     }
@@ -1056,7 +1075,6 @@ __global__ __launch_bounds__(256) void join_agg_fix_kernel(AggParams p, JoinAggP
   if (t >= q.T) return;
   const JoinTile td = q.tiles[t];
   const AggInst I = p.inst[0];
-  const int64_t GS = (int64_t)I.cap + 2;
   const uint64_t jmask = q.jcap - 1;
   const uint8_t* kv = (const uint8_t*)q.bufs[(int64_t)td.batch * q.bstride + q.pkey * 3];
   const void* kd = q.bufs[(int64_t)td.batch * q.bstride + q.pkey * 3 + 2];
@@ -1104,8 +1122,8 @@ __global__ __launch_bounds__(256) void join_agg_fix_kernel(AggParams p, JoinAggP
           raw = raw_bits(q.bufs[(int64_t)td.batch * q.bstride + c.idx * 3 + 2], c.type, r);
         }
         if (!raw_is_nan(raw, c.type)) continue;
-        const int64_t have = ln.col2 < 0 ? I.base[GS + g] : I.base[(2 + ln.col2) * GS + g];
-        if (have == grow) I.base[(2 + l) * GS + g] = raw;
+        const int64_t have = ln.col2 < 0 ? *tab_first(I, g) : tab_lanes(I, g)[ln.col2];
+        if (have == grow) tab_lanes(I, g)[l] = raw;
       }
     }
   }
@@ -1398,14 +1416,14 @@ __global__ __launch_bounds__(kUpdBlock, 2 * kUpdBlock / 256) void agg_part_updat
         } else {
 #pragma unroll
           for (int k = 0; k < kUpdRPT; k++)
-            if (ls[k] >= 0) apply_row(p, s_tab, LS, ls[k], row_of(k), grow_of(k));
+            if (ls[k] >= 0) apply_row_lds(p, s_tab, LS, ls[k], row_of(k), grow_of(k));
         }
         // records whose LDS probe failed go to the instance's HBM table
 #pragma unroll
         for (int k = 0; k < kUpdRPT; k++) {
           if (ls[k] != -2) continue;
           const int64_t g = table_slot(I, true, kw[k]);
-          if (g >= 0) apply_row(p, I.base, (int64_t)I.cap + 2, g, row_of(k), grow_of(k));
+          if (g >= 0) apply_row_hbm(p, I, g, row_of(k), grow_of(k));
         }
       }
       __syncthreads();
@@ -1486,21 +1504,23 @@ __global__ __launch_bounds__(256) void agg_hll_estimate_kernel(const unsigned in
 
 // Rehash of an instance into a larger table (keys are unique: plain stores).
 __global__ void agg_rehash_kernel(AggInst from, AggInst to, int nl) {
-  const int64_t FS = (int64_t)from.cap + 2, TS = (int64_t)to.cap + 2;
+  const int64_t FS = (int64_t)from.cap + 2;
   for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < FS;
        s += (int64_t)gridDim.x * blockDim.x) {
     int64_t key = from.base[s];
     int64_t d;
     if (s >= (int64_t)from.cap) {
-      if (from.base[FS + s] == INT64_MAX) continue;
+      if (*tab_first(from, s) == INT64_MAX) continue;
       d = (int64_t)to.cap + (s - (int64_t)from.cap);
     } else {
       if (key == kEmpty) continue;
       d = table_slot(to, true, key);
       if (d < 0) continue;
     }
-    to.base[TS + d] = from.base[FS + s];
-    for (int l = 0; l < nl; l++) to.base[(2 + l) * TS + d] = from.base[(2 + l) * FS + s];
+    *tab_first(to, d) = *tab_first(from, s);
+    const int64_t* fl = tab_lanes(from, s);
+    int64_t* tl = tab_lanes(to, d);
+    for (int l = 0; l < nl; l++) tl[l] = fl[l];
   }
 }
 
@@ -1665,9 +1685,9 @@ __global__ void agg_finish_kernel(FinishParams f) {
     int inst = (int)(f.skeys[i] >> 44);
     int64_t slot = (int64_t)f.svals[i];
     const AggInst I = f.inst[inst];
-    const int64_t S = (int64_t)I.cap + 2;
     const int64_t* b = I.base;
-    auto lane = [&](int l) -> int64_t { return b[(2 + l) * S + slot]; };
+    const int64_t* ls = tab_lanes(I, slot);
+    auto lane = [&](int l) -> int64_t { return ls[l]; };
     for (int c = 0; c < f.nres; c++) {
       const ResultSpec& rs = f.res[c];
       bool ok = true;
@@ -2048,6 +2068,7 @@ static int inst_alloc(gz_agg* h, int i, uint64_t cap) {
   ni.cap = cap;
   ni.counter = (unsigned int*)nc.p;
   ni.overflow = (int*)((char*)nc.p + 8);
+  ni.nl = nl;
   // initialize this one instance
   DBuf one;
   GZ_TRY(upload(ctx, &one, &ni, sizeof(AggInst)));
@@ -2530,6 +2551,7 @@ static int plan_table_alloc(gz_agg* h, int i, uint64_t cap) {
   ni.cap = cap;
   ni.counter = (unsigned int*)h->counters[i].p;
   ni.overflow = (int*)((char*)h->counters[i].p + 8);
+  ni.nl = nl;
   h->inst[i] = ni;
   h->inst_dirty = true;
   return GZ_OK;
