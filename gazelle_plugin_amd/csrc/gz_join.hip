@@ -37,12 +37,12 @@ constexpr int kJoinMaxCols = 32;
 #ifndef GZ_WRITE01_BLOCK
 #define GZ_WRITE01_BLOCK 1024
 #endif
-constexpr int kWrite01Block = GZ_WRITE01_BLOCK;
+constexpr int kWrite01Block = GZ_WRITE01_BLOCK;  // threads of the at-most-one-row write pass (4096-row tiles)
 // one-batch calls: 256-row workgroups, one row per thread, at most one wave of workgroups
 constexpr int kSingleTile = 256;
 constexpr int kSingleMaxBlocks = 64;
 constexpr int kProbeInlineBufs = 48;  // buffers of up to 16 probe columns
-constexpr int kLookSpins = 1 << 20;   // polls of an earlier block's count before giving up  // threads of the at-most-one-row write pass (4096-row tiles)
+constexpr int kLookSpins = 1 << 20;   // polls of an earlier block's count before giving up
 constexpr int64_t kJEmpty = INT64_MIN;
 
 __device__ __forceinline__ uint64_t jmix64(uint64_t k) {
@@ -575,7 +575,9 @@ __device__ __forceinline__ void store_value(const OutCol& oc, uint64_t o, uint64
 // posts its emitted-row count and sums the counts of the blocks before it (the
 // lanes of one wave poll one earlier block each) for its output offset.  Blocks
 // are dispatched in index order, so each wait is on a block already running;
-// the polls are bounded all the same and a wait that runs out is reported.
+// the polls are bounded all the same and a wait that runs out is reported.  A
+// posted word carries its whole message, so its atomics are relaxed (agent-
+// scope acquire / release would invalidate / write back the L2 at every poll).
 template <int TILE, int BLOCK, int SINGLE>
 __global__ __launch_bounds__(BLOCK) void join_probe_write01_kernel(ProbeParams p) {
   constexpr int RPT = TILE / BLOCK;
@@ -627,13 +629,13 @@ __global__ __launch_bounds__(BLOCK) void join_probe_write01_kernel(ProbeParams p
         for (int x = 0; x < NW; x++) rows += s_wt[k][x];
       const int id = (int)t;
       if (lane == 0)
-        __hip_atomic_store(p.look + id, (uint64_t)p.epoch << 32 | rows, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(p.look + id, (uint64_t)p.epoch << 32 | rows, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       uint32_t before = 0;
       bool late = false;
       if (lane < id) {
         uint64_t v;
         int spins = 0;
-        while (((v = __hip_atomic_load(p.look + lane, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >> 32) !=
+        while (((v = __hip_atomic_load(p.look + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) !=
                p.epoch) {
           if (++spins > kLookSpins) {
             late = true;
@@ -1427,8 +1429,9 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   for (auto& o : h->outs) any_bin_out |= type_is_binary(o.type);
   const int64_t rows0 = nb == 1 ? batches[0].num_rows : 0;
   const int64_t sblocks = (rows0 + kSingleTile - 1) / kSingleTile;
+  const bool at_most_one = p.join_type != GZ_JOIN_INNER || !p.J.rowlist;
   const bool single = nb == 1 && rows0 > 0 && sblocks <= kSingleMaxBlocks && nc * 3 <= kProbeInlineBufs &&
-                      (p.join_type != GZ_JOIN_INNER || !p.J.rowlist) && !any_bin_out;
+                      at_most_one && !any_bin_out;
   if (single) {
     if (!h->look.p) {
       GZ_TRY(ctx_ensure(ctx, &h->look, sizeof(uint64_t) * kSingleMaxBlocks));
@@ -1562,7 +1565,6 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   }
   for (int c = (int)h->pay_off.size(); c < nres; c++) p.pay_off[c] = -1;
   p.payload = use_payload ? (const u32x4*)h->payload.p : nullptr;
-  const bool at_most_one = p.join_type != GZ_JOIN_INNER || !p.J.rowlist;
   if (single) {
     {
       KernelTimer kt(ctx, "join_probe_write");

@@ -346,3 +346,64 @@ def test_probe_context_is_set_before_the_first_call(ctx):
     pk.close()
     other.close()
     rel.close()
+
+
+
+@pytest.mark.parametrize("join_type", ["inner", "semi", "anti"])
+def test_batched_probe_many_tiles(ctx, join_type):
+    """Batched probes whose rows emit at most one row each: 100-200 tiles of
+    4096 rows per call (every tile's output offset from the scan of the tile
+    counts), with null probe keys, three calls on one probe handle.  Expected
+    rows from the oracle's join, columns gathered with numpy."""
+    rng = np.random.default_rng({"inner": 71, "semi": 72, "anti": 73}[join_type])
+    bk = rng.permutation(1 << 20)[:50000].astype(np.int64)
+    bint = rng.integers(-1000, 1000, size=len(bk)).astype(np.int32)
+    bdbl = rng.standard_normal(len(bk))
+    cuts = ((0, 20000), (20000, 30000))
+    build = [HostBatch(n, [HostColumn(T.INT64, bk[b0:b0 + n].copy()), HostColumn(T.INT32, bint[b0:b0 + n].copy()),
+                           HostColumn(T.DOUBLE, bdbl[b0:b0 + n].copy())]) for b0, n in cuts]
+    btypes, ptypes = [T.INT64, T.INT32, T.DOUBLE], [T.INT64, T.INT32]
+    result = [[1, 0], [0, 1], [0, 2], [1, 1]] if join_type == "inner" else [[1, 0], [1, 1]]
+    rel = HashRelationKernel.Make(ctx, btypes, 0)
+    for b in build:
+        rel.Evaluate(DeviceBatch.from_host(b))
+    rel.Finish()
+    pr = ConditionedProbeKernel.Make(rel, ptypes, 0, join_type, False, result)
+    for call in range(3):
+        probe = []
+        for _ in range(int(rng.integers(40, 70))):
+            n = int(rng.integers(9000, 13000))
+            pk = np.where(rng.random(n) < 0.7, bk[rng.integers(0, len(bk), n)], rng.integers(0, 1 << 20, n))
+            probe.append(HostBatch(n, [HostColumn(T.INT64, pk.astype(np.int64), bitmap_from_bools(rng.random(n) >= 0.01)),
+                                       HostColumn(T.INT32, rng.integers(0, 1 << 30, size=n).astype(np.int32))]))
+        n, per = pr.ProcessBatches([DeviceBatch.from_host(b) for b in probe])
+        got = pr.result_columns(n)
+        outs = oracle.join([key_of(b, 0) for b in build], [key_of(b, 0) for b in probe], T.INT64,
+                           JT[join_type], False, 1)
+        ek, ekv, ei, ebi, ebd = [], [], [], [], []
+        for pb, out in zip(probe, outs):
+            prow = np.asarray(out[0] if join_type == "inner" else out, np.int64)
+            ek.append(pb.columns[0].values[prow])
+            ekv.append(bools_from_bitmap(pb.columns[0].validity, pb.num_rows)[prow])
+            ei.append(pb.columns[1].values[prow])
+            if join_type == "inner":
+                brow = np.asarray(out[1], np.int64)
+                ebi.append(bint[brow])
+                ebd.append(bdbl[brow])
+        assert [int(x) for x in per] == [len(x) for x in ek]
+        exp_cols = [(np.concatenate(ek), np.concatenate(ekv))]
+        if join_type == "inner":
+            exp_cols += [(np.concatenate(ebi), None), (np.concatenate(ebd), None)]
+        exp_cols.append((np.concatenate(ei), None))
+        assert n == len(exp_cols[0][0])
+        for (t, vals, ok), (ev, eok) in zip(got, exp_cols):
+            ok = np.asarray(ok, bool)
+            if eok is None:
+                assert ok.all()
+            else:
+                np.testing.assert_array_equal(ok, eok)
+            vals = np.asarray(vals)
+            m = ok if eok is not None else np.ones(len(ok), bool)
+            np.testing.assert_array_equal(vals[m].view(np.uint8), np.asarray(ev)[m].view(np.uint8))
+    pr.close()
+    rel.close()
