@@ -35,10 +35,12 @@
 
 namespace gz {
 
-constexpr int kMaxCols = 32;
-constexpr int kMaxKeys = 4;
+constexpr int kMaxCols = 128;  // input columns (and output columns)
+constexpr int kArgCols = 32;   // columns whose tables ride in the kernel arguments
+constexpr int kMaxKeys = 16;
 constexpr int kMaxPartitions = 2048;
 constexpr int kScanChunk = 16;  // tiles per chunk of the tile scan
+constexpr size_t kCountLdsMax = 128 * 1024;  // LDS bytes of the count kernel's per-tile counters
 constexpr int kPre = 4;         // fixed-width columns the scatter loads up front
 // ... fewer when a thread owns more rows (registers)
 __host__ __device__ constexpr int scatter_max_pre(int rows_per_thread) {
@@ -56,6 +58,24 @@ struct TileDesc {
   int32_t pad;
 };
 
+// A column of the split: constant per splitter except nullc and is_pre, which
+// depend on the call's buffers.
+struct SplitCol {
+  int kind, width;
+  int out_col;  // output column of an input column, -1 if dropped
+  int bin;      // binary column ordinal, -1 otherwise
+  int nullc;    // null-count slot of a nullable column, -1 otherwise
+  int is_pre;   // loaded up front by the scatter
+};
+
+// Buffers of an output column in a segment.
+struct SplitOut {
+  void* values;
+  uint32_t* valid;
+  int32_t* off;
+  uint8_t* data;
+};
+
 struct SplitParams {
   const TileDesc* tiles;
   int64_t T;
@@ -70,14 +90,10 @@ struct SplitParams {
   int key_col[kMaxKeys];
   int key_type[kMaxKeys];
   int ncols;                 // input columns
-  int kind[kMaxCols];
-  int width[kMaxCols];
-  int out_col[kMaxCols];     // output column of an input column, -1 if dropped
-  int bin[kMaxCols];         // binary column ordinal, -1 otherwise
-  int nullc[kMaxCols];       // null-count slot of a nullable column, -1 otherwise
+  SplitCol acol[kArgCols];   // the first kArgCols columns
+  const SplitCol* col;       // [ncols] (device): the columns from kArgCols on
   int npre;                  // fixed-width columns loaded up front by the scatter
   int pre_col[kPre];         // (the key column first on the int64-key path)
-  int is_pre[kMaxCols];
   int nbin;
   int bin_col0;              // input column of the first binary column
   // per-segment buffers.  A tile's counter row has R = P * (1 + nbin) + nnull
@@ -88,14 +104,26 @@ struct SplitParams {
   uint32_t* prefix;          // [T][R] exclusive over tiles
   const int64_t* row_start;  // [P+1]
   const int64_t* byte_start; // [nbin][P+1]
-  void* out_values[kMaxCols];
-  uint32_t* out_valid[kMaxCols];
-  int32_t* out_off[kMaxCols];
-  uint8_t* out_data[kMaxCols];
+  SplitOut aout[kArgCols];   // the first kArgCols output columns
+  const SplitOut* out;       // [output columns] (device): the columns from kArgCols on
   int* err;
   int cbuf;                  // LDS bytes for partition-ordered utf8 staging
   int rowid_oc;              // output column of the row indices, -1 if none
 };
+
+// Column and output tables: schemas of at most kArgCols columns read them
+// from the kernel arguments (WIDE = 0, no memory load), wider ones from the
+// device tables (WIDE = 1).
+template <int WIDE>
+__device__ __forceinline__ const SplitCol& col_of(const SplitParams& p, int c) {
+  return WIDE ? p.col[c] : p.acol[c];
+}
+template <int WIDE>
+__device__ __forceinline__ const SplitOut& out_of(const SplitParams& p, int oc) {
+  return WIDE ? p.out[oc] : p.aout[oc];
+}
+
+static_assert(sizeof(SplitParams) <= 4096, "kernel arguments");
 
 __device__ __forceinline__ void load_tile_ptrs(const SplitParams& p, int batch, const void** s_bp) {
   for (int i = threadIdx.x; i < p.ncols * 3; i += blockDim.x)
@@ -170,7 +198,7 @@ __device__ __forceinline__ int compute_pid(const SplitParams& p, const void* con
 
 // ---------------------------------------------------------------- pass 1
 // KEYK = 1: one int64 key without validity (the hot case), else generic.
-template <int BLOCK, int KEYK>
+template <int BLOCK, int KEYK, int WIDE>
 __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
   extern __shared__ uint32_t s_cnt[];  // [R]
   __shared__ const void* s_bp[kMaxCols * 3];
@@ -206,7 +234,7 @@ __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
           else if (p.nbin) {
             int64_t r = (int64_t)td.row_begin + lr;
             for (int c = 0; c < p.ncols; c++) {
-              int j = p.bin[c];
+              int j = col_of<WIDE>(p, c).bin;
               if (j < 0) continue;
               const int32_t* of = (const int32_t*)s_bp[c * 3 + 1];
               atomicAdd(&s_cnt[P * (1 + j) + pid], (uint32_t)(gld(of + r + 1) - gld(of + r)));
@@ -222,7 +250,7 @@ __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
       if (p.nbin) {
         int64_t r = (int64_t)td.row_begin + lr;
         for (int c = 0; c < p.ncols; c++) {
-          int j = p.bin[c];
+          int j = col_of<WIDE>(p, c).bin;
           if (j < 0) continue;
           const int32_t* off = (const int32_t*)s_bp[c * 3 + 1];
           atomicAdd(&s_cnt[P * (1 + j) + pid], (uint32_t)(gld(off + r + 1) - gld(off + r)));
@@ -233,7 +261,7 @@ __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
   // null counts: zero bits of each validity bitmap over the tile's rows
   // (tiles start on multiples of 8 rows inside a batch).
   for (int c = 0; c < p.ncols; c++) {
-    int slot = p.nullc[c];
+    int slot = col_of<WIDE>(p, c).nullc;
     if (slot < 0) continue;
     const uint8_t* vb = (const uint8_t*)s_bp[c * 3 + 0];
     if (!vb) continue;
@@ -377,7 +405,7 @@ __host__ __device__ inline ScatterLds scatter_layout(int TILE, int BLOCK, int P,
 // hashing and ranking instead of one round trip per column.  The tile's utf8
 // bytes are copied into LDS as one contiguous block (coalesced 16-byte loads);
 // each sorted slot then writes its string from LDS.
-template <int TILE, int BLOCK, int KEYK>
+template <int TILE, int BLOCK, int KEYK, int WIDE>
 __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatter_kernel(SplitParams p) {
   constexpr int NW = BLOCK / kWave;
   constexpr int RPT = TILE / BLOCK;  // rows per thread
@@ -424,7 +452,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
       const int c = p.pre_col[k];
       const uint8_t* v = colp(c, 2);
       const uint8_t* vbm = colp(c, 0);
-      const int W = p.width[c];
+      const int W = col_of<WIDE>(p, c).width;
 #pragma unroll
       for (int j = 0; j < RPT; j++) {
         if (lrow(j) < n) {
@@ -545,9 +573,10 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
   for (int k = 0; k < NPRE; k++) {
     if (k >= p.npre) break;
     const int c = p.pre_col[k];
-    const int oc = p.out_col[c];
-    if (p.out_valid[oc]) {
-      uint32_t* ov = p.out_valid[oc];
+    const int oc = col_of<WIDE>(p, c).out_col;
+    const SplitOut& O = out_of<WIDE>(p, oc);
+    if (O.valid) {
+      uint32_t* ov = O.valid;
 #pragma unroll
       for (int j = 0; j < RPT; j++) {
         if (lrow(j) < n && (nullm[j] >> k & 1)) {
@@ -556,8 +585,8 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
         }
       }
     }
-    const int W = p.width[c];
-    uint8_t* dst = (uint8_t*)p.out_values[oc];
+    const int W = col_of<WIDE>(p, c).width;
+    uint8_t* dst = (uint8_t*)O.values;
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
       if (lrow(j) < n) {
@@ -584,16 +613,18 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
 
   // ---- 4b: every other column
   for (int c = 0; c < p.ncols; c++) {
-    int oc = p.out_col[c];
-    if (oc < 0 || p.is_pre[c]) continue;
-    int kind = p.kind[c];
+    const SplitCol& C = col_of<WIDE>(p, c);
+    const int oc = C.out_col;
+    if (oc < 0 || C.is_pre) continue;
+    const int kind = C.kind;
     if (kind == K_NA) continue;
     const uint8_t* vb = colp(c, 0);
     const uint8_t* src = colp(c, 2);
 
     // validity: clear the bit of every null row at its destination
-    if (vb && p.out_valid[oc]) {
-      uint32_t* ov = p.out_valid[oc];
+    const SplitOut& O = out_of<WIDE>(p, oc);
+    if (vb && O.valid) {
+      uint32_t* ov = O.valid;
 #pragma unroll
       for (int j = 0; j < RPT; j++) {
         if (lrow(j) < n && !bit_get(vb, rb + lrow(j))) {
@@ -604,7 +635,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
     }
 
     if (kind == K_BOOL) {
-      uint32_t* ov = (uint32_t*)p.out_values[oc];
+      uint32_t* ov = (uint32_t*)O.values;
 #pragma unroll
       for (int j = 0; j < RPT; j++) {
         if (lrow(j) < n && !bit_get(src, rb + lrow(j))) {
@@ -613,10 +644,10 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
         }
       }
     } else if (kind == K_FIXED) {
-      const int W = p.width[c];
+      const int W = C.width;
       const int parts = W > 8 ? W / 8 : 1;  // 16-byte values move as two 8-byte halves
       const int EW = W > 8 ? 8 : W;
-      uint8_t* dst = (uint8_t*)p.out_values[oc];
+      uint8_t* dst = (uint8_t*)O.values;
       for (int part = 0; part < parts; part++) {
         if (EW == 8) {
           const uint64_t* s8 = (const uint64_t*)src;
@@ -664,7 +695,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
         __syncthreads();
       }
     } else if (kind == K_BINARY) {
-      const int j_bin = p.bin[c];
+      const int j_bin = C.bin;
       const bool first = c == bc;  // offsets prefetched, bytes possibly in LDS
       uint32_t* s_src = (uint32_t*)s_stage;  // [TILE] source byte offset by sorted slot
       uint32_t* s_boff = s_src + TILE;       // [TILE+1] sorted byte prefix
@@ -723,8 +754,8 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
         s_dadj[q] -= (int64_t)run0;
       }
       __syncthreads();
-      uint8_t* odata = p.out_data[oc];
-      int32_t* ooff = p.out_off[oc];
+      uint8_t* odata = O.data;
+      int32_t* ooff = O.off;
       // offsets, in sorted-slot order (contiguous per partition run); offsets of
       // partition q start at index row_start + q
 #pragma unroll 1
@@ -757,7 +788,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
     for (int j = 0; j < RPT; j++)
       if (lrow(j) < n) ((uint32_t*)s_stage)[pos[j]] = (uint32_t)(td.call_row + lrow(j));
     __syncthreads();
-    uint32_t* dst = (uint32_t*)p.out_values[p.rowid_oc];
+    uint32_t* dst = (uint32_t*)out_of<WIDE>(p, p.rowid_oc).values;
     for (int i = tid; i < n; i += BLOCK) gst(dst + drow(i), ((uint32_t*)s_stage)[i]);
   }
 }
@@ -804,6 +835,8 @@ struct gz_splitter {
   int32_t rr_state = 0;
   int tile_rows = 4096;
   SplitParams base;  // constant part of the kernel parameters
+  std::vector<SplitCol> cols;  // per input column (nullc and is_pre set per call)
+  DBuf d_cols, d_outs;         // a call's column and output tables
   std::vector<SplitSegment*> segments;
   DBuf d_tiles, d_bufs;
   bool stopped = false;
@@ -892,7 +925,7 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
   if (P > kMaxPartitions)
     return fail(GZ_NOT_IMPLEMENTED, "more than " + std::to_string(kMaxPartitions) +
                                         " partitions is not supported by the HIP splitter");
-  if (num_cols > kMaxCols) return fail(GZ_NOT_IMPLEMENTED, "more than 32 columns");
+  if (num_cols > kMaxCols) return fail(GZ_NOT_IMPLEMENTED, "more than " + std::to_string(kMaxCols) + " columns");
   gz_split_options o;
   gz_split_options_defaults(&o);
   if (opts) o = *opts;
@@ -906,7 +939,7 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
     return fail(GZ_INVALID, "RecordBatch field 0 should be int32");
   if (mode == 0) {
     if (num_keys < 0 || num_keys > kMaxKeys)
-      return fail(GZ_NOT_IMPLEMENTED, "at most 4 hash keys");
+      return fail(GZ_NOT_IMPLEMENTED, "at most " + std::to_string(kMaxKeys) + " hash keys");
     for (int k = 0; k < num_keys; k++) {
       int c = key_cols[k];
       if (c < 0 || c >= num_cols) return fail(GZ_INVALID, "key column out of range");
@@ -944,19 +977,22 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
   }
   b.ncols = num_cols;
   int oc = 0, nbin = 0;
+  h->cols.assign(num_cols, SplitCol{});
   for (int c = 0; c < num_cols; c++) {
     int t = col_types[c];
-    b.kind[c] = t == GZ_TYPE_NA ? K_NA : t == GZ_TYPE_BOOL ? K_BOOL : type_is_binary(t) ? K_BINARY : K_FIXED;
-    b.width[c] = type_width(t);
+    SplitCol& C = h->cols[c];
+    C.kind = t == GZ_TYPE_NA ? K_NA : t == GZ_TYPE_BOOL ? K_BOOL : type_is_binary(t) ? K_BINARY : K_FIXED;
+    C.width = type_width(t);
+    C.nullc = -1;
     if (mode == 2 && c == 0) {
-      b.out_col[c] = -1;
-      b.bin[c] = -1;
+      C.out_col = -1;
+      C.bin = -1;
       continue;
     }
-    b.out_col[c] = oc++;
+    C.out_col = oc++;
     h->out_types.push_back(t);
     if (type_is_binary(t) && nbin == 0) b.bin_col0 = c;
-    b.bin[c] = type_is_binary(t) ? nbin++ : -1;
+    C.bin = type_is_binary(t) ? nbin++ : -1;
   }
   b.nbin = nbin;
   b.rowid_oc = -1;
@@ -975,6 +1011,11 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
   if (scatter_lds(h->tile_rows, scatter_block(P), P, scatter_cbuf(h->tile_rows, nbin)) > 160 * 1024) {
     delete h;
     return fail(GZ_NOT_IMPLEMENTED, "partition count too large for the LDS-staged scatter");
+  }
+  // the count kernel's LDS counters: rows and utf8 bytes per partition, nulls per output column
+  if (sizeof(uint32_t) * ((size_t)P * (1 + nbin) + std::max(oc, 1)) > kCountLdsMax) {
+    delete h;
+    return fail(GZ_NOT_IMPLEMENTED, "num_partitions x (1 + utf8/binary columns) too large for the LDS counters");
   }
   *out = h;
   return GZ_OK;
@@ -1037,8 +1078,8 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
       }
       if (bt.num_rows > 0 && !slot[2]) return fail(GZ_INVALID, "missing values buffer");
       if (type_is_binary(t)) {
-        data_bytes[h->base.bin[c]] += (size_t)vsz;
-        data_ptrs[h->base.bin[c]].push_back(slot[2]);
+        data_bytes[h->cols[c].bin] += (size_t)vsz;
+        data_ptrs[h->cols[c].bin].push_back(slot[2]);
       }
       else if (t == GZ_TYPE_BOOL) {
         if (vsz < (bt.num_rows + 7) / 8) return fail(GZ_INVALID, "values buffer too small");
@@ -1104,8 +1145,10 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   seg->data.resize(nout);
   const int64_t nwords = (N + 63) / 64 * 2;  // 32-bit words of a 64-bit padded bitmap
   SplitParams prm = h->base;
+  std::vector<SplitCol> cols = h->cols;  // this call's column table
+  std::vector<SplitOut> outs(std::max(nout, 1), SplitOut{nullptr, nullptr, nullptr, nullptr});
   for (int c = 0; c < nc; c++) {
-    int oc = prm.out_col[c];
+    int oc = cols[c].out_col;
     if (oc < 0) continue;
     int t = h->types_in[c];
     if (t == GZ_TYPE_NA) continue;
@@ -1119,14 +1162,14 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     } else {
       if ((rc = ctx_alloc(ctx, (size_t)std::max<int64_t>(N, 1) * type_width(t), &seg->values[oc]))) return bail(rc);
     }
-    prm.out_values[oc] = seg->values[oc].p;
-    prm.out_valid[oc] = (uint32_t*)seg->validity[oc].p;
-    prm.out_off[oc] = (int32_t*)seg->offsets[oc].p;
-    prm.out_data[oc] = (uint8_t*)seg->data[oc].p;
+    outs[oc].values = seg->values[oc].p;
+    outs[oc].valid = (uint32_t*)seg->validity[oc].p;
+    outs[oc].off = (int32_t*)seg->offsets[oc].p;
+    outs[oc].data = (uint8_t*)seg->data[oc].p;
   }
   if (prm.rowid_oc >= 0) {
     if ((rc = ctx_alloc(ctx, (size_t)std::max<int64_t>(N, 1) * 4, &seg->values[prm.rowid_oc]))) return bail(rc);
-    prm.out_values[prm.rowid_oc] = seg->values[prm.rowid_oc].p;
+    outs[prm.rowid_oc].values = seg->values[prm.rowid_oc].p;
   }
   // output columns of an input column are indexed by the output index
   if ((rc = upload(ctx, &h->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size()))) return bail(rc);
@@ -1137,8 +1180,8 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   prm.bstride = nc * 3;
   prm.R = R;
   for (int c = 0; c < nc; c++) {
-    int oc = prm.out_col[c];
-    prm.nullc[c] = (oc >= 0 && has_validity[c] && h->types_in[c] != GZ_TYPE_NA) ? P * (1 + nbin) + oc : -1;
+    int oc = cols[c].out_col;
+    cols[c].nullc = (oc >= 0 && has_validity[c] && h->types_in[c] != GZ_TYPE_NA) ? P * (1 + nbin) + oc : -1;
   }
   prm.counts = (uint32_t*)seg->counts.p;
   prm.prefix = (uint32_t*)seg->prefix.p;
@@ -1159,21 +1202,30 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   }
   const int64_t grid = (T + 7) / 8 * 8;
   // hot path: hash of one int64 key that has no validity buffer in this call
-  const int keyk = (h->mode == 0 && prm.nkeys == 1 && prm.key_type[0] == GZ_TYPE_INT64 &&
+  // wide schemas read the column tables from device memory (generic hash path)
+  const int wide = (nc > kArgCols || nout > kArgCols) ? 1 : 0;
+  const int keyk = (!wide && h->mode == 0 && prm.nkeys == 1 && prm.key_type[0] == GZ_TYPE_INT64 &&
                     !has_validity[prm.key_col[0]]) ? 1 : 0;
   // fixed-width columns the scatter loads before ranking (the key first when the
   // int64-key path hashes it from those registers)
   prm.npre = 0;
-  for (int c = 0; c < nc; c++) prm.is_pre[c] = 0;
+  for (int c = 0; c < nc; c++) cols[c].is_pre = 0;
   const int max_pre = scatter_max_pre(TILE / scatter_block(P));
   auto add_pre = [&](int c) {
-    if (prm.npre >= max_pre || prm.is_pre[c] || prm.out_col[c] < 0 || prm.kind[c] != K_FIXED || prm.width[c] > 8)
+    if (prm.npre >= max_pre || cols[c].is_pre || cols[c].out_col < 0 || cols[c].kind != K_FIXED ||
+        cols[c].width > 8)
       return;
     prm.pre_col[prm.npre++] = c;
-    prm.is_pre[c] = 1;
+    cols[c].is_pre = 1;
   };
   if (keyk) add_pre(prm.key_col[0]);
   for (int c = 0; c < nc; c++) add_pre(c);
+  if (wide) {
+    if ((rc = upload(ctx, &h->d_cols, cols.data(), sizeof(SplitCol) * cols.size()))) return bail(rc);
+    prm.col = (const SplitCol*)h->d_cols.p;
+  } else {
+    for (int c = 0; c < nc; c++) prm.acol[c] = cols[c];
+  }
   if (T > 0) {
     KernelTimer kt(ctx, "split_count");
     const std::pair<hipEvent_t, hipEvent_t> ev = pid_event_pair(h);
@@ -1182,10 +1234,12 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
       h->pid_events.push_back(ev);
     }
     size_t lds = sizeof(uint32_t) * R;
-    if (keyk)
-      hipLaunchKernelGGL((split_count_kernel<256, 1>), dim3((unsigned)grid), dim3(256), lds, st, prm);
+    if (wide)
+      hipLaunchKernelGGL((split_count_kernel<256, 0, 1>), dim3((unsigned)grid), dim3(256), lds, st, prm);
+    else if (keyk)
+      hipLaunchKernelGGL((split_count_kernel<256, 1, 0>), dim3((unsigned)grid), dim3(256), lds, st, prm);
     else
-      hipLaunchKernelGGL((split_count_kernel<256, 0>), dim3((unsigned)grid), dim3(256), lds, st, prm);
+      hipLaunchKernelGGL((split_count_kernel<256, 0, 0>), dim3((unsigned)grid), dim3(256), lds, st, prm);
     if (ev.first) hipEventRecord(ev.second, st);
   }
   {
@@ -1203,7 +1257,7 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     }
     std::vector<int32_t*> offp(std::max(nbin, 1), nullptr);
     for (int c = 0; c < nc; c++)
-      if (prm.bin[c] >= 0) offp[prm.bin[c]] = prm.out_off[prm.out_col[c]];
+      if (cols[c].bin >= 0) offp[cols[c].bin] = outs[cols[c].out_col].off;
     if ((rc = upload(ctx, &seg->offtab, offp.data(), sizeof(int32_t*) * offp.size()))) return bail(rc);
     hipLaunchKernelGGL(split_layout_kernel, dim3(1), dim3(256), 0, st, (const uint64_t*)seg->tot.p,
                        P, nbin, (int64_t*)seg->row_start.p, (int64_t*)seg->byte_start.p,
@@ -1219,30 +1273,39 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     data_shared[b] = std::adjacent_find(v.begin(), v.end()) != v.end();
   }
   for (int c = 0; c < nc; c++) {
-    int oc = prm.out_col[c];
+    int oc = cols[c].out_col;
     if (oc < 0 || !type_is_binary(h->types_in[c])) continue;
-    size_t bytes = data_bytes[prm.bin[c]];
-    if (data_shared[prm.bin[c]] || bytes > ((size_t)16 << 30)) {
+    size_t bytes = data_bytes[cols[c].bin];
+    if (data_shared[cols[c].bin] || bytes > ((size_t)16 << 30)) {
       int64_t total = 0;
-      GZ_HIP(hipMemcpyAsync(&total, (int64_t*)seg->byte_start.p + (int64_t)prm.bin[c] * (P + 1) + P,
+      GZ_HIP(hipMemcpyAsync(&total, (int64_t*)seg->byte_start.p + (int64_t)cols[c].bin * (P + 1) + P,
                             sizeof(int64_t), hipMemcpyDeviceToHost, st));
       GZ_HIP(hipStreamSynchronize(st));
       bytes = (size_t)total;
     }
     if ((rc = ctx_alloc(ctx, std::max<size_t>(bytes, 4), &seg->data[oc]))) return bail(rc);
-    prm.out_data[oc] = (uint8_t*)seg->data[oc].p;
+    outs[oc].data = (uint8_t*)seg->data[oc].p;
+  }
+  if (wide) {
+    if ((rc = upload(ctx, &h->d_outs, outs.data(), sizeof(SplitOut) * outs.size()))) return bail(rc);
+    prm.out = (const SplitOut*)h->d_outs.p;
+  } else {
+    for (int oc = 0; oc < (int)outs.size(); oc++) prm.aout[oc] = outs[oc];
   }
   if (T > 0) {
     KernelTimer kt(ctx, "split_scatter");
     const int BLOCK = scatter_block(P);
     prm.cbuf = scatter_cbuf(TILE, nbin);
     size_t lds = scatter_lds(TILE, BLOCK, P, prm.cbuf);
-#define GZ_SCATTER(TL, BL, KK) \
-  hipLaunchKernelGGL((split_scatter_kernel<TL, BL, KK>), dim3((unsigned)grid), dim3(BL), lds, st, prm)
-    if (BLOCK == 512) { if (keyk) GZ_SCATTER(2048, 512, 1); else GZ_SCATTER(2048, 512, 0); }
-    else if (TILE == 4096) { if (keyk) GZ_SCATTER(4096, 1024, 1); else GZ_SCATTER(4096, 1024, 0); }
-    else if (TILE == 2048) { if (keyk) GZ_SCATTER(2048, 1024, 1); else GZ_SCATTER(2048, 1024, 0); }
-    else { if (keyk) GZ_SCATTER(1024, 1024, 1); else GZ_SCATTER(1024, 1024, 0); }
+#define GZ_SCATTER(TL, BL, KK, WD) \
+  hipLaunchKernelGGL((split_scatter_kernel<TL, BL, KK, WD>), dim3((unsigned)grid), dim3(BL), lds, st, prm)
+#define GZ_SCATTER_KW(TL, BL) \
+  if (wide) GZ_SCATTER(TL, BL, 0, 1); else if (keyk) GZ_SCATTER(TL, BL, 1, 0); else GZ_SCATTER(TL, BL, 0, 0)
+    if (BLOCK == 512) { GZ_SCATTER_KW(2048, 512); }
+    else if (TILE == 4096) { GZ_SCATTER_KW(4096, 1024); }
+    else if (TILE == 2048) { GZ_SCATTER_KW(2048, 1024); }
+    else { GZ_SCATTER_KW(1024, 1024); }
+#undef GZ_SCATTER_KW
 #undef GZ_SCATTER
   }
   GZ_HIP(hipGetLastError());
@@ -1383,8 +1446,8 @@ int gz_split_segment_column(gz_splitter* h, int32_t seg, int32_t col, void** val
   if (byte_start) {
     int in_col = -1;
     for (int c = 0; c < h->ncols_in; c++)
-      if (h->base.out_col[c] == col) in_col = c;
-    int j = in_col >= 0 ? h->base.bin[in_col] : -1;
+      if (h->cols[c].out_col == col) in_col = c;
+    int j = in_col >= 0 ? h->cols[in_col].bin : -1;
     if (j >= 0) memcpy(byte_start, &s->h_byte_start[(size_t)j * (h->P + 1)], sizeof(int64_t) * (h->P + 1));
   }
   return GZ_OK;
@@ -1428,8 +1491,7 @@ int gz_split_release(gz_splitter* h) {
   for (auto* s : h->segments) split_release_segment(h->ctx, s);
   stage_release(h->ctx, &h->stage);
   if (h->ipc) ipc_plan_free(h->ctx, h->ipc);
-  ctx_free(h->ctx, &h->d_tiles);
-  ctx_free(h->ctx, &h->d_bufs);
+  for (DBuf* b : {&h->d_tiles, &h->d_bufs, &h->d_cols, &h->d_outs}) ctx_free(h->ctx, b);
   delete h;
   return GZ_OK;
 }
@@ -2031,7 +2093,7 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
   // input column of each output column (range mode drops the pid column)
   std::vector<int> in_of(nout, -1);
   for (int c = 0; c < h->ncols_in; c++)
-    if (h->base.out_col[c] >= 0) in_of[h->base.out_col[c]] = c;
+    if (h->cols[c].out_col >= 0) in_of[h->cols[c].out_col] = c;
   // ---- pass 1: batches, their pieces per segment, validity pieces for null counts
   struct Span {
     int seg;
@@ -2320,7 +2382,7 @@ static int ipc_prepare(gz_splitter* h, const uint8_t* schema, int64_t schema_len
             int64_t bstart = 0;
             {
               int in_col = in_of[c];
-              const int j = in_col >= 0 ? h->base.bin[in_col] : -1;
+              const int j = in_col >= 0 ? h->cols[in_col].bin : -1;
               bstart = sg->h_byte_start[(size_t)j * (P + 1) + bt.p];
             }
             const int64_t o0 = span_bytes[sb * 2], nb2 = span_bytes[sb * 2 + 1];

@@ -223,3 +223,34 @@ def test_split_refuses_unpinned_float_keys(ctx, key_type, bits):
     with pytest.raises(L.NotImplementedStatus):
         sp.Stop()
     sp.close()
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_split_wide_schema_many_keys(ctx, batched):
+    """A wide row (128 columns: every fixed width, bool, utf8 and binary, nulls)
+    hashed on 16 key columns of every hashable type, split into 200
+    partitions, bit-exact with the oracle; per-batch and batched."""
+    base = [T.INT64, T.INT32, T.DOUBLE, T.STRING, T.BOOL, T.INT8, T.UINT16, T.DECIMAL128, T.INT16, T.FLOAT,
+            T.DATE32, T.BINARY, T.UINT64, T.UINT8, T.UINT32, T.NA]
+    types = (base * 8)[:128]
+    keys = [0, 1, 3, 4, 5, 8, 10, 16, 17, 19, 20, 21, 24, 26, 35, 40]
+    assert len(keys) == 16 and all(types[k] in (T.INT64, T.INT32, T.STRING, T.BOOL, T.INT8, T.INT16, T.DATE32)
+                                   for k in keys)
+    batches = random_batches(types, [3000, 1, 4096, 257], seed=29, null_frac=0.1)
+    exp = oracle.split(batches, types, 200, keys)
+    got = run_gpu(ctx, batches, types, 200, keys, batched=batched)
+    assert_layout_equal(got, exp, types)
+
+
+def test_split_limits(ctx):
+    """Beyond the device splitter's limits: NotImplemented, not a wrong split."""
+    from gazelle_plugin_amd import _lib as L
+
+    with pytest.raises(L.NotImplementedStatus):
+        Splitter.Make(ctx, "hash", [T.INT64] * 129, 200, [0])
+    with pytest.raises(L.NotImplementedStatus):
+        Splitter.Make(ctx, "hash", [T.INT64] * 20, 200, list(range(17)))
+    with pytest.raises(L.NotImplementedStatus):
+        Splitter.Make(ctx, "hash", [T.INT64, T.STRING], 4096, [0])
+    with pytest.raises(L.NotImplementedStatus):  # 2048 x (1 + 40) counters exceed the LDS budget
+        Splitter.Make(ctx, "hash", [T.INT64] + [T.STRING] * 40, 2048, [0])
