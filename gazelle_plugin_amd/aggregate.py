@@ -68,7 +68,7 @@ def batch_size() -> int:
 
 @dataclass
 class AggOptions:
-    placement: int = 0        # 0 autotuned, 1 LDS first level, 2 HBM only
+    placement: int = 0        # 0 chosen per instance, 1 LDS first level, 2 HBM only, 3 partitioned
     tile_rows: int = 0
     load_factor: float = 0.5
     expected_groups: int = 0

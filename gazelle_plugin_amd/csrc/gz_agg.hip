@@ -1333,16 +1333,15 @@ __global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, cons
 #ifndef GZ_UPD_RPT
 #define GZ_UPD_RPT 4
 #endif
-#ifndef GZ_UPD_BLOCK
-#define GZ_UPD_BLOCK 512
-#endif
 constexpr int kUpdRPT = GZ_UPD_RPT;      // records per thread per step of the update kernel
-constexpr int kUpdBlock = GZ_UPD_BLOCK;  // its workgroup size (two per CU)
 
 // Workgroups walk fixed chunks of the record array; inside a chunk, each
 // partition segment is aggregated in an LDS table and merged into the HBM table
 // of the segment's instance.
-__global__ __launch_bounds__(kUpdBlock, 2 * kUpdBlock / 256) void agg_part_update_kernel(AggParams p, const int64_t* recs, const int64_t* pstart,
+// UB: the workgroup size (512: two workgroups per CU, 256: four); the
+// autotuner picks one.
+template <int UB>
+__global__ __launch_bounds__(UB, 2 * UB / 256) void agg_part_update_kernel(AggParams p, const int64_t* recs, const int64_t* pstart,
                                                               const int* pinst, int B, int64_t chunk) {
   extern __shared__ int64_t s_tab[];
   const int64_t LS = p.lcap + 3;  // stride: lcap slots, 2 special, 1 trash
@@ -1850,6 +1849,7 @@ struct gz_agg {
   int64_t nrec = 0;
   AggParams p_part;
   unsigned grid_part = 0, blocks_part_upd = 0;
+  int upd_block = 512;      // workgroup size of the partitioned update (512 or 256)
   int64_t part_chunk = 0;
   int lds_part = 0;
 };
@@ -2708,9 +2708,14 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
       q.lcap >>= 1;
       lds = lds_bytes(h, q.lcap);
     }
+    // four 256-thread workgroups per CU share the LDS of two 512-thread ones
+    while (h->upd_block == 256 && lds > 40 * 1024 && q.lcap > 64) {
+      q.lcap >>= 1;
+      lds = lds_bytes(h, q.lcap);
+    }
     h->lds_part = lds;
-    h->blocks_part_upd = (unsigned)std::max<int64_t>(1, std::min<int64_t>((h->nrec + 4095) / 4096,
-                                                                          (int64_t)ctx->num_cus * 2));
+    h->blocks_part_upd = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>((h->nrec + 4095) / 4096, (int64_t)(1024 / h->upd_block) * ctx->num_cus));
     h->part_chunk = (h->nrec + h->blocks_part_upd - 1) / h->blocks_part_upd;
   }
   h->L.p_hbm = p;
@@ -2796,9 +2801,14 @@ static int agg_run_kernels(gz_agg* h) {
                        sizeof(int64_t) * kPartTile * (2 + h->p_part.nu), st, h->p_part,
                        (const uint32_t*)h->tprefix.p, h->R, (const int64_t*)h->pstart.p, (const int*)h->it0.p,
                        (int64_t*)h->recs.p);
-    hipLaunchKernelGGL(agg_part_update_kernel, dim3(h->blocks_part_upd), dim3(kUpdBlock), h->lds_part, st, h->p_part,
-                       (const int64_t*)h->recs.p, (const int64_t*)h->pstart.p, (const int*)h->pinst.p, h->B,
-                       h->part_chunk);
+    if (h->upd_block == 256)
+      hipLaunchKernelGGL(agg_part_update_kernel<256>, dim3(h->blocks_part_upd), dim3(256), h->lds_part, st,
+                         h->p_part, (const int64_t*)h->recs.p, (const int64_t*)h->pstart.p, (const int*)h->pinst.p,
+                         h->B, h->part_chunk);
+    else
+      hipLaunchKernelGGL(agg_part_update_kernel<512>, dim3(h->blocks_part_upd), dim3(512), h->lds_part, st,
+                         h->p_part, (const int64_t*)h->recs.p, (const int64_t*)h->pstart.p, (const int*)h->pinst.p,
+                         h->B, h->part_chunk);
   }
   if (h->L.T_fix)
     hipLaunchKernelGGL(agg_fix_kernel, dim3((unsigned)h->L.T_fix), dim3(256), 0, st, h->L.p_fix);
@@ -2859,73 +2869,98 @@ static int agg_launch(gz_agg* h) {
   return GZ_OK;
 }
 
-// Autotuner: candidate plans (update tile rows x LDS placement threshold x load
-// factor), each prepared and timed over two runs on the given batches; the
-// fastest stays prepared.
+// Autotuner: a coordinate descent over the plan's knobs, each candidate
+// prepared and timed over two runs (after a warm one) on the given batches:
+// starting from the default plan, every dimension in turn (update tile rows,
+// LDS placement threshold, groups per partition, i.e. the partitioned fan-out,
+// table load factor, workgroup size of the partitioned update) tries its other
+// values with the others at their best so far and keeps the fastest.  A
+// candidate whose table overflowed is not eligible.  *chosen: the number of the
+// winning candidate in the order tried (0: the default plan).
 static int agg_autotune(gz_agg* h, int32_t nb, const gz_batch* batches, const int32_t* inst_of, int32_t* chosen) {
-  struct Cand {
-    int tile_rows, threshold, part;
+  struct Knobs {
+    int tile_rows, threshold, part, block;
     double lf;
   };
-  std::vector<Cand> cands;
-  for (int tr : {2048, 8192})
-    for (int th : {0, 256, 1024})
-      for (int part : {0, 512}) cands.push_back({tr, th, part, 0.5});
   gz_context* ctx = h->ctx;
   const gz_agg_options saved = h->opts;
-  const int saved_th = h->lds_threshold, saved_part = h->part_groups;
+  const int saved_th = h->lds_threshold, saved_part = h->part_groups, saved_block = h->upd_block;
+  const bool free_placement = saved.placement == 0;
+  auto apply = [&](const Knobs& k) {
+    h->opts.tile_rows = k.tile_rows;
+    h->opts.load_factor = k.lf;
+    h->upd_block = k.block;
+    if (free_placement) {
+      h->lds_threshold = k.threshold;
+      h->part_groups = k.part;
+    }
+  };
   hipEvent_t e0, e1;
   GZ_HIP(hipEventCreate(&e0));
   GZ_HIP(hipEventCreate(&e1));
-  int best = -1;
+  int rc = GZ_OK, tried = 0, best_id = -1;
   float best_ms = 0;
-  int rc = GZ_OK;
-  for (size_t c = 0; c < cands.size() && rc == GZ_OK; c++) {
-    h->opts.tile_rows = cands[c].tile_rows;
-    h->opts.load_factor = cands[c].lf;
-    if (saved.placement == 0) {
-      h->lds_threshold = cands[c].threshold;
-      h->part_groups = cands[c].part;
-    } else if (c > 0 && (cands[c].threshold != cands[0].threshold || cands[c].part != cands[0].part)) {
-      continue;  // placement fixed by the caller
-    }
-    if ((rc = agg_prepare(h, nb, batches, inst_of))) break;
-    if ((rc = agg_run_kernels(h))) break;  // warm
-    hipEventRecord(e0, ctx->stream);
-    for (int r = 0; r < 2 && rc == GZ_OK; r++) rc = agg_run_kernels(h);
-    hipEventRecord(e1, ctx->stream);
-    if (hipEventSynchronize(e1) != hipSuccess) {
-      rc = fail(GZ_HIP_ERROR, "autotune timing");
-      break;
-    }
-    // a candidate whose table overflowed is not eligible
+  Knobs best{h->opts.tile_rows > 0 ? h->opts.tile_rows : 4096, saved_th, saved_part, saved_block,
+             saved.load_factor > 0 ? saved.load_factor : 0.5};
+  auto time_plan = [&](const Knobs& k, float* ms, bool* ok) -> int {
+    apply(k);
+    GZ_TRY(agg_prepare(h, nb, batches, inst_of));
+    GZ_TRY(agg_run_kernels(h));  // warm
+    GZ_HIP(hipEventRecord(e0, ctx->stream));
+    for (int r = 0; r < 2; r++) GZ_TRY(agg_run_kernels(h));
+    GZ_HIP(hipEventRecord(e1, ctx->stream));
+    GZ_HIP(hipEventSynchronize(e1));
     std::vector<int64_t> g;
     int ovf = 0;
-    if ((rc = read_counts(h, g, &ovf))) break;
+    GZ_TRY(read_counts(h, g, &ovf));
+    GZ_HIP(hipEventElapsedTime(ms, e0, e1));
+    *ok = !ovf;
+    return GZ_OK;
+  };
+  auto consider = [&](const Knobs& k) {
+    if (rc) return;
     float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    if (!ovf && (best < 0 || ms < best_ms)) {
-      best = (int)c;
+    bool ok = false;
+    rc = time_plan(k, &ms, &ok);
+    const int id = tried++;
+    if (rc == GZ_OK && ok && (best_id < 0 || ms < best_ms)) {
+      best_id = id;
       best_ms = ms;
+      best = k;
     }
+  };
+  consider(best);  // the default plan
+  for (int dim = 0; dim < 5 && rc == GZ_OK; dim++) {
+    if (!free_placement && (dim == 1 || dim == 2)) continue;  // placement fixed by the caller
+    const Knobs base = best;
+    std::vector<Knobs> alts;
+    auto with = [&](auto set) {
+      Knobs k = base;
+      set(k);
+      alts.push_back(k);
+    };
+    switch (dim) {
+      case 0: for (int v : {2048, 4096, 8192}) if (v != base.tile_rows) with([&](Knobs& k) { k.tile_rows = v; }); break;
+      case 1: for (int v : {0, 256, 1024}) if (v != base.threshold) with([&](Knobs& k) { k.threshold = v; }); break;
+      case 2: for (int v : {0, 256, 512}) if (v != base.part) with([&](Knobs& k) { k.part = v; }); break;
+      case 3: for (double v : {0.35, 0.5, 0.7}) if (v != base.lf) with([&](Knobs& k) { k.lf = v; }); break;
+      case 4: for (int v : {256, 512}) if (v != base.block) with([&](Knobs& k) { k.block = v; }); break;
+    }
+    for (auto& k : alts) consider(k);
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  if (rc == GZ_OK && best_id < 0) rc = fail(GZ_CAPACITY_ERROR, "no candidate plan fits");
   if (rc) {
     h->opts = saved;
     h->lds_threshold = saved_th;
     h->part_groups = saved_part;
+    h->upd_block = saved_block;
     return rc;
   }
-  if (best < 0) return fail(GZ_CAPACITY_ERROR, "no candidate plan fits");
-  h->opts.tile_rows = cands[best].tile_rows;
-  h->opts.load_factor = cands[best].lf;
-  if (saved.placement == 0) {
-    h->lds_threshold = cands[best].threshold;
-    h->part_groups = cands[best].part;
-  }
+  apply(best);
   GZ_TRY(agg_prepare(h, nb, batches, inst_of));
-  if (chosen) *chosen = best;
+  if (chosen) *chosen = best_id;
   return GZ_OK;
 }
 

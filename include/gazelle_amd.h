@@ -328,8 +328,11 @@ typedef struct {
 } gz_agg_action;
 
 typedef struct {
-  int32_t placement;      /* 0 = autotuned, 1 = LDS group table, 2 = HBM table */
-  int32_t tile_rows;      /* 0 = autotuned */
+  int32_t placement;      /* 0 = chosen per instance from its estimated groups (LDS
+                             first level, partitioned, or HBM table), 1 = LDS group
+                             table, 2 = HBM table, 3 = partitioned */
+  int32_t tile_rows;      /* rows per update tile; 0 = 4096 (512 for per-call updates
+                             of fewer than 2^20 rows) */
   double load_factor;     /* HBM table max load, 0 = 0.5 */
   int64_t expected_groups;/* sizing hint, 0 = unknown */
 } gz_agg_options;
@@ -386,7 +389,8 @@ int gz_agg_set_process(gz_agg* h, int32_t num_batches, const gz_batch* batches,
 int gz_agg_set_num_groups(gz_agg* h, int32_t inst, int64_t* out);
 int gz_agg_set_result_column(gz_agg* h, int32_t inst, int32_t col, void** values,
                              void** valid_bytes);
-/* Placement the autotuner chose for an instance: 1 = LDS first level, 2 = HBM. */
+/* Placement of an instance in the current plan: 1 = LDS first level, 2 = HBM,
+ * 3 = partitioned. */
 int gz_agg_set_placement(gz_agg* h, int32_t inst, int32_t* out);
 /* gz_agg_result_bitmap of one instance. */
 int gz_agg_set_result_bitmap(gz_agg* h, int32_t inst, int32_t col, void** bitmap);
@@ -405,10 +409,12 @@ int gz_agg_set_prepare(gz_agg* h, int32_t num_batches, const gz_batch* batches,
 int gz_agg_set_run(gz_agg* h);
 int gz_agg_set_capture(gz_agg* h);
 int gz_agg_set_launch(gz_agg* h);
-/* Autotuner: prepares and times candidate plans (update tile rows, LDS
- * placement threshold, table load factor) on the given batches and keeps the
- * fastest prepared; *chosen receives its index.  No counterpart in the
- * reference. */
+/* Autotuner: a coordinate descent over the plan's knobs (update tile rows, LDS
+ * placement threshold, groups per partition -- the partitioned fan-out --,
+ * table load factor, workgroup size of the partitioned update), every
+ * candidate prepared and timed on the given batches; the fastest stays
+ * prepared and *chosen receives its number in the order tried (0: the default
+ * plan).  No counterpart in the reference. */
 int gz_agg_set_autotune(gz_agg* h, int32_t num_batches, const gz_batch* batches,
                         const int32_t* instance_of_batch, int32_t* chosen);
 
