@@ -74,6 +74,9 @@ struct AggInst {
   int32_t fan_log2;       // partitioned placement: log2 of the partition count (0 = off)
   int32_t bucket_base;    // first global partition of the instance
   int32_t nl;             // lanes per slot
+  int32_t sub_log2;       // 0: one hash table over all slots; else partition-structured: partition b
+                          // (fan_log2 high hash bits) owns slots [b << sub_log2, (b + 1) << sub_log2),
+                          // probed from the low hash bits within them (an LDS table's layout)
 };
 
 // The HBM table's first-row word and lanes of slot s: keys and first rows are
@@ -298,24 +301,40 @@ __device__ __forceinline__ uint32_t load_keys(const AggParams& p, int batch, int
   return vm;
 }
 
-// Slot of a key in an HBM table (insert if new); -1 when the table is full.
+// Where a key's probe runs in an HBM table: slots base + ((s + i) & mask).
+__device__ __forceinline__ void probe_range(const AggInst& I, int64_t key, uint64_t* base, uint64_t* mask,
+                                            uint64_t* s) {
+  const uint64_t h = fmix64((uint64_t)key);
+  if (I.sub_log2) {
+    *base = (I.fan_log2 ? h >> (64 - I.fan_log2) : 0) << I.sub_log2;
+    *mask = (1ull << I.sub_log2) - 1;
+  } else {
+    *base = 0;
+    *mask = I.cap - 1;
+  }
+  *s = h & *mask;
+}
+
+// Slot of a key in an HBM table (insert if new); -1 when the table (or the
+// key's partition of a structured table) is full.
 __device__ __forceinline__ int64_t table_slot(const AggInst& I, bool valid, int64_t key) {
   if (!valid) return (int64_t)I.cap;
   if (key == kEmpty) return (int64_t)I.cap + 1;
   int64_t* keys = I.base;
-  uint64_t mask = I.cap - 1;
-  uint64_t s = fmix64((uint64_t)key) & mask;
-  for (uint64_t i = 0; i < I.cap; i++) {
-    int64_t k = keys[s];
-    if (k == key) return (int64_t)s;
+  uint64_t base, mask, s;
+  probe_range(I, key, &base, &mask, &s);
+  for (uint64_t i = 0; i <= mask; i++) {
+    const uint64_t q = base + s;
+    int64_t k = keys[q];
+    if (k == key) return (int64_t)q;
     if (k == kEmpty) {
-      unsigned long long prev = atomicCAS((unsigned long long*)&keys[s], (unsigned long long)kEmpty,
+      unsigned long long prev = atomicCAS((unsigned long long*)&keys[q], (unsigned long long)kEmpty,
                                           (unsigned long long)key);
       if ((int64_t)prev == kEmpty) {
         atomicAdd(I.counter, 1u);
-        return (int64_t)s;
+        return (int64_t)q;
       }
-      if ((int64_t)prev == key) return (int64_t)s;
+      if ((int64_t)prev == key) return (int64_t)q;
     }
     s = (s + 1) & mask;
   }
@@ -327,11 +346,11 @@ __device__ __forceinline__ int64_t table_slot(const AggInst& I, bool valid, int6
 __device__ __forceinline__ int64_t table_find(const AggInst& I, bool valid, int64_t key) {
   if (!valid) return (int64_t)I.cap;
   if (key == kEmpty) return (int64_t)I.cap + 1;
-  const uint64_t mask = I.cap - 1;
-  uint64_t s = fmix64((uint64_t)key) & mask;
-  for (uint64_t i = 0; i < I.cap; i++) {
-    const int64_t k = I.base[s];
-    if (k == key) return (int64_t)s;
+  uint64_t base, mask, s;
+  probe_range(I, key, &base, &mask, &s);
+  for (uint64_t i = 0; i <= mask; i++) {
+    const int64_t k = I.base[base + s];
+    if (k == key) return (int64_t)(base + s);
     if (k == kEmpty) return -1;
     s = (s + 1) & mask;
   }
@@ -697,6 +716,52 @@ __device__ void lds_merge(const AggParams& p, const AggInst& I, int64_t* l, int6
     else g = table_slot(I, true, key);
     if (g < 0) continue;
     merge_slot(p, I, g, l, LS, s);
+  }
+}
+
+// An LDS table whose groups no other workgroup updates (all rows of their
+// keys went through this table) written into partition b's slots of a
+// structured HBM table, whose layout it has: slot s of the LDS table is slot
+// (b << sub_log2) + s, so the whole table is stored as it is (empty slots
+// included), with no atomics; the special slots go to cap and cap + 1.
+__device__ void lds_write_image(const AggParams& p, const AggInst& I, int b, const int64_t* l, int64_t LS) {
+  __shared__ unsigned s_keys;
+  if (threadIdx.x == 0) s_keys = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)b << I.sub_log2;
+  unsigned mine = 0;
+  for (int64_t s = threadIdx.x; s < (int64_t)p.lcap + 2; s += blockDim.x) {
+    const bool special = s >= p.lcap;
+    if (special && l[LS + s] == INT64_MAX) continue;
+    const int64_t g = special ? (int64_t)I.cap + (s - p.lcap) : base + s;
+    const int64_t key = l[s];
+    if (!special) {
+      I.base[g] = key;
+      mine += key != kEmpty;
+    }
+    *tab_first(I, g) = l[LS + s];
+    int64_t* ln = tab_lanes(I, g);
+    for (int k = 0; k < p.nl; k++) ln[k] = l[(2 + k) * LS + s];
+  }
+  if (mine) atomicAdd(&s_keys, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_keys) atomicAdd(I.counter, s_keys);
+}
+
+// The same table written by claiming one slot per group (a key whose LDS probe
+// failed already sits in the HBM table, so the image would be wrong), then
+// plain stores of its first row and lanes over the slot's initial values.
+__device__ void lds_writeout(const AggParams& p, const AggInst& I, const int64_t* l, int64_t LS) {
+  for (int64_t s = threadIdx.x; s < (int64_t)p.lcap + 2; s += blockDim.x) {
+    const bool special = s >= p.lcap;
+    const int64_t key = l[s];
+    if (!special && key == kEmpty) continue;
+    if (special && l[LS + s] == INT64_MAX) continue;
+    const int64_t g = special ? (int64_t)I.cap + (s - p.lcap) : table_slot(I, true, key);
+    if (g < 0) continue;
+    *tab_first(I, g) = l[LS + s];
+    int64_t* ln = tab_lanes(I, g);
+    for (int k = 0; k < p.nl; k++) ln[k] = l[(2 + k) * LS + s];
   }
 }
 
@@ -1184,19 +1249,45 @@ __global__ void agg_part_totals_kernel(const uint32_t* tprefix, const uint64_t* 
   pcount[g] = (unsigned int)(hi - tprefix[(int64_t)it0[i] * R + b]);
 }
 
-// One block: partition starts (exclusive scan of the counts, total at [B]).
-__global__ __launch_bounds__(256) void agg_part_scan_kernel(const unsigned int* pcount, int B, int64_t* pstart) {
+// A work item of the partitioned update: records [a, e) of partition b.  An
+// owned item holds the whole partition, so its workgroup is the only writer of
+// the partition's groups; a partition of more than kPartItemRecords records is
+// cut into items of that many, merged with atomics.
+struct PartItem {
+  int32_t b, owned;
+  int64_t a, e;
+};
+constexpr int64_t kPartItemRecords = 16384;
+
+// One block: partition starts (exclusive scan of the counts, total at [B]),
+// then the work items of every partition (their number at *nitems).
+__global__ __launch_bounds__(256) void agg_part_scan_kernel(const unsigned int* pcount, int B, int64_t* pstart,
+                                                            PartItem* items, int64_t* nitems) {
   __shared__ uint64_t s_scan[256 / kWave + 1];
-  uint64_t carry = 0;
+  uint64_t carry = 0, icarry = 0;
   for (int base = 0; base < B; base += 256) {
     int i = base + threadIdx.x;
-    uint64_t v = i < B ? pcount[i] : 0;
+    const uint64_t v = i < B ? pcount[i] : 0;
     uint64_t total;
-    uint64_t ex = block_exclusive_scan<uint64_t>(v, s_scan, &total);
-    if (i < B) pstart[i] = (int64_t)(carry + ex);
+    const uint64_t ex = block_exclusive_scan<uint64_t>(v, s_scan, &total);
+    const uint64_t k = (v + kPartItemRecords - 1) / kPartItemRecords;  // items of partition i
+    uint64_t itotal;
+    const uint64_t iex = block_exclusive_scan<uint64_t>(k, s_scan, &itotal);
+    if (i < B) {
+      const int64_t a0 = (int64_t)(carry + ex);
+      pstart[i] = a0;
+      for (uint64_t j = 0; j < k; j++) {
+        const int64_t a = a0 + (int64_t)j * kPartItemRecords;
+        items[icarry + iex + j] = PartItem{i, k == 1 ? 1 : 0, a, min(a + kPartItemRecords, a0 + (int64_t)v)};
+      }
+    }
     carry += total;
+    icarry += itotal;
   }
-  if (threadIdx.x == 0) pstart[B] = (int64_t)carry;
+  if (threadIdx.x == 0) {
+    pstart[B] = (int64_t)carry;
+    *nitems = (int64_t)icarry;
+  }
 }
 
 // Records of a tile: partition counts in LDS, one range reserved per non-empty
@@ -1335,33 +1426,29 @@ __global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, cons
 #endif
 constexpr int kUpdRPT = GZ_UPD_RPT;      // records per thread per step of the update kernel
 
-// Workgroups walk fixed chunks of the record array; inside a chunk, each
-// partition segment is aggregated in an LDS table and merged into the HBM table
-// of the segment's instance.
+// Workgroups take the work items in turn: an item's records are aggregated in
+// an LDS table, which an owned item (a whole partition) writes out with one
+// slot claim per group and plain stores, and an item of a larger partition
+// merges with atomics.
 // UB: the workgroup size (512: two workgroups per CU, 256: four); the
 // autotuner picks one.
 template <int UB>
-__global__ __launch_bounds__(UB, 2 * UB / 256) void agg_part_update_kernel(AggParams p, const int64_t* recs, const int64_t* pstart,
-                                                              const int* pinst, int B, int64_t chunk) {
+__global__ __launch_bounds__(UB, 2 * UB / 256) void agg_part_update_kernel(AggParams p, const int64_t* recs,
+                                                                           const PartItem* items,
+                                                                           const int64_t* nitems, const int* pinst) {
   extern __shared__ int64_t s_tab[];
+  __shared__ int s_failed;        // a record of the item missed the LDS table
   const int64_t LS = p.lcap + 3;  // stride: lcap slots, 2 special, 1 trash
   const uint64_t lmask = (uint64_t)p.lcap - 1;
   const int64_t NR = p.rstride;
-  const int64_t total = pstart[B];
-  for (int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < total; c0 += (int64_t)gridDim.x * chunk) {
-    const int64_t c1 = min(total, c0 + chunk);
-    // partition holding record c0
-    int lo = 0, hi = B - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (pstart[mid] <= c0) lo = mid; else hi = mid - 1;
-    }
-    for (int b = lo; b < B; b++) {
-      const int64_t a = max(c0, pstart[b]), e = min(c1, pstart[b + 1]);
-      if (a >= c1) break;
-      if (a >= e) continue;
-      const AggInst I = p.inst[pinst[b]];
+  const int64_t ni = *nitems;
+  for (int64_t j = blockIdx.x; j < ni; j += gridDim.x) {
+    {
+      const PartItem it = items[j];
+      const int64_t a = it.a, e = it.e;
+      const AggInst I = p.inst[pinst[it.b]];
       __syncthreads();
+      if (threadIdx.x == 0) s_failed = 0;
       lds_reset(p, s_tab, LS);
       __syncthreads();
       // kUpdRPT records per thread per step, their key, row and first two
@@ -1421,12 +1508,15 @@ __global__ __launch_bounds__(UB, 2 * UB / 256) void agg_part_update_kernel(AggPa
 #pragma unroll
         for (int k = 0; k < kUpdRPT; k++) {
           if (ls[k] != -2) continue;
+          s_failed = 1;
           const int64_t g = table_slot(I, true, kw[k]);
           if (g >= 0) apply_row_hbm(p, I, g, row_of(k), grow_of(k));
         }
       }
       __syncthreads();
-      lds_merge(p, I, s_tab, LS);
+      if (it.owned && I.sub_log2 && !s_failed) lds_write_image(p, I, it.b - I.bucket_base, s_tab, LS);
+      else if (it.owned) lds_writeout(p, I, s_tab, LS);
+      else lds_merge(p, I, s_tab, LS);
     }
   }
 }
@@ -1843,14 +1933,13 @@ struct gz_agg {
   int lds_threshold = 256;  // groups up to which an instance gets the LDS first level
   int part_groups = GZ_PART_GROUPS;  // groups per partition on the partitioned placement (0 = off)
   // partitioned placement buffers of the plan
-  DBuf d_part, pcount, pstart, pinst, recs, tcount, tprefix, tcsum, ttot, it0, it1;
+  DBuf d_part, pcount, pstart, pinst, recs, tcount, tprefix, tcsum, ttot, it0, it1, pitems, pnitems;
   int R = 0;                // largest fan-out (columns of the tile count matrix)
   int B = 0;                // partitions over all partitioned instances
   int64_t nrec = 0;
   AggParams p_part;
   unsigned grid_part = 0, blocks_part_upd = 0;
   int upd_block = 512;      // workgroup size of the partitioned update (512 or 256)
-  int64_t part_chunk = 0;
   int lds_part = 0;
 };
 
@@ -2249,6 +2338,10 @@ static int agg_process(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     GZ_TRY(agg_prepare(h, nb, batches, inst_of));
     return agg_run(h);
   }
+  // tables a plan left partition-structured become one hash table first (the
+  // per-call kernels insert anywhere in it)
+  for (int i = 0; i < h->ninst; i++)
+    if (h->tables[i].p && h->inst[i].sub_log2) GZ_TRY(inst_alloc(h, i, h->inst[i].cap));
   // small calls (one batch at a time) get small tiles so a call still spreads
   // over many workgroups
   int64_t call_rows = 0;
@@ -2615,7 +2708,17 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   if (!std::is_sorted(split.begin(), split.end(), [](const AggTile& a, const AggTile& b) { return a.inst < b.inst; }))
     std::stable_sort(split.begin(), split.end(), [](const AggTile& a, const AggTile& b) { return a.inst < b.inst; });
   if (need_est && !tiles.empty()) GZ_HIP(hipStreamSynchronize(ctx->stream));
-  // tables (reused when large enough: the run resets them) and placement
+  // LDS table of the partitioned update: lcap slots (a power of two), so that
+  // two 512-thread (or four 256-thread) workgroups share a CU
+  int part_lcap = 1024;
+  while (part_lcap > 64 && (lds_bytes(h, part_lcap) > 96 * 1024 ||
+                            (h->upd_block == 256 && lds_bytes(h, part_lcap) > 40 * 1024)))
+    part_lcap >>= 1;
+  int part_sub = 0;
+  while ((1 << part_sub) < part_lcap) part_sub++;
+  // tables (reused when large enough: the run resets them) and placement.  A
+  // partitioned instance's table is partition-structured: its partitions'
+  // slot ranges have the update's LDS table layout (lcap slots each).
   std::vector<int64_t> prefix(h->ninst + 1, 0);
   for (int i = 0; i < h->ninst; i++) {
     double g;
@@ -2623,22 +2726,25 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     else if (h->opts.expected_groups > 0) g = std::min<double>((double)h->opts.expected_groups, (double)rows[i]);
     else if (rows[i] > 4096) g = std::min<double>(est[i] * 1.3 + 64, (double)rows[i]);
     else g = (double)rows[i];
-    uint64_t cap = h->plan.key_col < 0 ? 16 : pow2_at_least((uint64_t)(g / lf) + 16);
-    if (!h->tables[i].p || h->inst[i].cap < cap || h->inst[i].cap > 4 * cap) GZ_TRY(plan_table_alloc(h, i, cap));
-    h->g_bound[i] = 2 + (int64_t)g;
     const bool part_ok = p.nu >= 0 && h->plan.key_col >= 0 && h->part_groups > 0 &&
                          g <= (double)kMaxFanout * h->part_groups;
     if (h->opts.placement == 0)
       h->placement[i] = (g <= (double)h->lds_threshold || h->plan.key_col < 0) ? 1 : part_ok ? 3 : 2;
     else
       h->placement[i] = h->opts.placement == 3 && !part_ok ? 2 : h->opts.placement;
-    h->inst[i].fan_log2 = 0;
-    h->inst[i].bucket_base = 0;
+    int lg = 0;
     if (h->placement[i] == 3) {
-      int lg = 1;
+      lg = 1;
       while (lg < 8 && (double)(1 << lg) * h->part_groups < g) lg++;
-      h->inst[i].fan_log2 = lg;
     }
+    const uint64_t cap = h->plan.key_col < 0 ? 16
+                         : lg ? (uint64_t)part_lcap << lg
+                              : pow2_at_least((uint64_t)(g / lf) + 16);
+    if (!h->tables[i].p || h->inst[i].cap != cap) GZ_TRY(plan_table_alloc(h, i, cap));
+    h->g_bound[i] = 2 + (int64_t)g;
+    h->inst[i].fan_log2 = lg;
+    h->inst[i].sub_log2 = lg ? part_sub : 0;
+    h->inst[i].bucket_base = 0;
     prefix[i + 1] = prefix[i] + (int64_t)h->inst[i].cap + 2;
   }
   // partition numbering over the partitioned instances
@@ -2702,21 +2808,14 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     q.tiles = (const AggTile*)h->d_part.p;
     q.T = (int64_t)part_tiles.size();
     h->grid_part = (unsigned)((q.T + 7) / 8 * 8);
-    q.lcap = 1024;
-    int lds = lds_bytes(h, q.lcap);
-    while (lds > 96 * 1024 && q.lcap > 64) {
-      q.lcap >>= 1;
-      lds = lds_bytes(h, q.lcap);
-    }
-    // four 256-thread workgroups per CU share the LDS of two 512-thread ones
-    while (h->upd_block == 256 && lds > 40 * 1024 && q.lcap > 64) {
-      q.lcap >>= 1;
-      lds = lds_bytes(h, q.lcap);
-    }
-    h->lds_part = lds;
+    q.lcap = part_lcap;  // the structured tables' partition size
+    h->lds_part = lds_bytes(h, q.lcap);
+    // work items: at most one per partition plus one per kPartItemRecords records
+    const int64_t max_items = B + (h->nrec + kPartItemRecords - 1) / kPartItemRecords + 1;
+    GZ_TRY(ctx_ensure(ctx, &h->pitems, sizeof(PartItem) * max_items));
+    GZ_TRY(ctx_ensure(ctx, &h->pnitems, sizeof(int64_t)));
     h->blocks_part_upd = (unsigned)std::max<int64_t>(
-        1, std::min<int64_t>((h->nrec + 4095) / 4096, (int64_t)(1024 / h->upd_block) * ctx->num_cus));
-    h->part_chunk = (h->nrec + h->blocks_part_upd - 1) / h->blocks_part_upd;
+        1, std::min<int64_t>(max_items, (int64_t)(1024 / h->upd_block) * ctx->num_cus));
   }
   h->L.p_hbm = p;
   h->L.p_lds = p;
@@ -2796,19 +2895,19 @@ static int agg_run_kernels(gz_agg* h) {
                        (const AggInst*)h->d_inst.p, (const int*)h->pinst.p, (const int*)h->it0.p,
                        (const int*)h->it1.p, h->B, (unsigned int*)h->pcount.p);
     hipLaunchKernelGGL(agg_part_scan_kernel, dim3(1), dim3(256), 0, st, (const unsigned int*)h->pcount.p, h->B,
-                       (int64_t*)h->pstart.p);
+                       (int64_t*)h->pstart.p, (PartItem*)h->pitems.p, (int64_t*)h->pnitems.p);
     hipLaunchKernelGGL(agg_part_scatter_kernel, dim3(h->grid_part), dim3(256),
                        sizeof(int64_t) * kPartTile * (2 + h->p_part.nu), st, h->p_part,
                        (const uint32_t*)h->tprefix.p, h->R, (const int64_t*)h->pstart.p, (const int*)h->it0.p,
                        (int64_t*)h->recs.p);
     if (h->upd_block == 256)
       hipLaunchKernelGGL(agg_part_update_kernel<256>, dim3(h->blocks_part_upd), dim3(256), h->lds_part, st,
-                         h->p_part, (const int64_t*)h->recs.p, (const int64_t*)h->pstart.p, (const int*)h->pinst.p,
-                         h->B, h->part_chunk);
+                         h->p_part, (const int64_t*)h->recs.p, (const PartItem*)h->pitems.p,
+                         (const int64_t*)h->pnitems.p, (const int*)h->pinst.p);
     else
       hipLaunchKernelGGL(agg_part_update_kernel<512>, dim3(h->blocks_part_upd), dim3(512), h->lds_part, st,
-                         h->p_part, (const int64_t*)h->recs.p, (const int64_t*)h->pstart.p, (const int*)h->pinst.p,
-                         h->B, h->part_chunk);
+                         h->p_part, (const int64_t*)h->recs.p, (const PartItem*)h->pitems.p,
+                         (const int64_t*)h->pnitems.p, (const int*)h->pinst.p);
   }
   if (h->L.T_fix)
     hipLaunchKernelGGL(agg_fix_kernel, dim3((unsigned)h->L.T_fix), dim3(256), 0, st, h->L.p_fix);
@@ -3261,7 +3360,7 @@ int gz_agg_release(gz_agg* h) {
                   &h->svals, &h->tkeys, &h->tvals, &h->cursor, &h->ccpre, &h->counts, &h->sort_tmp, &h->L.d_hbm,
                   &h->L.d_lds, &h->L.d_slot_prefix, &h->L.d_bufs, &h->d_part, &h->pcount, &h->pstart, &h->pinst,
                   &h->recs, &h->tcount, &h->tprefix, &h->tcsum, &h->ttot, &h->it0, &h->it1, &h->d_fix,
-                  &h->d_fix_tiles, &h->d_gbatch, &h->L.d_fix, &h->L.d_gb, &h->d_blob, &h->d_cnt})
+                  &h->d_fix_tiles, &h->d_gbatch, &h->L.d_fix, &h->L.d_gb, &h->d_blob, &h->d_cnt, &h->pitems, &h->pnitems})
     ctx_free(ctx, b);
   for (auto& b : h->res_bits) ctx_free(ctx, &b);
   if (h->L.exec) hipGraphExecDestroy(h->L.exec);
