@@ -32,7 +32,6 @@
 
 #include "gz_device.h"
 #include "gz_internal.h"
-#include "gz_scan.h"
 
 namespace gz {
 
@@ -43,7 +42,8 @@ constexpr int64_t kEmpty = INT64_MIN;
 constexpr int kHllBits = 11;
 constexpr int kHllRegs = 1 << kHllBits;
 constexpr int kMaxUsed = 8;        // used input columns of a partition record
-constexpr int kMaxFanout = 256;    // partitions per instance on the partitioned path
+constexpr int kMaxFanout = 4096;   // partitions per instance on the partitioned path
+constexpr int kSmallFanout = 256;  // fan-outs up to this share a scatter launch with a small LDS footprint
 
 // Floating min/max keep one input value per group, as the reference's
 // accumulators do (MinAction / MaxAction, ext/actions_impl.cc:905-936,
@@ -74,6 +74,9 @@ struct AggInst {
   int32_t fan_log2;       // partitioned placement: log2 of the partition count (0 = off)
   int32_t bucket_base;    // first global partition of the instance
   int32_t nl;             // lanes per slot
+  int64_t cnt_off;        // partitioned: the instance's [partition][tile] record counts start here
+  int32_t cnt_tiles;      // ... its partition tiles (a count row per partition)
+  int32_t tile0;          // ... and its first partition tile
   int32_t sub_log2;       // 0: one hash table over all slots; else partition-structured: partition b
                           // (fan_log2 high hash bits) owns slots [b << sub_log2, (b + 1) << sub_log2),
                           // probed from the low hash bits within them (an LDS table's layout)
@@ -114,6 +117,7 @@ struct AggParams {
   int ucol[kMaxUsed];         // column of used slot k
   int uslot[kAggMaxCols];     // used slot of a column
   int64_t rstride;            // records: word w of record q at recs[w * rstride + q]
+  int64_t tile_base;          // partition tiles: index of tiles[0] among all partition tiles
   int* fix_flag;              // set once a NaN reaches a lane whose bits a result keeps (null: no such lane)
 };
 
@@ -1207,46 +1211,34 @@ __global__ __launch_bounds__(256) void join_agg_fix_kernel(AggParams p, JoinAggP
 // null), then the used input columns widened to 8 bytes.
 constexpr int kRecKeyNull = 15;
 constexpr int kPartTile = 1024;       // rows per tile of the partition kernels
-constexpr int kPartScanChunk = 16;    // tiles per chunk of their scan
 
 __device__ __forceinline__ int part_of(const AggInst& I, bool valid, int64_t key) {
   if (!valid || I.fan_log2 == 0) return 0;
   return (int)(fmix64((uint64_t)key) >> (64 - I.fan_log2));
 }
 
-// Rows per partition of every tile (LDS histogram), written as the tile's row
-// of a [T][R] count matrix (R = the largest fan-out; columns past the tile's
-// instance's fan-out are zero).  No global atomics: partition starts come from a
-// scan of this matrix (tiles of one instance are contiguous).
-__global__ __launch_bounds__(256) void agg_part_hist_kernel(AggParams p, uint32_t* tcount, int R) {
+// Rows per partition of every tile (LDS histogram), written partition-major:
+// instance i's counts are a [F][tiles] block at cnt_off, so one exclusive scan
+// over all blocks gives every (partition, tile) its first record, with the
+// records of a partition contiguous and tile-ordered.  No global atomics.
+__global__ __launch_bounds__(256) void agg_part_hist_kernel(AggParams p, uint32_t* cnt) {
   __shared__ unsigned int s_h[kMaxFanout];
   int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
   if (t < 0) return;
   const AggTile td = p.tiles[t];
   const AggInst I = p.inst[td.inst];
+  const int F = 1 << I.fan_log2;
+  const int64_t tl = p.tile_base + t - I.tile0;
   constexpr int RPT = kPartTile / 256;
   int64_t key[RPT];
   const uint32_t vm = load_keys<RPT, 256>(p, td.batch, td.row_begin, threadIdx.x, td.nrows, key);
-  for (int b = threadIdx.x; b < R; b += blockDim.x) s_h[b] = 0;
+  for (int b = threadIdx.x; b < F; b += blockDim.x) s_h[b] = 0;
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < RPT; k++)
     if (k * 256 + (int)threadIdx.x < td.nrows) atomicAdd(&s_h[part_of(I, (vm >> k) & 1, key[k])], 1u);
   __syncthreads();
-  for (int b = threadIdx.x; b < R; b += blockDim.x) gst(tcount + t * R + b, s_h[b]);
-}
-
-// Rows of every global partition g = (instance, b), from the tile prefixes at
-// the instance's first and one-past-last tile.
-__global__ void agg_part_totals_kernel(const uint32_t* tprefix, const uint64_t* tot, int R, int64_t T,
-                                       const AggInst* inst, const int* pinst, const int* it0, const int* it1, int B,
-                                       unsigned int* pcount) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= B) return;
-  const int i = pinst[g];
-  const int b = g - inst[i].bucket_base;
-  const uint64_t hi = it1[i] < T ? (uint64_t)tprefix[(int64_t)it1[i] * R + b] : tot[b];
-  pcount[g] = (unsigned int)(hi - tprefix[(int64_t)it0[i] * R + b]);
+  for (int b = threadIdx.x; b < F; b += blockDim.x) gst(cnt + I.cnt_off + (int64_t)b * I.cnt_tiles + tl, s_h[b]);
 }
 
 // A work item of the partitioned update: records [a, e) of partition b.  An
@@ -1259,34 +1251,34 @@ struct PartItem {
 };
 constexpr int64_t kPartItemRecords = 16384;
 
-// One block: partition starts (exclusive scan of the counts, total at [B]),
-// then the work items of every partition (their number at *nitems).
-__global__ __launch_bounds__(256) void agg_part_scan_kernel(const unsigned int* pcount, int B, int64_t* pstart,
-                                                            PartItem* items, int64_t* nitems) {
-  __shared__ uint64_t s_scan[256 / kWave + 1];
-  uint64_t carry = 0, icarry = 0;
-  for (int base = 0; base < B; base += 256) {
-    int i = base + threadIdx.x;
-    const uint64_t v = i < B ? pcount[i] : 0;
-    uint64_t total;
-    const uint64_t ex = block_exclusive_scan<uint64_t>(v, s_scan, &total);
-    const uint64_t k = (v + kPartItemRecords - 1) / kPartItemRecords;  // items of partition i
-    uint64_t itotal;
-    const uint64_t iex = block_exclusive_scan<uint64_t>(k, s_scan, &itotal);
-    if (i < B) {
-      const int64_t a0 = (int64_t)(carry + ex);
-      pstart[i] = a0;
-      for (uint64_t j = 0; j < k; j++) {
-        const int64_t a = a0 + (int64_t)j * kPartItemRecords;
-        items[icarry + iex + j] = PartItem{i, k == 1 ? 1 : 0, a, min(a + kPartItemRecords, a0 + (int64_t)v)};
-      }
+// First record of global partition g = (instance pinst[g], partition b): the
+// scanned count of its first tile; past the last partition, nrec.
+__device__ __forceinline__ int64_t part_first(const uint64_t* prefix, const AggInst* inst, const int* pinst,
+                                              int B, int64_t nrec, int g) {
+  if (g >= B) return nrec;
+  const AggInst& I = inst[pinst[g]];
+  return (int64_t)prefix[I.cnt_off + (int64_t)(g - I.bucket_base) * I.cnt_tiles];
+}
+
+// The work items: item g < B is partition g's first (an empty partition's is
+// empty); a partition of more than kPartItemRecords records appends its other
+// items after the first B, counted at *extra (zero before the launch).  One
+// thread per partition.
+__global__ __launch_bounds__(256) void agg_part_items_kernel(const uint64_t* prefix, const AggInst* inst,
+                                                             const int* pinst, int B, int64_t nrec, PartItem* items,
+                                                             unsigned long long* extra) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= B) return;
+  const int64_t a0 = part_first(prefix, inst, pinst, B, nrec, g);
+  const int64_t v = part_first(prefix, inst, pinst, B, nrec, g + 1) - a0;
+  const int64_t k = (v + kPartItemRecords - 1) / kPartItemRecords;  // items of partition g
+  items[g] = PartItem{g, k <= 1 ? 1 : 0, a0, a0 + min(v, kPartItemRecords)};
+  if (k > 1) {
+    const int64_t at = B + (int64_t)atomicAdd(extra, (unsigned long long)(k - 1));
+    for (int64_t j = 1; j < k; j++) {
+      const int64_t a = a0 + j * kPartItemRecords;
+      items[at + j - 1] = PartItem{g, 0, a, min(a + kPartItemRecords, a0 + v)};
     }
-    carry += total;
-    icarry += itotal;
-  }
-  if (threadIdx.x == 0) {
-    pstart[B] = (int64_t)carry;
-    *nitems = (int64_t)icarry;
   }
 }
 
@@ -1298,14 +1290,17 @@ __global__ __launch_bounds__(256) void agg_part_scan_kernel(const unsigned int* 
 // the ranking; the other used columns after it.
 constexpr int kPreU = 2;
 
-__global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, const uint32_t* tprefix, int R,
-                                                               const int64_t* pstart, const int* it0,
+// Dynamic LDS of the scatter for fan-outs up to fcap: the tile's records, then
+// per partition its first record, rows and tile offset, then the partition of
+// every sorted slot.
+__host__ __device__ constexpr size_t scatter_lds(int W, int fcap) {
+  return sizeof(int64_t) * (size_t)W * kPartTile + sizeof(uint64_t) * fcap + sizeof(uint32_t) * (2 * fcap + 1) +
+         sizeof(uint16_t) * kPartTile;
+}
+
+__global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, const uint64_t* prefix, int fcap,
                                                                int64_t* recs) {
-  extern __shared__ int64_t s_rec[];  // [W][kPartTile]
-  __shared__ unsigned int s_h[kMaxFanout];
-  __shared__ unsigned int s_off[kMaxFanout + 1];
-  __shared__ unsigned long long s_base[kMaxFanout];
-  __shared__ unsigned char s_pt[kPartTile];
+  extern __shared__ int64_t s_rec[];  // [W][kPartTile], then the per-partition arrays
   __shared__ uint64_t s_scan[256 / kWave + 1];
   int64_t t = xcd_tile(blockIdx.x, gridDim.x, p.T);
   if (t < 0) return;
@@ -1313,6 +1308,11 @@ __global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, cons
   const AggInst I = p.inst[td.inst];
   const int F = 1 << I.fan_log2;
   const int W = 2 + p.nu;
+  unsigned long long* s_base = (unsigned long long*)(s_rec + (size_t)W * kPartTile);
+  unsigned int* s_h = (unsigned int*)(s_base + fcap);
+  unsigned int* s_off = s_h + fcap;
+  uint16_t* s_pt = (uint16_t*)(s_off + fcap + 1);
+  const int64_t tl = p.tile_base + t - I.tile0;
   constexpr int RPT = kPartTile / 256;
   const int tid = threadIdx.x, n = td.nrows;
   const int64_t rb = td.row_begin;
@@ -1351,16 +1351,21 @@ __global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, cons
   }
   __syncthreads();
   {
-    // tile offsets of the partitions (F <= 256: one per thread)
-    const uint64_t c = tid < F ? s_h[tid] : 0;
-    uint64_t total;
-    const uint64_t ex = block_exclusive_scan<uint64_t>(c, s_scan, &total);
-    if (tid < F) {
-      s_off[tid] = (unsigned)ex;
-      s_base[tid] = (unsigned long long)pstart[I.bucket_base + tid] + tprefix[t * R + tid] -
-                    tprefix[(int64_t)it0[td.inst] * R + tid];
+    // tile offsets of the partitions (256 at a time) and each partition's first
+    // record for this tile
+    uint64_t carry = 0;
+    for (int b0 = 0; b0 < F; b0 += 256) {
+      const int b = b0 + tid;
+      const uint64_t c = b < F ? s_h[b] : 0;
+      uint64_t total;
+      const uint64_t ex = block_exclusive_scan<uint64_t>(c, s_scan, &total);
+      if (b < F) {
+        s_off[b] = (unsigned)(carry + ex);
+        s_base[b] = (unsigned long long)prefix[I.cnt_off + (int64_t)b * I.cnt_tiles + tl];
+      }
+      carry += total;
     }
-    if (tid == 0) s_off[F] = (unsigned)total;
+    if (tid == 0) s_off[F] = (unsigned)carry;
   }
   __syncthreads();
   int slot[RPT];
@@ -1378,7 +1383,7 @@ __global__ __launch_bounds__(256) void agg_part_scatter_kernel(AggParams p, cons
 #pragma unroll
       for (int u = 0; u < kPreU; u++)
         if (u < p.nu) s_rec[(2 + u) * kPartTile + slot[k]] = ((pvalid[u] >> k) & 1) ? pre[u][k] : 0;
-      s_pt[slot[k]] = (unsigned char)part[k];
+      s_pt[slot[k]] = (uint16_t)part[k];
     }
   }
   for (int u = kPreU; u < p.nu; u++) {
@@ -1434,18 +1439,20 @@ constexpr int kUpdRPT = GZ_UPD_RPT;      // records per thread per step of the u
 // autotuner picks one.
 template <int UB>
 __global__ __launch_bounds__(UB, 2 * UB / 256) void agg_part_update_kernel(AggParams p, const int64_t* recs,
-                                                                           const PartItem* items,
-                                                                           const int64_t* nitems, const int* pinst) {
+                                                                           const PartItem* items, int B,
+                                                                           const unsigned long long* extra,
+                                                                           const int* pinst) {
   extern __shared__ int64_t s_tab[];
   __shared__ int s_failed;        // a record of the item missed the LDS table
   const int64_t LS = p.lcap + 3;  // stride: lcap slots, 2 special, 1 trash
   const uint64_t lmask = (uint64_t)p.lcap - 1;
   const int64_t NR = p.rstride;
-  const int64_t ni = *nitems;
+  const int64_t ni = B + (int64_t)*extra;
   for (int64_t j = blockIdx.x; j < ni; j += gridDim.x) {
     {
       const PartItem it = items[j];
       const int64_t a = it.a, e = it.e;
+      if (a == e) continue;  // an empty partition: its slots keep their initial values
       const AggInst I = p.inst[pinst[it.b]];
       __syncthreads();
       if (threadIdx.x == 0) s_failed = 0;
@@ -1682,30 +1689,41 @@ __global__ void agg_inst_counts_kernel(const AggInst* inst, const unsigned long 
 }
 
 // First-seen order for few groups: the sort keys are distinct, so a key's
-// place is the number of smaller keys.  64 keys per workgroup, each of its four
-// waves counting over a quarter of all keys (read at one address per step by
-// the whole wave).
+// place is the number of smaller keys.  A workgroup ranks 64 keys (one per
+// lane); its eight waves split all keys between them and go through their share
+// 64 at a time: each lane loads one key, and the wave compares its own keys
+// with all 64 via readlane (no memory access per comparison).
 constexpr int64_t kRankSortMax = 16384;
+constexpr int kRankWaves = 8;
 
-__global__ __launch_bounds__(256) void agg_rank_sort_kernel(const unsigned long long* __restrict__ keys,
-                                                            const unsigned long long* __restrict__ vals, int64_t n,
-                                                            unsigned long long* skeys, unsigned long long* svals) {
-  __shared__ unsigned s_c[4][64];
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int k) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, k);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), k);
+  return (unsigned long long)hi << 32 | lo;
+}
+
+__global__ __launch_bounds__(64 * kRankWaves) void agg_rank_sort_kernel(const unsigned long long* __restrict__ keys,
+                                                                        const unsigned long long* __restrict__ vals,
+                                                                        int64_t n, unsigned long long* skeys,
+                                                                        unsigned long long* svals) {
+  __shared__ unsigned s_c[kRankWaves][64];
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * 64 + l;
   const unsigned long long mine = i < n ? keys[i] : ~0ull;
-  const int64_t q0 = n * w / 4, q1 = n * (w + 1) / 4;
+  const int64_t q0 = n * w / kRankWaves, q1 = n * (w + 1) / kRankWaves;
   unsigned c = 0;
-  int64_t j = q0;
-  for (; j + 4 <= q1; j += 4) {
-    const unsigned long long a = keys[j], b = keys[j + 1], d = keys[j + 2], e = keys[j + 3];
-    c += (a < mine) + (b < mine) + (d < mine) + (e < mine);
+  for (int64_t j0 = q0; j0 < q1; j0 += 64) {
+    // keys past the share read as all ones: never below a key (keys are below 2^63)
+    const unsigned long long kk = j0 + l < q1 ? keys[j0 + l] : ~0ull;
+#pragma unroll
+    for (int k = 0; k < 64; k++) c += readlane_u64(kk, k) < mine ? 1u : 0u;
   }
-  for (; j < q1; j++) c += keys[j] < mine;
   s_c[w][l] = c;
   __syncthreads();
   if (w == 0 && i < n) {
-    const unsigned r = s_c[0][l] + s_c[1][l] + s_c[2][l] + s_c[3][l];
+    unsigned r = 0;
+#pragma unroll
+    for (int x = 0; x < kRankWaves; x++) r += s_c[x][l];
     skeys[r] = mine;
     svals[r] = vals[i];
   }
@@ -1933,8 +1951,11 @@ struct gz_agg {
   int lds_threshold = 256;  // groups up to which an instance gets the LDS first level
   int part_groups = GZ_PART_GROUPS;  // groups per partition on the partitioned placement (0 = off)
   // partitioned placement buffers of the plan
-  DBuf d_part, pcount, pstart, pinst, recs, tcount, tprefix, tcsum, ttot, it0, it1, pitems, pnitems;
-  int R = 0;                // largest fan-out (columns of the tile count matrix)
+  DBuf d_part, pinst, recs, pcnt, pprefix, pscan_tmp, pitems, pnitems;
+  int64_t ncnt = 0;         // [partition][tile] counts over the partitioned instances
+  size_t pscan_bytes = 0;   // temporary storage of their scan
+  int64_t part_small_tiles = 0;                 // tiles of the instances with fan-out <= kSmallFanout (first)
+  int part_fcap_small = 0, part_fcap_large = 0;  // largest fan-out of either group
   int B = 0;                // partitions over all partitioned instances
   int64_t nrec = 0;
   AggParams p_part;
@@ -2573,7 +2594,7 @@ static int agg_finish(gz_agg* h) {
   // sort by (instance, first row): first-seen order inside each instance
   if (n > 0 && n <= kRankSortMax) {
     KernelTimer kt(ctx, "agg_sort");
-    hipLaunchKernelGGL(agg_rank_sort_kernel, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(agg_rank_sort_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64 * kRankWaves), 0, ctx->stream,
                        (const unsigned long long*)h->tkeys.p, (const unsigned long long*)h->tvals.p, n,
                        (unsigned long long*)h->skeys.p, (unsigned long long*)h->svals.p);
   } else if (n > 0) {
@@ -2720,23 +2741,26 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   // partitioned instance's table is partition-structured: its partitions'
   // slot ranges have the update's LDS table layout (lcap slots each).
   std::vector<int64_t> prefix(h->ninst + 1, 0);
+  std::vector<int64_t> ptiles(h->ninst, 0);  // partition tiles per instance
+  for (auto& t : split) ptiles[t.inst]++;
   for (int i = 0; i < h->ninst; i++) {
     double g;
     if (rows[i] == 0) g = 1;
     else if (h->opts.expected_groups > 0) g = std::min<double>((double)h->opts.expected_groups, (double)rows[i]);
     else if (rows[i] > 4096) g = std::min<double>(est[i] * 1.3 + 64, (double)rows[i]);
     else g = (double)rows[i];
+    // the partitioned placement: up to kMaxFanout partitions, and a bounded
+    // [partition][tile] count block
+    int lg = 1;
+    while (lg < 12 && (double)(1 << lg) * h->part_groups < g) lg++;
     const bool part_ok = p.nu >= 0 && h->plan.key_col >= 0 && h->part_groups > 0 &&
-                         g <= (double)kMaxFanout * h->part_groups;
+                         g <= (double)kMaxFanout * h->part_groups && ptiles[i] > 0 &&
+                         ((int64_t)ptiles[i] << lg) <= (1ll << 26);
     if (h->opts.placement == 0)
       h->placement[i] = (g <= (double)h->lds_threshold || h->plan.key_col < 0) ? 1 : part_ok ? 3 : 2;
     else
       h->placement[i] = h->opts.placement == 3 && !part_ok ? 2 : h->opts.placement;
-    int lg = 0;
-    if (h->placement[i] == 3) {
-      lg = 1;
-      while (lg < 8 && (double)(1 << lg) * h->part_groups < g) lg++;
-    }
+    if (h->placement[i] != 3) lg = 0;
     const uint64_t cap = h->plan.key_col < 0 ? 16
                          : lg ? (uint64_t)part_lcap << lg
                               : pow2_at_least((uint64_t)(g / lf) + 16);
@@ -2747,16 +2771,47 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     h->inst[i].bucket_base = 0;
     prefix[i + 1] = prefix[i] + (int64_t)h->inst[i].cap + 2;
   }
-  // partition numbering over the partitioned instances
+  // partitioned instances, small fan-outs first (their tiles form a scatter
+  // launch with a small LDS footprint): partition numbering, their tiles in
+  // that order, and each one's [partition][tile] count block
+  std::vector<int> porder;
+  for (int i = 0; i < h->ninst; i++)
+    if (h->placement[i] == 3) porder.push_back(i);
+  std::stable_sort(porder.begin(), porder.end(), [&](int x, int y) {
+    return (h->inst[x].fan_log2 > 8) < (h->inst[y].fan_log2 > 8);
+  });
+  std::vector<int> rank(h->ninst, -1);
+  for (size_t k = 0; k < porder.size(); k++) rank[porder[k]] = (int)k;
+  std::vector<AggTile> part_tiles;
+  for (auto& t : split)
+    if (h->placement[t.inst] == 3) part_tiles.push_back(t);
+  std::stable_sort(part_tiles.begin(), part_tiles.end(),
+                   [&](const AggTile& x, const AggTile& y) { return rank[x.inst] < rank[y.inst]; });
   int B = 0;
+  int64_t ncnt = 0, tpos = 0;
   std::vector<int> pinst;
-  for (int i = 0; i < h->ninst; i++) {
-    if (h->placement[i] != 3) continue;
-    h->inst[i].bucket_base = B;
-    for (int b = 0; b < (1 << h->inst[i].fan_log2); b++) pinst.push_back(i);
-    B += 1 << h->inst[i].fan_log2;
+  h->part_small_tiles = 0;
+  h->part_fcap_small = h->part_fcap_large = 0;
+  for (int i : porder) {
+    AggInst& I = h->inst[i];
+    const int F = 1 << I.fan_log2;
+    I.bucket_base = B;
+    for (int b = 0; b < F; b++) pinst.push_back(i);
+    B += F;
+    I.tile0 = (int32_t)tpos;
+    I.cnt_tiles = (int32_t)ptiles[i];
+    I.cnt_off = ncnt;
+    ncnt += (int64_t)F * ptiles[i];
+    tpos += ptiles[i];
+    if (F <= kSmallFanout) {
+      h->part_small_tiles += ptiles[i];
+      h->part_fcap_small = std::max(h->part_fcap_small, F);
+    } else {
+      h->part_fcap_large = std::max(h->part_fcap_large, F);
+    }
   }
   h->B = B;
+  h->ncnt = ncnt;
   GZ_TRY(ub.add(ctx, &h->d_inst, h->inst.data(), sizeof(AggInst) * h->inst.size()));
   h->inst_dirty = false;
   GZ_TRY(ub.add(ctx, &h->L.d_slot_prefix, prefix.data(), sizeof(int64_t) * prefix.size()));
@@ -2766,41 +2821,24 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
   GZ_TRY(ub.add(ctx, &h->d_lane_ops, ops.data(), sizeof(int) * ops.size()));
   p.inst = (const AggInst*)h->d_inst.p;
   // launch plan
-  std::vector<AggTile> lds_tiles, hbm_tiles, part_tiles;
-  bool all_part = true;
+  std::vector<AggTile> lds_tiles, hbm_tiles;
   for (auto& t : tiles) {
     if (h->placement[t.inst] == 3) continue;
-    all_part = false;
     (h->placement[t.inst] == 1 ? lds_tiles : hbm_tiles).push_back(t);
-  }
-  if (all_part) {
-    part_tiles.swap(split);
-  } else {
-    for (auto& t : split)
-      if (h->placement[t.inst] == 3) part_tiles.push_back(t);
   }
   h->grid_part = 0;
   h->nrec = 0;
   if (!part_tiles.empty()) {
-    std::vector<int> t0(h->ninst, 0), t1(h->ninst, 0);
-    for (int64_t t = (int64_t)part_tiles.size() - 1; t >= 0; t--) t0[part_tiles[t].inst] = (int)t;
-    for (int64_t t = 0; t < (int64_t)part_tiles.size(); t++) t1[part_tiles[t].inst] = (int)t + 1;
-    h->R = 0;
-    for (int i = 0; i < h->ninst; i++)
-      if (h->placement[i] == 3) h->R = std::max(h->R, 1 << h->inst[i].fan_log2);
-    const int64_t TP = (int64_t)part_tiles.size();
-    const int64_t C = (TP + kPartScanChunk - 1) / kPartScanChunk;
-    GZ_TRY(ub.add(ctx, &h->it0, t0.data(), sizeof(int) * t0.size()));
-    GZ_TRY(ub.add(ctx, &h->it1, t1.data(), sizeof(int) * t1.size()));
-    GZ_TRY(ctx_ensure(ctx, &h->tcount, sizeof(uint32_t) * TP * h->R));
-    GZ_TRY(ctx_ensure(ctx, &h->tprefix, sizeof(uint32_t) * TP * h->R));
-    GZ_TRY(ctx_ensure(ctx, &h->tcsum, sizeof(uint64_t) * C * h->R));
-    GZ_TRY(ctx_ensure(ctx, &h->ttot, sizeof(uint64_t) * h->R));
     for (auto& t : part_tiles) h->nrec += t.nrows;
     GZ_TRY(ub.add(ctx, &h->d_part, part_tiles.data(), sizeof(AggTile) * part_tiles.size()));
     GZ_TRY(ub.add(ctx, &h->pinst, pinst.data(), sizeof(int) * pinst.size()));
-    GZ_TRY(ctx_ensure(ctx, &h->pcount, sizeof(unsigned) * B));
-    GZ_TRY(ctx_ensure(ctx, &h->pstart, sizeof(int64_t) * (B + 1)));
+    GZ_TRY(ctx_ensure(ctx, &h->pcnt, sizeof(uint32_t) * ncnt));
+    GZ_TRY(ctx_ensure(ctx, &h->pprefix, sizeof(uint64_t) * ncnt));
+    size_t tmp = 0;
+    GZ_HIP(rocprim::exclusive_scan(nullptr, tmp, (const uint32_t*)h->pcnt.p, (uint64_t*)h->pprefix.p, (uint64_t)0,
+                                   (size_t)ncnt, rocprim::plus<uint64_t>(), ctx->stream));
+    GZ_TRY(ctx_ensure(ctx, &h->pscan_tmp, std::max<size_t>(tmp, 16)));
+    h->pscan_bytes = tmp;
     GZ_TRY(ctx_ensure(ctx, &h->recs, sizeof(int64_t) * (2 + p.nu) * h->nrec));
     AggParams& q = h->p_part;
     q = p;
@@ -2810,7 +2848,7 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     h->grid_part = (unsigned)((q.T + 7) / 8 * 8);
     q.lcap = part_lcap;  // the structured tables' partition size
     h->lds_part = lds_bytes(h, q.lcap);
-    // work items: at most one per partition plus one per kPartItemRecords records
+    // work items: one per partition plus one per further kPartItemRecords records
     const int64_t max_items = B + (h->nrec + kPartItemRecords - 1) / kPartItemRecords + 1;
     GZ_TRY(ctx_ensure(ctx, &h->pitems, sizeof(PartItem) * max_items));
     GZ_TRY(ctx_ensure(ctx, &h->pnitems, sizeof(int64_t)));
@@ -2879,35 +2917,36 @@ static int agg_run_kernels(gz_agg* h) {
   if (h->L.blocks_lds)
     hipLaunchKernelGGL(agg_update_lds_kernel, dim3(h->L.blocks_lds), dim3(256), h->L.lds, st, h->L.p_lds);
   if (h->grid_part) {
-    const int64_t TP = h->p_part.T;
-    const int64_t C = (TP + kPartScanChunk - 1) / kPartScanChunk;
-    hipLaunchKernelGGL(agg_part_hist_kernel, dim3(h->grid_part), dim3(256), 0, st, h->p_part,
-                       (uint32_t*)h->tcount.p, h->R);
-    hipLaunchKernelGGL(tile_scan_sum_kernel<kPartScanChunk>, dim3((unsigned)C), dim3(256), 0, st,
-                       (const uint32_t*)h->tcount.p, h->R, TP, (uint64_t*)h->tcsum.p);
-    hipLaunchKernelGGL(tile_scan_chunks_kernel, dim3((unsigned)h->R), dim3(256), 0, st,
-                       (uint64_t*)h->tcsum.p, h->R, C, (uint64_t*)h->ttot.p);
-    hipLaunchKernelGGL(tile_scan_write_kernel<kPartScanChunk>, dim3((unsigned)C), dim3(256), 0, st,
-                       (const uint32_t*)h->tcount.p, (const uint64_t*)h->tcsum.p, h->R, TP,
-                       (uint32_t*)h->tprefix.p);
-    hipLaunchKernelGGL(agg_part_totals_kernel, dim3((unsigned)((h->B + 255) / 256)), dim3(256), 0, st,
-                       (const uint32_t*)h->tprefix.p, (const uint64_t*)h->ttot.p, h->R, TP,
-                       (const AggInst*)h->d_inst.p, (const int*)h->pinst.p, (const int*)h->it0.p,
-                       (const int*)h->it1.p, h->B, (unsigned int*)h->pcount.p);
-    hipLaunchKernelGGL(agg_part_scan_kernel, dim3(1), dim3(256), 0, st, (const unsigned int*)h->pcount.p, h->B,
-                       (int64_t*)h->pstart.p, (PartItem*)h->pitems.p, (int64_t*)h->pnitems.p);
-    hipLaunchKernelGGL(agg_part_scatter_kernel, dim3(h->grid_part), dim3(256),
-                       sizeof(int64_t) * kPartTile * (2 + h->p_part.nu), st, h->p_part,
-                       (const uint32_t*)h->tprefix.p, h->R, (const int64_t*)h->pstart.p, (const int*)h->it0.p,
-                       (int64_t*)h->recs.p);
+    const int64_t TP = h->p_part.T, TA = h->part_small_tiles;
+    const int W = 2 + h->p_part.nu;
+    hipLaunchKernelGGL(agg_part_hist_kernel, dim3(h->grid_part), dim3(256), 0, st, h->p_part, (uint32_t*)h->pcnt.p);
+    GZ_HIP(rocprim::exclusive_scan(h->pscan_tmp.p, h->pscan_bytes, (const uint32_t*)h->pcnt.p,
+                                   (uint64_t*)h->pprefix.p, (uint64_t)0, (size_t)h->ncnt, rocprim::plus<uint64_t>(),
+                                   st));
+    hipLaunchKernelGGL(fill_u32_kernel_agg, dim3(1), dim3(64), 0, st, (uint32_t*)h->pnitems.p, (int64_t)2);
+    hipLaunchKernelGGL(agg_part_items_kernel, dim3((unsigned)((h->B + 255) / 256)), dim3(256), 0, st,
+                       (const uint64_t*)h->pprefix.p, (const AggInst*)h->d_inst.p, (const int*)h->pinst.p, h->B,
+                       h->nrec, (PartItem*)h->pitems.p, (unsigned long long*)h->pnitems.p);
+    // scatter: the small fan-outs' tiles, then the large ones' (per-partition LDS arrays sized for each)
+    for (int grp = 0; grp < 2; grp++) {
+      AggParams q = h->p_part;
+      const int64_t t0 = grp ? TA : 0, t1 = grp ? TP : TA;
+      if (t1 <= t0) continue;
+      q.tiles = h->p_part.tiles + t0;
+      q.T = t1 - t0;
+      q.tile_base = t0;
+      const int fcap = grp ? h->part_fcap_large : h->part_fcap_small;
+      hipLaunchKernelGGL(agg_part_scatter_kernel, dim3((unsigned)((q.T + 7) / 8 * 8)), dim3(256),
+                         scatter_lds(W, fcap), st, q, (const uint64_t*)h->pprefix.p, fcap, (int64_t*)h->recs.p);
+    }
     if (h->upd_block == 256)
       hipLaunchKernelGGL(agg_part_update_kernel<256>, dim3(h->blocks_part_upd), dim3(256), h->lds_part, st,
-                         h->p_part, (const int64_t*)h->recs.p, (const PartItem*)h->pitems.p,
-                         (const int64_t*)h->pnitems.p, (const int*)h->pinst.p);
+                         h->p_part, (const int64_t*)h->recs.p, (const PartItem*)h->pitems.p, h->B,
+                         (const unsigned long long*)h->pnitems.p, (const int*)h->pinst.p);
     else
       hipLaunchKernelGGL(agg_part_update_kernel<512>, dim3(h->blocks_part_upd), dim3(512), h->lds_part, st,
-                         h->p_part, (const int64_t*)h->recs.p, (const PartItem*)h->pitems.p,
-                         (const int64_t*)h->pnitems.p, (const int*)h->pinst.p);
+                         h->p_part, (const int64_t*)h->recs.p, (const PartItem*)h->pitems.p, h->B,
+                         (const unsigned long long*)h->pnitems.p, (const int*)h->pinst.p);
   }
   if (h->L.T_fix)
     hipLaunchKernelGGL(agg_fix_kernel, dim3((unsigned)h->L.T_fix), dim3(256), 0, st, h->L.p_fix);
@@ -3358,8 +3397,8 @@ int gz_agg_release(gz_agg* h) {
   for (auto& b : h->res_valid) ctx_free(ctx, &b);
   for (DBuf* b : {&h->d_inst, &h->d_lane_ops, &h->d_tiles, &h->d_bufs, &h->d_regs, &h->d_est, &h->skeys,
                   &h->svals, &h->tkeys, &h->tvals, &h->cursor, &h->ccpre, &h->counts, &h->sort_tmp, &h->L.d_hbm,
-                  &h->L.d_lds, &h->L.d_slot_prefix, &h->L.d_bufs, &h->d_part, &h->pcount, &h->pstart, &h->pinst,
-                  &h->recs, &h->tcount, &h->tprefix, &h->tcsum, &h->ttot, &h->it0, &h->it1, &h->d_fix,
+                  &h->L.d_lds, &h->L.d_slot_prefix, &h->L.d_bufs, &h->d_part, &h->pinst,
+                  &h->recs, &h->pcnt, &h->pprefix, &h->pscan_tmp, &h->d_fix,
                   &h->d_fix_tiles, &h->d_gbatch, &h->L.d_fix, &h->L.d_gb, &h->d_blob, &h->d_cnt, &h->pitems, &h->pnitems})
     ctx_free(ctx, b);
   for (auto& b : h->res_bits) ctx_free(ctx, &b);

@@ -198,9 +198,8 @@ def test_autotuned_plan_matches_oracle(ctx):
 
 @pytest.mark.parametrize("batched", [True, False])
 def test_high_cardinality_requested_partitioned(ctx, batched):
-    """More groups (300k) than 256 partitions of an LDS table hold: a request for
-    the partitioned placement falls back to the HBM table for this instance, and
-    the results equal the oracle's."""
+    """300k groups on the partitioned placement: 1024 partitions of ~300
+    groups, past the small-fan-out scatter; the results equal the oracle's."""
     batches = datagen.agg_host(330000, num_keys=300000, batch_rows=8192)
     got = run_gpu(ctx, batches, datagen.AGG_TYPES, CONFIG0_ACTIONS, batched, placement=3)
     assert_results_equal(got, expected(batches, datagen.AGG_TYPES, CONFIG0_ACTIONS), RTOL)
