@@ -7,6 +7,7 @@ U[0,2^40), int32, float64 N(0,1) with 2% nulls, utf8 of length U[8,32] over
 [a-z]) into 200 partitions by Spark Murmur3 (seed 42) pmod.  A step splits all
 16384 batches in one batched launch sequence; inputs are resident in HBM before
 timing starts.  The other workloads report in sub-objects of the same JSON line:
+  large_fanout           the same split into 65536 partitions (sorted path)
   aggregate.plain_query  8M rows, 10k groups, GROUP BY key: sum/avg price,
                          count(*), min/max/sum qty; batched vs per-batch kernels
   aggregate.batched_set  512 independent GROUP BYs, rows log-uniform 1k-1M,
@@ -381,6 +382,21 @@ def bench_split(args, ctx, stream):
     plain_rows_s = nb * datagen.BATCH_ROWS / (plain_ms / 1e3)
     rows_per_s = N / (ms_per_step / 1e3)
 
+    # the reference's largest fan-out, 65536 partitions (splitter.cc:218-221):
+    # the same rows through the sorted path (partition ids, stable radix sort by
+    # id, gathers), which the LDS-staged kernels hand over past 2048 partitions
+    big_p = 65536
+
+    def big_step():
+        sp = Splitter.Make(ctx, "hash", datagen.SPLIT_TYPES, big_p, [0], opts)
+        sp.SplitBatches(tab.batches, arr)
+        sp.close()
+
+    big_step()
+    big_ms, big_min = timed(big_step, stream, 3)
+    large_fanout = {"partitions": big_p, "ms": big_ms, "min_ms": big_min, "rows_per_s": N / (big_ms / 1e3),
+                    "GBps": algo_bytes / (big_ms / 1e3) / 1e9, "path": "sorted (radix sort by partition id + gathers)"}
+
     # partition writer: the data file (per-partition Arrow IPC streams) of one split
     writer = None
     if not args.no_writer:
@@ -459,7 +475,8 @@ def bench_split(args, ctx, stream):
            "step_GBps": algo_bytes / (ms_per_step / 1e3) / 1e9,
            "kernels_ms_avg": {k: v[0] / max(1, v[1]) for k, v in kt.items()},
            "plain": {"rows_per_s": plain_rows_s, "batches": nb, "ms": plain_ms},
-           "batched_over_plain": rows_per_s / plain_rows_s, "roofline": roof, "partition_writer": writer}
+           "batched_over_plain": rows_per_s / plain_rows_s, "roofline": roof, "large_fanout": large_fanout,
+           "partition_writer": writer}
     del tab, arr
     return res
 
@@ -491,8 +508,9 @@ def bench_agg(args, ctx, stream):
         it.finish()
         it.close()
 
-    batched()
-    bm, bmin = timed(batched, stream, 5)
+    for _ in range(3):
+        batched()
+    bm, bmin = timed(batched, stream, 20)
     pm, _ = timed(plain, stream, 1)
     out["plain_query"] = {"rows": tab.num_rows, "groups": 10000, "batches": len(tab.batches),
                           "batched_ms": bm, "batched_min_ms": bmin, "batched_rows_per_s": tab.num_rows / (bm / 1e3),
@@ -939,6 +957,7 @@ def main():
         "batched_over_plain": split["batched_over_plain"],
         "wall_s": split["wall_s"],
         "roofline": split["roofline"],
+        "large_fanout": split["large_fanout"],
         "partition_writer": split["partition_writer"],
         "cpu_baseline": cpu,
     }

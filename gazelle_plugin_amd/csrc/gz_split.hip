@@ -29,6 +29,9 @@
 #include <string>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
 #include "gz_device.h"
 #include "gz_internal.h"
 #include "gz_scan.h"
@@ -287,11 +290,15 @@ __global__ __launch_bounds__(BLOCK) void split_count_kernel(SplitParams p) {
 
 // One block: exclusive scans of the partition totals into row_start / byte_start,
 // first utf8 offset of every partition run set to 0, error on a partition whose
-// bytes exceed int32 offsets.
+// bytes exceed int32 offsets, and error 8 when a binary column's bytes exceed
+// the capacity of its data output (offsets past the declared data buffers).
+// tab: [nbin] offsets outputs, then [nbin] data capacities.
 __global__ __launch_bounds__(256) void split_layout_kernel(const uint64_t* tot, int P, int nbin,
                                                             int64_t* row_start,
                                                             int64_t* byte_start,
-                                                            int32_t** out_off, int* err) {
+                                                            const int64_t* tab, int* err) {
+  int32_t* const* out_off = (int32_t* const*)tab;
+  const int64_t* caps = tab + nbin;
   __shared__ uint64_t s_scan[256 / kWave + 1];
   for (int j = -1; j < nbin; j++) {
     const uint64_t* tt = tot + (int64_t)(j + 1) * P;
@@ -306,7 +313,10 @@ __global__ __launch_bounds__(256) void split_layout_kernel(const uint64_t* tot, 
       if (i < P) dst[i] = (int64_t)(carry + ex);
       carry += total;
     }
-    if (threadIdx.x == 0) dst[P] = (int64_t)carry;
+    if (threadIdx.x == 0) {
+      dst[P] = (int64_t)carry;
+      if (j >= 0 && (int64_t)carry > caps[j]) atomicOr(err, 8);
+    }
     __syncthreads();
   }
   // offsets[row_start[p] + p] = 0 for every utf8 column
@@ -340,6 +350,35 @@ __device__ __forceinline__ void copy_string(uint8_t* dst, const uint8_t* src, in
     gst((uint32_t*)(dst + i), x);
   }
   for (; i < len; i++) gst(dst + i, gld(src + i));
+}
+
+// Copy one string of len bytes HBM to HBM with dword loads: byte stores up to
+// the first 4-byte boundary of dst, then dwords, each put together from the two
+// aligned source dwords it straddles, then the tail bytes.  Only aligned dwords
+// that hold at least one of the string's bytes are read.
+__device__ __forceinline__ void copy_string_dw(uint8_t* dst, const uint8_t* src, int len) {
+  if (len <= 0) return;
+  const int head = min(len, (int)((4 - ((uintptr_t)dst & 3)) & 3));
+  for (int q = 0; q < head; q++) gst(dst + q, gld(src + q));
+  const int body = len - head;
+  const int nd = body >> 2;
+  const uint8_t* sb = src + head;
+  const uint32_t sh = (uint32_t)((uintptr_t)sb & 3);
+  const uint32_t* s32 = (const uint32_t*)(sb - sh);
+  uint32_t* d32 = (uint32_t*)(dst + head);
+  if (sh == 0) {
+    for (int m = 0; m < nd; m++) gst(d32 + m, gld(s32 + m));
+  } else if (nd > 0) {
+    // dwords 0 .. nd of s32 cover bytes sb .. sb + 4 nd - 1
+    uint32_t lo = gld(s32);
+    for (int m = 0; m < nd; m++) {
+      const uint32_t hi = gld(s32 + m + 1);
+      gst(d32 + m, funnel(lo, hi, sh));
+      lo = hi;
+    }
+  }
+  const int tb = body & 3;
+  for (int q = 0; q < tb; q++) gst((uint8_t*)(d32 + nd) + q, gld(sb + 4 * nd + q));
 }
 
 // Bytes [b0, b0 + len) of an LDS buffer to dst in HBM: byte stores up to the
@@ -435,6 +474,9 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
   auto colp = [&](int c, int k) -> const uint8_t* { return (const uint8_t*)bp[c * 3 + k]; };
   const int64_t rb = td.row_begin;
   auto lrow = [&](int j) { return w * RW + j * kWave + lane; };
+  // binary columns' bytes past their data outputs' capacity (set by the layout
+  // kernel, earlier in the stream): the split fails and no string is written
+  const bool bytes_bad = p.nbin && (gld(p.err) & 8);
 
   // ---- 0: loads that do not depend on the ranking
   for (int i = tid; i < NW * P; i += BLOCK) s_wcnt[i] = 0;
@@ -695,6 +737,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK * BLOCK / 256) void split_scatt
         __syncthreads();
       }
     } else if (kind == K_BINARY) {
+      if (bytes_bad) continue;
       const int j_bin = C.bin;
       const bool first = c == bc;  // offsets prefetched, bytes possibly in LDS
       uint32_t* s_src = (uint32_t*)s_stage;  // [TILE] source byte offset by sorted slot
@@ -800,6 +843,286 @@ static size_t scatter_lds(int TILE, int BLOCK, int P, int cbuf) { return scatter
 static int scatter_cbuf(int TILE, int nbin) { return nbin > 0 ? 21 * TILE : 0; }
 static int scatter_block(int P) { return P > 1024 ? 512 : 1024; }
 
+// ---------------------------------------------------------------- large fan-out
+// Past the LDS-staged kernels' reach (more than kMaxPartitions partitions, or
+// per-tile counters that do not fit LDS) a split is a stable sort of the rows
+// by partition id: every row's id and tile slot (split_pid_kernel), a radix
+// sort of the (id, slot) pairs (least-significant-digit first, so each
+// partition keeps input order), partition starts by binary search over the
+// sorted ids, then every output column gathered in sorted order.  A slot is
+// t << tile_bits | row of tile t.
+
+// Per output column of the gather: the input column, its kind and width, and
+// its output buffers.  kind K_NA with in_col -1 is the row-id column.
+struct GatherCol {
+  int in_col, kind, width, has_valid;
+  int null_slot;  // index of its null count in the totals
+  int bin, oc;    // binary column ordinal (-1 otherwise), output column
+  void* values;
+  uint32_t* valid;
+};
+
+struct GatherParams {
+  const TileDesc* tiles;
+  const void* const* bufs;  // as SplitParams::bufs
+  int bstride;
+  int tile_bits;
+  int64_t N;
+  const uint16_t* skey;  // sorted partition ids
+  const uint32_t* sval;  // sorted slots
+  const GatherCol* cols;
+  int ncols;
+  unsigned long long* tot;  // null counts at GatherCol::null_slot
+};
+
+__global__ __launch_bounds__(256) void split_pid_kernel(SplitParams p, int tile_bits, uint16_t* key, uint32_t* slot) {
+  __shared__ const void* s_bp[kMaxCols * 3];
+  const int64_t t = blockIdx.x;
+  if (t >= p.T) return;
+  const TileDesc td = p.tiles[t];
+  load_tile_ptrs(p, td.batch, s_bp);
+  __syncthreads();
+  for (int lr = threadIdx.x; lr < td.nrows; lr += 256) {
+    const int pid = compute_pid(p, s_bp, td, lr);
+    const int64_t cr = (int64_t)td.call_row + lr;
+    key[cr] = (uint16_t)pid;
+    slot[cr] = (uint32_t)t << tile_bits | (uint32_t)lr;
+  }
+}
+
+// row_start[q] = the first sorted row whose id is at least q, q in [0, P].
+__global__ void split_sorted_starts_kernel(const uint16_t* skey, int64_t N, int P, int64_t* row_start) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > P) return;
+  int64_t lo = 0, hi = N;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int)skey[mid] < q) lo = mid + 1;
+    else hi = mid;
+  }
+  row_start[q] = lo;
+}
+
+__device__ __forceinline__ void gather_source(const GatherParams& g, int64_t i, int64_t* r, int* batch,
+                                              int32_t* call_row) {
+  const uint32_t v = g.sval[i];
+  const TileDesc td = g.tiles[v >> g.tile_bits];
+  const int lr = (int)(v & ((1u << g.tile_bits) - 1));
+  *r = (int64_t)td.row_begin + lr;
+  *batch = td.batch;
+  *call_row = td.call_row + lr;
+}
+
+// Fixed-width values and row ids, four sorted rows per thread (i, i + S,
+// i + 2S, i + 3S for a grid of S threads), every load of a column issued
+// before its stores.
+__global__ __launch_bounds__(256) void split_gather_values_kernel(GatherParams g) {
+  constexpr int U = 4;
+  const int64_t S = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < g.N; i0 += U * S) {
+    int64_t r[U];
+    int b[U];
+    int32_t cr[U];
+    bool act[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      act[u] = i0 + u * S < g.N;
+      r[u] = 0;
+      b[u] = 0;
+      cr[u] = 0;
+      if (act[u]) gather_source(g, i0 + u * S, &r[u], &b[u], &cr[u]);
+    }
+    for (int k = 0; k < g.ncols; k++) {
+      const GatherCol& C = g.cols[k];
+      if (C.kind == K_NA) {
+        if (C.in_col < 0) {
+#pragma unroll
+          for (int u = 0; u < U; u++)
+            if (act[u]) gst((uint32_t*)C.values + i0 + u * S, (uint32_t)cr[u]);
+        }
+        continue;
+      }
+      if (C.kind != K_FIXED) continue;
+      uint64_t lo[U], hi[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        lo[u] = hi[u] = 0;
+        if (!act[u]) continue;
+        const uint8_t* src = (const uint8_t*)g.bufs[(int64_t)b[u] * g.bstride + C.in_col * 3 + 2];
+        switch (C.width) {
+          case 1: lo[u] = gld(src + r[u]); break;
+          case 2: lo[u] = gld((const uint16_t*)src + r[u]); break;
+          case 4: lo[u] = gld((const uint32_t*)src + r[u]); break;
+          case 8: lo[u] = gld((const uint64_t*)src + r[u]); break;
+          default:
+            lo[u] = gld((const uint64_t*)src + 2 * r[u]);
+            hi[u] = gld((const uint64_t*)src + 2 * r[u] + 1);
+        }
+      }
+      uint8_t* dst = (uint8_t*)C.values;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (!act[u]) continue;
+        const int64_t i = i0 + u * S;
+        switch (C.width) {
+          case 1: gst(dst + i, (uint8_t)lo[u]); break;
+          case 2: gst((uint16_t*)dst + i, (uint16_t)lo[u]); break;
+          case 4: gst((uint32_t*)dst + i, (uint32_t)lo[u]); break;
+          case 8: gst((uint64_t*)dst + i, lo[u]); break;
+          default:
+            gst((uint64_t*)dst + 2 * i, lo[u]);
+            gst((uint64_t*)dst + 2 * i + 1, hi[u]);
+        }
+      }
+    }
+  }
+}
+
+// Validity bitmaps and boolean values, one 32-bit word (32 sorted rows) per
+// thread, written whole: bits past the last row stay set, as in the bitmaps the
+// LDS-staged scatter pre-fills with ones.  Null counts are added per wave.
+__global__ __launch_bounds__(256) void split_gather_bits_kernel(GatherParams g, int64_t nwords) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t w_end = (nwords + stride - 1) / stride * stride;  // every lane runs the same trips
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < w_end; w += stride) {
+    const bool live = w < nwords;
+    int64_t r[32];
+    int b[32];
+    const int64_t i0 = w * 32;
+    const int rows = live ? (int)max<int64_t>(0, min<int64_t>(32, g.N - i0)) : 0;
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      int32_t cr;
+      r[k] = 0;
+      b[k] = 0;
+      if (k < rows) gather_source(g, i0 + k, &r[k], &b[k], &cr);
+    }
+    const uint32_t live_bits = rows == 32 ? 0xFFFFFFFFu : ((1u << rows) - 1u);
+    for (int c = 0; c < g.ncols; c++) {
+      const GatherCol& C = g.cols[c];
+      for (int part = 0; part < 2; part++) {  // 0: validity, 1: boolean values
+        uint32_t* out = part == 0 ? (C.has_valid ? C.valid : nullptr) : (C.kind == K_BOOL ? (uint32_t*)C.values : nullptr);
+        if (!out) continue;
+        uint32_t word = ~live_bits;
+        for (int k = 0; k < rows; k++) {
+          const uint8_t* bm = (const uint8_t*)g.bufs[(int64_t)b[k] * g.bstride + C.in_col * 3 + (part == 0 ? 0 : 2)];
+          const uint32_t bit = bm ? (uint32_t)bit_get(bm, r[k]) : 1u;
+          word |= bit << k;
+        }
+        if (live) gst(out + w, word);
+        if (part == 0) {
+          const unsigned long long nulls = wave_reduce_sum((unsigned long long)__popc(~word & live_bits));
+          if (__lane_id() == 0 && nulls) atomicAdd(g.tot + C.null_slot, nulls);
+        }
+      }
+    }
+  }
+}
+
+// utf8 lengths of one binary column in sorted order; len[N] = 0 so that an
+// exclusive scan of N + 1 entries ends with the total.
+__global__ __launch_bounds__(256) void split_gather_lens_kernel(GatherParams g, int in_col, int64_t* len) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= g.N; i += stride) {
+    if (i == g.N) {
+      len[i] = 0;
+      continue;
+    }
+    int64_t r;
+    int b;
+    int32_t cr;
+    gather_source(g, i, &r, &b, &cr);
+    const int32_t* off = (const int32_t*)g.bufs[(int64_t)b * g.bstride + in_col * 3 + 1];
+    len[i] = (int64_t)(gld(off + r + 1) - gld(off + r));
+  }
+}
+
+// Per partition: its first byte (byte_start[q] = scanned length at its first
+// row, q in [0, P]), the leading 0 of its offsets, and the 2 GiB check.
+__global__ void split_sorted_bytes_kernel(const int64_t* row_start, const int64_t* gb, int P, int64_t* byte_start,
+                                          int32_t* out_off, int* err) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > P) return;
+  const int64_t b0 = gb[row_start[q]];
+  byte_start[q] = b0;
+  if (q < P) {
+    out_off[row_start[q] + q] = 0;
+    if (gb[row_start[q + 1]] - b0 >= 0x80000000ll) atomicOr(err, 2);
+  }
+}
+
+// Offsets and bytes of one binary column, one sorted row per thread: row i of
+// partition q ends at offsets[i + q + 1] (each partition's offsets start with
+// its own 0), its bytes go to data + gb[i]; a row ending past the data
+// output's capacity (offsets past the declared data buffers) sets error 8 and
+// writes nothing.
+__global__ __launch_bounds__(256) void split_gather_strings_kernel(GatherParams g, int in_col, const int64_t* gb,
+                                                                   const int64_t* byte_start, int32_t* out_off,
+                                                                   uint8_t* data, int64_t cap, int* err) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.N; i += stride) {
+    int64_t r;
+    int b;
+    int32_t cr;
+    gather_source(g, i, &r, &b, &cr);
+    const int q = g.skey[i];
+    const int64_t e = gb[i + 1];
+    gst(out_off + i + q + 1, (int32_t)(e - byte_start[q]));
+    if (e > cap) {
+      atomicOr(err, 8);
+      continue;
+    }
+    const int32_t* off = (const int32_t*)g.bufs[(int64_t)b * g.bstride + in_col * 3 + 1];
+    const uint8_t* src = (const uint8_t*)g.bufs[(int64_t)b * g.bstride + in_col * 3 + 2];
+    copy_string_dw(data + gb[i], src + gld(off + r), (int)(e - gb[i]));
+  }
+}
+
+// Per input batch of a sorted segment: utf8 bytes of every binary column and
+// nulls of every output column, stats[batch * (nbin + nout) + (j | nbin + oc)]
+// (the partition writer's per-batch buffer sizes); one workgroup per tile.
+__global__ __launch_bounds__(256) void split_batch_stats_kernel(GatherParams g, int64_t T, int nbin, int nout,
+                                                                unsigned long long* stats) {
+  const int64_t t = blockIdx.x;
+  if (t >= T) return;
+  const TileDesc td = g.tiles[t];
+  unsigned long long* row = stats + (int64_t)td.batch * (nbin + nout);
+  for (int k = 0; k < g.ncols; k++) {
+    const GatherCol& C = g.cols[k];
+    if (C.in_col < 0) continue;
+    const void* const* bp = g.bufs + (int64_t)td.batch * g.bstride + C.in_col * 3;
+    if (C.bin >= 0 && threadIdx.x == 0) {
+      const int32_t* off = (const int32_t*)bp[1];
+      const int32_t b = gld(off + td.row_begin + td.nrows) - gld(off + td.row_begin);
+      if (b) atomicAdd(row + C.bin, (unsigned long long)b);
+    }
+    const uint8_t* vb = (const uint8_t*)bp[0];
+    if (!C.has_valid || !vb) continue;
+    // tiles start on multiples of 8 rows inside a batch
+    unsigned long long nulls = 0;
+    const int nbytes = (td.nrows + 7) >> 3;
+    for (int i = threadIdx.x; i < nbytes; i += 256) {
+      const uint32_t b = gld(vb + (td.row_begin >> 3) + i);
+      const int bits = min(8, td.nrows - i * 8);
+      nulls += (unsigned long long)__popc(~b & ((1u << bits) - 1u));
+    }
+    nulls = wave_reduce_sum(nulls);
+    if (__lane_id() == 0 && nulls) atomicAdd(row + nbin + C.oc, nulls);
+  }
+}
+
+// Rows per (input batch, partition) of a sorted segment, for the partition
+// writer's batch cuts: counts[batch * P + q].
+__global__ __launch_bounds__(256) void split_sorted_batch_counts_kernel(const uint16_t* skey, const uint32_t* sval,
+                                                                        int64_t N, const int32_t* tile_batch,
+                                                                        int tile_bits, int P, uint32_t* counts) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+    const int b = tile_batch[sval[i] >> tile_bits];
+    atomicAdd(counts + (int64_t)b * P + skey[i], 1u);
+  }
+}
+
 }  // namespace gz
 
 // =============================================================== host side
@@ -813,6 +1136,11 @@ struct SplitSegment {
   DBuf counts, prefix, csum, tot, row_start, byte_start, err, offtab;
   std::vector<DBuf> values, validity, offsets, data;
   std::vector<int32_t> tile_batch;  // batch of each tile
+  // a large fan-out segment: sorted partition ids and tile slots; its counts
+  // are rows per (batch, partition), made when first asked for
+  bool sorted = false;
+  int tile_bits = 0;
+  DBuf skey, sval, bstats;
   // host copies, valid after gz_split_stop
   std::vector<int64_t> h_row_start, h_byte_start;
   std::vector<unsigned long long> h_null;
@@ -838,6 +1166,7 @@ struct gz_splitter {
   std::vector<SplitCol> cols;  // per input column (nullc and is_pre set per call)
   DBuf d_cols, d_outs;         // a call's column and output tables
   std::vector<SplitSegment*> segments;
+  bool sorted = false;  // large fan-out: rows sorted by partition id, then gathered
   DBuf d_tiles, d_bufs;
   bool stopped = false;
   int64_t compute_pid_ns = 0;
@@ -878,7 +1207,7 @@ static std::pair<hipEvent_t, hipEvent_t> pid_event_pair(gz_splitter* h) {
 
 static int split_release_segment(gz_context* ctx, SplitSegment* s) {
   for (DBuf* b : {&s->counts, &s->prefix, &s->csum, &s->tot, &s->row_start, &s->byte_start, &s->err,
-                  &s->offtab})
+                  &s->offtab, &s->skey, &s->sval, &s->bstats})
     ctx_free(ctx, b);
   for (auto* v : {&s->values, &s->validity, &s->offsets, &s->data})
     for (auto& b : *v) ctx_free(ctx, &b);
@@ -922,9 +1251,6 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
   int P = mode == 3 ? 1 : num_partitions;
   // splitter.cc:218-221: partitions <= 64K, buffer_size <= 32K
   if (P <= 0 || P > 65536) return fail(GZ_INVALID, "num_partitions must be in [1, 65536]");
-  if (P > kMaxPartitions)
-    return fail(GZ_NOT_IMPLEMENTED, "more than " + std::to_string(kMaxPartitions) +
-                                        " partitions is not supported by the HIP splitter");
   if (num_cols > kMaxCols) return fail(GZ_NOT_IMPLEMENTED, "more than " + std::to_string(kMaxCols) + " columns");
   gz_split_options o;
   gz_split_options_defaults(&o);
@@ -1008,20 +1334,219 @@ int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
   // utf8 staging buffer)
   if (h->tile_rows == 0) h->tile_rows = 2048;
   if (P > 1024) h->tile_rows = 2048;  // 512-thread scatter for large fan-out
-  if (scatter_lds(h->tile_rows, scatter_block(P), P, scatter_cbuf(h->tile_rows, nbin)) > 160 * 1024) {
-    delete h;
-    return fail(GZ_NOT_IMPLEMENTED, "partition count too large for the LDS-staged scatter");
-  }
-  // the count kernel's LDS counters: rows and utf8 bytes per partition, nulls per output column
-  if (sizeof(uint32_t) * ((size_t)P * (1 + nbin) + std::max(oc, 1)) > kCountLdsMax) {
-    delete h;
-    return fail(GZ_NOT_IMPLEMENTED, "num_partitions x (1 + utf8/binary columns) too large for the LDS counters");
+  // Past the LDS-staged kernels' reach (the scatter's per-partition tables, or
+  // the count kernel's counters: rows and utf8 bytes per partition, nulls per
+  // output column) the split sorts rows by partition id and gathers them.
+  if (P > kMaxPartitions ||
+      scatter_lds(h->tile_rows, scatter_block(P), P, scatter_cbuf(h->tile_rows, nbin)) > 160 * 1024 ||
+      sizeof(uint32_t) * ((size_t)P * (1 + nbin) + std::max(oc, 1)) > kCountLdsMax) {
+    h->sorted = true;
+    h->tile_rows = 4096;
   }
   *out = h;
   return GZ_OK;
 }
 
 }  // extern "C"
+
+// Keeps a launched segment; the range splitter validates partition ids like
+// the reference (splitter.cc:1511-1516), so its call waits for the check.
+static int split_commit(gz_splitter* h, SplitSegment* seg) {
+  h->segments.push_back(seg);
+  if (h->mode == 2) {
+    int err = 0;
+    GZ_HIP(hipMemcpyAsync(&err, seg->err.p, sizeof(int), hipMemcpyDeviceToHost, h->ctx->stream));
+    GZ_HIP(hipStreamSynchronize(h->ctx->stream));
+    if (err & 1) return fail(GZ_INVALID, "Partition id is equal or greater than num_partitions");
+  }
+  return GZ_OK;
+}
+
+// The utf8 data output of a call is sized from the scanned byte total (one
+// host round trip) when batches share data buffers (slices of one buffer, whose
+// summed sizes may be many times the real size) or the declared sizes pass
+// 16 GiB; otherwise the declared data buffers' sizes bound it.
+static bool exact_data_size(bool shared, size_t declared) { return shared || declared > ((size_t)16 << 30); }
+
+static int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); }
+
+// A segment of a large fan-out splitter (see split_pid_kernel).  The row
+// outputs are allocated; the utf8 data outputs are sized and allocated here.
+static int split_run_sorted(gz_splitter* h, SplitSegment* seg, const SplitParams& prm,
+                            const std::vector<SplitCol>& cols, std::vector<SplitOut>& outs,
+                            const std::vector<size_t>& data_bytes, const std::vector<bool>& data_shared,
+                            int64_t nwords) {
+  gz_context* ctx = h->ctx;
+  hipStream_t st = ctx->stream;
+  const int P = h->P, nc = h->ncols_in, nbin = h->base.nbin;
+  const int nout = (int)h->out_types.size();
+  const int64_t N = seg->N, T = seg->T;
+  int tile_bits = 0;
+  while ((1 << tile_bits) < h->tile_rows) tile_bits++;
+  if (T > 0 && ((uint64_t)(T - 1) >> (32 - tile_bits)) != 0)
+    return fail(GZ_CAPACITY_ERROR, "too many batches in one large fan-out split call");
+  seg->sorted = true;
+  seg->tile_bits = tile_bits;
+  DBuf key_in, val_in, tmp, glen, gb, gcols;
+  auto done = [&](int code) {
+    for (DBuf* b : {&key_in, &val_in, &tmp, &glen, &gb, &gcols}) ctx_free(ctx, b);
+    return code;
+  };
+  auto hip = [&](hipError_t e, const char* what) {
+    return e == hipSuccess ? GZ_OK : fail(GZ_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  const size_t n1 = (size_t)std::max<int64_t>(N, 1);
+  int rc;
+  if ((rc = ctx_alloc(ctx, 2 * n1, &key_in)) || (rc = ctx_alloc(ctx, 4 * n1, &val_in)) ||
+      (rc = ctx_alloc(ctx, 2 * n1, &seg->skey)) || (rc = ctx_alloc(ctx, 4 * n1, &seg->sval)))
+    return done(rc);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(2 * (int64_t)seg->R)), dim3(256), 0, st, (uint32_t*)seg->tot.p,
+                     2 * (int64_t)seg->R, 0u);
+  // ---- partition ids, stable sort by id
+  if (T > 0) {
+    KernelTimer kt(ctx, "split_count");
+    const std::pair<hipEvent_t, hipEvent_t> ev = pid_event_pair(h);
+    if (ev.first) {
+      hipEventRecord(ev.first, st);
+      h->pid_events.push_back(ev);
+    }
+    hipLaunchKernelGGL(split_pid_kernel, dim3((unsigned)T), dim3(256), 0, st, prm, tile_bits, (uint16_t*)key_in.p,
+                       (uint32_t*)val_in.p);
+    if (ev.first) hipEventRecord(ev.second, st);
+  }
+  {
+    KernelTimer kt(ctx, "split_scan");
+    if (N > 0) {
+      unsigned end_bit = 1;
+      while ((1 << end_bit) < P) end_bit++;
+      size_t tb = 0;
+      if ((rc = hip(rocprim::radix_sort_pairs(nullptr, tb, (const uint16_t*)key_in.p, (uint16_t*)seg->skey.p,
+                                              (const uint32_t*)val_in.p, (uint32_t*)seg->sval.p, (size_t)N, 0u,
+                                              end_bit, st),
+                    "radix_sort_pairs")))
+        return done(rc);
+      if ((rc = ctx_alloc(ctx, std::max<size_t>(tb, 4), &tmp))) return done(rc);
+      if ((rc = hip(rocprim::radix_sort_pairs(tmp.p, tb, (const uint16_t*)key_in.p, (uint16_t*)seg->skey.p,
+                                              (const uint32_t*)val_in.p, (uint32_t*)seg->sval.p, (size_t)N, 0u,
+                                              end_bit, st),
+                    "radix_sort_pairs")))
+        return done(rc);
+    }
+    hipLaunchKernelGGL(split_sorted_starts_kernel, dim3((unsigned)((P + 256) / 256)), dim3(256), 0, st,
+                       (const uint16_t*)seg->skey.p, N, P, (int64_t*)seg->row_start.p);
+  }
+  // ---- gathers
+  KernelTimer kt(ctx, "split_scatter");
+  std::vector<GatherCol> gc;
+  for (int c = 0; c < nc; c++) {
+    const int oc = cols[c].out_col;
+    if (oc < 0 || cols[c].kind == K_NA) continue;
+    gc.push_back(GatherCol{c, cols[c].kind, cols[c].width, outs[oc].valid != nullptr, P * (1 + nbin) + oc,
+                           cols[c].bin, oc, outs[oc].values, outs[oc].valid});
+  }
+  if (prm.rowid_oc >= 0)
+    gc.push_back(GatherCol{-1, K_NA, 4, 0, P * (1 + nbin) + prm.rowid_oc, -1, prm.rowid_oc,
+                           outs[prm.rowid_oc].values, nullptr});
+  if ((rc = upload(ctx, &gcols, gc.data(), sizeof(GatherCol) * std::max<size_t>(gc.size(), 1)))) return done(rc);
+  GatherParams g;
+  g.tiles = prm.tiles;
+  g.bufs = prm.bufs;
+  g.bstride = prm.bstride;
+  g.tile_bits = tile_bits;
+  g.N = N;
+  g.skey = (const uint16_t*)seg->skey.p;
+  g.sval = (const uint32_t*)seg->sval.p;
+  g.cols = (const GatherCol*)gcols.p;
+  g.ncols = (int)gc.size();
+  g.tot = (unsigned long long*)seg->tot.p;
+  const int64_t nstat = (int64_t)std::max<int32_t>(seg->nbatches, 1) * (nbin + nout);
+  if ((rc = ctx_alloc(ctx, sizeof(unsigned long long) * std::max<int64_t>(nstat, 1), &seg->bstats))) return done(rc);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(2 * nstat)), dim3(256), 0, st, (uint32_t*)seg->bstats.p,
+                     2 * nstat, 0u);
+  if (T > 0)
+    hipLaunchKernelGGL(split_batch_stats_kernel, dim3((unsigned)T), dim3(256), 0, st, g, T, nbin, nout,
+                       (unsigned long long*)seg->bstats.p);
+  if (N > 0) {
+    hipLaunchKernelGGL(split_gather_values_kernel, dim3(grid_for((N + 3) / 4)), dim3(256), 0, st, g);
+    hipLaunchKernelGGL(split_gather_bits_kernel, dim3(grid_for(nwords)), dim3(256), 0, st, g, nwords);
+  }
+  // utf8 columns: lengths in sorted order, their scan, per-partition starts,
+  // then offsets and bytes
+  if (nbin) {
+    if ((rc = ctx_alloc(ctx, 8 * (n1 + 1), &glen)) || (rc = ctx_alloc(ctx, 8 * (n1 + 1), &gb))) return done(rc);
+  }
+  size_t sb = 0;
+  for (int c = 0; c < nc; c++) {
+    const int oc = cols[c].out_col, j = cols[c].bin;
+    if (oc < 0 || j < 0) continue;
+    int64_t* bstart = (int64_t*)seg->byte_start.p + (int64_t)j * (P + 1);
+    hipLaunchKernelGGL(split_gather_lens_kernel, dim3(grid_for(N + 1)), dim3(256), 0, st, g, c, (int64_t*)glen.p);
+    size_t need = 0;
+    if ((rc = hip(rocprim::exclusive_scan(nullptr, need, (const int64_t*)glen.p, (int64_t*)gb.p, (int64_t)0,
+                                          (size_t)N + 1, rocprim::plus<int64_t>(), st),
+                  "exclusive_scan")))
+      return done(rc);
+    if (need > sb) {
+      ctx_free(ctx, &tmp);
+      if ((rc = ctx_alloc(ctx, need, &tmp))) return done(rc);
+      sb = need;
+    }
+    if ((rc = hip(rocprim::exclusive_scan(tmp.p, need, (const int64_t*)glen.p, (int64_t*)gb.p, (int64_t)0,
+                                          (size_t)N + 1, rocprim::plus<int64_t>(), st),
+                  "exclusive_scan")))
+      return done(rc);
+    hipLaunchKernelGGL(split_sorted_bytes_kernel, dim3((unsigned)((P + 256) / 256)), dim3(256), 0, st,
+                       (const int64_t*)seg->row_start.p, (const int64_t*)gb.p, P, bstart, outs[oc].off,
+                       (int*)seg->err.p);
+    // data: the input data buffers bound it, unless batches share them (or the
+    // bound is past 16 GiB): then the scanned total, one host round trip
+    size_t bytes = data_bytes[j];
+    const bool exact = exact_data_size(data_shared[j], bytes);
+    if (exact) {
+      int64_t total = 0;
+      if ((rc = hip(hipMemcpyAsync(&total, (const int64_t*)gb.p + N, sizeof(int64_t), hipMemcpyDeviceToHost, st),
+                    "hipMemcpyAsync")) ||
+          (rc = hip(hipStreamSynchronize(st), "hipStreamSynchronize")))
+        return done(rc);
+      bytes = (size_t)total;
+    }
+    if ((rc = ctx_alloc(ctx, std::max<size_t>(bytes, 4), &seg->data[oc]))) return done(rc);
+    outs[oc].data = (uint8_t*)seg->data[oc].p;
+    if (N > 0)
+      hipLaunchKernelGGL(split_gather_strings_kernel, dim3(grid_for(N)), dim3(256), 0, st, g, c,
+                         (const int64_t*)gb.p, (const int64_t*)bstart, outs[oc].off, outs[oc].data,
+                         exact ? INT64_MAX : (int64_t)bytes, (int*)seg->err.p);
+  }
+  if ((rc = hip(hipGetLastError(), "split gather launch"))) return done(rc);
+  // the temporaries go back to the pool, which hands them only to work queued
+  // later on this stream
+  return done(GZ_OK);
+}
+
+// Rows per (batch, partition) of a large fan-out segment, counts[b * P + q],
+// made on first use.
+static int sorted_batch_counts(gz_splitter* h, SplitSegment* s) {
+  if (!s->sorted || s->counts.p) return GZ_OK;
+  gz_context* ctx = h->ctx;
+  const int P = h->P;
+  const int64_t nb = std::max<int32_t>(s->nbatches, 1);
+  int rc;
+  if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * (size_t)nb * P, &s->counts))) return rc;
+  DBuf tb;
+  if ((rc = upload(ctx, &tb, s->tile_batch.data(), sizeof(int32_t) * std::max<size_t>(s->tile_batch.size(), 1)))) {
+    ctx_free(ctx, &s->counts);
+    return rc;
+  }
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(nb * P)), dim3(256), 0, ctx->stream, (uint32_t*)s->counts.p,
+                     nb * P, 0u);
+  if (s->N > 0)
+    hipLaunchKernelGGL(split_sorted_batch_counts_kernel, dim3(grid_for(s->N)), dim3(256), 0, ctx->stream,
+                       (const uint16_t*)s->skey.p, (const uint32_t*)s->sval.p, s->N, (const int32_t*)tb.p,
+                       s->tile_bits, P, (uint32_t*)s->counts.p);
+  ctx_free(ctx, &tb);
+  GZ_HIP(hipGetLastError());
+  return GZ_OK;
+}
 
 // Run one segment over a list of batches (device buffers in the JNI convention).
 static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
@@ -1111,6 +1636,12 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     for (size_t i = 0; i < empty_data.size(); i++)
       if (ends[2 * i] != ends[2 * i + 1]) return fail(GZ_INVALID, "missing utf8 data buffer");
   }
+  // batches sharing a utf8 data buffer (slices of one buffer)
+  for (int b = 0; b < h->base.nbin; b++) {
+    auto& v = data_ptrs[b];
+    std::sort(v.begin(), v.end());
+    data_shared[b] = std::adjacent_find(v.begin(), v.end()) != v.end();
+  }
   h->rr_state = (int32_t)rr;
   auto* seg = new SplitSegment();
   seg->N = N;
@@ -1130,9 +1661,11 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
   const int R = P * (1 + nbin) + std::max(nout, 1);
   seg->R = R;
   const int64_t C = (std::max<int64_t>(T, 1) + kScanChunk - 1) / kScanChunk;
-  if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * R * std::max<int64_t>(T, 1), &seg->counts))) return bail(rc);
-  if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * R * std::max<int64_t>(T, 1), &seg->prefix))) return bail(rc);
-  if ((rc = ctx_alloc(ctx, sizeof(uint64_t) * R * C, &seg->csum))) return bail(rc);
+  if (!h->sorted) {
+    if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * R * std::max<int64_t>(T, 1), &seg->counts))) return bail(rc);
+    if ((rc = ctx_alloc(ctx, sizeof(uint32_t) * R * std::max<int64_t>(T, 1), &seg->prefix))) return bail(rc);
+    if ((rc = ctx_alloc(ctx, sizeof(uint64_t) * R * C, &seg->csum))) return bail(rc);
+  }
   if (nbin) {
     if ((rc = ctx_alloc(ctx, sizeof(int64_t) * nbin * (P + 1), &seg->byte_start))) return bail(rc);
   }
@@ -1196,6 +1729,10 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, st, (uint32_t*)p, n32, v);
   };
   fill(seg->err.p, 1, 0);
+  if (h->sorted) {
+    if ((rc = split_run_sorted(h, seg, prm, cols, outs, data_bytes, data_shared, nwords))) return bail(rc);
+    return split_commit(h, seg);
+  }
   for (int oc = 0; oc < nout; oc++) {
     if (seg->validity[oc].p) fill(seg->validity[oc].p, nwords, 0xFFFFFFFFu);
     if (h->out_types[oc] == GZ_TYPE_BOOL) fill(seg->values[oc].p, nwords, 0xFFFFFFFFu);
@@ -1255,28 +1792,26 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
     } else {
       fill(seg->tot.p, 2 * (int64_t)R, 0);
     }
-    std::vector<int32_t*> offp(std::max(nbin, 1), nullptr);
-    for (int c = 0; c < nc; c++)
-      if (cols[c].bin >= 0) offp[cols[c].bin] = outs[cols[c].out_col].off;
-    if ((rc = upload(ctx, &seg->offtab, offp.data(), sizeof(int32_t*) * offp.size()))) return bail(rc);
+    // offsets outputs, then the data capacities: the declared data buffers'
+    // sizes, or no limit where the data output is sized from the scanned totals
+    std::vector<int64_t> offp(2 * (size_t)std::max(nbin, 1), 0);
+    for (int c = 0; c < nc; c++) {
+      const int j = cols[c].bin;
+      if (j < 0) continue;
+      offp[j] = (int64_t)(uintptr_t)outs[cols[c].out_col].off;
+      offp[nbin + j] = exact_data_size(data_shared[j], data_bytes[j]) ? INT64_MAX : (int64_t)data_bytes[j];
+    }
+    if ((rc = upload(ctx, &seg->offtab, offp.data(), sizeof(int64_t) * offp.size()))) return bail(rc);
     hipLaunchKernelGGL(split_layout_kernel, dim3(1), dim3(256), 0, st, (const uint64_t*)seg->tot.p,
                        P, nbin, (int64_t*)seg->row_start.p, (int64_t*)seg->byte_start.p,
-                       (int32_t**)seg->offtab.p, (int*)seg->err.p);
+                       (const int64_t*)seg->offtab.p, (int*)seg->err.p);
   }
-  // utf8 data outputs: the sum of the input data buffers bounds them.  When
-  // batches share data buffers (slices of one buffer) that bound may be many
-  // times the real size, so those are sized exactly from the scanned byte
-  // totals (one host round trip); so is a bound above 16 GiB.
-  for (int b = 0; b < h->base.nbin; b++) {
-    auto& v = data_ptrs[b];
-    std::sort(v.begin(), v.end());
-    data_shared[b] = std::adjacent_find(v.begin(), v.end()) != v.end();
-  }
+  // utf8 data outputs (exact_data_size: bound or scanned total)
   for (int c = 0; c < nc; c++) {
     int oc = cols[c].out_col;
     if (oc < 0 || !type_is_binary(h->types_in[c])) continue;
     size_t bytes = data_bytes[cols[c].bin];
-    if (data_shared[cols[c].bin] || bytes > ((size_t)16 << 30)) {
+    if (exact_data_size(data_shared[cols[c].bin], bytes)) {
       int64_t total = 0;
       GZ_HIP(hipMemcpyAsync(&total, (int64_t*)seg->byte_start.p + (int64_t)cols[c].bin * (P + 1) + P,
                             sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -1309,15 +1844,7 @@ static int split_run(gz_splitter* h, int32_t nb, const gz_batch* batches) {
 #undef GZ_SCATTER
   }
   GZ_HIP(hipGetLastError());
-  h->segments.push_back(seg);
-  if (h->mode == 2) {
-    // the range splitter validates partition ids like the reference (splitter.cc:1511-1516)
-    int err = 0;
-    GZ_HIP(hipMemcpyAsync(&err, seg->err.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    GZ_HIP(hipStreamSynchronize(st));
-    if (err & 1) return fail(GZ_INVALID, "Partition id is equal or greater than num_partitions");
-  }
-  return GZ_OK;
+  return split_commit(h, seg);
 }
 
 // Runs the staged per-batch calls as one segment.
@@ -1349,6 +1876,7 @@ static int resolve_segment(gz_splitter* h, SplitSegment* s) {
   GZ_HIP(hipStreamSynchronize(ctx->stream));
   if (err & 1) return fail(GZ_INVALID, "Partition id is equal or greater than num_partitions");
   if (err & 2) return fail(GZ_CAPACITY_ERROR, "a partition's utf8 data exceeds 2 GiB");
+  if (err & 8) return fail(GZ_INVALID, "utf8 offsets reach past the declared data buffer");
   if (err & 4)
     return fail(GZ_NOT_IMPLEMENTED,
                 "a float or double hash partitioning key holds -0.0 or NaN (its Spark hash is version-dependent)");
@@ -1461,6 +1989,13 @@ int gz_split_segment_batch_counts(gz_splitter* h, int32_t seg, int32_t* num_batc
   if (num_batches) *num_batches = s->nbatches;
   if (!counts) return GZ_OK;
   const int P = h->P;
+  if (s->sorted) {
+    GZ_TRY(sorted_batch_counts(h, s));
+    GZ_HIP(hipMemcpyAsync(counts, s->counts.p, sizeof(int32_t) * (size_t)s->nbatches * P, hipMemcpyDeviceToHost,
+                          h->ctx->stream));
+    GZ_HIP(hipStreamSynchronize(h->ctx->stream));
+    return GZ_OK;
+  }
   // tile-major counters: row t holds the tile's rows per partition first
   std::vector<uint32_t> tc((size_t)s->R * std::max<int64_t>(s->T, 1));
   if (s->T > 0) {
@@ -1995,15 +2530,33 @@ static std::vector<CutBatch> ipc_cut(const std::vector<int32_t>& counts, int64_t
 static int segment_batch_stats(gz_splitter* h, SplitSegment* s, std::vector<int32_t>& counts,
                                std::vector<uint8_t>& nulls, std::vector<int64_t>& bin_bytes) {
   const int P = h->P, nbin = h->base.nbin, nout = (int)h->out_types.size();
+  counts.assign((size_t)s->nbatches * P, 0);
+  nulls.assign((size_t)s->nbatches * std::max(nout, 1), 0);
+  bin_bytes.assign((size_t)s->nbatches * std::max(nbin, 1), 0);
+  if (s->sorted) {
+    GZ_TRY(sorted_batch_counts(h, s));
+    std::vector<unsigned long long> st((size_t)s->nbatches * (nbin + nout) + 1);
+    if (s->nbatches > 0) {
+      GZ_HIP(hipMemcpyAsync(counts.data(), s->counts.p, sizeof(int32_t) * (size_t)s->nbatches * P,
+                            hipMemcpyDeviceToHost, h->ctx->stream));
+      GZ_HIP(hipMemcpyAsync(st.data(), s->bstats.p, sizeof(unsigned long long) * (size_t)s->nbatches * (nbin + nout),
+                            hipMemcpyDeviceToHost, h->ctx->stream));
+      GZ_HIP(hipStreamSynchronize(h->ctx->stream));
+    }
+    for (int32_t b = 0; b < s->nbatches; b++) {
+      const unsigned long long* row = &st[(size_t)b * (nbin + nout)];
+      for (int j = 0; j < nbin; j++) bin_bytes[(size_t)b * nbin + j] = (int64_t)row[j];
+      for (int oc = 0; oc < nout; oc++)
+        if (row[nbin + oc]) nulls[(size_t)b * nout + oc] = 1;
+    }
+    return GZ_OK;
+  }
   std::vector<uint32_t> tc((size_t)s->R * std::max<int64_t>(s->T, 1));
   if (s->T > 0) {
     GZ_HIP(hipMemcpyAsync(tc.data(), s->counts.p, sizeof(uint32_t) * s->R * s->T, hipMemcpyDeviceToHost,
                           h->ctx->stream));
     GZ_HIP(hipStreamSynchronize(h->ctx->stream));
   }
-  counts.assign((size_t)s->nbatches * P, 0);
-  nulls.assign((size_t)s->nbatches * std::max(nout, 1), 0);
-  bin_bytes.assign((size_t)s->nbatches * std::max(nbin, 1), 0);
   for (int64_t t = 0; t < s->T; t++) {
     const int b = s->tile_batch[t];
     const uint32_t* row = &tc[(size_t)t * s->R];

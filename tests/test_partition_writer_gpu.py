@@ -54,12 +54,13 @@ def _check_partition(arrays, exp, types, p):
             np.testing.assert_array_equal(got, exp_v, err_msg=f"values p={p} c={c}")
 
 
-@pytest.mark.parametrize("batched", [True, False])
-@pytest.mark.parametrize("buffer_size", [4096, 300])
-def test_data_file_decodes_to_the_split(ctx, batched, buffer_size):
+# P = 2500: the splitter's sorted path (more than 2048 partitions), whose
+# per-batch counts and buffer statistics the writer's cuts read
+@pytest.mark.parametrize("batched,buffer_size,P", [(True, 4096, 11), (True, 300, 11), (False, 4096, 11),
+                                                   (False, 300, 11), (True, 3, 2500)])
+def test_data_file_decodes_to_the_split(ctx, batched, buffer_size, P):
     types = [T.INT64, T.INT32, T.DOUBLE, T.STRING, T.BOOL, T.INT8, T.UINT16, T.NA, T.BINARY, T.DATE32]
     batches = random_batches(types, [5000, 0, 1, 4096, 777, 3000], seed=21, null_frac=0.15)
-    P = 11
     exp = oracle.split(batches, types, P, [0, 3])
     pids = oracle.split_pids(batches, types, P, [0, 3])
     sp = Splitter.Make(ctx, "hash", types, P, [0, 3], SplitOptions(buffer_size=buffer_size))

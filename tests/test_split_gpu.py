@@ -250,7 +250,5 @@ def test_split_limits(ctx):
         Splitter.Make(ctx, "hash", [T.INT64] * 129, 200, [0])
     with pytest.raises(L.NotImplementedStatus):
         Splitter.Make(ctx, "hash", [T.INT64] * 20, 200, list(range(17)))
-    with pytest.raises(L.NotImplementedStatus):
-        Splitter.Make(ctx, "hash", [T.INT64, T.STRING], 4096, [0])
-    with pytest.raises(L.NotImplementedStatus):  # 2048 x (1 + 40) counters exceed the LDS budget
-        Splitter.Make(ctx, "hash", [T.INT64] + [T.STRING] * 40, 2048, [0])
+    with pytest.raises(L.Invalid):  # the reference's limit (splitter.cc:218-221)
+        Splitter.Make(ctx, "hash", [T.INT64, T.STRING], 65537, [0])
