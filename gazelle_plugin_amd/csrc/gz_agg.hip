@@ -664,9 +664,11 @@ __global__ void agg_init_kernel(const AggInst* inst, int ninst, int nl, const in
   }
 }
 
-// Reset of every instance's table in one launch: slot g of the concatenated
-// slot space belongs to the instance whose range [prefix[i], prefix[i+1])
-// holds it.
+// Reset of every instance's table in one launch: index g of the concatenated
+// reset space belongs to the instance whose range [prefix[i], prefix[i+1])
+// holds it.  A partition-structured table's range holds only its two special
+// slots: its partitions are reset per partition once the work items are known
+// (agg_part_reset_kernel), or stored whole by the work item that owns them.
 __global__ void agg_init_all_kernel(const AggInst* inst, int ninst, const int64_t* prefix, int64_t total,
                                     int nl, const int* lane_ops) {
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
@@ -677,12 +679,12 @@ __global__ void agg_init_all_kernel(const AggInst* inst, int ninst, const int64_
       if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
     }
     const AggInst I = inst[lo];
-    const int64_t s = g - prefix[lo];
+    const int64_t s = (I.sub_log2 ? (int64_t)I.cap : 0) + (g - prefix[lo]);
     I.base[s] = kEmpty;
     *tab_first(I, s) = INT64_MAX;
     int64_t* ln = tab_lanes(I, s);
     for (int l = 0; l < nl; l++) ln[l] = lane_init(lane_ops[l]);
-    if (s == 0) {
+    if (g == prefix[lo]) {
       *I.counter = 0;
       *I.overflow = 0;
     }
@@ -726,8 +728,12 @@ __device__ void lds_merge(const AggParams& p, const AggInst& I, int64_t* l, int6
 // An LDS table whose groups no other workgroup updates (all rows of their
 // keys went through this table) written into partition b's slots of a
 // structured HBM table, whose layout it has: slot s of the LDS table is slot
-// (b << sub_log2) + s, so the whole table is stored as it is (empty slots
-// included), with no atomics; the special slots go to cap and cap + 1.
+// (b << sub_log2) + s, so every group is stored in place with no atomics; the
+// special slots go to cap and cap + 1.  The partition's slots are not reset
+// beforehand (no other workgroup touches them): every slot's key and first row
+// are stored (an empty slot's first row, INT64_MAX, marks it unoccupied for the
+// collection), and the lanes of every group; an empty slot's lanes are never
+// read.
 __device__ void lds_write_image(const AggParams& p, const AggInst& I, int b, const int64_t* l, int64_t LS) {
   __shared__ unsigned s_keys;
   if (threadIdx.x == 0) s_keys = 0;
@@ -737,13 +743,14 @@ __device__ void lds_write_image(const AggParams& p, const AggInst& I, int b, con
   for (int64_t s = threadIdx.x; s < (int64_t)p.lcap + 2; s += blockDim.x) {
     const bool special = s >= p.lcap;
     if (special && l[LS + s] == INT64_MAX) continue;
-    const int64_t g = special ? (int64_t)I.cap + (s - p.lcap) : base + s;
     const int64_t key = l[s];
+    const int64_t g = special ? (int64_t)I.cap + (s - p.lcap) : base + s;
+    *tab_first(I, g) = l[LS + s];
     if (!special) {
       I.base[g] = key;
-      mine += key != kEmpty;
+      if (key == kEmpty) continue;
+      mine++;
     }
-    *tab_first(I, g) = l[LS + s];
     int64_t* ln = tab_lanes(I, g);
     for (int k = 0; k < p.nl; k++) ln[k] = l[(2 + k) * LS + s];
   }
@@ -1282,6 +1289,36 @@ __global__ __launch_bounds__(256) void agg_part_items_kernel(const uint64_t* pre
   }
 }
 
+// Reset of a partition-structured table's partitions, by what the work items
+// do with them: a partition of more than kPartItemRecords records (its items
+// merge with atomics) gets every slot reset, an empty one only its keys and
+// first rows (no item writes it; the collection reads first rows); an owned
+// partition nothing (its item stores every slot's key and first row and every
+// group's lanes).  One workgroup per partition.  lane_op(l): lane l's operation.
+template <class LaneOp>
+__device__ void part_slots_reset(const AggInst& I, int b, bool bare, int nl, LaneOp lane_op) {
+  const int64_t base = (int64_t)b << I.sub_log2;
+  const int64_t n = (int64_t)1 << I.sub_log2;
+  for (int64_t s = threadIdx.x; s < n; s += blockDim.x) {
+    I.base[base + s] = kEmpty;
+    *tab_first(I, base + s) = INT64_MAX;
+    if (bare) continue;
+    int64_t* ln = tab_lanes(I, base + s);
+    for (int l = 0; l < nl; l++) ln[l] = lane_init(lane_op(l));
+  }
+}
+
+__global__ __launch_bounds__(256) void agg_part_reset_kernel(const uint64_t* prefix, const AggInst* inst,
+                                                             const int* pinst, int B, int64_t nrec, int nl,
+                                                             const int* lane_ops) {
+  const int g = blockIdx.x;
+  if (g >= B) return;
+  const int64_t v = part_first(prefix, inst, pinst, B, nrec, g + 1) - part_first(prefix, inst, pinst, B, nrec, g);
+  if (v > 0 && v <= kPartItemRecords) return;
+  const AggInst I = inst[pinst[g]];
+  part_slots_reset(I, g - I.bucket_base, v == 0, nl, [&](int l) { return lane_ops[l]; });
+}
+
 // Records of a tile: partition counts in LDS, one range reserved per non-empty
 // partition of the tile, the tile's records laid out in LDS in partition order
 // (word-major), then written out as one contiguous run per partition and word
@@ -1452,7 +1489,7 @@ __global__ __launch_bounds__(UB, 2 * UB / 256) void agg_part_update_kernel(AggPa
     {
       const PartItem it = items[j];
       const int64_t a = it.a, e = it.e;
-      if (a == e) continue;  // an empty partition: its slots keep their initial values
+      if (a == e) continue;  // an empty partition: its keys were reset (agg_part_reset_kernel)
       const AggInst I = p.inst[pinst[it.b]];
       __syncthreads();
       if (threadIdx.x == 0) s_failed = 0;
@@ -1511,19 +1548,58 @@ __global__ __launch_bounds__(UB, 2 * UB / 256) void agg_part_update_kernel(AggPa
           for (int k = 0; k < kUpdRPT; k++)
             if (ls[k] >= 0) apply_row_lds(p, s_tab, LS, ls[k], row_of(k), grow_of(k));
         }
-        // records whose LDS probe failed go to the instance's HBM table
+        // records whose LDS probe failed go to the instance's HBM table: at
+        // once for a chunk of a shared partition (reset before the launch),
+        // after the LDS pass for an owned one (its slots are not reset)
 #pragma unroll
         for (int k = 0; k < kUpdRPT; k++) {
           if (ls[k] != -2) continue;
           s_failed = 1;
+          if (it.owned) continue;
           const int64_t g = table_slot(I, true, kw[k]);
           if (g >= 0) apply_row_hbm(p, I, g, row_of(k), grow_of(k));
         }
       }
       __syncthreads();
-      if (it.owned && I.sub_log2 && !s_failed) lds_write_image(p, I, it.b - I.bucket_base, s_tab, LS);
-      else if (it.owned) lds_writeout(p, I, s_tab, LS);
-      else lds_merge(p, I, s_tab, LS);
+      if (!it.owned) {
+        lds_merge(p, I, s_tab, LS);
+      } else if (!s_failed) {
+        lds_write_image(p, I, it.b - I.bucket_base, s_tab, LS);
+      } else {
+        // the LDS table overflowed: reset the partition's slots, claim a slot
+        // for every group of the LDS table and store it, then apply the
+        // records whose keys the LDS table lacks with atomics (a key is either
+        // in the LDS table for all its records or for none)
+        part_slots_reset(I, it.b - I.bucket_base, false, p.nl, [&](int l) { return p.lanes[l].op; });
+        __threadfence();
+        __syncthreads();
+        lds_writeout(p, I, s_tab, LS);
+        __threadfence();
+        __syncthreads();
+        for (int64_t q = a + threadIdx.x; q < e; q += blockDim.x) {
+          const int64_t key = gld(recs + q);
+          const int64_t w1 = gld(recs + NR + q);
+          const uint32_t nulls = (uint32_t)((uint64_t)w1 >> 48);
+          if (((nulls >> kRecKeyNull) & 1) || key == kEmpty) continue;  // special slots never miss
+          bool found = false;
+          uint64_t sl = fmix64((uint64_t)key) & lmask;
+          for (int i = 0; i < 32; i++) {
+            const int64_t kk = s_tab[sl];
+            if (kk == key) {
+              found = true;
+              break;
+            }
+            if (kk == kEmpty) break;
+            sl = (sl + 1) & lmask;
+          }
+          if (found) continue;
+          const int64_t u0 = p.nu > 0 ? gld(recs + 2 * NR + q) : 0;
+          const int64_t u1 = p.nu > 1 ? gld(recs + 3 * NR + q) : 0;
+          const int64_t g = table_slot(I, true, key);
+          if (g >= 0)
+            apply_row_hbm(p, I, g, RecRow{p, u0, u1, recs + q, nulls}, (int64_t)((uint64_t)w1 & ((1ull << 48) - 1)));
+        }
+      }
     }
   }
 }
@@ -2769,7 +2845,7 @@ static int agg_prepare(gz_agg* h, int32_t nb, const gz_batch* batches, const int
     h->inst[i].fan_log2 = lg;
     h->inst[i].sub_log2 = lg ? part_sub : 0;
     h->inst[i].bucket_base = 0;
-    prefix[i + 1] = prefix[i] + (int64_t)h->inst[i].cap + 2;
+    prefix[i + 1] = prefix[i] + (h->inst[i].sub_log2 ? 0 : (int64_t)h->inst[i].cap) + 2;
   }
   // partitioned instances, small fan-outs first (their tiles form a scatter
   // launch with a small LDS footprint): partition numbering, their tiles in
@@ -2927,6 +3003,9 @@ static int agg_run_kernels(gz_agg* h) {
     hipLaunchKernelGGL(agg_part_items_kernel, dim3((unsigned)((h->B + 255) / 256)), dim3(256), 0, st,
                        (const uint64_t*)h->pprefix.p, (const AggInst*)h->d_inst.p, (const int*)h->pinst.p, h->B,
                        h->nrec, (PartItem*)h->pitems.p, (unsigned long long*)h->pnitems.p);
+    hipLaunchKernelGGL(agg_part_reset_kernel, dim3((unsigned)h->B), dim3(256), 0, st, (const uint64_t*)h->pprefix.p,
+                       (const AggInst*)h->d_inst.p, (const int*)h->pinst.p, h->B, h->nrec, nl,
+                       (const int*)h->d_lane_ops.p);
     // scatter: the small fan-outs' tiles, then the large ones' (per-partition LDS arrays sized for each)
     for (int grp = 0; grp < 2; grp++) {
       AggParams q = h->p_part;

@@ -205,6 +205,34 @@ def test_high_cardinality_requested_partitioned(ctx, batched):
     assert_results_equal(got, expected(batches, datagen.AGG_TYPES, CONFIG0_ACTIONS), RTOL)
 
 
+def _fmix64(x):
+    """The table hash of the device aggregate (MurmurHash3's 64-bit finalizer)."""
+    x = x.astype(np.uint64)
+    x ^= x >> np.uint64(33)
+    x *= np.uint64(0xFF51AFD7ED558CCD)
+    x ^= x >> np.uint64(33)
+    x *= np.uint64(0xC4CEB9FE1A85EC53)
+    x ^= x >> np.uint64(33)
+    return x
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_owned_partition_whose_lds_table_overflows(ctx, batched):
+    """Two partitions (the high hash bit) of 900-1023 groups each in 1024-slot
+    tables, ~12k records each: each partition is one work item, whose LDS
+    table (linear probing, at most 32 probes) overflows at that load; the item
+    then resets its partition's HBM slots, stores its LDS groups by claiming
+    slots and applies the records of the keys it lacks with atomics.  The
+    results equal the oracle's."""
+    actions = [("action_groupby", [0]), ("action_sum", [1]), ("action_countLiteral_1", [])]
+    batches = datagen.agg_host(24000, num_keys=1900, seed=9)
+    keys = np.unique(np.concatenate([b.columns[0].values for b in batches]))
+    counts = np.bincount((_fmix64(keys.view(np.uint64)) >> np.uint64(63)).astype(np.int64), minlength=2)
+    assert counts.min() >= 900 and counts.max() < 1024, counts
+    got = run_gpu(ctx, batches, datagen.AGG_TYPES, actions, batched, placement=3, expected_groups=1000)
+    assert_results_equal(got, expected(batches, datagen.AGG_TYPES, actions), RTOL)
+
+
 @pytest.mark.parametrize("placement", [0, 2, 3])
 def test_table_overflow_is_reported(ctx, placement):
     """A cardinality hint (500) far below the true 20000 groups overflows the
