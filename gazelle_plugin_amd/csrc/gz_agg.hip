@@ -560,18 +560,20 @@ __device__ __forceinline__ void merge_slot(const AggParams& p, const AggInst& I,
   }
 }
 
-// Branch-free application of RPT records to the lanes of an LDS table: one
-// switch per lane per call, and every record updates every lane (a record
-// without a slot of its own updates the trash slot; a null input contributes
-// the lane's neutral value), so no per-record branches or exec-mask changes.
-// Results equal apply_row's: adding 0 or 0.0 and min/max/or with the neutral
-// value change nothing (a sum lane is never -0.0: it starts at +0.0).
-// Records of at most two used columns (every operand in registers, no memory
-// access but the LDS atomics).  w1: record word 1 (first row | nulls << 48);
-// u0/u1: used words 0 and 1.
+// Application of RPT records to the lanes of an LDS table with no per-record
+// branch: one switch per lane per call, and every record with a slot updates
+// every lane (a null input contributes the lane's neutral value).  A record
+// without a slot of its own (past the item's end, or its LDS probe failed)
+// carries slot `none` and issues no atomic: its lane is masked off, since the
+// atomics of a wave's idle lanes on one shared slot would serialise.  Results
+// equal apply_row's: adding 0 or 0.0 and min/max/or with the neutral value
+// change nothing (a sum lane is never -0.0: it starts at +0.0).  Records of at
+// most two used columns (every operand in registers, no memory access but the
+// LDS atomics).  w1: record word 1 (first row | nulls << 48); u0/u1: used
+// words 0 and 1.
 template <int RPT>
 __device__ __forceinline__ void lds_apply_records(const AggParams& p, int64_t* tab, int64_t LS, const int (&slot)[RPT],
-                                                  const int64_t (&w1)[RPT], const int64_t (&u0)[RPT],
+                                                  int none, const int64_t (&w1)[RPT], const int64_t (&u0)[RPT],
                                                   const int64_t (&u1)[RPT]) {
   int64_t grow[RPT];
   uint32_t nulls[RPT];
@@ -579,7 +581,7 @@ __device__ __forceinline__ void lds_apply_records(const AggParams& p, int64_t* t
   for (int k = 0; k < RPT; k++) {
     grow[k] = (int64_t)((uint64_t)w1[k] & ((1ull << 48) - 1));
     nulls[k] = (uint32_t)((uint64_t)w1[k] >> 48);
-    atomicMin((long long*)(tab + LS + slot[k]), (long long)grow[k]);
+    if (slot[k] != none) atomicMin((long long*)(tab + LS + slot[k]), (long long)grow[k]);
   }
   for (int l = 0; l < p.nl; l++) {
     const Lane ln = p.lanes[l];
@@ -603,7 +605,7 @@ __device__ __forceinline__ void lds_apply_records(const AggParams& p, int64_t* t
       for (int k = 0; k < RPT; k++) dv[k] = (double)word(u, k);
     }
     auto f64 = [&](int k) -> double { return dv[k]; };
-#define GZ_EACH(STMT) _Pragma("unroll") for (int k = 0; k < RPT; k++) { unsigned long long* up = (unsigned long long*)(lp + slot[k]); long long* sp = (long long*)(lp + slot[k]); (void)up; (void)sp; STMT; }
+#define GZ_EACH(STMT) _Pragma("unroll") for (int k = 0; k < RPT; k++) { if (slot[k] == none) continue; unsigned long long* up = (unsigned long long*)(lp + slot[k]); long long* sp = (long long*)(lp + slot[k]); (void)up; (void)sp; STMT; }
     switch (ln.op) {
       case L_CNT: GZ_EACH(atomicAdd(up, 1ull)) break;
       case L_NN: GZ_EACH(atomicAdd(up, ok(u, k) ? 1ull : 0ull)) break;
@@ -1541,8 +1543,8 @@ __global__ __launch_bounds__(UB, 2 * UB / 256) void agg_part_update_kernel(AggPa
         if (p.nu <= 2) {
           int slot[kUpdRPT];
 #pragma unroll
-          for (int k = 0; k < kUpdRPT; k++) slot[k] = ls[k] >= 0 ? ls[k] : p.lcap + 2;  // trash slot
-          lds_apply_records<kUpdRPT>(p, s_tab, LS, slot, w1w, u0, u1);
+          for (int k = 0; k < kUpdRPT; k++) slot[k] = ls[k] >= 0 ? ls[k] : -1;
+          lds_apply_records<kUpdRPT>(p, s_tab, LS, slot, -1, w1w, u0, u1);
         } else {
 #pragma unroll
           for (int k = 0; k < kUpdRPT; k++)
