@@ -1842,6 +1842,7 @@ struct FinishParams {
   const unsigned long long* svals;  // slots
   int64_t n;
   int key_type;
+  int nl;  // lanes per slot
   int nres;
   ResultSpec res[kMaxResults];
   void* out_values[kMaxResults];
@@ -1864,15 +1865,29 @@ __device__ __forceinline__ void store_typed(void* out, int64_t i, int t, int64_t
   }
 }
 
+// One output group per thread, in first-seen order, so its slot is a random
+// one: the group's key and all its lanes are loaded at once (the lanes into the
+// thread's own column of LDS, s_lv[l * blockDim + thread]) before any result is
+// computed, so each of its lines is fetched once, not once per result column
+// after other groups' traffic has evicted it.
 __global__ void agg_finish_kernel(FinishParams f) {
+  extern __shared__ int64_t s_lv[];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n;
        i += (int64_t)gridDim.x * blockDim.x) {
     int inst = (int)(f.skeys[i] >> 44);
     int64_t slot = (int64_t)f.svals[i];
     const AggInst I = f.inst[inst];
-    const int64_t* b = I.base;
     const int64_t* ls = tab_lanes(I, slot);
-    auto lane = [&](int l) -> int64_t { return ls[l]; };
+    const int64_t kv = I.base[slot];
+    {
+      int64_t v[kMaxLanes];
+#pragma unroll
+      for (int l = 0; l < kMaxLanes; l++) v[l] = l < f.nl ? ls[l] : 0;
+#pragma unroll
+      for (int l = 0; l < kMaxLanes; l++)
+        if (l < f.nl) s_lv[l * blockDim.x + threadIdx.x] = v[l];
+    }
+    auto lane = [&](int l) -> int64_t { return s_lv[l * blockDim.x + threadIdx.x]; };
     for (int c = 0; c < f.nres; c++) {
       const ResultSpec& rs = f.res[c];
       bool ok = true;
@@ -1884,7 +1899,7 @@ __global__ void agg_finish_kernel(FinishParams f) {
       switch (rs.kind) {
         case GZ_ACTION_GROUPBY:
           ok = slot != (int64_t)I.cap;  // the null-key group
-          iv = b[slot];
+          iv = kv;
           if (rs.rtype == GZ_TYPE_DOUBLE || rs.rtype == GZ_TYPE_FLOAT) {
             // the group's first row's bits: the table holds -0.0 as 0.0 and
             // one NaN for all
@@ -2700,6 +2715,7 @@ static int agg_finish(gz_agg* h) {
   f.svals = (const unsigned long long*)h->svals.p;
   f.n = n;
   f.key_type = h->plan.key_type;
+  f.nl = (int)h->plan.lanes.size();
   f.nres = nres;
   for (int c = 0; c < nres; c++) {
     f.res[c] = h->plan.results[c];
@@ -2712,7 +2728,8 @@ static int agg_finish(gz_agg* h) {
   if (n > 0) {
     KernelTimer kt(ctx, "agg_finish");
     unsigned gx = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(agg_finish_kernel, dim3(gx), dim3(256), 0, ctx->stream, f);
+    hipLaunchKernelGGL(agg_finish_kernel, dim3(gx), dim3(256), sizeof(int64_t) * 256 * std::max(f.nl, 1),
+                       ctx->stream, f);
   }
   GZ_HIP(hipGetLastError());
   GZ_HIP(hipStreamSynchronize(ctx->stream));
