@@ -1698,19 +1698,33 @@ __global__ void agg_rehash_kernel(AggInst from, AggInst to, int nl) {
   }
 }
 
-// Occupied slots of all instances -> (instance << 44 | first_row, slot) pairs.
-// Occupied slots of every instance, compacted without atomics: blocks own
-// chunks of kCollectChunk slots of one instance (grid.y = instance); a first
-// pass counts each chunk, a scan over (instance, chunk) gives every chunk its
-// output position, a second pass writes (instance, first row) sort keys and slot
-// numbers in slot order.  A slot is occupied iff a row reached it.
+// Occupied slots of every instance, compacted without atomics, as
+// (instance << rbits | first_row, slot) pairs (rbits: bits of the largest
+// instance's row count): blocks own chunks of kCollectChunk slots of one
+// instance, numbered flat over the instances (instance i's chunks from
+// cpre[i]); a first pass counts each chunk, a scan over the chunks gives every
+// chunk its output position, a second pass writes (instance, first row) sort
+// keys and slot numbers in slot order.  A slot is occupied iff a row reached it.
 constexpr int kCollectChunk = 2048;  // 256 threads x 8 slots
 
-__global__ __launch_bounds__(256) void agg_occupancy_kernel(const AggInst* inst, int64_t X,
+// The instance of flat chunk g (cpre: [ninst + 1] chunk prefix), and its chunk
+// number within the instance.
+__device__ __forceinline__ int chunk_inst(const int64_t* cpre, int ninst, int64_t g, int64_t* x) {
+  int lo = 0, hi = ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cpre[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  *x = g - cpre[lo];
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void agg_occupancy_kernel(const AggInst* inst, const int64_t* cpre, int ninst,
                                                             unsigned long long* cc) {
   __shared__ uint64_t s_scan[256 / kWave + 1];
-  const int i = blockIdx.y;
-  const int64_t x = blockIdx.x;
+  int64_t x;
+  const int i = chunk_inst(cpre, ninst, blockIdx.x, &x);
   const AggInst I = inst[i];
   const int64_t S = (int64_t)I.cap + 2;
   const int64_t s0 = x * kCollectChunk + threadIdx.x * 8;
@@ -1722,15 +1736,15 @@ __global__ __launch_bounds__(256) void agg_occupancy_kernel(const AggInst* inst,
   }
   uint64_t total;
   block_exclusive_scan<uint64_t>(c, s_scan, &total);
-  if (threadIdx.x == 0) cc[(int64_t)i * X + x] = total;
+  if (threadIdx.x == 0) cc[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(256) void agg_collect_kernel(const AggInst* inst, int64_t X,
-                                                          const unsigned long long* ccpre,
+__global__ __launch_bounds__(256) void agg_collect_kernel(const AggInst* inst, const int64_t* cpre, int ninst,
+                                                          const unsigned long long* ccpre, int rbits,
                                                           unsigned long long* keys, unsigned long long* vals) {
   __shared__ uint64_t s_scan[256 / kWave + 1];
-  const int i = blockIdx.y;
-  const int64_t x = blockIdx.x;
+  int64_t x;
+  const int i = chunk_inst(cpre, ninst, blockIdx.x, &x);
   const AggInst I = inst[i];
   const int64_t S = (int64_t)I.cap + 2;
   if (x * kCollectChunk >= S) return;
@@ -1742,11 +1756,11 @@ __global__ __launch_bounds__(256) void agg_collect_kernel(const AggInst* inst, i
     fr[k] = s0 + k < S ? I.base[S + s0 + k] : INT64_MAX;
     c += fr[k] != INT64_MAX;
   }
-  uint64_t pos = ccpre[(int64_t)i * X + x] + block_exclusive_scan<uint64_t>(c, s_scan, nullptr);
+  uint64_t pos = ccpre[blockIdx.x] + block_exclusive_scan<uint64_t>(c, s_scan, nullptr);
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     if (fr[k] != INT64_MAX) {
-      keys[pos] = ((unsigned long long)i << 44) | (unsigned long long)fr[k];
+      keys[pos] = ((unsigned long long)i << rbits) | (unsigned long long)fr[k];
       vals[pos] = (unsigned long long)(s0 + k);
       pos++;
     }
@@ -1757,12 +1771,12 @@ __global__ __launch_bounds__(256) void agg_collect_kernel(const AggInst* inst, i
 // Occupied slots per instance, and at counts[ninst] the number of instances
 // whose table overflowed.
 __global__ void agg_inst_counts_kernel(const AggInst* inst, const unsigned long long* cc,
-                                       const unsigned long long* ccpre, int ninst, int64_t X,
+                                       const unsigned long long* ccpre, const int64_t* cpre, int ninst,
                                        unsigned long long* counts) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ninst) return;
-  const int64_t last = (int64_t)i * X + X - 1;
-  counts[i] = ccpre[last] + cc[last] - ccpre[(int64_t)i * X];
+  const int64_t last = cpre[i + 1] - 1;  // every instance has a chunk (cap + 2 >= 2 slots)
+  counts[i] = ccpre[last] + cc[last] - ccpre[cpre[i]];
   if (inst[i].overflow && *inst[i].overflow) atomicAdd(counts + ninst, 1ull);
 }
 
@@ -1842,7 +1856,8 @@ struct FinishParams {
   const unsigned long long* svals;  // slots
   int64_t n;
   int key_type;
-  int nl;  // lanes per slot
+  int rbits;  // sort key: instance << rbits | first row
+  int nl;     // lanes per slot
   int nres;
   ResultSpec res[kMaxResults];
   void* out_values[kMaxResults];
@@ -1874,7 +1889,7 @@ __global__ void agg_finish_kernel(FinishParams f) {
   extern __shared__ int64_t s_lv[];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int inst = (int)(f.skeys[i] >> 44);
+    int inst = (int)(f.skeys[i] >> f.rbits);
     int64_t slot = (int64_t)f.svals[i];
     const AggInst I = f.inst[inst];
     const int64_t* ls = tab_lanes(I, slot);
@@ -2012,7 +2027,7 @@ struct gz_agg {
   std::vector<int64_t> rows_seen;  // per instance
   std::vector<int64_t> g_bound;    // upper bound of groups per instance
   std::vector<int> placement;      // 1 = LDS first level, 2 = HBM only
-  DBuf d_inst, d_lane_ops, d_tiles, d_bufs, d_regs, d_est, d_blob, d_cnt;
+  DBuf d_inst, d_lane_ops, d_tiles, d_bufs, d_regs, d_est, d_blob, d_cnt, d_cpre;
   // finish
   bool finished = false;
   std::vector<int64_t> ngroups, group_off;
@@ -2629,25 +2644,31 @@ static int agg_finish(gz_agg* h) {
     h->inst_dirty = false;
   }
   int64_t maxg = 0;
-  uint64_t maxcap = 0;
+  std::vector<int64_t> cpre(h->ninst + 1, 0);  // chunks of kCollectChunk slots per instance
+  int64_t maxrows = 1;
   for (int i = 0; i < h->ninst; i++) {
     maxg += (int64_t)h->inst[i].cap + 2;
-    maxcap = std::max(maxcap, h->inst[i].cap + 2);
+    cpre[i + 1] = cpre[i] + (int64_t)((h->inst[i].cap + 2 + kCollectChunk - 1) / kCollectChunk);
+    maxrows = std::max(maxrows, h->rows_seen[i]);
   }
+  // first rows of instance i are below rows_seen[i]
+  int rbits = 1;
+  while ((1ll << rbits) < maxrows) rbits++;
   GZ_TRY(ctx_ensure(ctx, &h->skeys, sizeof(uint64_t) * maxg));
   GZ_TRY(ctx_ensure(ctx, &h->svals, sizeof(uint64_t) * maxg));
   GZ_TRY(ctx_ensure(ctx, &h->tkeys, sizeof(uint64_t) * maxg));
   GZ_TRY(ctx_ensure(ctx, &h->tvals, sizeof(uint64_t) * maxg));
   // compaction of the occupied slots: chunk counts, scan, ordered writes
-  const int64_t X = (int64_t)((maxcap + kCollectChunk - 1) / kCollectChunk);
-  const int64_t NC = X * h->ninst;
+  const int64_t NC = cpre[h->ninst];
+  GZ_TRY(upload(ctx, &h->d_cpre, cpre.data(), sizeof(int64_t) * cpre.size()));
   GZ_TRY(ctx_ensure(ctx, &h->cursor, sizeof(uint64_t) * NC));  // chunk counts
   GZ_TRY(ctx_ensure(ctx, &h->ccpre, sizeof(uint64_t) * NC));   // their exclusive scan
   GZ_TRY(ctx_ensure(ctx, &h->counts, sizeof(uint64_t) * (h->ninst + 1)));
   {
     KernelTimer kt(ctx, "agg_collect");
-    hipLaunchKernelGGL(agg_occupancy_kernel, dim3((unsigned)X, (unsigned)h->ninst), dim3(256), 0, ctx->stream,
-                       (const AggInst*)h->d_inst.p, X, (unsigned long long*)h->cursor.p);
+    hipLaunchKernelGGL(agg_occupancy_kernel, dim3((unsigned)NC), dim3(256), 0, ctx->stream,
+                       (const AggInst*)h->d_inst.p, (const int64_t*)h->d_cpre.p, h->ninst,
+                       (unsigned long long*)h->cursor.p);
     size_t tmp = 0;
     GZ_HIP(rocprim::exclusive_scan(nullptr, tmp, (const unsigned long long*)h->cursor.p,
                                    (unsigned long long*)h->ccpre.p, 0ull, (size_t)NC,
@@ -2656,14 +2677,16 @@ static int agg_finish(gz_agg* h) {
     GZ_HIP(rocprim::exclusive_scan(h->sort_tmp.p, tmp, (const unsigned long long*)h->cursor.p,
                                    (unsigned long long*)h->ccpre.p, 0ull, (size_t)NC,
                                    rocprim::plus<unsigned long long>(), ctx->stream));
-    hipLaunchKernelGGL(agg_collect_kernel, dim3((unsigned)X, (unsigned)h->ninst), dim3(256), 0, ctx->stream,
-                       (const AggInst*)h->d_inst.p, X, (const unsigned long long*)h->ccpre.p,
-                       (unsigned long long*)h->tkeys.p, (unsigned long long*)h->tvals.p);
+    hipLaunchKernelGGL(agg_collect_kernel, dim3((unsigned)NC), dim3(256), 0, ctx->stream,
+                       (const AggInst*)h->d_inst.p, (const int64_t*)h->d_cpre.p, h->ninst,
+                       (const unsigned long long*)h->ccpre.p, rbits, (unsigned long long*)h->tkeys.p,
+                       (unsigned long long*)h->tvals.p);
     hipLaunchKernelGGL(fill_u32_kernel_agg, dim3(1), dim3(64), 0, ctx->stream, (uint32_t*)h->counts.p + 2 * h->ninst,
                        (int64_t)2);
     hipLaunchKernelGGL(agg_inst_counts_kernel, dim3((unsigned)((h->ninst + 255) / 256)), dim3(256), 0, ctx->stream,
                        (const AggInst*)h->d_inst.p, (const unsigned long long*)h->cursor.p,
-                       (const unsigned long long*)h->ccpre.p, h->ninst, X, (unsigned long long*)h->counts.p);
+                       (const unsigned long long*)h->ccpre.p, (const int64_t*)h->d_cpre.p, h->ninst,
+                       (unsigned long long*)h->counts.p);
   }
   std::vector<unsigned long long> cnt(h->ninst + 1, 0);
   GZ_HIP(hipMemcpyAsync(cnt.data(), h->counts.p, sizeof(uint64_t) * (h->ninst + 1), hipMemcpyDeviceToHost,
@@ -2691,8 +2714,8 @@ static int agg_finish(gz_agg* h) {
                        (const unsigned long long*)h->tkeys.p, (const unsigned long long*)h->tvals.p, n,
                        (unsigned long long*)h->skeys.p, (unsigned long long*)h->svals.p);
   } else if (n > 0) {
-    int end_bit = 44;
-    while ((1ll << (end_bit - 44)) < h->ninst) end_bit++;
+    int end_bit = rbits;
+    while ((1ll << (end_bit - rbits)) < h->ninst) end_bit++;
     size_t tmp = 0;
     GZ_HIP(rocprim::radix_sort_pairs(nullptr, tmp, (unsigned long long*)h->tkeys.p,
                                      (unsigned long long*)h->skeys.p, (unsigned long long*)h->tvals.p,
@@ -2715,6 +2738,7 @@ static int agg_finish(gz_agg* h) {
   f.svals = (const unsigned long long*)h->svals.p;
   f.n = n;
   f.key_type = h->plan.key_type;
+  f.rbits = rbits;
   f.nl = (int)h->plan.lanes.size();
   f.nres = nres;
   for (int c = 0; c < nres; c++) {
@@ -3493,7 +3517,7 @@ int gz_agg_release(gz_agg* h) {
   for (auto& b : h->counters) ctx_free(ctx, &b);
   for (auto& b : h->res_values) ctx_free(ctx, &b);
   for (auto& b : h->res_valid) ctx_free(ctx, &b);
-  for (DBuf* b : {&h->d_inst, &h->d_lane_ops, &h->d_tiles, &h->d_bufs, &h->d_regs, &h->d_est, &h->skeys,
+  for (DBuf* b : {&h->d_cpre, &h->d_inst, &h->d_lane_ops, &h->d_tiles, &h->d_bufs, &h->d_regs, &h->d_est, &h->skeys,
                   &h->svals, &h->tkeys, &h->tvals, &h->cursor, &h->ccpre, &h->counts, &h->sort_tmp, &h->L.d_hbm,
                   &h->L.d_lds, &h->L.d_slot_prefix, &h->L.d_bufs, &h->d_part, &h->pinst,
                   &h->recs, &h->pcnt, &h->pprefix, &h->pscan_tmp, &h->d_fix,
