@@ -319,24 +319,45 @@ struct PayloadParams {
   u32x4* out;
 };
 
+__device__ __forceinline__ u32x4 payload_record(const PayloadParams& q, int64_t r) {
+  uint64_t lo = 0, hi = 0;
+  uint32_t vm = 0;
+  for (int c = 0; c < q.ncol; c++) {
+    const int w = q.width[c], off = q.off[c];
+    uint64_t v;
+    const uint8_t* src = q.bvalues[c] + r * w;
+    if (w == 8) v = gld((const uint64_t*)src);
+    else if (w == 4) v = gld((const uint32_t*)src);
+    else if (w == 2) v = gld((const uint16_t*)src);
+    else v = gld(src);
+    if (off < 8) lo |= v << (8 * off);
+    else hi |= v << (8 * (off - 8));
+    if (gld(q.bvalid[c] + r)) vm |= 1u << q.bit[c];
+  }
+  return u32x4{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, vm};
+}
+
 __global__ void join_payload_kernel(PayloadParams q) {
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < q.n;
+       r += (int64_t)gridDim.x * blockDim.x)
+    gst(q.out + r, payload_record(q, r));
+}
+
+// The payload of a table whose keys each name one build row, stored by slot:
+// the row a key's slot keeps (start[slot]: the inserting row, or the smallest
+// with skip-duplicate building) goes to record `slot`, the row of the key equal
+// to the empty marker to record cap; so the probe's match code (the slot) is
+// the record's index and the count pass reads no start.
+__global__ void join_payload_slot_kernel(PayloadParams q, const int32_t* row_slot, const int32_t* start,
+                                         int64_t special_start, uint64_t cap) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < q.n;
        r += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t lo = 0, hi = 0;
-    uint32_t vm = 0;
-    for (int c = 0; c < q.ncol; c++) {
-      const int w = q.width[c], off = q.off[c];
-      uint64_t v;
-      const uint8_t* src = q.bvalues[c] + r * w;
-      if (w == 8) v = gld((const uint64_t*)src);
-      else if (w == 4) v = gld((const uint32_t*)src);
-      else if (w == 2) v = gld((const uint16_t*)src);
-      else v = gld(src);
-      if (off < 8) lo |= v << (8 * off);
-      else hi |= v << (8 * (off - 8));
-      if (gld(q.bvalid[c] + r)) vm |= 1u << q.bit[c];
-    }
-    gst(q.out + r, u32x4{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, vm});
+    const int32_t s = row_slot[r];
+    int64_t dst;
+    if (s >= 0 && start[s] == (int32_t)r) dst = s;
+    else if (s == -2 && r == special_start) dst = (int64_t)cap;
+    else continue;
+    gst(q.out + dst, payload_record(q, r));
   }
 }
 
@@ -376,6 +397,8 @@ struct ProbeParams {
   // build row), -1 no match, -2 the kJEmpty key, -3 a null key
   int32_t* match;
   int match_rows;  // inner joins over unique keys: the count pass resolves rows
+  int by_slot;     // the payload is stored by slot (its record index is the match code)
+  int64_t b_special;  // payload record of the key equal to kJEmpty
   uint32_t* tile_count;
   const uint64_t* tile_off;
   int ncols;
@@ -682,7 +705,7 @@ __global__ __launch_bounds__(BLOCK) void join_probe_write01_kernel(ProbeParams p
       const int k = k0 + g;
       b[g] = -1;
       if (inner && (pre[k] >> 31))
-        b[g] = m[k] < 0 ? (int32_t)p.J.special_start : p.match_rows ? m[k] : gld(p.J.start + m[k]);
+        b[g] = m[k] < 0 ? (int32_t)p.b_special : p.match_rows || p.by_slot ? m[k] : gld(p.J.start + m[k]);
     }
 #pragma unroll
     for (int g = 0; g < G; g++) pl[g] = p.payload && b[g] >= 0 ? gld(p.payload + b[g]) : u32x4{0u, 0u, 0u, 0u};
@@ -1328,6 +1351,7 @@ struct gz_probe {
   DBuf payload;
   bool payload_ready = false;
   std::vector<int> pay_off;  // per output column: byte offset in the record, -1 not in it
+  int pay_by_slot = -1;      // the payload is stored by slot (-1: not decided yet)
   struct Out {
     int type = 0, width = 0;
     DBuf values, valid, offsets, data, lens, bits;  // bits: Arrow bitmap, packed on request
@@ -1352,6 +1376,34 @@ static int probe_codes(gz_probe* h, int32_t nb, const gz_batch* batches, std::ve
   with_code_column(h->ocols, h->otypes.data(), nb, batches, codes, addrs, sizes, out,
                    h->kcols.size() == 1 ? h->kcols[0] : -1);
   return GZ_OK;
+}
+
+// Payload layout of a probe handle's build output columns: fixed-width ones of
+// at most 8 bytes packed by decreasing width into 12 bytes; off[c] is output
+// column c's byte offset, -1 when not in the record.  False (every off -1) when
+// there is no such column or they do not fit.
+static bool payload_layout(const gz_probe* h, std::vector<int>* off) {
+  const int nres = (int)h->sides.size();
+  off->assign(nres, -1);
+  std::vector<int> order;
+  auto width = [&](int c) { return h->outs[c].type == GZ_TYPE_BOOL ? 1 : type_width(h->outs[c].type); };
+  for (int c = 0; c < nres; c++) {
+    const int t = h->outs[c].type;
+    if (h->sides[c] == 0 && !type_is_binary(t) && t != GZ_TYPE_NA && width(c) <= 8) order.push_back(c);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return width(a) > width(b); });
+  int o = 0;
+  for (int c : order) {
+    const int wd = width(c);
+    o = (o + wd - 1) / wd * wd;
+    if (o + wd > 12) {
+      off->assign(nres, -1);
+      return false;
+    }
+    (*off)[c] = o;
+    o += wd;
+  }
+  return !order.empty();
 }
 
 static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* out_rows,
@@ -1419,7 +1471,20 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   p.J.cap = r->cap;
   p.J.special_start = r->special_start;
   p.J.special_count = r->special_count;
-  p.match_rows = p.join_type == GZ_JOIN_INNER && !p.J.rowlist ? 1 : 0;
+  // an inner join over unique keys whose build outputs all fit the payload
+  // reads the payload by slot: its count pass keeps the slot as match code
+  if (h->pay_by_slot < 0) {
+    std::vector<int> off;
+    bool all = payload_layout(h, &off) && p.join_type == GZ_JOIN_INNER && !p.J.rowlist;
+    for (size_t c = 0; c < h->sides.size() && all; c++) {
+      const int t = h->outs[c].type;
+      if (type_is_binary(t) || (h->sides[c] == 0 && t != GZ_TYPE_NA && off[c] < 0)) all = false;
+    }
+    h->pay_by_slot = all ? 1 : 0;
+  }
+  p.by_slot = h->pay_by_slot;
+  p.match_rows = p.join_type == GZ_JOIN_INNER && !p.J.rowlist && !p.by_slot ? 1 : 0;
+  p.b_special = p.by_slot ? (int64_t)r->cap : r->special_start;
   std::vector<uint32_t> tc(std::max<int64_t>(T, 1), 0);
   uint64_t total = 0;
   // a call of one small batch whose probe rows emit at most one row each,
@@ -1521,40 +1586,35 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
     p.pair_build = (int32_t*)h->pair_build.p;
     p.write_pairs = 1;
   }
-  // build columns from the row-major payload: fixed-width columns packed by
-  // decreasing width into 12 bytes (a validity bit each in the last dword)
+  // build columns from the payload: fixed-width columns packed by decreasing
+  // width into 12 bytes (a validity bit each in the last dword), one record
+  // per build row, or per slot (pay_by_slot)
   if (!h->payload_ready && r->n > 0) {
-    h->pay_off.assign(nres, -1);
-    PayloadParams q;
-    memset(&q, 0, sizeof(q));
-    std::vector<int> order;
-    for (int c = 0; c < nres; c++)
-      if (p.out[c].side == 0 && !p.out[c].binary && p.out[c].type != GZ_TYPE_NA && p.out[c].width <= 8)
-        order.push_back(c);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return p.out[a].width > p.out[b].width; });
-    int off = 0;
-    bool fits = !order.empty();
-    for (int c : order) {
-      const int wd = p.out[c].width;
-      off = (off + wd - 1) / wd * wd;
-      if (off + wd > 12) { fits = false; break; }
-      q.bvalid[q.ncol] = p.out[c].bvalid;
-      q.bvalues[q.ncol] = p.out[c].bvalues;
-      q.width[q.ncol] = wd;
-      q.off[q.ncol] = off;
-      q.bit[q.ncol] = c;
-      h->pay_off[c] = off;
-      q.ncol++;
-      off += wd;
-    }
+    const bool fits = payload_layout(h, &h->pay_off);
     if (fits) {
-      GZ_TRY(ctx_ensure(ctx, &h->payload, sizeof(u32x4) * r->n));
+      PayloadParams q;
+      memset(&q, 0, sizeof(q));
+      for (int c = 0; c < nres; c++) {
+        if (h->pay_off[c] < 0) continue;
+        q.bvalid[q.ncol] = p.out[c].bvalid;
+        q.bvalues[q.ncol] = p.out[c].bvalues;
+        q.width[q.ncol] = p.out[c].width;
+        q.off[q.ncol] = h->pay_off[c];
+        q.bit[q.ncol] = c;
+        q.ncol++;
+      }
       q.n = r->n;
-      q.out = (u32x4*)h->payload.p;
       unsigned gb = (unsigned)std::min<int64_t>((r->n + 255) / 256, 8192);
-      hipLaunchKernelGGL(join_payload_kernel, dim3(gb), dim3(256), 0, st, q);
-    } else {
-      h->pay_off.assign(nres, -1);
+      if (h->pay_by_slot) {
+        GZ_TRY(ctx_ensure(ctx, &h->payload, sizeof(u32x4) * (r->cap + 1)));
+        q.out = (u32x4*)h->payload.p;
+        hipLaunchKernelGGL(join_payload_slot_kernel, dim3(gb), dim3(256), 0, st, q, (const int32_t*)r->row_slot.p,
+                           (const int32_t*)r->start.p, r->special_start, (uint64_t)r->cap);
+      } else {
+        GZ_TRY(ctx_ensure(ctx, &h->payload, sizeof(u32x4) * r->n));
+        q.out = (u32x4*)h->payload.p;
+        hipLaunchKernelGGL(join_payload_kernel, dim3(gb), dim3(256), 0, st, q);
+      }
     }
     h->payload_ready = true;
   }
