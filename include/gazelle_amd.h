@@ -137,10 +137,13 @@ void gz_split_options_defaults(gz_split_options* opts);
  * "single".  The Gandiva key expressions become column indices (field(k)
  * references, as ColumnarShuffleExchangeExec.scala:307-321 builds them).  For
  * "range" column 0 is the int32 partition id and is removed from the output
- * (splitter.cc:1497-1500).  Limits: up to 128 columns and 16 hash keys; up to
- * 2048 partitions (the reference takes up to 65536: more return
- * NotImplemented); num_partitions x (1 + utf8/binary columns) counters of at
- * most 128 KiB per tile. */
+ * (splitter.cc:1497-1500).  Limits: up to 128 columns and 16 hash keys; 1 to
+ * 65536 partitions, as the reference (splitter.cc:218-221; more is Invalid).
+ * Up to 2048 partitions, with num_partitions x (1 + utf8/binary columns)
+ * counters of at most 128 KiB, the LDS-staged kernels split; past that, a
+ * sorted path (partition ids, a stable radix sort by id, gathers) gives the
+ * same layout.  A call's batches sit in tiles of 4096 rows on the sorted path,
+ * and at most 2^20 tiles fit one call (CapacityError beyond). */
 int gz_split_make(gz_context* ctx, const char* short_name, int32_t num_cols,
                   const int32_t* col_types, int32_t num_partitions,
                   const int32_t* key_cols, int32_t num_keys,
