@@ -1258,7 +1258,7 @@ struct PartItem {
   int32_t b, owned;
   int64_t a, e;
 };
-constexpr int64_t kPartItemRecords = 16384;
+constexpr int64_t kPartItemRecords = 32768;
 
 // First record of global partition g = (instance pinst[g], partition b): the
 // scanned count of its first tile; past the last partition, nrec.
