@@ -1486,6 +1486,7 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   p.match_rows = p.join_type == GZ_JOIN_INNER && !p.J.rowlist && !p.by_slot ? 1 : 0;
   p.b_special = p.by_slot ? (int64_t)r->cap : r->special_start;
   std::vector<uint32_t> tc(std::max<int64_t>(T, 1), 0);
+  bool tc_device = false;  // the per-tile output counts are on the device only
   uint64_t total = 0;
   // a call of one small batch whose probe rows emit at most one row each,
   // without utf8 outputs: one launch probes and writes (outputs sized for the
@@ -1546,9 +1547,18 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
     GZ_TRY(ctx_ensure(ctx, &h->scan_tmp, tmp));
     GZ_HIP(rocprim::exclusive_scan(h->scan_tmp.p, tmp, (const uint32_t*)h->tile_count.p, (uint64_t*)h->tile_off.p,
                                    (uint64_t)0, (size_t)T, rocprim::plus<uint64_t>(), st));
-    GZ_HIP(hipMemcpyAsync(tc.data(), h->tile_count.p, sizeof(uint32_t) * T, hipMemcpyDeviceToHost, st));
+    // the output size: the last tile's offset plus its count (the per-tile
+    // counts come back only when rows per batch are asked for, after the
+    // write pass is queued)
+    uint64_t last_off = 0;
+    uint32_t last_cnt = 0;
+    GZ_HIP(hipMemcpyAsync(&last_off, (const uint64_t*)h->tile_off.p + (T - 1), sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipMemcpyAsync(&last_cnt, (const uint32_t*)h->tile_count.p + (T - 1), sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
-    for (int64_t t = 0; t < T; t++) total += tc[t];
+    total = last_off + last_cnt;
+    tc_device = true;
   }
   // outputs
   bool any_binary = false;
@@ -1713,6 +1723,10 @@ static int probe_run(gz_probe* h, int32_t nb, const gz_batch* batches, int64_t* 
   h->out_rows = (int64_t)total;
   if (out_rows) *out_rows = (int64_t)total;
   if (out_batch_rows) {
+    if (tc_device) {
+      GZ_HIP(hipMemcpyAsync(tc.data(), h->tile_count.p, sizeof(uint32_t) * T, hipMemcpyDeviceToHost, st));
+      GZ_HIP(hipStreamSynchronize(st));
+    }
     for (int bi = 0; bi < nb; bi++) out_batch_rows[bi] = 0;
     for (int64_t t = 0; t < T; t++) out_batch_rows[tiles[t].batch] += tc[t];
   }
