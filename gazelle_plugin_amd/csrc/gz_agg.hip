@@ -3204,7 +3204,7 @@ static int agg_autotune(gz_agg* h, int32_t nb, const gz_batch* batches, const in
     switch (dim) {
       case 0: for (int v : {2048, 4096, 8192}) if (v != base.tile_rows) with([&](Knobs& k) { k.tile_rows = v; }); break;
       case 1: for (int v : {0, 256, 1024}) if (v != base.threshold) with([&](Knobs& k) { k.threshold = v; }); break;
-      case 2: for (int v : {0, 256, 512}) if (v != base.part) with([&](Knobs& k) { k.part = v; }); break;
+      case 2: for (int v : {0, 256, 512, 768, 1024}) if (v != base.part) with([&](Knobs& k) { k.part = v; }); break;
       case 3: for (double v : {0.35, 0.5, 0.7}) if (v != base.lf) with([&](Knobs& k) { k.lf = v; }); break;
       case 4: for (int v : {256, 512}) if (v != base.block) with([&](Knobs& k) { k.block = v; }); break;
     }
