@@ -214,6 +214,13 @@ struct KeyScratch {
   DBuf rowslot, d_kb, d_prefix;
 };
 int kd_make(gz_context* ctx, int32_t K, const int32_t* key_types, KeyDict** out);
+// Order of K key columns for the key dictionary, which takes a utf8 key last:
+// perm[i] = the caller's key at position i (the one utf8 key, if any, moved to
+// the end).  With equal null and empty masks, the UnsafeRow bytes of two key
+// rows (unsafe_row.h:126-137: fixed-width values, then the utf8 bytes with no
+// length) can only be equal when their single utf8 key has the same length,
+// so their equality is field by field in any key order.
+std::vector<int> kd_key_order(int K, const int32_t* key_types);
 void kd_release(KeyDict* d);
 void kd_scratch_free(gz_context* ctx, KeyScratch* sc);
 // Codes of the batches' key tuples (key_cols of a ncols/types schema), on

@@ -960,7 +960,8 @@ struct gz_relation {
   // several key columns (or a utf8 key): rows are keyed by a dictionary code
   // appended as the last column
   KeyDict* kd = nullptr;
-  std::vector<int> kcols, otypes;
+  std::vector<int> kcols, otypes;  // kcols in the dictionary's key order
+  std::vector<int> kperm;          // dictionary position -> the caller's key index
   int ocols = 0;
   KeyScratch ksc;
   DBuf d_atiles, d_abufs;  // tile and buffer tables of a batched append
@@ -1891,8 +1892,13 @@ int gz_relation_make_keys(gz_context* ctx, int32_t num_cols, const int32_t* col_
   if (num_keys == 1 && col_types[key_cols[0]] != GZ_TYPE_STRING)
     return gz_relation_make(ctx, num_cols, col_types, key_cols[0], builder_type, out);
   if (num_cols + 1 > kJoinMaxCols) return fail(GZ_NOT_IMPLEMENTED, "more than 31 build columns with several keys");
+  // the dictionary's key order (one utf8 key last); probe keys and
+  // normalisation flags, given in the caller's order, follow it
   std::vector<int32_t> kt, ext(col_types, col_types + num_cols);
   for (int j = 0; j < num_keys; j++) kt.push_back(col_types[key_cols[j]]);
+  const std::vector<int> perm = kd_key_order(num_keys, kt.data());
+  kt.clear();
+  for (int j = 0; j < num_keys; j++) kt.push_back(col_types[key_cols[perm[j]]]);
   ext.push_back(GZ_TYPE_INT64);
   KeyDict* kd = nullptr;
   GZ_TRY(kd_make(ctx, num_keys, kt.data(), &kd));
@@ -1903,7 +1909,8 @@ int gz_relation_make_keys(gz_context* ctx, int32_t num_cols, const int32_t* col_
     return rc;
   }
   r->kd = kd;
-  r->kcols.assign(key_cols, key_cols + num_keys);
+  r->kperm = perm;
+  for (int j = 0; j < num_keys; j++) r->kcols.push_back(key_cols[perm[j]]);
   r->otypes.assign(col_types, col_types + num_cols);
   r->ocols = num_cols;
   *out = r;
@@ -1916,12 +1923,13 @@ int gz_relation_normalize_keys(gz_relation* r, int32_t num_keys, const int32_t* 
   const int K = r->kd ? (int)r->kcols.size() : 1;
   if (num_keys != K) return fail(GZ_INVALID, "the relation has " + std::to_string(K) + " key columns");
   uint32_t mask = 0;
-  for (int j = 0; j < K; j++) {
+  for (int i = 0; i < K; i++) {  // dictionary position i holds the caller's key j
+    const int j = r->kd ? r->kperm[i] : i;
     if (!normalize[j]) continue;
-    const int t = r->kd ? r->otypes[r->kcols[j]] : r->types[r->key_col];
+    const int t = r->kd ? r->otypes[r->kcols[i]] : r->types[r->key_col];
     if (t != GZ_TYPE_FLOAT && t != GZ_TYPE_DOUBLE)
       return fail(GZ_TYPE_ERROR, "only float and double keys are normalised (key " + std::to_string(j) + ")");
-    mask |= 1u << j;
+    mask |= 1u << i;
   }
   if (r->kd) return kd_set_normalize(r->kd, mask);
   r->normalize = mask != 0;
@@ -1940,10 +1948,10 @@ int gz_probe_make_keys(gz_relation* r, int32_t num_probe_cols, const int32_t* pr
                          num_result_cols, result_sides, result_indices, out);
   }
   if (num_keys != (int)r->kcols.size()) return fail(GZ_INVALID, "probe and build key counts differ");
-  for (int j = 0; j < num_keys; j++) {
-    if (probe_key_cols[j] < 0 || probe_key_cols[j] >= num_probe_cols) return fail(GZ_INVALID, "probe key out of range");
-    if (probe_types[probe_key_cols[j]] != r->otypes[r->kcols[j]])
-      return fail(GZ_TYPE_ERROR, "probe and build key types differ");
+  for (int i = 0; i < num_keys; i++) {  // dictionary position i: the caller's key kperm[i]
+    const int pc = probe_key_cols[r->kperm[i]];
+    if (pc < 0 || pc >= num_probe_cols) return fail(GZ_INVALID, "probe key out of range");
+    if (probe_types[pc] != r->otypes[r->kcols[i]]) return fail(GZ_TYPE_ERROR, "probe and build key types differ");
   }
   // Several keys: the reference's UnsafeRow anti probe (conditioned_probe_kernel.cc:957-982)
   // never reads is_null_aware_anti_join_: a probe row is kept unless its key
@@ -1963,7 +1971,7 @@ int gz_probe_make_keys(gz_relation* r, int32_t num_probe_cols, const int32_t* pr
   GZ_TRY(gz_probe_make(r, num_probe_cols + 1, ext.data(), num_probe_cols, join_type, na, num_result_cols,
                        result_sides, result_indices, &h));
   h->kd = r->kd;
-  h->kcols.assign(probe_key_cols, probe_key_cols + num_keys);
+  for (int i = 0; i < num_keys; i++) h->kcols.push_back(probe_key_cols[r->kperm[i]]);
   h->otypes.assign(probe_types, probe_types + num_probe_cols);
   h->ocols = num_probe_cols;
   *out = h;

@@ -22,11 +22,13 @@
 // The aggregate's own first-row order then gives the reference's group order,
 // and the finish gathers each key column from the stored tuples by code.
 //
-// A utf8 key (alone, or the last of several keys) enters the tuple as a 64-bit
-// hash of its bytes; equal hashes are confirmed on the bytes, and every code
-// keeps its string in a device byte heap.  Alone it groups like the reference's
-// LargeStringHashMap (the null key its own group); last in an UnsafeRow its
-// bytes end the row, so tuple equality is the row's byte equality there too.
+// A utf8 key (alone, or the last of several keys: callers order the keys with
+// kd_key_order) enters the tuple as a 64-bit hash of its bytes; equal hashes
+// are confirmed on the bytes, and every code keeps its string in a device byte
+// heap.  Alone it groups like the reference's LargeStringHashMap (the null key
+// its own group); in an UnsafeRow with one utf8 key, equal row bytes mean equal
+// fields wherever the utf8 key sits, so tuple equality is the row's byte
+// equality.
 #include <algorithm>
 #include <cstring>
 #include <rocprim/rocprim.hpp>
@@ -384,6 +386,20 @@ static bool key_type_ok(int t) {
   }
 }
 
+std::vector<int> kd_key_order(int K, const int32_t* key_types) {
+  std::vector<int> perm;
+  int nstr = 0, sj = -1;
+  for (int j = 0; j < K; j++)
+    if (key_types[j] == GZ_TYPE_STRING) {
+      nstr++;
+      sj = j;
+    }
+  for (int j = 0; j < K; j++)
+    if (nstr != 1 || j != sj) perm.push_back(j);
+  if (nstr == 1) perm.push_back(sj);
+  return perm;
+}
+
 int kd_make(gz_context* ctx, int32_t K, const int32_t* key_types, KeyDict** out) {
   if (K < 1 || K > kMaxKeys) return fail(GZ_NOT_IMPLEMENTED, "1 to 8 key columns");
   auto* d = new KeyDict();
@@ -465,8 +481,22 @@ int mk_make(gz_context* ctx, int32_t ncols, const int32_t* types, int32_t nact, 
       for (int r = 0; r < nres; r++) m->res_map.push_back(inner_pos++);
     }
   }
+  // the dictionary's key order: one utf8 key goes last; result key columns
+  // keep the plan's order
   std::vector<int32_t> ktypes;
   for (int c : m->key_cols) ktypes.push_back(types[c]);
+  const std::vector<int> perm = kd_key_order((int)ktypes.size(), ktypes.data());
+  std::vector<int> pos(perm.size());  // caller key j -> dictionary position
+  std::vector<int> kc;
+  ktypes.clear();
+  for (size_t i = 0; i < perm.size(); i++) {
+    pos[perm[i]] = (int)i;
+    kc.push_back(m->key_cols[perm[i]]);
+    ktypes.push_back(types[kc.back()]);
+  }
+  m->key_cols = kc;
+  for (int& r : m->res_map)
+    if (r < 0) r = -(pos[-r - 1] + 1);
   int rc = kd_make(ctx, (int32_t)ktypes.size(), ktypes.data(), &m->kd);
   if (rc) {
     delete m;

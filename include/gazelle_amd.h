@@ -348,8 +348,10 @@ void gz_agg_options_defaults(gz_agg_options* opts);
  * columns together with the reference's UnsafeRow key equality (the same keys
  * null, identical bytes in the others: hash_aggregate_kernel.cc:970-1000), each
  * key fixed-width numeric (integers, dates, timestamps, float, double) except
- * that the last may be utf8.  A utf8 key groups by its bytes, the null key its
- * own group (:1001-1020).  The gz_agg_set_* forms take one integer key only. */
+ * that one, in any position, may be utf8 (two utf8 keys are NotImplemented: the
+ * UnsafeRow joins their bytes with no length).  A utf8 key groups by its bytes,
+ * the null key its own group (:1001-1020).  The gz_agg_set_* forms take one
+ * integer key only. */
 int gz_agg_make(gz_context* ctx, int32_t num_cols, const int32_t* col_types,
                 int32_t num_actions, const gz_agg_action* actions,
                 const gz_agg_options* opts, gz_agg** out);

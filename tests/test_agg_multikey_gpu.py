@@ -122,15 +122,13 @@ def test_multi_key_types_and_equality_cases(ctx):
 
 
 def test_multi_key_rejects_what_it_does_not_group(ctx):
-    """A utf8 key before another key (its bytes would not end the UnsafeRow)
-    and the set/planned forms of a multi-key plan are NotImplemented."""
+    """Two utf8 keys (their bytes meet in the UnsafeRow with no length, so
+    ("ab", "c") and ("a", "bc") are one group there) and the set/planned forms
+    of a multi-key plan are NotImplemented."""
     h = ctypes.c_void_p()
     acts = (L.GzAggAction * 2)()
     acts[0].kind, acts[0].input0, acts[0].input1 = 0, 0, -1
     acts[1].kind, acts[1].input0, acts[1].input1 = 0, 1, -1
-    types = L.i32_array([T.STRING, T.INT64])
-    rc = L.lib().gz_agg_make(ctx.handle, 2, types, 2, acts, None, ctypes.byref(h))
-    assert rc == 10, rc  # GZ_NOT_IMPLEMENTED
     types = L.i32_array([T.STRING, T.STRING])
     assert L.lib().gz_agg_make(ctx.handle, 2, types, 2, acts, None, ctypes.byref(h)) == 10
     types = L.i32_array([T.INT64, T.INT32])
@@ -176,6 +174,21 @@ def test_utf8_key_alone(ctx, mode):
     got = run_gpu(ctx, batches, UTF8_TYPES, acts, mode)
     exp = expected(batches, UTF8_TYPES, acts)
     assert len(exp[1][1]) > 250
+    assert_results_equal(got, exp, RTOL)
+
+
+@pytest.mark.parametrize("mode", ["batched", "one", "calls"])
+def test_utf8_then_int_keys(ctx, mode):
+    """(utf8, int32) keys: the utf8 key first in the UnsafeRow.  With equal
+    null and empty masks, equal row bytes need equal utf8 lengths, so groups
+    are field by field; the key columns come back in the plan's order."""
+    batches = _utf8_batches([4096] * 4 + [77, 0, 1], seed=23)
+    acts = [("action_groupby", [1]), ("action_sum", [2]), ("action_groupby", [0]), ("action_countLiteral_1", []),
+            ("action_min", [2])]
+    got = run_gpu(ctx, batches, UTF8_TYPES, acts, mode)
+    exp = expected(batches, UTF8_TYPES, acts)
+    assert len(exp[0][1]) > 1000
+    _assert_keys_bits(got, exp, [2])
     assert_results_equal(got, exp, RTOL)
 
 

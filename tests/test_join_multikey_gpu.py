@@ -91,6 +91,34 @@ def test_int_and_utf8_keys_join(ctx):
         check_output(g, e)
 
 
+@pytest.mark.parametrize("join_type", ["inner", "semi", "anti"])
+def test_utf8_then_int_keys_join(ctx, join_type):
+    """(utf8, int32) keys, the utf8 key first on both sides: the same matches
+    as field-by-field equality (one utf8 key in the UnsafeRow)."""
+    rng = np.random.default_rng(4)
+    pool = [b"", b"a", b"ab", b"abc", b"b" * 70, b"\x00", b"ba"]
+
+    def batch(n, extra):
+        from gazelle_plugin_amd.batch import bitmap_from_bools
+        enc = [pool[i] for i in rng.integers(0, len(pool), n)]
+        off = np.zeros(n + 1, np.int32)
+        off[1:] = np.cumsum([len(e) for e in enc])
+        data = np.frombuffer(b"".join(enc), np.uint8).copy() if off[-1] else np.zeros(0, np.uint8)
+        return HostBatch(n, [HostColumn(T.INT32, rng.integers(0, 3, n).astype(np.int32),
+                                        bitmap_from_bools(rng.random(n) > 0.1)),
+                             HostColumn(T.STRING, data, bitmap_from_bools(rng.random(n) > 0.1), off),
+                             HostColumn(T.INT64, rng.integers(0, 1000, n).astype(np.int64) + extra)])
+
+    build = [batch(150, 0), batch(40, 10000)]
+    probe = [batch(300, 20000), batch(2, 30000)]
+    types = [T.INT32, T.STRING, T.INT64]
+    result = [(1, 2), (0, 2), (0, 1)] if join_type == "inner" else [(1, 2)]
+    got = run_case(ctx, build, probe, types, types, [1, 0], [1, 0], join_type, False, result, True)
+    exp = expected(build, probe, [1, 0], [1, 0], join_type, result)
+    for g, e in zip(got, exp):
+        check_output(g, e)
+
+
 def test_multi_key_join_rejects(ctx):
     """Key types that differ between the sides."""
     from gazelle_plugin_amd.join import HashRelationKernel
