@@ -788,7 +788,7 @@ __device__ void lds_reset(const AggParams& p, int64_t* l, int64_t LS) {
 
 __global__ __launch_bounds__(256) void agg_update_lds_kernel(AggParams p) {
   extern __shared__ int64_t s_tab[];
-  const int64_t LS = p.lcap + 3;  // stride: lcap slots, 2 special, 1 trash
+  const int64_t LS = p.lcap + 3;  // stride: lcap slots, 2 special, 1 spare
   const uint64_t lmask = (uint64_t)p.lcap - 1;
   int64_t t0 = (int64_t)blockIdx.x * p.tiles_per_block;
   int64_t t1 = min(p.T, t0 + p.tiles_per_block);
@@ -1485,7 +1485,7 @@ __global__ __launch_bounds__(UB, 2 * UB / 256) void agg_part_update_kernel(AggPa
                                                                            const int* pinst) {
   extern __shared__ int64_t s_tab[];
   __shared__ int s_failed;        // a record of the item missed the LDS table
-  const int64_t LS = p.lcap + 3;  // stride: lcap slots, 2 special, 1 trash
+  const int64_t LS = p.lcap + 3;  // stride: lcap slots, 2 special, 1 spare
   const uint64_t lmask = (uint64_t)p.lcap - 1;
   const int64_t NR = p.rstride;
   const int64_t ni = B + (int64_t)*extra;
@@ -2250,7 +2250,7 @@ static AggParams base_params(gz_agg* h) {
   return p;
 }
 
-// LDS table: lcap slots, the null-key and kEmpty-key slots, and a trash slot.
+// LDS table: lcap slots, the null-key and kEmpty-key slots, and a spare slot.
 static int lds_bytes(const gz_agg* h, int lcap) {
   return (int)((lcap + 3) * (2 + h->plan.lanes.size()) * 8);
 }
