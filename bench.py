@@ -893,8 +893,8 @@ def bench_serve(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; how the path would shard over several is "
-                                   "described in DESIGN.md, multi-GPU runs are not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the benchmark runs on one GPU: its workload is one GPU's share, and "
+                                   "runs over several GPUs are not launched", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
 
